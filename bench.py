@@ -1,0 +1,206 @@
+"""Benchmark of the deform step (one Jacobi iteration of the displacement-field optimiser over
+the whole grid) at 256^3 on one MI355X.
+
+Workload: two analytic torus renders (R = 0.2 m, r = 0.06 m, 30 degree tilt, 2 cm twist
+between them), integrated into 256^3 SDFs at 4 mm voxels (truncation 3 voxels), paper
+parameters (alpha 0.1, omega_k 0.5, gamma 0.1, omega_s 0.2).  A step is one deform
+iteration over all 16.7 M voxels; voxel-iterations are counted densely.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  `value` is the fused adjoint kernel with BF16 SDF storage; the
+plain FP32 reference-formula kernel runs on the same inputs and is reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "voxel-iterations/sec at 256³, adjoint+BF16 vs plain FP32; HBM GB/s vs peak"
+HBM_PEAK_GBS = 8000.0
+GRID = 256
+VOXEL = 0.004
+TRUNC = 3 * VOXEL
+MIN3D = (-0.512, -0.512, 0.288)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "adj_step_bf16_256_traffic.md")
+
+
+class HipEvents:
+    """hipEvent timing on a given stream through the HIP runtime libkf already loaded."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return float(ms.value)
+
+
+def read_traffic():
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 PMC summary."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def time_deform(ctx, live, can, kp, steps, warmup, events):
+    from killingfusion_amd import DisplacementField, compute_displacement_field
+    from killingfusion_amd import _lib
+    field = DisplacementField(ctx, live.dims)
+    if warmup:
+        kp.max_iters = warmup
+        compute_displacement_field(live, can, field, kp)
+    stream = _lib.load().kf_context_stream(ctx.handle)
+    e0, e1 = events.event(), events.event()
+    kp.max_iters = steps
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    events.record(e0, stream)
+    st = compute_displacement_field(live, can, field, kp)
+    events.record(e1, stream)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = events.elapsed_ms(e0, e1)
+    field.close()
+    return wall, gpu_ms, st
+
+
+def cpu_baseline(live_phi, can_phi, kp, iters):
+    """The reference CPU path (oracle FP64 restatement, all-voxels Jacobi mode, OpenMP over z as
+    KillingFusion.cpp:257-258) on the same 256^3 inputs, for a bounded number of iterations."""
+    import oracle as O
+    g = O.OGrid(dims=(GRID, GRID, GRID), min_corner=MIN3D, voxel=VOXEL, oob=TRUNC + 1e-5)
+    p = O.OParams(voxel_size=VOXEL, max_surface=TRUNC, clip=TRUNC, alpha=kp.alpha, omega_k=kp.omega_killing,
+                  gamma=kp.gamma_killing, omega_s=kp.omega_levelset, h=kp.delta_size, max_iters=iters)
+    u = np.zeros((3, g.n))
+    t0 = time.perf_counter()
+    _, st = O.deform_jacobi(live_phi.astype(np.float64), can_phi.astype(np.float64), u, g, p)
+    dt = time.perf_counter() - t0
+    return g.n * st["iters"] / dt, O.num_threads(), st["iters"], dt
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=0, help="iterations timed for the plain kernel (default steps/4)")
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build (one GPU per box).",
+              file=sys.stderr)
+        return 2
+
+    from killingfusion_amd import Context, KillingParams, SDF, scenes
+
+    t_setup = time.perf_counter()
+    d_can, d_live = scenes.torus_pair()
+    ctx = Context(0)
+    events = HipEvents()
+    dims = (GRID, GRID, GRID)
+
+    def volumes(dtype):
+        kw = dict(voxel_size=VOXEL, clip_distance=TRUNC, max_surface=TRUNC, dtype=dtype)
+        live, can = SDF(ctx, dims, MIN3D, **kw), SDF(ctx, dims, MIN3D, **kw)
+        live.integrateDepthFrame(d_live, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+        can.integrateDepthFrame(d_can, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+        return live, can
+
+    live_bf, can_bf = volumes("bf16")
+    live_32, can_32 = volumes("f32")
+    setup_s = time.perf_counter() - t_setup
+    nvox = GRID ** 3
+
+    kp_adj = KillingParams.paper(VOXEL, 3.0, method="adjoint")
+    wall, gpu_ms, st = time_deform(ctx, live_bf, can_bf, kp_adj, a.steps, a.warmup, events)
+    value = nvox * a.steps / wall
+    kernel_ms = gpu_ms / a.steps
+    bytes_per_vi = 2 * 2 + 24
+    achieved = bytes_per_vi * nvox / (kernel_ms * 1e-3) / 1e9
+
+    kp_adj32 = KillingParams.paper(VOXEL, 3.0, method="adjoint")
+    wall32, gpu_ms32, _ = time_deform(ctx, live_32, can_32, kp_adj32, a.steps, a.warmup // 2, events)
+
+    plain_steps = a.plain_steps or max(5, a.steps // 4)
+    kp_ref = KillingParams.paper(VOXEL, 3.0, method="reference")
+    wall_p, gpu_ms_p, st_p = time_deform(ctx, live_32, can_32, kp_ref, plain_steps, max(1, a.warmup // 4), events)
+    plain_value = nvox * plain_steps / wall_p
+    plain_kernel_ms = gpu_ms_p / plain_steps
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        lphi, _ = live_32.download()
+        cphi, _ = can_32.download()
+        cv, cores, it, dt = cpu_baseline(lphi, cphi, kp_ref, a.cpu_iters)
+        cpu = {"value": cv, "unit": "voxel-iterations/s", "cores": cores, "kind": "port",
+               "sample": f"oracle FP64 reference-formula Jacobi, {it} iterations over the same 256^3 grid "
+                         f"({dt:.1f} s, OpenMP over z)"}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-iterations/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (analytic torus renders, 640x480 float32 depth)",
+        "config": {"workload": "deform step, 256^3 torus pair, 4 mm voxels, truncation 3 voxels, paper params; "
+                               "fused adjoint kernel, BF16 SDF storage, FP32 displacements, FP64 reductions",
+                   "grid": GRID, "voxel_m": VOXEL, "phi_storage": "bf16", "method": "adjoint",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
+                     "kernel": "k_adj_step<StoreBF16, STEP>", "bytes_per_voxel_iteration": bytes_per_vi,
+                     "kernel_ms": kernel_ms},
+        "cpu_baseline": cpu,
+        "plain_fp32": {"value": plain_value, "ms_per_step": wall_p * 1e3 / plain_steps, "steps": plain_steps,
+                       "kernel_ms": plain_kernel_ms,
+                       "hbm_gbs_at_32B": 32 * nvox / (plain_kernel_ms * 1e-3) / 1e9},
+        "adjoint_fp32": {"value": nvox * a.steps / wall32, "kernel_ms": gpu_ms32 / a.steps,
+                         "hbm_gbs_at_32B": 32 * nvox / (gpu_ms32 / a.steps * 1e-3) / 1e9},
+        "speedup_vs_plain_fp32": value / plain_value,
+        "active_voxels": int(st["active"]),
+        "active_fraction": st["active"] / nvox,
+        "setup_s": setup_s,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,218 @@
+/*
+ * kf_api.h -- C ABI of the MI355X (gfx950) KillingFusion hot-path library
+ * (killingfusion_amd/libkf.so).
+ *
+ * The reference has no FFI: the path sits behind the C++ classes SDF
+ * (KillingFusion/include/SDF.h:16-200), DisplacementField
+ * (KillingFusion/include/DisplacementField.h:9-66) and the private optimiser
+ * methods of KillingFusion (KillingFusion/include/KillingFusion.h:28-58),
+ * driven by KillingFusion::processNextFrame (KillingFusion/src/KillingFusion.cpp:109-175).
+ * Each entry point below names the reference member it replaces.
+ *
+ * Conventions
+ *  - Every call returns kf_status; no C++ exception crosses this boundary.
+ *    kf_last_error() returns a message for the calling thread's last failure.
+ *  - Handles are bound to their context's HIP stream.  Calls are stream-ordered;
+ *    those that return host values (downloads, stats, mesh counts) synchronise
+ *    the stream first.  Handles are not thread-safe.
+ *  - Grids are dense, x fastest: index = z*nx*ny + y*nx + x
+ *    (SDF.cpp:27,189; DisplacementField.cpp:15,32).
+ *  - Signed distances are metres; displacements are voxel units
+ *    (DisplacementField.h:12); weights are int32 (the reference uses long,
+ *    SDF.h:25).
+ *  - Host arrays are plain pointers; displacements cross the ABI as SoA float32
+ *    [3][N] (ux plane, uy plane, uz plane).
+ */
+#ifndef KF_API_H
+#define KF_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KF_API_VERSION 1
+
+typedef enum kf_status {
+    KF_OK = 0,
+    KF_E_INVALID_ARG = 1,  /* null/short buffer, bad dims, mismatched shapes */
+    KF_E_DIVERGED = 2,     /* non-finite displacement (KillingFusion.cpp:327-332) */
+    KF_E_OOM = 3,          /* device allocation failed */
+    KF_E_HIP = 4,          /* HIP runtime error */
+    KF_E_NO_DEVICE = 5,    /* no usable gfx950 device */
+    KF_E_CAPACITY = 6      /* output capacity too small (count still returned) */
+} kf_status;
+
+/* Storage type of a volume's signed distances.  F32 stores metres; BF16/F16
+ * store phi / (max_surface + epsilon), so the untouched value Max+eps
+ * (SDF.cpp:133) is exactly 1.0 in every format.  Arithmetic is FP32 in all. */
+typedef enum kf_dtype { KF_F32 = 0, KF_BF16 = 1, KF_F16 = 2 } kf_dtype;
+
+/* Energy-gradient formulation used by kf_deform. */
+typedef enum kf_method {
+    KF_METHOD_REFERENCE = 0, /* KillingFusion::computeEnergyGradient, FD stencils (KillingFusion.cpp:399-476) */
+    KF_METHOD_ADJOINT = 1    /* exact gradient of the discrete energy of kf_energy_forward */
+} kf_method;
+
+typedef struct kf_context kf_context;
+typedef struct kf_volume kf_volume;
+typedef struct kf_field kf_field;
+typedef struct kf_pipeline kf_pipeline;
+
+/* Volume geometry: SDF::SDF(voxelSize, min, max, unknownClipDistance), SDF.cpp:14-30,
+ * with the dims given explicitly (kf_dims_from_bounds reproduces SDF.cpp:122). */
+typedef struct kf_volume_desc {
+    int32_t dims[3];
+    double min_corner[3];   /* m_min3dLoc */
+    double voxel_size;      /* m_voxelSize */
+    double clip_distance;   /* UnknownClipDistance (config.cpp:5) */
+    double max_surface;     /* MaxSurfaceVoxelDistance (config.cpp:6) */
+    double epsilon;         /* epsilon (config.cpp:57); untouched phi = max_surface + epsilon */
+    int32_t dtype;          /* kf_dtype */
+    int32_t reserved;
+} kf_volume_desc;
+
+/* Optimiser parameters: the compile-time globals of config.cpp:4-57 as a runtime struct. */
+typedef struct kf_params {
+    double voxel_size;      /* VoxelSize, divides the data residual (KillingFusion.cpp:454) */
+    double max_surface;     /* MaxSurfaceVoxelDistance, skip test (KillingFusion.cpp:273) */
+    double clip_distance;   /* UnknownClipDistance, skip test (KillingFusion.cpp:273) */
+    double epsilon;         /* epsilon (config.cpp:57) */
+    double alpha;           /* gradient step (config.cpp:45) */
+    double omega_killing;   /* omegaKilling (config.cpp:48) */
+    double gamma_killing;   /* gammaKilling (config.cpp:51) */
+    double omega_levelset;  /* omegaLevelSet (config.cpp:54) */
+    double delta_size;      /* deltaSize h, voxels (config.cpp:29) */
+    double killing_clamp;   /* 0.1 (DisplacementField.cpp:342) */
+    int32_t use_data, use_levelset, use_killing;  /* EnergyTypeUsed (config.cpp:23) */
+    int32_t max_iters;      /* KILLING_MAX_ITERATIONS (config.cpp:42) */
+    double stop_max_update; /* stop once max |du| < this (voxels); <= 0 runs max_iters */
+    int32_t method;         /* kf_method */
+    int32_t reserved;
+} kf_params;
+
+typedef struct kf_deform_stats {
+    int32_t iters;           /* iterations run */
+    int32_t reserved;
+    int64_t diverged_index;  /* first voxel with a non-finite displacement, or -1 */
+    double max_update;       /* max |du| of the last iteration, voxels (FP64) */
+    double energy;           /* adjoint: energy before the last update (FP64); reference: 0 */
+    uint64_t active_voxels;  /* voxels updated in the last iteration */
+} kf_deform_stats;
+
+/* ---- library / context ---- */
+const char *kf_status_string(kf_status s);
+const char *kf_last_error(void);
+int kf_api_version(void);
+/* Fills p with the reference defaults of config.cpp:4-57 (voxel 12 mm, Max = clip = 4 voxels,
+ * alpha 0.025, omega_k 0.04, gamma 0, omega_s 0.1, h 0.05, 1024 iterations). */
+kf_status kf_params_default(kf_params *p);
+/* stream: a hipStream_t, or NULL to create one owned by the context. */
+kf_status kf_context_create(int device, void *stream, kf_context **out);
+kf_status kf_context_destroy(kf_context *ctx);
+kf_status kf_context_synchronize(kf_context *ctx);
+/* The hipStream_t the context's work is ordered on (for callers that time or chain work). */
+void *kf_context_stream(kf_context *ctx);
+
+/* ---- SDF volume (SDF.h:16-200) ---- */
+/* SDF::computeVoxelGridSize, SDF.cpp:117-124: int((max-min)/voxel)+1 per axis. */
+kf_status kf_dims_from_bounds(const double min_corner[3], const double max_corner[3], double voxel,
+                              int32_t dims_out[3]);
+/* SDF::SDF + allocateMemoryForSDF, SDF.cpp:14-30,126-135: phi = max_surface+epsilon, w = 0. */
+kf_status kf_volume_create(kf_context *ctx, const kf_volume_desc *desc, kf_volume **out);
+kf_status kf_volume_destroy(kf_volume *vol);
+kf_status kf_volume_describe(const kf_volume *vol, kf_volume_desc *out);
+/* SDF::allocateMemoryForSDF fill, SDF.cpp:133-134. */
+kf_status kf_volume_reset(kf_volume *vol);
+/* Sets m_voxelGridTSDF / m_voxelGridWeight from host arrays (phi metres; weight may be NULL = 0). */
+kf_status kf_volume_upload(kf_volume *vol, const float *phi, const int32_t *weight);
+/* Reads them back (either pointer may be NULL). */
+kf_status kf_volume_download(const kf_volume *vol, float *phi, int32_t *weight);
+/* SDF(const SDF&) + copy of the voxel data (same dims and dtype). */
+kf_status kf_volume_copy(kf_volume *dst, const kf_volume *src);
+
+/* ---- displacement field (DisplacementField.h:9-66) ---- */
+/* DisplacementField::DisplacementField, DisplacementField.cpp:8-16 (zero field). */
+kf_status kf_field_create(kf_context *ctx, const int32_t dims[3], kf_field **out);
+kf_status kf_field_destroy(kf_field *f);
+/* DisplacementField::initializeAllVoxels, DisplacementField.cpp:82-88. */
+kf_status kf_field_fill(kf_field *f, const double value[3]);
+kf_status kf_field_upload(kf_field *f, const float *soa);
+kf_status kf_field_download(const kf_field *f, float *soa);
+/* DisplacementField::operator+, DisplacementField.cpp:73-80 (dst += src). */
+kf_status kf_field_add(kf_field *dst, const kf_field *src);
+
+/* ---- per-frame kernels ---- */
+/* SDF::integrateDepthFrame, SDF.cpp:137-197.  depth: host float32 [height][width], metres
+ * (0 = invalid); K row-major 3x3; C2W row-major 4x4 camera-to-world pose. */
+kf_status kf_integrate_depth(kf_volume *vol, const float *depth, int32_t width, int32_t height,
+                             const double K[9], const double C2W[16], double min_depth,
+                             double max_depth);
+/* KillingFusion::computeEnergyGradient (KillingFusion.cpp:399-442) at every voxel, written to
+ * grad_out (zero at voxels the optimiser would skip, KillingFusion.cpp:272-274). */
+kf_status kf_energy_gradient_ref(const kf_volume *live, const kf_volume *can, const kf_field *u,
+                                 const kf_params *p, kf_field *grad_out);
+/* Forward pass of the adjoint formulation: total energy (FP64, fixed-order sum) and,
+ * optionally, per-voxel energy (host float32 [N]).  No reference counterpart: the
+ * reference evaluates the Killing energy density only in self-tests
+ * (DisplacementField.cpp:171-193). */
+kf_status kf_energy_forward(const kf_volume *live, const kf_volume *can, const kf_field *u,
+                            const kf_params *p, double *energy_out, float *energy_per_voxel);
+/* Backward pass: exact dE/du at every voxel (zero data/level-set part at inactive voxels). */
+kf_status kf_energy_backward(const kf_volume *live, const kf_volume *can, const kf_field *u,
+                             const kf_params *p, kf_field *grad_out);
+/* KillingFusion::computeDisplacementField, all-voxels (Jacobi) mode
+ * (KillingFusion.cpp:248-351): live = src, can = dest, u = srcToDest, updated in place.
+ * p->method selects the reference gradient or the fused adjoint kernel. */
+kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, const kf_params *p,
+                    kf_deform_stats *stats);
+/* SDF::update(displacementField), SDF.cpp:300-328. */
+kf_status kf_warp(kf_volume *vol, const kf_field *u);
+/* SDF::fuse(otherSdf), SDF.cpp:199-241 (FUSE_BY_MERGE). */
+kf_status kf_fuse(kf_volume *can, const kf_volume *live);
+/* SDF::fuse(otherSdf, displacementField), SDF.cpp:244-298. */
+kf_status kf_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *u);
+/* SDF::getMesh (SDF.cpp:396-414) + ProcessVolumeCell (SDF.cpp:330-378): triangle soup in the
+ * reference's order, 9 floats per triangle (3 vertices, y negated as SimpleMesh::AddVertex,
+ * SimpleMesh.h:66-72); faces are implicit (3t, 3t+1, 3t+2).  *n_tris always receives the
+ * count; verts may be NULL (size query); KF_E_CAPACITY if cap_tris < count. */
+kf_status kf_marching_cubes(const kf_volume *vol, double iso, uint64_t *n_tris, float *verts,
+                            uint64_t cap_tris);
+
+/* ---- batched deform (independent volumes of uneven size, one launch per iteration) ---- */
+kf_status kf_batch_deform(const kf_volume *const *live, const kf_volume *const *can,
+                          kf_field *const *u, int32_t count, const kf_params *p,
+                          kf_deform_stats *stats);
+
+/* ---- per-frame pipeline: KillingFusion::processNextFrame, KillingFusion.cpp:109-175 ---- */
+typedef struct kf_pipeline_desc {
+    kf_volume_desc volume;   /* canonical and live volume geometry */
+    kf_params params;
+    double K[9];
+    double C2W[16];
+    double min_depth, max_depth;
+    int32_t width, height;
+} kf_pipeline_desc;
+
+typedef struct kf_frame_stats {
+    int32_t frame_index;
+    int32_t reserved;
+    kf_deform_stats deform;
+    double ms_integrate, ms_deform, ms_fuse, ms_total;  /* the reference's printf columns */
+} kf_frame_stats;
+
+kf_status kf_pipeline_create(kf_context *ctx, const kf_pipeline_desc *desc, kf_pipeline **out);
+kf_status kf_pipeline_destroy(kf_pipeline *pl);
+/* One call = one processNextFrame: frame 1 becomes the canonical SDF; later frames are
+ * integrated into a fresh live SDF, deformed onto the canonical one, warped and fused. */
+kf_status kf_pipeline_process_frame(kf_pipeline *pl, const float *depth, kf_frame_stats *stats);
+kf_volume *kf_pipeline_canonical(kf_pipeline *pl);
+kf_volume *kf_pipeline_live(kf_pipeline *pl);
+kf_field *kf_pipeline_field(kf_pipeline *pl);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,287 @@
+// k_mc.hip -- marching cubes with the reference's output order.
+//
+// SDF::getMesh (SDF.cpp:396-414) visits every cell (x, y, z) in z -> y -> x order, skips it
+// when its corner (x, y, z) is > MaxSurfaceVoxelDistance or < -UnknownClipDistance, and
+// appends each cell's triangles (Polygonise, MarchingCubes.h:366-422) as three fresh
+// vertices with y negated (SimpleMesh.h:66-80).  The GPU keeps that order with an ordered
+// compaction instead of atomics:
+//   1. count: each block owns CHUNK consecutive cells and sums their triangle counts;
+//   2. scan: one block turns the per-block totals into exclusive offsets;
+//   3. emit: each block recounts its cells round by round (BLOCK cells per round, in cell
+//      order), prefix-sums the counts across the block and writes every triangle at its
+//      final position.
+// Corner values are compared and interpolated in FP64 like the reference (built with
+// -ffp-contract=off), so counts, order and vertex values match an FP64 evaluation of the
+// same stored grid.
+#include "kf_internal.hpp"
+#include "kf_mc_cases.h"
+
+#include <vector>
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int ITEMS = 16;
+constexpr int CHUNK = BLOCK * ITEMS;
+
+__constant__ unsigned char c_tri_count[256];
+__constant__ unsigned char c_tri_edges[256][15];
+__constant__ unsigned short c_edge_mask[256];
+
+// Corner k of cell (x, y, z): ProcessVolumeCell, SDF.cpp:338-355.
+__constant__ signed char c_corner[8][3] = {{1, 0, 0}, {0, 0, 0}, {0, 1, 0}, {1, 1, 0},
+                                           {1, 0, 1}, {0, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+__constant__ unsigned char c_edge_corners[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                                     {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+template <class S>
+struct Cells {
+    const typename S::T *phi;
+    GridDims g;
+    float scale;
+    double oob, max_surface, clip, iso;
+    __device__ inline double at(int x, int y, int z) const
+    {
+        return g.in(x, y, z) ? (double)S::load(phi, g.idx(x, y, z), scale) : oob;
+    }
+    // Returns the cube index, or -1 when the cell is skipped or has no triangles.
+    __device__ inline int classify(int64_t i, double val[8], int &x, int &y, int &z) const
+    {
+        x = (int)(i % g.nx);
+        y = (int)((i / g.nx) % g.ny);
+        z = (int)(i / ((int64_t)g.nx * g.ny));
+        const double d = at(x, y, z);
+        if (d > max_surface || d < -clip)
+            return -1;
+        int ci = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            val[k] = at(x + c_corner[k][0], y + c_corner[k][1], z + c_corner[k][2]);
+            if (val[k] < iso)
+                ci |= 1 << k;
+        }
+        return c_edge_mask[ci] ? ci : -1;
+    }
+};
+
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_mc_count(Cells<S> C, unsigned long long *block_tot)
+{
+    const int64_t base = (int64_t)blockIdx.x * CHUNK;
+    unsigned int cnt = 0;
+    for (int k = 0; k < ITEMS; k++) {
+        const int64_t i = base + (int64_t)k * BLOCK + threadIdx.x;
+        if (i < C.g.n) {
+            double val[8];
+            int x, y, z;
+            const int ci = C.classify(i, val, x, y, z);
+            if (ci >= 0)
+                cnt += c_tri_count[ci];
+        }
+    }
+    __shared__ unsigned int s[BLOCK];
+    s[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int o = BLOCK / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        block_tot[blockIdx.x] = s[0];
+}
+
+// Exclusive scan of nb block totals in one block of 1024 threads; offs[nb] = total.
+__global__ __launch_bounds__(1024) void k_mc_scan(const unsigned long long *tot, unsigned long long *offs, int nb)
+{
+    __shared__ unsigned long long s[1024];
+    const int t = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = t * per, b1 = min(nb, b0 + per);
+    unsigned long long sum = 0;
+    for (int b = b0; b < b1; b++)
+        sum += tot[b];
+    s[t] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan.
+    for (int o = 1; o < 1024; o <<= 1) {
+        unsigned long long v = (t >= o) ? s[t - o] : 0ull;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    unsigned long long run = s[t] - sum;
+    for (int b = b0; b < b1; b++) {
+        offs[b] = run;
+        run += tot[b];
+    }
+    if (t == 1023)
+        offs[nb] = s[1023];
+}
+
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_mc_emit(Cells<S> C, const unsigned long long *offs, float *verts,
+                                                   unsigned long long cap)
+{
+    __shared__ unsigned int s[BLOCK];
+    const int64_t base = (int64_t)blockIdx.x * CHUNK;
+    unsigned long long run = offs[blockIdx.x];
+    const double hx = (double)C.g.nx / 2, hy = (double)C.g.ny / 2, hz = (double)C.g.nz / 2;
+    for (int k = 0; k < ITEMS; k++) {
+        const int64_t i = base + (int64_t)k * BLOCK + threadIdx.x;
+        double val[8];
+        int x = 0, y = 0, z = 0, ci = -1;
+        unsigned int cnt = 0;
+        if (i < C.g.n) {
+            ci = C.classify(i, val, x, y, z);
+            if (ci >= 0)
+                cnt = c_tri_count[ci];
+        }
+        // Block-wide exclusive scan of cnt in thread order (= cell order within the round).
+        s[threadIdx.x] = cnt;
+        __syncthreads();
+        for (int o = 1; o < BLOCK; o <<= 1) {
+            unsigned int v = ((int)threadIdx.x >= o) ? s[threadIdx.x - o] : 0u;
+            __syncthreads();
+            s[threadIdx.x] += v;
+            __syncthreads();
+        }
+        const unsigned int incl = s[threadIdx.x];
+        const unsigned int round_total = s[BLOCK - 1];
+        __syncthreads();
+        if (cnt) {
+            // VertexInterp (MarchingCubes.h:326-356) on the edges this case cuts.
+            double p[8][3];
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                p[c][0] = (double)(x + c_corner[c][0]) / hx - 1;
+                p[c][1] = (double)(y + c_corner[c][1]) / hy - 1;
+                p[c][2] = (double)(z + c_corner[c][2]) / hz - 1;
+            }
+            double vl[12][3];
+            const unsigned int em = c_edge_mask[ci];
+#pragma unroll
+            for (int e = 0; e < 12; e++) {
+                if (em & (1u << e)) {
+                    const int a = c_edge_corners[e][0], b = c_edge_corners[e][1];
+                    const double sc = (C.iso - val[a]) / (val[a] - val[b]);
+                    vl[e][0] = p[a][0] + sc * (p[a][0] - p[b][0]);
+                    vl[e][1] = p[a][1] + sc * (p[a][1] - p[b][1]);
+                    vl[e][2] = p[a][2] + sc * (p[a][2] - p[b][2]);
+                }
+            }
+            unsigned long long t0 = run + (incl - cnt);
+            for (unsigned int t = 0; t < cnt; t++) {
+                const unsigned long long tri = t0 + t;
+                if (tri >= cap)
+                    break;
+                float *o = verts + tri * 9;
+                for (int v = 0; v < 3; v++) {
+                    const int e = c_tri_edges[ci][t * 3 + v];
+                    o[v * 3 + 0] = (float)vl[e][0];
+                    o[v * 3 + 1] = (float)(-vl[e][1]);
+                    o[v * 3 + 2] = (float)vl[e][2];
+                }
+            }
+        }
+        run += round_total;
+    }
+}
+
+template <class S>
+Cells<S> make_cells(const kf_volume *v, double iso)
+{
+    Cells<S> C;
+    C.phi = (const typename S::T *)v->phi;
+    C.g = grid_of(v);
+    C.scale = v->scale();
+    C.oob = v->oob();
+    C.max_surface = v->max_surface;
+    C.clip = v->clip;
+    C.iso = iso;
+    return C;
+}
+
+}  // namespace
+
+// Uploads the case table to the current device's constant memory.
+kf_status kf_mc_init_tables()
+{
+    unsigned char cnt[256];
+    unsigned char edges[256][15];
+    unsigned short em[256];
+    for (int ci = 0; ci < 256; ci++) {
+        const char *cs = KF_MC_CASES[ci];
+        int k = 0;
+        for (; cs[k]; k++) {
+            const char ch = cs[k];
+            edges[ci][k] = (unsigned char)(ch <= '9' ? ch - '0' : ch - 'a' + 10);
+        }
+        for (int j = k; j < 15; j++)
+            edges[ci][j] = 0;
+        cnt[ci] = (unsigned char)(k / 3);
+        unsigned short m = 0;
+        for (int e = 0; e < 12; e++) {
+            const int a = (ci >> KF_MC_EDGE_CORNERS[e][0]) & 1, b = (ci >> KF_MC_EDGE_CORNERS[e][1]) & 1;
+            if (a != b)
+                m |= (unsigned short)(1u << e);
+        }
+        em[ci] = m;
+    }
+    KF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tri_count), cnt, sizeof(cnt)));
+    KF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tri_edges), edges, sizeof(edges)));
+    KF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_edge_mask), em, sizeof(em)));
+    return KF_OK;
+}
+
+kf_status kf_mc_run(const kf_volume *v, double iso, uint64_t *n_tris, float *verts_host, uint64_t cap)
+{
+    kf_context *ctx = v->ctx;
+    const int nb = (int)((v->n + CHUNK - 1) / CHUNK);
+    unsigned long long *tot = nullptr, *offs = nullptr;
+    KF_HIP(hipMallocAsync((void **)&tot, sizeof(unsigned long long) * (size_t)(2 * nb + 1), ctx->stream));
+    offs = tot + nb;
+    kf_status st = KF_OK;
+    unsigned long long total = 0;
+    KF_DISPATCH_DTYPE(v->dtype, S, {
+        Cells<S> C = make_cells<S>(v, iso);
+        hipLaunchKernelGGL((k_mc_count<S>), dim3(nb), dim3(BLOCK), 0, ctx->stream, C, tot);
+        hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, ctx->stream, (const unsigned long long *)tot, offs, nb);
+        hipError_t e = hipMemcpyAsync(&total, offs + nb, sizeof(total), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            kf_set_error(std::string("marching cubes count: ") + hipGetErrorString(e));
+            st = KF_E_HIP;
+        }
+        if (st == KF_OK) {
+            *n_tris = total;
+            if (verts_host && total > 0) {
+                if (cap < total) {
+                    kf_set_error("marching cubes: capacity below triangle count");
+                    st = KF_E_CAPACITY;
+                } else {
+                    float *dverts = nullptr;
+                    e = hipMallocAsync((void **)&dverts, sizeof(float) * 9 * (size_t)total, ctx->stream);
+                    if (e == hipSuccess) {
+                        hipLaunchKernelGGL((k_mc_emit<S>), dim3(nb), dim3(BLOCK), 0, ctx->stream, C,
+                                           (const unsigned long long *)offs, dverts, (unsigned long long)total);
+                        e = hipGetLastError();
+                        if (e == hipSuccess)
+                            e = hipMemcpyAsync(verts_host, dverts, sizeof(float) * 9 * (size_t)total,
+                                               hipMemcpyDeviceToHost, ctx->stream);
+                        hipFreeAsync(dverts, ctx->stream);
+                        if (e == hipSuccess)
+                            e = hipStreamSynchronize(ctx->stream);
+                    }
+                    if (e != hipSuccess) {
+                        kf_set_error(std::string("marching cubes emit: ") + hipGetErrorString(e));
+                        st = e == hipErrorOutOfMemory ? KF_E_OOM : KF_E_HIP;
+                    }
+                }
+            }
+        }
+    });
+    hipFreeAsync(tot, ctx->stream);
+    return st;
+}

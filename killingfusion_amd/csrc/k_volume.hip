@@ -1,0 +1,323 @@
+// k_volume.hip -- per-frame volume kernels: fill, storage conversion, projective
+// integration, warp, fuse, displacement-aware fuse, field fill/add.
+//
+// These kernels make discrete decisions (pixel pick, interpolated weights truncated to
+// integers) exactly where the reference does, so they compute in FP64 with the reference's
+// operation order, and this file is built with -ffp-contract=off: every product and sum
+// rounds on its own, as in the reference's x86-64 build.  They stream each voxel once.
+#include "kf_internal.hpp"
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+inline unsigned int blocks_for(int64_t n) { return (unsigned int)((n + BLOCK - 1) / BLOCK); }
+
+template <class S>
+__global__ void k_fill(typename S::T *phi, int32_t *w, int64_t n, float oob_m, float scale)
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) {
+        phi[i] = S::enc(oob_m, scale);
+        w[i] = 0;
+    }
+}
+
+template <class S>
+__global__ void k_encode(typename S::T *phi, const float *src, int64_t n, float scale)
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n)
+        phi[i] = S::enc(src[i], scale);
+}
+
+template <class S>
+__global__ void k_decode(float *dst, const typename S::T *phi, int64_t n, float scale)
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n)
+        dst[i] = S::load(phi, i, scale);
+}
+
+struct Mat9 { double m[9]; };
+struct Mat16 { double m[16]; };
+
+// SDF::integrateDepthFrame, SDF.cpp:149-196.
+template <class S>
+__global__ void k_integrate(typename S::T *phi, int32_t *w, GridDims g, double mnx, double mny, double mnz,
+                            double voxel, double clip, float scale, const float *__restrict__ depth, int W,
+                            int H, Mat9 K, Mat16 M, double min_depth, double max_depth)
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= g.n)
+        return;
+    const int x = (int)(i % g.nx);
+    const int y = (int)((i / g.nx) % g.ny);
+    const int z = (int)(i / ((int64_t)g.nx * g.ny));
+    const double Z = voxel * (z + 0.5) + mnz;
+    const double Y = voxel * (y + 0.5) + mny;
+    const double X = voxel * (x + 0.5) + mnx;
+    double c[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        c[r] = M.m[r * 4 + 0] * X + M.m[r * 4 + 1] * Y + M.m[r * 4 + 2] * Z + M.m[r * 4 + 3] * 1.0;
+    const double q0 = c[0] / c[2], q1 = c[1] / c[2];
+    const double px = K.m[0] * q0 + K.m[1] * q1 + K.m[2] * 1.0;
+    const double py = K.m[3] * q0 + K.m[4] * q1 + K.m[5] * 1.0;
+    // roundf on the float-narrowed coordinate, SDF.cpp:167,171.
+    const int col = (int)roundf((float)(px + 0.5));
+    if (col < 0 || col >= W)
+        return;
+    const int row = (int)roundf((float)(py + 0.5));
+    if (row < 0 || row >= H)
+        return;
+    const double d = (double)depth[(int64_t)row * W + col];
+    if (d < min_depth || d > max_depth)
+        return;
+    const double dist = d - c[2];
+    if (dist < -clip)
+        return;
+    const int32_t wi = w[i];
+    const double old = (double)S::load(phi, i, scale);
+    const double nv = (old * (double)wi + dist) / (double)(wi + 1);
+    phi[i] = S::enc((float)nv, scale);
+    w[i] = wi + 1;
+}
+
+template <class S>
+struct Reader {
+    const typename S::T *phi;
+    const int32_t *w;
+    GridDims g;
+    float scale;
+    double oob;
+    __device__ inline double dist(int x, int y, int z) const
+    {
+        return g.in(x, y, z) ? (double)S::load(phi, g.idx(x, y, z), scale) : oob;
+    }
+    __device__ inline double weight(int x, int y, int z) const
+    {
+        return g.in(x, y, z) ? (double)w[g.idx(x, y, z)] : 0.0;
+    }
+    // SDF::getDistance / getWeight(Vector3d), SDF.cpp:503-524, 584-604, in FP64.
+    __device__ inline void both(double lx, double ly, double lz, double &dd, double &ww) const
+    {
+        const double tx = lx - 0.5, ty = ly - 0.5, tz = lz - 0.5;
+        const int bx = (int)tx, by = (int)ty, bz = (int)tz;
+        const double fx = tx - (double)bx, fy = ty - (double)by, fz = tz - (double)bz;
+        ww = kf_interp3(weight(bx, by, bz), weight(bx + 1, by, bz), weight(bx, by + 1, bz),
+                        weight(bx + 1, by + 1, bz), weight(bx, by, bz + 1), weight(bx + 1, by, bz + 1),
+                        weight(bx, by + 1, bz + 1), weight(bx + 1, by + 1, bz + 1), fx, fy, fz);
+        dd = kf_interp3(dist(bx, by, bz), dist(bx + 1, by, bz), dist(bx, by + 1, bz), dist(bx + 1, by + 1, bz),
+                        dist(bx, by, bz + 1), dist(bx + 1, by, bz + 1), dist(bx, by + 1, bz + 1),
+                        dist(bx + 1, by + 1, bz + 1), fx, fy, fz);
+    }
+};
+
+// SDF::update(displacementField), SDF.cpp:300-328.
+template <class S>
+__global__ void k_warp(Reader<S> R, const float *__restrict__ u, typename S::T *out_phi, int32_t *out_w)
+{
+    const GridDims g = R.g;
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= g.n)
+        return;
+    const int x = (int)(i % g.nx);
+    const int y = (int)((i / g.nx) % g.ny);
+    const int z = (int)(i / ((int64_t)g.nx * g.ny));
+    const double lx = (double)x + 0.5 + (double)u[i];
+    const double ly = (double)y + 0.5 + (double)u[g.n + i];
+    const double lz = (double)z + 0.5 + (double)u[2 * g.n + i];
+    double dd, ww;
+    R.both(lx, ly, lz, dd, ww);
+    out_phi[i] = S::enc((float)dd, R.scale);
+    out_w[i] = (int32_t)(long long)ww;
+}
+
+// SDF::fuse(otherSdf), SDF.cpp:199-241, FUSE_BY_MERGE.
+template <class S1, class S2>
+__global__ void k_fuse(typename S1::T *phi1, int32_t *w1, float scale1, const typename S2::T *__restrict__ phi2,
+                       const int32_t *__restrict__ w2, float scale2, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n)
+        return;
+    const int32_t b = w2[i];
+    if (b == 0)
+        return;
+    const int32_t a = w1[i];
+    const double d2 = (double)S2::load(phi2, i, scale2);
+    if (a == 0) {
+        phi1[i] = S1::enc((float)d2, scale1);
+        w1[i] = b;
+    } else {
+        const double d1 = (double)S1::load(phi1, i, scale1);
+        const double nv = ((double)a * d1 + (double)b * d2) / (double)(a + b);
+        phi1[i] = S1::enc((float)nv, scale1);
+        w1[i] = a + b;
+    }
+}
+
+// SDF::fuse(otherSdf, displacementField), SDF.cpp:244-298.
+template <class S1, class S2>
+__global__ void k_fuse_warped(typename S1::T *phi1, int32_t *w1, float scale1, Reader<S2> R,
+                              const float *__restrict__ u)
+{
+    const GridDims g = R.g;
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= g.n)
+        return;
+    const int x = (int)(i % g.nx);
+    const int y = (int)((i / g.nx) % g.ny);
+    const int z = (int)(i / ((int64_t)g.nx * g.ny));
+    const double lx = (double)x + 0.5 + (double)u[i];
+    const double ly = (double)y + 0.5 + (double)u[g.n + i];
+    const double lz = (double)z + 0.5 + (double)u[2 * g.n + i];
+    double d2, b;
+    R.both(lx, ly, lz, d2, b);
+    if (fabs(b) < (double)1e-5f)
+        return;
+    const int32_t a = w1[i];
+    if (a == 0) {
+        phi1[i] = S1::enc((float)d2, scale1);
+        w1[i] = (int32_t)(long long)b;
+    } else {
+        const double d1 = (double)S1::load(phi1, i, scale1);
+        const double nv = ((double)a * d1 + b * d2) / ((double)a + b);
+        phi1[i] = S1::enc((float)nv, scale1);
+        w1[i] = (int32_t)(long long)((double)a + b);
+    }
+}
+
+__global__ void k_field_fill(float *u, int64_t n, float a, float b, float c)
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) {
+        u[i] = a;
+        u[n + i] = b;
+        u[2 * n + i] = c;
+    }
+}
+
+__global__ void k_field_add(float *dst, const float *__restrict__ src, int64_t n3)
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n3)
+        dst[i] = dst[i] + src[i];
+}
+
+template <class S>
+Reader<S> make_reader(const kf_volume *v)
+{
+    Reader<S> R;
+    R.phi = (const typename S::T *)v->phi;
+    R.w = v->w;
+    R.g = grid_of(v);
+    R.scale = v->scale();
+    R.oob = v->oob();
+    return R;
+}
+
+}  // namespace
+
+kf_status kf_launch_fill_volume(kf_volume *v)
+{
+    KF_DISPATCH_DTYPE(v->dtype, S, {
+        hipLaunchKernelGGL((k_fill<S>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream,
+                           (typename S::T *)v->phi, v->w, v->n, (float)v->oob(), v->scale());
+    });
+    KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
+kf_status kf_launch_upload_volume(kf_volume *v, const float *dphi, const int32_t *dw)
+{
+    (void)dw;
+    KF_DISPATCH_DTYPE(v->dtype, S, {
+        hipLaunchKernelGGL((k_encode<S>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream,
+                           (typename S::T *)v->phi, dphi, v->n, v->scale());
+    });
+    KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
+kf_status kf_launch_download_volume(const kf_volume *v, float *dphi)
+{
+    KF_DISPATCH_DTYPE(v->dtype, S, {
+        hipLaunchKernelGGL((k_decode<S>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream, dphi,
+                           (const typename S::T *)v->phi, v->n, v->scale());
+    });
+    KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
+kf_status kf_launch_integrate(kf_volume *v, const float *ddepth, int w, int h, const double K[9],
+                              const double W2C[16], double min_depth, double max_depth)
+{
+    Mat9 k;
+    Mat16 m;
+    for (int i = 0; i < 9; i++)
+        k.m[i] = K[i];
+    for (int i = 0; i < 16; i++)
+        m.m[i] = W2C[i];
+    KF_DISPATCH_DTYPE(v->dtype, S, {
+        hipLaunchKernelGGL((k_integrate<S>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream,
+                           (typename S::T *)v->phi, v->w, grid_of(v), v->min_corner[0], v->min_corner[1],
+                           v->min_corner[2], v->voxel, v->clip, v->scale(), ddepth, w, h, k, m, min_depth,
+                           max_depth);
+    });
+    KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
+kf_status kf_launch_warp(kf_volume *v, const kf_field *u, void *tmp_phi, int32_t *tmp_w)
+{
+    KF_DISPATCH_DTYPE(v->dtype, S, {
+        hipLaunchKernelGGL((k_warp<S>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream, make_reader<S>(v),
+                           (const float *)u->u(), (typename S::T *)tmp_phi, tmp_w);
+    });
+    KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
+kf_status kf_launch_fuse(kf_volume *can, const kf_volume *live)
+{
+    KF_DISPATCH_DTYPE(can->dtype, S1, {
+        KF_DISPATCH_DTYPE(live->dtype, S2, {
+            hipLaunchKernelGGL((k_fuse<S1, S2>), dim3(blocks_for(can->n)), dim3(BLOCK), 0, can->ctx->stream,
+                               (typename S1::T *)can->phi, can->w, can->scale(),
+                               (const typename S2::T *)live->phi, (const int32_t *)live->w, live->scale(), can->n);
+        });
+    });
+    KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
+kf_status kf_launch_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *u)
+{
+    KF_DISPATCH_DTYPE(can->dtype, S1, {
+        KF_DISPATCH_DTYPE(live->dtype, S2, {
+            hipLaunchKernelGGL((k_fuse_warped<S1, S2>), dim3(blocks_for(can->n)), dim3(BLOCK), 0,
+                               can->ctx->stream, (typename S1::T *)can->phi, can->w, can->scale(),
+                               make_reader<S2>(live), (const float *)u->u());
+        });
+    });
+    KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
+kf_status kf_launch_field_fill(kf_field *f, const double v[3])
+{
+    hipLaunchKernelGGL(k_field_fill, dim3(blocks_for(f->n)), dim3(BLOCK), 0, f->ctx->stream, f->u(), f->n,
+                       (float)v[0], (float)v[1], (float)v[2]);
+    KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
+kf_status kf_launch_field_add(kf_field *dst, const kf_field *src)
+{
+    hipLaunchKernelGGL(k_field_add, dim3(blocks_for(3 * dst->n)), dim3(BLOCK), 0, dst->ctx->stream, dst->u(),
+                       (const float *)src->u(), 3 * dst->n);
+    KF_LAUNCH_CHECK();
+    return KF_OK;
+}

@@ -1,0 +1,695 @@
+// kf_api.hip -- the C ABI of libkf (include/kf_api.h): handles, transfers and the
+// optimiser driver.  Kernels live in k_*.hip.
+#include "kf_internal.hpp"
+
+#include <climits>
+#include <cstring>
+#include <string>
+
+kf_status kf_mc_init_tables();
+
+namespace {
+
+thread_local std::string g_last_error;
+
+constexpr int STATE_BLOCK = 64;
+
+__global__ void k_state_reset(DeformState *st, double stop)
+{
+    if (threadIdx.x == 0) {
+        st->done = 0;
+        st->iters = 0;
+        st->ticket = 0;
+        st->pad0 = 0;
+        st->max_bits = 0ull;
+        st->active = 0ull;
+        st->diverged = LLONG_MAX;
+        st->stop = stop;
+        st->last_max = 0.0;
+        st->energy = 0.0;
+        st->last_active = 0ull;
+    }
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess)
+            prev = -1;
+        if (prev != dev)
+            (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev)
+            (void)hipSetDevice(prev);
+    }
+};
+
+bool valid_dims(const int32_t d[3])
+{
+    if (!d)
+        return false;
+    for (int a = 0; a < 3; a++)
+        if (d[a] <= 0 || d[a] > (1 << 15))
+            return false;
+    return (int64_t)d[0] * d[1] * d[2] <= (int64_t)1 << 33;
+}
+
+bool valid_params(const kf_params *p)
+{
+    return p && p->voxel_size > 0.0 && p->max_iters >= 0 && p->delta_size > 0.0 &&
+           (p->method == KF_METHOD_REFERENCE || p->method == KF_METHOD_ADJOINT);
+}
+
+// Gauss-Jordan inverse with partial pivoting (row-major 4x4).
+bool invert4(const double m[16], double out[16])
+{
+    double a[4][8];
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 8; c++)
+            a[r][c] = c < 4 ? m[r * 4 + c] : (c - 4 == r ? 1.0 : 0.0);
+    for (int c = 0; c < 4; c++) {
+        int piv = c;
+        for (int r = c + 1; r < 4; r++)
+            if (std::fabs(a[r][c]) > std::fabs(a[piv][c]))
+                piv = r;
+        if (a[piv][c] == 0.0)
+            return false;
+        if (piv != c)
+            for (int k = 0; k < 8; k++)
+                std::swap(a[c][k], a[piv][k]);
+        const double d = a[c][c];
+        for (int k = 0; k < 8; k++)
+            a[c][k] /= d;
+        for (int r = 0; r < 4; r++) {
+            if (r == c || a[r][c] == 0.0)
+                continue;
+            const double f = a[r][c];
+            for (int k = 0; k < 8; k++)
+                a[r][k] -= f * a[c][k];
+        }
+    }
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++)
+            out[r * 4 + c] = a[r][c + 4];
+    return true;
+}
+
+kf_status read_state(kf_context *ctx)
+{
+    KF_HIP(hipMemcpyAsync(ctx->state_host, ctx->state, sizeof(DeformState), hipMemcpyDeviceToHost, ctx->stream));
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+    return KF_OK;
+}
+
+}  // namespace
+
+void kf_set_error(const std::string &msg) { g_last_error = msg; }
+
+kf_status kf_ensure_partials(kf_context *ctx, size_t nblocks)
+{
+    if (nblocks <= ctx->partials_cap)
+        return KF_OK;
+    if (ctx->partials) {
+        KF_HIP(hipStreamSynchronize(ctx->stream));
+        KF_HIP(hipFree(ctx->partials));
+        ctx->partials = nullptr;
+        ctx->partials_cap = 0;
+    }
+    KF_HIP(hipMalloc((void **)&ctx->partials, sizeof(double) * nblocks));
+    ctx->partials_cap = nblocks;
+    return KF_OK;
+}
+
+kf_status kf_state_reset(kf_context *ctx, double stop)
+{
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(STATE_BLOCK), 0, ctx->stream, ctx->state, stop);
+    KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
+extern "C" {
+
+const char *kf_status_string(kf_status s)
+{
+    switch (s) {
+    case KF_OK: return "ok";
+    case KF_E_INVALID_ARG: return "invalid argument";
+    case KF_E_DIVERGED: return "displacement field diverged";
+    case KF_E_OOM: return "out of device memory";
+    case KF_E_HIP: return "HIP runtime error";
+    case KF_E_NO_DEVICE: return "no usable gfx950 device";
+    case KF_E_CAPACITY: return "output capacity too small";
+    }
+    return "unknown status";
+}
+
+const char *kf_last_error(void) { return g_last_error.c_str(); }
+
+int kf_api_version(void) { return KF_API_VERSION; }
+
+kf_status kf_params_default(kf_params *p)
+{
+    KF_CHECK_ARG(p, "params is null");
+    std::memset(p, 0, sizeof(*p));
+    p->voxel_size = 0.012;                 // config.cpp:4
+    p->max_surface = 0.012 * 4;            // config.cpp:6
+    p->clip_distance = 0.012 * 4;          // config.cpp:5
+    p->epsilon = 0.00001;                  // config.cpp:57
+    p->alpha = 0.025;                      // config.cpp:45
+    p->omega_killing = 0.04;               // config.cpp:48
+    p->gamma_killing = 0.0;                // config.cpp:51
+    p->omega_levelset = 0.1;               // config.cpp:54
+    p->delta_size = 0.05;                  // config.cpp:29
+    p->killing_clamp = 0.1;                // DisplacementField.cpp:342
+    p->use_data = p->use_levelset = p->use_killing = 1;  // config.cpp:23
+    p->max_iters = 1024;                   // config.cpp:42
+    p->stop_max_update = 0.0;
+    p->method = KF_METHOD_REFERENCE;
+    return KF_OK;
+}
+
+kf_status kf_context_create(int device, void *stream, kf_context **out)
+{
+    KF_CHECK_ARG(out, "out is null");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        kf_set_error("no HIP device visible");
+        return KF_E_NO_DEVICE;
+    }
+    KF_CHECK_ARG(device >= 0 && device < count, "device index out of range");
+    hipDeviceProp_t prop;
+    KF_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        kf_set_error(std::string("device is ") + prop.gcnArchName + ", libkf is built for gfx950");
+        return KF_E_NO_DEVICE;
+    }
+    DeviceGuard dg(device);
+    kf_context *ctx = new kf_context();
+    ctx->device = device;
+    ctx->num_cus = prop.multiProcessorCount;
+    kf_status s = KF_OK;
+    if (stream) {
+        ctx->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+            kf_set_error("hipStreamCreate failed");
+            delete ctx;
+            return KF_E_HIP;
+        }
+        ctx->own_stream = true;
+    }
+    if (hipMalloc((void **)&ctx->state, sizeof(DeformState)) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->state_host, sizeof(DeformState), hipHostMallocDefault) != hipSuccess) {
+        kf_set_error("context allocation failed");
+        kf_context_destroy(ctx);
+        return KF_E_OOM;
+    }
+    s = kf_mc_init_tables();
+    if (s == KF_OK)
+        s = kf_state_reset(ctx, 0.0);
+    if (s != KF_OK) {
+        kf_context_destroy(ctx);
+        return s;
+    }
+    *out = ctx;
+    return KF_OK;
+}
+
+kf_status kf_context_destroy(kf_context *ctx)
+{
+    if (!ctx)
+        return KF_OK;
+    DeviceGuard dg(ctx->device);
+    if (ctx->stream)
+        (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->state)
+        (void)hipFree(ctx->state);
+    if (ctx->state_host)
+        (void)hipHostFree(ctx->state_host);
+    if (ctx->partials)
+        (void)hipFree(ctx->partials);
+    if (ctx->own_stream && ctx->stream)
+        (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return KF_OK;
+}
+
+kf_status kf_context_synchronize(kf_context *ctx)
+{
+    KF_CHECK_ARG(ctx, "context is null");
+    DeviceGuard dg(ctx->device);
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+    return KF_OK;
+}
+
+void *kf_context_stream(kf_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+// ---------------------------------------------------------------------------
+// Volumes
+// ---------------------------------------------------------------------------
+
+kf_status kf_dims_from_bounds(const double min_corner[3], const double max_corner[3], double voxel, int32_t dims_out[3])
+{
+    KF_CHECK_ARG(min_corner && max_corner && dims_out && voxel > 0.0, "bad bounds");
+    for (int a = 0; a < 3; a++) {
+        const double c = (max_corner[a] - min_corner[a]) / voxel;
+        KF_CHECK_ARG(c >= 0.0 && c < (double)(1 << 15), "bounds give an invalid grid");
+        dims_out[a] = (int32_t)c + 1;  // SDF.cpp:122
+    }
+    return KF_OK;
+}
+
+kf_status kf_volume_create(kf_context *ctx, const kf_volume_desc *d, kf_volume **out)
+{
+    KF_CHECK_ARG(ctx && d && out, "null argument");
+    *out = nullptr;
+    KF_CHECK_ARG(valid_dims(d->dims), "dims must be positive and at most 32768 per axis");
+    KF_CHECK_ARG(d->voxel_size > 0.0, "voxel_size must be positive");
+    KF_CHECK_ARG(d->dtype == KF_F32 || d->dtype == KF_BF16 || d->dtype == KF_F16, "unknown dtype");
+    KF_CHECK_ARG(d->max_surface + d->epsilon > 0.0, "max_surface + epsilon must be positive");
+    DeviceGuard dg(ctx->device);
+    kf_volume *v = new kf_volume();
+    v->ctx = ctx;
+    for (int a = 0; a < 3; a++) {
+        v->dims[a] = d->dims[a];
+        v->min_corner[a] = d->min_corner[a];
+    }
+    v->n = (int64_t)d->dims[0] * d->dims[1] * d->dims[2];
+    v->voxel = d->voxel_size;
+    v->clip = d->clip_distance;
+    v->max_surface = d->max_surface;
+    v->epsilon = d->epsilon;
+    v->dtype = (kf_dtype)d->dtype;
+    if (hipMalloc(&v->phi, v->elem_size() * (size_t)v->n) != hipSuccess ||
+        hipMalloc((void **)&v->w, sizeof(int32_t) * (size_t)v->n) != hipSuccess) {
+        kf_set_error("volume allocation failed");
+        kf_volume_destroy(v);
+        return KF_E_OOM;
+    }
+    kf_status s = kf_launch_fill_volume(v);
+    if (s != KF_OK) {
+        kf_volume_destroy(v);
+        return s;
+    }
+    *out = v;
+    return KF_OK;
+}
+
+kf_status kf_volume_destroy(kf_volume *v)
+{
+    if (!v)
+        return KF_OK;
+    DeviceGuard dg(v->ctx->device);
+    (void)hipStreamSynchronize(v->ctx->stream);
+    if (v->phi)
+        (void)hipFree(v->phi);
+    if (v->w)
+        (void)hipFree(v->w);
+    delete v;
+    return KF_OK;
+}
+
+kf_status kf_volume_describe(const kf_volume *v, kf_volume_desc *out)
+{
+    KF_CHECK_ARG(v && out, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    for (int a = 0; a < 3; a++) {
+        out->dims[a] = v->dims[a];
+        out->min_corner[a] = v->min_corner[a];
+    }
+    out->voxel_size = v->voxel;
+    out->clip_distance = v->clip;
+    out->max_surface = v->max_surface;
+    out->epsilon = v->epsilon;
+    out->dtype = v->dtype;
+    return KF_OK;
+}
+
+kf_status kf_volume_reset(kf_volume *v)
+{
+    KF_CHECK_ARG(v, "volume is null");
+    DeviceGuard dg(v->ctx->device);
+    return kf_launch_fill_volume(v);
+}
+
+kf_status kf_volume_upload(kf_volume *v, const float *phi, const int32_t *weight)
+{
+    KF_CHECK_ARG(v && phi, "null argument");
+    DeviceGuard dg(v->ctx->device);
+    hipStream_t s = v->ctx->stream;
+    float *tmp = nullptr;
+    KF_HIP(hipMallocAsync((void **)&tmp, sizeof(float) * (size_t)v->n, s));
+    hipError_t e = hipMemcpyAsync(tmp, phi, sizeof(float) * (size_t)v->n, hipMemcpyHostToDevice, s);
+    kf_status st = KF_OK;
+    if (e == hipSuccess)
+        st = kf_launch_upload_volume(v, tmp, nullptr);
+    if (e == hipSuccess && st == KF_OK) {
+        if (weight)
+            e = hipMemcpyAsync(v->w, weight, sizeof(int32_t) * (size_t)v->n, hipMemcpyHostToDevice, s);
+        else
+            e = hipMemsetAsync(v->w, 0, sizeof(int32_t) * (size_t)v->n, s);
+    }
+    (void)hipFreeAsync(tmp, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        kf_set_error(std::string("volume upload: ") + hipGetErrorString(e));
+        return KF_E_HIP;
+    }
+    return st;
+}
+
+kf_status kf_volume_download(const kf_volume *v, float *phi, int32_t *weight)
+{
+    KF_CHECK_ARG(v, "volume is null");
+    DeviceGuard dg(v->ctx->device);
+    hipStream_t s = v->ctx->stream;
+    kf_status st = KF_OK;
+    hipError_t e = hipSuccess;
+    if (phi) {
+        float *tmp = nullptr;
+        KF_HIP(hipMallocAsync((void **)&tmp, sizeof(float) * (size_t)v->n, s));
+        st = kf_launch_download_volume(v, tmp);
+        if (st == KF_OK)
+            e = hipMemcpyAsync(phi, tmp, sizeof(float) * (size_t)v->n, hipMemcpyDeviceToHost, s);
+        (void)hipFreeAsync(tmp, s);
+    }
+    if (weight && e == hipSuccess && st == KF_OK)
+        e = hipMemcpyAsync(weight, v->w, sizeof(int32_t) * (size_t)v->n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        kf_set_error(std::string("volume download: ") + hipGetErrorString(e));
+        return KF_E_HIP;
+    }
+    return st;
+}
+
+kf_status kf_volume_copy(kf_volume *dst, const kf_volume *src)
+{
+    KF_CHECK_ARG(dst && src, "null argument");
+    KF_CHECK_ARG(same_dims(dst->dims, src->dims) && dst->dtype == src->dtype, "volumes differ in dims or dtype");
+    DeviceGuard dg(dst->ctx->device);
+    hipStream_t s = dst->ctx->stream;
+    KF_HIP(hipMemcpyAsync(dst->phi, src->phi, dst->elem_size() * (size_t)dst->n, hipMemcpyDeviceToDevice, s));
+    KF_HIP(hipMemcpyAsync(dst->w, src->w, sizeof(int32_t) * (size_t)dst->n, hipMemcpyDeviceToDevice, s));
+    return KF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Displacement fields
+// ---------------------------------------------------------------------------
+
+kf_status kf_field_create(kf_context *ctx, const int32_t dims[3], kf_field **out)
+{
+    KF_CHECK_ARG(ctx && out, "null argument");
+    *out = nullptr;
+    KF_CHECK_ARG(valid_dims(dims), "dims must be positive and at most 32768 per axis");
+    DeviceGuard dg(ctx->device);
+    kf_field *f = new kf_field();
+    f->ctx = ctx;
+    for (int a = 0; a < 3; a++)
+        f->dims[a] = dims[a];
+    f->n = (int64_t)dims[0] * dims[1] * dims[2];
+    for (int b = 0; b < 2; b++) {
+        if (hipMalloc((void **)&f->buf[b], sizeof(float) * 3 * (size_t)f->n) != hipSuccess) {
+            kf_set_error("field allocation failed");
+            kf_field_destroy(f);
+            return KF_E_OOM;
+        }
+    }
+    const double zero[3] = {0.0, 0.0, 0.0};
+    kf_status s = kf_launch_field_fill(f, zero);
+    if (s != KF_OK) {
+        kf_field_destroy(f);
+        return s;
+    }
+    *out = f;
+    return KF_OK;
+}
+
+kf_status kf_field_destroy(kf_field *f)
+{
+    if (!f)
+        return KF_OK;
+    DeviceGuard dg(f->ctx->device);
+    (void)hipStreamSynchronize(f->ctx->stream);
+    for (int b = 0; b < 2; b++)
+        if (f->buf[b])
+            (void)hipFree(f->buf[b]);
+    delete f;
+    return KF_OK;
+}
+
+kf_status kf_field_fill(kf_field *f, const double value[3])
+{
+    KF_CHECK_ARG(f && value, "null argument");
+    DeviceGuard dg(f->ctx->device);
+    return kf_launch_field_fill(f, value);
+}
+
+kf_status kf_field_upload(kf_field *f, const float *soa)
+{
+    KF_CHECK_ARG(f && soa, "null argument");
+    DeviceGuard dg(f->ctx->device);
+    KF_HIP(hipMemcpyAsync(f->u(), soa, sizeof(float) * 3 * (size_t)f->n, hipMemcpyHostToDevice, f->ctx->stream));
+    KF_HIP(hipStreamSynchronize(f->ctx->stream));
+    return KF_OK;
+}
+
+kf_status kf_field_download(const kf_field *f, float *soa)
+{
+    KF_CHECK_ARG(f && soa, "null argument");
+    DeviceGuard dg(f->ctx->device);
+    KF_HIP(hipMemcpyAsync(soa, f->u(), sizeof(float) * 3 * (size_t)f->n, hipMemcpyDeviceToHost, f->ctx->stream));
+    KF_HIP(hipStreamSynchronize(f->ctx->stream));
+    return KF_OK;
+}
+
+kf_status kf_field_add(kf_field *dst, const kf_field *src)
+{
+    KF_CHECK_ARG(dst && src, "null argument");
+    KF_CHECK_ARG(same_dims(dst->dims, src->dims), "fields differ in dims");
+    DeviceGuard dg(dst->ctx->device);
+    return kf_launch_field_add(dst, src);
+}
+
+// ---------------------------------------------------------------------------
+// Per-frame kernels
+// ---------------------------------------------------------------------------
+
+kf_status kf_integrate_depth(kf_volume *v, const float *depth, int32_t width, int32_t height, const double K[9],
+                             const double C2W[16], double min_depth, double max_depth)
+{
+    KF_CHECK_ARG(v && depth && K && C2W, "null argument");
+    KF_CHECK_ARG(width > 0 && height > 0 && (int64_t)width * height <= ((int64_t)1 << 28), "bad frame size");
+    double W2C[16];
+    KF_CHECK_ARG(invert4(C2W, W2C), "camera pose is singular");
+    DeviceGuard dg(v->ctx->device);
+    hipStream_t s = v->ctx->stream;
+    float *dd = nullptr;
+    const size_t bytes = sizeof(float) * (size_t)width * (size_t)height;
+    KF_HIP(hipMallocAsync((void **)&dd, bytes, s));
+    hipError_t e = hipMemcpyAsync(dd, depth, bytes, hipMemcpyHostToDevice, s);
+    kf_status st = KF_OK;
+    if (e == hipSuccess)
+        st = kf_launch_integrate(v, dd, width, height, K, W2C, min_depth, max_depth);
+    (void)hipFreeAsync(dd, s);
+    if (e != hipSuccess) {
+        kf_set_error(std::string("depth upload: ") + hipGetErrorString(e));
+        return KF_E_HIP;
+    }
+    return st;
+}
+
+static kf_status check_triplet(const kf_volume *live, const kf_volume *can, const kf_field *u)
+{
+    KF_CHECK_ARG(live && can && u, "null argument");
+    KF_CHECK_ARG(same_dims(live->dims, can->dims) && same_dims(live->dims, u->dims),
+                 "live, canonical and field dims differ");
+    KF_CHECK_ARG(live->ctx == can->ctx && live->ctx == u->ctx, "handles belong to different contexts");
+    KF_CHECK_ARG(live->dtype == can->dtype, "live and canonical volumes must share a dtype");
+    return KF_OK;
+}
+
+kf_status kf_energy_gradient_ref(const kf_volume *live, const kf_volume *can, const kf_field *u, const kf_params *p,
+                                 kf_field *grad_out)
+{
+    kf_status s = check_triplet(live, can, u);
+    if (s != KF_OK)
+        return s;
+    KF_CHECK_ARG(valid_params(p), "invalid params");
+    KF_CHECK_ARG(grad_out && grad_out != u && same_dims(grad_out->dims, u->dims), "grad_out must be a distinct field of the same dims");
+    DeviceGuard dg(live->ctx->device);
+    return kf_launch_ref_step(live, can, u->u(), grad_out->u(), p, KF_MODE_GRAD);
+}
+
+kf_status kf_energy_forward(const kf_volume *live, const kf_volume *can, const kf_field *u, const kf_params *p,
+                            double *energy_out, float *energy_per_voxel)
+{
+    kf_status s = check_triplet(live, can, u);
+    if (s != KF_OK)
+        return s;
+    KF_CHECK_ARG(valid_params(p) && energy_out, "invalid params or null energy_out");
+    kf_context *ctx = live->ctx;
+    DeviceGuard dg(ctx->device);
+    float *ev = nullptr;
+    if (energy_per_voxel)
+        KF_HIP(hipMallocAsync((void **)&ev, sizeof(float) * (size_t)live->n, ctx->stream));
+    s = kf_state_reset(ctx, 0.0);
+    if (s == KF_OK)
+        s = kf_launch_adj_step(live, can, u->u(), nullptr, p, KF_MODE_ENERGY, ev);
+    if (s == KF_OK && ev) {
+        hipError_t e = hipMemcpyAsync(energy_per_voxel, ev, sizeof(float) * (size_t)live->n, hipMemcpyDeviceToHost,
+                                      ctx->stream);
+        if (e != hipSuccess) {
+            kf_set_error(std::string("energy download: ") + hipGetErrorString(e));
+            s = KF_E_HIP;
+        }
+    }
+    if (ev)
+        (void)hipFreeAsync(ev, ctx->stream);
+    if (s == KF_OK)
+        s = read_state(ctx);
+    if (s == KF_OK)
+        *energy_out = ctx->state_host->energy;
+    return s;
+}
+
+kf_status kf_energy_backward(const kf_volume *live, const kf_volume *can, const kf_field *u, const kf_params *p,
+                             kf_field *grad_out)
+{
+    kf_status s = check_triplet(live, can, u);
+    if (s != KF_OK)
+        return s;
+    KF_CHECK_ARG(valid_params(p), "invalid params");
+    KF_CHECK_ARG(grad_out && grad_out != u && same_dims(grad_out->dims, u->dims), "grad_out must be a distinct field of the same dims");
+    DeviceGuard dg(live->ctx->device);
+    return kf_launch_adj_step(live, can, u->u(), grad_out->u(), p, KF_MODE_GRAD, nullptr);
+}
+
+kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, const kf_params *p,
+                    kf_deform_stats *stats)
+{
+    kf_status s = check_triplet(live, can, u);
+    if (s != KF_OK)
+        return s;
+    KF_CHECK_ARG(valid_params(p), "invalid params");
+    kf_context *ctx = live->ctx;
+    DeviceGuard dg(ctx->device);
+    s = kf_state_reset(ctx, p->stop_max_update);
+    if (s != KF_OK)
+        return s;
+    const int cur0 = u->cur;
+    const bool check = p->stop_max_update > 0.0;
+    constexpr int CHUNK = 16;
+    for (int k = 0; k < p->max_iters; k++) {
+        float *in = u->buf[cur0 ^ (k & 1)];
+        float *out = u->buf[cur0 ^ ((k + 1) & 1)];
+        s = p->method == KF_METHOD_ADJOINT ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr)
+                                           : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP);
+        if (s != KF_OK)
+            return s;
+        if (check && (k + 1) % CHUNK == 0 && k + 1 < p->max_iters) {
+            s = read_state(ctx);
+            if (s != KF_OK)
+                return s;
+            if (ctx->state_host->done)
+                break;
+        }
+    }
+    s = read_state(ctx);
+    if (s != KF_OK)
+        return s;
+    const DeformState &h = *ctx->state_host;
+    u->cur = cur0 ^ (int)(h.iters & 1u);
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->iters = (int32_t)h.iters;
+        stats->diverged_index = h.diverged == LLONG_MAX ? -1 : (int64_t)h.diverged;
+        stats->max_update = h.last_max;
+        stats->energy = p->method == KF_METHOD_ADJOINT ? h.energy : 0.0;
+        stats->active_voxels = h.last_active;
+    }
+    if (h.diverged != LLONG_MAX) {
+        kf_set_error("deformation field diverged at voxel " + std::to_string((long long)h.diverged));
+        return KF_E_DIVERGED;
+    }
+    return KF_OK;
+}
+
+kf_status kf_warp(kf_volume *v, const kf_field *u)
+{
+    KF_CHECK_ARG(v && u, "null argument");
+    KF_CHECK_ARG(same_dims(v->dims, u->dims), "volume and field dims differ");
+    DeviceGuard dg(v->ctx->device);
+    hipStream_t s = v->ctx->stream;
+    void *np = nullptr;
+    int32_t *nw = nullptr;
+    KF_HIP(hipMallocAsync(&np, v->elem_size() * (size_t)v->n, s));
+    hipError_t e = hipMallocAsync((void **)&nw, sizeof(int32_t) * (size_t)v->n, s);
+    if (e != hipSuccess) {
+        (void)hipFreeAsync(np, s);
+        kf_set_error("warp scratch allocation failed");
+        return KF_E_OOM;
+    }
+    kf_status st = kf_launch_warp(v, u, np, nw);
+    if (st != KF_OK) {
+        (void)hipFreeAsync(np, s);
+        (void)hipFreeAsync(nw, s);
+        return st;
+    }
+    // The reference swaps the new grids in (SDF.cpp:326-327); so does this.
+    (void)hipFreeAsync(v->phi, s);
+    (void)hipFreeAsync(v->w, s);
+    v->phi = np;
+    v->w = nw;
+    return KF_OK;
+}
+
+kf_status kf_fuse(kf_volume *can, const kf_volume *live)
+{
+    KF_CHECK_ARG(can && live, "null argument");
+    KF_CHECK_ARG(same_dims(can->dims, live->dims), "volumes differ in dims");
+    KF_CHECK_ARG(can != live, "fusing a volume into itself needs a copy first");
+    DeviceGuard dg(can->ctx->device);
+    return kf_launch_fuse(can, live);
+}
+
+kf_status kf_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *u)
+{
+    KF_CHECK_ARG(can && live && u, "null argument");
+    KF_CHECK_ARG(same_dims(can->dims, live->dims) && same_dims(can->dims, u->dims), "dims differ");
+    KF_CHECK_ARG(can != live, "fusing a volume into itself needs a copy first");
+    DeviceGuard dg(can->ctx->device);
+    return kf_launch_fuse_warped(can, live, u);
+}
+
+kf_status kf_marching_cubes(const kf_volume *v, double iso, uint64_t *n_tris, float *verts, uint64_t cap_tris)
+{
+    KF_CHECK_ARG(v && n_tris, "null argument");
+    DeviceGuard dg(v->ctx->device);
+    return kf_mc_run(v, iso, n_tris, verts, cap_tris);
+}
+
+kf_status kf_batch_deform(const kf_volume *const *live, const kf_volume *const *can, kf_field *const *u,
+                          int32_t count, const kf_params *p, kf_deform_stats *stats)
+{
+    KF_CHECK_ARG(live && can && u && count >= 0, "null argument");
+    kf_status worst = KF_OK;
+    for (int32_t b = 0; b < count; b++) {
+        kf_status s = kf_deform(live[b], can[b], u[b], p, stats ? &stats[b] : nullptr);
+        if (s == KF_E_DIVERGED)
+            worst = s;
+        else if (s != KF_OK)
+            return s;
+    }
+    return worst;
+}
+
+}  // extern "C"

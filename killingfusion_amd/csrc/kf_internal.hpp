@@ -1,0 +1,214 @@
+// kf_internal.hpp -- shared host structs, storage codecs and device helpers of libkf.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "kf_api.h"
+
+// ---------------------------------------------------------------------------
+// Error plumbing
+// ---------------------------------------------------------------------------
+
+void kf_set_error(const std::string &msg);
+
+#define KF_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            kf_set_error(std::string(#call " failed: ") + hipGetErrorString(e_));             \
+            return e_ == hipErrorOutOfMemory ? KF_E_OOM : KF_E_HIP;                           \
+        }                                                                                     \
+    } while (0)
+
+#define KF_CHECK_ARG(cond, msg)                                                               \
+    do {                                                                                      \
+        if (!(cond)) {                                                                        \
+            kf_set_error(msg);                                                                \
+            return KF_E_INVALID_ARG;                                                          \
+        }                                                                                     \
+    } while (0)
+
+#define KF_LAUNCH_CHECK()                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess) {                                                               \
+            kf_set_error(std::string("kernel launch failed: ") + hipGetErrorString(e_));      \
+            return KF_E_HIP;                                                                  \
+        }                                                                                     \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Device-side optimiser state (one per context, reused by every deform call)
+// ---------------------------------------------------------------------------
+
+struct DeformState {
+    unsigned int done;          // set by the last block when the stop rule fires
+    unsigned int iters;         // iterations completed
+    unsigned int ticket;        // block arrival counter of the running launch
+    unsigned int pad0;
+    unsigned long long max_bits;    // running max |du| of the running launch (double bits, >= 0)
+    unsigned long long active;      // active voxels of the running launch
+    long long diverged;             // lowest index with a non-finite update, INT64_MAX if none
+    double stop;                    // stop threshold (<= 0: never)
+    double last_max;                // max |du| of the last completed launch
+    double energy;                  // energy of the last completed launch
+    unsigned long long last_active; // active voxels of the last completed launch
+};
+
+// ---------------------------------------------------------------------------
+// Host handles
+// ---------------------------------------------------------------------------
+
+struct kf_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    DeformState *state = nullptr;     // device
+    DeformState *state_host = nullptr;  // pinned mirror for stats readback
+    double *partials = nullptr;       // device, per-block FP64 partial sums
+    size_t partials_cap = 0;
+    int num_cus = 256;
+};
+
+struct kf_volume {
+    kf_context *ctx = nullptr;
+    int32_t dims[3] = {0, 0, 0};
+    int64_t n = 0;
+    double min_corner[3] = {0, 0, 0};
+    double voxel = 0, clip = 0, max_surface = 0, epsilon = 0;
+    kf_dtype dtype = KF_F32;
+    void *phi = nullptr;     // device: n elements of dtype
+    int32_t *w = nullptr;    // device: n
+    double oob() const { return max_surface + epsilon; }
+    // Decode scale: stored value * scale = metres (1 for F32).
+    float scale() const { return dtype == KF_F32 ? 1.0f : (float)(max_surface + epsilon); }
+    size_t elem_size() const { return dtype == KF_F32 ? 4 : 2; }
+};
+
+struct kf_field {
+    kf_context *ctx = nullptr;
+    int32_t dims[3] = {0, 0, 0};
+    int64_t n = 0;
+    float *buf[2] = {nullptr, nullptr};  // SoA [3][n] each; buf[cur] is the field's value
+    int cur = 0;
+    float *u() const { return buf[cur]; }
+    float *scratch() const { return buf[cur ^ 1]; }
+};
+
+// ---------------------------------------------------------------------------
+// Storage codecs.  F32 holds metres; BF16/F16 hold phi / (Max + eps).
+// ---------------------------------------------------------------------------
+
+struct StoreF32 {
+    using T = float;
+    static constexpr int dtype = KF_F32;
+    __device__ __host__ static inline float load(const T *p, int64_t i, float) { return p[i]; }
+    __device__ __host__ static inline T enc(float v, float) { return v; }
+    __device__ __host__ static inline float dec(T v, float) { return v; }
+};
+
+struct StoreBF16 {
+    using T = uint16_t;
+    static constexpr int dtype = KF_BF16;
+    __device__ __host__ static inline float dec(T v, float scale)
+    {
+        union { uint32_t u; float f; } c;
+        c.u = (uint32_t)v << 16;
+        return c.f * scale;
+    }
+    __device__ __host__ static inline float load(const T *p, int64_t i, float scale) { return dec(p[i], scale); }
+    // Round to nearest even (NaN kept quiet).
+    __device__ __host__ static inline T enc(float v, float scale)
+    {
+        union { uint32_t u; float f; } c;
+        c.f = v / scale;
+        if ((c.u & 0x7fffffffu) > 0x7f800000u)
+            return (T)((c.u >> 16) | 0x40u);
+        uint32_t r = c.u + 0x7fffu + ((c.u >> 16) & 1u);
+        return (T)(r >> 16);
+    }
+};
+
+struct StoreF16 {
+    using T = _Float16;
+    static constexpr int dtype = KF_F16;
+    __device__ __host__ static inline float dec(T v, float scale) { return (float)v * scale; }
+    __device__ __host__ static inline float load(const T *p, int64_t i, float scale) { return dec(p[i], scale); }
+    __device__ __host__ static inline T enc(float v, float scale) { return (T)(v / scale); }
+};
+
+// Dispatch a templated functor on a volume's dtype.
+#define KF_DISPATCH_DTYPE(dt, S, ...)                                                         \
+    switch (dt) {                                                                             \
+    case KF_F32: { using S = StoreF32; __VA_ARGS__; } break;                                  \
+    case KF_BF16: { using S = StoreBF16; __VA_ARGS__; } break;                                \
+    case KF_F16: { using S = StoreF16; __VA_ARGS__; } break;                                  \
+    default: kf_set_error("unknown dtype"); return KF_E_INVALID_ARG;                          \
+    }
+
+// ---------------------------------------------------------------------------
+// Grid geometry passed to kernels by value
+// ---------------------------------------------------------------------------
+
+struct GridDims {
+    int nx, ny, nz;
+    int64_t n;
+    __device__ __host__ inline bool in(int x, int y, int z) const
+    {
+        return x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
+    }
+    __device__ __host__ inline int64_t idx(int x, int y, int z) const
+    {
+        return ((int64_t)z * ny + y) * nx + x;
+    }
+};
+
+inline GridDims grid_of(const kf_volume *v) { return GridDims{v->dims[0], v->dims[1], v->dims[2], v->n}; }
+inline GridDims grid_of(const kf_field *f) { return GridDims{f->dims[0], f->dims[1], f->dims[2], f->n}; }
+
+inline bool same_dims(const int32_t a[3], const int32_t b[3]) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2]; }
+
+// Interpolation chain of utils.h:6-46 (x, then y, then z; v0*(1-x) + v1*x).
+template <typename R>
+__device__ __host__ inline R kf_lerp(R v0, R v1, R x) { return v0 * (R(1) - x) + v1 * x; }
+
+template <typename R>
+__device__ __host__ inline R kf_interp3(R v000, R v001, R v010, R v011, R v100, R v101, R v110, R v111,
+                                        R x, R y, R z)
+{
+    R s = kf_lerp(kf_lerp(v000, v001, x), kf_lerp(v010, v011, x), y);
+    R t = kf_lerp(kf_lerp(v100, v101, x), kf_lerp(v110, v111, x), y);
+    return kf_lerp(s, t, z);
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers (defined in the kernel translation units)
+// ---------------------------------------------------------------------------
+
+kf_status kf_ensure_partials(kf_context *ctx, size_t nblocks);
+kf_status kf_state_reset(kf_context *ctx, double stop);
+
+kf_status kf_launch_fill_volume(kf_volume *v);
+kf_status kf_launch_upload_volume(kf_volume *v, const float *dphi_f32, const int32_t *dw);
+kf_status kf_launch_download_volume(const kf_volume *v, float *dphi_f32);
+kf_status kf_launch_integrate(kf_volume *v, const float *ddepth, int w, int h, const double K[9],
+                              const double W2C[16], double min_depth, double max_depth);
+kf_status kf_launch_warp(kf_volume *v, const kf_field *u, void *tmp_phi, int32_t *tmp_w);
+kf_status kf_launch_fuse(kf_volume *can, const kf_volume *live);
+kf_status kf_launch_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *u);
+kf_status kf_launch_field_fill(kf_field *f, const double v[3]);
+kf_status kf_launch_field_add(kf_field *dst, const kf_field *src);
+
+struct DeformArgs;
+kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const float *u_in,
+                             float *u_out, const kf_params *p, int mode);
+kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const float *u_in,
+                             float *u_out, const kf_params *p, int mode, float *e_voxel);
+
+kf_status kf_mc_run(const kf_volume *v, double iso, uint64_t *n_tris, float *verts_host, uint64_t cap);
+
+// Step-kernel modes.
+enum { KF_MODE_STEP = 0, KF_MODE_GRAD = 1, KF_MODE_ENERGY = 2 };

@@ -1,0 +1,269 @@
+"""Parity of libkf's HIP kernels (through the C ABI) with the FP64 oracle on the same inputs.
+
+Bars: bit-exact for integer/index work (weights, triangle counts and order) and for the
+FP64-decision kernels compared after the same float32 rounding; FP32 deform results within
+the stated tolerances of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from killingfusion_amd import (DisplacementField, KillingParams, SDF, compute_displacement_field,
+                               energy_backward, energy_forward, energy_gradient_ref, scenes)
+from killingfusion_amd._lib import KF_E_DIVERGED, KFError
+
+from helpers import C1, bf16_roundtrip, f16_roundtrip, make_pair, ogrid, oparams, oracle_integrate, sphere_frames
+
+pytestmark = pytest.mark.gpu
+
+
+def c1_params(**kw):
+    return KillingParams.paper(C1["voxel"], 3.0, **kw)
+
+
+@pytest.fixture(scope="module")
+def c1(ctx):
+    d1, d2 = sphere_frames()
+    live, can = make_pair(ctx, C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"], d2, d1)
+    g = ogrid(C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"])
+    return dict(live=live, can=can, g=g, d1=d1, d2=d2)
+
+
+def test_integrate_bit_exact(c1):
+    print("case: integrate 64^3 sphere frame 2", flush=True)
+    phi, w = c1["live"].download()
+    ophi, ow = oracle_integrate(c1["g"], C1["trunc"], c1["d2"])
+    assert (w == ow).all()
+    assert np.array_equal(phi, ophi.astype(np.float32))
+    assert (w > 0).sum() > 10000
+
+
+def test_fuse_and_warp_bit_exact(ctx, c1):
+    print("case: fuse / fuse_warped / warp 64^3", flush=True)
+    g = c1["g"]
+    lphi, lw = c1["live"].download()
+    cphi, cw = c1["can"].download()
+    u = scenes.seeded_field(C1["dims"], amplitude=1.5, seed=21)
+    field = DisplacementField(ctx, C1["dims"])
+    field.upload(u)
+    # fuse
+    a = SDF(ctx, C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"], C1["trunc"])
+    a.copy_from(c1["can"])
+    a.fuse(c1["live"])
+    gphi, gw = a.download()
+    ophi, ow = cphi.astype(np.float64), cw.astype(np.int64)
+    O.fuse(ophi, ow, lphi.astype(np.float64), lw.astype(np.int64))
+    assert (gw == ow).all() and np.array_equal(gphi, ophi.astype(np.float32))
+    # fuse_warped
+    a.copy_from(c1["can"])
+    a.fuse(c1["live"], field)
+    gphi, gw = a.download()
+    ophi, ow = cphi.astype(np.float64), cw.astype(np.int64)
+    O.fuse_warped(ophi, ow, lphi.astype(np.float64), lw.astype(np.int64), u.astype(np.float64), g)
+    assert (gw == ow).all() and np.array_equal(gphi, ophi.astype(np.float32))
+    # warp
+    a.copy_from(c1["live"])
+    a.update(field)
+    gphi, gw = a.download()
+    ophi, ow = lphi.astype(np.float64), lw.astype(np.int64)
+    O.warp(ophi, ow, u.astype(np.float64), g)
+    assert (gw == ow).all() and np.array_equal(gphi, ophi.astype(np.float32))
+
+
+def test_marching_cubes_bit_exact(c1):
+    print("case: marching cubes 64^3 live + canonical", flush=True)
+    for vol in (c1["live"], c1["can"]):
+        phi, _ = vol.download()
+        mesh = vol.getMesh()
+        ref = O.marching_cubes(phi.astype(np.float64), c1["g"], C1["trunc"], C1["trunc"])
+        assert mesh.tris.shape[0] == ref.shape[0] > 1000
+        assert np.array_equal(mesh.tris, ref.astype(np.float32))
+        assert np.array_equal(mesh.triangles[-1], [3 * len(ref) - 3, 3 * len(ref) - 2, 3 * len(ref) - 1])
+
+
+def test_reference_gradient_matches_oracle(ctx, c1):
+    print("case: reference energy gradient, 64^3, u ~ U(+-0.5)", flush=True)
+    kp = c1_params()
+    u = scenes.seeded_field(C1["dims"], amplitude=0.5, seed=5)
+    field = DisplacementField(ctx, C1["dims"])
+    field.upload(u)
+    out = DisplacementField(ctx, C1["dims"])
+    energy_gradient_ref(c1["live"], c1["can"], field, kp, out)
+    gg = out.download().astype(np.float64)
+    lphi, _ = c1["live"].download()
+    cphi, _ = c1["can"].download()
+    og = O.energy_gradient_grid(lphi.astype(np.float64), cphi.astype(np.float64), u.astype(np.float64), c1["g"],
+                                oparams(kp), masked=True)
+    gm = np.any(gg != 0, axis=0)
+    om = np.any(og != 0, axis=0)
+    # skip-mask decisions agree except where psi sits within float noise of a threshold
+    assert (gm != om).sum() <= 2
+    both = gm & om
+    assert both.sum() > 3000
+    err = np.abs(gg[:, both] - og[:, both])
+    assert err.max() <= 1e-4 * (1 + np.abs(og[:, both]).max())
+
+
+def test_reference_deform_config1(ctx, c1):
+    print("case: reference deform 64^3, <= 64 iterations, stop at 0.1 voxel", flush=True)
+    kp = c1_params(max_iters=64, stop_max_update=0.1)
+    field = DisplacementField(ctx, C1["dims"])
+    st = compute_displacement_field(c1["live"], c1["can"], field, kp)
+    ug = field.download().astype(np.float64)
+    lphi, _ = c1["live"].download()
+    cphi, _ = c1["can"].download()
+    uo, so = O.deform_jacobi(lphi, cphi, np.zeros((3, c1["g"].n)), c1["g"], oparams(kp))
+    assert st["iters"] == so["iters"] and st["active"] == so["active"]
+    assert st["max_update"] == pytest.approx(so["max_update"], rel=1e-4)
+    assert np.abs(ug - uo).max() <= 1e-4  # voxels
+
+
+def test_reference_deform_fixed_iterations(ctx, c1):
+    print("case: reference deform 64^3, 12 fixed iterations", flush=True)
+    kp = c1_params(max_iters=12)
+    field = DisplacementField(ctx, C1["dims"])
+    st = compute_displacement_field(c1["live"], c1["can"], field, kp)
+    ug = field.download().astype(np.float64)
+    lphi, _ = c1["live"].download()
+    cphi, _ = c1["can"].download()
+    uo, so = O.deform_jacobi(lphi, cphi, np.zeros((3, c1["g"].n)), c1["g"], oparams(kp))
+    assert st["iters"] == so["iters"] == 12
+    err = np.abs(ug - uo).max(axis=0)
+    # a voxel whose skip decision flips on float noise takes one different step; allow a
+    # handful and require the rest within 1e-4 voxel
+    assert (err > 1e-4).sum() <= 5
+    assert np.median(err[np.abs(uo).max(axis=0) > 0]) < 1e-6
+
+
+def test_adjoint_gradient_and_energy_match_oracle(ctx, c1):
+    print("case: adjoint forward/backward 64^3", flush=True)
+    kp = c1_params(method="adjoint")
+    u = scenes.seeded_field(C1["dims"], amplitude=0.5, seed=9)
+    field = DisplacementField(ctx, C1["dims"])
+    field.upload(u)
+    out = DisplacementField(ctx, C1["dims"])
+    energy_backward(c1["live"], c1["can"], field, kp, out)
+    gg = out.download().astype(np.float64)
+    e, ev = energy_forward(c1["live"], c1["can"], field, kp, per_voxel=True)
+    lphi, _ = c1["live"].download()
+    cphi, _ = c1["can"].download()
+    og, om = O.adj_gradient(lphi.astype(np.float64), cphi.astype(np.float64), u.astype(np.float64), c1["g"],
+                            oparams(kp))
+    oe, oev, _ = O.adj_energy(lphi.astype(np.float64), cphi.astype(np.float64), u.astype(np.float64), c1["g"],
+                              oparams(kp), per_voxel=True)
+    err = np.abs(gg - og)
+    assert err.max() <= 1e-4 * (1 + np.abs(og).max())
+    assert e == pytest.approx(oe, rel=1e-5)
+    assert np.abs(ev - oev).max() <= 1e-5 * (1 + np.abs(oev).max())
+
+
+def test_adjoint_deform_matches_fp64_oracle(ctx, c1):
+    print("case: adjoint deform 64^3, 100 iterations", flush=True)
+    kp = c1_params(method="adjoint", max_iters=100)
+    field = DisplacementField(ctx, C1["dims"])
+    st = compute_displacement_field(c1["live"], c1["can"], field, kp)
+    ug = field.download().astype(np.float64)
+    lphi, _ = c1["live"].download()
+    cphi, _ = c1["can"].download()
+    uo, so = O.adj_deform(lphi.astype(np.float64), cphi.astype(np.float64), np.zeros((3, c1["g"].n)), c1["g"],
+                          oparams(kp))
+    assert st["iters"] == so["iters"] == 100
+    assert np.abs(ug - uo).max() <= 1e-3
+    assert st["energy"] == pytest.approx(so["energy"], rel=1e-4)
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "f16"])
+def test_low_precision_storage(ctx, c1, dtype):
+    print(f"case: {dtype} storage round trip + adjoint deform 64^3", flush=True)
+    scale = C1["trunc"] + 1e-5
+    lphi, lw = c1["live"].download()
+    cphi, cw = c1["can"].download()
+    rt = bf16_roundtrip if dtype == "bf16" else f16_roundtrip
+    kw = dict(voxel_size=C1["voxel"], clip_distance=C1["trunc"], max_surface=C1["trunc"], dtype=dtype)
+    live = SDF(ctx, C1["dims"], C1["min3d"], **kw)
+    can = SDF(ctx, C1["dims"], C1["min3d"], **kw)
+    live.upload(lphi, lw)
+    can.upload(cphi, cw)
+    qphi, qw = live.download()
+    assert np.array_equal(qphi, rt(lphi, scale)) and (qw == lw).all()
+    # untouched voxels keep exactly Max + eps
+    assert (qphi[lw == 0] == np.float32(scale)).all()
+    kp = c1_params(method="adjoint", max_iters=50)
+    field = DisplacementField(ctx, C1["dims"])
+    compute_displacement_field(live, can, field, kp)
+    ug = field.download().astype(np.float64)
+    qcan, _ = can.download()
+    uo, _ = O.adj_deform(qphi.astype(np.float64), qcan.astype(np.float64), np.zeros((3, c1["g"].n)), c1["g"],
+                         oparams(kp))
+    assert np.abs(ug - uo).max() <= 1e-3
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 3, 5), (7, 1, 3), (5, 6, 4)])
+def test_ragged_tiny_grids(ctx, dims):
+    print(f"case: ragged grid {dims}", flush=True)
+    rng = np.random.default_rng(sum(dims))
+    n = int(np.prod(dims))
+    trunc = 0.03
+    g = ogrid(dims, (0, 0, 0), 0.01, trunc)
+    lphi = rng.uniform(-0.03, 0.03, n).astype(np.float32)
+    cphi = rng.uniform(-0.03, 0.03, n).astype(np.float32)
+    w = rng.integers(0, 3, n).astype(np.int32)
+    live = SDF(ctx, dims, (0, 0, 0), 0.01, trunc, trunc)
+    can = SDF(ctx, dims, (0, 0, 0), 0.01, trunc, trunc)
+    live.upload(lphi, w)
+    can.upload(cphi, w)
+    u = rng.uniform(-0.6, 0.6, (3, n)).astype(np.float32)
+    for method in ("reference", "adjoint"):
+        kp = KillingParams.paper(0.01, 3.0, max_iters=3, method=method)
+        field = DisplacementField(ctx, dims)
+        field.upload(u)
+        compute_displacement_field(live, can, field, kp)
+        ug = field.download().astype(np.float64)
+        fn = O.deform_jacobi if method == "reference" else O.adj_deform
+        uo, _ = fn(lphi.astype(np.float64), cphi.astype(np.float64), u.astype(np.float64), g, oparams(kp))
+        assert np.abs(ug - uo).max() <= 1e-4
+    tris = live.getMesh().tris
+    ref = O.marching_cubes(lphi.astype(np.float64), g, trunc, trunc)
+    assert tris.shape[0] == ref.shape[0] and np.array_equal(tris, ref.astype(np.float32))
+    field = DisplacementField(ctx, dims)
+    field.upload(u)
+    a = SDF(ctx, dims, (0, 0, 0), 0.01, trunc, trunc)
+    a.copy_from(live)
+    a.update(field)
+    gphi, gw = a.download()
+    ophi, ow = lphi.astype(np.float64), w.astype(np.int64)
+    O.warp(ophi, ow, u.astype(np.float64), g)
+    assert (gw == ow).all() and np.array_equal(gphi, ophi.astype(np.float32))
+
+
+def test_untouched_volume_and_zero_iterations(ctx):
+    print("case: empty volumes, max_iters = 0", flush=True)
+    dims = (16, 12, 10)
+    live = SDF(ctx, dims, (0, 0, 0), 0.01, 0.03, 0.03)
+    can = SDF(ctx, dims, (0, 0, 0), 0.01, 0.03, 0.03)
+    field = DisplacementField(ctx, dims)
+    for method in ("reference", "adjoint"):
+        st = compute_displacement_field(live, can, field, KillingParams.paper(0.01, max_iters=5, method=method))
+        assert st["active"] == 0 and st["iters"] == 5
+        assert not field.download().any()
+    u = np.full((3, int(np.prod(dims))), 0.25, dtype=np.float32)
+    field.upload(u)
+    st = compute_displacement_field(live, can, field, KillingParams.paper(0.01, max_iters=0))
+    assert st["iters"] == 0 and np.array_equal(field.download(), u)
+    assert live.getMesh().tris.shape[0] == 0
+
+
+def test_divergence_reported(ctx, c1):
+    print("case: NaN displacement -> KF_E_DIVERGED", flush=True)
+    lphi, _ = c1["live"].download()
+    act = np.nonzero(np.abs(lphi) < 0.01)[0]
+    k = int(act[len(act) // 2])
+    u = np.zeros((3, c1["g"].n), dtype=np.float32)
+    u[0, k] = np.nan
+    for method in ("reference", "adjoint"):
+        field = DisplacementField(ctx, C1["dims"])
+        field.upload(u)
+        with pytest.raises(KFError) as ei:
+            compute_displacement_field(c1["live"], c1["can"], field, c1_params(max_iters=4, method=method))
+        assert ei.value.status == KF_E_DIVERGED
+        assert f"voxel {k}" in str(ei.value) or "diverged at voxel" in str(ei.value)
