@@ -8,6 +8,7 @@ reference code; every call runs the HIP kernels of libkf.so on the context's GPU
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass, fields
 
 import numpy as np
@@ -72,12 +73,18 @@ class Context:
         check(self._L.kf_context_create(device, None, C.byref(h)), "kf_context_create")
         self.handle = h
         self.device = device
+        self._children = weakref.WeakSet()  # handles to release before the context
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def synchronize(self):
         check(self._L.kf_context_synchronize(self.handle), "kf_context_synchronize")
 
     def close(self):
         if self.handle:
+            for ch in list(self._children):
+                ch.close()
             self._L.kf_context_destroy(self.handle)
             self.handle = None
 
@@ -133,6 +140,7 @@ class SDF:
         h = C.c_void_p()
         check(self._L.kf_volume_create(ctx.handle, C.byref(d), C.byref(h)), "kf_volume_create")
         self.handle = h
+        ctx._adopt(self)
         self.dims = tuple(int(x) for x in dims)
         self.min3d = tuple(float(x) for x in min3d)
         self.voxel_size = voxel_size
@@ -228,6 +236,7 @@ class DisplacementField:
         h = C.c_void_p()
         check(self._L.kf_field_create(ctx.handle, d, C.byref(h)), "kf_field_create")
         self.handle = h
+        ctx._adopt(self)
         self.dims = tuple(int(x) for x in dims)
 
     @property
@@ -349,6 +358,7 @@ class KillingFusion:
         h = C.c_void_p()
         check(self._L.kf_pipeline_create(ctx.handle, C.byref(d), C.byref(h)), "kf_pipeline_create")
         self.handle = h
+        ctx._adopt(self)
         self.dims = tuple(dims)
 
     def processNextFrame(self, depth) -> dict:
