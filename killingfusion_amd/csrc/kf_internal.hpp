@@ -171,6 +171,17 @@ inline GridDims grid_of(const kf_field *f) { return GridDims{f->dims[0], f->dims
 
 inline bool same_dims(const int32_t a[3], const int32_t b[3]) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2]; }
 
+// MI355X dispatches workgroup b to XCD (b mod 8), and each XCD has its own L2.  This maps
+// the dispatch index to a tile index so that every XCD sweeps one contiguous range of tiles:
+// tiles that share halo rows and slices then run on the same XCD and hit its L2.
+constexpr int KF_NUM_XCD = 8;
+__device__ __host__ inline unsigned int kf_xcd_tile(unsigned int b, unsigned int nb)
+{
+    const unsigned int xcd = b % KF_NUM_XCD, local = b / KF_NUM_XCD;
+    const unsigned int per = nb / KF_NUM_XCD, rem = nb % KF_NUM_XCD;
+    return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
+
 // Interpolation chain of utils.h:6-46 (x, then y, then z; v0*(1-x) + v1*x).
 template <typename R>
 __device__ __host__ inline R kf_lerp(R v0, R v1, R x) { return v0 * (R(1) - x) + v1 * x; }
