@@ -48,7 +48,7 @@ __device__ inline float uval(const float *__restrict__ u, const GridDims &g, int
 template <class S, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_adj_step(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, AdjConsts c, DeformState *st,
-                                                    double *partials, float *__restrict__ e_voxel)
+                                                    BlockPartial *partials, float *__restrict__ e_voxel)
 {
     if (MODE == KF_MODE_STEP && st->done)
         return;
@@ -62,9 +62,11 @@ __global__ __launch_bounds__(BLOCK) void k_adj_step(PhiRead<S> L, PhiRead<S> Cn,
         const int z = (int)(i / ((int64_t)g.nx * g.ny));
         const float u0 = u_in[i], u1 = u_in[g.n + i], u2 = u_in[2 * g.n + i];
         // One 8-tap gather gives psi, grad and the mixed second derivatives.
-        const float qx = (float)x + u0, qy = (float)y + u1, qz = (float)z + u2;
-        const int bx = (int)qx, by = (int)qy, bz = (int)qz;
-        const float fx = qx - (float)bx, fy = qy - (float)by, fz = qz - (float)bz;
+        int bx, by, bz;
+        float fx, fy, fz;
+        kf_split(x, u0, bx, fx);
+        kf_split(y, u1, by, fy);
+        kf_split(z, u2, bz, fz);
         const float v000 = L.at(bx, by, bz), v001 = L.at(bx + 1, by, bz);
         const float v010 = L.at(bx, by + 1, bz), v011 = L.at(bx + 1, by + 1, bz);
         const float v100 = L.at(bx, by, bz + 1), v101 = L.at(bx + 1, by, bz + 1);
@@ -195,8 +197,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_step(PhiRead<S> L, PhiRead<S> Cn,
         esum = (double)e;
     }
     if (MODE != KF_MODE_GRAD) {
-        kf_block_reduce3<BLOCK>(esum, mx, act);
-        kf_block_finalize<BLOCK>(st, partials, gridDim.x, blockIdx.x, esum, mx, act);
+        kf_block_publish<BLOCK>(partials, blockIdx.x, esum, mx, act);
     }
 }
 
@@ -248,5 +249,5 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
                                u_out, c, ctx->state, ctx->partials, e_voxel);
     });
     KF_LAUNCH_CHECK();
-    return KF_OK;
+    return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb);
 }

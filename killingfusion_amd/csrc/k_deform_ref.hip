@@ -25,14 +25,18 @@ struct PhiView {
     {
         return g.in(x, y, z) ? S::load(p, g.idx(x, y, z), scale) : oob;
     }
-    // SDF::getDistance(Vector3d), SDF.cpp:503-524.
-    __device__ inline float trilinear(float lx, float ly, float lz) const
+    // SDF::getDistance(Vector3d), SDF.cpp:503-524, at the point i + r of the shifted (-0.5)
+    // coordinates: base = (int)(i + r), weights = (i + r) - base.
+    __device__ inline float trilinear(int ix, int iy, int iz, float rx, float ry, float rz) const
     {
-        float tx = lx - 0.5f, ty = ly - 0.5f, tz = lz - 0.5f;
-        int bx = (int)tx, by = (int)ty, bz = (int)tz;
+        int bx, by, bz;
+        float fx, fy, fz;
+        kf_split(ix, rx, bx, fx);
+        kf_split(iy, ry, by, fy);
+        kf_split(iz, rz, bz, fz);
         return kf_interp3(at(bx, by, bz), at(bx + 1, by, bz), at(bx, by + 1, bz), at(bx + 1, by + 1, bz),
                           at(bx, by, bz + 1), at(bx + 1, by, bz + 1), at(bx, by + 1, bz + 1),
-                          at(bx + 1, by + 1, bz + 1), tx - (float)bx, ty - (float)by, tz - (float)bz);
+                          at(bx + 1, by + 1, bz + 1), fx, fy, fz);
     }
 };
 
@@ -49,11 +53,15 @@ __device__ inline void disp_at(const float *__restrict__ u, const GridDims &g, i
     o[2] = u[2 * g.n + i];
 }
 
-// DisplacementField::getDisplacementAtf, DisplacementField.cpp:40-58.
-__device__ inline void disp_at_f(const float *__restrict__ u, const GridDims &g, float lx, float ly, float lz,
-                                 float o[3])
+// DisplacementField::getDisplacementAtf, DisplacementField.cpp:40-58, at the point i + r.
+__device__ inline void disp_at_f(const float *__restrict__ u, const GridDims &g, int ix, int iy, int iz, float rx,
+                                 float ry, float rz, float o[3])
 {
-    int bx = (int)lx, by = (int)ly, bz = (int)lz;
+    int bx, by, bz;
+    float fx, fy, fz;
+    kf_split(ix, rx, bx, fx);
+    kf_split(iy, ry, by, fy);
+    kf_split(iz, rz, bz, fz);
     float c[8][3];
     disp_at(u, g, bx, by, bz, c[0]);
     disp_at(u, g, bx + 1, by, bz, c[1]);
@@ -63,19 +71,20 @@ __device__ inline void disp_at_f(const float *__restrict__ u, const GridDims &g,
     disp_at(u, g, bx + 1, by, bz + 1, c[5]);
     disp_at(u, g, bx, by + 1, bz + 1, c[6]);
     disp_at(u, g, bx + 1, by + 1, bz + 1, c[7]);
-    float fx = lx - (float)bx, fy = ly - (float)by, fz = lz - (float)bz;
 #pragma unroll
     for (int k = 0; k < 3; k++)
         o[k] = kf_interp3(c[0][k], c[1][k], c[2][k], c[3][k], c[4][k], c[5][k], c[6][k], c[7][k], fx, fy, fz);
 }
 
-// SDF::getDistancef, SDF.cpp:577-582.
+// SDF::getDistancef, SDF.cpp:577-582, at loc = i + r: phi at loc + u_f(loc) + 0.5, whose
+// shifted coordinate is i + (r + u_f(loc)).
 template <class S>
-__device__ inline float distance_f(const PhiView<S> &P, const float *__restrict__ u, float lx, float ly, float lz)
+__device__ inline float distance_f(const PhiView<S> &P, const float *__restrict__ u, int ix, int iy, int iz, float rx,
+                                   float ry, float rz)
 {
     float d[3];
-    disp_at_f(u, P.g, lx, ly, lz, d);
-    return P.trilinear(lx + d[0] + 0.5f, ly + d[1] + 0.5f, lz + d[2] + 0.5f);
+    disp_at_f(u, P.g, ix, iy, iz, rx, ry, rz, d);
+    return P.trilinear(ix, iy, iz, rx + d[0], ry + d[1], rz + d[2]);
 }
 
 // Offsets (units of h) of the 19-sample stencils, SDF.cpp:847-865 / DisplacementField.cpp:305-323.
@@ -95,17 +104,17 @@ __device__ inline void ref_gradient(const PhiView<S> &L, const PhiView<S> &Cn, c
                                     const RefConsts &c, int x, int y, int z, float psi, float out[3])
 {
     const float h = c.h;
-    float gl[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+    // Samples sit at gridLocation + offset, gridLocation = idx + 0.5 (SDF.cpp:729,845).
     float dg[3] = {0.f, 0.f, 0.f}, lg[3] = {0.f, 0.f, 0.f}, kg[3] = {0.f, 0.f, 0.f};
     float gr[3] = {0.f, 0.f, 0.f};
     if (c.use_data || c.use_ls) {
         // SDF::computeDistanceGradient(idx, disp), SDF.cpp:726-744.
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            float lp[3] = {gl[0], gl[1], gl[2]}, lm[3] = {gl[0], gl[1], gl[2]};
-            lp[a] = gl[a] + h;
-            lm[a] = gl[a] - h;
-            gr[a] = distance_f(L, u, lp[0], lp[1], lp[2]) - distance_f(L, u, lm[0], lm[1], lm[2]);
+            float lp[3] = {0.5f, 0.5f, 0.5f}, lm[3] = {0.5f, 0.5f, 0.5f};
+            lp[a] = 0.5f + h;
+            lm[a] = 0.5f - h;
+            gr[a] = distance_f(L, u, x, y, z, lp[0], lp[1], lp[2]) - distance_f(L, u, x, y, z, lm[0], lm[1], lm[2]);
         }
 #pragma unroll
         for (int a = 0; a < 3; a++)
@@ -123,8 +132,8 @@ __device__ inline void ref_gradient(const PhiView<S> &L, const PhiView<S> &Cn, c
         // SDF::computeDistanceHessian(idx, disp), SDF.cpp:842-886.
         float f[19];
         for (int k = 0; k < 19; k++)
-            f[k] = distance_f(L, u, gl[0] + h * (float)c_off[k][0], gl[1] + h * (float)c_off[k][1],
-                              gl[2] + h * (float)c_off[k][2]);
+            f[k] = distance_f(L, u, x, y, z, 0.5f + h * (float)c_off[k][0], 0.5f + h * (float)c_off[k][1],
+                              0.5f + h * (float)c_off[k][2]);
         float den = (4.0f * h * h);
         float fxx = (f[1] - 2.0f * f[0] + f[2]) / den;
         float fyy = (f[3] - 2.0f * f[0] + f[4]) / den;
@@ -148,37 +157,37 @@ __device__ inline void ref_gradient(const PhiView<S> &L, const PhiView<S> &Cn, c
         float xx[3] = {0, 0, 0}, yy[3] = {0, 0, 0}, zz[3] = {0, 0, 0};
         float xy[3] = {0, 0, 0}, xz[3] = {0, 0, 0}, yz[3] = {0, 0, 0};
         float d0[3];
-        disp_at_f(u, L.g, (float)x, (float)y, (float)z, d0);
+        disp_at_f(u, L.g, x, y, z, 0.0f, 0.0f, 0.0f, d0);
         float d[3];
         // pure second differences
-        disp_at_f(u, L.g, (float)x + 2.0f * h, (float)y, (float)z, d);
+        disp_at_f(u, L.g, x, y, z, 2.0f * h, 0.0f, 0.0f, d);
         for (int k = 0; k < 3; k++) xx[k] = d[k] - 2.0f * d0[k];
-        disp_at_f(u, L.g, (float)x - 2.0f * h, (float)y, (float)z, d);
+        disp_at_f(u, L.g, x, y, z, -2.0f * h, 0.0f, 0.0f, d);
         for (int k = 0; k < 3; k++) xx[k] = xx[k] + d[k];
-        disp_at_f(u, L.g, (float)x, (float)y + 2.0f * h, (float)z, d);
+        disp_at_f(u, L.g, x, y, z, 0.0f, 2.0f * h, 0.0f, d);
         for (int k = 0; k < 3; k++) yy[k] = d[k] - 2.0f * d0[k];
-        disp_at_f(u, L.g, (float)x, (float)y - 2.0f * h, (float)z, d);
+        disp_at_f(u, L.g, x, y, z, 0.0f, -2.0f * h, 0.0f, d);
         for (int k = 0; k < 3; k++) yy[k] = yy[k] + d[k];
-        disp_at_f(u, L.g, (float)x, (float)y, (float)z + 2.0f * h, d);
+        disp_at_f(u, L.g, x, y, z, 0.0f, 0.0f, 2.0f * h, d);
         for (int k = 0; k < 3; k++) zz[k] = d[k] - 2.0f * d0[k];
-        disp_at_f(u, L.g, (float)x, (float)y, (float)z - 2.0f * h, d);
+        disp_at_f(u, L.g, x, y, z, 0.0f, 0.0f, -2.0f * h, d);
         for (int k = 0; k < 3; k++) zz[k] = zz[k] + d[k];
         // mixed: (+,+) + (-,-) - (+,-) - (-,+)
         float pp[3], mm[3], pm[3], mp[3];
-        disp_at_f(u, L.g, (float)x + h, (float)y, (float)z + h, pp);
-        disp_at_f(u, L.g, (float)x - h, (float)y, (float)z - h, mm);
-        disp_at_f(u, L.g, (float)x + h, (float)y, (float)z - h, pm);
-        disp_at_f(u, L.g, (float)x - h, (float)y, (float)z + h, mp);
+        disp_at_f(u, L.g, x, y, z, h, 0.0f, h, pp);
+        disp_at_f(u, L.g, x, y, z, -h, 0.0f, -h, mm);
+        disp_at_f(u, L.g, x, y, z, h, 0.0f, -h, pm);
+        disp_at_f(u, L.g, x, y, z, -h, 0.0f, h, mp);
         for (int k = 0; k < 3; k++) xz[k] = pp[k] + mm[k] - pm[k] - mp[k];
-        disp_at_f(u, L.g, (float)x, (float)y + h, (float)z + h, pp);
-        disp_at_f(u, L.g, (float)x, (float)y - h, (float)z - h, mm);
-        disp_at_f(u, L.g, (float)x, (float)y + h, (float)z - h, pm);
-        disp_at_f(u, L.g, (float)x, (float)y - h, (float)z + h, mp);
+        disp_at_f(u, L.g, x, y, z, 0.0f, h, h, pp);
+        disp_at_f(u, L.g, x, y, z, 0.0f, -h, -h, mm);
+        disp_at_f(u, L.g, x, y, z, 0.0f, h, -h, pm);
+        disp_at_f(u, L.g, x, y, z, 0.0f, -h, h, mp);
         for (int k = 0; k < 3; k++) yz[k] = pp[k] + mm[k] - pm[k] - mp[k];
-        disp_at_f(u, L.g, (float)x + h, (float)y + h, (float)z, pp);
-        disp_at_f(u, L.g, (float)x - h, (float)y - h, (float)z, mm);
-        disp_at_f(u, L.g, (float)x + h, (float)y - h, (float)z, pm);
-        disp_at_f(u, L.g, (float)x - h, (float)y + h, (float)z, mp);
+        disp_at_f(u, L.g, x, y, z, h, h, 0.0f, pp);
+        disp_at_f(u, L.g, x, y, z, -h, -h, 0.0f, mm);
+        disp_at_f(u, L.g, x, y, z, h, -h, 0.0f, pm);
+        disp_at_f(u, L.g, x, y, z, -h, h, 0.0f, mp);
         for (int k = 0; k < 3; k++) xy[k] = pp[k] + mm[k] - pm[k] - mp[k];
         float div0 = xx[0] + xy[1] + xz[2];
         float div1 = xy[0] + yy[1] + yz[2];
@@ -205,7 +214,7 @@ __device__ inline void ref_gradient(const PhiView<S> &L, const PhiView<S> &Cn, c
 template <class S, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_ref_step(PhiView<S> L, PhiView<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, RefConsts c, DeformState *st,
-                                                    double *partials)
+                                                    BlockPartial *partials)
 {
     if (MODE == KF_MODE_STEP && st->done)
         return;
@@ -219,7 +228,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_step(PhiView<S> L, PhiView<S> Cn,
         const int z = (int)(i / ((int64_t)g.nx * g.ny));
         float u0 = u_in[i], u1 = u_in[g.n + i], u2 = u_in[2 * g.n + i];
         // SDF::getDistance(idx, disp), SDF.cpp:570-575; skip test KillingFusion.cpp:272-274.
-        float psi = L.trilinear((float)x + u0 + 0.5f, (float)y + u1 + 0.5f, (float)z + u2 + 0.5f);
+        float psi = L.trilinear(x, y, z, u0, u1, u2);
         bool active = !(psi > c.hi || psi < c.lo);
         float gr[3] = {0.f, 0.f, 0.f};
         if (active) {
@@ -247,8 +256,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_step(PhiView<S> L, PhiView<S> Cn,
     }
     if (MODE == KF_MODE_STEP) {
         double sum = 0.0;
-        kf_block_reduce3<BLOCK>(sum, mx, act);
-        kf_block_finalize<BLOCK>(st, partials, gridDim.x, blockIdx.x, sum, mx, act);
+        kf_block_publish<BLOCK>(partials, blockIdx.x, sum, mx, act);
     }
 }
 
@@ -298,5 +306,5 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
                                u_out, c, ctx->state, ctx->partials);
     });
     KF_LAUNCH_CHECK();
-    return KF_OK;
+    return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb);
 }

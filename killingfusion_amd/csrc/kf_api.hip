@@ -1,6 +1,6 @@
 // kf_api.hip -- the C ABI of libkf (include/kf_api.h): handles, transfers and the
 // optimiser driver.  Kernels live in k_*.hip.
-#include "kf_internal.hpp"
+#include "kf_reduce.hpp"
 
 #include <climits>
 #include <cstring>
@@ -19,10 +19,6 @@ __global__ void k_state_reset(DeformState *st, double stop)
     if (threadIdx.x == 0) {
         st->done = 0;
         st->iters = 0;
-        st->ticket = 0;
-        st->pad0 = 0;
-        st->max_bits = 0ull;
-        st->active = 0ull;
         st->diverged = LLONG_MAX;
         st->stop = stop;
         st->last_max = 0.0;
@@ -119,8 +115,48 @@ kf_status kf_ensure_partials(kf_context *ctx, size_t nblocks)
         ctx->partials = nullptr;
         ctx->partials_cap = 0;
     }
-    KF_HIP(hipMalloc((void **)&ctx->partials, sizeof(double) * nblocks));
+    KF_HIP(hipMalloc((void **)&ctx->partials, sizeof(BlockPartial) * nblocks));
     ctx->partials_cap = nblocks;
+    return KF_OK;
+}
+
+namespace {
+
+constexpr int FIN_BLOCK = 1024;
+
+// Ends one deform iteration: folds the per-block partials in index order (so the FP64
+// energy sum is reproducible), records max |du| and the active count, advances the
+// iteration counter and raises `done` when the stop rule fires or a voxel diverged.
+__global__ __launch_bounds__(FIN_BLOCK) void k_finalize(DeformState *st, const BlockPartial *parts, unsigned int nb)
+{
+    if (st->done)
+        return;
+    double s = 0.0, m = 0.0;
+    unsigned long long c = 0;
+    for (unsigned int b = threadIdx.x; b < nb; b += FIN_BLOCK) {
+        const BlockPartial p = parts[b];
+        s += p.sum;
+        m = fmax(m, p.max);
+        c += p.cnt;
+    }
+    kf_block_reduce3<FIN_BLOCK>(s, m, c);
+    if (threadIdx.x == 0) {
+        st->last_max = m;
+        st->energy = s;
+        st->last_active = c;
+        st->iters += 1;
+        if (st->diverged != LLONG_MAX || (st->stop > 0.0 && m < st->stop))
+            st->done = 1;
+    }
+}
+
+}  // namespace
+
+kf_status kf_launch_finalize(kf_context *ctx, unsigned int nb)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->state,
+                       (const BlockPartial *)ctx->partials, nb);
+    KF_LAUNCH_CHECK();
     return KF_OK;
 }
 

@@ -45,17 +45,21 @@ void kf_set_error(const std::string &msg);
 // ---------------------------------------------------------------------------
 
 struct DeformState {
-    unsigned int done;          // set by the last block when the stop rule fires
+    unsigned int done;          // set by k_finalize when the stop rule fires or a voxel diverged
     unsigned int iters;         // iterations completed
-    unsigned int ticket;        // block arrival counter of the running launch
-    unsigned int pad0;
-    unsigned long long max_bits;    // running max |du| of the running launch (double bits, >= 0)
-    unsigned long long active;      // active voxels of the running launch
     long long diverged;             // lowest index with a non-finite update, INT64_MAX if none
     double stop;                    // stop threshold (<= 0: never)
     double last_max;                // max |du| of the last completed launch
     double energy;                  // energy of the last completed launch
     unsigned long long last_active; // active voxels of the last completed launch
+};
+
+// Per-block partial results of one step launch, folded by k_finalize.
+struct BlockPartial {
+    double sum;               // energy
+    double max;               // max |du|
+    unsigned long long cnt;   // active voxels
+    unsigned long long pad;
 };
 
 // ---------------------------------------------------------------------------
@@ -68,7 +72,7 @@ struct kf_context {
     bool own_stream = false;
     DeformState *state = nullptr;     // device
     DeformState *state_host = nullptr;  // pinned mirror for stats readback
-    double *partials = nullptr;       // device, per-block FP64 partial sums
+    BlockPartial *partials = nullptr;  // device, one slot per block of a step launch
     size_t partials_cap = 0;
     int num_cus = 256;
 };
@@ -182,6 +186,24 @@ __device__ __host__ inline unsigned int kf_xcd_tile(unsigned int b, unsigned int
     return xcd * per + (xcd < rem ? xcd : rem) + local;
 }
 
+// Base index and weight of the sample at i + r (i an integer grid index, r a small offset),
+// with the reference's truncating cast: base = (int)(i + r), frac = (i + r) - base
+// (SDF.cpp:509, DisplacementField.cpp:43).  i + r is never formed in FP32: at i ~ 200 that
+// would round r to ~1e-5 voxel and move samples across cell faces; taking the floor of r
+// alone keeps the weight accurate to the precision of r.
+__device__ __host__ inline void kf_split(int i, float r, int &b, float &f)
+{
+    const float fl = floorf(r);
+    int bi = i + (int)fl;
+    float fr = r - fl;
+    if (bi < 0 && fr > 0.0f) {  // i + r < 0, not an integer: truncation rounds toward zero
+        bi += 1;
+        fr -= 1.0f;
+    }
+    b = bi;
+    f = fr;
+}
+
 // Interpolation chain of utils.h:6-46 (x, then y, then z; v0*(1-x) + v1*x).
 template <typename R>
 __device__ __host__ inline R kf_lerp(R v0, R v1, R x) { return v0 * (R(1) - x) + v1 * x; }
@@ -201,6 +223,8 @@ __device__ __host__ inline R kf_interp3(R v000, R v001, R v010, R v011, R v100, 
 
 kf_status kf_ensure_partials(kf_context *ctx, size_t nblocks);
 kf_status kf_state_reset(kf_context *ctx, double stop);
+// Folds nb block partials into the iteration state (one block, fixed order).
+kf_status kf_launch_finalize(kf_context *ctx, unsigned int nb);
 
 kf_status kf_launch_fill_volume(kf_volume *v);
 kf_status kf_launch_upload_volume(kf_volume *v, const float *dphi_f32, const int32_t *dw);

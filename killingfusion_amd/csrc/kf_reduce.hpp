@@ -1,10 +1,8 @@
-// kf_reduce.hpp -- wavefront/block reductions and the last-block finalize of a
-// deform iteration (FP64 max |du|, FP64 energy sum in a fixed order, active count).
+// kf_reduce.hpp -- wavefront/block reductions of a deform iteration's FP64 max |du|,
+// FP64 energy sum (fixed order) and active count.
 #pragma once
 
 #include "kf_internal.hpp"
-
-#include <climits>
 
 // Wavefront (64 lanes) butterfly reductions.
 __device__ inline double kf_wave_sum(double v)
@@ -64,49 +62,20 @@ __device__ inline void kf_block_reduce3(double &sum, double &mx, unsigned long l
     }
 }
 
-// Publish this block's partials; the block that arrives last folds every block's
-// energy partial in index order, finalises the iteration into st and re-arms the
-// counters for the next launch.  Must be called by all threads of the block.
+// Thread 0 stores the block's (sum, max, count) in its slot; k_finalize (kf_api.hip) folds
+// the slots in index order after the launch.  No same-address atomics: with tens of
+// thousands of blocks those serialise in L2 and cost more than the stencil itself.
 template <int BLOCK>
-__device__ inline void kf_block_finalize(DeformState *st, double *partials, unsigned int nblocks,
-                                         unsigned int block_id, double sum, double mx,
-                                         unsigned long long cnt)
+__device__ inline void kf_block_publish(BlockPartial *parts, unsigned int slot, double sum, double mx,
+                                        unsigned long long cnt)
 {
-    __shared__ bool s_last;
+    kf_block_reduce3<BLOCK>(sum, mx, cnt);
     if (threadIdx.x == 0) {
-        partials[block_id] = sum;
-        if (mx > 0.0)
-            atomicMax(&st->max_bits, (unsigned long long)__double_as_longlong(mx));
-        if (cnt)
-            atomicAdd(&st->active, cnt);
-        __threadfence();
-        unsigned int t = atomicAdd(&st->ticket, 1u);
-        s_last = (t == nblocks - 1);
-    }
-    __syncthreads();
-    if (!s_last)
-        return;
-    __threadfence();
-    double acc = 0.0;
-    for (unsigned int b = threadIdx.x; b < nblocks; b += BLOCK)
-        acc += __builtin_nontemporal_load(&partials[b]);
-    double dummy_m = 0.0;
-    unsigned long long dummy_c = 0;
-    kf_block_reduce3<BLOCK>(acc, dummy_m, dummy_c);
-    if (threadIdx.x == 0) {
-        unsigned long long mb = atomicAdd(&st->max_bits, 0ull);
-        unsigned long long act = atomicAdd(&st->active, 0ull);
-        double m = __longlong_as_double((long long)mb);
-        st->last_max = m;
-        st->energy = acc;
-        st->last_active = act;
-        st->iters += 1;
-        long long dv = atomicAdd((unsigned long long *)&st->diverged, 0ull);
-        if (dv != LLONG_MAX || (st->stop > 0.0 && m < st->stop))
-            st->done = 1;
-        st->max_bits = 0ull;
-        st->active = 0ull;
-        st->ticket = 0u;
-        __threadfence();
+        BlockPartial p;
+        p.sum = sum;
+        p.max = mx;
+        p.cnt = cnt;
+        p.pad = 0ull;
+        parts[slot] = p;
     }
 }

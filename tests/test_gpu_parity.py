@@ -139,10 +139,7 @@ def test_reference_deform_fixed_iterations(ctx, c1):
     uo, so = O.deform_jacobi(lphi, cphi, np.zeros((3, c1["g"].n)), c1["g"], oparams(kp))
     err = err_report("ref 12 it", ug, uo)
     assert st["iters"] == so["iters"] == 12
-    # a voxel whose skip decision flips on float noise takes one different step; allow a
-    # handful and require the rest within 1e-4 voxel
-    assert (err > 1e-4).sum() <= 5
-    assert np.median(err[np.abs(uo).max(axis=0) > 0]) < 1e-6
+    assert (err > 1e-4).sum() <= 5 and err.max() <= 1e-3
 
 
 def test_adjoint_gradient_and_energy_match_oracle(ctx, c1):
