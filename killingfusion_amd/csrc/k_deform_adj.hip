@@ -18,22 +18,23 @@
 // 1-neighbourhood of u (12 neighbours), and writes u_j - alpha dE/du_j for active voxels.
 // Compulsory HBM traffic per voxel-iteration: 2 s_phi + 24 bytes.
 //
-// Layout of the launch: a block owns a 64 x 4 (x, y) column of the grid and marches TZ
-// slices in z.  The u neighbourhood lives in LDS as three rolling slices (z-1, z, z+1) with
-// a one-voxel halo in x and y, so each u value is fetched from memory about once per
-// block instead of 13 times per voxel; the live SDF gather is 8 taps straight from L1/L2
-// (the sample moves with u).  Blocks are numbered so each XCD sweeps a contiguous z range
-// and the slices shared by neighbouring tiles stay in that XCD's L2.
+// Two launch shapes share the per-voxel arithmetic (adj_voxel):
+//  * k_adj_rows (rows whose length is a multiple of 4): a thread owns 4 consecutive x voxels,
+//    a wavefront 256 voxels of a row, a block 4 rows of one slice.  All u rows the stencil
+//    needs are independent float4 loads (x neighbours come from lane shuffles), issued
+//    together, then the 32 gather taps -- two rounds of wide, independent memory requests
+//    per thread and no barriers, which keeps enough bytes in flight to cover HBM latency.
+//  * k_adj_any (any shape): one voxel per thread, neighbours read individually.
+// Blocks are numbered so each XCD sweeps a contiguous z range and the rows and slices shared
+// by neighbouring blocks stay in that XCD's L2.
 #include "kf_reduce.hpp"
 
 namespace {
 
-constexpr int TX = 64;          // x extent of a tile: one wavefront
-constexpr int TY = 4;           // y extent: four wavefronts
-constexpr int BLOCK = TX * TY;
-constexpr int LW = TX + 2;      // LDS row with x halo
-constexpr int LH = TY + 2;      // LDS rows with y halo
-constexpr int TZ = 16;          // slices marched per block
+constexpr int BLOCK = 256;
+constexpr int VEC = 4;                  // voxels per thread along x in k_adj_rows
+constexpr int ROW = 64 * VEC;           // voxels of a row segment per wavefront
+constexpr int ROWS = BLOCK / 64;        // rows per block
 
 struct AdjConsts {
     float inv_2v, inv_v, hi, lo, eps2, alpha, omega_k, gamma, omega_s;
@@ -50,7 +51,8 @@ struct PhiRead {
     {
         return g.in(x, y, z) ? S::load(p, g.idx(x, y, z), scale) : oob;
     }
-    // Trilinear value, gradient and mixed second derivatives at base (bx, by, bz) + (fx, fy, fz).
+    // The 8 corners of cell (bx, by, bz) in utils.h order (x fastest); out-of-grid corners
+    // read Max + eps (SDF.cpp:482-487).
     __device__ inline void cell(int bx, int by, int bz, float v[8]) const
     {
         if (bx >= 0 && by >= 0 && bz >= 0 && bx + 1 < g.nx && by + 1 < g.ny && bz + 1 < g.nz) {
@@ -77,208 +79,344 @@ struct PhiRead {
     }
 };
 
+// u neighbourhood of voxel j: centre, face neighbours up[b][a] = u^a_{j+e_b},
+// um[b][a] = u^a_{j-e_b}, and the six edge neighbours dgl[a][b] = u^b_{j+e_a-e_b} (a != b)
+// the gamma term needs.  Zero outside the grid.
+struct Nbhd {
+    float uc[3];
+    float up[3][3], um[3][3];
+    float dgl[3][3];
+};
+
+struct VoxelOut {
+    float g[3];
+    float e;
+    bool active;
+};
+
+// dE/du_j and the energy owned by voxel j (see the file header for E).  `v` are the 8
+// live-SDF corners of the cell holding j + u_j, (fx, fy, fz) its weights.
+template <bool NEED_E>
+__device__ inline void adj_voxel(const AdjConsts &c, const float v[8], float fx, float fy, float fz, float can,
+                                 const Nbhd &nb, int x, int y, int z, VoxelOut &o)
+{
+    const float psi = kf_interp3(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], fx, fy, fz);
+    o.active = !(psi > c.hi || psi < c.lo);
+    float gr0 = 0.f, gr1 = 0.f, gr2 = 0.f, e = 0.f;
+    if (o.active) {
+        const float dx00 = v[1] - v[0], dx10 = v[3] - v[2], dx01 = v[5] - v[4], dx11 = v[7] - v[6];
+        const float dy00 = v[2] - v[0], dy10 = v[3] - v[1], dy01 = v[6] - v[4], dy11 = v[7] - v[5];
+        const float dz00 = v[4] - v[0], dz10 = v[5] - v[1], dz01 = v[6] - v[2], dz11 = v[7] - v[3];
+        const float g0 = kf_lerp(kf_lerp(dx00, dx10, fy), kf_lerp(dx01, dx11, fy), fz);
+        const float g1 = kf_lerp(kf_lerp(dy00, dy10, fx), kf_lerp(dy01, dy11, fx), fz);
+        const float g2 = kf_lerp(kf_lerp(dz00, dz10, fx), kf_lerp(dz01, dz11, fx), fy);
+        if (c.use_data) {
+            const float res = psi - can;
+            if (NEED_E)
+                e += res * res * c.inv_2v;
+            const float s = res * c.inv_v;
+            gr0 += s * g0;
+            gr1 += s * g1;
+            gr2 += s * g2;
+        }
+        if (c.use_ls) {
+            const float hxy = kf_lerp(dx10 - dx00, dx11 - dx01, fz);
+            const float hxz = kf_lerp(dx01 - dx00, dx11 - dx10, fy);
+            const float hyz = kf_lerp(dy01 - dy00, dy11 - dy10, fx);
+            const float sl = sqrtf(g0 * g0 + g1 * g1 + g2 * g2 + c.eps2);
+            if (NEED_E)
+                e += c.omega_s * 0.5f * (sl - 1.0f) * (sl - 1.0f);
+            const float k = c.omega_s * (sl - 1.0f) / sl;
+            gr0 += k * (hxy * g1 + hxz * g2);
+            gr1 += k * (hxy * g0 + hyz * g2);
+            gr2 += k * (hxz * g0 + hyz * g1);
+        }
+    }
+    if (c.use_k) {
+        float kg[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float lap = 0.f;
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+                lap += nb.up[b][a] - 2.0f * nb.uc[a] + nb.um[b][a];
+            float dv = 0.f;
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                // D_a u^b at j and at j - e_b
+                const float d_j = nb.up[a][b] - nb.uc[b];
+                const float d_m = (a == b) ? (nb.uc[b] - nb.um[b][b]) : (nb.dgl[a][b] - nb.um[b][b]);
+                dv += d_j - d_m;
+            }
+            kg[a] = -2.0f * (lap + c.gamma * dv);
+        }
+        gr0 += c.omega_k * kg[0];
+        gr1 += c.omega_k * kg[1];
+        gr2 += c.omega_k * kg[2];
+        if (NEED_E) {
+            // ||D u_j||^2 + gamma tr((D u_j)^2) + the ghost terms of the i = j - e_c voxels
+            // on low faces.
+            float ek = 0.f, tr = 0.f;
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    const float dab = nb.up[b][a] - nb.uc[a];
+                    const float dba = nb.up[a][b] - nb.uc[b];
+                    ek += dab * dab;
+                    tr += dab * dba;
+                }
+            ek += c.gamma * tr;
+            const int jj[3] = {x, y, z};
+            const float s2 = nb.uc[0] * nb.uc[0] + nb.uc[1] * nb.uc[1] + nb.uc[2] * nb.uc[2];
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++)
+                if (jj[cc] == 0)
+                    ek += s2 + c.gamma * nb.uc[cc] * nb.uc[cc];
+            e += c.omega_k * ek;
+        }
+    }
+    o.g[0] = gr0;
+    o.g[1] = gr1;
+    o.g[2] = gr2;
+    o.e = e;
+}
+
+// Jacobi update of one voxel: active voxels move by -alpha dE/du; non-finite results are
+// reported (KillingFusion.cpp:327-332).  Returns the new value in n[3].
+__device__ inline void adj_update(const AdjConsts &c, const VoxelOut &o, const float uc[3], int64_t i,
+                                  DeformState *st, double &mx, float n[3])
+{
+    n[0] = uc[0];
+    n[1] = uc[1];
+    n[2] = uc[2];
+    if (o.active) {
+        const float d0 = -c.alpha * o.g[0], d1 = -c.alpha * o.g[1], d2 = -c.alpha * o.g[2];
+        n[0] = uc[0] + d0;
+        n[1] = uc[1] + d1;
+        n[2] = uc[2] + d2;
+        if (!(isfinite(n[0]) && isfinite(n[1]) && isfinite(n[2])))
+            atomicMin((long long *)&st->diverged, (long long)i);
+        else
+            mx = fmax(mx, sqrt((double)d0 * d0 + (double)d1 * d1 + (double)d2 * d2));
+    }
+}
+
 __device__ inline float uval(const float *__restrict__ u, const GridDims &g, int c, int x, int y, int z)
 {
     return g.in(x, y, z) ? u[(int64_t)c * g.n + g.idx(x, y, z)] : 0.0f;
 }
 
-// Fills LDS slot `s` with slice zz of u (zero outside the grid), one-voxel halo in x and y.
-__device__ inline void load_slice(float (*su)[3][LH][LW], int s, const float *__restrict__ u, const GridDims &g,
-                                  int x0, int y0, int zz, int lane, int wy)
-{
-    const int x = x0 + lane, y = y0 + wy;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        su[s][c][wy + 1][lane + 1] = uval(u, g, c, x, y, zz);
-        if (wy == 0)
-            su[s][c][0][lane + 1] = uval(u, g, c, x, y0 - 1, zz);
-        if (wy == TY - 1)
-            su[s][c][TY + 1][lane + 1] = uval(u, g, c, x, y0 + TY, zz);
-    }
-    if (lane == 0 || lane == TX - 1) {
-        const int col = lane == 0 ? 0 : TX + 1;
-        const int xx = lane == 0 ? x0 - 1 : x0 + TX;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            su[s][c][wy + 1][col] = uval(u, g, c, xx, y, zz);
-            if (wy == 0)
-                su[s][c][0][col] = uval(u, g, c, xx, y0 - 1, zz);
-            if (wy == TY - 1)
-                su[s][c][TY + 1][col] = uval(u, g, c, xx, y0 + TY, zz);
-        }
-    }
-}
-
+// ---------------------------------------------------------------------------
+// Any shape: one voxel per thread.
+// ---------------------------------------------------------------------------
 template <class S, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_adj_step(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
-                                                    float *__restrict__ u_out, AdjConsts c, DeformState *st,
-                                                    BlockPartial *partials, float *__restrict__ e_voxel, int ntx,
-                                                    int nty)
+__global__ __launch_bounds__(BLOCK) void k_adj_any(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
+                                                   float *__restrict__ u_out, AdjConsts c, DeformState *st,
+                                                   BlockPartial *partials, float *__restrict__ e_voxel)
 {
     if (MODE == KF_MODE_STEP && st->done)
         return;
-    __shared__ float su[3][3][LH][LW];
+    const GridDims g = L.g;
+    const int64_t i = (int64_t)kf_xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    double mx = 0.0, esum = 0.0;
+    unsigned long long act = 0;
+    if (i < g.n) {
+        const int x = (int)(i % g.nx);
+        const int y = (int)((i / g.nx) % g.ny);
+        const int z = (int)(i / ((int64_t)g.nx * g.ny));
+        Nbhd nb;
+        nb.uc[0] = u_in[i];
+        nb.uc[1] = u_in[g.n + i];
+        nb.uc[2] = u_in[2 * g.n + i];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            nb.up[0][a] = uval(u_in, g, a, x + 1, y, z);
+            nb.um[0][a] = uval(u_in, g, a, x - 1, y, z);
+            nb.up[1][a] = uval(u_in, g, a, x, y + 1, z);
+            nb.um[1][a] = uval(u_in, g, a, x, y - 1, z);
+            nb.up[2][a] = uval(u_in, g, a, x, y, z + 1);
+            nb.um[2][a] = uval(u_in, g, a, x, y, z - 1);
+        }
+        nb.dgl[0][1] = uval(u_in, g, 1, x + 1, y - 1, z);
+        nb.dgl[0][2] = uval(u_in, g, 2, x + 1, y, z - 1);
+        nb.dgl[1][0] = uval(u_in, g, 0, x - 1, y + 1, z);
+        nb.dgl[1][2] = uval(u_in, g, 2, x, y + 1, z - 1);
+        nb.dgl[2][0] = uval(u_in, g, 0, x - 1, y, z + 1);
+        nb.dgl[2][1] = uval(u_in, g, 1, x, y - 1, z + 1);
+        int bx, by, bz;
+        float fx, fy, fz, v[8];
+        kf_split(x, nb.uc[0], bx, fx);
+        kf_split(y, nb.uc[1], by, fy);
+        kf_split(z, nb.uc[2], bz, fz);
+        L.cell(bx, by, bz, v);
+        VoxelOut o;
+        adj_voxel<MODE != KF_MODE_GRAD>(c, v, fx, fy, fz, c.use_data ? Cn.at(x, y, z) : 0.f, nb, x, y, z, o);
+        if (o.active)
+            act = 1;
+        if (MODE == KF_MODE_GRAD) {
+            u_out[i] = o.g[0];
+            u_out[g.n + i] = o.g[1];
+            u_out[2 * g.n + i] = o.g[2];
+        } else if (MODE == KF_MODE_ENERGY) {
+            if (e_voxel)
+                e_voxel[i] = o.e;
+        } else {
+            float n[3];
+            adj_update(c, o, nb.uc, i, st, mx, n);
+            u_out[i] = n[0];
+            u_out[g.n + i] = n[1];
+            u_out[2 * g.n + i] = n[2];
+        }
+        esum = (double)o.e;
+    }
+    if (MODE != KF_MODE_GRAD)
+        kf_block_publish<BLOCK>(partials, blockIdx.x, esum, mx, act);
+}
+
+// ---------------------------------------------------------------------------
+// Rows of a multiple of 4 voxels: 4 voxels per thread, float4 row loads.
+// ---------------------------------------------------------------------------
+
+// Row segment of component c at (x0 .. x0+3, y, z); zero when the row is out of the grid.
+__device__ inline float4 row4(const float *__restrict__ u, const GridDims &g, int c, int x0, int y, int z)
+{
+    if (x0 < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz)
+        return *reinterpret_cast<const float4 *>(u + (int64_t)c * g.n + g.idx(x0, y, z));
+    return make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__device__ inline float f4(const float4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+
+// Value just left of the thread's segment (x0 - 1) and just right (x0 + 4): from the
+// neighbouring lanes' segments, or loaded by the wavefront's edge lanes.
+__device__ inline float left_of(const float4 &v, const float *__restrict__ u, const GridDims &g, int c, int x0, int y,
+                                int z, int lane)
+{
+    float l = __shfl_up(v.w, 1, 64);
+    if (lane == 0)
+        l = uval(u, g, c, x0 - 1, y, z);
+    return l;
+}
+
+__device__ inline float right_of(const float4 &v, const float *__restrict__ u, const GridDims &g, int c, int x0, int y,
+                                 int z, int lane)
+{
+    float r = __shfl_down(v.x, 1, 64);
+    if (lane == 63)
+        r = uval(u, g, c, x0 + VEC, y, z);
+    return r;
+}
+
+template <class S, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
+                                                    float *__restrict__ u_out, AdjConsts c, DeformState *st,
+                                                    BlockPartial *partials, float *__restrict__ e_voxel, int nseg,
+                                                    int nyt)
+{
+    if (MODE == KF_MODE_STEP && st->done)
+        return;
     const GridDims g = L.g;
     const unsigned int tile = kf_xcd_tile(blockIdx.x, gridDim.x);
-    const int tx = (int)(tile % (unsigned)ntx);
-    const int ty = (int)((tile / (unsigned)ntx) % (unsigned)nty);
-    const int tz = (int)(tile / ((unsigned)ntx * (unsigned)nty));
-    const int lane = threadIdx.x % TX, wy = threadIdx.x / TX;
-    const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
-    const int x = x0 + lane, y = y0 + wy;
-    const bool valid = x < g.nx && y < g.ny;
-    const int zend = min(z0 + TZ, g.nz);
+    const int seg = (int)(tile % (unsigned)nseg);
+    const int yt = (int)((tile / (unsigned)nseg) % (unsigned)nyt);
+    const int z = (int)(tile / ((unsigned)nseg * (unsigned)nyt));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int y = yt * ROWS + w;
+    const int x0 = seg * ROW + lane * VEC;
+
+    // Every row this thread's 4 voxels touch, as independent float4 loads.
+    float4 C[3], YP[3], YM[3], ZP[3], ZM[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        C[a] = row4(u_in, g, a, x0, y, z);
+        YP[a] = row4(u_in, g, a, x0, y + 1, z);
+        YM[a] = row4(u_in, g, a, x0, y - 1, z);
+        ZP[a] = row4(u_in, g, a, x0, y, z + 1);
+        ZM[a] = row4(u_in, g, a, x0, y, z - 1);
+    }
+    const float4 YPZM2 = row4(u_in, g, 2, x0, y + 1, z - 1);  // u^2 at (x, y+1, z-1)
+    const float4 YMZP1 = row4(u_in, g, 1, x0, y - 1, z + 1);  // u^1 at (x, y-1, z+1)
+    float canv[VEC] = {0.f, 0.f, 0.f, 0.f};
+    const bool valid = x0 < g.nx && y < g.ny;
+    if (valid && c.use_data) {
+#pragma unroll
+        for (int k = 0; k < VEC; k++)
+            canv[k] = S::load(Cn.p, g.idx(x0 + k, y, z), Cn.scale);
+    }
+    // x neighbours across segment boundaries (all lanes take part in the shuffles).
+    float CL[3], CR[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        CL[a] = left_of(C[a], u_in, g, a, x0, y, z, lane);
+        CR[a] = right_of(C[a], u_in, g, a, x0, y, z, lane);
+    }
+    const float YM1R = right_of(YM[1], u_in, g, 1, x0, y - 1, z, lane);  // u^1 (x0+4, y-1, z)
+    const float YP0L = left_of(YP[0], u_in, g, 0, x0, y + 1, z, lane);   // u^0 (x0-1, y+1, z)
+    const float ZM2R = right_of(ZM[2], u_in, g, 2, x0, y, z - 1, lane);  // u^2 (x0+4, y, z-1)
+    const float ZP0L = left_of(ZP[0], u_in, g, 0, x0, y, z + 1, lane);   // u^0 (x0-1, y, z+1)
 
     double mx = 0.0, esum = 0.0;
     unsigned long long act = 0;
-
-    int sp = 0, sc = 1, sn = 2;  // LDS slots of slices z-1, z, z+1
-    load_slice(su, sp, u_in, g, x0, y0, z0 - 1, lane, wy);
-    load_slice(su, sc, u_in, g, x0, y0, z0, lane, wy);
-    for (int z = z0; z < zend; z++) {
-        load_slice(su, sn, u_in, g, x0, y0, z + 1, lane, wy);
-        __syncthreads();
-        if (valid) {
-            const int64_t i = g.idx(x, y, z);
-            const int r = wy + 1, q = lane + 1;
-            const float u0 = su[sc][0][r][q], u1 = su[sc][1][r][q], u2 = su[sc][2][r][q];
-            // One 8-tap gather gives psi, grad and the mixed second derivatives at j + u_j.
+    if (valid) {
+        // All 32 gather taps first, then the arithmetic.
+        float v[VEC][8], fx[VEC], fy[VEC], fz[VEC];
+#pragma unroll
+        for (int k = 0; k < VEC; k++) {
             int bx, by, bz;
-            float fx, fy, fz;
-            kf_split(x, u0, bx, fx);
-            kf_split(y, u1, by, fy);
-            kf_split(z, u2, bz, fz);
-            float v[8];
-            L.cell(bx, by, bz, v);
-            const float psi = kf_interp3(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], fx, fy, fz);
-            const bool active = !(psi > c.hi || psi < c.lo);
-            float gr0 = 0.f, gr1 = 0.f, gr2 = 0.f;
-            float e = 0.f;
-            if (active) {
-                act += 1;
-                const float dx00 = v[1] - v[0], dx10 = v[3] - v[2], dx01 = v[5] - v[4], dx11 = v[7] - v[6];
-                const float dy00 = v[2] - v[0], dy10 = v[3] - v[1], dy01 = v[6] - v[4], dy11 = v[7] - v[5];
-                const float dz00 = v[4] - v[0], dz10 = v[5] - v[1], dz01 = v[6] - v[2], dz11 = v[7] - v[3];
-                const float g0 = kf_lerp(kf_lerp(dx00, dx10, fy), kf_lerp(dx01, dx11, fy), fz);
-                const float g1 = kf_lerp(kf_lerp(dy00, dy10, fx), kf_lerp(dy01, dy11, fx), fz);
-                const float g2 = kf_lerp(kf_lerp(dz00, dz10, fx), kf_lerp(dz01, dz11, fx), fy);
-                if (c.use_data) {
-                    const float res = psi - Cn.at(x, y, z);
-                    e += res * res * c.inv_2v;
-                    const float s = res * c.inv_v;
-                    gr0 += s * g0;
-                    gr1 += s * g1;
-                    gr2 += s * g2;
-                }
-                if (c.use_ls) {
-                    const float hxy = kf_lerp(dx10 - dx00, dx11 - dx01, fz);
-                    const float hxz = kf_lerp(dx01 - dx00, dx11 - dx10, fy);
-                    const float hyz = kf_lerp(dy01 - dy00, dy11 - dy10, fx);
-                    const float sl = sqrtf(g0 * g0 + g1 * g1 + g2 * g2 + c.eps2);
-                    e += c.omega_s * 0.5f * (sl - 1.0f) * (sl - 1.0f);
-                    const float k = c.omega_s * (sl - 1.0f) / sl;
-                    gr0 += k * (hxy * g1 + hxz * g2);
-                    gr1 += k * (hxy * g0 + hyz * g2);
-                    gr2 += k * (hxz * g0 + hyz * g1);
-                }
-            }
-            if (c.use_k) {
-                // up[b][a] = u^a_{j+e_b}, um[b][a] = u^a_{j-e_b}; zero outside the grid.
-                float up[3][3], um[3][3];
-                const float u_c[3] = {u0, u1, u2};
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    up[0][a] = su[sc][a][r][q + 1];
-                    um[0][a] = su[sc][a][r][q - 1];
-                    up[1][a] = su[sc][a][r + 1][q];
-                    um[1][a] = su[sc][a][r - 1][q];
-                    up[2][a] = su[sn][a][r][q];
-                    um[2][a] = su[sp][a][r][q];
-                }
-                // dgl[a][b] = u^b_{j+e_a-e_b}, a != b
-                float dgl[3][3];
-                dgl[0][1] = su[sc][1][r - 1][q + 1];
-                dgl[0][2] = su[sp][2][r][q + 1];
-                dgl[1][0] = su[sc][0][r + 1][q - 1];
-                dgl[1][2] = su[sp][2][r + 1][q];
-                dgl[2][0] = su[sn][0][r][q - 1];
-                dgl[2][1] = su[sn][1][r - 1][q];
-                float kg[3];
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    float lap = 0.f;
-#pragma unroll
-                    for (int b = 0; b < 3; b++)
-                        lap += up[b][a] - 2.0f * u_c[a] + um[b][a];
-                    float dv = 0.f;
-#pragma unroll
-                    for (int b = 0; b < 3; b++) {
-                        // D_a u^b at j and at j - e_b
-                        const float d_j = up[a][b] - u_c[b];
-                        const float d_m = (a == b) ? (u_c[b] - um[b][b]) : (dgl[a][b] - um[b][b]);
-                        dv += d_j - d_m;
-                    }
-                    kg[a] = -2.0f * (lap + c.gamma * dv);
-                }
-                gr0 += c.omega_k * kg[0];
-                gr1 += c.omega_k * kg[1];
-                gr2 += c.omega_k * kg[2];
-                if (MODE != KF_MODE_GRAD) {
-                    // Energy owned by j: ||D u_j||^2 + gamma tr((D u_j)^2) + ghost terms on low faces.
-                    float D[3][3];
-#pragma unroll
-                    for (int a = 0; a < 3; a++)
-#pragma unroll
-                        for (int b = 0; b < 3; b++)
-                            D[a][b] = up[b][a] - u_c[a];
-                    float ek = 0.f, tr = 0.f;
-#pragma unroll
-                    for (int a = 0; a < 3; a++)
-#pragma unroll
-                        for (int b = 0; b < 3; b++) {
-                            ek += D[a][b] * D[a][b];
-                            tr += D[a][b] * D[b][a];
-                        }
-                    ek += c.gamma * tr;
-                    const int jj[3] = {x, y, z};
-                    const float s2 = u0 * u0 + u1 * u1 + u2 * u2;
-#pragma unroll
-                    for (int cc = 0; cc < 3; cc++)
-                        if (jj[cc] == 0)
-                            ek += s2 + c.gamma * u_c[cc] * u_c[cc];
-                    e += c.omega_k * ek;
-                }
-            }
-            if (MODE == KF_MODE_GRAD) {
-                u_out[i] = gr0;
-                u_out[g.n + i] = gr1;
-                u_out[2 * g.n + i] = gr2;
-            } else if (MODE == KF_MODE_ENERGY) {
-                if (e_voxel)
-                    e_voxel[i] = e;
-            } else {
-                float n0 = u0, n1 = u1, n2 = u2;
-                if (active) {
-                    const float d0 = -c.alpha * gr0, d1 = -c.alpha * gr1, d2 = -c.alpha * gr2;
-                    n0 = u0 + d0;
-                    n1 = u1 + d1;
-                    n2 = u2 + d2;
-                    if (!(isfinite(n0) && isfinite(n1) && isfinite(n2)))
-                        atomicMin((long long *)&st->diverged, (long long)i);
-                    else
-                        mx = fmax(mx, sqrt((double)d0 * d0 + (double)d1 * d1 + (double)d2 * d2));
-                }
-                u_out[i] = n0;
-                u_out[g.n + i] = n1;
-                u_out[2 * g.n + i] = n2;
-            }
-            esum += (double)e;
+            kf_split(x0 + k, f4(C[0], k), bx, fx[k]);
+            kf_split(y, f4(C[1], k), by, fy[k]);
+            kf_split(z, f4(C[2], k), bz, fz[k]);
+            L.cell(bx, by, bz, v[k]);
         }
-        __syncthreads();  // slot sp is refilled next step
-        const int t = sp;
-        sp = sc;
-        sc = sn;
-        sn = t;
+        float outv[3][VEC], ev[VEC];
+#pragma unroll
+        for (int k = 0; k < VEC; k++) {
+            Nbhd nb;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                nb.uc[a] = f4(C[a], k);
+                nb.up[0][a] = k < VEC - 1 ? f4(C[a], k + 1) : CR[a];
+                nb.um[0][a] = k > 0 ? f4(C[a], k - 1) : CL[a];
+                nb.up[1][a] = f4(YP[a], k);
+                nb.um[1][a] = f4(YM[a], k);
+                nb.up[2][a] = f4(ZP[a], k);
+                nb.um[2][a] = f4(ZM[a], k);
+            }
+            nb.dgl[0][1] = k < VEC - 1 ? f4(YM[1], k + 1) : YM1R;
+            nb.dgl[0][2] = k < VEC - 1 ? f4(ZM[2], k + 1) : ZM2R;
+            nb.dgl[1][0] = k > 0 ? f4(YP[0], k - 1) : YP0L;
+            nb.dgl[1][2] = f4(YPZM2, k);
+            nb.dgl[2][0] = k > 0 ? f4(ZP[0], k - 1) : ZP0L;
+            nb.dgl[2][1] = f4(YMZP1, k);
+            VoxelOut o;
+            adj_voxel<MODE != KF_MODE_GRAD>(c, v[k], fx[k], fy[k], fz[k], canv[k], nb, x0 + k, y, z, o);
+            if (o.active)
+                act += 1;
+            ev[k] = o.e;
+            esum += (double)o.e;
+            if (MODE == KF_MODE_GRAD) {
+                outv[0][k] = o.g[0];
+                outv[1][k] = o.g[1];
+                outv[2][k] = o.g[2];
+            } else if (MODE == KF_MODE_STEP) {
+                float n[3];
+                adj_update(c, o, nb.uc, g.idx(x0 + k, y, z), st, mx, n);
+                outv[0][k] = n[0];
+                outv[1][k] = n[1];
+                outv[2][k] = n[2];
+            }
+        }
+        const int64_t i0 = g.idx(x0, y, z);
+        if (MODE == KF_MODE_ENERGY) {
+            if (e_voxel)
+                *reinterpret_cast<float4 *>(e_voxel + i0) = make_float4(ev[0], ev[1], ev[2], ev[3]);
+        } else {
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                *reinterpret_cast<float4 *>(u_out + (int64_t)a * g.n + i0) =
+                    make_float4(outv[a][0], outv[a][1], outv[a][2], outv[a][3]);
+        }
     }
     if (MODE != KF_MODE_GRAD)
         kf_block_publish<BLOCK>(partials, blockIdx.x, esum, mx, act);
@@ -293,6 +431,18 @@ PhiRead<S> make_read(const kf_volume *v)
     P.scale = v->scale();
     P.oob = (float)v->oob();
     return P;
+}
+
+template <class S, int MODE>
+void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float *u_out, const AdjConsts &c,
+            kf_context *ctx, float *e_voxel, bool rows, unsigned int nb, int nseg, int nyt)
+{
+    if (rows)
+        hipLaunchKernelGGL((k_adj_rows<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
+                           ctx->state, ctx->partials, e_voxel, nseg, nyt);
+    else
+        hipLaunchKernelGGL((k_adj_any<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
+                           ctx->state, ctx->partials, e_voxel);
 }
 
 }  // namespace
@@ -315,24 +465,23 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     c.use_ls = p->use_levelset;
     c.use_k = p->use_killing;
     KF_CHECK_ARG(live->dtype == can->dtype, "live and canonical volumes must share a dtype");
-    const int ntx = (live->dims[0] + TX - 1) / TX;
-    const int nty = (live->dims[1] + TY - 1) / TY;
-    const int ntz = (live->dims[2] + TZ - 1) / TZ;
-    const unsigned int nb = (unsigned int)(ntx * nty * ntz);
+    // float4 row loads need rows of a multiple of 4 voxels (plane starts are then 16-byte aligned).
+    const bool rows = live->dims[0] % VEC == 0;
+    const int nseg = (live->dims[0] + ROW - 1) / ROW;
+    const int nyt = (live->dims[1] + ROWS - 1) / ROWS;
+    const unsigned int nb = rows ? (unsigned int)(nseg * nyt * live->dims[2])
+                                 : (unsigned int)((live->n + BLOCK - 1) / BLOCK);
     kf_status s = kf_ensure_partials(ctx, nb);
     if (s != KF_OK)
         return s;
     KF_DISPATCH_DTYPE(live->dtype, S, {
         PhiRead<S> L = make_read<S>(live), Cn = make_read<S>(can);
         if (mode == KF_MODE_GRAD)
-            hipLaunchKernelGGL((k_adj_step<S, KF_MODE_GRAD>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in,
-                               u_out, c, ctx->state, ctx->partials, e_voxel, ntx, nty);
+            launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt);
         else if (mode == KF_MODE_ENERGY)
-            hipLaunchKernelGGL((k_adj_step<S, KF_MODE_ENERGY>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn,
-                               u_in, u_out, c, ctx->state, ctx->partials, e_voxel, ntx, nty);
+            launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt);
         else
-            hipLaunchKernelGGL((k_adj_step<S, KF_MODE_STEP>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in,
-                               u_out, c, ctx->state, ctx->partials, e_voxel, ntx, nty);
+            launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt);
     });
     KF_LAUNCH_CHECK();
     return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb);
