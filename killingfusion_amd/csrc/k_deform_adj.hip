@@ -356,22 +356,62 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
     const float ZM2R = right_of(ZM[2], u_in, g, 2, x0, y, z - 1, lane);  // u^2 (x0+4, y, z-1)
     const float ZP0L = left_of(ZP[0], u_in, g, 0, x0, y, z + 1, lane);   // u^0 (x0-1, y, z+1)
 
+    // Quiet thread: every u value its four voxels touch is zero (everywhere away from the
+    // surface band).  Then psi_j = phi_live[j] exactly (zero interpolation weights), and an
+    // inactive quiet voxel has zero gradient and zero energy, so its arithmetic is skipped;
+    // its loads and stores still happen.
+    unsigned int bits = 0u;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        bits |= __float_as_uint(C[a].x) | __float_as_uint(C[a].y) | __float_as_uint(C[a].z) | __float_as_uint(C[a].w);
+        bits |= __float_as_uint(YP[a].x) | __float_as_uint(YP[a].y) | __float_as_uint(YP[a].z) | __float_as_uint(YP[a].w);
+        bits |= __float_as_uint(YM[a].x) | __float_as_uint(YM[a].y) | __float_as_uint(YM[a].z) | __float_as_uint(YM[a].w);
+        bits |= __float_as_uint(ZP[a].x) | __float_as_uint(ZP[a].y) | __float_as_uint(ZP[a].z) | __float_as_uint(ZP[a].w);
+        bits |= __float_as_uint(ZM[a].x) | __float_as_uint(ZM[a].y) | __float_as_uint(ZM[a].z) | __float_as_uint(ZM[a].w);
+        bits |= __float_as_uint(CL[a]) | __float_as_uint(CR[a]);
+    }
+    bits |= __float_as_uint(YPZM2.x) | __float_as_uint(YPZM2.y) | __float_as_uint(YPZM2.z) | __float_as_uint(YPZM2.w);
+    bits |= __float_as_uint(YMZP1.x) | __float_as_uint(YMZP1.y) | __float_as_uint(YMZP1.z) | __float_as_uint(YMZP1.w);
+    bits |= __float_as_uint(YM1R) | __float_as_uint(YP0L) | __float_as_uint(ZM2R) | __float_as_uint(ZP0L);
+    const bool quiet = (bits & 0x7fffffffu) == 0u;
+
     double mx = 0.0, esum = 0.0;
     unsigned long long act = 0;
     if (valid) {
-        // All 32 gather taps first, then the arithmetic.
+        bool need[VEC];
+        if (quiet) {
+#pragma unroll
+            for (int k = 0; k < VEC; k++) {
+                const float psi = S::load(L.p, g.idx(x0 + k, y, z), L.scale);
+                need[k] = !(psi > c.hi || psi < c.lo);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VEC; k++)
+                need[k] = true;
+        }
+        // All gather taps first, then the arithmetic.
         float v[VEC][8], fx[VEC], fy[VEC], fz[VEC];
 #pragma unroll
         for (int k = 0; k < VEC; k++) {
-            int bx, by, bz;
-            kf_split(x0 + k, f4(C[0], k), bx, fx[k]);
-            kf_split(y, f4(C[1], k), by, fy[k]);
-            kf_split(z, f4(C[2], k), bz, fz[k]);
-            L.cell(bx, by, bz, v[k]);
+            if (need[k]) {
+                int bx, by, bz;
+                kf_split(x0 + k, f4(C[0], k), bx, fx[k]);
+                kf_split(y, f4(C[1], k), by, fy[k]);
+                kf_split(z, f4(C[2], k), bz, fz[k]);
+                L.cell(bx, by, bz, v[k]);
+            }
         }
         float outv[3][VEC], ev[VEC];
 #pragma unroll
         for (int k = 0; k < VEC; k++) {
+            if (!need[k]) {
+                ev[k] = 0.f;
+                outv[0][k] = MODE == KF_MODE_GRAD ? 0.f : f4(C[0], k);
+                outv[1][k] = MODE == KF_MODE_GRAD ? 0.f : f4(C[1], k);
+                outv[2][k] = MODE == KF_MODE_GRAD ? 0.f : f4(C[2], k);
+                continue;
+            }
             Nbhd nb;
 #pragma unroll
             for (int a = 0; a < 3; a++) {
