@@ -286,26 +286,33 @@ __device__ inline float4 row4(const float *__restrict__ u, const GridDims &g, in
     return make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-__device__ inline float f4(const float4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
-
-// Value just left of the thread's segment (x0 - 1) and just right (x0 + 4): from the
-// neighbouring lanes' segments, or loaded by the wavefront's edge lanes.
-__device__ inline float left_of(const float4 &v, const float *__restrict__ u, const GridDims &g, int c, int x0, int y,
-                                int z, int lane)
+__device__ inline unsigned int bits4(const float4 &v)
 {
-    float l = __shfl_up(v.w, 1, 64);
-    if (lane == 0)
-        l = uval(u, g, c, x0 - 1, y, z);
-    return l;
+    return __float_as_uint(v.x) | __float_as_uint(v.y) | __float_as_uint(v.z) | __float_as_uint(v.w);
 }
 
-__device__ inline float right_of(const float4 &v, const float *__restrict__ u, const GridDims &g, int c, int x0, int y,
-                                 int z, int lane)
+__device__ inline float f4(const float4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+
+// u neighbourhood of voxel (x, y, z) read value by value (cache hits: the rows were just
+// streamed in by the quiet test).
+__device__ inline void gather_nbhd(const float *__restrict__ u, const GridDims &g, int x, int y, int z, Nbhd &nb)
 {
-    float r = __shfl_down(v.x, 1, 64);
-    if (lane == 63)
-        r = uval(u, g, c, x0 + VEC, y, z);
-    return r;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        nb.uc[a] = uval(u, g, a, x, y, z);
+        nb.up[0][a] = uval(u, g, a, x + 1, y, z);
+        nb.um[0][a] = uval(u, g, a, x - 1, y, z);
+        nb.up[1][a] = uval(u, g, a, x, y + 1, z);
+        nb.um[1][a] = uval(u, g, a, x, y - 1, z);
+        nb.up[2][a] = uval(u, g, a, x, y, z + 1);
+        nb.um[2][a] = uval(u, g, a, x, y, z - 1);
+    }
+    nb.dgl[0][1] = uval(u, g, 1, x + 1, y - 1, z);
+    nb.dgl[0][2] = uval(u, g, 2, x + 1, y, z - 1);
+    nb.dgl[1][0] = uval(u, g, 0, x - 1, y + 1, z);
+    nb.dgl[1][2] = uval(u, g, 2, x, y + 1, z - 1);
+    nb.dgl[2][0] = uval(u, g, 0, x - 1, y, z + 1);
+    nb.dgl[2][1] = uval(u, g, 1, x, y - 1, z + 1);
 }
 
 template <class S, int MODE>
@@ -324,113 +331,72 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int y = yt * ROWS + w;
     const int x0 = seg * ROW + lane * VEC;
-
-    // Every row this thread's 4 voxels touch, as independent float4 loads.
-    float4 C[3], YP[3], YM[3], ZP[3], ZM[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        C[a] = row4(u_in, g, a, x0, y, z);
-        YP[a] = row4(u_in, g, a, x0, y + 1, z);
-        YM[a] = row4(u_in, g, a, x0, y - 1, z);
-        ZP[a] = row4(u_in, g, a, x0, y, z + 1);
-        ZM[a] = row4(u_in, g, a, x0, y, z - 1);
-    }
-    const float4 YPZM2 = row4(u_in, g, 2, x0, y + 1, z - 1);  // u^2 at (x, y+1, z-1)
-    const float4 YMZP1 = row4(u_in, g, 1, x0, y - 1, z + 1);  // u^1 at (x, y-1, z+1)
-    float canv[VEC] = {0.f, 0.f, 0.f, 0.f};
     const bool valid = x0 < g.nx && y < g.ny;
-    if (valid && c.use_data) {
-#pragma unroll
-        for (int k = 0; k < VEC; k++)
-            canv[k] = S::load(Cn.p, g.idx(x0 + k, y, z), Cn.scale);
-    }
-    // x neighbours across segment boundaries (all lanes take part in the shuffles).
-    float CL[3], CR[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        CL[a] = left_of(C[a], u_in, g, a, x0, y, z, lane);
-        CR[a] = right_of(C[a], u_in, g, a, x0, y, z, lane);
-    }
-    const float YM1R = right_of(YM[1], u_in, g, 1, x0, y - 1, z, lane);  // u^1 (x0+4, y-1, z)
-    const float YP0L = left_of(YP[0], u_in, g, 0, x0, y + 1, z, lane);   // u^0 (x0-1, y+1, z)
-    const float ZM2R = right_of(ZM[2], u_in, g, 2, x0, y, z - 1, lane);  // u^2 (x0+4, y, z-1)
-    const float ZP0L = left_of(ZP[0], u_in, g, 0, x0, y, z + 1, lane);   // u^0 (x0-1, y, z+1)
 
-    // Quiet thread: every u value its four voxels touch is zero (everywhere away from the
-    // surface band).  Then psi_j = phi_live[j] exactly (zero interpolation weights), and an
-    // inactive quiet voxel has zero gradient and zero energy, so its arithmetic is skipped;
-    // its loads and stores still happen.
+    // Phase 1 -- quiet test.  Every u row the four voxels' stencils touch is streamed in as
+    // an independent float4 load and folded into one bit mask; the x neighbours just outside
+    // the segment belong to the adjacent lanes, whose masks are folded in too.  A quiet
+    // thread (all zero, i.e. away from the surface band) has psi_j = phi_live[j] exactly
+    // (zero interpolation weights), and its inactive voxels have zero gradient and energy.
+    float4 C[3];
     unsigned int bits = 0u;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        bits |= __float_as_uint(C[a].x) | __float_as_uint(C[a].y) | __float_as_uint(C[a].z) | __float_as_uint(C[a].w);
-        bits |= __float_as_uint(YP[a].x) | __float_as_uint(YP[a].y) | __float_as_uint(YP[a].z) | __float_as_uint(YP[a].w);
-        bits |= __float_as_uint(YM[a].x) | __float_as_uint(YM[a].y) | __float_as_uint(YM[a].z) | __float_as_uint(YM[a].w);
-        bits |= __float_as_uint(ZP[a].x) | __float_as_uint(ZP[a].y) | __float_as_uint(ZP[a].z) | __float_as_uint(ZP[a].w);
-        bits |= __float_as_uint(ZM[a].x) | __float_as_uint(ZM[a].y) | __float_as_uint(ZM[a].z) | __float_as_uint(ZM[a].w);
-        bits |= __float_as_uint(CL[a]) | __float_as_uint(CR[a]);
+        C[a] = row4(u_in, g, a, x0, y, z);
+        bits |= bits4(C[a]) | bits4(row4(u_in, g, a, x0, y + 1, z)) | bits4(row4(u_in, g, a, x0, y - 1, z)) |
+                bits4(row4(u_in, g, a, x0, y, z + 1)) | bits4(row4(u_in, g, a, x0, y, z - 1));
     }
-    bits |= __float_as_uint(YPZM2.x) | __float_as_uint(YPZM2.y) | __float_as_uint(YPZM2.z) | __float_as_uint(YPZM2.w);
-    bits |= __float_as_uint(YMZP1.x) | __float_as_uint(YMZP1.y) | __float_as_uint(YMZP1.z) | __float_as_uint(YMZP1.w);
-    bits |= __float_as_uint(YM1R) | __float_as_uint(YP0L) | __float_as_uint(ZM2R) | __float_as_uint(ZP0L);
-    const bool quiet = (bits & 0x7fffffffu) == 0u;
+    bits |= bits4(row4(u_in, g, 2, x0, y + 1, z - 1)) | bits4(row4(u_in, g, 1, x0, y - 1, z + 1));
+    unsigned int nbits = __shfl_up(bits, 1, 64) | __shfl_down(bits, 1, 64);
+    if (lane == 0 && x0 > 0)
+        nbits = 0xffffffffu;  // segment edge: the left neighbour lives in another wavefront
+    if (lane == 63 && x0 + VEC < g.nx)
+        nbits = 0xffffffffu;
+    const bool quiet = ((bits | nbits) & 0x7fffffffu) == 0u;
 
+    float canv[VEC] = {0.f, 0.f, 0.f, 0.f};
     double mx = 0.0, esum = 0.0;
     unsigned long long act = 0;
     if (valid) {
+        const int64_t i0 = g.idx(x0, y, z);
+        if (c.use_data)
+            S::load4(Cn.p, i0, Cn.scale, canv);
         bool need[VEC];
         if (quiet) {
+            float psi[VEC];
+            S::load4(L.p, i0, L.scale, psi);
 #pragma unroll
-            for (int k = 0; k < VEC; k++) {
-                const float psi = S::load(L.p, g.idx(x0 + k, y, z), L.scale);
-                need[k] = !(psi > c.hi || psi < c.lo);
-            }
+            for (int k = 0; k < VEC; k++)
+                need[k] = !(psi[k] > c.hi || psi[k] < c.lo);
         } else {
 #pragma unroll
             for (int k = 0; k < VEC; k++)
                 need[k] = true;
         }
-        // All gather taps first, then the arithmetic.
-        float v[VEC][8], fx[VEC], fy[VEC], fz[VEC];
-#pragma unroll
-        for (int k = 0; k < VEC; k++) {
-            if (need[k]) {
-                int bx, by, bz;
-                kf_split(x0 + k, f4(C[0], k), bx, fx[k]);
-                kf_split(y, f4(C[1], k), by, fy[k]);
-                kf_split(z, f4(C[2], k), bz, fz[k]);
-                L.cell(bx, by, bz, v[k]);
-            }
-        }
         float outv[3][VEC], ev[VEC];
 #pragma unroll
         for (int k = 0; k < VEC; k++) {
-            if (!need[k]) {
-                ev[k] = 0.f;
-                outv[0][k] = MODE == KF_MODE_GRAD ? 0.f : f4(C[0], k);
-                outv[1][k] = MODE == KF_MODE_GRAD ? 0.f : f4(C[1], k);
-                outv[2][k] = MODE == KF_MODE_GRAD ? 0.f : f4(C[2], k);
+            ev[k] = 0.f;
+            outv[0][k] = MODE == KF_MODE_GRAD ? 0.f : f4(C[0], k);
+            outv[1][k] = MODE == KF_MODE_GRAD ? 0.f : f4(C[1], k);
+            outv[2][k] = MODE == KF_MODE_GRAD ? 0.f : f4(C[2], k);
+        }
+        // Phase 2 -- the voxels that can contribute: full energy and gradient.
+#pragma unroll 1
+        for (int k = 0; k < VEC; k++) {
+            if (!need[k])
                 continue;
-            }
+            const int x = x0 + k;
             Nbhd nb;
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                nb.uc[a] = f4(C[a], k);
-                nb.up[0][a] = k < VEC - 1 ? f4(C[a], k + 1) : CR[a];
-                nb.um[0][a] = k > 0 ? f4(C[a], k - 1) : CL[a];
-                nb.up[1][a] = f4(YP[a], k);
-                nb.um[1][a] = f4(YM[a], k);
-                nb.up[2][a] = f4(ZP[a], k);
-                nb.um[2][a] = f4(ZM[a], k);
-            }
-            nb.dgl[0][1] = k < VEC - 1 ? f4(YM[1], k + 1) : YM1R;
-            nb.dgl[0][2] = k < VEC - 1 ? f4(ZM[2], k + 1) : ZM2R;
-            nb.dgl[1][0] = k > 0 ? f4(YP[0], k - 1) : YP0L;
-            nb.dgl[1][2] = f4(YPZM2, k);
-            nb.dgl[2][0] = k > 0 ? f4(ZP[0], k - 1) : ZP0L;
-            nb.dgl[2][1] = f4(YMZP1, k);
+            gather_nbhd(u_in, g, x, y, z, nb);
+            int bx, by, bz;
+            float fx, fy, fz, v[8];
+            kf_split(x, nb.uc[0], bx, fx);
+            kf_split(y, nb.uc[1], by, fy);
+            kf_split(z, nb.uc[2], bz, fz);
+            L.cell(bx, by, bz, v);
             VoxelOut o;
-            adj_voxel<MODE != KF_MODE_GRAD>(c, v[k], fx[k], fy[k], fz[k], canv[k], nb, x0 + k, y, z, o);
+            adj_voxel<MODE != KF_MODE_GRAD>(c, v, fx, fy, fz, canv[k], nb, x, y, z, o);
             if (o.active)
                 act += 1;
             ev[k] = o.e;
@@ -441,13 +407,12 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
                 outv[2][k] = o.g[2];
             } else if (MODE == KF_MODE_STEP) {
                 float n[3];
-                adj_update(c, o, nb.uc, g.idx(x0 + k, y, z), st, mx, n);
+                adj_update(c, o, nb.uc, i0 + k, st, mx, n);
                 outv[0][k] = n[0];
                 outv[1][k] = n[1];
                 outv[2][k] = n[2];
             }
         }
-        const int64_t i0 = g.idx(x0, y, z);
         if (MODE == KF_MODE_ENERGY) {
             if (e_voxel)
                 *reinterpret_cast<float4 *>(e_voxel + i0) = make_float4(ev[0], ev[1], ev[2], ev[3]);

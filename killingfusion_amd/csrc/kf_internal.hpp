@@ -110,6 +110,15 @@ struct StoreF32 {
     using T = float;
     static constexpr int dtype = KF_F32;
     __device__ __host__ static inline float load(const T *p, int64_t i, float) { return p[i]; }
+    // Four consecutive values starting at a multiple of 4 (one 16-byte load).
+    __device__ static inline void load4(const T *p, int64_t i, float, float o[4])
+    {
+        const float4 v = *reinterpret_cast<const float4 *>(p + i);
+        o[0] = v.x;
+        o[1] = v.y;
+        o[2] = v.z;
+        o[3] = v.w;
+    }
     __device__ __host__ static inline T enc(float v, float) { return v; }
     __device__ __host__ static inline float dec(T v, float) { return v; }
 };
@@ -124,6 +133,15 @@ struct StoreBF16 {
         return c.f * scale;
     }
     __device__ __host__ static inline float load(const T *p, int64_t i, float scale) { return dec(p[i], scale); }
+    // Four consecutive values starting at a multiple of 4 (one 8-byte load).
+    __device__ static inline void load4(const T *p, int64_t i, float scale, float o[4])
+    {
+        const uint2 v = *reinterpret_cast<const uint2 *>(p + i);
+        o[0] = __uint_as_float(v.x << 16) * scale;
+        o[1] = __uint_as_float(v.x & 0xffff0000u) * scale;
+        o[2] = __uint_as_float(v.y << 16) * scale;
+        o[3] = __uint_as_float(v.y & 0xffff0000u) * scale;
+    }
     // Round to nearest even (NaN kept quiet).
     __device__ __host__ static inline T enc(float v, float scale)
     {
@@ -141,6 +159,15 @@ struct StoreF16 {
     static constexpr int dtype = KF_F16;
     __device__ __host__ static inline float dec(T v, float scale) { return (float)v * scale; }
     __device__ __host__ static inline float load(const T *p, int64_t i, float scale) { return dec(p[i], scale); }
+    // Four consecutive values starting at a multiple of 4 (one 8-byte load).
+    __device__ static inline void load4(const T *p, int64_t i, float scale, float o[4])
+    {
+        const uint2 v = *reinterpret_cast<const uint2 *>(p + i);
+        o[0] = (float)__builtin_bit_cast(_Float16, (unsigned short)(v.x & 0xffffu)) * scale;
+        o[1] = (float)__builtin_bit_cast(_Float16, (unsigned short)(v.x >> 16)) * scale;
+        o[2] = (float)__builtin_bit_cast(_Float16, (unsigned short)(v.y & 0xffffu)) * scale;
+        o[3] = (float)__builtin_bit_cast(_Float16, (unsigned short)(v.y >> 16)) * scale;
+    }
     __device__ __host__ static inline T enc(float v, float scale) { return (T)(v / scale); }
 };
 
