@@ -315,11 +315,20 @@ __device__ inline void gather_nbhd(const float *__restrict__ u, const GridDims &
     nb.dgl[2][1] = uval(u, g, 1, x, y - 1, z + 1);
 }
 
+// Index of the row tile (segment, 4-row group, slice) that tile coordinates name, or -1.
+__device__ inline int tile_of(int seg, int yt, int z, int nseg, int nyt, int nz)
+{
+    if (seg < 0 || seg >= nseg || yt < 0 || yt >= nyt || z < 0 || z >= nz)
+        return -1;
+    return (z * nyt + yt) * nseg + seg;
+}
+
 template <class S, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, AdjConsts c, DeformState *st,
                                                     BlockPartial *partials, float *__restrict__ e_voxel, int nseg,
-                                                    int nyt)
+                                                    int nyt, const uint8_t *__restrict__ flags_in,
+                                                    uint8_t *__restrict__ flags_out)
 {
     if (MODE == KF_MODE_STEP && st->done)
         return;
@@ -333,46 +342,60 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
     const int x0 = seg * ROW + lane * VEC;
     const bool valid = x0 < g.nx && y < g.ny;
 
-    // Phase 1 -- quiet test.  Every u row the four voxels' stencils touch is streamed in as
-    // an independent float4 load and folded into one bit mask; the x neighbours just outside
-    // the segment belong to the adjacent lanes, whose masks are folded in too.  A quiet
-    // thread (all zero, i.e. away from the surface band) has psi_j = phi_live[j] exactly
-    // (zero interpolation weights), and its inactive voxels have zero gradient and energy.
+    // Phase 1 -- quiet test.  A voxel is quiet when every u value its stencil touches is
+    // zero (everywhere away from the surface band); then psi_j = phi_live[j] exactly (zero
+    // interpolation weights) and an inactive quiet voxel has zero gradient and energy.
+    // With tile flags (flags_in[t] != 0 iff tile t of u_in holds a nonzero value) a block whose
+    // 27 neighbouring tiles are all zero is quiet as a whole and reads only its own row;
+    // otherwise every u row the four voxels touch is streamed in as an independent float4
+    // load and folded into a bit mask, with the adjacent lanes' masks covering x +- 1.
     float4 C[3];
-    unsigned int bits = 0u;
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
+    for (int a = 0; a < 3; a++)
         C[a] = row4(u_in, g, a, x0, y, z);
-        bits |= bits4(C[a]) | bits4(row4(u_in, g, a, x0, y + 1, z)) | bits4(row4(u_in, g, a, x0, y - 1, z)) |
-                bits4(row4(u_in, g, a, x0, y, z + 1)) | bits4(row4(u_in, g, a, x0, y, z - 1));
-    }
-    bits |= bits4(row4(u_in, g, 2, x0, y + 1, z - 1)) | bits4(row4(u_in, g, 1, x0, y - 1, z + 1));
-    unsigned int nbits = __shfl_up(bits, 1, 64) | __shfl_down(bits, 1, 64);
-    if (lane == 0 && x0 > 0)
-        nbits = 0xffffffffu;  // segment edge: the left neighbour lives in another wavefront
-    if (lane == 63 && x0 + VEC < g.nx)
-        nbits = 0xffffffffu;
-    const bool quiet = ((bits | nbits) & 0x7fffffffu) == 0u;
-
-    float canv[VEC] = {0.f, 0.f, 0.f, 0.f};
-    double mx = 0.0, esum = 0.0;
-    unsigned long long act = 0;
+    float live4[VEC] = {0.f, 0.f, 0.f, 0.f}, canv[VEC] = {0.f, 0.f, 0.f, 0.f};
     if (valid) {
         const int64_t i0 = g.idx(x0, y, z);
+        S::load4(L.p, i0, L.scale, live4);
         if (c.use_data)
             S::load4(Cn.p, i0, Cn.scale, canv);
-        bool need[VEC];
-        if (quiet) {
-            float psi[VEC];
-            S::load4(L.p, i0, L.scale, psi);
-#pragma unroll
-            for (int k = 0; k < VEC; k++)
-                need[k] = !(psi[k] > c.hi || psi[k] < c.lo);
-        } else {
-#pragma unroll
-            for (int k = 0; k < VEC; k++)
-                need[k] = true;
+    }
+    bool tile_nz = true;
+    if (flags_in) {
+        int pred = ((bits4(C[0]) | bits4(C[1]) | bits4(C[2])) & 0x7fffffffu) != 0u;
+        if (threadIdx.x < 27) {
+            const int t = tile_of(seg + (int)(threadIdx.x % 3) - 1, yt + (int)(threadIdx.x / 3 % 3) - 1,
+                                  z + (int)(threadIdx.x / 9) - 1, nseg, nyt, g.nz);
+            if (t >= 0 && flags_in[t])
+                pred = 1;
         }
+        tile_nz = __syncthreads_or(pred) != 0;
+    }
+    bool quiet = true;
+    if (tile_nz) {
+        unsigned int bits = 0u;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            bits |= bits4(C[a]) | bits4(row4(u_in, g, a, x0, y + 1, z)) | bits4(row4(u_in, g, a, x0, y - 1, z)) |
+                    bits4(row4(u_in, g, a, x0, y, z + 1)) | bits4(row4(u_in, g, a, x0, y, z - 1));
+        bits |= bits4(row4(u_in, g, 2, x0, y + 1, z - 1)) | bits4(row4(u_in, g, 1, x0, y - 1, z + 1));
+        unsigned int nbits = __shfl_up(bits, 1, 64) | __shfl_down(bits, 1, 64);
+        if (lane == 0 && x0 > 0)
+            nbits = 0xffffffffu;  // segment edge: the left neighbour lives in another wavefront
+        if (lane == 63 && x0 + VEC < g.nx)
+            nbits = 0xffffffffu;
+        quiet = ((bits | nbits) & 0x7fffffffu) == 0u;
+    }
+
+    double mx = 0.0, esum = 0.0;
+    unsigned long long act = 0;
+    int out_nz = 0;
+    if (valid) {
+        const int64_t i0 = g.idx(x0, y, z);
+        bool need[VEC];
+#pragma unroll
+        for (int k = 0; k < VEC; k++)
+            need[k] = !quiet || !(live4[k] > c.hi || live4[k] < c.lo);
         float outv[3][VEC], ev[VEC];
 #pragma unroll
         for (int k = 0; k < VEC; k++) {
@@ -418,13 +441,40 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
                 *reinterpret_cast<float4 *>(e_voxel + i0) = make_float4(ev[0], ev[1], ev[2], ev[3]);
         } else {
 #pragma unroll
-            for (int a = 0; a < 3; a++)
-                *reinterpret_cast<float4 *>(u_out + (int64_t)a * g.n + i0) =
-                    make_float4(outv[a][0], outv[a][1], outv[a][2], outv[a][3]);
+            for (int a = 0; a < 3; a++) {
+                const float4 o4 = make_float4(outv[a][0], outv[a][1], outv[a][2], outv[a][3]);
+                out_nz |= (bits4(o4) & 0x7fffffffu) != 0u;
+                *reinterpret_cast<float4 *>(u_out + (int64_t)a * g.n + i0) = o4;
+            }
         }
+    }
+    if (MODE == KF_MODE_STEP && flags_out) {
+        const int nz_tile = __syncthreads_or(out_nz);
+        if (threadIdx.x == 0)
+            flags_out[tile] = (uint8_t)(nz_tile != 0);
     }
     if (MODE != KF_MODE_GRAD)
         kf_block_publish<BLOCK>(partials, blockIdx.x, esum, mx, act);
+}
+
+// Tile flags of a field: flags[t] = 1 iff row tile t of u holds a nonzero value.
+__global__ __launch_bounds__(BLOCK) void k_adj_flags(const float *__restrict__ u, GridDims g, int nseg, int nyt,
+                                                     uint8_t *__restrict__ flags)
+{
+    const unsigned int tile = blockIdx.x;
+    const int seg = (int)(tile % (unsigned)nseg);
+    const int yt = (int)((tile / (unsigned)nseg) % (unsigned)nyt);
+    const int z = (int)(tile / ((unsigned)nseg * (unsigned)nyt));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int y = yt * ROWS + w;
+    const int x0 = seg * ROW + lane * VEC;
+    unsigned int b = 0u;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        b |= bits4(row4(u, g, a, x0, y, z));
+    const int nz_tile = __syncthreads_or((b & 0x7fffffffu) != 0u);
+    if (threadIdx.x == 0)
+        flags[tile] = (uint8_t)(nz_tile != 0);
 }
 
 template <class S>
@@ -440,11 +490,12 @@ PhiRead<S> make_read(const kf_volume *v)
 
 template <class S, int MODE>
 void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float *u_out, const AdjConsts &c,
-            kf_context *ctx, float *e_voxel, bool rows, unsigned int nb, int nseg, int nyt)
+            kf_context *ctx, float *e_voxel, bool rows, unsigned int nb, int nseg, int nyt, const uint8_t *flags_in,
+            uint8_t *flags_out)
 {
     if (rows)
         hipLaunchKernelGGL((k_adj_rows<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
-                           ctx->state, ctx->partials, e_voxel, nseg, nyt);
+                           ctx->state, ctx->partials, e_voxel, nseg, nyt, flags_in, flags_out);
     else
         hipLaunchKernelGGL((k_adj_any<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
                            ctx->state, ctx->partials, e_voxel);
@@ -452,8 +503,26 @@ void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float 
 
 }  // namespace
 
+size_t kf_adj_tile_count(const int32_t dims[3])
+{
+    if (dims[0] % VEC != 0)
+        return 0;
+    return (size_t)((dims[0] + ROW - 1) / ROW) * (size_t)((dims[1] + ROWS - 1) / ROWS) * (size_t)dims[2];
+}
+
+kf_status kf_launch_adj_flags(const kf_field *f, const float *u, uint8_t *flags)
+{
+    const int nseg = (f->dims[0] + ROW - 1) / ROW, nyt = (f->dims[1] + ROWS - 1) / ROWS;
+    const unsigned int nb = (unsigned int)kf_adj_tile_count(f->dims);
+    KF_CHECK_ARG(nb > 0, "tile flags need rows of a multiple of 4 voxels");
+    hipLaunchKernelGGL(k_adj_flags, dim3(nb), dim3(BLOCK), 0, f->ctx->stream, u, grid_of(f), nseg, nyt, flags);
+    KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
 kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const float *u_in, float *u_out,
-                             const kf_params *p, int mode, float *e_voxel)
+                             const kf_params *p, int mode, float *e_voxel, const uint8_t *flags_in,
+                             uint8_t *flags_out)
 {
     kf_context *ctx = live->ctx;
     AdjConsts c{};
@@ -482,11 +551,12 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     KF_DISPATCH_DTYPE(live->dtype, S, {
         PhiRead<S> L = make_read<S>(live), Cn = make_read<S>(can);
         if (mode == KF_MODE_GRAD)
-            launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt);
+            launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, nullptr, nullptr);
         else if (mode == KF_MODE_ENERGY)
-            launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt);
+            launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, nullptr, nullptr);
         else
-            launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt);
+            launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, rows ? flags_in : nullptr,
+                                    rows ? flags_out : nullptr);
     });
     KF_LAUNCH_CHECK();
     return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb);

@@ -475,9 +475,12 @@ kf_status kf_field_destroy(kf_field *f)
         return KF_OK;
     DeviceGuard dg(f->ctx->device);
     (void)hipStreamSynchronize(f->ctx->stream);
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < 2; b++) {
         if (f->buf[b])
             (void)hipFree(f->buf[b]);
+        if (f->flags[b])
+            (void)hipFree(f->flags[b]);
+    }
     delete f;
     return KF_OK;
 }
@@ -623,12 +626,27 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
         return s;
     const int cur0 = u->cur;
     const bool check = p->stop_max_update > 0.0;
+    // Tile flags for the adjoint row kernel, recomputed from the field's current values
+    // (anything may have written them since the last deform).
+    const bool flagged = p->method == KF_METHOD_ADJOINT && kf_adj_tile_count(u->dims) > 0 && p->max_iters > 0;
+    if (flagged) {
+        const size_t nt = kf_adj_tile_count(u->dims);
+        for (int b = 0; b < 2; b++)
+            if (!u->flags[b])
+                KF_HIP(hipMalloc((void **)&u->flags[b], nt));
+        s = kf_launch_adj_flags(u, u->buf[cur0], u->flags[cur0]);
+        if (s != KF_OK)
+            return s;
+    }
     constexpr int CHUNK = 16;
     for (int k = 0; k < p->max_iters; k++) {
         float *in = u->buf[cur0 ^ (k & 1)];
         float *out = u->buf[cur0 ^ ((k + 1) & 1)];
-        s = p->method == KF_METHOD_ADJOINT ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr)
-                                           : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP);
+        s = p->method == KF_METHOD_ADJOINT
+                ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr,
+                                     flagged ? u->flags[cur0 ^ (k & 1)] : nullptr,
+                                     flagged ? u->flags[cur0 ^ ((k + 1) & 1)] : nullptr)
+                : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP);
         if (s != KF_OK)
             return s;
         if (check && (k + 1) % CHUNK == 0 && k + 1 < p->max_iters) {
