@@ -278,12 +278,55 @@ __global__ __launch_bounds__(BLOCK) void k_adj_any(PhiRead<S> L, PhiRead<S> Cn, 
 // Rows of a multiple of 4 voxels: 4 voxels per thread, float4 row loads.
 // ---------------------------------------------------------------------------
 
+// The row kernel indexes with 32-bit element offsets from the kernel-argument base
+// pointers (3 N < 2^30 is checked at launch), which keeps one VGPR per address.
+__device__ inline unsigned int nvox32(const GridDims &g) { return (unsigned int)g.n; }
+
 // Row segment of component c at (x0 .. x0+3, y, z); zero when the row is out of the grid.
 __device__ inline float4 row4(const float *__restrict__ u, const GridDims &g, int c, int x0, int y, int z)
 {
     if (x0 < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz)
-        return *reinterpret_cast<const float4 *>(u + (int64_t)c * g.n + g.idx(x0, y, z));
+        return *reinterpret_cast<const float4 *>(u + ((unsigned int)c * nvox32(g) + g.idx32(x0, y, z)));
     return make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__device__ inline float uval32(const float *__restrict__ u, const GridDims &g, int c, int x, int y, int z)
+{
+    return g.in(x, y, z) ? u[(unsigned int)c * nvox32(g) + g.idx32(x, y, z)] : 0.0f;
+}
+
+template <class S>
+__device__ inline float at32(const PhiRead<S> &P, int x, int y, int z)
+{
+    return P.g.in(x, y, z) ? S::load(P.p, (int64_t)P.g.idx32(x, y, z), P.scale) : P.oob;
+}
+
+// PhiRead::cell with 32-bit offsets.
+template <class S>
+__device__ inline void cell32(const PhiRead<S> &P, int bx, int by, int bz, float v[8])
+{
+    const GridDims &g = P.g;
+    if (bx >= 0 && by >= 0 && bz >= 0 && bx + 1 < g.nx && by + 1 < g.ny && bz + 1 < g.nz) {
+        const typename S::T *p0 = P.p + g.idx32(bx, by, bz);
+        const unsigned int sy = (unsigned int)g.nx, sz = (unsigned int)g.nx * (unsigned int)g.ny;
+        v[0] = S::load(p0, 0, P.scale);
+        v[1] = S::load(p0, 1, P.scale);
+        v[2] = S::load(p0, sy, P.scale);
+        v[3] = S::load(p0, sy + 1, P.scale);
+        v[4] = S::load(p0, sz, P.scale);
+        v[5] = S::load(p0, sz + 1, P.scale);
+        v[6] = S::load(p0, sz + sy, P.scale);
+        v[7] = S::load(p0, sz + sy + 1, P.scale);
+    } else {
+        v[0] = at32(P, bx, by, bz);
+        v[1] = at32(P, bx + 1, by, bz);
+        v[2] = at32(P, bx, by + 1, bz);
+        v[3] = at32(P, bx + 1, by + 1, bz);
+        v[4] = at32(P, bx, by, bz + 1);
+        v[5] = at32(P, bx + 1, by, bz + 1);
+        v[6] = at32(P, bx, by + 1, bz + 1);
+        v[7] = at32(P, bx + 1, by + 1, bz + 1);
+    }
 }
 
 __device__ inline unsigned int bits4(const float4 &v)
@@ -299,20 +342,20 @@ __device__ inline void gather_nbhd(const float *__restrict__ u, const GridDims &
 {
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        nb.uc[a] = uval(u, g, a, x, y, z);
-        nb.up[0][a] = uval(u, g, a, x + 1, y, z);
-        nb.um[0][a] = uval(u, g, a, x - 1, y, z);
-        nb.up[1][a] = uval(u, g, a, x, y + 1, z);
-        nb.um[1][a] = uval(u, g, a, x, y - 1, z);
-        nb.up[2][a] = uval(u, g, a, x, y, z + 1);
-        nb.um[2][a] = uval(u, g, a, x, y, z - 1);
+        nb.uc[a] = uval32(u, g, a, x, y, z);
+        nb.up[0][a] = uval32(u, g, a, x + 1, y, z);
+        nb.um[0][a] = uval32(u, g, a, x - 1, y, z);
+        nb.up[1][a] = uval32(u, g, a, x, y + 1, z);
+        nb.um[1][a] = uval32(u, g, a, x, y - 1, z);
+        nb.up[2][a] = uval32(u, g, a, x, y, z + 1);
+        nb.um[2][a] = uval32(u, g, a, x, y, z - 1);
     }
-    nb.dgl[0][1] = uval(u, g, 1, x + 1, y - 1, z);
-    nb.dgl[0][2] = uval(u, g, 2, x + 1, y, z - 1);
-    nb.dgl[1][0] = uval(u, g, 0, x - 1, y + 1, z);
-    nb.dgl[1][2] = uval(u, g, 2, x, y + 1, z - 1);
-    nb.dgl[2][0] = uval(u, g, 0, x - 1, y, z + 1);
-    nb.dgl[2][1] = uval(u, g, 1, x, y - 1, z + 1);
+    nb.dgl[0][1] = uval32(u, g, 1, x + 1, y - 1, z);
+    nb.dgl[0][2] = uval32(u, g, 2, x + 1, y, z - 1);
+    nb.dgl[1][0] = uval32(u, g, 0, x - 1, y + 1, z);
+    nb.dgl[1][2] = uval32(u, g, 2, x, y + 1, z - 1);
+    nb.dgl[2][0] = uval32(u, g, 0, x - 1, y, z + 1);
+    nb.dgl[2][1] = uval32(u, g, 1, x, y - 1, z + 1);
 }
 
 // Index of the row tile (segment, 4-row group, slice) that tile coordinates name, or -1.
@@ -353,13 +396,9 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
 #pragma unroll
     for (int a = 0; a < 3; a++)
         C[a] = row4(u_in, g, a, x0, y, z);
-    float live4[VEC] = {0.f, 0.f, 0.f, 0.f}, canv[VEC] = {0.f, 0.f, 0.f, 0.f};
-    if (valid) {
-        const int64_t i0 = g.idx(x0, y, z);
-        S::load4(L.p, i0, L.scale, live4);
-        if (c.use_data)
-            S::load4(Cn.p, i0, Cn.scale, canv);
-    }
+    float live4[VEC] = {0.f, 0.f, 0.f, 0.f};
+    if (valid)
+        S::load4(L.p, (int64_t)g.idx32(x0, y, z), L.scale, live4);
     bool tile_nz = true;
     if (flags_in) {
         int pred = ((bits4(C[0]) | bits4(C[1]) | bits4(C[2])) & 0x7fffffffu) != 0u;
@@ -373,10 +412,11 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
     }
     bool quiet = true;
     if (tile_nz) {
-        unsigned int bits = 0u;
-#pragma unroll
+        // One component at a time keeps at most five rows in flight (register budget).
+        unsigned int bits = bits4(C[0]) | bits4(C[1]) | bits4(C[2]);
+#pragma unroll 1
         for (int a = 0; a < 3; a++)
-            bits |= bits4(C[a]) | bits4(row4(u_in, g, a, x0, y + 1, z)) | bits4(row4(u_in, g, a, x0, y - 1, z)) |
+            bits |= bits4(row4(u_in, g, a, x0, y + 1, z)) | bits4(row4(u_in, g, a, x0, y - 1, z)) |
                     bits4(row4(u_in, g, a, x0, y, z + 1)) | bits4(row4(u_in, g, a, x0, y, z - 1));
         bits |= bits4(row4(u_in, g, 2, x0, y + 1, z - 1)) | bits4(row4(u_in, g, 1, x0, y - 1, z + 1));
         unsigned int nbits = __shfl_up(bits, 1, 64) | __shfl_down(bits, 1, 64);
@@ -391,25 +431,33 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
     unsigned long long act = 0;
     int out_nz = 0;
     if (valid) {
-        const int64_t i0 = g.idx(x0, y, z);
-        bool need[VEC];
+        const unsigned int i0 = g.idx32(x0, y, z), n32 = nvox32(g);
+        unsigned int need = 0u;  // bit k: voxel x0 + k can contribute
 #pragma unroll
         for (int k = 0; k < VEC; k++)
-            need[k] = !quiet || !(live4[k] > c.hi || live4[k] < c.lo);
-        float outv[3][VEC], ev[VEC];
+            if (!quiet || !(live4[k] > c.hi || live4[k] < c.lo))
+                need |= 1u << k;
+        // Defaults of voxels that cannot contribute, four at a time; the voxels of phase 2
+        // overwrite their own elements below (same thread, so program order holds).
+        if (MODE == KF_MODE_ENERGY) {
+            if (e_voxel)
+                *reinterpret_cast<float4 *>(e_voxel + i0) = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
 #pragma unroll
-        for (int k = 0; k < VEC; k++) {
-            ev[k] = 0.f;
-            outv[0][k] = MODE == KF_MODE_GRAD ? 0.f : f4(C[0], k);
-            outv[1][k] = MODE == KF_MODE_GRAD ? 0.f : f4(C[1], k);
-            outv[2][k] = MODE == KF_MODE_GRAD ? 0.f : f4(C[2], k);
+            for (int a = 0; a < 3; a++) {
+                const float4 d = MODE == KF_MODE_GRAD ? make_float4(0.f, 0.f, 0.f, 0.f) : C[a];
+                *reinterpret_cast<float4 *>(u_out + ((unsigned int)a * n32 + i0)) = d;
+            }
+            if (MODE == KF_MODE_STEP)
+                out_nz = ((bits4(C[0]) | bits4(C[1]) | bits4(C[2])) & 0x7fffffffu) != 0u;
         }
         // Phase 2 -- the voxels that can contribute: full energy and gradient.
 #pragma unroll 1
         for (int k = 0; k < VEC; k++) {
-            if (!need[k])
+            if (!(need & (1u << k)))
                 continue;
             const int x = x0 + k;
+            const unsigned int i = i0 + (unsigned int)k;
             Nbhd nb;
             gather_nbhd(u_in, g, x, y, z, nb);
             int bx, by, bz;
@@ -417,34 +465,27 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
             kf_split(x, nb.uc[0], bx, fx);
             kf_split(y, nb.uc[1], by, fy);
             kf_split(z, nb.uc[2], bz, fz);
-            L.cell(bx, by, bz, v);
+            cell32(L, bx, by, bz, v);
             VoxelOut o;
-            adj_voxel<MODE != KF_MODE_GRAD>(c, v, fx, fy, fz, canv[k], nb, x, y, z, o);
+            adj_voxel<MODE != KF_MODE_GRAD>(c, v, fx, fy, fz, c.use_data ? S::load(Cn.p, (int64_t)i, Cn.scale) : 0.f,
+                                            nb, x, y, z, o);
             if (o.active)
                 act += 1;
-            ev[k] = o.e;
             esum += (double)o.e;
             if (MODE == KF_MODE_GRAD) {
-                outv[0][k] = o.g[0];
-                outv[1][k] = o.g[1];
-                outv[2][k] = o.g[2];
-            } else if (MODE == KF_MODE_STEP) {
+                u_out[i] = o.g[0];
+                u_out[n32 + i] = o.g[1];
+                u_out[2u * n32 + i] = o.g[2];
+            } else if (MODE == KF_MODE_ENERGY) {
+                if (e_voxel)
+                    e_voxel[i] = o.e;
+            } else {
                 float n[3];
-                adj_update(c, o, nb.uc, i0 + k, st, mx, n);
-                outv[0][k] = n[0];
-                outv[1][k] = n[1];
-                outv[2][k] = n[2];
-            }
-        }
-        if (MODE == KF_MODE_ENERGY) {
-            if (e_voxel)
-                *reinterpret_cast<float4 *>(e_voxel + i0) = make_float4(ev[0], ev[1], ev[2], ev[3]);
-        } else {
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                const float4 o4 = make_float4(outv[a][0], outv[a][1], outv[a][2], outv[a][3]);
-                out_nz |= (bits4(o4) & 0x7fffffffu) != 0u;
-                *reinterpret_cast<float4 *>(u_out + (int64_t)a * g.n + i0) = o4;
+                adj_update(c, o, nb.uc, (int64_t)i, st, mx, n);
+                u_out[i] = n[0];
+                u_out[n32 + i] = n[1];
+                u_out[2u * n32 + i] = n[2];
+                out_nz |= ((__float_as_uint(n[0]) | __float_as_uint(n[1]) | __float_as_uint(n[2])) & 0x7fffffffu) != 0u;
             }
         }
     }
@@ -505,7 +546,7 @@ void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float 
 
 size_t kf_adj_tile_count(const int32_t dims[3])
 {
-    if (dims[0] % VEC != 0)
+    if (dims[0] % VEC != 0 || 3 * (int64_t)dims[0] * dims[1] * dims[2] >= ((int64_t)1 << 30))
         return 0;
     return (size_t)((dims[0] + ROW - 1) / ROW) * (size_t)((dims[1] + ROWS - 1) / ROWS) * (size_t)dims[2];
 }
@@ -539,8 +580,9 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     c.use_ls = p->use_levelset;
     c.use_k = p->use_killing;
     KF_CHECK_ARG(live->dtype == can->dtype, "live and canonical volumes must share a dtype");
-    // float4 row loads need rows of a multiple of 4 voxels (plane starts are then 16-byte aligned).
-    const bool rows = live->dims[0] % VEC == 0;
+    // float4 row loads need rows of a multiple of 4 voxels (plane starts are then 16-byte aligned);
+    // 32-bit offsets need 3 N < 2^30.
+    const bool rows = kf_adj_tile_count(live->dims) > 0;
     const int nseg = (live->dims[0] + ROW - 1) / ROW;
     const int nyt = (live->dims[1] + ROWS - 1) / ROWS;
     const unsigned int nb = rows ? (unsigned int)(nseg * nyt * live->dims[2])

@@ -196,6 +196,11 @@ struct GridDims {
     {
         return ((int64_t)z * ny + y) * nx + x;
     }
+    // 32-bit index for kernels restricted to fields of fewer than 2^30 floats per buffer.
+    __device__ __host__ inline unsigned int idx32(int x, int y, int z) const
+    {
+        return ((unsigned int)z * (unsigned int)ny + (unsigned int)y) * (unsigned int)nx + (unsigned int)x;
+    }
 };
 
 inline GridDims grid_of(const kf_volume *v) { return GridDims{v->dims[0], v->dims[1], v->dims[2], v->n}; }
