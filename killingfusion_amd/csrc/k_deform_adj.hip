@@ -34,7 +34,8 @@ namespace {
 constexpr int BLOCK = 256;
 constexpr int VEC = 4;                  // voxels per thread along x in k_adj_rows
 constexpr int ROW = 64 * VEC;           // voxels of a row segment per wavefront
-constexpr int ROWS = BLOCK / 64;        // rows per block
+constexpr int RPT = 2;                  // rows per thread in k_adj_rows (y and y + 4)
+constexpr int ROWS = BLOCK / 64 * RPT;  // rows per block (a row tile)
 
 struct AdjConsts {
     float inv_2v, inv_v, hi, lo, eps2, alpha, omega_k, gamma, omega_s;
@@ -381,27 +382,44 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
     const int yt = (int)((tile / (unsigned)nseg) % (unsigned)nyt);
     const int z = (int)(tile / ((unsigned)nseg * (unsigned)nyt));
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int y = yt * ROWS + w;
     const int x0 = seg * ROW + lane * VEC;
-    const bool valid = x0 < g.nx && y < g.ny;
+    const unsigned int n32 = nvox32(g);
+    int yr[RPT];
+    bool valid[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+        yr[r] = yt * ROWS + w + r * (BLOCK / 64);
+        valid[r] = x0 < g.nx && yr[r] < g.ny;
+    }
 
     // Phase 1 -- quiet test.  A voxel is quiet when every u value its stencil touches is
     // zero (everywhere away from the surface band); then psi_j = phi_live[j] exactly (zero
     // interpolation weights) and an inactive quiet voxel has zero gradient and energy.
     // With tile flags (flags_in[t] != 0 iff tile t of u_in holds a nonzero value) a block whose
-    // 27 neighbouring tiles are all zero is quiet as a whole and reads only its own row;
-    // otherwise every u row the four voxels touch is streamed in as an independent float4
-    // load and folded into a bit mask, with the adjacent lanes' masks covering x +- 1.
-    float4 C[3];
+    // 27 neighbouring tiles are all zero is quiet as a whole and reads only its own rows;
+    // otherwise every u row a thread's voxels touch is streamed in as a float4 load and folded
+    // into a bit mask, with the adjacent lanes' masks covering x +- 1.  Both rows' loads are
+    // issued before anything waits on them.
+    float4 C[RPT][3];
+    float live4[RPT][VEC];
 #pragma unroll
-    for (int a = 0; a < 3; a++)
-        C[a] = row4(u_in, g, a, x0, y, z);
-    float live4[VEC] = {0.f, 0.f, 0.f, 0.f};
-    if (valid)
-        S::load4(L.p, (int64_t)g.idx32(x0, y, z), L.scale, live4);
+    for (int r = 0; r < RPT; r++) {
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            C[r][a] = row4(u_in, g, a, x0, yr[r], z);
+#pragma unroll
+        for (int k = 0; k < VEC; k++)
+            live4[r][k] = 0.f;
+        if (valid[r])
+            S::load4(L.p, (int64_t)g.idx32(x0, yr[r], z), L.scale, live4[r]);
+    }
+    unsigned int own = 0u;
+#pragma unroll
+    for (int r = 0; r < RPT; r++)
+        own |= bits4(C[r][0]) | bits4(C[r][1]) | bits4(C[r][2]);
     bool tile_nz = true;
     if (flags_in) {
-        int pred = ((bits4(C[0]) | bits4(C[1]) | bits4(C[2])) & 0x7fffffffu) != 0u;
+        int pred = (own & 0x7fffffffu) != 0u;
         if (threadIdx.x < 27) {
             const int t = tile_of(seg + (int)(threadIdx.x % 3) - 1, yt + (int)(threadIdx.x / 3 % 3) - 1,
                                   z + (int)(threadIdx.x / 9) - 1, nseg, nyt, g.nz);
@@ -410,83 +428,100 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
         }
         tile_nz = __syncthreads_or(pred) != 0;
     }
-    bool quiet = true;
+    unsigned int need = 0u;  // bit r * VEC + k: voxel (x0 + k, yr[r]) can contribute
     if (tile_nz) {
-        // One component at a time keeps at most five rows in flight (register budget).
-        unsigned int bits = bits4(C[0]) | bits4(C[1]) | bits4(C[2]);
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            const int y = yr[r];
+            // One component at a time keeps at most four rows in flight (register budget).
+            unsigned int bits = bits4(C[r][0]) | bits4(C[r][1]) | bits4(C[r][2]);
 #pragma unroll 1
-        for (int a = 0; a < 3; a++)
-            bits |= bits4(row4(u_in, g, a, x0, y + 1, z)) | bits4(row4(u_in, g, a, x0, y - 1, z)) |
-                    bits4(row4(u_in, g, a, x0, y, z + 1)) | bits4(row4(u_in, g, a, x0, y, z - 1));
-        bits |= bits4(row4(u_in, g, 2, x0, y + 1, z - 1)) | bits4(row4(u_in, g, 1, x0, y - 1, z + 1));
-        unsigned int nbits = __shfl_up(bits, 1, 64) | __shfl_down(bits, 1, 64);
-        if (lane == 0 && x0 > 0)
-            nbits = 0xffffffffu;  // segment edge: the left neighbour lives in another wavefront
-        if (lane == 63 && x0 + VEC < g.nx)
-            nbits = 0xffffffffu;
-        quiet = ((bits | nbits) & 0x7fffffffu) == 0u;
+            for (int a = 0; a < 3; a++)
+                bits |= bits4(row4(u_in, g, a, x0, y + 1, z)) | bits4(row4(u_in, g, a, x0, y - 1, z)) |
+                        bits4(row4(u_in, g, a, x0, y, z + 1)) | bits4(row4(u_in, g, a, x0, y, z - 1));
+            bits |= bits4(row4(u_in, g, 2, x0, y + 1, z - 1)) | bits4(row4(u_in, g, 1, x0, y - 1, z + 1));
+            unsigned int nbits = __shfl_up(bits, 1, 64) | __shfl_down(bits, 1, 64);
+            if (lane == 0 && x0 > 0)
+                nbits = 0xffffffffu;  // segment edge: the left neighbour lives in another wavefront
+            if (lane == 63 && x0 + VEC < g.nx)
+                nbits = 0xffffffffu;
+            const bool quiet = ((bits | nbits) & 0x7fffffffu) == 0u;
+#pragma unroll
+            for (int k = 0; k < VEC; k++)
+                if (!quiet || !(live4[r][k] > c.hi || live4[r][k] < c.lo))
+                    need |= 1u << (r * VEC + k);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < RPT; r++)
+#pragma unroll
+            for (int k = 0; k < VEC; k++)
+                if (!(live4[r][k] > c.hi || live4[r][k] < c.lo))
+                    need |= 1u << (r * VEC + k);
     }
 
-    double mx = 0.0, esum = 0.0;
-    unsigned long long act = 0;
+    // Defaults of the voxels that cannot contribute, four at a time; phase-2 voxels overwrite
+    // their own elements below (same thread, so program order holds).
     int out_nz = 0;
-    if (valid) {
-        const unsigned int i0 = g.idx32(x0, y, z), n32 = nvox32(g);
-        unsigned int need = 0u;  // bit k: voxel x0 + k can contribute
 #pragma unroll
-        for (int k = 0; k < VEC; k++)
-            if (!quiet || !(live4[k] > c.hi || live4[k] < c.lo))
-                need |= 1u << k;
-        // Defaults of voxels that cannot contribute, four at a time; the voxels of phase 2
-        // overwrite their own elements below (same thread, so program order holds).
+    for (int r = 0; r < RPT; r++) {
+        if (!valid[r])
+            continue;
+        const unsigned int i0 = g.idx32(x0, yr[r], z);
         if (MODE == KF_MODE_ENERGY) {
             if (e_voxel)
                 *reinterpret_cast<float4 *>(e_voxel + i0) = make_float4(0.f, 0.f, 0.f, 0.f);
         } else {
 #pragma unroll
             for (int a = 0; a < 3; a++) {
-                const float4 d = MODE == KF_MODE_GRAD ? make_float4(0.f, 0.f, 0.f, 0.f) : C[a];
+                const float4 d = MODE == KF_MODE_GRAD ? make_float4(0.f, 0.f, 0.f, 0.f) : C[r][a];
                 *reinterpret_cast<float4 *>(u_out + ((unsigned int)a * n32 + i0)) = d;
             }
-            if (MODE == KF_MODE_STEP)
-                out_nz = ((bits4(C[0]) | bits4(C[1]) | bits4(C[2])) & 0x7fffffffu) != 0u;
         }
-        // Phase 2 -- the voxels that can contribute: full energy and gradient.
+    }
+    if (MODE == KF_MODE_STEP)
+        out_nz = (own & 0x7fffffffu) != 0u;
+
+    // Phase 2 -- the voxels that can contribute: full energy and gradient.
+    double mx = 0.0, esum = 0.0;
+    unsigned long long act = 0;
 #pragma unroll 1
-        for (int k = 0; k < VEC; k++) {
-            if (!(need & (1u << k)))
-                continue;
-            const int x = x0 + k;
-            const unsigned int i = i0 + (unsigned int)k;
-            Nbhd nb;
-            gather_nbhd(u_in, g, x, y, z, nb);
-            int bx, by, bz;
-            float fx, fy, fz, v[8];
-            kf_split(x, nb.uc[0], bx, fx);
-            kf_split(y, nb.uc[1], by, fy);
-            kf_split(z, nb.uc[2], bz, fz);
-            cell32(L, bx, by, bz, v);
-            VoxelOut o;
-            adj_voxel<MODE != KF_MODE_GRAD>(c, v, fx, fy, fz, c.use_data ? S::load(Cn.p, (int64_t)i, Cn.scale) : 0.f,
-                                            nb, x, y, z, o);
-            if (o.active)
-                act += 1;
-            esum += (double)o.e;
-            if (MODE == KF_MODE_GRAD) {
-                u_out[i] = o.g[0];
-                u_out[n32 + i] = o.g[1];
-                u_out[2u * n32 + i] = o.g[2];
-            } else if (MODE == KF_MODE_ENERGY) {
-                if (e_voxel)
-                    e_voxel[i] = o.e;
-            } else {
-                float n[3];
-                adj_update(c, o, nb.uc, (int64_t)i, st, mx, n);
-                u_out[i] = n[0];
-                u_out[n32 + i] = n[1];
-                u_out[2u * n32 + i] = n[2];
-                out_nz |= ((__float_as_uint(n[0]) | __float_as_uint(n[1]) | __float_as_uint(n[2])) & 0x7fffffffu) != 0u;
-            }
+    for (int b = 0; b < RPT * VEC; b++) {
+        if (!(need & (1u << b)))
+            continue;
+        const int r = b / VEC, k = b % VEC;
+        const int x = x0 + k, y = r == 0 ? yr[0] : yr[RPT - 1];
+        if (!(x < g.nx && y < g.ny))
+            continue;
+        const unsigned int i = g.idx32(x, y, z);
+        Nbhd nb;
+        gather_nbhd(u_in, g, x, y, z, nb);
+        int bx, by, bz;
+        float fx, fy, fz, v[8];
+        kf_split(x, nb.uc[0], bx, fx);
+        kf_split(y, nb.uc[1], by, fy);
+        kf_split(z, nb.uc[2], bz, fz);
+        cell32(L, bx, by, bz, v);
+        VoxelOut o;
+        adj_voxel<MODE != KF_MODE_GRAD>(c, v, fx, fy, fz, c.use_data ? S::load(Cn.p, (int64_t)i, Cn.scale) : 0.f,
+                                        nb, x, y, z, o);
+        if (o.active)
+            act += 1;
+        esum += (double)o.e;
+        if (MODE == KF_MODE_GRAD) {
+            u_out[i] = o.g[0];
+            u_out[n32 + i] = o.g[1];
+            u_out[2u * n32 + i] = o.g[2];
+        } else if (MODE == KF_MODE_ENERGY) {
+            if (e_voxel)
+                e_voxel[i] = o.e;
+        } else {
+            float n[3];
+            adj_update(c, o, nb.uc, (int64_t)i, st, mx, n);
+            u_out[i] = n[0];
+            u_out[n32 + i] = n[1];
+            u_out[2u * n32 + i] = n[2];
+            out_nz |= ((__float_as_uint(n[0]) | __float_as_uint(n[1]) | __float_as_uint(n[2])) & 0x7fffffffu) != 0u;
         }
     }
     if (MODE == KF_MODE_STEP && flags_out) {
@@ -507,12 +542,13 @@ __global__ __launch_bounds__(BLOCK) void k_adj_flags(const float *__restrict__ u
     const int yt = (int)((tile / (unsigned)nseg) % (unsigned)nyt);
     const int z = (int)(tile / ((unsigned)nseg * (unsigned)nyt));
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int y = yt * ROWS + w;
     const int x0 = seg * ROW + lane * VEC;
     unsigned int b = 0u;
 #pragma unroll
-    for (int a = 0; a < 3; a++)
-        b |= bits4(row4(u, g, a, x0, y, z));
+    for (int r = 0; r < RPT; r++)
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            b |= bits4(row4(u, g, a, x0, yt * ROWS + w + r * (BLOCK / 64), z));
     const int nz_tile = __syncthreads_or((b & 0x7fffffffu) != 0u);
     if (threadIdx.x == 0)
         flags[tile] = (uint8_t)(nz_tile != 0);

@@ -133,11 +133,23 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(DeformState *st, const B
         return;
     double s = 0.0, m = 0.0;
     unsigned long long c = 0;
-    for (unsigned int b = threadIdx.x; b < nb; b += FIN_BLOCK) {
-        const BlockPartial p = parts[b];
-        s += p.sum;
-        m = fmax(m, p.max);
-        c += p.cnt;
+    // Blocks of 8 strided slots per thread: the loads of a block are issued together.
+    for (unsigned int b0 = threadIdx.x; b0 < nb; b0 += 8 * FIN_BLOCK) {
+        BlockPartial p[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const unsigned int b = b0 + (unsigned int)j * FIN_BLOCK;
+            if (b < nb)
+                p[j] = parts[b];
+            else
+                p[j] = BlockPartial{0.0, 0.0, 0ull, 0ull};
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            s += p[j].sum;
+            m = fmax(m, p[j].max);
+            c += p[j].cnt;
+        }
     }
     kf_block_reduce3<FIN_BLOCK>(s, m, c);
     if (threadIdx.x == 0) {
