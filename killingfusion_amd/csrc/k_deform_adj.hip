@@ -19,11 +19,12 @@
 // Compulsory HBM traffic per voxel-iteration: 2 s_phi + 24 bytes.
 //
 // Two launch shapes share the per-voxel arithmetic (adj_voxel):
-//  * k_adj_rows (rows whose length is a multiple of 4): a thread owns 4 consecutive x voxels,
-//    a wavefront 256 voxels of a row, a block 4 rows of one slice.  All u rows the stencil
-//    needs are independent float4 loads (x neighbours come from lane shuffles), issued
-//    together, then the 32 gather taps -- two rounds of wide, independent memory requests
-//    per thread and no barriers, which keeps enough bytes in flight to cover HBM latency.
+//  * k_adj_rows (rows whose length is a multiple of 4): a thread owns 4 consecutive x voxels
+//    of two rows, a wavefront 256 voxels of a row, a block 8 rows of one slice.  A bit mask of
+//    the groups of u that may be nonzero (written by the previous iteration) tells each thread
+//    whether its stencil sees only zeros; such quiet threads stream their own u, phi_live and
+//    the output rows as float4 / 8-byte loads and stores, and only the voxels that can
+//    contribute (active, or near nonzero u) run the full stencil.
 //  * k_adj_any (any shape): one voxel per thread, neighbours read individually.
 // Blocks are numbered so each XCD sweeps a contiguous z range and the rows and slices shared
 // by neighbouring blocks stay in that XCD's L2.
@@ -359,20 +360,47 @@ __device__ inline void gather_nbhd(const float *__restrict__ u, const GridDims &
     nb.dgl[2][1] = uval32(u, g, 1, x, y - 1, z + 1);
 }
 
-// Index of the row tile (segment, 4-row group, slice) that tile coordinates name, or -1.
-__device__ inline int tile_of(int seg, int yt, int z, int nseg, int nyt, int nz)
+// Nonzero masks.  mask bit (row, group) = 1 iff some u component of the 4-voxel group
+// (x = 4 group .. 4 group + 3) of that row may be nonzero.  A row (y, z) holds MW words per
+// 256-voxel segment (one word per half wavefront), so each wavefront owns its two words and
+// writes them with a ballot -- no atomics.
+constexpr int MW = ROW / VEC / 32;
+
+// Bits of groups gx - 1 .. gx + 1 of row (y, z) (gx = 64 seg + lane); 0 outside the grid.
+__device__ inline unsigned int nbr_bits(const unsigned int *__restrict__ m, int wpr, const GridDims &g, int y, int z,
+                                        int seg, int lane)
 {
-    if (seg < 0 || seg >= nseg || yt < 0 || yt >= nyt || z < 0 || z >= nz)
-        return -1;
-    return (z * nyt + yt) * nseg + seg;
+    if (y < 0 || y >= g.ny || z < 0 || z >= g.nz)
+        return 0u;
+    const unsigned int *row = m + ((unsigned int)z * (unsigned int)g.ny + (unsigned int)y) * (unsigned int)wpr;
+    const int gx = seg * (ROW / VEC) + lane;
+    const int wi = gx >> 5, b = gx & 31;
+    unsigned int r = row[wi] & (b == 0 ? 0x3u : (0x7u << (b - 1)));
+    if (b == 0 && wi > 0)
+        r |= row[wi - 1] >> 31;
+    if (b == 31 && wi + 1 < wpr)
+        r |= row[wi + 1] & 1u;
+    return r;
+}
+
+// Writes the wavefront's mask words of row (y, z): bit lane = nz of that lane's group.
+__device__ inline void put_bits(unsigned int *__restrict__ m, int wpr, const GridDims &g, int y, int z, int seg,
+                                int lane, bool nz)
+{
+    const unsigned long long bal = __ballot(nz);
+    if (lane == 0 && y < g.ny) {
+        unsigned int *row = m + ((unsigned int)z * (unsigned int)g.ny + (unsigned int)y) * (unsigned int)wpr;
+        row[seg * MW] = (unsigned int)bal;
+        row[seg * MW + 1] = (unsigned int)(bal >> 32);
+    }
 }
 
 template <class S, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, AdjConsts c, DeformState *st,
                                                     BlockPartial *partials, float *__restrict__ e_voxel, int nseg,
-                                                    int nyt, const uint8_t *__restrict__ flags_in,
-                                                    uint8_t *__restrict__ flags_out)
+                                                    int nyt, const unsigned int *__restrict__ mask_in,
+                                                    unsigned int *__restrict__ mask_out)
 {
     if (MODE == KF_MODE_STEP && st->done)
         return;
@@ -383,6 +411,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
     const int z = (int)(tile / ((unsigned)nseg * (unsigned)nyt));
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int x0 = seg * ROW + lane * VEC;
+    const int wpr = nseg * MW;
     const unsigned int n32 = nvox32(g);
     int yr[RPT];
     bool valid[RPT];
@@ -394,77 +423,46 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
 
     // Phase 1 -- quiet test.  A voxel is quiet when every u value its stencil touches is
     // zero (everywhere away from the surface band); then psi_j = phi_live[j] exactly (zero
-    // interpolation weights) and an inactive quiet voxel has zero gradient and energy.
-    // With tile flags (flags_in[t] != 0 iff tile t of u_in holds a nonzero value) a block whose
-    // 27 neighbouring tiles are all zero is quiet as a whole and reads only its own rows;
-    // otherwise every u row a thread's voxels touch is streamed in as a float4 load and folded
-    // into a bit mask, with the adjacent lanes' masks covering x +- 1.  Both rows' loads are
-    // issued before anything waits on them.
+    // interpolation weights) and an inactive quiet voxel has zero gradient and energy.  The
+    // stencil of a thread's 4 voxels reaches groups gx - 1 .. gx + 1 of seven rows; their mask
+    // bits decide quietness without touching u.  Both rows' loads are issued up front.
     float4 C[RPT][3];
     float live4[RPT][VEC];
+    unsigned int nb_bits[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
+        const int y = yr[r];
 #pragma unroll
         for (int a = 0; a < 3; a++)
-            C[r][a] = row4(u_in, g, a, x0, yr[r], z);
+            C[r][a] = row4(u_in, g, a, x0, y, z);
 #pragma unroll
         for (int k = 0; k < VEC; k++)
             live4[r][k] = 0.f;
         if (valid[r])
-            S::load4(L.p, (int64_t)g.idx32(x0, yr[r], z), L.scale, live4[r]);
-    }
-    unsigned int own = 0u;
-#pragma unroll
-    for (int r = 0; r < RPT; r++)
-        own |= bits4(C[r][0]) | bits4(C[r][1]) | bits4(C[r][2]);
-    bool tile_nz = true;
-    if (flags_in) {
-        int pred = (own & 0x7fffffffu) != 0u;
-        if (threadIdx.x < 27) {
-            const int t = tile_of(seg + (int)(threadIdx.x % 3) - 1, yt + (int)(threadIdx.x / 3 % 3) - 1,
-                                  z + (int)(threadIdx.x / 9) - 1, nseg, nyt, g.nz);
-            if (t >= 0 && flags_in[t])
-                pred = 1;
-        }
-        tile_nz = __syncthreads_or(pred) != 0;
+            S::load4(L.p, (int64_t)g.idx32(x0, y, z), L.scale, live4[r]);
+        nb_bits[r] = nbr_bits(mask_in, wpr, g, y, z, seg, lane) | nbr_bits(mask_in, wpr, g, y + 1, z, seg, lane) |
+                     nbr_bits(mask_in, wpr, g, y - 1, z, seg, lane) | nbr_bits(mask_in, wpr, g, y, z + 1, seg, lane) |
+                     nbr_bits(mask_in, wpr, g, y, z - 1, seg, lane) |
+                     nbr_bits(mask_in, wpr, g, y + 1, z - 1, seg, lane) |
+                     nbr_bits(mask_in, wpr, g, y - 1, z + 1, seg, lane);
     }
     unsigned int need = 0u;  // bit r * VEC + k: voxel (x0 + k, yr[r]) can contribute
-    if (tile_nz) {
 #pragma unroll
-        for (int r = 0; r < RPT; r++) {
-            const int y = yr[r];
-            // One component at a time keeps at most four rows in flight (register budget).
-            unsigned int bits = bits4(C[r][0]) | bits4(C[r][1]) | bits4(C[r][2]);
-#pragma unroll 1
-            for (int a = 0; a < 3; a++)
-                bits |= bits4(row4(u_in, g, a, x0, y + 1, z)) | bits4(row4(u_in, g, a, x0, y - 1, z)) |
-                        bits4(row4(u_in, g, a, x0, y, z + 1)) | bits4(row4(u_in, g, a, x0, y, z - 1));
-            bits |= bits4(row4(u_in, g, 2, x0, y + 1, z - 1)) | bits4(row4(u_in, g, 1, x0, y - 1, z + 1));
-            unsigned int nbits = __shfl_up(bits, 1, 64) | __shfl_down(bits, 1, 64);
-            if (lane == 0 && x0 > 0)
-                nbits = 0xffffffffu;  // segment edge: the left neighbour lives in another wavefront
-            if (lane == 63 && x0 + VEC < g.nx)
-                nbits = 0xffffffffu;
-            const bool quiet = ((bits | nbits) & 0x7fffffffu) == 0u;
+    for (int r = 0; r < RPT; r++) {
+        const bool quiet = nb_bits[r] == 0u && ((bits4(C[r][0]) | bits4(C[r][1]) | bits4(C[r][2])) & 0x7fffffffu) == 0u;
 #pragma unroll
-            for (int k = 0; k < VEC; k++)
-                if (!quiet || !(live4[r][k] > c.hi || live4[r][k] < c.lo))
-                    need |= 1u << (r * VEC + k);
-        }
-    } else {
-#pragma unroll
-        for (int r = 0; r < RPT; r++)
-#pragma unroll
-            for (int k = 0; k < VEC; k++)
-                if (!(live4[r][k] > c.hi || live4[r][k] < c.lo))
-                    need |= 1u << (r * VEC + k);
+        for (int k = 0; k < VEC; k++)
+            if (!quiet || !(live4[r][k] > c.hi || live4[r][k] < c.lo))
+                need |= 1u << (r * VEC + k);
     }
 
     // Defaults of the voxels that cannot contribute, four at a time; phase-2 voxels overwrite
     // their own elements below (same thread, so program order holds).
-    int out_nz = 0;
+    bool row_nz[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
+        row_nz[r] = MODE == KF_MODE_STEP &&
+                    ((bits4(C[r][0]) | bits4(C[r][1]) | bits4(C[r][2])) & 0x7fffffffu) != 0u;
         if (!valid[r])
             continue;
         const unsigned int i0 = g.idx32(x0, yr[r], z);
@@ -479,8 +477,6 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
             }
         }
     }
-    if (MODE == KF_MODE_STEP)
-        out_nz = (own & 0x7fffffffu) != 0u;
 
     // Phase 2 -- the voxels that can contribute: full energy and gradient.
     double mx = 0.0, esum = 0.0;
@@ -521,21 +517,26 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
             u_out[i] = n[0];
             u_out[n32 + i] = n[1];
             u_out[2u * n32 + i] = n[2];
-            out_nz |= ((__float_as_uint(n[0]) | __float_as_uint(n[1]) | __float_as_uint(n[2])) & 0x7fffffffu) != 0u;
+            if (((__float_as_uint(n[0]) | __float_as_uint(n[1]) | __float_as_uint(n[2])) & 0x7fffffffu) != 0u) {
+                if (r == 0)
+                    row_nz[0] = true;
+                else
+                    row_nz[RPT - 1] = true;
+            }
         }
     }
-    if (MODE == KF_MODE_STEP && flags_out) {
-        const int nz_tile = __syncthreads_or(out_nz);
-        if (threadIdx.x == 0)
-            flags_out[tile] = (uint8_t)(nz_tile != 0);
+    if (MODE == KF_MODE_STEP && mask_out) {
+#pragma unroll
+        for (int r = 0; r < RPT; r++)
+            put_bits(mask_out, wpr, g, yr[r], z, seg, lane, row_nz[r]);
     }
     if (MODE != KF_MODE_GRAD)
         kf_block_publish<BLOCK>(partials, blockIdx.x, esum, mx, act);
 }
 
-// Tile flags of a field: flags[t] = 1 iff row tile t of u holds a nonzero value.
-__global__ __launch_bounds__(BLOCK) void k_adj_flags(const float *__restrict__ u, GridDims g, int nseg, int nyt,
-                                                     uint8_t *__restrict__ flags)
+// Nonzero mask of field values u (see nbr_bits).
+__global__ __launch_bounds__(BLOCK) void k_adj_mask(const float *__restrict__ u, GridDims g, int nseg, int nyt,
+                                                    unsigned int *__restrict__ mask)
 {
     const unsigned int tile = blockIdx.x;
     const int seg = (int)(tile % (unsigned)nseg);
@@ -543,15 +544,15 @@ __global__ __launch_bounds__(BLOCK) void k_adj_flags(const float *__restrict__ u
     const int z = (int)(tile / ((unsigned)nseg * (unsigned)nyt));
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int x0 = seg * ROW + lane * VEC;
-    unsigned int b = 0u;
 #pragma unroll
-    for (int r = 0; r < RPT; r++)
+    for (int r = 0; r < RPT; r++) {
+        const int y = yt * ROWS + w + r * (BLOCK / 64);
+        unsigned int b = 0u;
 #pragma unroll
         for (int a = 0; a < 3; a++)
-            b |= bits4(row4(u, g, a, x0, yt * ROWS + w + r * (BLOCK / 64), z));
-    const int nz_tile = __syncthreads_or((b & 0x7fffffffu) != 0u);
-    if (threadIdx.x == 0)
-        flags[tile] = (uint8_t)(nz_tile != 0);
+            b |= bits4(row4(u, g, a, x0, y, z));
+        put_bits(mask, nseg * MW, g, y, z, seg, lane, (b & 0x7fffffffu) != 0u);
+    }
 }
 
 template <class S>
@@ -567,12 +568,12 @@ PhiRead<S> make_read(const kf_volume *v)
 
 template <class S, int MODE>
 void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float *u_out, const AdjConsts &c,
-            kf_context *ctx, float *e_voxel, bool rows, unsigned int nb, int nseg, int nyt, const uint8_t *flags_in,
-            uint8_t *flags_out)
+            kf_context *ctx, float *e_voxel, bool rows, unsigned int nb, int nseg, int nyt,
+            const unsigned int *mask_in, unsigned int *mask_out)
 {
     if (rows)
         hipLaunchKernelGGL((k_adj_rows<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
-                           ctx->state, ctx->partials, e_voxel, nseg, nyt, flags_in, flags_out);
+                           ctx->state, ctx->partials, e_voxel, nseg, nyt, mask_in, mask_out);
     else
         hipLaunchKernelGGL((k_adj_any<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
                            ctx->state, ctx->partials, e_voxel);
@@ -580,26 +581,26 @@ void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float 
 
 }  // namespace
 
-size_t kf_adj_tile_count(const int32_t dims[3])
+size_t kf_adj_mask_words(const int32_t dims[3])
 {
     if (dims[0] % VEC != 0 || 3 * (int64_t)dims[0] * dims[1] * dims[2] >= ((int64_t)1 << 30))
         return 0;
-    return (size_t)((dims[0] + ROW - 1) / ROW) * (size_t)((dims[1] + ROWS - 1) / ROWS) * (size_t)dims[2];
+    return (size_t)((dims[0] + ROW - 1) / ROW) * MW * (size_t)dims[1] * (size_t)dims[2];
 }
 
-kf_status kf_launch_adj_flags(const kf_field *f, const float *u, uint8_t *flags)
+kf_status kf_launch_adj_mask(const kf_field *f, const float *u, unsigned int *mask)
 {
     const int nseg = (f->dims[0] + ROW - 1) / ROW, nyt = (f->dims[1] + ROWS - 1) / ROWS;
-    const unsigned int nb = (unsigned int)kf_adj_tile_count(f->dims);
-    KF_CHECK_ARG(nb > 0, "tile flags need rows of a multiple of 4 voxels");
-    hipLaunchKernelGGL(k_adj_flags, dim3(nb), dim3(BLOCK), 0, f->ctx->stream, u, grid_of(f), nseg, nyt, flags);
+    KF_CHECK_ARG(kf_adj_mask_words(f->dims) > 0, "nonzero masks need rows of a multiple of 4 voxels");
+    const unsigned int nb = (unsigned int)(nseg * nyt * f->dims[2]);
+    hipLaunchKernelGGL(k_adj_mask, dim3(nb), dim3(BLOCK), 0, f->ctx->stream, u, grid_of(f), nseg, nyt, mask);
     KF_LAUNCH_CHECK();
     return KF_OK;
 }
 
 kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const float *u_in, float *u_out,
-                             const kf_params *p, int mode, float *e_voxel, const uint8_t *flags_in,
-                             uint8_t *flags_out)
+                             const kf_params *p, int mode, float *e_voxel, const unsigned int *mask_in,
+                             unsigned int *mask_out)
 {
     kf_context *ctx = live->ctx;
     AdjConsts c{};
@@ -616,9 +617,9 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     c.use_ls = p->use_levelset;
     c.use_k = p->use_killing;
     KF_CHECK_ARG(live->dtype == can->dtype, "live and canonical volumes must share a dtype");
-    // float4 row loads need rows of a multiple of 4 voxels (plane starts are then 16-byte aligned);
-    // 32-bit offsets need 3 N < 2^30.
-    const bool rows = kf_adj_tile_count(live->dims) > 0;
+    // The row kernel needs a nonzero mask of u_in, rows of a multiple of 4 voxels (float4 loads,
+    // 16-byte aligned plane starts) and 3 N < 2^30 (32-bit offsets).
+    const bool rows = mask_in != nullptr && kf_adj_mask_words(live->dims) > 0;
     const int nseg = (live->dims[0] + ROW - 1) / ROW;
     const int nyt = (live->dims[1] + ROWS - 1) / ROWS;
     const unsigned int nb = rows ? (unsigned int)(nseg * nyt * live->dims[2])
@@ -629,12 +630,11 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     KF_DISPATCH_DTYPE(live->dtype, S, {
         PhiRead<S> L = make_read<S>(live), Cn = make_read<S>(can);
         if (mode == KF_MODE_GRAD)
-            launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, nullptr, nullptr);
+            launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in, nullptr);
         else if (mode == KF_MODE_ENERGY)
-            launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, nullptr, nullptr);
+            launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in, nullptr);
         else
-            launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, rows ? flags_in : nullptr,
-                                    rows ? flags_out : nullptr);
+            launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in, mask_out);
     });
     KF_LAUNCH_CHECK();
     return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb);

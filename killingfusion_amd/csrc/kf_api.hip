@@ -94,6 +94,16 @@ bool invert4(const double m[16], double out[16])
     return true;
 }
 
+// Allocates the field's two nonzero-mask buffers on first use.
+kf_status ensure_masks(kf_field *f)
+{
+    const size_t words = kf_adj_mask_words(f->dims);
+    for (int b = 0; b < 2; b++)
+        if (!f->masks[b])
+            KF_HIP(hipMalloc((void **)&f->masks[b], sizeof(unsigned int) * words));
+    return KF_OK;
+}
+
 kf_status read_state(kf_context *ctx)
 {
     KF_HIP(hipMemcpyAsync(ctx->state_host, ctx->state, sizeof(DeformState), hipMemcpyDeviceToHost, ctx->stream));
@@ -490,8 +500,8 @@ kf_status kf_field_destroy(kf_field *f)
     for (int b = 0; b < 2; b++) {
         if (f->buf[b])
             (void)hipFree(f->buf[b]);
-        if (f->flags[b])
-            (void)hipFree(f->flags[b]);
+        if (f->masks[b])
+            (void)hipFree(f->masks[b]);
     }
     delete f;
     return KF_OK;
@@ -593,8 +603,16 @@ kf_status kf_energy_forward(const kf_volume *live, const kf_volume *can, const k
     if (energy_per_voxel)
         KF_HIP(hipMallocAsync((void **)&ev, sizeof(float) * (size_t)live->n, ctx->stream));
     s = kf_state_reset(ctx, 0.0);
+    const unsigned int *mask = nullptr;
+    if (s == KF_OK && kf_adj_mask_words(u->dims) > 0) {
+        kf_field *uf = const_cast<kf_field *>(u);  // the masks are a cache owned by the field
+        s = ensure_masks(uf);
+        if (s == KF_OK)
+            s = kf_launch_adj_mask(uf, uf->u(), uf->masks[uf->cur]);
+        mask = uf->masks[uf->cur];
+    }
     if (s == KF_OK)
-        s = kf_launch_adj_step(live, can, u->u(), nullptr, p, KF_MODE_ENERGY, ev);
+        s = kf_launch_adj_step(live, can, u->u(), nullptr, p, KF_MODE_ENERGY, ev, mask);
     if (s == KF_OK && ev) {
         hipError_t e = hipMemcpyAsync(energy_per_voxel, ev, sizeof(float) * (size_t)live->n, hipMemcpyDeviceToHost,
                                       ctx->stream);
@@ -621,7 +639,17 @@ kf_status kf_energy_backward(const kf_volume *live, const kf_volume *can, const 
     KF_CHECK_ARG(valid_params(p), "invalid params");
     KF_CHECK_ARG(grad_out && grad_out != u && same_dims(grad_out->dims, u->dims), "grad_out must be a distinct field of the same dims");
     DeviceGuard dg(live->ctx->device);
-    return kf_launch_adj_step(live, can, u->u(), grad_out->u(), p, KF_MODE_GRAD, nullptr);
+    const unsigned int *mask = nullptr;
+    if (kf_adj_mask_words(u->dims) > 0) {
+        kf_field *uf = const_cast<kf_field *>(u);  // the masks are a cache owned by the field
+        s = ensure_masks(uf);
+        if (s == KF_OK)
+            s = kf_launch_adj_mask(uf, uf->u(), uf->masks[uf->cur]);
+        if (s != KF_OK)
+            return s;
+        mask = uf->masks[uf->cur];
+    }
+    return kf_launch_adj_step(live, can, u->u(), grad_out->u(), p, KF_MODE_GRAD, nullptr, mask);
 }
 
 kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, const kf_params *p,
@@ -638,15 +666,13 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
         return s;
     const int cur0 = u->cur;
     const bool check = p->stop_max_update > 0.0;
-    // Tile flags for the adjoint row kernel, recomputed from the field's current values
+    // Nonzero masks for the adjoint row kernel, recomputed from the field's current values
     // (anything may have written them since the last deform).
-    const bool flagged = p->method == KF_METHOD_ADJOINT && kf_adj_tile_count(u->dims) > 0 && p->max_iters > 0;
+    const bool flagged = p->method == KF_METHOD_ADJOINT && p->max_iters > 0 && kf_adj_mask_words(u->dims) > 0;
     if (flagged) {
-        const size_t nt = kf_adj_tile_count(u->dims);
-        for (int b = 0; b < 2; b++)
-            if (!u->flags[b])
-                KF_HIP(hipMalloc((void **)&u->flags[b], nt));
-        s = kf_launch_adj_flags(u, u->buf[cur0], u->flags[cur0]);
+        s = ensure_masks(u);
+        if (s == KF_OK)
+            s = kf_launch_adj_mask(u, u->buf[cur0], u->masks[cur0]);
         if (s != KF_OK)
             return s;
     }
@@ -656,8 +682,8 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
         float *out = u->buf[cur0 ^ ((k + 1) & 1)];
         s = p->method == KF_METHOD_ADJOINT
                 ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr,
-                                     flagged ? u->flags[cur0 ^ (k & 1)] : nullptr,
-                                     flagged ? u->flags[cur0 ^ ((k + 1) & 1)] : nullptr)
+                                     flagged ? u->masks[cur0 ^ (k & 1)] : nullptr,
+                                     flagged ? u->masks[cur0 ^ ((k + 1) & 1)] : nullptr)
                 : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP);
         if (s != KF_OK)
             return s;

@@ -98,7 +98,7 @@ struct kf_field {
     int64_t n = 0;
     float *buf[2] = {nullptr, nullptr};  // SoA [3][n] each; buf[cur] is the field's value
     int cur = 0;
-    uint8_t *flags[2] = {nullptr, nullptr};  // adjoint tile flags of buf[0] / buf[1] (deform only)
+    unsigned int *masks[2] = {nullptr, nullptr};  // nonzero masks of buf[0] / buf[1] (adjoint kernels)
     float *u() const { return buf[cur]; }
     float *scratch() const { return buf[cur ^ 1]; }
 };
@@ -273,14 +273,15 @@ kf_status kf_launch_field_add(kf_field *dst, const kf_field *src);
 struct DeformArgs;
 kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const float *u_in,
                              float *u_out, const kf_params *p, int mode);
-// flags_in / flags_out: optional tile flags of u_in / u_out (STEP mode; see k_deform_adj.hip).
+// mask_in / mask_out: nonzero masks of u_in / u_out (see k_deform_adj.hip); without mask_in the
+// one-voxel-per-thread kernel runs.
 kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const float *u_in,
                              float *u_out, const kf_params *p, int mode, float *e_voxel,
-                             const uint8_t *flags_in = nullptr, uint8_t *flags_out = nullptr);
-// Number of row tiles of the adjoint row kernel for these dims (0: that kernel does not apply).
-size_t kf_adj_tile_count(const int32_t dims[3]);
-// Recomputes the tile flags of field values u.
-kf_status kf_launch_adj_flags(const kf_field *f, const float *u, uint8_t *flags);
+                             const unsigned int *mask_in = nullptr, unsigned int *mask_out = nullptr);
+// Words of a nonzero mask for these dims (0: the row kernel does not apply).
+size_t kf_adj_mask_words(const int32_t dims[3]);
+// Computes the nonzero mask of field values u.
+kf_status kf_launch_adj_mask(const kf_field *f, const float *u, unsigned int *mask);
 
 kf_status kf_mc_run(const kf_volume *v, double iso, uint64_t *n_tris, float *verts_host, uint64_t cap);
 
