@@ -98,7 +98,7 @@ typedef struct kf_deform_stats {
     int32_t reserved;
     int64_t diverged_index;  /* first voxel with a non-finite displacement, or -1 */
     double max_update;       /* max |du| of the last iteration, voxels (FP64) */
-    double energy;           /* adjoint: energy before the last update (FP64); reference: 0 */
+    double energy;           /* adjoint: energy of the final field (FP64); reference: 0 */
     uint64_t active_voxels;  /* voxels updated in the last iteration */
 } kf_deform_stats;
 

@@ -20,11 +20,12 @@
 //
 // Two launch shapes share the per-voxel arithmetic (adj_voxel):
 //  * k_adj_rows (rows whose length is a multiple of 4): a thread owns 4 consecutive x voxels
-//    of two rows, a wavefront 256 voxels of a row, a block 8 rows of one slice.  A bit mask of
-//    the groups of u that may be nonzero (written by the previous iteration) tells each thread
-//    whether its stencil sees only zeros; such quiet threads stream their own u, phi_live and
-//    the output rows as float4 / 8-byte loads and stores, and only the voxels that can
-//    contribute (active, or near nonzero u) run the full stencil.
+//    of two rows, a wavefront 256 voxels of a row, a block 8 rows of one slice.  A Jacobi step
+//    moves only active voxels: a voxel whose own u is zero samples phi_live[j] exactly, so the
+//    rows of u, phi_live and u_out stream as float4 / 8-byte accesses and only active voxels
+//    (and moved ones, to find out) run the stencil.  Energy and gradient passes, which need
+//    every voxel whose stencil sees a nonzero u, take that set from a bit mask of the 4-voxel
+//    groups of u that may be nonzero.
 //  * k_adj_any (any shape): one voxel per thread, neighbours read individually.
 // Blocks are numbered so each XCD sweeps a contiguous z range and the rows and slices shared
 // by neighbouring blocks stay in that XCD's L2.
@@ -253,7 +254,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_any(PhiRead<S> L, PhiRead<S> Cn, 
         kf_split(z, nb.uc[2], bz, fz);
         L.cell(bx, by, bz, v);
         VoxelOut o;
-        adj_voxel<MODE != KF_MODE_GRAD>(c, v, fx, fy, fz, c.use_data ? Cn.at(x, y, z) : 0.f, nb, x, y, z, o);
+        adj_voxel<MODE == KF_MODE_ENERGY>(c, v, fx, fy, fz, c.use_data ? Cn.at(x, y, z) : 0.f, nb, x, y, z, o);
         if (o.active)
             act = 1;
         if (MODE == KF_MODE_GRAD) {
@@ -399,8 +400,7 @@ template <class S, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, AdjConsts c, DeformState *st,
                                                     BlockPartial *partials, float *__restrict__ e_voxel, int nseg,
-                                                    int nyt, const unsigned int *__restrict__ mask_in,
-                                                    unsigned int *__restrict__ mask_out)
+                                                    int nyt, const unsigned int *__restrict__ mask_in)
 {
     if (MODE == KF_MODE_STEP && st->done)
         return;
@@ -440,29 +440,37 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
             live4[r][k] = 0.f;
         if (valid[r])
             S::load4(L.p, (int64_t)g.idx32(x0, y, z), L.scale, live4[r]);
-        nb_bits[r] = nbr_bits(mask_in, wpr, g, y, z, seg, lane) | nbr_bits(mask_in, wpr, g, y + 1, z, seg, lane) |
-                     nbr_bits(mask_in, wpr, g, y - 1, z, seg, lane) | nbr_bits(mask_in, wpr, g, y, z + 1, seg, lane) |
-                     nbr_bits(mask_in, wpr, g, y, z - 1, seg, lane) |
-                     nbr_bits(mask_in, wpr, g, y + 1, z - 1, seg, lane) |
-                     nbr_bits(mask_in, wpr, g, y - 1, z + 1, seg, lane);
+        nb_bits[r] = 0u;
+        if (MODE != KF_MODE_STEP)
+            nb_bits[r] = nbr_bits(mask_in, wpr, g, y, z, seg, lane) | nbr_bits(mask_in, wpr, g, y + 1, z, seg, lane) |
+                         nbr_bits(mask_in, wpr, g, y - 1, z, seg, lane) |
+                         nbr_bits(mask_in, wpr, g, y, z + 1, seg, lane) |
+                         nbr_bits(mask_in, wpr, g, y, z - 1, seg, lane) |
+                         nbr_bits(mask_in, wpr, g, y + 1, z - 1, seg, lane) |
+                         nbr_bits(mask_in, wpr, g, y - 1, z + 1, seg, lane);
     }
-    unsigned int need = 0u;  // bit r * VEC + k: voxel (x0 + k, yr[r]) can contribute
+    // need bit r * VEC + k: voxel (x0 + k, yr[r]) goes to phase 2.  A Jacobi step moves only
+    // active voxels, so it needs the voxels that are active or whose sample moved (u_j != 0,
+    // activity unknown until the gather); energy and gradient passes also need every voxel
+    // whose stencil sees a nonzero u.
+    unsigned int need = 0u;
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
         const bool quiet = nb_bits[r] == 0u && ((bits4(C[r][0]) | bits4(C[r][1]) | bits4(C[r][2])) & 0x7fffffffu) == 0u;
 #pragma unroll
-        for (int k = 0; k < VEC; k++)
-            if (!quiet || !(live4[r][k] > c.hi || live4[r][k] < c.lo))
+        for (int k = 0; k < VEC; k++) {
+            const bool moved = ((__float_as_uint(f4(C[r][0], k)) | __float_as_uint(f4(C[r][1], k)) |
+                                 __float_as_uint(f4(C[r][2], k))) & 0x7fffffffu) != 0u;
+            const bool in_band = !(live4[r][k] > c.hi || live4[r][k] < c.lo);
+            if (MODE == KF_MODE_STEP ? (moved || in_band) : (!quiet || in_band))
                 need |= 1u << (r * VEC + k);
+        }
     }
 
     // Defaults of the voxels that cannot contribute, four at a time; phase-2 voxels overwrite
     // their own elements below (same thread, so program order holds).
-    bool row_nz[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
-        row_nz[r] = MODE == KF_MODE_STEP &&
-                    ((bits4(C[r][0]) | bits4(C[r][1]) | bits4(C[r][2])) & 0x7fffffffu) != 0u;
         if (!valid[r])
             continue;
         const unsigned int i0 = g.idx32(x0, yr[r], z);
@@ -490,17 +498,25 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
         if (!(x < g.nx && y < g.ny))
             continue;
         const unsigned int i = g.idx32(x, y, z);
-        Nbhd nb;
-        gather_nbhd(u_in, g, x, y, z, nb);
         int bx, by, bz;
         float fx, fy, fz, v[8];
-        kf_split(x, nb.uc[0], bx, fx);
-        kf_split(y, nb.uc[1], by, fy);
-        kf_split(z, nb.uc[2], bz, fz);
+        {
+            const float u0 = u_in[i], u1 = u_in[n32 + i], u2 = u_in[2u * n32 + i];
+            kf_split(x, u0, bx, fx);
+            kf_split(y, u1, by, fy);
+            kf_split(z, u2, bz, fz);
+        }
         cell32(L, bx, by, bz, v);
+        if (MODE == KF_MODE_STEP) {
+            const float psi = kf_interp3(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], fx, fy, fz);
+            if (psi > c.hi || psi < c.lo)
+                continue;  // inactive: keeps u (already stored)
+        }
+        Nbhd nb;
+        gather_nbhd(u_in, g, x, y, z, nb);
         VoxelOut o;
-        adj_voxel<MODE != KF_MODE_GRAD>(c, v, fx, fy, fz, c.use_data ? S::load(Cn.p, (int64_t)i, Cn.scale) : 0.f,
-                                        nb, x, y, z, o);
+        adj_voxel<MODE == KF_MODE_ENERGY>(c, v, fx, fy, fz, c.use_data ? S::load(Cn.p, (int64_t)i, Cn.scale) : 0.f,
+                                          nb, x, y, z, o);
         if (o.active)
             act += 1;
         esum += (double)o.e;
@@ -517,18 +533,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
             u_out[i] = n[0];
             u_out[n32 + i] = n[1];
             u_out[2u * n32 + i] = n[2];
-            if (((__float_as_uint(n[0]) | __float_as_uint(n[1]) | __float_as_uint(n[2])) & 0x7fffffffu) != 0u) {
-                if (r == 0)
-                    row_nz[0] = true;
-                else
-                    row_nz[RPT - 1] = true;
-            }
         }
-    }
-    if (MODE == KF_MODE_STEP && mask_out) {
-#pragma unroll
-        for (int r = 0; r < RPT; r++)
-            put_bits(mask_out, wpr, g, yr[r], z, seg, lane, row_nz[r]);
     }
     if (MODE != KF_MODE_GRAD)
         kf_block_publish<BLOCK>(partials, blockIdx.x, esum, mx, act);
@@ -569,11 +574,11 @@ PhiRead<S> make_read(const kf_volume *v)
 template <class S, int MODE>
 void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float *u_out, const AdjConsts &c,
             kf_context *ctx, float *e_voxel, bool rows, unsigned int nb, int nseg, int nyt,
-            const unsigned int *mask_in, unsigned int *mask_out)
+            const unsigned int *mask_in)
 {
     if (rows)
         hipLaunchKernelGGL((k_adj_rows<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
-                           ctx->state, ctx->partials, e_voxel, nseg, nyt, mask_in, mask_out);
+                           ctx->state, ctx->partials, e_voxel, nseg, nyt, mask_in);
     else
         hipLaunchKernelGGL((k_adj_any<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
                            ctx->state, ctx->partials, e_voxel);
@@ -599,8 +604,7 @@ kf_status kf_launch_adj_mask(const kf_field *f, const float *u, unsigned int *ma
 }
 
 kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const float *u_in, float *u_out,
-                             const kf_params *p, int mode, float *e_voxel, const unsigned int *mask_in,
-                             unsigned int *mask_out)
+                             const kf_params *p, int mode, float *e_voxel, const unsigned int *mask_in)
 {
     kf_context *ctx = live->ctx;
     AdjConsts c{};
@@ -617,9 +621,10 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     c.use_ls = p->use_levelset;
     c.use_k = p->use_killing;
     KF_CHECK_ARG(live->dtype == can->dtype, "live and canonical volumes must share a dtype");
-    // The row kernel needs a nonzero mask of u_in, rows of a multiple of 4 voxels (float4 loads,
-    // 16-byte aligned plane starts) and 3 N < 2^30 (32-bit offsets).
-    const bool rows = mask_in != nullptr && kf_adj_mask_words(live->dims) > 0;
+    // The row kernel needs rows of a multiple of 4 voxels (float4 loads, 16-byte aligned plane
+    // starts), 3 N < 2^30 (32-bit offsets) and, for energy and gradient passes, the nonzero mask
+    // of u_in.
+    const bool rows = kf_adj_mask_words(live->dims) > 0 && (mode == KF_MODE_STEP || mask_in != nullptr);
     const int nseg = (live->dims[0] + ROW - 1) / ROW;
     const int nyt = (live->dims[1] + ROWS - 1) / ROWS;
     const unsigned int nb = rows ? (unsigned int)(nseg * nyt * live->dims[2])
@@ -630,11 +635,11 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     KF_DISPATCH_DTYPE(live->dtype, S, {
         PhiRead<S> L = make_read<S>(live), Cn = make_read<S>(can);
         if (mode == KF_MODE_GRAD)
-            launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in, nullptr);
+            launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in);
         else if (mode == KF_MODE_ENERGY)
-            launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in, nullptr);
+            launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in);
         else
-            launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in, mask_out);
+            launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, nullptr);
     });
     KF_LAUNCH_CHECK();
     return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb);

@@ -666,25 +666,12 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
         return s;
     const int cur0 = u->cur;
     const bool check = p->stop_max_update > 0.0;
-    // Nonzero masks for the adjoint row kernel, recomputed from the field's current values
-    // (anything may have written them since the last deform).
-    const bool flagged = p->method == KF_METHOD_ADJOINT && p->max_iters > 0 && kf_adj_mask_words(u->dims) > 0;
-    if (flagged) {
-        s = ensure_masks(u);
-        if (s == KF_OK)
-            s = kf_launch_adj_mask(u, u->buf[cur0], u->masks[cur0]);
-        if (s != KF_OK)
-            return s;
-    }
     constexpr int CHUNK = 16;
     for (int k = 0; k < p->max_iters; k++) {
         float *in = u->buf[cur0 ^ (k & 1)];
         float *out = u->buf[cur0 ^ ((k + 1) & 1)];
-        s = p->method == KF_METHOD_ADJOINT
-                ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr,
-                                     flagged ? u->masks[cur0 ^ (k & 1)] : nullptr,
-                                     flagged ? u->masks[cur0 ^ ((k + 1) & 1)] : nullptr)
-                : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP);
+        s = p->method == KF_METHOD_ADJOINT ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr)
+                                           : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP);
         if (s != KF_OK)
             return s;
         if (check && (k + 1) % CHUNK == 0 && k + 1 < p->max_iters) {
@@ -698,14 +685,22 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
     s = read_state(ctx);
     if (s != KF_OK)
         return s;
-    const DeformState &h = *ctx->state_host;
+    const DeformState h = *ctx->state_host;
     u->cur = cur0 ^ (int)(h.iters & 1u);
+    double energy = 0.0;
+    if (p->method == KF_METHOD_ADJOINT && h.diverged == LLONG_MAX) {
+        // The step kernels skip the energy of voxels that do not move; one forward pass on the
+        // final field gives the energy the optimiser reached.
+        s = kf_energy_forward(live, can, u, p, &energy, nullptr);
+        if (s != KF_OK)
+            return s;
+    }
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->iters = (int32_t)h.iters;
         stats->diverged_index = h.diverged == LLONG_MAX ? -1 : (int64_t)h.diverged;
         stats->max_update = h.last_max;
-        stats->energy = p->method == KF_METHOD_ADJOINT ? h.energy : 0.0;
+        stats->energy = energy;
         stats->active_voxels = h.last_active;
     }
     if (h.diverged != LLONG_MAX) {

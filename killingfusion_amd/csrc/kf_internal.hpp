@@ -273,11 +273,11 @@ kf_status kf_launch_field_add(kf_field *dst, const kf_field *src);
 struct DeformArgs;
 kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const float *u_in,
                              float *u_out, const kf_params *p, int mode);
-// mask_in / mask_out: nonzero masks of u_in / u_out (see k_deform_adj.hip); without mask_in the
-// one-voxel-per-thread kernel runs.
+// mask_in: nonzero mask of u_in, needed by energy and gradient passes of the row kernel (see
+// k_deform_adj.hip); without it those passes run one voxel per thread.
 kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const float *u_in,
                              float *u_out, const kf_params *p, int mode, float *e_voxel,
-                             const unsigned int *mask_in = nullptr, unsigned int *mask_out = nullptr);
+                             const unsigned int *mask_in = nullptr);
 // Words of a nonzero mask for these dims (0: the row kernel does not apply).
 size_t kf_adj_mask_words(const int32_t dims[3]);
 // Computes the nonzero mask of field values u.

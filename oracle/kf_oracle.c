@@ -1067,8 +1067,9 @@ int kfo_adj_deform(const double *live, const double *can, double *u, const kfo_g
     st->active = 0;
     for (int it = 0; it < p->max_iters; it++) {
         int64_t act = 0;
-        st->energy = kfo_adj_energy(live, can, u, g, p, NULL, &act);
         kfo_adj_gradient(live, can, u, g, p, grad, mask);
+        for (int64_t i = 0; i < n; i++)
+            act += mask[i];
         double m = 0.0;
         for (int64_t i = 0; i < n; i++) {
             if (!mask[i])
@@ -1092,6 +1093,8 @@ int kfo_adj_deform(const double *live, const double *can, double *u, const kfo_g
         if (p->stop_max_update > 0.0 && m < p->stop_max_update)
             break;
     }
+    if (st->diverged_index < 0)
+        st->energy = kfo_adj_energy(live, can, u, g, p, NULL, NULL);
     free(grad);
     free(mask);
     return st->diverged_index >= 0;

@@ -70,7 +70,7 @@ typedef struct kfo_stats {
     int32_t iters;
     int64_t diverged_index;  /* -1 if all finite */
     double max_update;       /* max |du| of the last iteration (voxel) */
-    double energy;           /* adjoint only: energy at the start of the last iteration */
+    double energy;           /* adjoint only: energy of the final field */
     int64_t active;          /* active voxels in the last iteration */
 } kfo_stats;
 
