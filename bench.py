@@ -107,6 +107,25 @@ def cpu_baseline(live_phi, can_phi, kp, iters):
     return g.n * st["iters"] / dt, O.num_threads(), st["iters"], dt
 
 
+def calibrate_copy(launches=5):
+    """A kernel of known HBM traffic for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters:
+    kf_field_add on two 256^3 fields reads 2 x 3N floats and writes 3N floats per launch
+    (402,653,184 B read, 201,326,592 B written)."""
+    from killingfusion_amd import Context, DisplacementField
+    ctx = Context(0)
+    dims = (GRID, GRID, GRID)
+    a, b = DisplacementField(ctx, dims), DisplacementField(ctx, dims)
+    a.initializeAllVoxels((0.1, 0.2, 0.3))
+    b.initializeAllVoxels((0.1, 0.2, 0.3))
+    for _ in range(launches):
+        a += b
+    ctx.synchronize()
+    print(json.dumps({"calibration": "kf_field_add 256^3", "launches": launches,
+                      "read_bytes_per_launch": 2 * 3 * 4 * GRID ** 3, "write_bytes_per_launch": 3 * 4 * GRID ** 3}),
+          flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,11 +134,15 @@ def main(argv=None):
     ap.add_argument("--plain-steps", type=int, default=0, help="iterations timed for the plain kernel (default steps/4)")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run the known-traffic copy kernel used to scale rocprofv3 PMC counters")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build (one GPU per box).",
               file=sys.stderr)
         return 2
+    if a.calibrate_copy:
+        return calibrate_copy()
 
     from killingfusion_amd import Context, KillingParams, SDF, scenes
 

@@ -188,7 +188,7 @@ __device__ inline void adj_voxel(const AdjConsts &c, const float v[8], float fx,
 // Jacobi update of one voxel: active voxels move by -alpha dE/du; non-finite results are
 // reported (KillingFusion.cpp:327-332).  Returns the new value in n[3].
 __device__ inline void adj_update(const AdjConsts &c, const VoxelOut &o, const float uc[3], int64_t i,
-                                  DeformState *st, double &mx, float n[3])
+                                  long long *div, double &mx, float n[3])
 {
     n[0] = uc[0];
     n[1] = uc[1];
@@ -199,7 +199,7 @@ __device__ inline void adj_update(const AdjConsts &c, const VoxelOut &o, const f
         n[1] = uc[1] + d1;
         n[2] = uc[2] + d2;
         if (!(isfinite(n[0]) && isfinite(n[1]) && isfinite(n[2])))
-            atomicMin((long long *)&st->diverged, (long long)i);
+            atomicMin(div, (long long)i);
         else
             mx = fmax(mx, sqrt((double)d0 * d0 + (double)d1 * d1 + (double)d2 * d2));
     }
@@ -210,15 +210,38 @@ __device__ inline float uval(const float *__restrict__ u, const GridDims &g, int
     return g.in(x, y, z) ? u[(int64_t)c * g.n + g.idx(x, y, z)] : 0.0f;
 }
 
+// Gate of step launch k (parity k & 1): the launch runs iff st->run[parity].  Its first block
+// folds launch k - 1 (prev_nb partials) and then arms run[parity ^ 1] for launch k + 1, so
+// a launch never reads a gate field it writes: every wave of every block sees the same
+// answer.  Launch k itself may run after launch k - 1 met the stop rule; it then only writes
+// the buffer that is not the result (see kf_deform), and launch k + 1 does nothing.
+template <int MODE>
+__device__ inline bool step_gate(DeformState *st, const BlockPartial *prev, unsigned int prev_nb, int parity)
+{
+    if (MODE != KF_MODE_STEP)
+        return true;
+    const bool go = st->run[parity] != 0u;
+    if (blockIdx.x == 0) {
+        if (go && prev_nb > 0) {
+            kf_fold_launch<BLOCK>(st, prev, prev_nb, parity ^ 1);
+            __syncthreads();  // the block's own publish reuses the reduction LDS
+        }
+        if (threadIdx.x == 0)
+            st->run[parity ^ 1] = (go && !st->done) ? 1u : 0u;
+    }
+    return go;
+}
+
 // ---------------------------------------------------------------------------
 // Any shape: one voxel per thread.
 // ---------------------------------------------------------------------------
 template <class S, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_adj_any(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
                                                    float *__restrict__ u_out, AdjConsts c, DeformState *st,
-                                                   BlockPartial *partials, float *__restrict__ e_voxel)
+                                                   BlockPartial *partials, float *__restrict__ e_voxel, int parity,
+                                                   const BlockPartial *prev, unsigned int prev_nb)
 {
-    if (MODE == KF_MODE_STEP && st->done)
+    if (!step_gate<MODE>(st, prev, prev_nb, parity))
         return;
     const GridDims g = L.g;
     const int64_t i = (int64_t)kf_xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
@@ -266,7 +289,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_any(PhiRead<S> L, PhiRead<S> Cn, 
                 e_voxel[i] = o.e;
         } else {
             float n[3];
-            adj_update(c, o, nb.uc, i, st, mx, n);
+            adj_update(c, o, nb.uc, i, &st->div_slot[parity], mx, n);
             u_out[i] = n[0];
             u_out[g.n + i] = n[1];
             u_out[2 * g.n + i] = n[2];
@@ -400,9 +423,10 @@ template <class S, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, AdjConsts c, DeformState *st,
                                                     BlockPartial *partials, float *__restrict__ e_voxel, int nseg,
-                                                    int nyt, const unsigned int *__restrict__ mask_in)
+                                                    int nyt, const unsigned int *__restrict__ mask_in, int parity,
+                                                    const BlockPartial *prev, unsigned int prev_nb)
 {
-    if (MODE == KF_MODE_STEP && st->done)
+    if (!step_gate<MODE>(st, prev, prev_nb, parity))
         return;
     const GridDims g = L.g;
     const unsigned int tile = kf_xcd_tile(blockIdx.x, gridDim.x);
@@ -462,7 +486,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
             const bool moved = ((__float_as_uint(f4(C[r][0], k)) | __float_as_uint(f4(C[r][1], k)) |
                                  __float_as_uint(f4(C[r][2], k))) & 0x7fffffffu) != 0u;
             const bool in_band = !(live4[r][k] > c.hi || live4[r][k] < c.lo);
-            if (MODE == KF_MODE_STEP ? (moved || in_band) : (!quiet || in_band))
+            if (valid[r] && (MODE == KF_MODE_STEP ? (moved || in_band) : (!quiet || in_band)))
                 need |= 1u << (r * VEC + k);
         }
     }
@@ -486,15 +510,52 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
         }
     }
 
-    // Phase 2 -- the voxels that can contribute: full energy and gradient.
+    // Phase 2 -- the voxels that can contribute: full energy and gradient.  The wavefront's
+    // need bits become one job list (ordered by bit, then lane), so a band segment held by one
+    // or two lanes is spread over the wavefront: a lane runs one gather chain instead of up to
+    // eight in a row.  A job's stores may hit elements another lane of this wavefront stored
+    // as defaults above; a wavefront's vector memory operations are performed in order.
+    unsigned long long bal[RPT * VEC];
+    int total = 0;
+#pragma unroll
+    for (int b = 0; b < RPT * VEC; b++) {
+        bal[b] = __ballot((need >> b) & 1u);
+        total += __popcll(bal[b]);
+    }
     double mx = 0.0, esum = 0.0;
     unsigned long long act = 0;
-#pragma unroll 1
-    for (int b = 0; b < RPT * VEC; b++) {
-        if (!(need & (1u << b)))
-            continue;
-        const int r = b / VEC, k = b % VEC;
-        const int x = x0 + k, y = r == 0 ? yr[0] : yr[RPT - 1];
+    for (int j = lane; j < total; j += 64) {
+        // Job j -> need bit `bit` of lane `src`: skip whole bits, then find the rem-th set
+        // lane of that bit's ballot by halving.
+        int rem = j, bit = 0;
+        unsigned long long m = 0ull;
+        bool found = false;
+#pragma unroll
+        for (int b = 0; b < RPT * VEC; b++) {
+            const int cnt = __popcll(bal[b]);
+            if (!found) {
+                if (rem < cnt) {
+                    bit = b;
+                    m = bal[b];
+                    found = true;
+                } else {
+                    rem -= cnt;
+                }
+            }
+        }
+        int src = 0;
+#pragma unroll
+        for (int h = 32; h >= 1; h >>= 1) {
+            const int lo = __popcll(m & ((1ull << h) - 1ull));
+            if (rem >= lo) {
+                rem -= lo;
+                m >>= h;
+                src += h;
+            }
+        }
+        const int r = bit / VEC;
+        const int x = seg * ROW + src * VEC + bit % VEC;
+        const int y = yt * ROWS + w + r * (BLOCK / 64);
         if (!(x < g.nx && y < g.ny))
             continue;
         const unsigned int i = g.idx32(x, y, z);
@@ -506,12 +567,9 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
             kf_split(y, u1, by, fy);
             kf_split(z, u2, bz, fz);
         }
+        // The cell corners and the u neighbourhood load together (one round trip); an
+        // inactive voxel of a step keeps the u_j already stored.
         cell32(L, bx, by, bz, v);
-        if (MODE == KF_MODE_STEP) {
-            const float psi = kf_interp3(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], fx, fy, fz);
-            if (psi > c.hi || psi < c.lo)
-                continue;  // inactive: keeps u (already stored)
-        }
         Nbhd nb;
         gather_nbhd(u_in, g, x, y, z, nb);
         VoxelOut o;
@@ -527,9 +585,9 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
         } else if (MODE == KF_MODE_ENERGY) {
             if (e_voxel)
                 e_voxel[i] = o.e;
-        } else {
+        } else if (o.active) {
             float n[3];
-            adj_update(c, o, nb.uc, (int64_t)i, st, mx, n);
+            adj_update(c, o, nb.uc, (int64_t)i, &st->div_slot[parity], mx, n);
             u_out[i] = n[0];
             u_out[n32 + i] = n[1];
             u_out[2u * n32 + i] = n[2];
@@ -574,14 +632,15 @@ PhiRead<S> make_read(const kf_volume *v)
 template <class S, int MODE>
 void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float *u_out, const AdjConsts &c,
             kf_context *ctx, float *e_voxel, bool rows, unsigned int nb, int nseg, int nyt,
-            const unsigned int *mask_in)
+            const unsigned int *mask_in, int parity, const BlockPartial *prev, unsigned int prev_nb)
 {
+    BlockPartial *parts = kf_partials(ctx, parity);
     if (rows)
         hipLaunchKernelGGL((k_adj_rows<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
-                           ctx->state, ctx->partials, e_voxel, nseg, nyt, mask_in);
+                           ctx->state, parts, e_voxel, nseg, nyt, mask_in, parity, prev, prev_nb);
     else
         hipLaunchKernelGGL((k_adj_any<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
-                           ctx->state, ctx->partials, e_voxel);
+                           ctx->state, parts, e_voxel, parity, prev, prev_nb);
 }
 
 }  // namespace
@@ -604,7 +663,8 @@ kf_status kf_launch_adj_mask(const kf_field *f, const float *u, unsigned int *ma
 }
 
 kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const float *u_in, float *u_out,
-                             const kf_params *p, int mode, float *e_voxel, const unsigned int *mask_in)
+                             const kf_params *p, int mode, float *e_voxel, const unsigned int *mask_in, int iter,
+                             unsigned int *nb_out)
 {
     kf_context *ctx = live->ctx;
     AdjConsts c{};
@@ -632,15 +692,24 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     kf_status s = kf_ensure_partials(ctx, nb);
     if (s != KF_OK)
         return s;
+    if (nb_out)
+        *nb_out = nb;
+    // Energy and gradient passes use partial set 0; step launch k uses set k & 1 and folds
+    // launch k - 1 (same shape: the dims do not change within a deform call).
+    const int parity = mode == KF_MODE_STEP ? (iter & 1) : 0;
+    const BlockPartial *prev = kf_partials(ctx, parity ^ 1);
+    const unsigned int prev_nb = mode == KF_MODE_STEP && iter > 0 ? nb : 0u;
     KF_DISPATCH_DTYPE(live->dtype, S, {
         PhiRead<S> L = make_read<S>(live), Cn = make_read<S>(can);
         if (mode == KF_MODE_GRAD)
-            launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in);
+            launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in, 0, prev, 0u);
         else if (mode == KF_MODE_ENERGY)
-            launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in);
+            launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in, 0, prev,
+                                      0u);
         else
-            launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, nullptr);
+            launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, nullptr, parity, prev,
+                                    prev_nb);
     });
     KF_LAUNCH_CHECK();
-    return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb);
+    return mode == KF_MODE_ENERGY ? kf_launch_finalize(ctx, nb, 0) : KF_OK;
 }

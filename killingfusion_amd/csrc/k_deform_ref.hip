@@ -214,7 +214,7 @@ __device__ inline void ref_gradient(const PhiView<S> &L, const PhiView<S> &Cn, c
 template <class S, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_ref_step(PhiView<S> L, PhiView<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, RefConsts c, DeformState *st,
-                                                    BlockPartial *partials)
+                                                    BlockPartial *partials, int parity)
 {
     if (MODE == KF_MODE_STEP && st->done)
         return;
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_step(PhiView<S> L, PhiView<S> Cn,
             if (active) {
                 double dn = sqrt((double)d0 * d0 + (double)d1 * d1 + (double)d2 * d2);
                 if (!(isfinite(n0) && isfinite(n1) && isfinite(n2)))
-                    atomicMin((long long *)&st->diverged, (long long)i);
+                    atomicMin((long long *)&st->div_slot[parity], (long long)i);
                 else
                     mx = dn;
             }
@@ -274,9 +274,10 @@ PhiView<S> make_view(const kf_volume *v)
 }  // namespace
 
 kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const float *u_in, float *u_out,
-                             const kf_params *p, int mode)
+                             const kf_params *p, int mode, int iter)
 {
     kf_context *ctx = live->ctx;
+    const int parity = iter & 1;
     RefConsts c{};
     c.voxel_size = (float)p->voxel_size;
     c.hi = (float)(p->max_surface - p->epsilon);
@@ -300,11 +301,11 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
         PhiView<S> L = make_view<S>(live), Cn = make_view<S>(can);
         if (mode == KF_MODE_GRAD)
             hipLaunchKernelGGL((k_ref_step<S, KF_MODE_GRAD>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in,
-                               u_out, c, ctx->state, ctx->partials);
+                               u_out, c, ctx->state, kf_partials(ctx, parity), parity);
         else
             hipLaunchKernelGGL((k_ref_step<S, KF_MODE_STEP>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in,
-                               u_out, c, ctx->state, ctx->partials);
+                               u_out, c, ctx->state, kf_partials(ctx, parity), parity);
     });
     KF_LAUNCH_CHECK();
-    return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb);
+    return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb, parity);
 }

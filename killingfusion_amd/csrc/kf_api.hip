@@ -19,7 +19,11 @@ __global__ void k_state_reset(DeformState *st, double stop)
     if (threadIdx.x == 0) {
         st->done = 0;
         st->iters = 0;
+        st->run[0] = 1;
+        st->run[1] = 1;
         st->diverged = LLONG_MAX;
+        st->div_slot[0] = LLONG_MAX;
+        st->div_slot[1] = LLONG_MAX;
         st->stop = stop;
         st->last_max = 0.0;
         st->energy = 0.0;
@@ -125,7 +129,7 @@ kf_status kf_ensure_partials(kf_context *ctx, size_t nblocks)
         ctx->partials = nullptr;
         ctx->partials_cap = 0;
     }
-    KF_HIP(hipMalloc((void **)&ctx->partials, sizeof(BlockPartial) * nblocks));
+    KF_HIP(hipMalloc((void **)&ctx->partials, 2 * sizeof(BlockPartial) * nblocks));
     ctx->partials_cap = nblocks;
     return KF_OK;
 }
@@ -134,50 +138,23 @@ namespace {
 
 constexpr int FIN_BLOCK = 1024;
 
-// Ends one deform iteration: folds the per-block partials in index order (so the FP64
+// Ends one deform iteration: folds one step launch's partials in index order (so the FP64
 // energy sum is reproducible), records max |du| and the active count, advances the
 // iteration counter and raises `done` when the stop rule fires or a voxel diverged.
-__global__ __launch_bounds__(FIN_BLOCK) void k_finalize(DeformState *st, const BlockPartial *parts, unsigned int nb)
+__global__ __launch_bounds__(FIN_BLOCK) void k_finalize(DeformState *st, const BlockPartial *parts, unsigned int nb,
+                                                        int parity)
 {
     if (st->done)
         return;
-    double s = 0.0, m = 0.0;
-    unsigned long long c = 0;
-    // Blocks of 8 strided slots per thread: the loads of a block are issued together.
-    for (unsigned int b0 = threadIdx.x; b0 < nb; b0 += 8 * FIN_BLOCK) {
-        BlockPartial p[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const unsigned int b = b0 + (unsigned int)j * FIN_BLOCK;
-            if (b < nb)
-                p[j] = parts[b];
-            else
-                p[j] = BlockPartial{0.0, 0.0, 0ull, 0ull};
-        }
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            s += p[j].sum;
-            m = fmax(m, p[j].max);
-            c += p[j].cnt;
-        }
-    }
-    kf_block_reduce3<FIN_BLOCK>(s, m, c);
-    if (threadIdx.x == 0) {
-        st->last_max = m;
-        st->energy = s;
-        st->last_active = c;
-        st->iters += 1;
-        if (st->diverged != LLONG_MAX || (st->stop > 0.0 && m < st->stop))
-            st->done = 1;
-    }
+    kf_fold_launch<FIN_BLOCK>(st, parts, nb, parity);
 }
 
 }  // namespace
 
-kf_status kf_launch_finalize(kf_context *ctx, unsigned int nb)
+kf_status kf_launch_finalize(kf_context *ctx, unsigned int nb, int parity)
 {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->state,
-                       (const BlockPartial *)ctx->partials, nb);
+                       (const BlockPartial *)kf_partials(ctx, parity), nb, parity);
     KF_LAUNCH_CHECK();
     return KF_OK;
 }
@@ -666,14 +643,22 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
         return s;
     const int cur0 = u->cur;
     const bool check = p->stop_max_update > 0.0;
+    const bool adj = p->method == KF_METHOD_ADJOINT;
     constexpr int CHUNK = 16;
+    // Launch k reads buffer cur0 ^ (k & 1) and writes the other one.  An adjoint launch k folds
+    // launch k - 1 in its first block, so launch k itself may run before the stop rule (or a
+    // divergence) of k - 1 is known: it only writes the buffer that is not the result, and
+    // launch k + 1 sees `done` and does nothing.
+    int launched = 0;
+    unsigned int nb = 0;
     for (int k = 0; k < p->max_iters; k++) {
         float *in = u->buf[cur0 ^ (k & 1)];
         float *out = u->buf[cur0 ^ ((k + 1) & 1)];
-        s = p->method == KF_METHOD_ADJOINT ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr)
-                                           : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP);
+        s = adj ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr, nullptr, k, &nb)
+                : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP, k);
         if (s != KF_OK)
             return s;
+        launched = k + 1;
         if (check && (k + 1) % CHUNK == 0 && k + 1 < p->max_iters) {
             s = read_state(ctx);
             if (s != KF_OK)
@@ -681,6 +666,11 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
             if (ctx->state_host->done)
                 break;
         }
+    }
+    if (adj && launched > 0) {
+        s = kf_launch_finalize(ctx, nb, (launched - 1) & 1);
+        if (s != KF_OK)
+            return s;
     }
     s = read_state(ctx);
     if (s != KF_OK)

@@ -45,9 +45,13 @@ void kf_set_error(const std::string &msg);
 // ---------------------------------------------------------------------------
 
 struct DeformState {
-    unsigned int done;          // set by k_finalize when the stop rule fires or a voxel diverged
-    unsigned int iters;         // iterations completed
+    unsigned int done;          // set when the stop rule fires or a voxel diverged
+    unsigned int iters;         // iterations completed (launches folded while not done)
+    unsigned int run[2];        // adjoint step launch k runs iff run[k & 1]; launch k arms run[(k + 1) & 1]
+    unsigned int pad;
     long long diverged;             // lowest index with a non-finite update, INT64_MAX if none
+    long long div_slot[2];          // step launch k takes atomicMin into div_slot[k & 1]; the fold of
+                                    // launch k moves it to `diverged` and re-arms it
     double stop;                    // stop threshold (<= 0: never)
     double last_max;                // max |du| of the last completed launch
     double energy;                  // energy of the last completed launch
@@ -72,8 +76,8 @@ struct kf_context {
     bool own_stream = false;
     DeformState *state = nullptr;     // device
     DeformState *state_host = nullptr;  // pinned mirror for stats readback
-    BlockPartial *partials = nullptr;  // device, one slot per block of a step launch
-    size_t partials_cap = 0;
+    BlockPartial *partials = nullptr;  // device, two sets (launch parity) of one slot per block
+    size_t partials_cap = 0;           // slots per set
     int num_cus = 256;
 };
 
@@ -255,9 +259,11 @@ __device__ __host__ inline R kf_interp3(R v000, R v001, R v010, R v011, R v100, 
 // ---------------------------------------------------------------------------
 
 kf_status kf_ensure_partials(kf_context *ctx, size_t nblocks);
+inline BlockPartial *kf_partials(kf_context *ctx, int parity) { return ctx->partials + (size_t)(parity & 1) * ctx->partials_cap; }
 kf_status kf_state_reset(kf_context *ctx, double stop);
-// Folds nb block partials into the iteration state (one block, fixed order).
-kf_status kf_launch_finalize(kf_context *ctx, unsigned int nb);
+// Folds the nb block partials of the launch with this parity into the iteration state
+// (one block, fixed order).
+kf_status kf_launch_finalize(kf_context *ctx, unsigned int nb, int parity);
 
 kf_status kf_launch_fill_volume(kf_volume *v);
 kf_status kf_launch_upload_volume(kf_volume *v, const float *dphi_f32, const int32_t *dw);
@@ -271,13 +277,18 @@ kf_status kf_launch_field_fill(kf_field *f, const double v[3]);
 kf_status kf_launch_field_add(kf_field *dst, const kf_field *src);
 
 struct DeformArgs;
+// Step launches take their index `iter` within a deform call: launch k writes partial set k & 1.
+// The reference step folds its own partials right after the launch.
 kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const float *u_in,
-                             float *u_out, const kf_params *p, int mode);
+                             float *u_out, const kf_params *p, int mode, int iter = 0);
 // mask_in: nonzero mask of u_in, needed by energy and gradient passes of the row kernel (see
 // k_deform_adj.hip); without it those passes run one voxel per thread.
+// An adjoint step launch k > 0 folds launch k - 1 in its first block, so a deform loop ends
+// with kf_launch_finalize(ctx, *nb_out, last & 1).  Energy passes fold themselves.
 kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const float *u_in,
                              float *u_out, const kf_params *p, int mode, float *e_voxel,
-                             const unsigned int *mask_in = nullptr);
+                             const unsigned int *mask_in = nullptr, int iter = 0,
+                             unsigned int *nb_out = nullptr);
 // Words of a nonzero mask for these dims (0: the row kernel does not apply).
 size_t kf_adj_mask_words(const int32_t dims[3]);
 // Computes the nonzero mask of field values u.

@@ -4,6 +4,8 @@
 
 #include "kf_internal.hpp"
 
+#include <climits>
+
 // Wavefront (64 lanes) butterfly reductions.
 __device__ inline double kf_wave_sum(double v)
 {
@@ -77,5 +79,47 @@ __device__ inline void kf_block_publish(BlockPartial *parts, unsigned int slot, 
         p.cnt = cnt;
         p.pad = 0ull;
         parts[slot] = p;
+    }
+}
+
+// One whole block folds the nb partials of one step launch (slot order fixed: strided by
+// BLOCK, eight loads in flight per thread) into the iteration state.  `parity` names the
+// launch's divergence slot.  A launch folded after the state is done changes nothing.
+// Uses the LDS of kf_block_reduce3: a caller that reduces again afterwards puts a barrier
+// in between.
+template <int BLOCK>
+__device__ inline void kf_fold_launch(DeformState *st, const BlockPartial *parts, unsigned int nb, int parity)
+{
+    double s = 0.0, m = 0.0;
+    unsigned long long c = 0;
+    for (unsigned int b0 = threadIdx.x; b0 < nb; b0 += 8 * BLOCK) {
+        BlockPartial p[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const unsigned int b = b0 + (unsigned int)j * BLOCK;
+            if (b < nb)
+                p[j] = parts[b];
+            else
+                p[j] = BlockPartial{0.0, 0.0, 0ull, 0ull};
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            s += p[j].sum;
+            m = fmax(m, p[j].max);
+            c += p[j].cnt;
+        }
+    }
+    kf_block_reduce3<BLOCK>(s, m, c);
+    if (threadIdx.x == 0 && !st->done) {
+        const long long dv = st->div_slot[parity & 1];
+        st->div_slot[parity & 1] = LLONG_MAX;
+        st->last_max = m;
+        st->energy = s;
+        st->last_active = c;
+        st->iters += 1;
+        if (dv != LLONG_MAX)
+            st->diverged = dv;
+        if (dv != LLONG_MAX || (st->stop > 0.0 && m < st->stop))
+            st->done = 1;
     }
 }

@@ -205,6 +205,37 @@ def test_adjoint_deform_matches_fp64_oracle(ctx, c1):
     assert st["energy"] == pytest.approx(so["energy"], rel=1e-4)
 
 
+def test_adjoint_stop_rule(ctx, c1):
+    """The stop rule (max |du| below a threshold; the reference loop itself always runs its
+    iteration count) fires between host checks: the step launch after the stopping one runs
+    before the stop is known, so this checks that the reported iteration count and field are
+    those of the stopping iteration."""
+    print("case: adjoint deform 64^3 with a stop threshold", flush=True)
+    lphi, _ = c1["live"].download()
+    cphi, _ = c1["can"].download()
+    lphi, cphi = lphi.astype(np.float64), cphi.astype(np.float64)
+    # Per-iteration max |du| of the oracle; pick the first iteration from 2 on that sets a new
+    # minimum by a clear margin, and a threshold between it and the previous minimum.
+    u = np.zeros((3, c1["g"].n))
+    ms = []
+    for _ in range(45):
+        u, so = O.adj_deform(lphi, cphi, u, c1["g"], oparams(c1_params(method="adjoint", max_iters=1)))
+        ms.append(so["max_update"])
+    kstar = next(k for k in range(2, 45) if ms[k] < min(ms[:k]) * 0.98)
+    thr = 0.5 * (ms[kstar] + min(ms[:kstar]))
+    kp = c1_params(method="adjoint", max_iters=200, stop_max_update=thr)
+    uo, so = O.adj_deform(lphi, cphi, np.zeros((3, c1["g"].n)), c1["g"], oparams(kp))
+    assert so["iters"] == kstar + 1
+    field = DisplacementField(ctx, C1["dims"])
+    st = compute_displacement_field(c1["live"], c1["can"], field, kp)
+    ug = field.download().astype(np.float64)
+    err_report("adj stop rule", ug, uo)
+    print("iters", st["iters"], so["iters"], "max_update", st["max_update"], so["max_update"], flush=True)
+    assert st["iters"] == so["iters"]
+    assert st["max_update"] == pytest.approx(so["max_update"], rel=1e-3)
+    assert np.abs(ug - uo).max() <= 1e-3
+
+
 @pytest.mark.parametrize("dtype", ["bf16", "f16"])
 def test_low_precision_storage(ctx, c1, dtype):
     print(f"case: {dtype} storage round trip + adjoint deform 64^3", flush=True)
