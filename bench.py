@@ -107,6 +107,56 @@ def cpu_baseline(live_phi, can_phi, kp, iters):
     return g.n * st["iters"] / dt, O.num_threads(), st["iters"], dt
 
 
+def bench_batched(ctx, events, iters=50, reps=8, count=16):
+    """Independent volumes of uneven size in one batch: cube edges drawn from 48..160, random
+    ellipsoid pairs, BF16 storage, `iters` adjoint iterations per call (one persistent launch
+    per iteration, the loop a replayed hipGraph, tile shape from the autotuner).  Compared
+    with the same volumes deformed one kf_deform call at a time."""
+    from killingfusion_amd import (DisplacementField, KillingParams, SDF, batch_autotune, batch_deform,
+                                   compute_displacement_field, scenes, _lib)
+    vox = scenes.BATCH_VOXEL
+    lives, cans, fields = [], [], []
+    for n, mn, d1, d2 in scenes.batch_scene(count):
+        kw = dict(voxel_size=vox, clip_distance=3 * vox, max_surface=3 * vox, dtype="bf16")
+        live, can = SDF(ctx, (n, n, n), mn, **kw), SDF(ctx, (n, n, n), mn, **kw)
+        live.integrateDepthFrame(d2, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+        can.integrateDepthFrame(d1, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+        lives.append(live)
+        cans.append(can)
+        fields.append(DisplacementField(ctx, (n, n, n)))
+    nvox = sum(v.n for v in lives)
+    kp = KillingParams.paper(vox, 3.0, method="adjoint", max_iters=iters)
+    cfg = batch_autotune(lives, cans, fields, kp)
+    batch_deform(lives, cans, fields, kp, stats=False)  # captures the graph
+    stream = _lib.load().kf_context_stream(ctx.handle)
+    e0, e1 = events.event(), events.event()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    events.record(e0, stream)
+    for _ in range(reps):
+        batch_deform(lives, cans, fields, kp, stats=False)
+    events.record(e1, stream)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = events.elapsed_ms(e0, e1)
+    # one volume at a time (kf_deform per volume, each with its stats readback)
+    t1 = time.perf_counter()
+    for _ in range(2):
+        for live, can, f in zip(lives, cans, fields):
+            compute_displacement_field(live, can, f, kp)
+    ctx.synchronize()
+    wall_loop = (time.perf_counter() - t1) / 2
+    t2 = time.perf_counter()
+    batch_deform(lives, cans, fields, kp, stats=True)
+    wall_stats = time.perf_counter() - t2
+    value = nvox * iters * reps / wall
+    return {"volumes": count, "edges": [v.dims[0] for v in lives], "voxels": nvox, "iterations": iters,
+            "value": value, "unit": "voxel-iterations/s", "ms_per_call": wall * 1e3 / reps,
+            "gpu_ms_per_call": gpu_ms / reps, "ms_per_call_with_stats": wall_stats * 1e3,
+            "looped_ms_per_batch": wall_loop * 1e3, "speedup_vs_looped": wall_loop / (wall / reps),
+            "tile_shape": cfg, "phi_storage": "bf16"}
+
+
 def calibrate_copy(launches=5):
     """A kernel of known HBM traffic for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters:
     kf_field_add on two 256^3 fields reads 2 x 3N floats and writes 3N floats per launch
@@ -134,6 +184,7 @@ def main(argv=None):
     ap.add_argument("--plain-steps", type=int, default=0, help="iterations timed for the plain kernel (default steps/4)")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-volume measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the known-traffic copy kernel used to scale rocprofv3 PMC counters")
     a = ap.parse_args(argv)
@@ -180,6 +231,8 @@ def main(argv=None):
     plain_value = nvox * plain_steps / wall_p
     plain_kernel_ms = gpu_ms_p / plain_steps
 
+    batched = None if a.no_batched else bench_batched(ctx, events)
+
     cpu = None
     if not a.no_cpu_baseline:
         lphi, _ = live_32.download()
@@ -217,6 +270,7 @@ def main(argv=None):
         "adjoint_fp32": {"value": nvox * a.steps / wall32, "kernel_ms": gpu_ms32 / a.steps,
                          "hbm_gbs_at_32B": 32 * nvox / (gpu_ms32 / a.steps * 1e-3) / 1e9},
         "speedup_vs_plain_fp32": value / plain_value,
+        "batched": batched,
         "active_voxels": int(st["active"]),
         "active_fraction": st["active"] / nvox,
         "setup_s": setup_s,

@@ -181,10 +181,36 @@ kf_status kf_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *
 kf_status kf_marching_cubes(const kf_volume *vol, double iso, uint64_t *n_tris, float *verts,
                             uint64_t cap_tris);
 
-/* ---- batched deform (independent volumes of uneven size, one launch per iteration) ---- */
+/* ---- batched deform (independent volumes of uneven size) ----
+ * No reference counterpart: the reference deforms one live SDF per call
+ * (KillingFusion::computeDisplacementField, KillingFusion.cpp:240-397); this runs
+ * kf_deform on count independent (live[b], can[b], u[b]) triplets with one set of params.
+ * With the adjoint method every iteration is one persistent launch over the tiles of all
+ * volumes, the iteration loop is a captured hipGraph (replayed while the batch keeps its
+ * shape), and the tile shape is the autotuner's choice for the batch's shape.  Results equal
+ * per-volume kf_deform bit for bit.  stats (may be NULL) receives one entry per volume;
+ * energies are evaluated only when stats is given.  The reference method runs volume by
+ * volume.  Returns KF_E_DIVERGED if any volume diverged (the others still complete). */
 kf_status kf_batch_deform(const kf_volume *const *live, const kf_volume *const *can,
                           kf_field *const *u, int32_t count, const kf_params *p,
                           kf_deform_stats *stats);
+
+/* Tile shape of the batched launch. */
+typedef struct kf_batch_config {
+    int32_t voxels_per_thread;  /* x voxels per lane, 64 apart (1, 2 or 4) */
+    int32_t rows_per_thread;    /* rows per lane, 4 apart (1 or 2) */
+    int32_t blocks;             /* grid of the persistent launch (== tiles: one block per tile) */
+    int32_t candidates;         /* shapes timed by the tuner */
+    double us_per_iteration;    /* the chosen shape's time per launch on this batch */
+} kf_batch_config;
+
+/* Autotuner of kf_batch_deform (no reference counterpart): times every candidate shape on
+ * this batch (launches that read u[b] and write only its scratch buffer, so the fields keep
+ * their values) and caches the fastest for batches of the same dims and dtype.  force != 0
+ * re-times a cached shape.  best may be NULL. */
+kf_status kf_batch_autotune(const kf_volume *const *live, const kf_volume *const *can,
+                            kf_field *const *u, int32_t count, const kf_params *p, int32_t force,
+                            kf_batch_config *best);
 
 /* ---- per-frame pipeline: KillingFusion::processNextFrame, KillingFusion.cpp:109-175 ---- */
 typedef struct kf_pipeline_desc {

@@ -5,5 +5,5 @@ package holds its Python mirror (`api`), the build script (`build`) and the synt
 workload generators (`scenes`).
 """
 from .api import (Context, DisplacementField, KillingFusion, KillingParams, SDF, SimpleMesh,  # noqa: F401
-                  batch_deform, compute_displacement_field, dims_from_bounds, energy_backward,
+                  batch_autotune, batch_deform, compute_displacement_field, dims_from_bounds, energy_backward,
                   energy_forward, energy_gradient_ref)

@@ -38,6 +38,11 @@ class DeformStats(C.Structure):
                 ("max_update", C.c_double), ("energy", C.c_double), ("active_voxels", C.c_uint64)]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("voxels_per_thread", C.c_int32), ("rows_per_thread", C.c_int32), ("blocks", C.c_int32),
+                ("candidates", C.c_int32), ("us_per_iteration", C.c_double)]
+
+
 class PipelineDesc(C.Structure):
     _fields_ = [("volume", VolumeDesc), ("params", Params), ("K", C.c_double * 9), ("C2W", C.c_double * 16),
                 ("min_depth", C.c_double), ("max_depth", C.c_double), ("width", C.c_int32),
@@ -90,6 +95,8 @@ SIGNATURES = {
     "kf_marching_cubes": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_uint64), _fp, C.c_uint64]),
     "kf_batch_deform": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.c_int32,
                                   C.POINTER(Params), C.POINTER(DeformStats)]),
+    "kf_batch_autotune": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.c_int32,
+                                    C.POINTER(Params), C.c_int32, C.POINTER(BatchConfig)]),
     "kf_pipeline_create": (C.c_int, [_vp, C.POINTER(PipelineDesc), C.POINTER(_vp)]),
     "kf_pipeline_destroy": (C.c_int, [_vp]),
     "kf_pipeline_process_frame": (C.c_int, [_vp, _fp, C.POINTER(FrameStats)]),

@@ -319,16 +319,38 @@ def energy_backward(src: SDF, dest: SDF, field: DisplacementField, params: Killi
           "kf_energy_backward")
 
 
-def batch_deform(lives, cans, fields_, params: KillingParams):
-    L = _lib.load()
+def _batch_args(lives, cans, fields_):
     n = len(lives)
-    a = (C.c_void_p * n)(*[v.handle.value for v in lives])
-    b = (C.c_void_p * n)(*[v.handle.value for v in cans])
-    c = (C.c_void_p * n)(*[f.handle.value for f in fields_])
-    st = (_lib.DeformStats * n)()
+    if not (len(cans) == n and len(fields_) == n):
+        raise ValueError("batch lists differ in length")
+    a = (C.c_void_p * max(n, 1))(*[v.handle.value for v in lives])
+    b = (C.c_void_p * max(n, 1))(*[v.handle.value for v in cans])
+    c = (C.c_void_p * max(n, 1))(*[f.handle.value for f in fields_])
+    return a, b, c, n
+
+
+def batch_deform(lives, cans, fields_, params: KillingParams, stats: bool = True):
+    """kf_batch_deform: compute_displacement_field on independent volumes in one batch.
+    Returns one stats dict per volume (None when stats=False, which also skips the final
+    energy evaluation)."""
+    L = _lib.load()
+    a, b, c, n = _batch_args(lives, cans, fields_)
+    st = (_lib.DeformStats * max(n, 1))() if stats else None
     p = params.c()
     check(L.kf_batch_deform(a, b, c, n, C.byref(p), st), "kf_batch_deform")
-    return [_stats(s) for s in st]
+    return [_stats(st[i]) for i in range(n)] if stats else None
+
+
+def batch_autotune(lives, cans, fields_, params: KillingParams, force: bool = False):
+    """kf_batch_autotune: times the batched launch's tile shapes on this batch; returns the
+    chosen shape."""
+    L = _lib.load()
+    a, b, c, n = _batch_args(lives, cans, fields_)
+    cfg = _lib.BatchConfig()
+    p = params.c()
+    check(L.kf_batch_autotune(a, b, c, n, C.byref(p), int(bool(force)), C.byref(cfg)), "kf_batch_autotune")
+    return {"voxels_per_thread": cfg.voxels_per_thread, "rows_per_thread": cfg.rows_per_thread,
+            "blocks": cfg.blocks, "candidates": cfg.candidates, "us_per_iteration": cfg.us_per_iteration}
 
 
 class KillingFusion:

@@ -58,7 +58,7 @@ bool valid_dims(const int32_t d[3])
     return (int64_t)d[0] * d[1] * d[2] <= (int64_t)1 << 33;
 }
 
-bool valid_params(const kf_params *p)
+bool valid_params_impl(const kf_params *p)
 {
     return p && p->voxel_size > 0.0 && p->max_iters >= 0 && p->delta_size > 0.0 &&
            (p->method == KF_METHOD_REFERENCE || p->method == KF_METHOD_ADJOINT);
@@ -118,6 +118,8 @@ kf_status read_state(kf_context *ctx)
 }  // namespace
 
 void kf_set_error(const std::string &msg) { g_last_error = msg; }
+
+bool kf_valid_params(const kf_params *p) { return valid_params_impl(p); }
 
 kf_status kf_ensure_partials(kf_context *ctx, size_t nblocks)
 {
@@ -268,6 +270,7 @@ kf_status kf_context_destroy(kf_context *ctx)
         (void)hipHostFree(ctx->state_host);
     if (ctx->partials)
         (void)hipFree(ctx->partials);
+    kf_batch_release(ctx);
     if (ctx->own_stream && ctx->stream)
         (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -545,7 +548,9 @@ kf_status kf_integrate_depth(kf_volume *v, const float *depth, int32_t width, in
     return st;
 }
 
-static kf_status check_triplet(const kf_volume *live, const kf_volume *can, const kf_field *u)
+}  // extern "C"
+
+kf_status kf_check_triplet(const kf_volume *live, const kf_volume *can, const kf_field *u)
 {
     KF_CHECK_ARG(live && can && u, "null argument");
     KF_CHECK_ARG(same_dims(live->dims, can->dims) && same_dims(live->dims, u->dims),
@@ -555,13 +560,15 @@ static kf_status check_triplet(const kf_volume *live, const kf_volume *can, cons
     return KF_OK;
 }
 
+extern "C" {
+
 kf_status kf_energy_gradient_ref(const kf_volume *live, const kf_volume *can, const kf_field *u, const kf_params *p,
                                  kf_field *grad_out)
 {
-    kf_status s = check_triplet(live, can, u);
+    kf_status s = kf_check_triplet(live, can, u);
     if (s != KF_OK)
         return s;
-    KF_CHECK_ARG(valid_params(p), "invalid params");
+    KF_CHECK_ARG(kf_valid_params(p), "invalid params");
     KF_CHECK_ARG(grad_out && grad_out != u && same_dims(grad_out->dims, u->dims), "grad_out must be a distinct field of the same dims");
     DeviceGuard dg(live->ctx->device);
     return kf_launch_ref_step(live, can, u->u(), grad_out->u(), p, KF_MODE_GRAD);
@@ -570,10 +577,10 @@ kf_status kf_energy_gradient_ref(const kf_volume *live, const kf_volume *can, co
 kf_status kf_energy_forward(const kf_volume *live, const kf_volume *can, const kf_field *u, const kf_params *p,
                             double *energy_out, float *energy_per_voxel)
 {
-    kf_status s = check_triplet(live, can, u);
+    kf_status s = kf_check_triplet(live, can, u);
     if (s != KF_OK)
         return s;
-    KF_CHECK_ARG(valid_params(p) && energy_out, "invalid params or null energy_out");
+    KF_CHECK_ARG(kf_valid_params(p) && energy_out, "invalid params or null energy_out");
     kf_context *ctx = live->ctx;
     DeviceGuard dg(ctx->device);
     float *ev = nullptr;
@@ -610,10 +617,10 @@ kf_status kf_energy_forward(const kf_volume *live, const kf_volume *can, const k
 kf_status kf_energy_backward(const kf_volume *live, const kf_volume *can, const kf_field *u, const kf_params *p,
                              kf_field *grad_out)
 {
-    kf_status s = check_triplet(live, can, u);
+    kf_status s = kf_check_triplet(live, can, u);
     if (s != KF_OK)
         return s;
-    KF_CHECK_ARG(valid_params(p), "invalid params");
+    KF_CHECK_ARG(kf_valid_params(p), "invalid params");
     KF_CHECK_ARG(grad_out && grad_out != u && same_dims(grad_out->dims, u->dims), "grad_out must be a distinct field of the same dims");
     DeviceGuard dg(live->ctx->device);
     const unsigned int *mask = nullptr;
@@ -632,10 +639,10 @@ kf_status kf_energy_backward(const kf_volume *live, const kf_volume *can, const 
 kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, const kf_params *p,
                     kf_deform_stats *stats)
 {
-    kf_status s = check_triplet(live, can, u);
+    kf_status s = kf_check_triplet(live, can, u);
     if (s != KF_OK)
         return s;
-    KF_CHECK_ARG(valid_params(p), "invalid params");
+    KF_CHECK_ARG(kf_valid_params(p), "invalid params");
     kf_context *ctx = live->ctx;
     DeviceGuard dg(ctx->device);
     s = kf_state_reset(ctx, p->stop_max_update);
@@ -752,21 +759,6 @@ kf_status kf_marching_cubes(const kf_volume *v, double iso, uint64_t *n_tris, fl
     KF_CHECK_ARG(v && n_tris, "null argument");
     DeviceGuard dg(v->ctx->device);
     return kf_mc_run(v, iso, n_tris, verts, cap_tris);
-}
-
-kf_status kf_batch_deform(const kf_volume *const *live, const kf_volume *const *can, kf_field *const *u,
-                          int32_t count, const kf_params *p, kf_deform_stats *stats)
-{
-    KF_CHECK_ARG(live && can && u && count >= 0, "null argument");
-    kf_status worst = KF_OK;
-    for (int32_t b = 0; b < count; b++) {
-        kf_status s = kf_deform(live[b], can[b], u[b], p, stats ? &stats[b] : nullptr);
-        if (s == KF_E_DIVERGED)
-            worst = s;
-        else if (s != KF_OK)
-            return s;
-    }
-    return worst;
 }
 
 }  // extern "C"

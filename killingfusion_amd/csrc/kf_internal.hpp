@@ -79,6 +79,7 @@ struct kf_context {
     BlockPartial *partials = nullptr;  // device, two sets (launch parity) of one slot per block
     size_t partials_cap = 0;           // slots per set
     int num_cus = 256;
+    struct kf_batch_cache *batch = nullptr;  // batched-deform workspace, tuned shapes, captured graph
 };
 
 struct kf_volume {
@@ -259,6 +260,9 @@ __device__ __host__ inline R kf_interp3(R v000, R v001, R v010, R v011, R v100, 
 // ---------------------------------------------------------------------------
 
 kf_status kf_ensure_partials(kf_context *ctx, size_t nblocks);
+bool kf_valid_params(const kf_params *p);
+kf_status kf_check_triplet(const kf_volume *live, const kf_volume *can, const kf_field *u);
+void kf_batch_release(kf_context *ctx);
 inline BlockPartial *kf_partials(kf_context *ctx, int parity) { return ctx->partials + (size_t)(parity & 1) * ctx->partials_cap; }
 kf_status kf_state_reset(kf_context *ctx, double stop);
 // Folds the nb block partials of the launch with this parity into the iteration state

@@ -212,11 +212,11 @@ def torus_pair(twist=0.02):
     return d1, d2
 
 
-def ellipsoid_pair(seed: int, index: int):
-    """Random ellipsoid (semi-axes U(0.08, 0.2) m) at (0, 0, 0.8); frame 2 deformed by three
-    low-frequency sinusoids of total amplitude <= 1.5 cm."""
+def ellipsoid_pair(seed: int, index: int, max_axis: float = 0.2):
+    """Random ellipsoid (semi-axes U(0.08, 0.2) m, clipped to max_axis) at (0, 0, 0.8); frame 2
+    deformed by three low-frequency sinusoids of total amplitude <= 1.5 cm."""
     u = uniform(seed, 32, offset=32 * index)
-    axes = 0.08 + 0.12 * u[0:3]
+    axes = np.minimum(0.08 + 0.12 * u[0:3], max_axis)
     terms = []
     amps = u[3:6]
     amps = 0.015 * amps / max(amps.sum(), 1e-9)
@@ -229,6 +229,28 @@ def ellipsoid_pair(seed: int, index: int):
     d1 = _march(ellipsoid_fn(c, axes), 0.3, 1.5, lipschitz=2.5)
     d2 = _march(ellipsoid_fn(c, axes, terms), 0.3, 1.5, lipschitz=3.0)
     return d1, d2, axes
+
+
+BATCH_VOXEL = 0.006  # one voxel size for the whole batch (one set of optimiser params)
+
+
+def batch_edges(count: int = 16, seed: int = 0x5EED, lo: int = 48, hi: int = 160):
+    """Cube edges drawn uniformly from lo..hi."""
+    u = uniform(seed ^ 0xB47C4, count)
+    return [int(lo + min(int(x * (hi - lo + 1)), hi - lo)) for x in u]
+
+
+def batch_scene(count: int = 16, seed: int = 0x5EED, edges=None):
+    """Independent volumes of uneven size: volume b is an n_b^3 grid at BATCH_VOXEL centred on
+    (0, 0, 0.8), holding ellipsoid_pair(seed, b) with its semi-axes clipped to 0.4 of the grid
+    extent so the object fits.  Returns [(n, min_corner, depth_canonical, depth_live)]."""
+    edges = batch_edges(count, seed) if edges is None else list(edges)
+    out = []
+    for b, n in enumerate(edges):
+        ext = n * BATCH_VOXEL
+        d1, d2, _ = ellipsoid_pair(seed, b, max_axis=0.4 * ext)
+        out.append((n, (-ext / 2, -ext / 2, 0.8 - ext / 2), d1, d2))
+    return out
 
 
 def hinge_frame(t: float, seed: int = 0x5EED, frame: int = 0, noise: bool = True):
