@@ -20,7 +20,7 @@
 //
 // Two launch shapes share the per-voxel arithmetic (adj_voxel):
 //  * k_adj_rows (rows whose length is a multiple of 4): a thread owns 4 consecutive x voxels
-//    of two rows, a wavefront 256 voxels of a row, a block 8 rows of one slice.  A Jacobi step
+//    of a row, a wavefront 256 voxels of a row, a block 4 rows of one slice.  A Jacobi step
 //    moves only active voxels: a voxel whose own u is zero samples phi_live[j] exactly, so the
 //    rows of u, phi_live and u_out stream as float4 / 8-byte accesses and only active voxels
 //    (and moved ones, to find out) run the stencil.  Energy and gradient passes, which need
@@ -35,7 +35,7 @@ namespace {
 
 constexpr int VEC = 4;                  // voxels per thread along x in k_adj_rows
 constexpr int ROW = 64 * VEC;           // voxels of a row segment per wavefront
-constexpr int RPT = 2;                  // rows per thread in k_adj_rows (y and y + 4)
+constexpr int RPT = 1;                  // rows per thread in k_adj_rows (4 rows per block)
 constexpr int ROWS = BLOCK / 64 * RPT;  // rows per block (a row tile)
 
 // ---------------------------------------------------------------------------

@@ -61,9 +61,9 @@ __device__ inline void batch_tile(const BatchVol<S> &V, unsigned int t, const Ad
             const int x = seg * SEGW + 64 * k + lane, y = yt * TROWS + w + r * WAVES;
             ok[r][k] = x < g.nx && y < g.ny;
             ii[r][k] = ok[r][k] ? g.idx32(x, y, z) : 0u;
-            cu[r][k][0] = ok[r][k] ? u_in[ii[r][k]] : 0.f;
-            cu[r][k][1] = ok[r][k] ? u_in[n32 + ii[r][k]] : 0.f;
-            cu[r][k][2] = ok[r][k] ? u_in[2u * n32 + ii[r][k]] : 0.f;
+            cu[r][k][0] = ok[r][k] ? kf_load_stream(u_in + ii[r][k]) : 0.f;
+            cu[r][k][1] = ok[r][k] ? kf_load_stream(u_in + n32 + ii[r][k]) : 0.f;
+            cu[r][k][2] = ok[r][k] ? kf_load_stream(u_in + 2u * n32 + ii[r][k]) : 0.f;
             live[r][k] = ok[r][k] ? S::load(V.L.p, (int64_t)ii[r][k], V.L.scale) : 0.f;
         }
     unsigned int need = 0u;

@@ -9,6 +9,19 @@ namespace {
 
 constexpr int BLOCK = 256;
 
+// Loads of u that stream through once per iteration carry the non-temporal hint: the lines
+// are evicted first, which keeps the caches for the buffer this iteration writes (the next
+// iteration reads it back) and for the SDFs.  Measured at 256^3: a step with these hints
+// runs in ~75% of the time of the same step with plain loads; the hint on the stores or on
+// the SDF loads made it slower.
+typedef float kf_f4v __attribute__((ext_vector_type(4)));
+__device__ inline float4 kf_load_stream4(const float *p)
+{
+    const kf_f4v v = __builtin_nontemporal_load(reinterpret_cast<const kf_f4v *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline float kf_load_stream(const float *p) { return __builtin_nontemporal_load(p); }
+
 struct AdjConsts {
     float inv_2v, inv_v, hi, lo, eps2, alpha, omega_k, gamma, omega_s;
     int use_data, use_ls, use_k;
@@ -210,7 +223,7 @@ __device__ inline unsigned int nvox32(const GridDims &g) { return (unsigned int)
 __device__ inline float4 row4(const float *__restrict__ u, const GridDims &g, int c, int x0, int y, int z)
 {
     if (x0 < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz)
-        return *reinterpret_cast<const float4 *>(u + ((unsigned int)c * nvox32(g) + g.idx32(x0, y, z)));
+        return kf_load_stream4(u + ((unsigned int)c * nvox32(g) + g.idx32(x0, y, z)));
     return make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
