@@ -157,6 +157,39 @@ def bench_batched(ctx, events, iters=50, reps=8, count=16):
             "tile_shape": cfg, "phi_storage": "bf16"}
 
 
+def bench_sequence(ctx, frames=100, mesh_every=10):
+    """The full per-frame loop: `frames` noisy depth frames of a capsule hinge sweeping 0 to 45
+    degrees, a 192^3 BF16 canonical SDF; per frame integrate, deform (adjoint, 100 iterations),
+    warp and fuse (kf_pipeline_process_frame, the processNextFrame counterpart), and a
+    canonical marching-cubes mesh every `mesh_every` frames.  Depth frames start in host
+    memory, as they come from a camera: their upload is inside the timed loop."""
+    from killingfusion_amd import KillingFusion, KillingParams, scenes
+    depths = scenes.hinge_sequence(frames)
+    n, vox = scenes.SEQ_GRID, scenes.SEQ_VOXEL
+    kp = KillingParams.paper(vox, 3.0, method="adjoint", max_iters=100)
+    pl = KillingFusion(ctx, (n, n, n), scenes.SEQ_MIN, vox, kp, scenes.K, dtype="bf16")
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ms = {"integrate": 0.0, "deform": 0.0, "fuse": 0.0}
+    tris = []
+    for i, d in enumerate(depths):
+        st = pl.processNextFrame(d)
+        if i:
+            ms["integrate"] += st["ms_integrate"]
+            ms["deform"] += st["ms_deform"]
+            ms["fuse"] += st["ms_fuse"]
+        if (i + 1) % mesh_every == 0:
+            tris.append(int(pl.canonical_mesh().tris.shape[0]))
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    pl.close()
+    k = max(frames - 1, 1)
+    return {"frames": frames, "grid": n, "voxel_m": vox, "phi_storage": "bf16", "iterations_per_frame": 100,
+            "value": frames / wall, "unit": "frames/s", "wall_s": wall,
+            "ms_per_frame": {key: v / k for key, v in ms.items()},
+            "mesh_every": mesh_every, "mesh_triangles": tris}
+
+
 def calibrate_copy(launches=5):
     """A kernel of known HBM traffic for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters:
     kf_field_add on two 256^3 fields reads 2 x 3N floats and writes 3N floats per launch
@@ -185,6 +218,7 @@ def main(argv=None):
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-volume measurement")
+    ap.add_argument("--no-sequence", action="store_true", help="skip the 100-frame sequence measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the known-traffic copy kernel used to scale rocprofv3 PMC counters")
     a = ap.parse_args(argv)
@@ -232,6 +266,7 @@ def main(argv=None):
     plain_kernel_ms = gpu_ms_p / plain_steps
 
     batched = None if a.no_batched else bench_batched(ctx, events)
+    sequence = None if a.no_sequence else bench_sequence(ctx)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -271,6 +306,7 @@ def main(argv=None):
                          "hbm_gbs_at_32B": 32 * nvox / (gpu_ms32 / a.steps * 1e-3) / 1e9},
         "speedup_vs_plain_fp32": value / plain_value,
         "batched": batched,
+        "sequence": sequence,
         "active_voxels": int(st["active"]),
         "active_fraction": st["active"] / nvox,
         "setup_s": setup_s,

@@ -270,6 +270,71 @@ def hinge_frame(t: float, seed: int = 0x5EED, frame: int = 0, noise: bool = True
     return d
 
 
+def _ray_capsule(dn, a, b, r):
+    """Distance along unit rays dn (from the camera centre) to capsule (segment a-b, radius
+    r), inf where missed: the cylinder body by the quadratic in the ray parameter, then the
+    end-cap spheres where the body hit falls outside the segment."""
+    ba = b - a
+    oa = -a
+    baba = float(ba @ ba)
+    bard = dn @ ba
+    baoa = float(ba @ oa)
+    rdoa = dn @ oa
+    oaoa = float(oa @ oa)
+    qa = baba - bard * bard
+    qb = baba * rdoa - baoa * bard
+    qc = baba * oaoa - baoa * baoa - r * r * baba
+    h = qb * qb - qa * qc
+    t = np.full(dn.shape[0], np.inf)
+    ok = (h >= 0.0) & (qa > 1e-12)
+    tb = np.where(ok, (-qb - np.sqrt(np.maximum(h, 0.0))) / np.where(qa > 1e-12, qa, 1.0), np.inf)
+    y = baoa + tb * bard
+    body = ok & (y > 0.0) & (y < baba) & (tb > 0.0)
+    t[body] = tb[body]
+    # caps: sphere at a (y <= 0) or b (y >= baba)
+    for centre in (a, b):
+        oc = -centre
+        b2 = dn @ oc
+        c2 = float(oc @ oc) - r * r
+        h2 = b2 * b2 - c2
+        ts = np.where(h2 >= 0.0, -b2 - np.sqrt(np.maximum(h2, 0.0)), np.inf)
+        ts = np.where(ts > 0.0, ts, np.inf)
+        t = np.where(~body, np.minimum(t, ts), t)
+    return t
+
+
+def hinge_frame_exact(t: float, seed: int = 0x5EED, frame: int = 0, noise: bool = True):
+    """hinge_frame rendered in closed form (ray-capsule intersection instead of sphere
+    tracing): the same scene and noise, a few milliseconds per frame."""
+    r, L = 0.05, 0.25
+    h = np.array([0.0, 0.0, 0.8])
+    a1 = h + np.array([-L, 0.0, 0.0])
+    ang = math.radians(45.0 * t)
+    b2 = h + L * np.array([math.cos(ang), 0.0, -math.sin(ang)])
+    dx, dy = _rays()
+    d = np.stack([dx, dy, np.ones_like(dx)], axis=1)
+    norm = np.linalg.norm(d, axis=1)
+    dn = d / norm[:, None]
+    dist = np.minimum(_ray_capsule(dn, a1, h, r), _ray_capsule(dn, h, b2, r))
+    z = dist / norm  # camera z of the hit (the ray has unit z per unit of d)
+    hit = np.isfinite(z) & (z > 0.3) & (z < 1.5)
+    depth = np.where(hit, z, 0.0).reshape(HEIGHT, WIDTH)
+    if noise:
+        nz = normal(seed, depth.size, offset=frame * depth.size).reshape(depth.shape)
+        depth = np.where(depth > 0, depth + nz * 0.001 * depth * depth, 0.0)
+    return depth.astype(np.float32)
+
+
+SEQ_GRID, SEQ_VOXEL = 192, 0.0035
+SEQ_MIN = (-0.336, -0.336, 0.414)  # 192 x 3.5 mm = 0.672 m cube around the hinge at z = 0.8
+
+
+def hinge_sequence(frames: int = 100, seed: int = 0x5EED):
+    """The full-sequence workload: `frames` noisy depth frames of the capsule hinge sweeping
+    0 to 45 degrees."""
+    return [hinge_frame_exact(i / max(frames - 1, 1), seed, i) for i in range(frames)]
+
+
 def seeded_field(dims, amplitude=0.5, seed=0x5EED):
     """SoA float32 [3, N] displacements uniform in [-amplitude, amplitude) voxels."""
     n = int(dims[0]) * int(dims[1]) * int(dims[2])

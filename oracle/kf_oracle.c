@@ -1053,6 +1053,41 @@ void kfo_adj_gradient(const double *live, const double *can, const double *u, co
             }
 }
 
+/* kfo_adj_gradient restricted to what a Jacobi step uses: the mask everywhere and the
+ * gradient at active voxels (zero elsewhere).  A voxel with u_j = 0 samples the live SDF at
+ * the node j itself (zero interpolation weights), so psi_j = live[j] exactly and an
+ * out-of-band live[j] makes it inactive without the sample.  Returns the active count. */
+static int64_t adj_gradient_active(const double *live, const double *can, const double *u, const kfo_grid *g,
+                                   const kfo_params *p, double *grad, uint8_t *mask)
+{
+    int64_t n = NVOX(g);
+    int nz = g->n[2];
+    int64_t *za = (int64_t *)calloc((size_t)nz, sizeof(int64_t));
+#pragma omp parallel for schedule(dynamic)
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < g->n[1]; y++)
+            for (int x = 0; x < g->n[0]; x++) {
+                int64_t i = IDX(g, x, y, z);
+                double el, gr[3] = {0, 0, 0}, kg[3] = {0, 0, 0};
+                int act;
+                if (u[i] == 0.0 && u[n + i] == 0.0 && u[2 * n + i] == 0.0 && !jacobi_active(live[i], p))
+                    act = 0;
+                else
+                    act = adj_local(live, can, u, g, p, x, y, z, &el, gr);
+                if (act && p->use_killing)
+                    adj_killing_grad_at(u, g, p->gamma, x, y, z, kg);
+                for (int a = 0; a < 3; a++)
+                    grad[(int64_t)a * n + i] = act ? gr[a] + p->omega_k * kg[a] : 0.0;
+                mask[i] = (uint8_t)act;
+                za[z] += act;
+            }
+    int64_t act = 0;
+    for (int z = 0; z < nz; z++)
+        act += za[z];
+    free(za);
+    return act;
+}
+
 /* Jacobi gradient descent on E: active voxels move by -alpha dE/du. */
 int kfo_adj_deform(const double *live, const double *can, double *u, const kfo_grid *g,
                    const kfo_params *p, kfo_stats *st)
@@ -1066,10 +1101,7 @@ int kfo_adj_deform(const double *live, const double *can, double *u, const kfo_g
     st->energy = 0.0;
     st->active = 0;
     for (int it = 0; it < p->max_iters; it++) {
-        int64_t act = 0;
-        kfo_adj_gradient(live, can, u, g, p, grad, mask);
-        for (int64_t i = 0; i < n; i++)
-            act += mask[i];
+        const int64_t act = adj_gradient_active(live, can, u, g, p, grad, mask);
         double m = 0.0;
         for (int64_t i = 0; i < n; i++) {
             if (!mask[i])

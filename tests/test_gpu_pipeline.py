@@ -1,0 +1,59 @@
+"""The per-frame driver (kf_pipeline_process_frame, KillingFusion::processNextFrame,
+KillingFusion.cpp:109-175) against the same sequence of oracle stages: integrate into a
+fresh live SDF, deform onto the canonical SDF from a zero field, warp, fuse."""
+import numpy as np
+import pytest
+
+import oracle as O
+from killingfusion_amd import KillingFusion, KillingParams, scenes
+
+from helpers import C1, ogrid, oparams, sphere_frames
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_sequence(frames, g, kp, adjoint):
+    trunc = C1["trunc"]
+    can_phi, can_w = O.sdf_init(g)
+    iters = []
+    for i, d in enumerate(frames):
+        if i == 0:
+            O.integrate(can_phi, can_w, g, trunc, d, scenes.K, scenes.C2W, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+            iters.append(0)
+            continue
+        lphi, lw = O.sdf_init(g)
+        O.integrate(lphi, lw, g, trunc, d, scenes.K, scenes.C2W, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+        fn = O.adj_deform if adjoint else O.deform_jacobi
+        u, so = fn(lphi, can_phi, np.zeros((3, g.n)), g, oparams(kp))
+        iters.append(so["iters"])
+        O.warp(lphi, lw, u, g)
+        O.fuse(can_phi, can_w, lphi, lw)
+    return can_phi, can_w, iters
+
+
+@pytest.mark.parametrize("method", ["reference", "adjoint"])
+def test_pipeline_matches_oracle_stages(ctx, method):
+    print(f"case: pipeline, 3 frames of the 64^3 sphere pair, {method} method", flush=True)
+    d1, d2 = sphere_frames()
+    frames = [d1, d2, d1]
+    if method == "reference":  # config.cpp parameters: FP32 deform parity 1e-4 voxel
+        kp = KillingParams(voxel_size=C1["voxel"], max_surface=C1["trunc"], clip_distance=C1["trunc"], max_iters=16)
+    else:
+        kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint", max_iters=32)
+    pl = KillingFusion(ctx, C1["dims"], C1["min3d"], C1["voxel"], kp, scenes.K)
+    g = ogrid(C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"])
+    stats = [pl.processNextFrame(d) for d in frames]
+    gphi, gw = pl.canonical_download()
+    ophi, ow, oiters = oracle_sequence(frames, g, kp, method == "adjoint")
+    assert [s["deform"]["iters"] for s in stats[1:]] == oiters[1:]
+    touched = ow > 0
+    wdiff = gw != ow
+    same = ~wdiff & touched
+    e = np.abs(gphi.astype(np.float64) - ophi)
+    print(f"touched {touched.sum()} weight mismatches {wdiff.sum()} "
+          f"phi err (equal weights): max {e[same].max():.3e} p99.9 {np.quantile(e[same], 0.999):.3e}", flush=True)
+    assert touched.sum() > 10000
+    assert wdiff.sum() <= 1e-3 * touched.sum()
+    tol = 1e-5 if method == "reference" else 5e-5  # metres
+    assert e[same].max() <= tol
+    assert all(s["ms_total"] > 0 for s in stats)

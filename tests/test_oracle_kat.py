@@ -486,3 +486,26 @@ def test_adjoint_killing_equals_reference_without_clamp():
         i = z * n * n + y * n + x
         kref = O.killing_gradient2(u, g, x, y, z, h=0.05, gamma=0.0, clamp=1e9)
         assert np.allclose(kref, grad[:, i] / (2 * 0.05), rtol=1e-9, atol=1e-12)
+
+
+def test_adjoint_deform_steps_equal_dense_gradient_steps():
+    # kfo_adj_deform evaluates the gradient only where a step uses it (active voxels; a voxel
+    # with u_j = 0 and an out-of-band live[j] is inactive without sampling).  Its iterations
+    # must equal Jacobi steps taken with the dense kfo_adj_gradient: u <- u - alpha mask grad.
+    n = 12
+    rng = np.random.default_rng(5)
+    g = O.OGrid(dims=(n, n + 1, n - 1), voxel=0.01, oob=0.03 + 1e-5)
+    p = O.OParams(voxel_size=0.01, max_surface=0.03, clip=0.03, alpha=0.1, omega_k=0.5, gamma=0.1, omega_s=0.2,
+                  max_iters=3)
+    z, y, x = np.meshgrid(np.arange(n - 1), np.arange(n + 1), np.arange(n), indexing="ij")
+    live = (0.01 * (np.sqrt((x - 5.6) ** 2 + (y - 6.2) ** 2 + (z - 4.9) ** 2) - 3.5)).ravel()
+    live = np.where(np.abs(live) < 0.03, live, 0.03 + 1e-5)  # truncated band, untouched outside
+    can = live + rng.uniform(-0.002, 0.002, live.size)
+    for u0 in (np.zeros((3, g.n)), np.where(rng.random((3, g.n)) < 0.3, rng.uniform(-0.4, 0.4, (3, g.n)), 0.0)):
+        u = u0.copy()
+        for _ in range(3):
+            grad, mask = O.adj_gradient(live, can, u, g, p)
+            u = u - p.alpha * grad * mask
+        ud, st = O.adj_deform(live, can, u0, g, p)
+        assert st["iters"] == 3 and st["active"] == int(mask.sum()) > 50
+        assert np.array_equal(ud, u)
