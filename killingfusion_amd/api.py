@@ -13,7 +13,7 @@ from dataclasses import dataclass, fields
 
 import numpy as np
 
-from . import _lib
+from . import _lib, formats
 from ._lib import check
 
 _fp = C.POINTER(C.c_float)
@@ -120,6 +120,20 @@ class SimpleMesh:
         t = self.tris.shape[0]
         return np.arange(3 * t, dtype=np.uint32).reshape(t, 3)
 
+    def WriteMesh(self, filename) -> bool:
+        """SimpleMesh::WriteMesh (SimpleMesh.h:113-141): OFF text."""
+        try:
+            formats.write_off(filename, self.tris)
+        except OSError:
+            return False
+        return True
+
+    @classmethod
+    def from_off(cls, filename) -> "SimpleMesh":
+        """SimpleMesh(std::string filepath) (SimpleMesh.h:25-58)."""
+        v, f = formats.read_off(filename)
+        return cls(v[f].astype(np.float32))
+
 
 class SDF:
     """A TSDF volume on the GPU (SDF.h:16-200)."""
@@ -157,6 +171,11 @@ class SDF:
     @property
     def n(self):
         return self.dims[0] * self.dims[1] * self.dims[2]
+
+    def dumpToBinFile(self, path, truncationDistanceInVoxelSizeUnit: float, minimumWeightThreshold: float = 0.0):
+        """SDF::dumpToBinFile (SDF.cpp:432-460); the weight threshold is unused there too."""
+        phi, _ = self.download()
+        formats.write_sdf_bin(path, self.dims, self.min3d, self.voxel_size, truncationDistanceInVoxelSizeUnit, phi)
 
     def getGridSize(self):
         return self.dims
@@ -242,6 +261,10 @@ class DisplacementField:
     @property
     def n(self):
         return self.dims[0] * self.dims[1] * self.dims[2]
+
+    def dumpToBinFile(self, path, voxel_size: float = 0.0):
+        """DisplacementField::dumpToBinFile (DisplacementField.cpp:349-366)."""
+        formats.write_displacement_bin(path, self.dims, voxel_size, self.download())
 
     def initializeAllVoxels(self, value):
         v = np.array(value, dtype=np.float64)

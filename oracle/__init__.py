@@ -94,6 +94,7 @@ def lib():
             "kfo_mc_case": (C.c_char_p, [C.c_int]),
             "kfo_adj_energy": (C.c_double, [_dp, _dp, _dp, g, p, _dp, _i64p]),
             "kfo_adj_gradient": (None, [_dp, _dp, _dp, g, p, _dp, _u8p]),
+            "kfo_adj_energy_local": (C.c_double, [_dp, _dp, _dp, g, p, C.c_int, C.c_int, C.c_int]),
             "kfo_adj_deform": (C.c_int, [_dp, _dp, _dp, g, p, s]),
             "kfo_num_threads": (C.c_int, []),
         }
@@ -335,6 +336,12 @@ def adj_energy(live, can, u, g: OGrid, p: OParams, per_voxel=False):
     e = lib().kfo_adj_energy(_d(live), _d(can), _d(u), C.byref(g.c()), C.byref(p.c()),
                              _d(ev) if per_voxel else None, C.byref(act))
     return (e, ev, act.value) if per_voxel else e
+
+
+def adj_energy_local(live, can, u, g: OGrid, p: OParams, x: int, y: int, z: int):
+    """The part of the adjoint energy that depends on u at (x, y, z) (arrays must already be
+    contiguous float64)."""
+    return lib().kfo_adj_energy_local(_d(live), _d(can), _d(u), C.byref(g.c()), C.byref(p.c()), x, y, z)
 
 
 def adj_gradient(live, can, u, g: OGrid, p: OParams):

@@ -1033,6 +1033,24 @@ double kfo_adj_energy(const double *live, const double *can, const double *u, co
     return tot;
 }
 
+double kfo_adj_energy_local(const double *live, const double *can, const double *u, const kfo_grid *g,
+                            const kfo_params *p, int x, int y, int z)
+{
+    const int off[4][3] = {{0, 0, 0}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+    double e = 0.0;
+    for (int k = 0; k < 4; k++) {
+        int q[3] = {x + off[k][0], y + off[k][1], z + off[k][2]};
+        if (q[0] < 0 || q[1] < 0 || q[2] < 0)
+            continue;
+        double el, gr[3];
+        adj_local(live, can, u, g, p, q[0], q[1], q[2], &el, gr);
+        e += el;
+        if (p->use_killing)
+            e += p->omega_k * adj_killing_energy_at(u, g, p->gamma, q[0], q[1], q[2]);
+    }
+    return e;
+}
+
 void kfo_adj_gradient(const double *live, const double *can, const double *u, const kfo_grid *g,
                       const kfo_params *p, double *grad, uint8_t *mask)
 {

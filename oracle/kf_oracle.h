@@ -147,6 +147,12 @@ const char *kfo_mc_case(int cube_index);
 /* ---- adjoint formulation (no reference counterpart) ---- */
 double kfo_adj_energy(const double *live, const double *can, const double *u, const kfo_grid *g,
                       const kfo_params *p, double *e_voxel, int64_t *active);
+/* The terms of kfo_adj_energy that depend on u at voxel (x, y, z): the per-voxel energies of
+ * (x, y, z) and of its lower neighbours (x-1, y, z), (x, y-1, z), (x, y, z-1) (forward
+ * differences reach (x, y, z) only from those).  Differences of it in u_(x,y,z) equal
+ * differences of the total energy. */
+double kfo_adj_energy_local(const double *live, const double *can, const double *u, const kfo_grid *g,
+                            const kfo_params *p, int x, int y, int z);
 void kfo_adj_gradient(const double *live, const double *can, const double *u, const kfo_grid *g,
                       const kfo_params *p, double *grad, uint8_t *mask);
 int kfo_adj_deform(const double *live, const double *can, double *u, const kfo_grid *g,

@@ -509,3 +509,26 @@ def test_adjoint_deform_steps_equal_dense_gradient_steps():
         ud, st = O.adj_deform(live, can, u0, g, p)
         assert st["iters"] == 3 and st["active"] == int(mask.sum()) > 50
         assert np.array_equal(ud, u)
+
+
+def test_adjoint_local_energy_carries_every_term_of_a_voxel():
+    # kfo_adj_energy_local(x, y, z) must change exactly as the total energy does when u at
+    # (x, y, z) moves (including the zero-padding ghost terms on low faces).
+    n = 9
+    rng = np.random.default_rng(21)
+    g = O.OGrid(dims=(n, n, n), voxel=0.01, oob=0.03 + 1e-5)
+    p = O.OParams(voxel_size=0.01, max_surface=0.03, clip=0.03, alpha=0.1, omega_k=0.5, gamma=0.1, omega_s=0.2)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    live = (0.01 * (np.sqrt((x - 4.1) ** 2 + (y - 4.6) ** 2 + (z - 3.9) ** 2) - 2.5)).ravel()
+    can = live + rng.uniform(-0.002, 0.002, n ** 3)
+    u = rng.uniform(-0.4, 0.4, (3, n ** 3))
+    picks = [(0, 0, 0), (0, 4, 4), (8, 8, 8), (4, 0, 7)] + [tuple(rng.integers(0, n, 3)) for _ in range(20)]
+    for (px, py, pz) in picks:
+        i = (pz * n + py) * n + px
+        for a in range(3):
+            up, um = u.copy(), u.copy()
+            up[a, i] += 0.05
+            um[a, i] -= 0.03
+            dl = O.adj_energy_local(live, can, up, g, p, px, py, pz) - O.adj_energy_local(live, can, um, g, p, px, py, pz)
+            dt = O.adj_energy(live, can, up, g, p) - O.adj_energy(live, can, um, g, p)
+            assert dl == pytest.approx(dt, rel=1e-9, abs=1e-9)
