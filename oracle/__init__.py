@@ -97,6 +97,7 @@ def lib():
             "kfo_adj_energy_local": (C.c_double, [_dp, _dp, _dp, g, p, C.c_int, C.c_int, C.c_int]),
             "kfo_adj_deform": (C.c_int, [_dp, _dp, _dp, g, p, s]),
             "kfo_num_threads": (C.c_int, []),
+            "kfo_set_num_threads": (None, [C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -363,3 +364,7 @@ def adj_deform(live, can, u, g: OGrid, p: OParams):
 
 def num_threads() -> int:
     return lib().kfo_num_threads()
+
+
+def set_num_threads(n: int) -> None:
+    lib().kfo_set_num_threads(int(n))

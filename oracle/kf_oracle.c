@@ -1162,3 +1162,13 @@ int kfo_num_threads(void)
     return 1;
 #endif
 }
+
+void kfo_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0)
+        omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

@@ -159,6 +159,8 @@ int kfo_adj_deform(const double *live, const double *can, double *u, const kfo_g
                    const kfo_params *p, kfo_stats *st);
 
 int kfo_num_threads(void);
+/* Threads of the following parallel regions (the timing of the CPU baseline on 1 core). */
+void kfo_set_num_threads(int n);
 
 #ifdef __cplusplus
 }
