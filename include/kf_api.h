@@ -197,10 +197,10 @@ kf_status kf_batch_deform(const kf_volume *const *live, const kf_volume *const *
 
 /* Tile shape of the batched launch. */
 typedef struct kf_batch_config {
-    int32_t voxels_per_thread;  /* x voxels per lane, 64 apart (1, 2 or 4) */
-    int32_t rows_per_thread;    /* rows per lane, 4 apart (1 or 2) */
+    int32_t voxels_per_thread;  /* a tile is 256 x this consecutive voxels of one volume (1, 2, 4 or 8) */
     int32_t blocks;             /* grid of the persistent launch (== tiles: one block per tile) */
     int32_t candidates;         /* shapes timed by the tuner */
+    int32_t reserved;
     double us_per_iteration;    /* the chosen shape's time per launch on this batch */
 } kf_batch_config;
 

@@ -39,8 +39,8 @@ class DeformStats(C.Structure):
 
 
 class BatchConfig(C.Structure):
-    _fields_ = [("voxels_per_thread", C.c_int32), ("rows_per_thread", C.c_int32), ("blocks", C.c_int32),
-                ("candidates", C.c_int32), ("us_per_iteration", C.c_double)]
+    _fields_ = [("voxels_per_thread", C.c_int32), ("blocks", C.c_int32), ("candidates", C.c_int32),
+                ("reserved", C.c_int32), ("us_per_iteration", C.c_double)]
 
 
 class PipelineDesc(C.Structure):

@@ -372,8 +372,8 @@ def batch_autotune(lives, cans, fields_, params: KillingParams, force: bool = Fa
     cfg = _lib.BatchConfig()
     p = params.c()
     check(L.kf_batch_autotune(a, b, c, n, C.byref(p), int(bool(force)), C.byref(cfg)), "kf_batch_autotune")
-    return {"voxels_per_thread": cfg.voxels_per_thread, "rows_per_thread": cfg.rows_per_thread,
-            "blocks": cfg.blocks, "candidates": cfg.candidates, "us_per_iteration": cfg.us_per_iteration}
+    return {"voxels_per_thread": cfg.voxels_per_thread, "blocks": cfg.blocks, "candidates": cfg.candidates,
+            "us_per_iteration": cfg.us_per_iteration}
 
 
 class KillingFusion:

@@ -262,6 +262,75 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
         kf_block_publish<BLOCK>(partials, blockIdx.x, esum, mx, act);
 }
 
+// ---------------------------------------------------------------------------
+// Jacobi step over the flat voxel index: a thread owns 4 consecutive voxels (one float4 of each
+// u plane), a block 1024 consecutive voxels, whatever the row length -- a step's streaming part
+// is the same for every voxel, so rows that are not a multiple of the wavefront's width leave
+// no lanes idle.  (x, y, z) is recovered only for the compacted stencil jobs.
+// ---------------------------------------------------------------------------
+constexpr int FLAT = BLOCK * VEC;  // voxels per block
+
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
+                                                    float *__restrict__ u_out, AdjConsts c, DeformState *st,
+                                                    BlockPartial *partials, int parity, const BlockPartial *prev,
+                                                    unsigned int prev_nb)
+{
+    if (!step_gate<KF_MODE_STEP>(st, prev, prev_nb, parity))
+        return;
+    const GridDims g = L.g;
+    const unsigned int n32 = nvox32(g);
+    const unsigned int tile = kf_xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned int i0 = tile * (unsigned int)FLAT + threadIdx.x * (unsigned int)VEC;
+    const bool valid = i0 < n32;  // n is a multiple of 4, so the whole float4 is inside
+    float4 C[3];
+    float live4[VEC];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        C[a] = valid ? kf_load_stream4(u_in + ((unsigned int)a * n32 + i0)) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < VEC; k++)
+        live4[k] = 0.f;
+    if (valid)
+        S::load4(L.p, (int64_t)i0, L.scale, live4);
+    unsigned int need = 0u;
+    if (valid) {
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            *reinterpret_cast<float4 *>(u_out + ((unsigned int)a * n32 + i0)) = C[a];
+#pragma unroll
+        for (int k = 0; k < VEC; k++) {
+            const bool moved = ((__float_as_uint(f4(C[0], k)) | __float_as_uint(f4(C[1], k)) |
+                                 __float_as_uint(f4(C[2], k))) & 0x7fffffffu) != 0u;
+            const bool in_band = !(live4[k] > c.hi || live4[k] < c.lo);
+            if (moved || in_band)
+                need |= 1u << k;
+        }
+    }
+    // The stencil for active or moved voxels, compacted over the wavefront as in k_adj_rows.
+    unsigned long long bal[VEC];
+    int total = 0;
+#pragma unroll
+    for (int b = 0; b < VEC; b++) {
+        bal[b] = __ballot((need >> b) & 1u);
+        total += __popcll(bal[b]);
+    }
+    double mx = 0.0, esum = 0.0;
+    unsigned long long act = 0;
+    const unsigned int wave0 = tile * (unsigned int)FLAT + (threadIdx.x & ~63u) * (unsigned int)VEC;
+    const unsigned int nxy = (unsigned int)g.nx * (unsigned int)g.ny;
+    for (int j = threadIdx.x & 63; j < total; j += 64) {
+        int bit, src;
+        kf_job_decode<VEC>(bal, j, bit, src);
+        const unsigned int i = wave0 + (unsigned int)src * VEC + (unsigned int)bit;
+        const unsigned int z = i / nxy, r = i - z * nxy;
+        const unsigned int y = r / (unsigned int)g.nx, x = r - y * (unsigned int)g.nx;
+        adj_job<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, &st->div_slot[parity], nullptr, (int)x, (int)y, (int)z, mx,
+                                 esum, act);
+    }
+    kf_block_publish<BLOCK>(partials, blockIdx.x, esum, mx, act);
+}
+
 // Nonzero mask of field values u (see nbr_bits).
 __global__ __launch_bounds__(BLOCK) void k_adj_mask(const float *__restrict__ u, GridDims g, int nseg, int nyt,
                                                     unsigned int *__restrict__ mask)
@@ -289,7 +358,10 @@ void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float 
             const unsigned int *mask_in, int parity, const BlockPartial *prev, unsigned int prev_nb)
 {
     BlockPartial *parts = kf_partials(ctx, parity);
-    if (rows)
+    if (MODE == KF_MODE_STEP && rows)
+        hipLaunchKernelGGL((k_adj_flat<S>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c, ctx->state,
+                           parts, parity, prev, prev_nb);
+    else if (rows)
         hipLaunchKernelGGL((k_adj_rows<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
                            ctx->state, parts, e_voxel, nseg, nyt, mask_in, parity, prev, prev_nb);
     else
@@ -326,11 +398,17 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     // The row kernel needs rows of a multiple of 4 voxels (float4 loads, 16-byte aligned plane
     // starts), 3 N < 2^30 (32-bit offsets) and, for energy and gradient passes, the nonzero mask
     // of u_in.
-    const bool rows = kf_adj_mask_words(live->dims) > 0 && (mode == KF_MODE_STEP || mask_in != nullptr);
+    // A step runs the flat kernel when N is a multiple of 4 and 3 N < 2^30 (float4 plane
+    // accesses, 32-bit offsets); energy and gradient passes run the row kernel when rows are a
+    // multiple of 4 and the nonzero mask of u_in is given.  Everything else runs one voxel per
+    // thread.
+    const bool flat = mode == KF_MODE_STEP && live->n % VEC == 0 && 3 * live->n < ((int64_t)1 << 30);
+    const bool rows = flat || (mode != KF_MODE_STEP && kf_adj_mask_words(live->dims) > 0 && mask_in != nullptr);
     const int nseg = (live->dims[0] + ROW - 1) / ROW;
     const int nyt = (live->dims[1] + ROWS - 1) / ROWS;
-    const unsigned int nb = rows ? (unsigned int)(nseg * nyt * live->dims[2])
-                                 : (unsigned int)((live->n + BLOCK - 1) / BLOCK);
+    const unsigned int nb = flat   ? (unsigned int)((live->n + FLAT - 1) / FLAT)
+                            : rows ? (unsigned int)(nseg * nyt * live->dims[2])
+                                   : (unsigned int)((live->n + BLOCK - 1) / BLOCK);
     kf_status s = kf_ensure_partials(ctx, nb);
     if (s != KF_OK)
         return s;

@@ -7,8 +7,9 @@
 // (kf_api.hip) stops one volume; the per-voxel arithmetic is kf_adjoint.hpp's, so results
 // equal kf_deform's bit for bit.
 //
-// A tile is 4 RPT rows of one z slice times 64 VPT voxels of x; lane l owns x = 64 k + l
-// (k < VPT), so rows of any length stream coalesced without alignment.  The launch sequence
+// A tile is BLOCK VPT consecutive voxels of one volume's flat index; thread t owns voxels
+// tile0 + 256 k + t (k < VPT), so every load and store of a wavefront is 256 contiguous bytes and
+// rows of any length leave no lane idle.  The launch sequence
 // (state reset, K steps, last fold) is captured once into a hipGraph and replayed while the
 // batch keeps its shape; the tile shape and grid size are the fastest of the candidates
 // timed on the batch itself (kf_batch_autotune).
@@ -29,84 +30,73 @@ struct BatchVol {
     float *u[2];          // launch k reads u[k & 1] and writes u[(k + 1) & 1]
     unsigned int tile0;   // first tile of the volume in the batch's tile space
     unsigned int ntiles;
-    int nseg, nyt;        // x segments per row, row tiles per slice
 };
-
-constexpr int WAVES = BLOCK / 64;
 
 // One tile of a Jacobi step: stream u and phi_live, store every voxel's u_j, then run the
 // stencil for the voxels that are active or already moved, compacted over the wavefront.
-template <class S, int VPT, int RPT>
+template <class S, int VPT>
 __device__ inline void batch_tile(const BatchVol<S> &V, unsigned int t, const AdjConsts &c, long long *div,
                                   int parity, double &mx, unsigned long long &act)
 {
-    constexpr int SEGW = 64 * VPT;
-    constexpr int TROWS = WAVES * RPT;
     const GridDims g = V.L.g;
-    const int seg = (int)(t % (unsigned)V.nseg);
-    const int yt = (int)((t / (unsigned)V.nseg) % (unsigned)V.nyt);
-    const int z = (int)(t / ((unsigned)V.nseg * (unsigned)V.nyt));
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned int n32 = nvox32(g);
+    const unsigned int base = t * (unsigned int)(BLOCK * VPT);
     const float *__restrict__ u_in = V.u[parity];
     float *__restrict__ u_out = V.u[parity ^ 1];
-    const unsigned int n32 = nvox32(g);
 
-    float cu[RPT][VPT][3], live[RPT][VPT];
-    bool ok[RPT][VPT];
-    unsigned int ii[RPT][VPT];
+    float cu[VPT][3], live[VPT];
+    bool ok[VPT];
 #pragma unroll
-    for (int r = 0; r < RPT; r++)
-#pragma unroll
-        for (int k = 0; k < VPT; k++) {
-            const int x = seg * SEGW + 64 * k + lane, y = yt * TROWS + w + r * WAVES;
-            ok[r][k] = x < g.nx && y < g.ny;
-            ii[r][k] = ok[r][k] ? g.idx32(x, y, z) : 0u;
-            cu[r][k][0] = ok[r][k] ? kf_load_stream(u_in + ii[r][k]) : 0.f;
-            cu[r][k][1] = ok[r][k] ? kf_load_stream(u_in + n32 + ii[r][k]) : 0.f;
-            cu[r][k][2] = ok[r][k] ? kf_load_stream(u_in + 2u * n32 + ii[r][k]) : 0.f;
-            live[r][k] = ok[r][k] ? S::load(V.L.p, (int64_t)ii[r][k], V.L.scale) : 0.f;
-        }
+    for (int k = 0; k < VPT; k++) {
+        const unsigned int i = base + (unsigned int)k * BLOCK + threadIdx.x;
+        ok[k] = i < n32;
+        const unsigned int ii = ok[k] ? i : 0u;
+        cu[k][0] = ok[k] ? kf_load_stream(u_in + ii) : 0.f;
+        cu[k][1] = ok[k] ? kf_load_stream(u_in + n32 + ii) : 0.f;
+        cu[k][2] = ok[k] ? kf_load_stream(u_in + 2u * n32 + ii) : 0.f;
+        live[k] = ok[k] ? S::load(V.L.p, (int64_t)ii, V.L.scale) : 0.f;
+    }
     unsigned int need = 0u;
 #pragma unroll
-    for (int r = 0; r < RPT; r++)
-#pragma unroll
-        for (int k = 0; k < VPT; k++) {
-            if (!ok[r][k])
-                continue;
-            const unsigned int i = ii[r][k];
-            u_out[i] = cu[r][k][0];
-            u_out[n32 + i] = cu[r][k][1];
-            u_out[2u * n32 + i] = cu[r][k][2];
-            const bool moved = ((__float_as_uint(cu[r][k][0]) | __float_as_uint(cu[r][k][1]) |
-                                 __float_as_uint(cu[r][k][2])) & 0x7fffffffu) != 0u;
-            const bool in_band = !(live[r][k] > c.hi || live[r][k] < c.lo);
-            if (moved || in_band)
-                need |= 1u << (r * VPT + k);
-        }
-    // Phase 2 as in k_adj_rows: one job list per wavefront; a job's stores may hit elements
+    for (int k = 0; k < VPT; k++) {
+        if (!ok[k])
+            continue;
+        const unsigned int i = base + (unsigned int)k * BLOCK + threadIdx.x;
+        u_out[i] = cu[k][0];
+        u_out[n32 + i] = cu[k][1];
+        u_out[2u * n32 + i] = cu[k][2];
+        const bool moved = ((__float_as_uint(cu[k][0]) | __float_as_uint(cu[k][1]) | __float_as_uint(cu[k][2])) &
+                            0x7fffffffu) != 0u;
+        const bool in_band = !(live[k] > c.hi || live[k] < c.lo);
+        if (moved || in_band)
+            need |= 1u << k;
+    }
+    // Phase 2 as in k_adj_flat: one job list per wavefront; a job's stores may hit elements
     // another lane stored above, and a wavefront's vector memory operations are performed in
     // order.
-    unsigned long long bal[RPT * VPT];
+    unsigned long long bal[VPT];
     int total = 0;
 #pragma unroll
-    for (int b = 0; b < RPT * VPT; b++) {
+    for (int b = 0; b < VPT; b++) {
         bal[b] = __ballot((need >> b) & 1u);
         total += __popcll(bal[b]);
     }
     double esum = 0.0;
-    for (int j = lane; j < total; j += 64) {
+    const unsigned int wave0 = base + (threadIdx.x & ~63u);
+    const unsigned int nxy = (unsigned int)g.nx * (unsigned int)g.ny;
+    for (int j = threadIdx.x & 63; j < total; j += 64) {
         int bit, src;
-        kf_job_decode<RPT * VPT>(bal, j, bit, src);
-        const int x = seg * SEGW + 64 * (bit % VPT) + src;
-        const int y = yt * TROWS + w + (bit / VPT) * WAVES;
-        if (x < g.nx && y < g.ny)
-            adj_job<S, KF_MODE_STEP>(V.L, V.Cn, u_in, u_out, c, div, nullptr, x, y, z, mx, esum, act);
+        kf_job_decode<VPT>(bal, j, bit, src);
+        const unsigned int i = wave0 + (unsigned int)bit * BLOCK + (unsigned int)src;
+        const unsigned int z = i / nxy, r = i - z * nxy;
+        const unsigned int y = r / (unsigned int)g.nx, x = r - y * (unsigned int)g.nx;
+        adj_job<S, KF_MODE_STEP>(V.L, V.Cn, u_in, u_out, c, div, nullptr, (int)x, (int)y, (int)z, mx, esum, act);
     }
 }
 
 // Step launch `iter` of the batch.  parts holds two sets of cap slots (launch parity); the
 // tile t of a launch publishes into slot t of its set.
-template <class S, int VPT, int RPT>
+template <class S, int VPT>
 __global__ __launch_bounds__(BLOCK) void k_adj_batch(const BatchVol<S> *__restrict__ vols, int count,
                                                      unsigned int total, AdjConsts c, DeformState *states,
                                                      BlockPartial *parts, unsigned int cap, int iter)
@@ -138,7 +128,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_batch(const BatchVol<S> *__restri
             continue;
         double mx = 0.0;
         unsigned long long act = 0;
-        batch_tile<S, VPT, RPT>(vols[lo], t - vols[lo].tile0, c, &states[lo].div_slot[parity], parity, mx, act);
+        batch_tile<S, VPT>(vols[lo], t - vols[lo].tile0, c, &states[lo].div_slot[parity], parity, mx, act);
         kf_block_publish<BLOCK>(mine, t, 0.0, mx, act);
         __syncthreads();  // thread 0 reads the publish LDS before the next tile writes it
     }
@@ -172,14 +162,13 @@ __global__ void k_batch_reset(DeformState *states, int count, double stop)
 }
 
 struct BatchShape {
-    int vpt = 4, rpt = 2;
+    int vpt = 4;
     unsigned int blocks = 0;  // 0: one block per tile
     int candidates = 0;
     double us = 0.0;
 };
 
-constexpr int VPTS[3] = {1, 2, 4};
-constexpr int RPTS[2] = {1, 2};
+constexpr int VPTS[4] = {1, 2, 4, 8};
 constexpr int GRID_MULT[4] = {0, 2, 4, 8};  // blocks = mult x CUs (0: one block per tile)
 
 struct DevGuard {
@@ -299,10 +288,10 @@ std::string shape_key(const kf_volume *const *live, int count)
     return k;
 }
 
-// Table of the batch for tile shape (vpt, rpt); returns the number of tiles.
+// Table of the batch for tiles of BLOCK vpt voxels; returns the number of tiles.
 template <class S>
 size_t build_table(const kf_volume *const *live, const kf_volume *const *can, kf_field *const *u, int count,
-                   int vpt, int rpt, std::vector<BatchVol<S>> &tab)
+                   int vpt, std::vector<BatchVol<S>> &tab)
 {
     tab.resize((size_t)count);
     size_t t0 = 0;
@@ -312,10 +301,8 @@ size_t build_table(const kf_volume *const *live, const kf_volume *const *can, kf
         e.Cn = make_read<S>(can[v]);
         e.u[0] = u[v]->buf[u[v]->cur];
         e.u[1] = u[v]->buf[u[v]->cur ^ 1];
-        e.nseg = (live[v]->dims[0] + 64 * vpt - 1) / (64 * vpt);
-        e.nyt = (live[v]->dims[1] + WAVES * rpt - 1) / (WAVES * rpt);
         e.tile0 = (unsigned int)t0;
-        e.ntiles = (unsigned int)((size_t)e.nseg * (size_t)e.nyt * (size_t)live[v]->dims[2]);
+        e.ntiles = (unsigned int)(((size_t)live[v]->n + (size_t)BLOCK * vpt - 1) / ((size_t)BLOCK * vpt));
         t0 += e.ntiles;
     }
     return t0;
@@ -327,16 +314,14 @@ kf_status launch_step(kf_context *ctx, const BatchShape &sh, const BatchVol<S> *
 {
     const unsigned int grid = sh.blocks ? sh.blocks : total;
     hipStream_t st = ctx->stream;
-#define KF_BATCH_LAUNCH(V, R)                                                                                    \
-    hipLaunchKernelGGL((k_adj_batch<S, V, R>), dim3(grid), dim3(BLOCK), 0, st, dtab, count, total, c, states, \
-                       parts, cap, iter)
-    switch (sh.vpt * 10 + sh.rpt) {
-    case 11: KF_BATCH_LAUNCH(1, 1); break;
-    case 12: KF_BATCH_LAUNCH(1, 2); break;
-    case 21: KF_BATCH_LAUNCH(2, 1); break;
-    case 22: KF_BATCH_LAUNCH(2, 2); break;
-    case 41: KF_BATCH_LAUNCH(4, 1); break;
-    case 42: KF_BATCH_LAUNCH(4, 2); break;
+#define KF_BATCH_LAUNCH(V)                                                                                    \
+    hipLaunchKernelGGL((k_adj_batch<S, V>), dim3(grid), dim3(BLOCK), 0, st, dtab, count, total, c, states, parts, \
+                       cap, iter)
+    switch (sh.vpt) {
+    case 1: KF_BATCH_LAUNCH(1); break;
+    case 2: KF_BATCH_LAUNCH(2); break;
+    case 4: KF_BATCH_LAUNCH(4); break;
+    case 8: KF_BATCH_LAUNCH(8); break;
     default: kf_set_error("unsupported batch tile shape"); return KF_E_INVALID_ARG;
     }
 #undef KF_BATCH_LAUNCH
@@ -362,8 +347,7 @@ kf_status tune(kf_context *ctx, const kf_volume *const *live, const kf_volume *c
     std::vector<BatchVol<S>> tab;
     size_t max_tiles = 0;
     for (int vpt : VPTS)
-        for (int rpt : RPTS)
-            max_tiles = std::max(max_tiles, build_table<S>(live, can, u, count, vpt, rpt, tab));
+        max_tiles = std::max(max_tiles, build_table<S>(live, can, u, count, vpt, tab));
     kf_status s = ensure_workspace(ctx, sizeof(BatchVol<S>) * (size_t)count, count, max_tiles);
     if (s != KF_OK)
         return s;
@@ -379,52 +363,49 @@ kf_status tune(kf_context *ctx, const kf_volume *const *live, const kf_volume *c
     best = BatchShape{};
     best.us = 1e30;
     int cands = 0;
-    for (int vpt : VPTS)
-        for (int rpt : RPTS) {
-            const size_t total = build_table<S>(live, can, u, count, vpt, rpt, tab);
-            if (s == KF_OK && hipMemcpyAsync(b->table, tab.data(), sizeof(BatchVol<S>) * tab.size(),
-                                             hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
-                kf_set_error("batch table upload failed");
+    for (int vpt : VPTS) {
+        const size_t total = build_table<S>(live, can, u, count, vpt, tab);
+        if (s == KF_OK && hipMemcpyAsync(b->table, tab.data(), sizeof(BatchVol<S>) * tab.size(),
+                                         hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+            kf_set_error("batch table upload failed");
+            s = KF_E_HIP;
+        }
+        if (s == KF_OK)
+            s = reset_states(ctx, count, 0.0);
+        unsigned int last_grid = 0;
+        for (int mult : GRID_MULT) {
+            if (s != KF_OK)
+                break;
+            BatchShape sh;
+            sh.vpt = vpt;
+            sh.blocks = mult ? (unsigned int)std::min<size_t>((size_t)mult * ctx->num_cus, total) : 0u;
+            const unsigned int grid = sh.blocks ? sh.blocks : (unsigned int)total;
+            if (grid == last_grid || (mult && sh.blocks == total))
+                continue;
+            last_grid = grid;
+            const auto *dtab = (const BatchVol<S> *)b->table;
+            for (int r = 0; r < WARM && s == KF_OK; r++)
+                s = launch_step<S>(ctx, sh, dtab, count, (unsigned int)total, c, b->states, b->parts,
+                                   (unsigned int)b->parts_cap, 0);
+            if (s == KF_OK && hipEventRecord(e0, ctx->stream) != hipSuccess)
+                s = KF_E_HIP;
+            for (int r = 0; r < REPS && s == KF_OK; r++)
+                s = launch_step<S>(ctx, sh, dtab, count, (unsigned int)total, c, b->states, b->parts,
+                                   (unsigned int)b->parts_cap, 0);
+            float ms = 0.f;
+            if (s == KF_OK && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                               hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) {
+                kf_set_error("timing the batch candidates failed");
                 s = KF_E_HIP;
             }
-            if (s == KF_OK)
-                s = reset_states(ctx, count, 0.0);
-            unsigned int last_grid = 0;
-            for (int mult : GRID_MULT) {
-                if (s != KF_OK)
-                    break;
-                BatchShape sh;
-                sh.vpt = vpt;
-                sh.rpt = rpt;
-                sh.blocks = mult ? (unsigned int)std::min<size_t>((size_t)mult * ctx->num_cus, total) : 0u;
-                const unsigned int grid = sh.blocks ? sh.blocks : (unsigned int)total;
-                if (grid == last_grid || (mult && sh.blocks == total))
-                    continue;
-                last_grid = grid;
-                const auto *dtab = (const BatchVol<S> *)b->table;
-                for (int r = 0; r < WARM && s == KF_OK; r++)
-                    s = launch_step<S>(ctx, sh, dtab, count, (unsigned int)total, c, b->states, b->parts,
-                                       (unsigned int)b->parts_cap, 0);
-                if (s == KF_OK && hipEventRecord(e0, ctx->stream) != hipSuccess)
-                    s = KF_E_HIP;
-                for (int r = 0; r < REPS && s == KF_OK; r++)
-                    s = launch_step<S>(ctx, sh, dtab, count, (unsigned int)total, c, b->states, b->parts,
-                                       (unsigned int)b->parts_cap, 0);
-                float ms = 0.f;
-                if (s == KF_OK && (hipEventRecord(e1, ctx->stream) != hipSuccess ||
-                                   hipEventSynchronize(e1) != hipSuccess ||
-                                   hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) {
-                    kf_set_error("timing the batch candidates failed");
-                    s = KF_E_HIP;
-                }
-                if (s != KF_OK)
-                    break;
-                cands++;
-                sh.us = 1e3 * ms / REPS;
-                if (sh.us < best.us)
-                    best = sh;
-            }
+            if (s != KF_OK)
+                break;
+            cands++;
+            sh.us = 1e3 * ms / REPS;
+            if (sh.us < best.us)
+                best = sh;
         }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     best.candidates = cands;
@@ -465,7 +446,7 @@ kf_status run_batch(kf_context *ctx, const kf_volume *const *live, const kf_volu
     if (s != KF_OK)
         return s;
     std::vector<BatchVol<S>> tab;
-    const size_t total = build_table<S>(live, can, u, count, sh.vpt, sh.rpt, tab);
+    const size_t total = build_table<S>(live, can, u, count, sh.vpt, tab);
     s = ensure_workspace(ctx, sizeof(BatchVol<S>) * (size_t)count, count, total);
     if (s != KF_OK)
         return s;
@@ -477,7 +458,7 @@ kf_status run_batch(kf_context *ctx, const kf_volume *const *live, const kf_volu
     const AdjConsts c = kf_adj_consts(p);
     const int K = p->max_iters;
     char buf[256];
-    std::snprintf(buf, sizeof(buf), "%d/%d/%u/%d/%zu/%d/%d/%a/%p/%p/%p/%zu", (int)S::dtype, sh.vpt * 10 + sh.rpt,
+    std::snprintf(buf, sizeof(buf), "%d/%d/%u/%d/%zu/%d/%d/%a/%p/%p/%p/%zu", (int)S::dtype, sh.vpt,
                   sh.blocks, count, total, K, ctx->num_cus, p->stop_max_update, b->table, (void *)b->states,
                   (void *)b->parts, b->parts_cap);
     std::string key(buf);
@@ -621,9 +602,8 @@ kf_status kf_batch_autotune(const kf_volume *const *live, const kf_volume *const
     if (best) {
         std::memset(best, 0, sizeof(*best));
         best->voxels_per_thread = sh.vpt;
-        best->rows_per_thread = sh.rpt;
         std::vector<BatchVol<StoreF32>> tab;  // tile counts do not depend on the dtype
-        const size_t total = build_table<StoreF32>(live, can, u, count, sh.vpt, sh.rpt, tab);
+        const size_t total = build_table<StoreF32>(live, can, u, count, sh.vpt, tab);
         best->blocks = (int32_t)(sh.blocks ? sh.blocks : total);
         best->candidates = sh.candidates;
         best->us_per_iteration = sh.us;

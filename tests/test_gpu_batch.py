@@ -22,7 +22,8 @@ TRUNC = 3 * VOX
 
 @pytest.fixture(scope="module")
 def batch(ctx):
-    """Three cubes (48, 61, 77: ragged against every tile width) and one non-cubic volume."""
+    """Three cubes (48, 61, 77: odd and even sizes, none a multiple of a tile) and one non-cubic
+    volume."""
     print("case: batch of 4 ellipsoid volumes (48^3, 61^3, 77^3, 70x45x58), 6 mm", flush=True)
     vols = []
     for n, mn, d1, d2 in scenes.batch_scene(3, seed=0x5EED, edges=[48, 61, 77]):
@@ -141,8 +142,8 @@ def test_autotune_and_retuned_shape_give_same_fields(ctx, batch):
     fields = fresh_fields(ctx, batch)
     cfg = batch_autotune(lives, cans, fields, kp, force=True)
     print("tuned", cfg, flush=True)
-    assert cfg["candidates"] >= 6 and cfg["us_per_iteration"] > 0
-    assert cfg["voxels_per_thread"] in (1, 2, 4) and cfg["rows_per_thread"] in (1, 2) and cfg["blocks"] > 0
+    assert cfg["candidates"] >= 4 and cfg["us_per_iteration"] > 0
+    assert cfg["voxels_per_thread"] in (1, 2, 4, 8) and cfg["blocks"] > 0
     # tuning launches write only the fields' scratch buffers
     assert all(not f.download().any() for f in fields)
     batch_deform(lives, cans, fields, kp, stats=False)
