@@ -309,7 +309,7 @@ def main(argv=None):
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
-                     "kernel": "k_adj_rows<StoreBF16, STEP>", "bytes_per_voxel_iteration": bytes_per_vi,
+                     "kernel": "k_adj_flat<StoreBF16>", "bytes_per_voxel_iteration": bytes_per_vi,
                      "kernel_ms": kernel_ms},
         "cpu_baseline": cpu,
         "plain_fp32": {"value": plain_value, "ms_per_step": wall_p * 1e3 / plain_steps, "steps": plain_steps,
