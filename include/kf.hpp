@@ -132,6 +132,28 @@ private:
     std::vector<float> verts_;
 };
 
+/* Welded marching-cubes mesh (kf_marching_cubes_indexed): vertices, unit normals, and
+ * triangles as vertex indices in the soup's order. */
+struct IndexedMesh {
+    std::vector<float> vertices, normals;  // 3 per vertex
+    std::vector<uint32_t> triangles;       // 3 per triangle
+    size_t numVertices() const { return vertices.size() / 3; }
+    size_t numTriangles() const { return triangles.size() / 3; }
+    bool WriteMesh(const std::string &filename) const
+    {
+        std::ofstream out(filename);
+        if (!out.is_open())
+            return false;
+        out << "OFF" << std::endl << numVertices() << " " << numTriangles() << " 0" << std::endl;
+        for (size_t i = 0; i < numVertices(); i++)
+            out << (double)vertices[3 * i] << " " << (double)vertices[3 * i + 1] << " " << (double)vertices[3 * i + 2]
+                << std::endl;
+        for (size_t t = 0; t < numTriangles(); t++)
+            out << "3 " << triangles[3 * t] << " " << triangles[3 * t + 1] << " " << triangles[3 * t + 2] << std::endl;
+        return true;
+    }
+};
+
 class DisplacementField {
 public:
     /* DisplacementField::DisplacementField, DisplacementField.cpp:8-16: a zero field. */
@@ -255,6 +277,21 @@ public:
         m.vertices().resize((size_t)nt * 9);
         if (nt)
             check(kf_marching_cubes(h_, iso, &nt, m.vertices().data(), nt), "kf_marching_cubes");
+        return m;
+    }
+    /* The welded mesh of getMesh with per-vertex normals. */
+    IndexedMesh getMeshIndexed(double iso = 0.0) const
+    {
+        uint64_t nv = 0, nt = 0;
+        check(kf_marching_cubes_indexed(h_, iso, &nv, &nt, nullptr, nullptr, nullptr, 0, 0), "kf_marching_cubes_indexed");
+        IndexedMesh m;
+        m.vertices.resize(3 * (size_t)nv);
+        m.normals.resize(3 * (size_t)nv);
+        m.triangles.resize(3 * (size_t)nt);
+        if (nt)
+            check(kf_marching_cubes_indexed(h_, iso, &nv, &nt, m.vertices.data(), m.normals.data(), m.triangles.data(), nv,
+                                            nt),
+                  "kf_marching_cubes_indexed");
         return m;
     }
     void upload(const std::vector<float> &phi, const std::vector<int32_t> &w)

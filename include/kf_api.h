@@ -180,6 +180,15 @@ kf_status kf_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *
  * count; verts may be NULL (size query); KF_E_CAPACITY if cap_tris < count. */
 kf_status kf_marching_cubes(const kf_volume *vol, double iso, uint64_t *n_tris, float *verts,
                             uint64_t cap_tris);
+/* Welded (indexed) variant of kf_marching_cubes; no reference counterpart: the reference keeps
+ * the soup, and its viewer computes one normal per face (mainGui.cpp:68-89).  One vertex per cut
+ * grid edge that a visited cell uses, numbered in grid-edge order (3 float, y negated as the
+ * soup); triangles in the soup's order as 3 vertex indices; normals (may be NULL) are the unit
+ * SDF gradient at each vertex.  *n_verts / *n_tris always receive the counts; verts, normals and
+ * tris may all be NULL (size query); KF_E_CAPACITY if a capacity is short. */
+kf_status kf_marching_cubes_indexed(const kf_volume *vol, double iso, uint64_t *n_verts, uint64_t *n_tris,
+                                    float *verts, float *normals, uint32_t *tris, uint64_t cap_verts,
+                                    uint64_t cap_tris);
 
 /* ---- batched deform (independent volumes of uneven size) ----
  * No reference counterpart: the reference deforms one live SDF per call

@@ -4,6 +4,6 @@ The compute runs in libkf.so (HIP kernels behind the C ABI of include/kf_api.h);
 package holds its Python mirror (`api`), the build script (`build`) and the synthetic
 workload generators (`scenes`).
 """
-from .api import (Context, DisplacementField, KillingFusion, KillingParams, SDF, SimpleMesh,  # noqa: F401
+from .api import (Context, DisplacementField, IndexedMesh, KillingFusion, KillingParams, SDF, SimpleMesh,  # noqa: F401
                   batch_autotune, batch_deform, compute_displacement_field, dims_from_bounds, energy_backward,
                   energy_forward, energy_gradient_ref)

@@ -93,6 +93,8 @@ SIGNATURES = {
     "kf_fuse": (C.c_int, [_vp, _vp]),
     "kf_fuse_warped": (C.c_int, [_vp, _vp, _vp]),
     "kf_marching_cubes": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_uint64), _fp, C.c_uint64]),
+    "kf_marching_cubes_indexed": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _fp, _fp,
+                                            C.POINTER(C.c_uint32), C.c_uint64, C.c_uint64]),
     "kf_batch_deform": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.c_int32,
                                   C.POINTER(Params), C.POINTER(DeformStats)]),
     "kf_batch_autotune": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.c_int32,

@@ -135,6 +135,21 @@ class SimpleMesh:
         return cls(v[f].astype(np.float32))
 
 
+class IndexedMesh:
+    """Welded marching-cubes output (kf_marching_cubes_indexed): vertices (V, 3), unit normals
+    (V, 3) and triangles (T, 3) as vertex indices, in the soup's triangle order."""
+
+    def __init__(self, vertices: np.ndarray, normals: np.ndarray, triangles: np.ndarray):
+        self.vertices, self.normals, self.triangles = vertices, normals, triangles
+
+    def WriteMesh(self, filename) -> bool:
+        try:
+            formats.write_off_indexed(filename, self.vertices, self.triangles)
+        except OSError:
+            return False
+        return True
+
+
 class SDF:
     """A TSDF volume on the GPU (SDF.h:16-200)."""
 
@@ -232,6 +247,21 @@ class SDF:
             check(self._L.kf_marching_cubes(self.handle, iso, C.byref(n), out.ctypes.data_as(_fp), n.value),
                   "kf_marching_cubes")
         return SimpleMesh(out[: n.value])
+
+    def getMeshIndexed(self, iso: float = 0.0, normals: bool = True) -> IndexedMesh:
+        """The welded mesh of getMesh (one vertex per cut grid edge) with per-vertex normals."""
+        nv, nt = C.c_uint64(0), C.c_uint64(0)
+        check(self._L.kf_marching_cubes_indexed(self.handle, iso, C.byref(nv), C.byref(nt), None, None, None, 0, 0),
+              "kf_marching_cubes_indexed")
+        v = np.empty((max(nv.value, 1), 3), dtype=np.float32)
+        n = np.empty((max(nv.value, 1), 3), dtype=np.float32) if normals else None
+        t = np.empty((max(nt.value, 1), 3), dtype=np.uint32)
+        if nt.value:
+            check(self._L.kf_marching_cubes_indexed(
+                self.handle, iso, C.byref(nv), C.byref(nt), v.ctypes.data_as(_fp),
+                n.ctypes.data_as(_fp) if normals else None, t.ctypes.data_as(C.POINTER(C.c_uint32)), nv.value,
+                nt.value), "kf_marching_cubes_indexed")
+        return IndexedMesh(v[: nv.value], n[: nv.value] if normals else None, t[: nt.value])
 
     def close(self):
         if getattr(self, "handle", None):

@@ -761,4 +761,15 @@ kf_status kf_marching_cubes(const kf_volume *v, double iso, uint64_t *n_tris, fl
     return kf_mc_run(v, iso, n_tris, verts, cap_tris);
 }
 
+kf_status kf_marching_cubes_indexed(const kf_volume *v, double iso, uint64_t *n_verts, uint64_t *n_tris, float *verts,
+                                    float *normals, uint32_t *tris, uint64_t cap_verts, uint64_t cap_tris)
+{
+    KF_CHECK_ARG(v && n_verts && n_tris, "null argument");
+    KF_CHECK_ARG(3 * ((int64_t)v->dims[0] + 1) * ((int64_t)v->dims[1] + 1) * ((int64_t)v->dims[2] + 1) <
+                     ((int64_t)1 << 32),
+                 "indexed marching cubes numbers grid edges with 32 bits");
+    DeviceGuard dg(v->ctx->device);
+    return kf_mc_run_indexed(v, iso, n_verts, n_tris, verts, normals, tris, cap_verts, cap_tris);
+}
+
 }  // extern "C"

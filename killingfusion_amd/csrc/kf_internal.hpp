@@ -299,6 +299,8 @@ size_t kf_adj_mask_words(const int32_t dims[3]);
 kf_status kf_launch_adj_mask(const kf_field *f, const float *u, unsigned int *mask);
 
 kf_status kf_mc_run(const kf_volume *v, double iso, uint64_t *n_tris, float *verts_host, uint64_t cap);
+kf_status kf_mc_run_indexed(const kf_volume *v, double iso, uint64_t *n_verts, uint64_t *n_tris, float *verts_host,
+                            float *normals_host, uint32_t *tris_host, uint64_t cap_verts, uint64_t cap_tris);
 
 // Step-kernel modes.
 enum { KF_MODE_STEP = 0, KF_MODE_GRAD = 1, KF_MODE_ENERGY = 2 };

@@ -73,6 +73,17 @@ def write_off(path, tris):
         f.write("\n".join(lines) + "\n")
 
 
+def write_off_indexed(path, vertices, faces):
+    """OFF of an indexed mesh: vertices [V, 3], faces [F, 3]."""
+    v = np.asarray(vertices, dtype=np.float64).reshape(-1, 3)
+    f = np.asarray(faces, dtype=np.int64).reshape(-1, 3)
+    lines = ["OFF", f"{v.shape[0]} {f.shape[0]} 0"]
+    lines += ["%g %g %g" % tuple(x) for x in v]
+    lines += ["3 %d %d %d" % tuple(t) for t in f]
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
 def read_off(path):
     """Returns (vertices [V, 3] float64, faces [F, 3] int64)."""
     with open(path) as f:

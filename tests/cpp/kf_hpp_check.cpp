@@ -148,6 +148,20 @@ int main(int argc, char **argv)
         kf::SimpleMesh mesh = kfu.canonicalMesh();
         expect(mesh.triangles() > 100, "canonical mesh");
 
+        // Welded mesh of the same plane: the soup's triangles over shared vertices.
+        kf::IndexedMesh wm = plane.getMeshIndexed();
+        kf::SimpleMesh pm = plane.getMesh();
+        bool welded = wm.numTriangles() == pm.triangles() && wm.numVertices() < 3 * wm.numTriangles();
+        for (size_t k = 0; welded && k < wm.triangles.size(); k++) {
+            const uint32_t vi = wm.triangles[k];
+            welded = vi < wm.numVertices() &&
+                     std::fabs(wm.vertices[3 * vi] - pm.vertices()[3 * k]) <= 1e-6f &&
+                     std::fabs(wm.vertices[3 * vi + 1] - pm.vertices()[3 * k + 1]) <= 1e-6f &&
+                     std::fabs(wm.vertices[3 * vi + 2] - pm.vertices()[3 * k + 2]) <= 1e-6f;
+        }
+        expect(welded && pm.triangles() > 0, "welded mesh indexes the soup's vertices");
+        expect(wm.WriteMesh(out + "/plane_welded.off"), "welded WriteMesh");
+
         // 7. Formats: OFF round trip and the two .bin dumps.
         const std::string off = out + "/plane.off";
         expect(mesh.WriteMesh(off), "WriteMesh");
