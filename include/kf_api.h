@@ -190,6 +190,25 @@ kf_status kf_marching_cubes_indexed(const kf_volume *vol, double iso, uint64_t *
                                     float *verts, float *normals, uint32_t *tris, uint64_t cap_verts,
                                     uint64_t cap_tris);
 
+/* ---- rigid pre-registration (no reference counterpart: the reference leaves SDF-2-SDF
+ * registration as a future task, KillingFusion.cpp:76,133) ---- */
+typedef struct kf_rigid_stats {
+    int32_t iters;           /* Gauss-Newton updates applied */
+    int32_t reserved;
+    uint64_t voxels;         /* canonical band voxels whose sample fell inside the live grid */
+    double initial_energy;   /* 1/2 sum (phi_live(T x) - phi_can(x))^2 at the input pose (m^2) */
+    double energy;           /* the same at the returned pose */
+    double last_step;        /* |rotation| (rad) + |translation| / voxel of the last update */
+} kf_rigid_stats;
+/* Rigid pose T (row-major 4x4, canonical world -> live world) minimising the squared SDF
+ * difference over the canonical truncation band, by Gauss-Newton from the given pose (in/out).
+ * FP64 accumulation, reproducible. */
+kf_status kf_rigid_align(const kf_volume *live, const kf_volume *can, int32_t max_iters, double pose[16],
+                         kf_rigid_stats *stats);
+/* u_j = T(x_j) - j in the live grid's index units for every voxel j of the canonical grid: the
+ * displacement field of a rigid pose, a starting point for kf_deform. */
+kf_status kf_field_from_rigid(kf_field *u, const kf_volume *can, const kf_volume *live, const double pose[16]);
+
 /* ---- batched deform (independent volumes of uneven size) ----
  * No reference counterpart: the reference deforms one live SDF per call
  * (KillingFusion::computeDisplacementField, KillingFusion.cpp:240-397); this runs

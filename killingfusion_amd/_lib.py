@@ -43,6 +43,11 @@ class BatchConfig(C.Structure):
                 ("reserved", C.c_int32), ("us_per_iteration", C.c_double)]
 
 
+class RigidStats(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("reserved", C.c_int32), ("voxels", C.c_uint64),
+                ("initial_energy", C.c_double), ("energy", C.c_double), ("last_step", C.c_double)]
+
+
 class PipelineDesc(C.Structure):
     _fields_ = [("volume", VolumeDesc), ("params", Params), ("K", C.c_double * 9), ("C2W", C.c_double * 16),
                 ("min_depth", C.c_double), ("max_depth", C.c_double), ("width", C.c_int32),
@@ -99,6 +104,8 @@ SIGNATURES = {
                                   C.POINTER(Params), C.POINTER(DeformStats)]),
     "kf_batch_autotune": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.c_int32,
                                     C.POINTER(Params), C.c_int32, C.POINTER(BatchConfig)]),
+    "kf_rigid_align": (C.c_int, [_vp, _vp, C.c_int32, _dp, C.POINTER(RigidStats)]),
+    "kf_field_from_rigid": (C.c_int, [_vp, _vp, _vp, _dp]),
     "kf_pipeline_create": (C.c_int, [_vp, C.POINTER(PipelineDesc), C.POINTER(_vp)]),
     "kf_pipeline_destroy": (C.c_int, [_vp]),
     "kf_pipeline_process_frame": (C.c_int, [_vp, _fp, C.POINTER(FrameStats)]),

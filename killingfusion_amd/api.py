@@ -372,6 +372,25 @@ def energy_backward(src: SDF, dest: SDF, field: DisplacementField, params: Killi
           "kf_energy_backward")
 
 
+def rigid_align(live: SDF, can: SDF, max_iters: int = 30, pose=None):
+    """kf_rigid_align: the rigid pose (4x4, canonical world -> live world) that best maps the
+    canonical SDF onto the live one, by Gauss-Newton from `pose` (identity by default)."""
+    L = _lib.load()
+    T = np.ascontiguousarray(np.eye(4) if pose is None else pose, dtype=np.float64).reshape(16).copy()
+    st = _lib.RigidStats()
+    check(L.kf_rigid_align(live.handle, can.handle, int(max_iters), T.ctypes.data_as(_dp), C.byref(st)),
+          "kf_rigid_align")
+    return T.reshape(4, 4), dict(iters=st.iters, voxels=st.voxels, initial_energy=st.initial_energy,
+                                 energy=st.energy, last_step=st.last_step)
+
+
+def field_from_rigid(field: DisplacementField, can: SDF, live: SDF, pose):
+    """kf_field_from_rigid: the displacement field (live index units) of a rigid pose."""
+    L = _lib.load()
+    T = np.ascontiguousarray(pose, dtype=np.float64).reshape(16)
+    check(L.kf_field_from_rigid(field.handle, can.handle, live.handle, T.ctypes.data_as(_dp)), "kf_field_from_rigid")
+
+
 def _batch_args(lives, cans, fields_):
     n = len(lives)
     if not (len(cans) == n and len(fields_) == n):

@@ -30,6 +30,7 @@ SOURCES = {
     "k_deform_ref.hip": [],
     "k_deform_adj.hip": [],
     "k_deform_batch.hip": [],
+    "k_rigid.hip": [],
 }
 
 

@@ -171,22 +171,7 @@ struct BatchShape {
 constexpr int VPTS[4] = {1, 2, 4, 8};
 constexpr int GRID_MULT[4] = {0, 2, 4, 8};  // blocks = mult x CUs (0: one block per tile)
 
-struct DevGuard {
-    int prev = -1;
-    explicit DevGuard(int dev)
-    {
-        if (hipGetDevice(&prev) != hipSuccess)
-            prev = -1;
-        if (prev != dev)
-            (void)hipSetDevice(dev);
-    }
-    ~DevGuard()
-    {
-        int cur = -1;
-        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev)
-            (void)hipSetDevice(prev);
-    }
-};
+using DevGuard = KfDeviceGuard;
 
 }  // namespace
 

@@ -31,22 +31,7 @@ __global__ void k_state_reset(DeformState *st, double stop)
     }
 }
 
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev)
-    {
-        if (hipGetDevice(&prev) != hipSuccess)
-            prev = -1;
-        if (prev != dev)
-            (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard()
-    {
-        int cur = -1;
-        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev)
-            (void)hipSetDevice(prev);
-    }
-};
+using DeviceGuard = KfDeviceGuard;
 
 bool valid_dims(const int32_t d[3])
 {

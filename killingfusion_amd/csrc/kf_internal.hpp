@@ -213,6 +213,26 @@ inline GridDims grid_of(const kf_field *f) { return GridDims{f->dims[0], f->dims
 
 inline bool same_dims(const int32_t a[3], const int32_t b[3]) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2]; }
 
+// Makes a context's device current for the scope of an entry point and restores the caller's.
+struct KfDeviceGuard {
+    int prev = -1;
+    explicit KfDeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess)
+            prev = -1;
+        if (prev != dev)
+            (void)hipSetDevice(dev);
+    }
+    ~KfDeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev)
+            (void)hipSetDevice(prev);
+    }
+    KfDeviceGuard(const KfDeviceGuard &) = delete;
+    KfDeviceGuard &operator=(const KfDeviceGuard &) = delete;
+};
+
 // MI355X dispatches workgroup b to XCD (b mod 8), and each XCD has its own L2.  This maps
 // the dispatch index to a tile index so that every XCD sweeps one contiguous range of tiles:
 // tiles that share halo rows and slices then run on the same XCD and hit its L2.

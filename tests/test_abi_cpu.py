@@ -60,6 +60,8 @@ def test_status_strings_and_errors_without_device():
     assert L.kf_marching_cubes_indexed(None, 0.0, None, None, None, None, None, 0, 0) == _lib.KF_E_INVALID_ARG
     assert L.kf_integrate_depth(None, None, 640, 480, None, None, 0.1, 3.0) == _lib.KF_E_INVALID_ARG
     assert L.kf_batch_deform(None, None, None, 2, None, None) == _lib.KF_E_INVALID_ARG
+    assert L.kf_rigid_align(None, None, 5, None, None) == _lib.KF_E_INVALID_ARG
+    assert L.kf_field_from_rigid(None, None, None, None) == _lib.KF_E_INVALID_ARG
     assert L.kf_batch_autotune(None, None, None, 2, None, 0, None) == _lib.KF_E_INVALID_ARG
     bad = np.zeros(3, dtype=np.float64)
     out = (C.c_int32 * 3)()
