@@ -358,6 +358,39 @@ inline kf_deform_stats computeDisplacementField(const SDF *src, const SDF *dest,
     return st;
 }
 
+/* Rigid pre-registration (no reference counterpart): the pose (row-major 4x4, canonical world
+ * -> live world) minimising the squared SDF difference over the canonical band, from `pose`. */
+inline kf_rigid_stats rigidAlign(const SDF *live, const SDF *can, std::array<double, 16> &pose, int maxIters = 30)
+{
+    kf_rigid_stats st{};
+    check(kf_rigid_align(live->handle(), can->handle(), maxIters, pose.data(), &st), "kf_rigid_align");
+    return st;
+}
+
+/* The displacement field of a rigid pose: a starting point for computeDisplacementField. */
+inline void fieldFromRigid(DisplacementField *u, const SDF *can, const SDF *live, const std::array<double, 16> &pose)
+{
+    check(kf_field_from_rigid(u->handle(), can->handle(), live->handle(), pose.data()), "kf_field_from_rigid");
+}
+
+/* Independent volumes in one batch (kf_batch_deform): one stats entry per volume. */
+inline std::vector<kf_deform_stats> batchDeform(const std::vector<const SDF *> &live, const std::vector<const SDF *> &can,
+                                                const std::vector<DisplacementField *> &u, const Params &p)
+{
+    if (live.size() != can.size() || live.size() != u.size())
+        throw std::invalid_argument("batch lists differ in length");
+    std::vector<const kf_volume *> l, c;
+    std::vector<kf_field *> f;
+    for (size_t b = 0; b < live.size(); b++) {
+        l.push_back(live[b]->handle());
+        c.push_back(can[b]->handle());
+        f.push_back(u[b]->handle());
+    }
+    std::vector<kf_deform_stats> st(live.size());
+    check(kf_batch_deform(l.data(), c.data(), f.data(), (int32_t)live.size(), &p, st.data()), "kf_batch_deform");
+    return st;
+}
+
 /* The per-frame driver: one processNextFrame per depth frame over a device-resident
  * canonical SDF (frame 1 becomes the canonical SDF). */
 class KillingFusion {

@@ -162,6 +162,24 @@ int main(int argc, char **argv)
         expect(welded && pm.triangles() > 0, "welded mesh indexes the soup's vertices");
         expect(wm.WriteMesh(out + "/plane_welded.off"), "welded WriteMesh");
 
+        // Rigid pre-registration of the plane onto itself keeps the identity; the batch of one
+        // volume equals computeDisplacementField.
+        std::array<double, 16> pose = I4;
+        kf_rigid_stats rs = kf::rigidAlign(&plane, &plane, pose, 5);
+        bool ident = rs.voxels > 0 && rs.energy == 0.0;
+        for (int k = 0; k < 16; k++)
+            ident = ident && std::fabs(pose[k] - I4[k]) < 1e-12;
+        expect(ident, "rigid alignment of a volume with itself is the identity");
+        {
+            kf::DisplacementField u1(ctx, dims, voxel), u2(ctx, dims, voxel);
+            kf::Params pb = kf::Params::paper(voxel, 3.0, KF_METHOD_ADJOINT);
+            pb.max_iters = 12;
+            kf::computeDisplacementField(&live, &plane, &u1, pb);
+            auto bst = kf::batchDeform({&live}, {&plane}, {&u2}, pb);
+            expect(bst.size() == 1 && bst[0].iters == 12 && u1.download() == u2.download(),
+                   "a batch of one equals computeDisplacementField");
+        }
+
         // 7. Formats: OFF round trip and the two .bin dumps.
         const std::string off = out + "/plane.off";
         expect(mesh.WriteMesh(off), "WriteMesh");
