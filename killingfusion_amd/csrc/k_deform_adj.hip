@@ -268,50 +268,62 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
 // is the same for every voxel, so rows that are not a multiple of the wavefront's width leave
 // no lanes idle.  (x, y, z) is recovered only for the compacted stencil jobs.
 // ---------------------------------------------------------------------------
-constexpr int FLAT = BLOCK * VEC;  // voxels per block
+constexpr int FPT = 1;                 // float4 groups per thread
+constexpr int FB = BLOCK;              // threads per block
+constexpr int FLAT = FB * VEC * FPT;   // voxels per block
 
 template <class S>
-__global__ __launch_bounds__(BLOCK) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
+__global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, AdjConsts c, DeformState *st,
                                                     BlockPartial *partials, int parity, const BlockPartial *prev,
                                                     unsigned int prev_nb)
 {
-    if (!step_gate<KF_MODE_STEP>(st, prev, prev_nb, parity))
+    if (!step_gate<KF_MODE_STEP, FB>(st, prev, prev_nb, parity))
         return;
     const GridDims g = L.g;
     const unsigned int n32 = nvox32(g);
     const unsigned int tile = kf_xcd_tile(blockIdx.x, gridDim.x);
-    const unsigned int i0 = tile * (unsigned int)FLAT + threadIdx.x * (unsigned int)VEC;
-    const bool valid = i0 < n32;  // n is a multiple of 4, so the whole float4 is inside
-    float4 C[3];
-    float live4[VEC];
+    // group q of thread t: voxels tile FLAT + q FB VEC + t VEC .. + 3 (n is a multiple of 4, so
+    // a group is either whole or outside)
+    unsigned int i0[FPT];
+    bool valid[FPT];
+    float4 C[FPT][3];
+    float live4[FPT][VEC];
 #pragma unroll
-    for (int a = 0; a < 3; a++)
-        C[a] = valid ? kf_load_stream4(u_in + ((unsigned int)a * n32 + i0)) : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-    for (int k = 0; k < VEC; k++)
-        live4[k] = 0.f;
-    if (valid)
-        S::load4(L.p, (int64_t)i0, L.scale, live4);
-    unsigned int need = 0u;
-    if (valid) {
+    for (int q = 0; q < FPT; q++) {
+        i0[q] = tile * (unsigned int)FLAT + (unsigned int)q * (FB * VEC) + threadIdx.x * (unsigned int)VEC;
+        valid[q] = i0[q] < n32;
 #pragma unroll
         for (int a = 0; a < 3; a++)
-            *reinterpret_cast<float4 *>(u_out + ((unsigned int)a * n32 + i0)) = C[a];
+            C[q][a] = valid[q] ? kf_load_stream4(u_in + ((unsigned int)a * n32 + i0[q])) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < VEC; k++)
+            live4[q][k] = 0.f;
+        if (valid[q])
+            S::load4(L.p, (int64_t)i0[q], L.scale, live4[q]);
+    }
+    unsigned int need = 0u;
+#pragma unroll
+    for (int q = 0; q < FPT; q++) {
+        if (!valid[q])
+            continue;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            *reinterpret_cast<float4 *>(u_out + ((unsigned int)a * n32 + i0[q])) = C[q][a];
 #pragma unroll
         for (int k = 0; k < VEC; k++) {
-            const bool moved = ((__float_as_uint(f4(C[0], k)) | __float_as_uint(f4(C[1], k)) |
-                                 __float_as_uint(f4(C[2], k))) & 0x7fffffffu) != 0u;
-            const bool in_band = !(live4[k] > c.hi || live4[k] < c.lo);
+            const bool moved = ((__float_as_uint(f4(C[q][0], k)) | __float_as_uint(f4(C[q][1], k)) |
+                                 __float_as_uint(f4(C[q][2], k))) & 0x7fffffffu) != 0u;
+            const bool in_band = !(live4[q][k] > c.hi || live4[q][k] < c.lo);
             if (moved || in_band)
-                need |= 1u << k;
+                need |= 1u << (q * VEC + k);
         }
     }
     // The stencil for active or moved voxels, compacted over the wavefront as in k_adj_rows.
-    unsigned long long bal[VEC];
+    unsigned long long bal[FPT * VEC];
     int total = 0;
 #pragma unroll
-    for (int b = 0; b < VEC; b++) {
+    for (int b = 0; b < FPT * VEC; b++) {
         bal[b] = __ballot((need >> b) & 1u);
         total += __popcll(bal[b]);
     }
@@ -321,14 +333,15 @@ __global__ __launch_bounds__(BLOCK) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn,
     const unsigned int nxy = (unsigned int)g.nx * (unsigned int)g.ny;
     for (int j = threadIdx.x & 63; j < total; j += 64) {
         int bit, src;
-        kf_job_decode<VEC>(bal, j, bit, src);
-        const unsigned int i = wave0 + (unsigned int)src * VEC + (unsigned int)bit;
+        kf_job_decode<FPT * VEC>(bal, j, bit, src);
+        const unsigned int i =
+            wave0 + (unsigned int)(bit / VEC) * (FB * VEC) + (unsigned int)src * VEC + (unsigned int)(bit % VEC);
         const unsigned int z = i / nxy, r = i - z * nxy;
         const unsigned int y = r / (unsigned int)g.nx, x = r - y * (unsigned int)g.nx;
         adj_job<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, &st->div_slot[parity], nullptr, (int)x, (int)y, (int)z, mx,
                                  esum, act);
     }
-    kf_block_publish<BLOCK>(partials, blockIdx.x, esum, mx, act);
+    kf_block_publish<FB>(partials, blockIdx.x, esum, mx, act);
 }
 
 // Nonzero mask of field values u (see nbr_bits).
@@ -359,7 +372,7 @@ void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float 
 {
     BlockPartial *parts = kf_partials(ctx, parity);
     if (MODE == KF_MODE_STEP && rows)
-        hipLaunchKernelGGL((k_adj_flat<S>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c, ctx->state,
+        hipLaunchKernelGGL((k_adj_flat<S>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out, c, ctx->state,
                            parts, parity, prev, prev_nb);
     else if (rows)
         hipLaunchKernelGGL((k_adj_rows<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,

@@ -198,7 +198,7 @@ __device__ inline float uval(const float *__restrict__ u, const GridDims &g, int
 // a launch never reads a gate field it writes: every wave of every block sees the same
 // answer.  Launch k itself may run after launch k - 1 met the stop rule; it then only writes
 // the buffer that is not the result (see kf_deform), and launch k + 1 does nothing.
-template <int MODE>
+template <int MODE, int NT = BLOCK>
 __device__ inline bool step_gate(DeformState *st, const BlockPartial *prev, unsigned int prev_nb, int parity)
 {
     if (MODE != KF_MODE_STEP)
@@ -206,7 +206,7 @@ __device__ inline bool step_gate(DeformState *st, const BlockPartial *prev, unsi
     const bool go = st->run[parity] != 0u;
     if (blockIdx.x == 0) {
         if (go && prev_nb > 0) {
-            kf_fold_launch<BLOCK>(st, prev, prev_nb, parity ^ 1);
+            kf_fold_launch<NT>(st, prev, prev_nb, parity ^ 1);
             __syncthreads();  // the block's own publish reuses the reduction LDS
         }
         if (threadIdx.x == 0)
