@@ -28,6 +28,14 @@
  * u[0..N) = ux, u[N..2N) = uy, u[2N..3N) = uz, in voxel units.  Weights are int64
  * (the reference's `long`, SDF.h:25).
  *
+ * Pins (tests/test_oracle_kat.py): every function is checked against the reference's own
+ * self-test expectations, closed forms, brute force or an independent transcription, as
+ * listed per function there.  Parity unpinned: results that depend on Eigen's evaluation
+ * order -- the K (X/Z, Y/Z, 1) projection and the 4x4 inverse of integration (SDF.cpp:146-164),
+ * norm() in the Killing clamp (DisplacementField.cpp:343-344) and the level-set term
+ * (KillingFusion.cpp:474) -- are restated in plain left-to-right FP64 and can differ from the
+ * reference binary in the last bits; nothing at the stated tolerances depends on them.
+ *
  * The second half of this file is the FP64 oracle of the adjoint formulation
  * (a discrete energy and its exact gradient) that the device library's fused
  * forward/backward kernels implement.  It has no counterpart in the reference;

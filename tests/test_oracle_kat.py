@@ -532,3 +532,46 @@ def test_adjoint_local_energy_carries_every_term_of_a_voxel():
             dl = O.adj_energy_local(live, can, up, g, p, px, py, pz) - O.adj_energy_local(live, can, um, g, p, px, py, pz)
             dt = O.adj_energy(live, can, up, g, p) - O.adj_energy(live, can, um, g, p)
             assert dl == pytest.approx(dt, rel=1e-9, abs=1e-9)
+
+
+def test_gauss_seidel_serial_equals_a_transcription_of_the_per_voxel_loop():
+    # KillingFusion.cpp:353-396 run in z -> y -> x order: skip test on the displaced distance
+    # (symmetric +-(Max - eps)), then repeated in-place steps at that voxel until the step or
+    # the gradient is below the threshold or the iteration cap is hit.  An independent Python
+    # transcription over the oracle's own per-voxel pieces (get_distance, energy_gradient).
+    import math
+    g = O.OGrid(dims=(6, 5, 4), voxel=0.01, oob=0.03 + 1e-5)
+    p = O.OParams(voxel_size=0.01, max_surface=0.03, clip=0.03, alpha=0.1, omega_k=0.5, gamma=0.1, omega_s=0.2,
+                  max_iters=5, gs_threshold=1e-4)
+    rng = np.random.default_rng(17)
+    z, y, x = np.meshgrid(np.arange(4), np.arange(5), np.arange(6), indexing="ij")
+    live = (0.01 * (x + 0.5 * y - 0.3 * z - 2.2)).ravel().astype(np.float64)
+    can = live + rng.uniform(-0.003, 0.003, live.size)
+    u0 = rng.uniform(-0.3, 0.3, (3, live.size))
+    uo, _ = O.deform_gs_serial(live, can, u0, g, p)
+    u = u0.copy()
+    nx, ny = 6, 5
+    visited = 0
+    for zz in range(4):
+        for yy in range(5):
+            for xx in range(6):
+                i = (zz * ny + yy) * nx + xx
+                # SDF::getDistance(idx, disp), SDF.cpp:570-575: idx + u + 0.5
+                d = O.get_distance(live, g, (xx + u[0, i] + 0.5, yy + u[1, i] + 0.5, zz + u[2, i] + 0.5))
+                if d > p.max_surface - p.epsilon or d < -p.max_surface + p.epsilon:
+                    continue
+                visited += 1
+                it = 0
+                while True:
+                    gr = O.energy_gradient(live, can, u, g, p, xx, yy, zz)
+                    du = [-p.alpha * gr[0], -p.alpha * gr[1], -p.alpha * gr[2]]
+                    for a in range(3):
+                        u[a, i] = u[a, i] + du[a]
+                    if math.sqrt(du[0] * du[0] + du[1] * du[1] + du[2] * du[2]) <= p.gs_threshold:
+                        break
+                    it += 1
+                    if not (math.sqrt(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]) > p.gs_threshold
+                            and it < p.max_iters):
+                        break
+    assert visited > 20
+    assert np.array_equal(uo, u)
