@@ -282,7 +282,11 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
         return;
     const GridDims g = L.g;
     const unsigned int n32 = nvox32(g);
-    const unsigned int tile = kf_xcd_tile(blockIdx.x, gridDim.x);
+    // Odd launches sweep the grid backwards: a launch reads the buffer the previous one wrote,
+    // and starting where that one ended finds the most recently written lines still in the
+    // last-level cache.  (About 5% per launch at 256^3; the result does not depend on order.)
+    const unsigned int tile0 = kf_xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned int tile = parity ? gridDim.x - 1u - tile0 : tile0;
     // group q of thread t: voxels tile FLAT + q FB VEC + t VEC .. + 3 (n is a multiple of 4, so
     // a group is either whole or outside)
     unsigned int i0[FPT];

@@ -115,7 +115,10 @@ __global__ __launch_bounds__(BLOCK) void k_adj_batch(const BatchVol<S> *__restri
             st->run[parity ^ 1] = (go && !st->done) ? 1u : 0u;
     }
     BlockPartial *mine = parts + (size_t)parity * cap;
-    for (unsigned int t = blockIdx.x; t < total; t += gridDim.x) {
+    // Odd launches visit the tiles backwards, so they start on the lines the previous launch
+    // wrote last (still in the last-level cache).
+    for (unsigned int tt = blockIdx.x; tt < total; tt += gridDim.x) {
+        const unsigned int t = parity ? total - 1u - tt : tt;
         int lo = 0, hi = count - 1;  // volume of tile t (tile0 ascending)
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
