@@ -234,8 +234,9 @@ typedef struct kf_batch_config {
 
 /* Autotuner of kf_batch_deform (no reference counterpart): times every candidate shape on
  * this batch (launches that read u[b] and write only its scratch buffer, so the fields keep
- * their values) and caches the fastest for batches of the same dims and dtype.  force != 0
- * re-times a cached shape.  best may be NULL. */
+ * their values) and caches the fastest for batches of the same dims and dtype.  force = 1
+ * re-times a cached shape; force = 2 pins the shape given in *best (voxels_per_thread, blocks;
+ * blocks 0 = one block per tile) without timing.  best may be NULL unless force = 2. */
 kf_status kf_batch_autotune(const kf_volume *const *live, const kf_volume *const *can,
                             kf_field *const *u, int32_t count, const kf_params *p, int32_t force,
                             kf_batch_config *best);

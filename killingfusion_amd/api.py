@@ -413,14 +413,19 @@ def batch_deform(lives, cans, fields_, params: KillingParams, stats: bool = True
     return [_stats(st[i]) for i in range(n)] if stats else None
 
 
-def batch_autotune(lives, cans, fields_, params: KillingParams, force: bool = False):
-    """kf_batch_autotune: times the batched launch's tile shapes on this batch; returns the
-    chosen shape."""
+def batch_autotune(lives, cans, fields_, params: KillingParams, force: bool = False, shape=None):
+    """kf_batch_autotune: times the batched launch's tile shapes on this batch and returns the
+    chosen one; shape=(voxels_per_thread, blocks) pins that shape instead (blocks 0 = one
+    block per tile)."""
     L = _lib.load()
     a, b, c, n = _batch_args(lives, cans, fields_)
     cfg = _lib.BatchConfig()
+    mode = int(bool(force))
+    if shape is not None:
+        cfg.voxels_per_thread, cfg.blocks = int(shape[0]), int(shape[1])
+        mode = 2
     p = params.c()
-    check(L.kf_batch_autotune(a, b, c, n, C.byref(p), int(bool(force)), C.byref(cfg)), "kf_batch_autotune")
+    check(L.kf_batch_autotune(a, b, c, n, C.byref(p), mode, C.byref(cfg)), "kf_batch_autotune")
     return {"voxels_per_thread": cfg.voxels_per_thread, "blocks": cfg.blocks, "candidates": cfg.candidates,
             "us_per_iteration": cfg.us_per_iteration}
 

@@ -30,6 +30,8 @@ struct BatchVol {
     float *u[2];          // launch k reads u[k & 1] and writes u[(k + 1) & 1]
     unsigned int tile0;   // first tile of the volume in the batch's tile space
     unsigned int ntiles;
+    int vec4;             // N is a multiple of 4: tiles stream as float4 groups
+    int pad;
 };
 
 // One tile of a Jacobi step: stream u and phi_live, store every voxel's u_j, then run the
@@ -94,6 +96,76 @@ __device__ inline void batch_tile(const BatchVol<S> &V, unsigned int t, const Ad
     }
 }
 
+// The same tile for a volume whose N is a multiple of 4, streamed as float4 groups like
+// k_adj_flat: thread t owns groups base + 1024 q + 4 t (q < G), so a tile still holds BLOCK 4 G
+// consecutive voxels.
+template <class S, int G>
+__device__ inline void batch_tile_v4(const BatchVol<S> &V, unsigned int t, const AdjConsts &c, long long *div,
+                                     int parity, double &mx, unsigned long long &act)
+{
+    constexpr int VEC = 4;
+    const GridDims g = V.L.g;
+    const unsigned int n32 = nvox32(g);
+    const unsigned int base = t * (unsigned int)(BLOCK * VEC * G);
+    const float *__restrict__ u_in = V.u[parity];
+    float *__restrict__ u_out = V.u[parity ^ 1];
+    unsigned int i0[G];
+    bool ok[G];
+    float4 C[G][3];
+    float live4[G][VEC];
+#pragma unroll
+    for (int q = 0; q < G; q++) {
+        i0[q] = base + (unsigned int)q * (BLOCK * VEC) + threadIdx.x * VEC;
+        ok[q] = i0[q] < n32;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            C[q][a] = ok[q] ? kf_load_stream4(u_in + ((unsigned int)a * n32 + i0[q])) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < VEC; k++)
+            live4[q][k] = 0.f;
+        if (ok[q])
+            S::load4(V.L.p, (int64_t)i0[q], V.L.scale, live4[q]);
+    }
+    unsigned int need = 0u;
+#pragma unroll
+    for (int q = 0; q < G; q++) {
+        if (!ok[q])
+            continue;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            *reinterpret_cast<float4 *>(u_out + ((unsigned int)a * n32 + i0[q])) = C[q][a];
+#pragma unroll
+        for (int k = 0; k < VEC; k++) {
+            const float c0 = k == 0 ? C[q][0].x : k == 1 ? C[q][0].y : k == 2 ? C[q][0].z : C[q][0].w;
+            const float c1 = k == 0 ? C[q][1].x : k == 1 ? C[q][1].y : k == 2 ? C[q][1].z : C[q][1].w;
+            const float c2 = k == 0 ? C[q][2].x : k == 1 ? C[q][2].y : k == 2 ? C[q][2].z : C[q][2].w;
+            const bool moved = ((__float_as_uint(c0) | __float_as_uint(c1) | __float_as_uint(c2)) & 0x7fffffffu) != 0u;
+            const bool in_band = !(live4[q][k] > c.hi || live4[q][k] < c.lo);
+            if (moved || in_band)
+                need |= 1u << (q * VEC + k);
+        }
+    }
+    unsigned long long bal[G * VEC];
+    int total = 0;
+#pragma unroll
+    for (int b = 0; b < G * VEC; b++) {
+        bal[b] = __ballot((need >> b) & 1u);
+        total += __popcll(bal[b]);
+    }
+    double esum = 0.0;
+    const unsigned int wave0 = base + (threadIdx.x & ~63u) * VEC;
+    const unsigned int nxy = (unsigned int)g.nx * (unsigned int)g.ny;
+    for (int j = threadIdx.x & 63; j < total; j += 64) {
+        int bit, src;
+        kf_job_decode<G * VEC>(bal, j, bit, src);
+        const unsigned int i = wave0 + (unsigned int)(bit / VEC) * (BLOCK * VEC) + (unsigned int)src * VEC +
+                               (unsigned int)(bit % VEC);
+        const unsigned int z = i / nxy, r = i - z * nxy;
+        const unsigned int y = r / (unsigned int)g.nx, x = r - y * (unsigned int)g.nx;
+        adj_job<S, KF_MODE_STEP>(V.L, V.Cn, u_in, u_out, c, div, nullptr, (int)x, (int)y, (int)z, mx, esum, act);
+    }
+}
+
 // Step launch `iter` of the batch.  parts holds two sets of cap slots (launch parity); the
 // tile t of a launch publishes into slot t of its set.
 template <class S, int VPT>
@@ -118,7 +190,9 @@ __global__ __launch_bounds__(BLOCK) void k_adj_batch(const BatchVol<S> *__restri
     // Odd launches visit the tiles backwards, so they start on the lines the previous launch
     // wrote last (still in the last-level cache).
     for (unsigned int tt = blockIdx.x; tt < total; tt += gridDim.x) {
-        const unsigned int t = parity ? total - 1u - tt : tt;
+        // each XCD (dispatch index mod 8) takes one contiguous range of tiles
+        const unsigned int tx = kf_xcd_tile(tt, total);
+        const unsigned int t = parity ? total - 1u - tx : tx;
         int lo = 0, hi = count - 1;  // volume of tile t (tile0 ascending)
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -131,7 +205,11 @@ __global__ __launch_bounds__(BLOCK) void k_adj_batch(const BatchVol<S> *__restri
             continue;
         double mx = 0.0;
         unsigned long long act = 0;
-        batch_tile<S, VPT>(vols[lo], t - vols[lo].tile0, c, &states[lo].div_slot[parity], parity, mx, act);
+        if (VPT >= 4 && vols[lo].vec4)
+            batch_tile_v4<S, (VPT >= 4 ? VPT / 4 : 1)>(vols[lo], t - vols[lo].tile0, c, &states[lo].div_slot[parity],
+                                                       parity, mx, act);
+        else
+            batch_tile<S, VPT>(vols[lo], t - vols[lo].tile0, c, &states[lo].div_slot[parity], parity, mx, act);
         kf_block_publish<BLOCK>(mine, t, 0.0, mx, act);
         __syncthreads();  // thread 0 reads the publish LDS before the next tile writes it
     }
@@ -291,6 +369,8 @@ size_t build_table(const kf_volume *const *live, const kf_volume *const *can, kf
         e.u[1] = u[v]->buf[u[v]->cur ^ 1];
         e.tile0 = (unsigned int)t0;
         e.ntiles = (unsigned int)(((size_t)live[v]->n + (size_t)BLOCK * vpt - 1) / ((size_t)BLOCK * vpt));
+        e.vec4 = live[v]->n % 4 == 0 ? 1 : 0;
+        e.pad = 0;
         t0 += e.ntiles;
     }
     return t0;
@@ -584,15 +664,35 @@ kf_status kf_batch_autotune(const kf_volume *const *live, const kf_volume *const
     kf_context *ctx = live[0]->ctx;
     DevGuard dg(ctx->device);
     BatchShape sh;
-    KF_DISPATCH_DTYPE(live[0]->dtype, S, s = tuned_shape<S>(ctx, live, can, u, count, p, force != 0, sh));
-    if (s != KF_OK)
-        return s;
+    if (force == 2) {
+        // Pin the shape given in *best (no timing).
+        KF_CHECK_ARG(best && (best->voxels_per_thread == 1 || best->voxels_per_thread == 2 ||
+                              best->voxels_per_thread == 4 || best->voxels_per_thread == 8) &&
+                         best->blocks >= 0,
+                     "force = 2 needs a shape in *best (voxels_per_thread 1, 2, 4 or 8; blocks >= 0)");
+        sh.vpt = best->voxels_per_thread;
+        sh.blocks = (unsigned int)best->blocks;
+        kf_batch_cache *b = cache_of(ctx);
+        const std::string key = shape_key(live, count);
+        bool found = false;
+        for (auto &e : b->tuned)
+            if (e.first == key) {
+                e.second = sh;
+                found = true;
+            }
+        if (!found)
+            b->tuned.emplace_back(key, sh);
+    } else {
+        KF_DISPATCH_DTYPE(live[0]->dtype, S, s = tuned_shape<S>(ctx, live, can, u, count, p, force != 0, sh));
+        if (s != KF_OK)
+            return s;
+    }
     if (best) {
         std::memset(best, 0, sizeof(*best));
         best->voxels_per_thread = sh.vpt;
         std::vector<BatchVol<StoreF32>> tab;  // tile counts do not depend on the dtype
         const size_t total = build_table<StoreF32>(live, can, u, count, sh.vpt, tab);
-        best->blocks = (int32_t)(sh.blocks ? sh.blocks : total);
+        best->blocks = (int32_t)(sh.blocks && sh.blocks < total ? sh.blocks : total);
         best->candidates = sh.candidates;
         best->us_per_iteration = sh.us;
     }

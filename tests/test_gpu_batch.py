@@ -151,6 +151,24 @@ def test_autotune_and_retuned_shape_give_same_fields(ctx, batch):
         assert np.array_equal(f.download(), ref[b]), b
 
 
+def test_every_tile_shape_gives_the_same_fields(ctx, batch):
+    # float4 tiles (volumes with N a multiple of 4: 48^3 and 70x45x58 here) and scalar tiles,
+    # persistent and one-block-per-tile grids: the per-voxel arithmetic is the same.
+    kp = params(max_iters=12)
+    ref, ref_st = per_volume(ctx, batch, kp)
+    lives, cans = lists(batch)
+    for vpt in (1, 2, 4, 8):
+        for blocks in (0, 300):
+            cfg = batch_autotune(lives, cans, fresh_fields(ctx, batch), kp, shape=(vpt, blocks))
+            assert cfg["voxels_per_thread"] == vpt
+            fields = fresh_fields(ctx, batch)
+            st = batch_deform(lives, cans, fields, kp)
+            for b, f in enumerate(fields):
+                assert np.array_equal(f.download(), ref[b]), (vpt, blocks, b)
+                assert st[b]["iters"] == ref_st[b]["iters"] and st[b]["active"] == ref_st[b]["active"]
+    batch_autotune(lives, cans, fresh_fields(ctx, batch), kp, force=True)
+
+
 def test_batch_argument_errors(ctx, batch):
     lives, cans = lists(batch)
     f = DisplacementField(ctx, lives[0].dims)
