@@ -56,6 +56,18 @@ typedef enum kf_method {
     KF_METHOD_ADJOINT = 1    /* exact gradient of the discrete energy of kf_energy_forward */
 } kf_method;
 
+/* Which voxels an adjoint Jacobi step visits (the field it produces is the same bit for bit).
+ * A step moves voxel j only when -clip <= phi_live(j + u_j) <= Max - eps
+ * (KillingFusion.cpp:272-274), and with u_j = 0 that reads phi_live[j]; so the voxels that
+ * can move during one kf_deform call are those with u_j != 0 or phi_live[j] in that band,
+ * fixed at the start of the call.  The active-set sweep lists them once and steps only
+ * them; the dense sweep streams the whole grid every iteration. */
+typedef enum kf_sweep {
+    KF_SWEEP_AUTO = 0,   /* active set when it holds at most 1/8 of the voxels, else dense */
+    KF_SWEEP_DENSE = 1,  /* every voxel every iteration */
+    KF_SWEEP_ACTIVE = 2  /* the listed voxels only */
+} kf_sweep;
+
 typedef struct kf_context kf_context;
 typedef struct kf_volume kf_volume;
 typedef struct kf_field kf_field;
@@ -90,16 +102,17 @@ typedef struct kf_params {
     int32_t max_iters;      /* KILLING_MAX_ITERATIONS (config.cpp:42) */
     double stop_max_update; /* stop once max |du| < this (voxels); <= 0 runs max_iters */
     int32_t method;         /* kf_method */
-    int32_t reserved;
+    int32_t sweep;          /* kf_sweep (adjoint method only; the reference method is dense) */
 } kf_params;
 
 typedef struct kf_deform_stats {
     int32_t iters;           /* iterations run */
-    int32_t reserved;
+    int32_t sweep;           /* the kf_sweep that ran (KF_SWEEP_DENSE or KF_SWEEP_ACTIVE) */
     int64_t diverged_index;  /* first voxel with a non-finite displacement, or -1 */
     double max_update;       /* max |du| of the last iteration, voxels (FP64) */
     double energy;           /* adjoint: energy of the final field (FP64); reference: 0 */
     uint64_t active_voxels;  /* voxels updated in the last iteration */
+    uint64_t listed_voxels;  /* voxels that could move (adjoint; 0 when the list was not built) */
 } kf_deform_stats;
 
 /* ---- library / context ---- */

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libkf.so")
 KF_OK, KF_E_INVALID_ARG, KF_E_DIVERGED, KF_E_OOM, KF_E_HIP, KF_E_NO_DEVICE, KF_E_CAPACITY = range(7)
 KF_F32, KF_BF16, KF_F16 = 0, 1, 2
 KF_METHOD_REFERENCE, KF_METHOD_ADJOINT = 0, 1
+KF_SWEEP_AUTO, KF_SWEEP_DENSE, KF_SWEEP_ACTIVE = 0, 1, 2
 
 DTYPES = {"f32": KF_F32, "bf16": KF_BF16, "f16": KF_F16}
 
@@ -30,12 +31,13 @@ class Params(C.Structure):
                 ("gamma_killing", C.c_double), ("omega_levelset", C.c_double), ("delta_size", C.c_double),
                 ("killing_clamp", C.c_double), ("use_data", C.c_int32), ("use_levelset", C.c_int32),
                 ("use_killing", C.c_int32), ("max_iters", C.c_int32), ("stop_max_update", C.c_double),
-                ("method", C.c_int32), ("reserved", C.c_int32)]
+                ("method", C.c_int32), ("sweep", C.c_int32)]
 
 
 class DeformStats(C.Structure):
-    _fields_ = [("iters", C.c_int32), ("reserved", C.c_int32), ("diverged_index", C.c_int64),
-                ("max_update", C.c_double), ("energy", C.c_double), ("active_voxels", C.c_uint64)]
+    _fields_ = [("iters", C.c_int32), ("sweep", C.c_int32), ("diverged_index", C.c_int64),
+                ("max_update", C.c_double), ("energy", C.c_double), ("active_voxels", C.c_uint64),
+                ("listed_voxels", C.c_uint64)]
 
 
 class BatchConfig(C.Structure):

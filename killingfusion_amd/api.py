@@ -25,6 +25,9 @@ def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
+_SWEEPS = {"auto": 0, "dense": 1, "active": 2}  # kf_sweep
+
+
 @dataclass
 class KillingParams:
     """Optimiser parameters (the reference's config.cpp:4-57 globals as a runtime struct).
@@ -45,6 +48,7 @@ class KillingParams:
     max_iters: int = 1024
     stop_max_update: float = 0.0
     method: str = "reference"  # or "adjoint"
+    sweep: str = "auto"  # adjoint steps: "auto", "dense" (every voxel) or "active" (voxels that can move)
 
     @classmethod
     def paper(cls, voxel_size: float, trunc_voxels: float = 3.0, **kw):
@@ -60,6 +64,10 @@ class KillingParams:
             v = getattr(self, f.name)
             if f.name == "method":
                 v = _lib.KF_METHOD_ADJOINT if v == "adjoint" else _lib.KF_METHOD_REFERENCE
+            elif f.name == "sweep":
+                if v not in _SWEEPS:
+                    raise ValueError(f"sweep must be one of {sorted(_SWEEPS)}, not {v!r}")
+                v = _SWEEPS[v]
             setattr(p, f.name, v)
         return p
 
@@ -329,7 +337,8 @@ class DisplacementField:
 
 def _stats(s: _lib.DeformStats) -> dict:
     return dict(iters=s.iters, diverged_index=s.diverged_index, max_update=s.max_update, energy=s.energy,
-                active=s.active_voxels)
+                active=s.active_voxels, listed=s.listed_voxels,
+                sweep={1: "dense", 2: "active"}.get(s.sweep, "dense"))
 
 
 def compute_displacement_field(src: SDF, dest: SDF, field: DisplacementField, params: KillingParams) -> dict:

@@ -29,6 +29,7 @@ SOURCES = {
     "k_mc.hip": ["-ffp-contract=off"],
     "k_deform_ref.hip": [],
     "k_deform_adj.hip": [],
+    "k_deform_active.hip": [],
     "k_deform_batch.hip": [],
     "k_rigid.hip": [],
 }

@@ -589,6 +589,7 @@ kf_status run_batch(kf_context *ctx, const kf_volume *const *live, const kf_volu
             o.diverged_index = h.diverged == LLONG_MAX ? -1 : (int64_t)h.diverged;
             o.max_update = h.last_max;
             o.active_voxels = h.last_active;
+            o.sweep = KF_SWEEP_DENSE;
         }
     }
     if (stats) {

@@ -309,10 +309,11 @@ PhiRead<S> make_read(const kf_volume *v)
 
 // The full stencil of voxel (x, y, z) (in the grid): u_j, the 8 live corners around
 // j + u_j, the u neighbourhood and phi_can[j].  A step writes the moved u_j of an active
-// voxel (the caller stored u_j before); a gradient pass writes dE/du_j; an energy pass
-// writes the voxel's energy when e_voxel is given.  Loads of the cell and the neighbourhood
-// are issued together (one round trip after u_j).
-template <class S, int MODE>
+// voxel (the caller stored u_j before) or, with WRITE_ALL, the new u_j of every voxel; a
+// gradient pass writes dE/du_j; an energy pass writes the voxel's energy when e_voxel is
+// given.  Loads of the cell and the neighbourhood are issued together (one round trip after
+// u_j).
+template <class S, int MODE, bool WRITE_ALL = false>
 __device__ inline void adj_job(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *__restrict__ u_in,
                                float *__restrict__ u_out, const AdjConsts &c, long long *div,
                                float *__restrict__ e_voxel, int x, int y, int z, double &mx, double &esum,
@@ -345,7 +346,7 @@ __device__ inline void adj_job(const PhiRead<S> &L, const PhiRead<S> &Cn, const 
     } else if (MODE == KF_MODE_ENERGY) {
         if (e_voxel)
             e_voxel[i] = o.e;
-    } else if (o.active) {
+    } else if (o.active || WRITE_ALL) {
         float n[3];
         adj_update(c, o, nb.uc, (int64_t)i, div, mx, n);
         u_out[i] = n[0];

@@ -28,6 +28,7 @@ __global__ void k_state_reset(DeformState *st, double stop)
         st->last_max = 0.0;
         st->energy = 0.0;
         st->last_active = 0ull;
+        st->listed = 0u;
     }
 }
 
@@ -46,7 +47,8 @@ bool valid_dims(const int32_t d[3])
 bool valid_params_impl(const kf_params *p)
 {
     return p && p->voxel_size > 0.0 && p->max_iters >= 0 && p->delta_size > 0.0 &&
-           (p->method == KF_METHOD_REFERENCE || p->method == KF_METHOD_ADJOINT);
+           (p->method == KF_METHOD_REFERENCE || p->method == KF_METHOD_ADJOINT) &&
+           (p->sweep == KF_SWEEP_AUTO || p->sweep == KF_SWEEP_DENSE || p->sweep == KF_SWEEP_ACTIVE);
 }
 
 // Gauss-Jordan inverse with partial pivoting (row-major 4x4).
@@ -118,6 +120,21 @@ kf_status kf_ensure_partials(kf_context *ctx, size_t nblocks)
     }
     KF_HIP(hipMalloc((void **)&ctx->partials, 2 * sizeof(BlockPartial) * nblocks));
     ctx->partials_cap = nblocks;
+    return KF_OK;
+}
+
+kf_status kf_ensure_active_list(kf_context *ctx, size_t n)
+{
+    if (n <= ctx->active_cap)
+        return KF_OK;
+    if (ctx->active_list) {
+        KF_HIP(hipStreamSynchronize(ctx->stream));
+        KF_HIP(hipFree(ctx->active_list));
+        ctx->active_list = nullptr;
+        ctx->active_cap = 0;
+    }
+    KF_HIP(hipMalloc((void **)&ctx->active_list, sizeof(unsigned int) * n));
+    ctx->active_cap = n;
     return KF_OK;
 }
 
@@ -255,6 +272,8 @@ kf_status kf_context_destroy(kf_context *ctx)
         (void)hipHostFree(ctx->state_host);
     if (ctx->partials)
         (void)hipFree(ctx->partials);
+    if (ctx->active_list)
+        (void)hipFree(ctx->active_list);
     kf_batch_release(ctx);
     if (ctx->own_stream && ctx->stream)
         (void)hipStreamDestroy(ctx->stream);
@@ -636,7 +655,23 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
     const int cur0 = u->cur;
     const bool check = p->stop_max_update > 0.0;
     const bool adj = p->method == KF_METHOD_ADJOINT;
-    constexpr int CHUNK = 16;
+    // Active-set sweep (k_deform_active.hip): list the voxels that can move, then step only
+    // them.  The build also copies u into the second buffer, which the list steps rely on.
+    bool listed = false;
+    unsigned int count = 0;
+    if (adj && p->max_iters > 0 && p->sweep != KF_SWEEP_DENSE && kf_active_supported(live)) {
+        s = kf_ensure_active_list(ctx, (size_t)live->n);
+        if (s == KF_OK)
+            s = kf_launch_active_build(live, u->buf[cur0], u->buf[cur0 ^ 1], p, ctx->active_list);
+        if (s == KF_OK)
+            s = read_state(ctx);
+        if (s != KF_OK)
+            return s;
+        count = ctx->state_host->listed;
+        listed = p->sweep == KF_SWEEP_ACTIVE || (int64_t)count * 8 <= live->n;
+    }
+    // A list step is a few microseconds; check the stop rule less often so the host stays ahead.
+    const int CHUNK = listed ? 64 : 16;
     // Launch k reads buffer cur0 ^ (k & 1) and writes the other one.  An adjoint launch k folds
     // launch k - 1 in its first block, so launch k itself may run before the stop rule (or a
     // divergence) of k - 1 is known: it only writes the buffer that is not the result, and
@@ -646,8 +681,9 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
     for (int k = 0; k < p->max_iters; k++) {
         float *in = u->buf[cur0 ^ (k & 1)];
         float *out = u->buf[cur0 ^ ((k + 1) & 1)];
-        s = adj ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr, nullptr, k, &nb)
-                : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP, k);
+        s = listed ? kf_launch_adj_list(live, can, in, out, p, k, ctx->active_list, count, &nb)
+            : adj  ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr, nullptr, k, &nb)
+                   : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP, k);
         if (s != KF_OK)
             return s;
         launched = k + 1;
@@ -684,6 +720,8 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
         stats->max_update = h.last_max;
         stats->energy = energy;
         stats->active_voxels = h.last_active;
+        stats->sweep = listed ? KF_SWEEP_ACTIVE : KF_SWEEP_DENSE;
+        stats->listed_voxels = count;
     }
     if (h.diverged != LLONG_MAX) {
         kf_set_error("deformation field diverged at voxel " + std::to_string((long long)h.diverged));

@@ -48,7 +48,7 @@ struct DeformState {
     unsigned int done;          // set when the stop rule fires or a voxel diverged
     unsigned int iters;         // iterations completed (launches folded while not done)
     unsigned int run[2];        // adjoint step launch k runs iff run[k & 1]; launch k arms run[(k + 1) & 1]
-    unsigned int pad;
+    unsigned int listed;        // voxels in the active-set list of this deform call
     long long diverged;             // lowest index with a non-finite update, INT64_MAX if none
     long long div_slot[2];          // step launch k takes atomicMin into div_slot[k & 1]; the fold of
                                     // launch k moves it to `diverged` and re-arms it
@@ -80,6 +80,8 @@ struct kf_context {
     size_t partials_cap = 0;           // slots per set
     int num_cus = 256;
     struct kf_batch_cache *batch = nullptr;  // batched-deform workspace, tuned shapes, captured graph
+    unsigned int *active_list = nullptr;     // device: voxel indices of the active-set sweep
+    size_t active_cap = 0;                   // entries
 };
 
 struct kf_volume {
@@ -313,6 +315,17 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
                              float *u_out, const kf_params *p, int mode, float *e_voxel,
                              const unsigned int *mask_in = nullptr, int iter = 0,
                              unsigned int *nb_out = nullptr);
+// Active-set sweep (k_deform_active.hip).  kf_launch_active_build lists the voxels that can
+// move (count in state->listed; the state is reset first) and copies u into u_copy;
+// kf_launch_adj_list is one step over that list, with the same launch-to-launch folding as
+// kf_launch_adj_step.
+bool kf_active_supported(const kf_volume *live);
+kf_status kf_ensure_active_list(kf_context *ctx, size_t n);
+kf_status kf_launch_active_build(const kf_volume *live, const float *u, float *u_copy, const kf_params *p,
+                                 unsigned int *list);
+kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const float *u_in, float *u_out,
+                             const kf_params *p, int iter, const unsigned int *list, unsigned int count,
+                             unsigned int *nb_out);
 // Words of a nonzero mask for these dims (0: the row kernel does not apply).
 size_t kf_adj_mask_words(const int32_t dims[3]);
 // Computes the nonzero mask of field values u.

@@ -1,0 +1,167 @@
+"""Active-set sweep of the adjoint deform (kf_params.sweep) against the dense sweep, through the
+C ABI.
+
+A Jacobi step moves a voxel only when the live SDF at j + u_j lies in the band
+(KillingFusion.cpp:272-274), and with u_j = 0 that reads phi_live[j]; so the voxels with
+u_j = 0 and phi_live[j] outside the band never move, and the active-set sweep leaves them
+out.  Both sweeps run the same per-voxel arithmetic, so the fields (compared as bit
+patterns) and every statistic must agree exactly.  The dense sweep itself is held to the
+FP64 oracle in test_gpu_parity.py.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from killingfusion_amd import DisplacementField, KillingParams, compute_displacement_field, scenes
+from killingfusion_amd._lib import KF_E_DIVERGED, KFError
+
+from helpers import C1, make_pair, sphere_frames
+
+pytestmark = pytest.mark.gpu
+
+VOX = scenes.BATCH_VOXEL
+TRUNC = 3 * VOX
+
+
+def run(ctx, live, can, kp, sweep, u0=None):
+    f = DisplacementField(ctx, live.dims)
+    if u0 is not None:
+        f.upload(u0)
+    try:
+        st = compute_displacement_field(live, can, f, dataclasses.replace(kp, sweep=sweep))
+        return f.download(), st
+    finally:
+        f.close()
+
+
+def same_bits(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+
+
+def listed_on_cpu(live, u0, kp):
+    """The voxels that can move: u_j != 0 (sign bit ignored) or phi_live[j] in the band."""
+    phi, _ = live.download()
+    hi = np.float32(kp.max_surface - kp.epsilon)
+    lo = np.float32(-kp.clip_distance)
+    band = ~((phi > hi) | (phi < lo))
+    if u0 is None:
+        return int(band.sum())
+    moved = (np.asarray(u0, np.float32).view(np.uint32) & 0x7FFFFFFF).any(axis=0)
+    return int((band | moved).sum())
+
+
+def assert_same(a, b, tag):
+    (ua, sa), (ub, sb) = a, b
+    print(f"{tag}: iters {sb['iters']} listed {sb['listed']} active {sb['active']} max {sb['max_update']:.3e}",
+          flush=True)
+    assert sa["sweep"] == "dense" and sb["sweep"] == "active"
+    assert same_bits(ua, ub), f"{tag}: fields differ"
+    for k in ("iters", "max_update", "active", "energy", "diverged_index"):
+        assert sa[k] == sb[k], (tag, k, sa[k], sb[k])
+
+
+@pytest.fixture(scope="module")
+def sphere(ctx):
+    d1, d2 = sphere_frames()
+    out = {}
+    for dt in ("f32", "bf16", "f16"):
+        out[dt] = make_pair(ctx, C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"], d2, d1, dtype=dt)
+    return out
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f16"])
+def test_active_equals_dense_sphere(ctx, sphere, dtype):
+    print(f"case: 64^3 sphere pair, {dtype}, 60 adjoint iterations, dense vs active", flush=True)
+    live, can = sphere[dtype]
+    kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint", max_iters=60)
+    a = run(ctx, live, can, kp, "dense")
+    b = run(ctx, live, can, kp, "active")
+    assert_same(a, b, f"sphere {dtype}")
+    assert b[1]["listed"] == listed_on_cpu(live, None, kp)
+    assert 0 < b[1]["active"] <= b[1]["listed"] < live.n // 20
+
+
+@pytest.mark.parametrize("dims", [(61, 45, 58), (70, 45, 58), (48, 48, 48)])
+def test_active_equals_dense_uneven(ctx, dims):
+    """Odd and even shapes: N % 4 != 0 runs the dense one-voxel-per-thread kernel."""
+    print(f"case: ellipsoid pair {dims}, 6 mm, bf16, 50 iterations, dense vs active", flush=True)
+    ext = [d * VOX for d in dims]
+    d1, d2, _ = scenes.ellipsoid_pair(0x5EED, 3, max_axis=0.4 * min(ext))
+    live, can = make_pair(ctx, dims, (-ext[0] / 2, -ext[1] / 2, 0.8 - ext[2] / 2), VOX, TRUNC, d2, d1, "bf16")
+    kp = KillingParams.paper(VOX, 3.0, method="adjoint", max_iters=50)
+    assert_same(run(ctx, live, can, kp, "dense"), run(ctx, live, can, kp, "active"), f"ellipsoid {dims}")
+
+
+def test_active_with_initial_field(ctx, sphere):
+    """A nonzero starting field in a sub-box (voxels far from the surface move too), then a field
+    nonzero everywhere (every voxel listed; the automatic choice falls back to dense)."""
+    live, can = sphere["f32"]
+    dims = C1["dims"]
+    kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint", max_iters=30)
+    u0 = scenes.seeded_field(dims, amplitude=0.4, seed=7).reshape(3, dims[2], dims[1], dims[0])
+    part = np.zeros_like(u0)
+    part[:, 5:20, 30:50, 10:40] = u0[:, 5:20, 30:50, 10:40]
+    part[0, 0, 0, 0] = -0.0  # a negative zero does not list a voxel
+    part = part.reshape(3, -1)
+    print("case: 64^3 sphere, f32, initial field in a sub-box, 30 iterations", flush=True)
+    b = run(ctx, live, can, kp, "active", part)
+    assert_same(run(ctx, live, can, kp, "dense", part), b, "sub-box field")
+    assert b[1]["listed"] == listed_on_cpu(live, part, kp)
+    full = u0.reshape(3, -1)
+    print("case: 64^3 sphere, f32, initial field nonzero everywhere, 30 iterations", flush=True)
+    b = run(ctx, live, can, kp, "active", full)
+    assert_same(run(ctx, live, can, kp, "dense", full), b, "full field")
+    assert b[1]["listed"] == live.n
+    _, auto = run(ctx, live, can, kp, "auto", full)
+    assert auto["sweep"] == "dense" and auto["listed"] == live.n
+    _, auto = run(ctx, live, can, kp, "auto")
+    assert auto["sweep"] == "active"
+
+
+def test_active_stop_rule(ctx, sphere):
+    live, can = sphere["bf16"]
+    kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint", max_iters=400, stop_max_update=2e-2)
+    print("case: 64^3 sphere, bf16, stop rule 2e-2 voxel, dense vs active", flush=True)
+    a = run(ctx, live, can, kp, "dense")
+    b = run(ctx, live, can, kp, "active")
+    assert_same(a, b, "stop rule")
+    assert 2 <= b[1]["iters"] < 400 and b[1]["max_update"] < 2e-2
+
+
+def test_active_divergence(ctx, sphere):
+    """A NaN planted in the starting field at an in-band voxel: both sweeps stop with the same
+    lowest non-finite voxel."""
+    live, can = sphere["f32"]
+    kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint", max_iters=8)
+    phi, _ = live.download()
+    act = np.nonzero(np.abs(phi) < 0.5 * C1["trunc"])[0]
+    u0 = np.zeros((3, live.n), dtype=np.float32)
+    u0[2, int(act[len(act) // 2])] = np.nan
+    msgs = []
+    for sweep in ("dense", "active"):
+        print(f"case: 64^3 sphere, f32, NaN in the starting field, {sweep}", flush=True)
+        f = DisplacementField(ctx, C1["dims"])
+        f.upload(u0)
+        with pytest.raises(KFError) as e:
+            compute_displacement_field(live, can, f, dataclasses.replace(kp, sweep=sweep))
+        assert e.value.status == KF_E_DIVERGED
+        msgs.append(str(e.value))
+        f.close()
+    print(msgs, flush=True)
+    assert msgs[0] == msgs[1]
+
+
+def test_active_equals_dense_256(ctx):
+    """The benchmark's workload: 256^3 torus pair, 4 mm voxels, BF16, 100 iterations."""
+    print("case: 256^3 torus pair, bf16, 100 iterations, dense vs active", flush=True)
+    d_can, d_live = scenes.torus_pair()
+    dims = (256, 256, 256)
+    live, can = make_pair(ctx, dims, (-0.512, -0.512, 0.288), 0.004, 0.012, d_live, d_can, "bf16")
+    kp = KillingParams.paper(0.004, 3.0, method="adjoint", max_iters=100)
+    a = run(ctx, live, can, kp, "dense")
+    b = run(ctx, live, can, kp, "active")
+    assert_same(a, b, "torus 256^3")
+    assert b[1]["listed"] == listed_on_cpu(live, None, kp)
+    live.close()
+    can.close()
