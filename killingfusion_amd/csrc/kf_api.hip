@@ -123,21 +123,6 @@ kf_status kf_ensure_partials(kf_context *ctx, size_t nblocks)
     return KF_OK;
 }
 
-kf_status kf_ensure_active_list(kf_context *ctx, size_t n)
-{
-    if (n <= ctx->active_cap)
-        return KF_OK;
-    if (ctx->active_list) {
-        KF_HIP(hipStreamSynchronize(ctx->stream));
-        KF_HIP(hipFree(ctx->active_list));
-        ctx->active_list = nullptr;
-        ctx->active_cap = 0;
-    }
-    KF_HIP(hipMalloc((void **)&ctx->active_list, sizeof(unsigned int) * n));
-    ctx->active_cap = n;
-    return KF_OK;
-}
-
 namespace {
 
 constexpr int FIN_BLOCK = 1024;
@@ -272,8 +257,7 @@ kf_status kf_context_destroy(kf_context *ctx)
         (void)hipHostFree(ctx->state_host);
     if (ctx->partials)
         (void)hipFree(ctx->partials);
-    if (ctx->active_list)
-        (void)hipFree(ctx->active_list);
+    kf_active_release(ctx);
     kf_batch_release(ctx);
     if (ctx->own_stream && ctx->stream)
         (void)hipStreamDestroy(ctx->stream);
@@ -655,20 +639,20 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
     const int cur0 = u->cur;
     const bool check = p->stop_max_update > 0.0;
     const bool adj = p->method == KF_METHOD_ADJOINT;
-    // Active-set sweep (k_deform_active.hip): list the voxels that can move, then step only
-    // them.  The build also copies u into the second buffer, which the list steps rely on.
+    // Active-set sweep (k_deform_active.hip): count the voxels that can move, list them, step
+    // only them (in compact arrays) and write the result back into the field's buffer.
     bool listed = false;
     unsigned int count = 0;
     if (adj && p->max_iters > 0 && p->sweep != KF_SWEEP_DENSE && kf_active_supported(live)) {
-        s = kf_ensure_active_list(ctx, (size_t)live->n);
-        if (s == KF_OK)
-            s = kf_launch_active_build(live, u->buf[cur0], u->buf[cur0 ^ 1], p, ctx->active_list);
+        s = kf_active_count(live, u->buf[cur0], p);
         if (s == KF_OK)
             s = read_state(ctx);
         if (s != KF_OK)
             return s;
         count = ctx->state_host->listed;
         listed = p->sweep == KF_SWEEP_ACTIVE || (int64_t)count * 8 <= live->n;
+        if (listed && (s = kf_active_prepare(live, u->buf[cur0], count)) != KF_OK)
+            return s;
     }
     // A list step is a few microseconds; check the stop rule less often so the host stays ahead.
     const int CHUNK = listed ? 64 : 16;
@@ -681,7 +665,7 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
     for (int k = 0; k < p->max_iters; k++) {
         float *in = u->buf[cur0 ^ (k & 1)];
         float *out = u->buf[cur0 ^ ((k + 1) & 1)];
-        s = listed ? kf_launch_adj_list(live, can, in, out, p, k, ctx->active_list, count, &nb)
+        s = listed ? kf_launch_adj_list(live, can, p, k, count, &nb)
             : adj  ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr, nullptr, k, &nb)
                    : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP, k);
         if (s != KF_OK)
@@ -704,12 +688,26 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
     if (s != KF_OK)
         return s;
     const DeformState h = *ctx->state_host;
-    u->cur = cur0 ^ (int)(h.iters & 1u);
+    if (listed) {
+        s = kf_active_scatter(live, u->buf[cur0], count, h.iters);
+        if (s != KF_OK)
+            return s;
+    } else {
+        u->cur = cur0 ^ (int)(h.iters & 1u);
+    }
     double energy = 0.0;
     if (p->method == KF_METHOD_ADJOINT && h.diverged == LLONG_MAX) {
         // The step kernels skip the energy of voxels that do not move; one forward pass on the
-        // final field gives the energy the optimiser reached.
-        s = kf_energy_forward(live, can, u, p, &energy, nullptr);
+        // final field gives the energy the optimiser reached (over the list, when there is one).
+        if (listed) {
+            s = kf_active_energy(live, can, p, count, h.iters);
+            if (s == KF_OK)
+                s = read_state(ctx);
+            if (s == KF_OK)
+                energy = ctx->state_host->energy;
+        } else {
+            s = kf_energy_forward(live, can, u, p, &energy, nullptr);
+        }
         if (s != KF_OK)
             return s;
     }

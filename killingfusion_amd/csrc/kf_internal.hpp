@@ -70,6 +70,20 @@ struct BlockPartial {
 // Host handles
 // ---------------------------------------------------------------------------
 
+// Device workspace of the active-set sweep, grown on demand and kept by the context.
+struct kf_active_ws {
+    unsigned long long *mask = nullptr;  // one bit per voxel: listed
+    size_t mask_cap = 0;                 // words
+    unsigned int *span_count = nullptr;  // listed voxels per span of the marking pass, then span offsets
+    size_t span_cap = 0;
+    unsigned int *pos_of = nullptr;      // list slot of each listed voxel (other entries unused)
+    size_t pos_cap = 0;
+    unsigned int *list = nullptr;        // listed voxel indices, ascending
+    float4 *u = nullptr;                 // two compact sets of u, [2][cap] (x, y, z, 0)
+    int4 *nbr = nullptr;                 // [cap][3]: list slots of the 12 Killing-term neighbours, -1 = +0
+    size_t cap = 0;                      // list slots
+};
+
 struct kf_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -80,8 +94,7 @@ struct kf_context {
     size_t partials_cap = 0;           // slots per set
     int num_cus = 256;
     struct kf_batch_cache *batch = nullptr;  // batched-deform workspace, tuned shapes, captured graph
-    unsigned int *active_list = nullptr;     // device: voxel indices of the active-set sweep
-    size_t active_cap = 0;                   // entries
+    kf_active_ws active;                     // active-set sweep workspace (k_deform_active.hip)
 };
 
 struct kf_volume {
@@ -315,17 +328,21 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
                              float *u_out, const kf_params *p, int mode, float *e_voxel,
                              const unsigned int *mask_in = nullptr, int iter = 0,
                              unsigned int *nb_out = nullptr);
-// Active-set sweep (k_deform_active.hip).  kf_launch_active_build lists the voxels that can
-// move (count in state->listed; the state is reset first) and copies u into u_copy;
-// kf_launch_adj_list is one step over that list, with the same launch-to-launch folding as
-// kf_launch_adj_step.
+// Active-set sweep (k_deform_active.hip), one deform call: kf_active_count lists nothing yet but
+// leaves the number of voxels that can move in state->listed (reset the state first);
+// kf_active_prepare lists them and copies u into compact set 0; kf_launch_adj_list is step
+// launch `iter` (compact set iter & 1 -> the other), with the same launch-to-launch folding
+// as kf_launch_adj_step; kf_active_scatter writes compact set iters & 1 back into u.
 bool kf_active_supported(const kf_volume *live);
-kf_status kf_ensure_active_list(kf_context *ctx, size_t n);
-kf_status kf_launch_active_build(const kf_volume *live, const float *u, float *u_copy, const kf_params *p,
-                                 unsigned int *list);
-kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const float *u_in, float *u_out,
-                             const kf_params *p, int iter, const unsigned int *list, unsigned int count,
-                             unsigned int *nb_out);
+void kf_active_release(kf_context *ctx);
+kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params *p);
+kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int count);
+kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const kf_params *p, int iter,
+                             unsigned int count, unsigned int *nb_out);
+kf_status kf_active_scatter(const kf_volume *live, float *u, unsigned int count, unsigned int iters);
+// Energy of compact set iters & 1 into state->energy (resets the state first).
+kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count,
+                           unsigned int iters);
 // Words of a nonzero mask for these dims (0: the row kernel does not apply).
 size_t kf_adj_mask_words(const int32_t dims[3]);
 // Computes the nonzero mask of field values u.
