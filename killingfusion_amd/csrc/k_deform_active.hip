@@ -27,6 +27,10 @@
 // has the same per-voxel terms, summed in FP64 in another order.
 #include "kf_adjoint.hpp"
 
+#include <cstring>
+#include <string>
+#include <vector>
+
 namespace {
 
 constexpr int LB = 64;                 // threads per block of the list step (one wavefront)
@@ -123,7 +127,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_active_scan(const unsigned int *
 // Thread k of block b lists the voxels of mask word b WPB + k, in voxel order.
 __global__ __launch_bounds__(WPB) void k_active_compact(const unsigned long long *__restrict__ mask, unsigned int nwords,
                                                         const unsigned int *__restrict__ span_count,
-                                                        const unsigned int *__restrict__ span_off,
+                                                        const unsigned int *__restrict__ span_off, unsigned int base,
                                                         unsigned int *__restrict__ list, unsigned int *__restrict__ pos_of)
 {
     if (span_count[blockIdx.x] == 0u)
@@ -146,7 +150,7 @@ __global__ __launch_bounds__(WPB) void k_active_compact(const unsigned long long
     unsigned int wbase = 0;
     for (int k = 0; k < w; k++)
         wbase += s_w[k];
-    unsigned int pos = span_off[blockIdx.x] + wbase + v - c;
+    unsigned int pos = base + span_off[blockIdx.x] + wbase + v - c;
     while (m) {
         const unsigned int i = wd * 64u + (unsigned int)__builtin_ctzll(m);
         m &= m - 1ull;
@@ -156,16 +160,17 @@ __global__ __launch_bounds__(WPB) void k_active_compact(const unsigned long long
     }
 }
 
-// Slot t: u of the listed voxel (compact set 0) and the list slots of its 12 neighbours.
+// Slot t (base .. base + count - 1): u of the listed voxel (compact set 0) and the list slots
+// of its 12 neighbours.
 __global__ __launch_bounds__(BLOCK) void k_active_neighbours(GridDims g, const unsigned long long *__restrict__ mask,
                                                              const unsigned int *__restrict__ pos_of,
-                                                             const unsigned int *__restrict__ list, unsigned int count,
-                                                             const float *__restrict__ u, float4 *__restrict__ uc,
-                                                             int4 *__restrict__ nbr)
+                                                             const unsigned int *__restrict__ list, unsigned int base,
+                                                             unsigned int count, const float *__restrict__ u,
+                                                             float4 *__restrict__ uc, int4 *__restrict__ nbr)
 {
-    const unsigned int t = blockIdx.x * (unsigned int)BLOCK + threadIdx.x;
-    if (t >= count)
+    if (blockIdx.x * (unsigned int)BLOCK + threadIdx.x >= count)
         return;
+    const unsigned int t = base + blockIdx.x * (unsigned int)BLOCK + threadIdx.x;
     const unsigned int i = list[t];
     const unsigned int n32 = nvox32(g);
     uc[t] = make_float4(u[i], u[n32 + i], u[2u * n32 + i], 0.0f);
@@ -294,6 +299,109 @@ __device__ inline void voxel_xyz(const GridDims &g, unsigned int i, int &x, int 
     x = r - y * g.nx;
 }
 
+// Round 2 and the update of listed slot t of voxel i, given its own u and neighbour slots:
+// the live cell around i + u_i, phi_can[i], the neighbours' u, then adj_voxel / adj_update.
+template <class S>
+__device__ inline void list_step(const PhiRead<S> &L, const PhiRead<S> &Cn, const float4 *__restrict__ uin,
+                                 float4 *__restrict__ uout, const AdjConsts &c, long long *div, unsigned int t,
+                                 unsigned int i, const float4 &own, const int ns[NNB], double &mx,
+                                 unsigned long long &act)
+{
+    int x, y, z, bx, by, bz;
+    voxel_xyz(L.g, i, x, y, z);
+    Nbhd nb;
+    nb.uc[0] = own.x;
+    nb.uc[1] = own.y;
+    nb.uc[2] = own.z;
+    float fx, fy, fz, v[8];
+    kf_split(x, nb.uc[0], bx, fx);
+    kf_split(y, nb.uc[1], by, fy);
+    kf_split(z, nb.uc[2], bz, fz);
+    cell32(L, bx, by, bz, v);
+    const float can = c.use_data ? S::load(Cn.p, (int64_t)i, Cn.scale) : 0.f;
+    list_nbhd(uin, ns, nb);
+    VoxelOut o;
+    adj_voxel<false>(c, v, fx, fy, fz, can, nb, x, y, z, o);
+    if (o.active)
+        act = 1;
+    float n[3];
+    adj_update(c, o, nb.uc, (int64_t)i, div, mx, n);
+    uout[t] = make_float4(n[0], n[1], n[2], 0.0f);
+}
+
+// Energy of field `uc` (see the file header of k_deform_adj.hip for E) owned by slot t: the
+// term of its listed voxel j and of each unlisted voxel i = j - e_b (in the grid) for which j
+// is the first listed voxel among i + e_0, i + e_1, i + e_2 -- so every unlisted voxel next to
+// a listed one is counted once.  An unlisted voxel holds u = +0 and phi_live[i] outside the
+// band (it is inactive), so only its Killing term is nonzero, and that reads u at i + e_b only.
+template <class S>
+__device__ inline double list_energy(const PhiRead<S> &L, const PhiRead<S> &Cn, const float4 *__restrict__ uc,
+                                     const AdjConsts &c, unsigned int t, unsigned int i, const float4 &own,
+                                     const int ns[NNB])
+{
+    int x, y, z, bx, by, bz;
+    voxel_xyz(L.g, i, x, y, z);
+    Nbhd nb;
+    nb.uc[0] = own.x;
+    nb.uc[1] = own.y;
+    nb.uc[2] = own.z;
+    float fx, fy, fz, v[8];
+    kf_split(x, nb.uc[0], bx, fx);
+    kf_split(y, nb.uc[1], by, fy);
+    kf_split(z, nb.uc[2], bz, fz);
+    cell32(L, bx, by, bz, v);
+    const float can = c.use_data ? S::load(Cn.p, (int64_t)i, Cn.scale) : 0.f;
+    list_nbhd(uc, ns, nb);
+    VoxelOut o;
+    adj_voxel<true>(c, v, fx, fy, fz, can, nb, x, y, z, o);
+    double esum = (double)o.e;
+    // Slots: faces -x -y -z = 1, 3, 5; the edge slots i - e_b + e_b' are
+    // (b, b') = (1, 0): 6, (2, 0): 7, (2, 1): 9, (0, 1): 8, (0, 2): 10, (1, 2): 11.
+    const int jj[3] = {x, y, z};
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        if (jj[b] == 0 || ns[2 * b + 1] >= 0)
+            continue;  // i outside the grid, or listed
+        const bool earlier = b == 1 ? ns[6] >= 0 : b == 2 ? (ns[7] >= 0 || ns[9] >= 0) : false;
+        if (earlier)
+            continue;  // counted by the listed voxel i + e_b', b' < b
+        // u at i + e_0, i + e_1, i + e_2
+        int sl[3];
+        if (b == 0) {
+            sl[0] = (int)t; sl[1] = ns[8]; sl[2] = ns[10];
+        } else if (b == 1) {
+            sl[0] = ns[6]; sl[1] = (int)t; sl[2] = ns[11];
+        } else {
+            sl[0] = ns[7]; sl[1] = ns[9]; sl[2] = (int)t;
+        }
+        Nbhd ni;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const float4 q = sl[d] >= 0 ? uc[sl[d]] : zero;
+            ni.uc[d] = 0.0f;
+            ni.up[d][0] = q.x;
+            ni.up[d][1] = q.y;
+            ni.up[d][2] = q.z;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                ni.um[d][a] = 0.0f;
+                ni.dgl[d][a] = 0.0f;
+            }
+        }
+        const int xi = x - (b == 0), yi = y - (b == 1), zi = z - (b == 2);
+        const float li = S::load(L.p, (int64_t)L.g.idx32(xi, yi, zi), L.scale);
+        float vi[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            vi[k] = li;
+        VoxelOut oi;
+        adj_voxel<true>(c, vi, 0.0f, 0.0f, 0.0f, 0.0f, ni, xi, yi, zi, oi);
+        esum += (double)oi.e;
+    }
+    return esum;
+}
+
 // One Jacobi iteration over the list.  Block 0 only folds the previous launch and arms the
 // next one (step_gate's first-block duty), beside the stencils rather than in front of one.
 // A work block loads its list entries, own u and neighbour slots together with the gate flag
@@ -325,36 +433,11 @@ __global__ __launch_bounds__(LB) void k_adj_list(PhiRead<S> L, PhiRead<S> Cn, co
         return;
     double mx = 0.0;
     unsigned long long act = 0;
-    if (mine) {
-        int x, y, z, bx, by, bz;
-        voxel_xyz(L.g, i, x, y, z);
-        Nbhd nb;
-        nb.uc[0] = own.x;
-        nb.uc[1] = own.y;
-        nb.uc[2] = own.z;
-        float fx, fy, fz, v[8];
-        kf_split(x, nb.uc[0], bx, fx);
-        kf_split(y, nb.uc[1], by, fy);
-        kf_split(z, nb.uc[2], bz, fz);
-        cell32(L, bx, by, bz, v);
-        const float can = c.use_data ? S::load(Cn.p, (int64_t)i, Cn.scale) : 0.f;
-        list_nbhd(uin, ns, nb);
-        VoxelOut o;
-        adj_voxel<false>(c, v, fx, fy, fz, can, nb, x, y, z, o);
-        if (o.active)
-            act = 1;
-        float n[3];
-        adj_update(c, o, nb.uc, (int64_t)i, &st->div_slot[parity], mx, n);
-        uout[t] = make_float4(n[0], n[1], n[2], 0.0f);
-    }
+    if (mine)
+        list_step(L, Cn, uin, uout, c, &st->div_slot[parity], t, i, own, ns, mx, act);
     kf_block_publish<LB>(partials, w, 0.0, mx, act);
 }
 
-// Energy of field `uc` (see the file header of k_deform_adj.hip for E): slot t adds the term
-// of its listed voxel j and of each unlisted voxel i = j - e_b (in the grid) for which j is
-// the first listed voxel among i + e_0, i + e_1, i + e_2 -- so every unlisted voxel next to a
-// listed one is counted once.  An unlisted voxel holds u = +0 and phi_live[i] outside the band
-// (it is inactive), so only its Killing term is nonzero, and that reads u at i + e_b only.
 template <class S>
 __global__ __launch_bounds__(LB) void k_active_energy(PhiRead<S> L, PhiRead<S> Cn, const float4 *__restrict__ uc,
                                                       AdjConsts c, BlockPartial *partials,
@@ -364,74 +447,136 @@ __global__ __launch_bounds__(LB) void k_active_energy(PhiRead<S> L, PhiRead<S> C
     const unsigned int t = kf_xcd_tile(blockIdx.x, gridDim.x) * (unsigned int)LB + threadIdx.x;
     double esum = 0.0;
     if (t < count) {
-        const unsigned int i = list[t];
-        const float4 own = uc[t];
         int ns[NNB];
         load_slots(nbr, t, ns);
-        int x, y, z, bx, by, bz;
-        voxel_xyz(L.g, i, x, y, z);
-        Nbhd nb;
-        nb.uc[0] = own.x;
-        nb.uc[1] = own.y;
-        nb.uc[2] = own.z;
-        float fx, fy, fz, v[8];
-        kf_split(x, nb.uc[0], bx, fx);
-        kf_split(y, nb.uc[1], by, fy);
-        kf_split(z, nb.uc[2], bz, fz);
-        cell32(L, bx, by, bz, v);
-        const float can = c.use_data ? S::load(Cn.p, (int64_t)i, Cn.scale) : 0.f;
-        list_nbhd(uc, ns, nb);
-        VoxelOut o;
-        adj_voxel<true>(c, v, fx, fy, fz, can, nb, x, y, z, o);
-        esum += (double)o.e;
-        // unlisted neighbours on the low side: i - e_b, owned when no i - e_b + e_b' (b' < b)
-        // is listed.  Slots: faces -x -y -z = 1, 3, 5; i - e_b + e_b' are edge slots
-        // (b, b') = (1, 0): 6, (2, 0): 7, (2, 1): 9; i - e_b + e_b'' (b'' > b): (0, 1): 8,
-        // (0, 2): 10, (1, 2): 11.
-        const int jj[3] = {x, y, z};
-        const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int b = 0; b < 3; b++) {
-            if (jj[b] == 0 || ns[2 * b + 1] >= 0)
-                continue;  // i outside the grid, or listed
-            const bool earlier = b == 1 ? ns[6] >= 0 : b == 2 ? (ns[7] >= 0 || ns[9] >= 0) : false;
-            if (earlier)
-                continue;  // counted by the listed voxel i + e_b', b' < b
-            // u at i + e_0, i + e_1, i + e_2
-            int sl[3];
-            if (b == 0) {
-                sl[0] = (int)t; sl[1] = ns[8]; sl[2] = ns[10];
-            } else if (b == 1) {
-                sl[0] = ns[6]; sl[1] = (int)t; sl[2] = ns[11];
-            } else {
-                sl[0] = ns[7]; sl[1] = ns[9]; sl[2] = (int)t;
-            }
-            Nbhd ni;
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const float4 q = sl[d] >= 0 ? uc[sl[d]] : zero;
-                ni.uc[d] = 0.0f;
-                ni.up[d][0] = q.x;
-                ni.up[d][1] = q.y;
-                ni.up[d][2] = q.z;
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    ni.um[d][a] = 0.0f;
-                    ni.dgl[d][a] = 0.0f;
-                }
-            }
-            const int xi = x - (b == 0), yi = y - (b == 1), zi = z - (b == 2);
-            const float li = S::load(L.p, (int64_t)L.g.idx32(xi, yi, zi), L.scale);
-            float vi[8];
-#pragma unroll
-            for (int k = 0; k < 8; k++)
-                vi[k] = li;
-            VoxelOut oi;
-            adj_voxel<true>(c, vi, 0.0f, 0.0f, 0.0f, 0.0f, ni, xi, yi, zi, oi);
-            esum += (double)oi.e;
-        }
+        esum = list_energy(L, Cn, uc, c, t, list[t], uc[t], ns);
     }
     kf_block_publish<LB>(partials, blockIdx.x, esum, 0.0, 0ull);
+}
+
+// ---------------------------------------------------------------------------
+// Batched active-set sweep: the lists of several volumes in one slot space.  Volume v owns
+// work blocks blk0 .. blk0 + nblk - 1 and their slots (LB per block; slots past the volume's
+// count hold the index NONE); each volume has its own iteration state, gate and partials.
+// ---------------------------------------------------------------------------
+constexpr unsigned int NONE = 0xffffffffu;
+
+template <class S>
+struct ActiveVol {
+    PhiRead<S> L, Cn;
+    float *u;            // the field's current buffer (SoA [3][n])
+    unsigned int blk0;   // first work block
+    unsigned int nblk;   // work blocks (at least one)
+};
+
+// Launch k: blocks 0 .. nvol - 1 fold launch k - 1 of their volume (none when first) and arm
+// launch k + 1; the other blocks are work blocks.
+template <class S>
+__global__ __launch_bounds__(LB) void k_adj_list_batch(const ActiveVol<S> *__restrict__ vols, int nvol,
+                                                       const unsigned int *__restrict__ blk_vol,
+                                                       const float4 *__restrict__ uin, float4 *__restrict__ uout,
+                                                       AdjConsts c, DeformState *states, BlockPartial *partials,
+                                                       int parity, const BlockPartial *prev, int first,
+                                                       const unsigned int *__restrict__ list,
+                                                       const int4 *__restrict__ nbr)
+{
+    if ((int)blockIdx.x < nvol) {
+        const unsigned int b0 = vols[blockIdx.x].blk0, nb = vols[blockIdx.x].nblk;
+        list_fold<LB>(states + blockIdx.x, prev + b0, first ? 0u : nb, parity);
+        return;
+    }
+    const unsigned int w = kf_xcd_tile(blockIdx.x - (unsigned int)nvol, gridDim.x - (unsigned int)nvol);
+    const unsigned int t = w * (unsigned int)LB + threadIdx.x;
+    const unsigned int i = list[t];
+    const float4 own = uin[t];
+    int ns[NNB];
+    load_slots(nbr, t, ns);
+    const unsigned int v = blk_vol[w];
+    DeformState *st = states + v;
+    if (st->run[parity] == 0u)
+        return;
+    double mx = 0.0;
+    unsigned long long act = 0;
+    if (i != NONE)
+        list_step(vols[v].L, vols[v].Cn, uin, uout, c, &st->div_slot[parity], t, i, own, ns, mx, act);
+    kf_block_publish<LB>(partials, w, 0.0, mx, act);
+}
+
+// Energy of each volume's final compact set (iters & 1 of its state) into partials, one per
+// work block.
+template <class S>
+__global__ __launch_bounds__(LB) void k_active_energy_batch(const ActiveVol<S> *__restrict__ vols,
+                                                            const unsigned int *__restrict__ blk_vol,
+                                                            const float4 *__restrict__ u0, unsigned int set,
+                                                            AdjConsts c, const DeformState *states,
+                                                            BlockPartial *partials, const unsigned int *__restrict__ list,
+                                                            const int4 *__restrict__ nbr)
+{
+    const unsigned int w = kf_xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned int t = w * (unsigned int)LB + threadIdx.x;
+    const unsigned int v = blk_vol[w];
+    const float4 *uc = u0 + (size_t)(states[v].iters & 1u) * set;
+    const unsigned int i = list[t];
+    double esum = 0.0;
+    if (i != NONE) {
+        int ns[NNB];
+        load_slots(nbr, t, ns);
+        esum = list_energy(vols[v].L, vols[v].Cn, uc, c, t, i, uc[t], ns);
+    }
+    kf_block_publish<LB>(partials, w, esum, 0.0, 0ull);
+}
+
+// Block v: the energy partials of volume v in slot order -> energy[v].
+template <class S>
+__global__ __launch_bounds__(LB) void k_energy_fold_batch(const ActiveVol<S> *__restrict__ vols,
+                                                          const BlockPartial *__restrict__ partials,
+                                                          double *__restrict__ energy)
+{
+    const unsigned int b0 = vols[blockIdx.x].blk0, nb = vols[blockIdx.x].nblk;
+    double s = 0.0, m = 0.0;
+    unsigned long long c = 0;
+    for (unsigned int b = threadIdx.x; b < nb; b += LB)
+        s += partials[b0 + b].sum;
+    kf_block_reduce3<LB>(s, m, c);
+    if (threadIdx.x == 0)
+        energy[blockIdx.x] = s;
+}
+
+// Each volume's final compact set back into its field.
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_active_scatter_batch(const ActiveVol<S> *__restrict__ vols,
+                                                                const unsigned int *__restrict__ blk_vol,
+                                                                const float4 *__restrict__ u0, unsigned int set,
+                                                                const DeformState *states,
+                                                                const unsigned int *__restrict__ list,
+                                                                unsigned int slots)
+{
+    const unsigned int t = blockIdx.x * (unsigned int)BLOCK + threadIdx.x;
+    if (t >= slots)
+        return;
+    const unsigned int i = list[t];
+    if (i == NONE)
+        return;
+    const unsigned int v = blk_vol[t / (unsigned int)LB];
+    const float4 q = u0[(size_t)(states[v].iters & 1u) * set + t];
+    float *u = vols[v].u;
+    const unsigned int n32 = nvox32(vols[v].L.g);
+    u[i] = q.x;
+    u[n32 + i] = q.y;
+    u[2u * n32 + i] = q.z;
+}
+
+__global__ void k_states_reset(DeformState *st, int n, double stop)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n)
+        return;
+    DeformState s{};
+    s.run[0] = s.run[1] = 1u;
+    s.diverged = LLONG_MAX;
+    s.div_slot[0] = s.div_slot[1] = LLONG_MAX;
+    s.stop = stop;
+    st[v] = s;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_active_scatter(const unsigned int *__restrict__ list,
@@ -477,6 +622,12 @@ void kf_active_release(kf_context *ctx)
     (void)hipFree(a.list);
     (void)hipFree(a.u);
     (void)hipFree(a.nbr);
+    (void)hipFree(a.bstates);
+    (void)hipFree(a.benergy);
+    (void)hipFree(a.blk_vol);
+    (void)hipFree(a.vtab);
+    if (a.bstates_host)
+        (void)hipHostFree(a.bstates_host);
     a = kf_active_ws{};
 }
 
@@ -523,11 +674,11 @@ kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int 
     if (s != KF_OK)
         return s;
     hipLaunchKernelGGL(k_active_compact, dim3(nspan), dim3(WPB), 0, ctx->stream, a.mask, nwords, a.span_count,
-                       a.span_count + nspan, a.list, a.pos_of);
+                       a.span_count + nspan, 0u, a.list, a.pos_of);
     KF_LAUNCH_CHECK();
     if (count > 0) {
         hipLaunchKernelGGL(k_active_neighbours, dim3((count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream,
-                           grid_of(live), a.mask, a.pos_of, a.list, count, u, a.u, a.nbr);
+                           grid_of(live), a.mask, a.pos_of, a.list, 0u, count, u, a.u, a.nbr);
         KF_LAUNCH_CHECK();
     }
     return KF_OK;
@@ -594,5 +745,207 @@ kf_status kf_active_scatter(const kf_volume *live, float *u, unsigned int count,
     hipLaunchKernelGGL(k_active_scatter, dim3((count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, a.list, uc,
                        count, u, (unsigned int)live->n);
     KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
+namespace {
+
+template <class S>
+kf_status batch_active(kf_context *ctx, const kf_volume *const *live, const kf_volume *const *can,
+                       kf_field *const *u, int count, const kf_params *p, kf_deform_stats *stats, bool force,
+                       bool *used)
+{
+    kf_active_ws &a = ctx->active;
+    hipStream_t st = ctx->stream;
+    *used = false;
+    std::vector<size_t> moff(count), soff(count);
+    std::vector<unsigned int> nspan(count);
+    size_t mtot = 0, stot = 0;
+    int64_t nmax = 0, nvox = 0;
+    for (int v = 0; v < count; v++) {
+        nspan[v] = (unsigned int)((live[v]->n + SPAN - 1) / SPAN);
+        moff[v] = mtot;
+        mtot += (size_t)nspan[v] * WPB;
+        soff[v] = stot;
+        stot += 2 * (size_t)nspan[v];
+        nmax = live[v]->n > nmax ? live[v]->n : nmax;
+        nvox += live[v]->n;
+    }
+    kf_status s = grow(a.mask, a.mask_cap, mtot, st);
+    if (s == KF_OK)
+        s = grow(a.span_count, a.span_cap, stot, st);
+    if (s == KF_OK && (size_t)count > a.bcap) {
+        if (a.bstates_host) {
+            KF_HIP(hipStreamSynchronize(st));
+            (void)hipHostFree(a.bstates_host);
+            a.bstates_host = nullptr;
+        }
+        size_t c1 = a.bcap, c2 = a.bcap;
+        s = grow(a.bstates, c1, (size_t)count, st);
+        if (s == KF_OK)
+            s = grow(a.benergy, c2, (size_t)count, st);
+        if (s == KF_OK && hipHostMalloc((void **)&a.bstates_host, (sizeof(DeformState) + sizeof(double)) * count,
+                                        hipHostMallocDefault) != hipSuccess) {
+            kf_set_error("pinned state allocation failed");
+            s = KF_E_OOM;
+        }
+        a.bcap = s == KF_OK ? (size_t)count : 0;
+    }
+    if (s != KF_OK)
+        return s;
+    DeformState *sh = a.bstates_host;
+    double *eh = reinterpret_cast<double *>(sh + count);
+    const AdjConsts c = kf_adj_consts(p);
+    hipLaunchKernelGGL(k_states_reset, dim3((count + 63) / 64), dim3(64), 0, st, a.bstates, count, p->stop_max_update);
+    KF_LAUNCH_CHECK();
+    for (int v = 0; v < count; v++) {
+        hipLaunchKernelGGL((k_active_mark<S>), dim3(nspan[v]), dim3(BLOCK), 0, st, make_read<S>(live[v]),
+                           u[v]->u(), c.lo, c.hi, a.mask + moff[v], a.span_count + soff[v]);
+        KF_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_active_scan, dim3(1), dim3(SCAN_BLOCK), 0, st, a.span_count + soff[v], nspan[v],
+                           a.span_count + soff[v] + nspan[v], a.bstates + v);
+        KF_LAUNCH_CHECK();
+    }
+    KF_HIP(hipMemcpyAsync(sh, a.bstates, sizeof(DeformState) * count, hipMemcpyDeviceToHost, st));
+    KF_HIP(hipStreamSynchronize(st));
+    std::vector<unsigned int> cnt(count), blk0(count), nblk(count);
+    int64_t listed = 0;
+    unsigned int nwb = 0;
+    for (int v = 0; v < count; v++) {
+        cnt[v] = sh[v].listed;
+        listed += cnt[v];
+        blk0[v] = nwb;
+        nblk[v] = cnt[v] ? (cnt[v] + LB - 1) / LB : 1u;
+        nwb += nblk[v];
+    }
+    if (!force && listed * 8 > nvox)
+        return KF_OK;  // the dense batched sweep is cheaper
+    *used = true;
+    const size_t slots = (size_t)nwb * LB;
+    if (slots > a.cap) {
+        size_t c1 = a.cap, c2 = 2 * a.cap, c3 = 3 * a.cap;
+        s = grow(a.list, c1, slots, st);
+        if (s == KF_OK)
+            s = grow(a.u, c2, 2 * slots, st);
+        if (s == KF_OK)
+            s = grow(a.nbr, c3, 3 * slots, st);
+        a.cap = s == KF_OK ? slots : 0;
+    }
+    if (s == KF_OK)
+        s = grow(a.pos_of, a.pos_cap, (size_t)nmax, st);
+    if (s == KF_OK)
+        s = grow(a.blk_vol, a.blk_cap, (size_t)nwb, st);
+    if (s == KF_OK)
+        s = grow(a.vtab, a.vtab_cap, sizeof(ActiveVol<S>) * count, st);
+    if (s == KF_OK)
+        s = kf_ensure_partials(ctx, nwb);
+    if (s != KF_OK)
+        return s;
+    std::vector<ActiveVol<S>> tab(count);
+    std::vector<unsigned int> bv(nwb);
+    for (int v = 0; v < count; v++) {
+        tab[v].L = make_read<S>(live[v]);
+        tab[v].Cn = make_read<S>(can[v]);
+        tab[v].u = u[v]->u();
+        tab[v].blk0 = blk0[v];
+        tab[v].nblk = nblk[v];
+        for (unsigned int b = 0; b < nblk[v]; b++)
+            bv[blk0[v] + b] = (unsigned int)v;
+    }
+    const ActiveVol<S> *dtab = reinterpret_cast<const ActiveVol<S> *>(a.vtab);
+    KF_HIP(hipMemcpyAsync(a.vtab, tab.data(), sizeof(ActiveVol<S>) * count, hipMemcpyHostToDevice, st));
+    KF_HIP(hipMemcpyAsync(a.blk_vol, bv.data(), sizeof(unsigned int) * nwb, hipMemcpyHostToDevice, st));
+    KF_HIP(hipMemsetAsync(a.list, 0xff, sizeof(unsigned int) * slots, st));
+    for (int v = 0; v < count; v++) {
+        const unsigned int nwords = (unsigned int)((live[v]->n + 63) / 64);
+        hipLaunchKernelGGL(k_active_compact, dim3(nspan[v]), dim3(WPB), 0, st, a.mask + moff[v], nwords,
+                           a.span_count + soff[v], a.span_count + soff[v] + nspan[v], blk0[v] * (unsigned int)LB,
+                           a.list, a.pos_of);
+        KF_LAUNCH_CHECK();
+        if (cnt[v] > 0) {
+            hipLaunchKernelGGL(k_active_neighbours, dim3((cnt[v] + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st,
+                               grid_of(live[v]), a.mask + moff[v], a.pos_of, a.list, blk0[v] * (unsigned int)LB, cnt[v],
+                               u[v]->u(), a.u, a.nbr);
+            KF_LAUNCH_CHECK();
+        }
+    }
+    // The iterations: launch k reads compact set k & 1 and writes the other one; per-volume
+    // gates stop each volume on its own (stop rule or divergence).
+    const bool check = p->stop_max_update > 0.0;
+    const unsigned int set = (unsigned int)a.cap;
+    int launched = 0;
+    for (int k = 0; k < p->max_iters; k++) {
+        const int par = k & 1;
+        hipLaunchKernelGGL((k_adj_list_batch<S>), dim3((unsigned int)count + nwb), dim3(LB), 0, st, dtab, count,
+                           a.blk_vol, a.u + (size_t)par * set, a.u + (size_t)(par ^ 1) * set, c, a.bstates,
+                           kf_partials(ctx, par), par, kf_partials(ctx, par ^ 1), k == 0 ? 1 : 0, a.list, a.nbr);
+        KF_LAUNCH_CHECK();
+        launched = k + 1;
+        if (check && launched % 64 == 0 && launched < p->max_iters) {
+            KF_HIP(hipMemcpyAsync(sh, a.bstates, sizeof(DeformState) * count, hipMemcpyDeviceToHost, st));
+            KF_HIP(hipStreamSynchronize(st));
+            bool all = true;
+            for (int v = 0; v < count; v++)
+                all = all && sh[v].done;
+            if (all)
+                break;
+        }
+    }
+    if (launched > 0) {
+        // fold the last launch (fold blocks only)
+        const int par = launched & 1;
+        hipLaunchKernelGGL((k_adj_list_batch<S>), dim3((unsigned int)count), dim3(LB), 0, st, dtab, count, a.blk_vol,
+                           a.u, a.u, c, a.bstates, kf_partials(ctx, par), par, kf_partials(ctx, par ^ 1), 0, a.list,
+                           a.nbr);
+        KF_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL((k_active_scatter_batch<S>), dim3((unsigned int)((slots + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
+                       st, dtab, a.blk_vol, a.u, set, a.bstates, a.list, (unsigned int)slots);
+    KF_LAUNCH_CHECK();
+    if (stats) {
+        hipLaunchKernelGGL((k_active_energy_batch<S>), dim3(nwb), dim3(LB), 0, st, dtab, a.blk_vol, a.u, set, c,
+                           a.bstates, kf_partials(ctx, 0), a.list, a.nbr);
+        KF_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_energy_fold_batch<S>), dim3((unsigned int)count), dim3(LB), 0, st, dtab,
+                           kf_partials(ctx, 0), a.benergy);
+        KF_LAUNCH_CHECK();
+        KF_HIP(hipMemcpyAsync(eh, a.benergy, sizeof(double) * count, hipMemcpyDeviceToHost, st));
+    }
+    KF_HIP(hipMemcpyAsync(sh, a.bstates, sizeof(DeformState) * count, hipMemcpyDeviceToHost, st));
+    KF_HIP(hipStreamSynchronize(st));
+    kf_status worst = KF_OK;
+    std::string msg;
+    for (int v = 0; v < count; v++) {
+        const DeformState &h = sh[v];
+        if (h.diverged != LLONG_MAX) {
+            worst = KF_E_DIVERGED;
+            if (msg.empty())
+                msg = "volume " + std::to_string(v) + ": deformation field diverged at voxel " +
+                      std::to_string((long long)h.diverged);
+        }
+        if (stats) {
+            kf_deform_stats &o = stats[v];
+            std::memset(&o, 0, sizeof(o));
+            o.iters = (int32_t)h.iters;
+            o.sweep = KF_SWEEP_ACTIVE;
+            o.diverged_index = h.diverged == LLONG_MAX ? -1 : (int64_t)h.diverged;
+            o.max_update = h.last_max;
+            o.energy = h.diverged == LLONG_MAX ? eh[v] : 0.0;
+            o.active_voxels = h.last_active;
+            o.listed_voxels = cnt[v];
+        }
+    }
+    if (worst != KF_OK)
+        kf_set_error(msg);
+    return worst;
+}
+
+}  // namespace
+
+kf_status kf_batch_active(kf_context *ctx, const kf_volume *const *live, const kf_volume *const *can,
+                          kf_field *const *u, int count, const kf_params *p, kf_deform_stats *stats, bool force,
+                          bool *used)
+{
+    KF_DISPATCH_DTYPE(live[0]->dtype, S, return batch_active<S>(ctx, live, can, u, count, p, stats, force, used));
     return KF_OK;
 }

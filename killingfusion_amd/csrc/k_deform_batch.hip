@@ -650,6 +650,12 @@ kf_status kf_batch_deform(const kf_volume *const *live, const kf_volume *const *
     }
     kf_context *ctx = live[0]->ctx;
     DevGuard dg(ctx->device);
+    if (p->sweep != KF_SWEEP_DENSE && p->max_iters > 0) {
+        bool used = false;
+        s = kf_batch_active(ctx, live, can, u, count, p, stats, p->sweep == KF_SWEEP_ACTIVE, &used);
+        if (s != KF_OK || used)
+            return s;
+    }
     KF_DISPATCH_DTYPE(live[0]->dtype, S, return run_batch<S>(ctx, live, can, u, count, p, stats));
     return KF_OK;
 }

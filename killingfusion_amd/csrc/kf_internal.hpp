@@ -82,6 +82,15 @@ struct kf_active_ws {
     float4 *u = nullptr;                 // two compact sets of u, [2][cap] (x, y, z, 0)
     int4 *nbr = nullptr;                 // [cap][3]: list slots of the 12 Killing-term neighbours, -1 = +0
     size_t cap = 0;                      // list slots
+    // batched sweep: per-volume states and energies, work block -> volume, volume table
+    DeformState *bstates = nullptr;
+    double *benergy = nullptr;
+    DeformState *bstates_host = nullptr;  // pinned: states, then the energies
+    size_t bcap = 0;                      // volumes
+    unsigned int *blk_vol = nullptr;
+    size_t blk_cap = 0;
+    unsigned char *vtab = nullptr;
+    size_t vtab_cap = 0;                  // bytes
 };
 
 struct kf_context {
@@ -340,6 +349,12 @@ kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int 
 kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const kf_params *p, int iter,
                              unsigned int count, unsigned int *nb_out);
 kf_status kf_active_scatter(const kf_volume *live, float *u, unsigned int count, unsigned int iters);
+// kf_batch_deform over the voxels that can move in each volume (one launch per iteration for
+// the whole batch).  Sets *used = false and does nothing more when, without `force`, the
+// lists hold more than 1/8 of the voxels.
+kf_status kf_batch_active(kf_context *ctx, const kf_volume *const *live, const kf_volume *const *can,
+                          kf_field *const *u, int count, const kf_params *p, kf_deform_stats *stats, bool force,
+                          bool *used);
 // Energy of compact set iters & 1 into state->energy (resets the state first).
 kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count,
                            unsigned int iters);

@@ -8,7 +8,7 @@ replays it several times with other GPU work in between and checks every replay.
 import numpy as np
 import pytest
 
-from killingfusion_amd import (DisplacementField, KillingParams, batch_autotune, batch_deform,
+from killingfusion_amd import (SDF, DisplacementField, KillingParams, batch_autotune, batch_deform,
                                compute_displacement_field, scenes)
 from killingfusion_amd._lib import KF_E_DIVERGED, KF_E_INVALID_ARG, KFError
 
@@ -35,8 +35,8 @@ def batch(ctx):
     return vols
 
 
-def params(**kw):
-    return KillingParams.paper(VOX, 3.0, method="adjoint", **kw)
+def params(sweep="dense", **kw):
+    return KillingParams.paper(VOX, 3.0, method="adjoint", sweep=sweep, **kw)
 
 
 def per_volume(ctx, vols, kp, inits=None):
@@ -64,8 +64,10 @@ def lists(vols):
     return [v[0] for v in vols], [v[1] for v in vols]
 
 
-def test_batch_equals_per_volume_across_replays(ctx, batch):
-    kp = params(max_iters=50)
+@pytest.mark.parametrize("sweep", ["dense", "active"])
+def test_batch_equals_per_volume_across_replays(ctx, batch, sweep):
+    print(f"case: batch of 4, 50 iterations, {sweep} sweep", flush=True)
+    kp = params(sweep, max_iters=50)
     ref, ref_st = per_volume(ctx, batch, kp)
     assert all(s["active"] > 0 for s in ref_st)
     lives, cans = lists(batch)
@@ -79,21 +81,24 @@ def test_batch_equals_per_volume_across_replays(ctx, batch):
             assert np.array_equal(f.download(), ref[b]), (rep, b)
         if st is not None:
             for b in range(len(batch)):
-                for k in ("iters", "active", "max_update", "diverged_index"):
+                assert st[b]["sweep"] == sweep and ref_st[b]["sweep"] == sweep
+                for k in ("iters", "active", "max_update", "diverged_index", "listed"):
                     assert st[b][k] == ref_st[b][k], (rep, b, k)
                 assert st[b]["energy"] == pytest.approx(ref_st[b]["energy"], rel=1e-12)
         # other work on the same stream between replays
         compute_displacement_field(lives[0], cans[0], DisplacementField(ctx, lives[0].dims), params(max_iters=3))
 
 
-def test_batch_stop_rule_per_volume(ctx, batch):
+@pytest.mark.parametrize("sweep", ["dense", "active"])
+def test_batch_stop_rule_per_volume(ctx, batch, sweep):
     # A threshold that stops volume 0 at a known iteration: between its running minimum of
     # max |du| and the first clearly lower value.
+    print(f"case: batch of 4, stop rule, {sweep} sweep", flush=True)
     live, can = batch[0]
     ms = [compute_displacement_field(live, can, DisplacementField(ctx, live.dims), params(max_iters=k))["max_update"]
           for k in range(1, 31)]
     kstar = next(k for k in range(2, 30) if ms[k] < min(ms[:k]) * 0.98)
-    kp = params(max_iters=200, stop_max_update=0.5 * (ms[kstar] + min(ms[:kstar])))
+    kp = params(sweep, max_iters=200, stop_max_update=0.5 * (ms[kstar] + min(ms[:kstar])))
     ref, ref_st = per_volume(ctx, batch, kp)
     assert ref_st[0]["iters"] == kstar + 1
     lives, cans = lists(batch)
@@ -108,8 +113,10 @@ def test_batch_stop_rule_per_volume(ctx, batch):
             assert np.array_equal(f.download(), ref[b]), (rep, b)
 
 
-def test_batch_divergence_in_one_volume(ctx, batch):
-    kp = params(max_iters=6)
+@pytest.mark.parametrize("sweep", ["dense", "active"])
+def test_batch_divergence_in_one_volume(ctx, batch, sweep):
+    print(f"case: batch of 4, NaN planted in volume 1, {sweep} sweep", flush=True)
+    kp = params(sweep, max_iters=6)
     inits = [np.zeros((3, int(np.prod(v[0].dims))), dtype=np.float32) for v in batch]
     phi, _ = batch[1][0].download()
     act = np.nonzero(np.abs(phi) < 0.5 * TRUNC)[0]
@@ -180,3 +187,22 @@ def test_batch_argument_errors(ctx, batch):
         batch_deform(lives[:2], cans[:2], [g, f], params(max_iters=2))
     assert ei.value.status == KF_E_INVALID_ARG
     assert batch_deform([], [], [], params(max_iters=2)) == []
+
+
+def test_active_batch_with_an_empty_volume(ctx, batch):
+    """A volume with no surface (nothing integrated: no voxel can move) next to the others:
+    its iterations still count, its field stays zero, and the rest match kf_deform."""
+    print("case: batch of 4 + one 40^3 volume with nothing integrated, active sweep", flush=True)
+    kw = dict(voxel_size=VOX, clip_distance=TRUNC, max_surface=TRUNC, dtype="bf16")
+    e_live, e_can = SDF(ctx, (40, 40, 40), (0.0, 0.0, 0.5), **kw), SDF(ctx, (40, 40, 40), (0.0, 0.0, 0.5), **kw)
+    vols = list(batch) + [(e_live, e_can)]
+    kp = params("active", max_iters=12)
+    ref, ref_st = per_volume(ctx, vols, kp)
+    lives, cans = lists(vols)
+    fields = fresh_fields(ctx, vols)
+    st = batch_deform(lives, cans, fields, kp)
+    assert st[-1]["listed"] == 0 and st[-1]["iters"] == 12 and st[-1]["energy"] == 0.0
+    assert not fields[-1].download().any()
+    for b, f in enumerate(fields):
+        assert np.array_equal(f.download(), ref[b]), b
+        assert st[b]["iters"] == ref_st[b]["iters"] and st[b]["listed"] == ref_st[b]["listed"], b
