@@ -196,9 +196,12 @@ __global__ __launch_bounds__(BLOCK) void k_active_neighbours(GridDims g, const u
 // Fold of step launch k - 1 by block 0 of launch k: kf_fold_launch's result, with every load
 // (the gate flag, the state fields the fold reads and the partials) issued before any of them
 // is needed, so the fold costs one round trip.  Partials are read even when the gate is
-// closed; their values are then ignored.
+// closed; their values are then ignored.  A launch that published no partials (prev_nb = 0:
+// no stop rule, not the last launch) still counts as an iteration and reports its divergence;
+// its max |du| and active count read as 0 until the last launch's are folded.
 template <int NT>
-__device__ inline void list_fold(DeformState *st, const BlockPartial *prev, unsigned int prev_nb, int parity)
+__device__ inline void list_fold(DeformState *st, const BlockPartial *prev, unsigned int prev_nb, bool has_prev,
+                                 int parity)
 {
     const unsigned int go = st->run[parity];
     const int pp = (parity ^ 1) & 1;
@@ -229,7 +232,7 @@ __device__ inline void list_fold(DeformState *st, const BlockPartial *prev, unsi
     }
     kf_block_reduce3<NT>(s, m, c);
     if (threadIdx.x == 0) {
-        if (go && prev_nb > 0 && !done) {
+        if (go && has_prev && !done) {
             st->div_slot[pp] = LLONG_MAX;
             st->last_max = m;
             st->energy = s;
@@ -410,11 +413,12 @@ template <class S>
 __global__ __launch_bounds__(LB) void k_adj_list(PhiRead<S> L, PhiRead<S> Cn, const float4 *__restrict__ uin,
                                                  float4 *__restrict__ uout, AdjConsts c, DeformState *st,
                                                  BlockPartial *partials, int parity, const BlockPartial *prev,
-                                                 unsigned int prev_nb, const unsigned int *__restrict__ list,
+                                                 unsigned int prev_nb, int has_prev, int publish,
+                                                 const unsigned int *__restrict__ list,
                                                  const int4 *__restrict__ nbr, unsigned int count)
 {
     if (blockIdx.x == 0) {
-        list_fold<LB>(st, prev, prev_nb, parity);
+        list_fold<LB>(st, prev, prev_nb, has_prev != 0, parity);
         return;
     }
     const unsigned int nw = gridDim.x - 1u, w = blockIdx.x - 1u;
@@ -435,7 +439,8 @@ __global__ __launch_bounds__(LB) void k_adj_list(PhiRead<S> L, PhiRead<S> Cn, co
     unsigned long long act = 0;
     if (mine)
         list_step(L, Cn, uin, uout, c, &st->div_slot[parity], t, i, own, ns, mx, act);
-    kf_block_publish<LB>(partials, w, 0.0, mx, act);
+    if (publish)
+        kf_block_publish<LB>(partials, w, 0.0, mx, act);
 }
 
 template <class S>
@@ -469,20 +474,22 @@ struct ActiveVol {
     unsigned int nblk;   // work blocks (at least one)
 };
 
-// Launch k: blocks 0 .. nvol - 1 fold launch k - 1 of their volume (none when first) and arm
-// launch k + 1; the other blocks are work blocks.
+// Launch k: blocks 0 .. nvol - 1 fold launch k - 1 of their volume (has_prev; its partials
+// when it published them) and arm launch k + 1; the other blocks are work blocks, which
+// publish partials when `publish` is set.
 template <class S>
 __global__ __launch_bounds__(LB) void k_adj_list_batch(const ActiveVol<S> *__restrict__ vols, int nvol,
                                                        const unsigned int *__restrict__ blk_vol,
                                                        const float4 *__restrict__ uin, float4 *__restrict__ uout,
                                                        AdjConsts c, DeformState *states, BlockPartial *partials,
-                                                       int parity, const BlockPartial *prev, int first,
+                                                       int parity, const BlockPartial *prev, int has_prev,
+                                                       int prev_published, int publish,
                                                        const unsigned int *__restrict__ list,
                                                        const int4 *__restrict__ nbr)
 {
     if ((int)blockIdx.x < nvol) {
         const unsigned int b0 = vols[blockIdx.x].blk0, nb = vols[blockIdx.x].nblk;
-        list_fold<LB>(states + blockIdx.x, prev + b0, first ? 0u : nb, parity);
+        list_fold<LB>(states + blockIdx.x, prev + b0, prev_published ? nb : 0u, has_prev != 0, parity);
         return;
     }
     const unsigned int w = kf_xcd_tile(blockIdx.x - (unsigned int)nvol, gridDim.x - (unsigned int)nvol);
@@ -499,7 +506,8 @@ __global__ __launch_bounds__(LB) void k_adj_list_batch(const ActiveVol<S> *__res
     unsigned long long act = 0;
     if (i != NONE)
         list_step(vols[v].L, vols[v].Cn, uin, uout, c, &st->div_slot[parity], t, i, own, ns, mx, act);
-    kf_block_publish<LB>(partials, w, 0.0, mx, act);
+    if (publish)
+        kf_block_publish<LB>(partials, w, 0.0, mx, act);
 }
 
 // Energy of each volume's final compact set (iters & 1 of its state) into partials, one per
@@ -687,6 +695,11 @@ kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int 
 kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const kf_params *p, int iter,
                              unsigned int count, unsigned int *nb_out)
 {
+    // Per-iteration partials (max |du|, active count) feed only the stop rule and the last
+    // iteration's statistics: without a stop rule only the last launch publishes them.
+    const bool check = p->stop_max_update > 0.0;
+    const bool publish = check || iter == p->max_iters - 1;
+    const bool prev_published = check || iter == 0;  // launch iter - 1 published (never, for iter 0)
     kf_context *ctx = live->ctx;
     kf_active_ws &a = ctx->active;
     KF_CHECK_ARG(live->dtype == can->dtype, "live and canonical volumes must share a dtype");
@@ -701,13 +714,13 @@ kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const 
         *nb_out = nb;
     const int parity = iter & 1;
     const BlockPartial *prev = kf_partials(ctx, parity ^ 1);
-    const unsigned int prev_nb = iter > 0 ? nb : 0u;
+    const unsigned int prev_nb = iter > 0 && prev_published ? nb : 0u;
     const float4 *uin = a.u + (size_t)parity * a.cap;
     float4 *uout = a.u + (size_t)(parity ^ 1) * a.cap;
     KF_DISPATCH_DTYPE(live->dtype, S, {
         hipLaunchKernelGGL((k_adj_list<S>), dim3(nb + 1), dim3(LB), 0, ctx->stream, make_read<S>(live),
                            make_read<S>(can), uin, uout, c, ctx->state, kf_partials(ctx, parity), parity, prev, prev_nb,
-                           a.list, a.nbr, count);
+                           iter > 0 ? 1 : 0, publish ? 1 : 0, a.list, a.nbr, count);
     });
     KF_LAUNCH_CHECK();
     return KF_OK;
@@ -876,9 +889,11 @@ kf_status batch_active(kf_context *ctx, const kf_volume *const *live, const kf_v
     int launched = 0;
     for (int k = 0; k < p->max_iters; k++) {
         const int par = k & 1;
+        const int publish = check || k == p->max_iters - 1;
         hipLaunchKernelGGL((k_adj_list_batch<S>), dim3((unsigned int)count + nwb), dim3(LB), 0, st, dtab, count,
                            a.blk_vol, a.u + (size_t)par * set, a.u + (size_t)(par ^ 1) * set, c, a.bstates,
-                           kf_partials(ctx, par), par, kf_partials(ctx, par ^ 1), k == 0 ? 1 : 0, a.list, a.nbr);
+                           kf_partials(ctx, par), par, kf_partials(ctx, par ^ 1), k > 0 ? 1 : 0, check ? 1 : 0,
+                           publish, a.list, a.nbr);
         KF_LAUNCH_CHECK();
         launched = k + 1;
         if (check && launched % 64 == 0 && launched < p->max_iters) {
@@ -895,8 +910,8 @@ kf_status batch_active(kf_context *ctx, const kf_volume *const *live, const kf_v
         // fold the last launch (fold blocks only)
         const int par = launched & 1;
         hipLaunchKernelGGL((k_adj_list_batch<S>), dim3((unsigned int)count), dim3(LB), 0, st, dtab, count, a.blk_vol,
-                           a.u, a.u, c, a.bstates, kf_partials(ctx, par), par, kf_partials(ctx, par ^ 1), 0, a.list,
-                           a.nbr);
+                           a.u, a.u, c, a.bstates, kf_partials(ctx, par), par, kf_partials(ctx, par ^ 1), 1, 1, 0,
+                           a.list, a.nbr);
         KF_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL((k_active_scatter_batch<S>), dim3((unsigned int)((slots + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
