@@ -178,7 +178,8 @@ kf_status kf_energy_backward(const kf_volume *live, const kf_volume *can, const 
                              const kf_params *p, kf_field *grad_out);
 /* KillingFusion::computeDisplacementField, all-voxels (Jacobi) mode
  * (KillingFusion.cpp:248-351): live = src, can = dest, u = srcToDest, updated in place.
- * p->method selects the reference gradient or the fused adjoint kernel. */
+ * p->method selects the reference gradient or the fused adjoint kernel; with the adjoint,
+ * p->sweep selects the dense or the active-set sweep (same field; see kf_sweep). */
 kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, const kf_params *p,
                     kf_deform_stats *stats);
 /* SDF::update(displacementField), SDF.cpp:300-328. */
@@ -226,12 +227,14 @@ kf_status kf_field_from_rigid(kf_field *u, const kf_volume *can, const kf_volume
  * No reference counterpart: the reference deforms one live SDF per call
  * (KillingFusion::computeDisplacementField, KillingFusion.cpp:240-397); this runs
  * kf_deform on count independent (live[b], can[b], u[b]) triplets with one set of params.
- * With the adjoint method every iteration is one persistent launch over the tiles of all
- * volumes, the iteration loop is a captured hipGraph (replayed while the batch keeps its
- * shape), and the tile shape is the autotuner's choice for the batch's shape.  Results equal
- * per-volume kf_deform bit for bit.  stats (may be NULL) receives one entry per volume;
- * energies are evaluated only when stats is given.  The reference method runs volume by
- * volume.  Returns KF_E_DIVERGED if any volume diverged (the others still complete). */
+ * With the adjoint method and the active-set sweep (p->sweep), every iteration is one launch
+ * over the lists of all volumes, each volume with its own stop rule.  With the dense sweep,
+ * every iteration is one persistent launch over the tiles of all volumes, the iteration loop
+ * is a captured hipGraph (replayed while the batch keeps its shape), and the tile shape is
+ * the autotuner's choice for the batch's shape.  Results equal per-volume kf_deform bit for
+ * bit.  stats (may be NULL) receives one entry per volume; energies are evaluated only when
+ * stats is given.  The reference method runs volume by volume.  Returns KF_E_DIVERGED if any
+ * volume diverged (the others still complete). */
 kf_status kf_batch_deform(const kf_volume *const *live, const kf_volume *const *can,
                           kf_field *const *u, int32_t count, const kf_params *p,
                           kf_deform_stats *stats);
