@@ -82,10 +82,10 @@ __global__ __launch_bounds__(BLOCK) void k_active_mark(PhiRead<S> L, const float
     }
 }
 
-// Exclusive scan of the span counts (one block); the total goes to st->listed.
+// Exclusive scan of the span counts (one block); the total goes to *total.
 __global__ __launch_bounds__(SCAN_BLOCK) void k_active_scan(const unsigned int *__restrict__ span_count,
                                                             unsigned int nspan, unsigned int *__restrict__ span_off,
-                                                            DeformState *st)
+                                                            unsigned int *total)
 {
     const unsigned int per = (nspan + SCAN_BLOCK - 1) / SCAN_BLOCK;
     const unsigned int b0 = threadIdx.x * per;
@@ -121,7 +121,58 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_active_scan(const unsigned int *
             run += span_count[b0 + k];
         }
     if (threadIdx.x == SCAN_BLOCK - 1)
-        st->listed = run;
+        *total = run;
+}
+
+// The voxels a warp (SDF::update) can change: u_j != 0 (any component; -0 counts as 0, it
+// moves nothing), plus voxels whose phi is -0 or not finite.  Elsewhere the warp reads its own
+// cell with zero weights and returns phi and w unchanged, provided every phi is finite (a
+// non-finite corner times a zero weight is NaN): a non-finite phi sets *flag.
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_moved_mark(const typename S::T *__restrict__ phi, float scale,
+                                                      const float *__restrict__ u, unsigned int n32,
+                                                      unsigned long long *__restrict__ mask,
+                                                      unsigned int *__restrict__ span_count, unsigned int *flag)
+{
+    const unsigned int base = blockIdx.x * (unsigned int)SPAN;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned int cnt = 0;
+#pragma unroll 4
+    for (int k = 0; k < SPAN / BLOCK; k++) {
+        const unsigned int w0 = base + (unsigned int)(k * BLOCK + w * 64);
+        const unsigned int i = w0 + (unsigned int)lane;
+        bool cand = false;
+        if (i < n32) {
+            const float u0 = kf_load_stream(u + i), u1 = kf_load_stream(u + n32 + i);
+            const float u2 = kf_load_stream(u + 2u * n32 + i);
+            const float f = S::load(phi, (int64_t)i, scale);
+            const bool finite = isfinite(f);
+            if (!finite)
+                atomicOr(flag, 1u);
+            cand = ((__float_as_uint(u0) | __float_as_uint(u1) | __float_as_uint(u2)) & 0x7fffffffu) != 0u ||
+                   __float_as_uint(f) == 0x80000000u || !finite;
+        }
+        const unsigned long long bal = __ballot(cand);
+        if (lane == 0 && w0 < n32)
+            mask[w0 >> 6] = bal;
+        cnt += (unsigned int)__popcll(bal);
+    }
+    __shared__ unsigned int s_cnt[BLOCK / 64];
+    if (lane == 0)
+        s_cnt[w] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int t = 0;
+        for (int k = 0; k < BLOCK / 64; k++)
+            t += s_cnt[k];
+        span_count[blockIdx.x] = t;
+    }
+}
+
+__global__ void k_clear_words(unsigned int *p, int n)
+{
+    if ((int)threadIdx.x < n)
+        p[threadIdx.x] = 0u;
 }
 
 // Thread k of block b lists the voxels of mask word b WPB + k, in voxel order.
@@ -636,6 +687,10 @@ void kf_active_release(kf_context *ctx)
     (void)hipFree(a.vtab);
     if (a.bstates_host)
         (void)hipHostFree(a.bstates_host);
+    (void)hipFree(a.aux);
+    if (a.aux_host)
+        (void)hipHostFree(a.aux_host);
+    (void)hipFree(a.wtmp);
     a = kf_active_ws{};
 }
 
@@ -657,7 +712,7 @@ kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params
     });
     KF_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_active_scan, dim3(1), dim3(SCAN_BLOCK), 0, ctx->stream, a.span_count, nspan,
-                       a.span_count + nspan, ctx->state);
+                       a.span_count + nspan, &ctx->state->listed);
     KF_LAUNCH_CHECK();
     return KF_OK;
 }
@@ -816,7 +871,7 @@ kf_status batch_active(kf_context *ctx, const kf_volume *const *live, const kf_v
                            u[v]->u(), c.lo, c.hi, a.mask + moff[v], a.span_count + soff[v]);
         KF_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_active_scan, dim3(1), dim3(SCAN_BLOCK), 0, st, a.span_count + soff[v], nspan[v],
-                           a.span_count + soff[v] + nspan[v], a.bstates + v);
+                           a.span_count + soff[v] + nspan[v], &a.bstates[v].listed);
         KF_LAUNCH_CHECK();
     }
     KF_HIP(hipMemcpyAsync(sh, a.bstates, sizeof(DeformState) * count, hipMemcpyDeviceToHost, st));
@@ -962,5 +1017,60 @@ kf_status kf_batch_active(kf_context *ctx, const kf_volume *const *live, const k
                           bool *used)
 {
     KF_DISPATCH_DTYPE(live[0]->dtype, S, return batch_active<S>(ctx, live, can, u, count, p, stats, force, used));
+    return KF_OK;
+}
+
+kf_status kf_list_moved(const kf_volume *v, const float *u, unsigned int *count, bool *nonfinite)
+{
+    kf_context *ctx = v->ctx;
+    kf_active_ws &a = ctx->active;
+    hipStream_t st = ctx->stream;
+    KF_CHECK_ARG(kf_active_supported(v), "listing needs 3 N < 2^30");
+    const unsigned int nspan = (unsigned int)((v->n + SPAN - 1) / SPAN);
+    const unsigned int nwords = (unsigned int)((v->n + 63) / 64);
+    kf_status s = grow(a.mask, a.mask_cap, (size_t)nspan * WPB, st);
+    if (s == KF_OK)
+        s = grow(a.span_count, a.span_cap, 2 * (size_t)nspan, st);
+    if (s == KF_OK)
+        s = grow(a.pos_of, a.pos_cap, (size_t)v->n, st);
+    if (s == KF_OK && !a.aux) {
+        KF_HIP(hipMalloc((void **)&a.aux, 4 * sizeof(unsigned int)));
+        if (hipHostMalloc((void **)&a.aux_host, 4 * sizeof(unsigned int), hipHostMallocDefault) != hipSuccess) {
+            kf_set_error("pinned counter allocation failed");
+            s = KF_E_OOM;
+        }
+    }
+    if (s != KF_OK)
+        return s;
+    hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, st, a.aux, 4);
+    KF_LAUNCH_CHECK();
+    KF_DISPATCH_DTYPE(v->dtype, S, {
+        hipLaunchKernelGGL((k_moved_mark<S>), dim3(nspan), dim3(BLOCK), 0, st, (const typename S::T *)v->phi,
+                           v->scale(), u, (unsigned int)v->n, a.mask, a.span_count, a.aux + 1);
+    });
+    KF_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_active_scan, dim3(1), dim3(SCAN_BLOCK), 0, st, a.span_count, nspan, a.span_count + nspan,
+                       a.aux);
+    KF_LAUNCH_CHECK();
+    KF_HIP(hipMemcpyAsync(a.aux_host, a.aux, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    KF_HIP(hipStreamSynchronize(st));
+    *count = a.aux_host[0];
+    *nonfinite = a.aux_host[1] != 0u;
+    if (*nonfinite || *count == 0)
+        return KF_OK;
+    if ((size_t)*count > a.cap) {
+        size_t c1 = a.cap, c2 = 2 * a.cap, c3 = 3 * a.cap;
+        s = grow(a.list, c1, *count, st);
+        if (s == KF_OK)
+            s = grow(a.u, c2, 2 * (size_t)*count, st);
+        if (s == KF_OK)
+            s = grow(a.nbr, c3, 3 * (size_t)*count, st);
+        a.cap = s == KF_OK ? *count : 0;
+        if (s != KF_OK)
+            return s;
+    }
+    hipLaunchKernelGGL(k_active_compact, dim3(nspan), dim3(WPB), 0, st, a.mask, nwords, a.span_count,
+                       a.span_count + nspan, 0u, a.list, a.pos_of);
+    KF_LAUNCH_CHECK();
     return KF_OK;
 }

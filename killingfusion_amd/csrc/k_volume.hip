@@ -134,6 +134,43 @@ __global__ void k_warp(Reader<S> R, const float *__restrict__ u, typename S::T *
     out_w[i] = (int32_t)(long long)ww;
 }
 
+// SDF::update for the listed voxels only (the voxels it can change; k_moved_mark): the new
+// phi and weight of slot t, computed exactly as k_warp does, into compact arrays -- every
+// read sees the unwarped SDF -- then written in place by k_warp_scatter.
+template <class S>
+__global__ void k_warp_list(Reader<S> R, const float *__restrict__ u, const unsigned int *__restrict__ list,
+                            unsigned int count, typename S::T *__restrict__ new_phi, int32_t *__restrict__ new_w)
+{
+    const unsigned int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= count)
+        return;
+    const GridDims g = R.g;
+    const int64_t i = list[t];
+    const int x = (int)(i % g.nx);
+    const int y = (int)((i / g.nx) % g.ny);
+    const int z = (int)(i / ((int64_t)g.nx * g.ny));
+    const double lx = (double)x + 0.5 + (double)u[i];
+    const double ly = (double)y + 0.5 + (double)u[g.n + i];
+    const double lz = (double)z + 0.5 + (double)u[2 * g.n + i];
+    double dd, ww;
+    R.both(lx, ly, lz, dd, ww);
+    new_phi[t] = S::enc((float)dd, R.scale);
+    new_w[t] = (int32_t)(long long)ww;
+}
+
+template <class S>
+__global__ void k_warp_scatter(const unsigned int *__restrict__ list, unsigned int count,
+                               const typename S::T *__restrict__ new_phi, const int32_t *__restrict__ new_w,
+                               typename S::T *phi, int32_t *w)
+{
+    const unsigned int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= count)
+        return;
+    const unsigned int i = list[t];
+    phi[i] = new_phi[t];
+    w[i] = new_w[t];
+}
+
 // SDF::fuse(otherSdf), SDF.cpp:199-241, FUSE_BY_MERGE.
 template <class S1, class S2>
 __global__ void k_fuse(typename S1::T *phi1, int32_t *w1, float scale1, const typename S2::T *__restrict__ phi2,
@@ -319,5 +356,44 @@ kf_status kf_launch_field_add(kf_field *dst, const kf_field *src)
     hipLaunchKernelGGL(k_field_add, dim3(blocks_for(3 * dst->n)), dim3(BLOCK), 0, dst->ctx->stream, dst->u(),
                        (const float *)src->u(), 3 * dst->n);
     KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
+kf_status kf_warp_sparse(kf_volume *v, const kf_field *u, bool *done)
+{
+    *done = false;
+    if (!kf_active_supported(v))
+        return KF_OK;
+    unsigned int count = 0;
+    bool nonfinite = false;
+    kf_status s = kf_list_moved(v, u->u(), &count, &nonfinite);
+    if (s != KF_OK || nonfinite || (int64_t)count * 8 > v->n)
+        return s;
+    kf_context *ctx = v->ctx;
+    kf_active_ws &a = ctx->active;
+    if (count > 0) {
+        const size_t need = (size_t)count * (v->elem_size() + sizeof(int32_t));
+        if (need > a.wtmp_cap) {
+            if (a.wtmp) {
+                KF_HIP(hipStreamSynchronize(ctx->stream));
+                KF_HIP(hipFree(a.wtmp));
+                a.wtmp = nullptr;
+                a.wtmp_cap = 0;
+            }
+            KF_HIP(hipMalloc((void **)&a.wtmp, need));
+            a.wtmp_cap = need;
+        }
+        int32_t *nw = reinterpret_cast<int32_t *>(a.wtmp);
+        void *np = a.wtmp + (size_t)count * sizeof(int32_t);
+        const unsigned int nb = (count + BLOCK - 1) / BLOCK;
+        KF_DISPATCH_DTYPE(v->dtype, S, {
+            hipLaunchKernelGGL((k_warp_list<S>), dim3(nb), dim3(BLOCK), 0, ctx->stream, make_reader<S>(v),
+                               (const float *)u->u(), a.list, count, (typename S::T *)np, nw);
+            hipLaunchKernelGGL((k_warp_scatter<S>), dim3(nb), dim3(BLOCK), 0, ctx->stream, a.list, count,
+                               (const typename S::T *)np, (const int32_t *)nw, (typename S::T *)v->phi, v->w);
+        });
+        KF_LAUNCH_CHECK();
+    }
+    *done = true;
     return KF_OK;
 }

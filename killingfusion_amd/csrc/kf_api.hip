@@ -733,6 +733,12 @@ kf_status kf_warp(kf_volume *v, const kf_field *u)
     KF_CHECK_ARG(v && u, "null argument");
     KF_CHECK_ARG(same_dims(v->dims, u->dims), "volume and field dims differ");
     DeviceGuard dg(v->ctx->device);
+    // Only voxels with u != 0 can change (k_moved_mark); when they are few, warp those in
+    // place.
+    bool sparse = false;
+    const kf_status ss = kf_warp_sparse(v, u, &sparse);
+    if (ss != KF_OK || sparse)
+        return ss;
     hipStream_t s = v->ctx->stream;
     void *np = nullptr;
     int32_t *nw = nullptr;

@@ -91,6 +91,12 @@ struct kf_active_ws {
     size_t blk_cap = 0;
     unsigned char *vtab = nullptr;
     size_t vtab_cap = 0;                  // bytes
+    // listing of the voxels a warp changes: counters (device / pinned host) and the new
+    // phi, weight of each listed voxel
+    unsigned int *aux = nullptr;
+    unsigned int *aux_host = nullptr;
+    unsigned char *wtmp = nullptr;
+    size_t wtmp_cap = 0;                  // bytes
 };
 
 struct kf_context {
@@ -355,6 +361,13 @@ kf_status kf_active_scatter(const kf_volume *live, float *u, unsigned int count,
 kf_status kf_batch_active(kf_context *ctx, const kf_volume *const *live, const kf_volume *const *can,
                           kf_field *const *u, int count, const kf_params *p, kf_deform_stats *stats, bool force,
                           bool *used);
+// Lists, in ctx->active.list (ascending), the voxels SDF::update can change given u (see
+// k_moved_mark); reads back their number and whether some phi is not finite (then nothing
+// is listed and the caller must warp every voxel).
+kf_status kf_list_moved(const kf_volume *v, const float *u, unsigned int *count, bool *nonfinite);
+// SDF::update over those voxels only, in place (k_volume.hip).  Sets *done = false, changing
+// nothing, when the list holds more than 1/8 of the voxels or some phi is not finite.
+kf_status kf_warp_sparse(kf_volume *v, const kf_field *u, bool *done);
 // Energy of compact set iters & 1 into state->energy (resets the state first).
 kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count,
                            unsigned int iters);
