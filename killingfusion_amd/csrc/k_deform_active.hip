@@ -44,10 +44,12 @@ constexpr int SCAN_BLOCK = 1024;
 __constant__ int c_nb_off[NNB][3] = {{1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0}, {0, 0, 1},  {0, 0, -1},
                                      {1, -1, 0}, {1, 0, -1}, {-1, 1, 0}, {0, 1, -1}, {-1, 0, 1}, {0, -1, 1}};
 
-template <class S>
+// READ_U false: every u is +0 (the caller has just cleared the field).  *nonfinite is set when
+// some live SDF value is not finite.
+template <class S, bool READ_U>
 __global__ __launch_bounds__(BLOCK) void k_active_mark(PhiRead<S> L, const float *__restrict__ u, float lo, float hi,
                                                        unsigned long long *__restrict__ mask,
-                                                       unsigned int *__restrict__ span_count)
+                                                       unsigned int *__restrict__ span_count, unsigned int *nonfinite)
 {
     const unsigned int n32 = nvox32(L.g);
     const unsigned int base = blockIdx.x * (unsigned int)SPAN;
@@ -59,11 +61,14 @@ __global__ __launch_bounds__(BLOCK) void k_active_mark(PhiRead<S> L, const float
         const unsigned int i = w0 + (unsigned int)lane;
         bool cand = false;
         if (i < n32) {
-            const float u0 = kf_load_stream(u + i), u1 = kf_load_stream(u + n32 + i);
-            const float u2 = kf_load_stream(u + 2u * n32 + i);
+            unsigned int ub = 0u;
+            if (READ_U)
+                ub = __float_as_uint(kf_load_stream(u + i)) | __float_as_uint(kf_load_stream(u + n32 + i)) |
+                     __float_as_uint(kf_load_stream(u + 2u * n32 + i));
             const float live = S::load(L.p, (int64_t)i, L.scale);
-            cand = (__float_as_uint(u0) | __float_as_uint(u1) | __float_as_uint(u2)) != 0u ||
-                   !(live > hi || live < lo);
+            if (!isfinite(live))
+                atomicOr(nonfinite, 1u);
+            cand = ub != 0u || !(live > hi || live < lo);
         }
         const unsigned long long bal = __ballot(cand);
         if (lane == 0 && w0 < n32)
@@ -494,12 +499,14 @@ __global__ __launch_bounds__(LB) void k_adj_list(PhiRead<S> L, PhiRead<S> Cn, co
         kf_block_publish<LB>(partials, w, 0.0, mx, act);
 }
 
+// Energy of the final compact set (st->iters & 1) into one partial per block.
 template <class S>
-__global__ __launch_bounds__(LB) void k_active_energy(PhiRead<S> L, PhiRead<S> Cn, const float4 *__restrict__ uc,
-                                                      AdjConsts c, BlockPartial *partials,
-                                                      const unsigned int *__restrict__ list,
+__global__ __launch_bounds__(LB) void k_active_energy(PhiRead<S> L, PhiRead<S> Cn, const float4 *__restrict__ u0,
+                                                      unsigned int set, const DeformState *st, AdjConsts c,
+                                                      BlockPartial *partials, const unsigned int *__restrict__ list,
                                                       const int4 *__restrict__ nbr, unsigned int count)
 {
+    const float4 *uc = u0 + (size_t)(st->iters & 1u) * set;
     const unsigned int t = kf_xcd_tile(blockIdx.x, gridDim.x) * (unsigned int)LB + threadIdx.x;
     double esum = 0.0;
     if (t < count) {
@@ -508,6 +515,19 @@ __global__ __launch_bounds__(LB) void k_active_energy(PhiRead<S> L, PhiRead<S> C
         esum = list_energy(L, Cn, uc, c, t, list[t], uc[t], ns);
     }
     kf_block_publish<LB>(partials, blockIdx.x, esum, 0.0, 0ull);
+}
+
+// The energy partials in slot order -> st->list_energy.
+__global__ __launch_bounds__(1024) void k_list_energy_fold(DeformState *st, const BlockPartial *__restrict__ parts,
+                                                           unsigned int nb)
+{
+    double s = 0.0, m = 0.0;
+    unsigned long long c = 0;
+    for (unsigned int b = threadIdx.x; b < nb; b += 1024)
+        s += parts[b].sum;
+    kf_block_reduce3<1024>(s, m, c);
+    if (threadIdx.x == 0)
+        st->list_energy = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -639,14 +659,15 @@ __global__ void k_states_reset(DeformState *st, int n, double stop)
 }
 
 __global__ __launch_bounds__(BLOCK) void k_active_scatter(const unsigned int *__restrict__ list,
-                                                          const float4 *__restrict__ uc, unsigned int count,
+                                                          const float4 *__restrict__ u0, unsigned int set,
+                                                          const DeformState *st, unsigned int count,
                                                           float *__restrict__ u, unsigned int n32)
 {
     const unsigned int t = blockIdx.x * (unsigned int)BLOCK + threadIdx.x;
     if (t >= count)
         return;
     const unsigned int i = list[t];
-    const float4 v = uc[t];
+    const float4 v = u0[(size_t)(st->iters & 1u) * set + t];
     u[i] = v.x;
     u[n32 + i] = v.y;
     u[2u * n32 + i] = v.z;
@@ -694,10 +715,11 @@ void kf_active_release(kf_context *ctx)
     a = kf_active_ws{};
 }
 
-kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params *p)
+kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params *p, bool u_zero)
 {
     kf_context *ctx = live->ctx;
     kf_active_ws &a = ctx->active;
+    a.warp_vol = nullptr;
     KF_CHECK_ARG(kf_active_supported(live), "the active-set sweep needs 3 N < 2^30");
     const AdjConsts c = kf_adj_consts(p);
     const unsigned int nspan = (unsigned int)((live->n + SPAN - 1) / SPAN);
@@ -707,8 +729,12 @@ kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params
     if (s != KF_OK)
         return s;
     KF_DISPATCH_DTYPE(live->dtype, S, {
-        hipLaunchKernelGGL((k_active_mark<S>), dim3(nspan), dim3(BLOCK), 0, ctx->stream, make_read<S>(live), u, c.lo,
-                           c.hi, a.mask, a.span_count);
+        if (u_zero)
+            hipLaunchKernelGGL((k_active_mark<S, false>), dim3(nspan), dim3(BLOCK), 0, ctx->stream, make_read<S>(live),
+                               u, c.lo, c.hi, a.mask, a.span_count, &ctx->state->nonfinite);
+        else
+            hipLaunchKernelGGL((k_active_mark<S, true>), dim3(nspan), dim3(BLOCK), 0, ctx->stream, make_read<S>(live),
+                               u, c.lo, c.hi, a.mask, a.span_count, &ctx->state->nonfinite);
     });
     KF_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_active_scan, dim3(1), dim3(SCAN_BLOCK), 0, ctx->stream, a.span_count, nspan,
@@ -781,8 +807,7 @@ kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const 
     return KF_OK;
 }
 
-kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count,
-                           unsigned int iters)
+kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count)
 {
     kf_context *ctx = live->ctx;
     kf_active_ws &a = ctx->active;
@@ -790,28 +815,28 @@ kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf
     const AdjConsts c = kf_adj_consts(p);
     const unsigned int nb = count ? (count + LB - 1) / LB : 1u;
     kf_status s = kf_ensure_partials(ctx, nb);
-    if (s == KF_OK)
-        s = kf_state_reset(ctx, 0.0);
     if (s != KF_OK)
         return s;
-    const float4 *uc = a.u + (size_t)(iters & 1u) * a.cap;
     KF_DISPATCH_DTYPE(live->dtype, S, {
         hipLaunchKernelGGL((k_active_energy<S>), dim3(nb), dim3(LB), 0, ctx->stream, make_read<S>(live),
-                           make_read<S>(can), uc, c, kf_partials(ctx, 0), a.list, a.nbr, count);
+                           make_read<S>(can), a.u, (unsigned int)a.cap, ctx->state, c, kf_partials(ctx, 0), a.list,
+                           a.nbr, count);
     });
     KF_LAUNCH_CHECK();
-    return kf_launch_finalize(ctx, nb, 0);
+    hipLaunchKernelGGL(k_list_energy_fold, dim3(1), dim3(1024), 0, ctx->stream, ctx->state,
+                       (const BlockPartial *)kf_partials(ctx, 0), nb);
+    KF_LAUNCH_CHECK();
+    return KF_OK;
 }
 
-kf_status kf_active_scatter(const kf_volume *live, float *u, unsigned int count, unsigned int iters)
+kf_status kf_active_scatter(const kf_volume *live, float *u, unsigned int count)
 {
     kf_context *ctx = live->ctx;
     kf_active_ws &a = ctx->active;
     if (count == 0)
         return KF_OK;
-    const float4 *uc = a.u + (size_t)(iters & 1u) * a.cap;
-    hipLaunchKernelGGL(k_active_scatter, dim3((count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, a.list, uc,
-                       count, u, (unsigned int)live->n);
+    hipLaunchKernelGGL(k_active_scatter, dim3((count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, a.list, a.u,
+                       (unsigned int)a.cap, ctx->state, count, u, (unsigned int)live->n);
     KF_LAUNCH_CHECK();
     return KF_OK;
 }
@@ -826,6 +851,7 @@ kf_status batch_active(kf_context *ctx, const kf_volume *const *live, const kf_v
     kf_active_ws &a = ctx->active;
     hipStream_t st = ctx->stream;
     *used = false;
+    a.warp_vol = nullptr;
     std::vector<size_t> moff(count), soff(count);
     std::vector<unsigned int> nspan(count);
     size_t mtot = 0, stot = 0;
@@ -867,8 +893,8 @@ kf_status batch_active(kf_context *ctx, const kf_volume *const *live, const kf_v
     hipLaunchKernelGGL(k_states_reset, dim3((count + 63) / 64), dim3(64), 0, st, a.bstates, count, p->stop_max_update);
     KF_LAUNCH_CHECK();
     for (int v = 0; v < count; v++) {
-        hipLaunchKernelGGL((k_active_mark<S>), dim3(nspan[v]), dim3(BLOCK), 0, st, make_read<S>(live[v]),
-                           u[v]->u(), c.lo, c.hi, a.mask + moff[v], a.span_count + soff[v]);
+        hipLaunchKernelGGL((k_active_mark<S, true>), dim3(nspan[v]), dim3(BLOCK), 0, st, make_read<S>(live[v]),
+                           u[v]->u(), c.lo, c.hi, a.mask + moff[v], a.span_count + soff[v], &a.bstates[v].nonfinite);
         KF_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_active_scan, dim3(1), dim3(SCAN_BLOCK), 0, st, a.span_count + soff[v], nspan[v],
                            a.span_count + soff[v] + nspan[v], &a.bstates[v].listed);
@@ -1024,6 +1050,7 @@ kf_status kf_list_moved(const kf_volume *v, const float *u, unsigned int *count,
 {
     kf_context *ctx = v->ctx;
     kf_active_ws &a = ctx->active;
+    a.warp_vol = nullptr;
     hipStream_t st = ctx->stream;
     KF_CHECK_ARG(kf_active_supported(v), "listing needs 3 N < 2^30");
     const unsigned int nspan = (unsigned int)((v->n + SPAN - 1) / SPAN);

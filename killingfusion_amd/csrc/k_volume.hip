@@ -359,16 +359,11 @@ kf_status kf_launch_field_add(kf_field *dst, const kf_field *src)
     return KF_OK;
 }
 
-kf_status kf_warp_sparse(kf_volume *v, const kf_field *u, bool *done)
+namespace {
+
+// Warps the count voxels of ctx->active.list in place (k_warp_list, then k_warp_scatter).
+kf_status warp_listed(kf_volume *v, const kf_field *u, unsigned int count)
 {
-    *done = false;
-    if (!kf_active_supported(v))
-        return KF_OK;
-    unsigned int count = 0;
-    bool nonfinite = false;
-    kf_status s = kf_list_moved(v, u->u(), &count, &nonfinite);
-    if (s != KF_OK || nonfinite || (int64_t)count * 8 > v->n)
-        return s;
     kf_context *ctx = v->ctx;
     kf_active_ws &a = ctx->active;
     if (count > 0) {
@@ -394,6 +389,35 @@ kf_status kf_warp_sparse(kf_volume *v, const kf_field *u, bool *done)
         });
         KF_LAUNCH_CHECK();
     }
-    *done = true;
     return KF_OK;
+}
+
+}  // namespace
+
+kf_status kf_warp_sparse(kf_volume *v, const kf_field *u, bool *done)
+{
+    *done = false;
+    if (!kf_active_supported(v))
+        return KF_OK;
+    unsigned int count = 0;
+    bool nonfinite = false;
+    kf_status s = kf_list_moved(v, u->u(), &count, &nonfinite);
+    if (s != KF_OK || nonfinite || (int64_t)count * 8 > v->n)
+        return s;
+    s = warp_listed(v, u, count);
+    *done = s == KF_OK;
+    return s;
+}
+
+kf_status kf_warp_after_deform(kf_volume *v, const kf_field *u, bool *done)
+{
+    *done = false;
+    kf_active_ws &a = v->ctx->active;
+    if (a.warp_vol != v || a.warp_field != u)
+        return KF_OK;
+    const unsigned int count = a.warp_count;
+    a.warp_vol = nullptr;
+    const kf_status s = warp_listed(v, u, count);
+    *done = s == KF_OK;
+    return s;
 }

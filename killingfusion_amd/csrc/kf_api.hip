@@ -29,6 +29,8 @@ __global__ void k_state_reset(DeformState *st, double stop)
         st->energy = 0.0;
         st->last_active = 0ull;
         st->listed = 0u;
+        st->nonfinite = 0u;
+        st->list_energy = 0.0;
     }
 }
 
@@ -538,6 +540,117 @@ kf_status kf_integrate_depth(kf_volume *v, const float *depth, int32_t width, in
 
 }  // extern "C"
 
+kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *u, const kf_params *p,
+                         kf_deform_stats *stats, bool u_zero)
+{
+    kf_status s = kf_check_triplet(live, can, u);
+    if (s != KF_OK)
+        return s;
+    KF_CHECK_ARG(kf_valid_params(p), "invalid params");
+    kf_context *ctx = live->ctx;
+    DeviceGuard dg(ctx->device);
+    s = kf_state_reset(ctx, p->stop_max_update);
+    if (s != KF_OK)
+        return s;
+    ctx->active.warp_vol = nullptr;
+    const int cur0 = u->cur;
+    const bool check = p->stop_max_update > 0.0;
+    const bool adj = p->method == KF_METHOD_ADJOINT;
+    // Active-set sweep (k_deform_active.hip): count the voxels that can move, list them, step
+    // only them (in compact arrays) and write the result back into the field's buffer.
+    bool listed = false;
+    unsigned int count = 0;
+    if (adj && p->max_iters > 0 && p->sweep != KF_SWEEP_DENSE && kf_active_supported(live)) {
+        s = kf_active_count(live, u->buf[cur0], p, u_zero);
+        if (s == KF_OK)
+            s = read_state(ctx);
+        if (s != KF_OK)
+            return s;
+        count = ctx->state_host->listed;
+        listed = p->sweep == KF_SWEEP_ACTIVE || (int64_t)count * 8 <= live->n;
+        if (listed && (s = kf_active_prepare(live, u->buf[cur0], count)) != KF_OK)
+            return s;
+    }
+    // A list step is a few microseconds; check the stop rule less often so the host stays ahead.
+    const int CHUNK = listed ? 64 : 16;
+    // Launch k reads buffer cur0 ^ (k & 1) and writes the other one.  An adjoint launch k folds
+    // launch k - 1 in its first block, so launch k itself may run before the stop rule (or a
+    // divergence) of k - 1 is known: it only writes the buffer that is not the result, and
+    // launch k + 1 sees `done` and does nothing.
+    int launched = 0;
+    unsigned int nb = 0;
+    for (int k = 0; k < p->max_iters; k++) {
+        float *in = u->buf[cur0 ^ (k & 1)];
+        float *out = u->buf[cur0 ^ ((k + 1) & 1)];
+        s = listed ? kf_launch_adj_list(live, can, p, k, count, &nb)
+            : adj  ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr, nullptr, k, &nb)
+                   : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP, k);
+        if (s != KF_OK)
+            return s;
+        launched = k + 1;
+        if (check && (k + 1) % CHUNK == 0 && k + 1 < p->max_iters) {
+            s = read_state(ctx);
+            if (s != KF_OK)
+                return s;
+            if (ctx->state_host->done)
+                break;
+        }
+    }
+    if (adj && launched > 0) {
+        s = kf_launch_finalize(ctx, nb, (launched - 1) & 1);
+        if (s != KF_OK)
+            return s;
+    }
+    // The step kernels skip the energy of voxels that do not move; one forward pass on the
+    // final field gives the energy the optimiser reached.  With a list, the result's compact
+    // set (iters & 1) is chosen on the device, so the field, the energy and the state need one
+    // readback; the energy of a diverged field is computed and ignored.
+    if (listed) {
+        s = kf_active_scatter(live, u->buf[cur0], count);
+        if (s == KF_OK)
+            s = kf_active_energy(live, can, p, count);
+        if (s != KF_OK)
+            return s;
+    }
+    s = read_state(ctx);
+    if (s != KF_OK)
+        return s;
+    const DeformState h = *ctx->state_host;
+    if (!listed)
+        u->cur = cur0 ^ (int)(h.iters & 1u);
+    double energy = 0.0;
+    if (p->method == KF_METHOD_ADJOINT && h.diverged == LLONG_MAX) {
+        if (listed) {
+            energy = h.list_energy;
+        } else {
+            s = kf_energy_forward(live, can, u, p, &energy, nullptr);
+            if (s != KF_OK)
+                return s;
+        }
+    }
+    if (listed && !h.nonfinite) {
+        // The list covers every voxel a warp of live by u can change (see k_deform_active.hip).
+        ctx->active.warp_vol = live;
+        ctx->active.warp_field = u;
+        ctx->active.warp_count = count;
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->iters = (int32_t)h.iters;
+        stats->diverged_index = h.diverged == LLONG_MAX ? -1 : (int64_t)h.diverged;
+        stats->max_update = h.last_max;
+        stats->energy = energy;
+        stats->active_voxels = h.last_active;
+        stats->sweep = listed ? KF_SWEEP_ACTIVE : KF_SWEEP_DENSE;
+        stats->listed_voxels = count;
+    }
+    if (h.diverged != LLONG_MAX) {
+        kf_set_error("deformation field diverged at voxel " + std::to_string((long long)h.diverged));
+        return KF_E_DIVERGED;
+    }
+    return KF_OK;
+}
+
 kf_status kf_check_triplet(const kf_volume *live, const kf_volume *can, const kf_field *u)
 {
     KF_CHECK_ARG(live && can && u, "null argument");
@@ -627,105 +740,7 @@ kf_status kf_energy_backward(const kf_volume *live, const kf_volume *can, const 
 kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, const kf_params *p,
                     kf_deform_stats *stats)
 {
-    kf_status s = kf_check_triplet(live, can, u);
-    if (s != KF_OK)
-        return s;
-    KF_CHECK_ARG(kf_valid_params(p), "invalid params");
-    kf_context *ctx = live->ctx;
-    DeviceGuard dg(ctx->device);
-    s = kf_state_reset(ctx, p->stop_max_update);
-    if (s != KF_OK)
-        return s;
-    const int cur0 = u->cur;
-    const bool check = p->stop_max_update > 0.0;
-    const bool adj = p->method == KF_METHOD_ADJOINT;
-    // Active-set sweep (k_deform_active.hip): count the voxels that can move, list them, step
-    // only them (in compact arrays) and write the result back into the field's buffer.
-    bool listed = false;
-    unsigned int count = 0;
-    if (adj && p->max_iters > 0 && p->sweep != KF_SWEEP_DENSE && kf_active_supported(live)) {
-        s = kf_active_count(live, u->buf[cur0], p);
-        if (s == KF_OK)
-            s = read_state(ctx);
-        if (s != KF_OK)
-            return s;
-        count = ctx->state_host->listed;
-        listed = p->sweep == KF_SWEEP_ACTIVE || (int64_t)count * 8 <= live->n;
-        if (listed && (s = kf_active_prepare(live, u->buf[cur0], count)) != KF_OK)
-            return s;
-    }
-    // A list step is a few microseconds; check the stop rule less often so the host stays ahead.
-    const int CHUNK = listed ? 64 : 16;
-    // Launch k reads buffer cur0 ^ (k & 1) and writes the other one.  An adjoint launch k folds
-    // launch k - 1 in its first block, so launch k itself may run before the stop rule (or a
-    // divergence) of k - 1 is known: it only writes the buffer that is not the result, and
-    // launch k + 1 sees `done` and does nothing.
-    int launched = 0;
-    unsigned int nb = 0;
-    for (int k = 0; k < p->max_iters; k++) {
-        float *in = u->buf[cur0 ^ (k & 1)];
-        float *out = u->buf[cur0 ^ ((k + 1) & 1)];
-        s = listed ? kf_launch_adj_list(live, can, p, k, count, &nb)
-            : adj  ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr, nullptr, k, &nb)
-                   : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP, k);
-        if (s != KF_OK)
-            return s;
-        launched = k + 1;
-        if (check && (k + 1) % CHUNK == 0 && k + 1 < p->max_iters) {
-            s = read_state(ctx);
-            if (s != KF_OK)
-                return s;
-            if (ctx->state_host->done)
-                break;
-        }
-    }
-    if (adj && launched > 0) {
-        s = kf_launch_finalize(ctx, nb, (launched - 1) & 1);
-        if (s != KF_OK)
-            return s;
-    }
-    s = read_state(ctx);
-    if (s != KF_OK)
-        return s;
-    const DeformState h = *ctx->state_host;
-    if (listed) {
-        s = kf_active_scatter(live, u->buf[cur0], count, h.iters);
-        if (s != KF_OK)
-            return s;
-    } else {
-        u->cur = cur0 ^ (int)(h.iters & 1u);
-    }
-    double energy = 0.0;
-    if (p->method == KF_METHOD_ADJOINT && h.diverged == LLONG_MAX) {
-        // The step kernels skip the energy of voxels that do not move; one forward pass on the
-        // final field gives the energy the optimiser reached (over the list, when there is one).
-        if (listed) {
-            s = kf_active_energy(live, can, p, count, h.iters);
-            if (s == KF_OK)
-                s = read_state(ctx);
-            if (s == KF_OK)
-                energy = ctx->state_host->energy;
-        } else {
-            s = kf_energy_forward(live, can, u, p, &energy, nullptr);
-        }
-        if (s != KF_OK)
-            return s;
-    }
-    if (stats) {
-        std::memset(stats, 0, sizeof(*stats));
-        stats->iters = (int32_t)h.iters;
-        stats->diverged_index = h.diverged == LLONG_MAX ? -1 : (int64_t)h.diverged;
-        stats->max_update = h.last_max;
-        stats->energy = energy;
-        stats->active_voxels = h.last_active;
-        stats->sweep = listed ? KF_SWEEP_ACTIVE : KF_SWEEP_DENSE;
-        stats->listed_voxels = count;
-    }
-    if (h.diverged != LLONG_MAX) {
-        kf_set_error("deformation field diverged at voxel " + std::to_string((long long)h.diverged));
-        return KF_E_DIVERGED;
-    }
-    return KF_OK;
+    return kf_deform_impl(live, can, u, p, stats, false);
 }
 
 kf_status kf_warp(kf_volume *v, const kf_field *u)

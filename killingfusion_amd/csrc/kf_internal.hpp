@@ -56,6 +56,9 @@ struct DeformState {
     double last_max;                // max |du| of the last completed launch
     double energy;                  // energy of the last completed launch
     unsigned long long last_active; // active voxels of the last completed launch
+    unsigned int nonfinite;         // the active-set listing met a non-finite live SDF value
+    unsigned int pad2;
+    double list_energy;             // energy of the final field, active-set sweep
 };
 
 // Per-block partial results of one step launch, folded by k_finalize.
@@ -97,6 +100,12 @@ struct kf_active_ws {
     unsigned int *aux_host = nullptr;
     unsigned char *wtmp = nullptr;
     size_t wtmp_cap = 0;                  // bytes
+    // The list a kf_deform active-set sweep left behind covers every voxel a warp of `warp_vol`
+    // by `warp_field` can change (when no live SDF value was non-finite); the per-frame driver
+    // warps over it.  Any other use of the workspace clears it.
+    const kf_volume *warp_vol = nullptr;
+    const kf_field *warp_field = nullptr;
+    unsigned int warp_count = 0;
 };
 
 struct kf_context {
@@ -350,11 +359,12 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
 // as kf_launch_adj_step; kf_active_scatter writes compact set iters & 1 back into u.
 bool kf_active_supported(const kf_volume *live);
 void kf_active_release(kf_context *ctx);
-kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params *p);
+kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params *p, bool u_zero);
 kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int count);
 kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const kf_params *p, int iter,
                              unsigned int count, unsigned int *nb_out);
-kf_status kf_active_scatter(const kf_volume *live, float *u, unsigned int count, unsigned int iters);
+// Writes the final compact set (state->iters & 1, read on the device) back into u.
+kf_status kf_active_scatter(const kf_volume *live, float *u, unsigned int count);
 // kf_batch_deform over the voxels that can move in each volume (one launch per iteration for
 // the whole batch).  Sets *used = false and does nothing more when, without `force`, the
 // lists hold more than 1/8 of the voxels.
@@ -368,9 +378,14 @@ kf_status kf_list_moved(const kf_volume *v, const float *u, unsigned int *count,
 // SDF::update over those voxels only, in place (k_volume.hip).  Sets *done = false, changing
 // nothing, when the list holds more than 1/8 of the voxels or some phi is not finite.
 kf_status kf_warp_sparse(kf_volume *v, const kf_field *u, bool *done);
-// Energy of compact set iters & 1 into state->energy (resets the state first).
-kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count,
-                           unsigned int iters);
+// Energy of the final compact set into state->list_energy (no other state field changes).
+kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count);
+// SDF::update of v by u over the list a kf_deform active-set sweep of (v, u) just left behind
+// (k_volume.hip); *done = false, changing nothing, when there is none.
+kf_status kf_warp_after_deform(kf_volume *v, const kf_field *u, bool *done);
+// kf_deform; u_zero: the caller has just set every u to +0 (the listing skips reading u).
+kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *u, const kf_params *p,
+                         kf_deform_stats *stats, bool u_zero);
 // Words of a nonzero mask for these dims (0: the row kernel does not apply).
 size_t kf_adj_mask_words(const int32_t dims[3]);
 // Computes the nonzero mask of field values u.

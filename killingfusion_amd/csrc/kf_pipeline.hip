@@ -96,10 +96,15 @@ kf_status kf_pipeline_process_frame(kf_pipeline *pl, const float *depth, kf_fram
         if (s == KF_OK)
             s = kf_field_fill(pl->field, zero);
         if (s == KF_OK && (d.params.use_data || d.params.use_levelset || d.params.use_killing))
-            s = kf_deform(pl->live, pl->can, pl->field, &d.params, &fs.deform);
+            s = kf_deform_impl(pl->live, pl->can, pl->field, &d.params, &fs.deform, true);  // u was just zeroed
         KF_HIP(hipEventRecord(pl->ev[2], st));
-        if (s == KF_OK)
-            s = kf_warp(pl->live, pl->field);
+        if (s == KF_OK) {
+            // an active-set deform leaves the list of every voxel the warp can change
+            bool warped = false;
+            s = kf_warp_after_deform(pl->live, pl->field, &warped);
+            if (s == KF_OK && !warped)
+                s = kf_warp(pl->live, pl->field);
+        }
         if (s == KF_OK)
             s = kf_fuse(pl->can, pl->live);
         KF_HIP(hipEventRecord(pl->ev[3], st));

@@ -95,6 +95,17 @@ def test_active_equals_dense_sphere(ctx, sphere, dtype):
     assert 0 < b[1]["active"] <= b[1]["listed"] < live.n // 20
 
 
+@pytest.mark.parametrize("terms", [(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1)])
+def test_active_equals_dense_energy_terms(ctx, sphere, terms):
+    """Each energy term alone and in pairs (EnergyTypeUsed, config.cpp:23)."""
+    d, s_, k = terms
+    print(f"case: 64^3 sphere, f32, data {d} level set {s_} Killing {k}, 40 iterations", flush=True)
+    live, can = sphere["f32"]
+    kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint", max_iters=40, use_data=bool(d),
+                             use_levelset=bool(s_), use_killing=bool(k))
+    assert_same(run(ctx, live, can, kp, "dense"), run(ctx, live, can, kp, "active"), f"terms {terms}")
+
+
 @pytest.mark.parametrize("dims", [(61, 45, 58), (70, 45, 58), (48, 48, 48)])
 def test_active_equals_dense_uneven(ctx, dims):
     """Odd and even shapes: N % 4 != 0 runs the dense one-voxel-per-thread kernel."""
