@@ -128,6 +128,11 @@ kf_status kf_context_destroy(kf_context *ctx);
 kf_status kf_context_synchronize(kf_context *ctx);
 /* The hipStream_t the context's work is ordered on (for callers that time or chain work). */
 void *kf_context_stream(kf_context *ctx);
+/* Page-locked host memory (no reference counterpart): depth frames held in it reach the GPU
+ * by DMA, without the staging copy a pageable buffer costs (kf_integrate_depth,
+ * kf_pipeline_process_frame).  Free with kf_host_free. */
+kf_status kf_host_alloc(size_t bytes, void **out);
+kf_status kf_host_free(void *p);
 
 /* ---- SDF volume (SDF.h:16-200) ---- */
 /* SDF::computeVoxelGridSize, SDF.cpp:117-124: int((max-min)/voxel)+1 per axis. */

@@ -6,4 +6,4 @@ workload generators (`scenes`).
 """
 from .api import (Context, DisplacementField, IndexedMesh, KillingFusion, KillingParams, SDF, SimpleMesh,  # noqa: F401
                   batch_autotune, batch_deform, compute_displacement_field, dims_from_bounds, energy_backward,
-                  energy_forward, energy_gradient_ref, field_from_rigid, rigid_align)
+                  energy_forward, energy_gradient_ref, field_from_rigid, host_array, rigid_align)

@@ -28,6 +28,20 @@ def _f32(a):
 _SWEEPS = {"auto": 0, "dense": 1, "active": 2}  # kf_sweep
 
 
+def host_array(shape, dtype=np.float32) -> np.ndarray:
+    """A numpy array in page-locked host memory (kf_host_alloc), freed with the array: frames
+    kept in it upload to the GPU by DMA, without a staging copy."""
+    L = _lib.load()
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) * dt.itemsize
+    ptr = C.c_void_p()
+    check(L.kf_host_alloc(max(n, 1), C.byref(ptr)), "kf_host_alloc")
+    buf = (C.c_char * max(n, 1)).from_address(ptr.value)
+    arr = np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+    weakref.finalize(buf, L.kf_host_free, C.c_void_p(ptr.value))
+    return arr
+
+
 @dataclass
 class KillingParams:
     """Optimiser parameters (the reference's config.cpp:4-57 globals as a runtime struct).

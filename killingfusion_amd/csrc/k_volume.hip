@@ -42,11 +42,13 @@ __global__ void k_decode(float *dst, const typename S::T *phi, int64_t n, float 
 struct Mat9 { double m[9]; };
 struct Mat16 { double m[16]; };
 
-// SDF::integrateDepthFrame, SDF.cpp:149-196.
-template <class S>
+// SDF::integrateDepthFrame, SDF.cpp:149-196.  FRESH: the same pass also resets the volume
+// (SDF.cpp:133-134: phi = Max + eps, w = 0) before integrating, as a reset followed by an
+// integration would -- same values, one write per voxel and no read.
+template <class S, bool FRESH>
 __global__ void k_integrate(typename S::T *phi, int32_t *w, GridDims g, double mnx, double mny, double mnz,
                             double voxel, double clip, float scale, const float *__restrict__ depth, int W,
-                            int H, Mat9 K, Mat16 M, double min_depth, double max_depth)
+                            int H, Mat9 K, Mat16 M, double min_depth, double max_depth, float oob_m)
 {
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= g.n)
@@ -66,19 +68,25 @@ __global__ void k_integrate(typename S::T *phi, int32_t *w, GridDims g, double m
     const double py = K.m[3] * q0 + K.m[4] * q1 + K.m[5] * 1.0;
     // roundf on the float-narrowed coordinate, SDF.cpp:167,171.
     const int col = (int)roundf((float)(px + 0.5));
-    if (col < 0 || col >= W)
-        return;
     const int row = (int)roundf((float)(py + 0.5));
-    if (row < 0 || row >= H)
+    bool take = col >= 0 && col < W && row >= 0 && row < H;
+    double dist = 0.0;
+    if (take) {
+        const double d = (double)depth[(int64_t)row * W + col];
+        take = !(d < min_depth || d > max_depth);
+        dist = d - c[2];
+        take = take && !(dist < -clip);
+    }
+    const typename S::T init = S::enc(oob_m, scale);
+    if (!take) {
+        if (FRESH) {
+            phi[i] = init;
+            w[i] = 0;
+        }
         return;
-    const double d = (double)depth[(int64_t)row * W + col];
-    if (d < min_depth || d > max_depth)
-        return;
-    const double dist = d - c[2];
-    if (dist < -clip)
-        return;
-    const int32_t wi = w[i];
-    const double old = (double)S::load(phi, i, scale);
+    }
+    const int32_t wi = FRESH ? 0 : w[i];
+    const double old = FRESH ? (double)S::dec(init, scale) : (double)S::load(phi, i, scale);
     const double nv = (old * (double)wi + dist) / (double)(wi + 1);
     phi[i] = S::enc((float)nv, scale);
     w[i] = wi + 1;
@@ -289,7 +297,7 @@ kf_status kf_launch_download_volume(const kf_volume *v, float *dphi)
 }
 
 kf_status kf_launch_integrate(kf_volume *v, const float *ddepth, int w, int h, const double K[9],
-                              const double W2C[16], double min_depth, double max_depth)
+                              const double W2C[16], double min_depth, double max_depth, bool fresh)
 {
     Mat9 k;
     Mat16 m;
@@ -298,10 +306,16 @@ kf_status kf_launch_integrate(kf_volume *v, const float *ddepth, int w, int h, c
     for (int i = 0; i < 16; i++)
         m.m[i] = W2C[i];
     KF_DISPATCH_DTYPE(v->dtype, S, {
-        hipLaunchKernelGGL((k_integrate<S>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream,
-                           (typename S::T *)v->phi, v->w, grid_of(v), v->min_corner[0], v->min_corner[1],
-                           v->min_corner[2], v->voxel, v->clip, v->scale(), ddepth, w, h, k, m, min_depth,
-                           max_depth);
+        if (fresh)
+            hipLaunchKernelGGL((k_integrate<S, true>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream,
+                               (typename S::T *)v->phi, v->w, grid_of(v), v->min_corner[0], v->min_corner[1],
+                               v->min_corner[2], v->voxel, v->clip, v->scale(), ddepth, w, h, k, m, min_depth,
+                               max_depth, (float)v->oob());
+        else
+            hipLaunchKernelGGL((k_integrate<S, false>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream,
+                               (typename S::T *)v->phi, v->w, grid_of(v), v->min_corner[0], v->min_corner[1],
+                               v->min_corner[2], v->voxel, v->clip, v->scale(), ddepth, w, h, k, m, min_depth,
+                               max_depth, (float)v->oob());
     });
     KF_LAUNCH_CHECK();
     return KF_OK;

@@ -177,6 +177,25 @@ const char *kf_last_error(void) { return g_last_error.c_str(); }
 
 int kf_api_version(void) { return KF_API_VERSION; }
 
+kf_status kf_host_alloc(size_t bytes, void **out)
+{
+    KF_CHECK_ARG(out && bytes > 0, "null out or zero size");
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        kf_set_error("page-locked host allocation failed");
+        return KF_E_OOM;
+    }
+    return KF_OK;
+}
+
+kf_status kf_host_free(void *p)
+{
+    if (p)
+        KF_HIP(hipHostFree(p));
+    return KF_OK;
+}
+
 kf_status kf_params_default(kf_params *p)
 {
     KF_CHECK_ARG(p, "params is null");
@@ -517,6 +536,14 @@ kf_status kf_field_add(kf_field *dst, const kf_field *src)
 kf_status kf_integrate_depth(kf_volume *v, const float *depth, int32_t width, int32_t height, const double K[9],
                              const double C2W[16], double min_depth, double max_depth)
 {
+    return kf_integrate_depth_impl(v, depth, width, height, K, C2W, min_depth, max_depth, false);
+}
+
+}  // extern "C"
+
+kf_status kf_integrate_depth_impl(kf_volume *v, const float *depth, int32_t width, int32_t height, const double K[9],
+                                  const double C2W[16], double min_depth, double max_depth, bool fresh)
+{
     KF_CHECK_ARG(v && depth && K && C2W, "null argument");
     KF_CHECK_ARG(width > 0 && height > 0 && (int64_t)width * height <= ((int64_t)1 << 28), "bad frame size");
     double W2C[16];
@@ -529,7 +556,7 @@ kf_status kf_integrate_depth(kf_volume *v, const float *depth, int32_t width, in
     hipError_t e = hipMemcpyAsync(dd, depth, bytes, hipMemcpyHostToDevice, s);
     kf_status st = KF_OK;
     if (e == hipSuccess)
-        st = kf_launch_integrate(v, dd, width, height, K, W2C, min_depth, max_depth);
+        st = kf_launch_integrate(v, dd, width, height, K, W2C, min_depth, max_depth, fresh);
     (void)hipFreeAsync(dd, s);
     if (e != hipSuccess) {
         kf_set_error(std::string("depth upload: ") + hipGetErrorString(e));
@@ -538,7 +565,6 @@ kf_status kf_integrate_depth(kf_volume *v, const float *depth, int32_t width, in
     return st;
 }
 
-}  // extern "C"
 
 kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *u, const kf_params *p,
                          kf_deform_stats *stats, bool u_zero)

@@ -331,8 +331,12 @@ kf_status kf_launch_finalize(kf_context *ctx, unsigned int nb, int parity);
 kf_status kf_launch_fill_volume(kf_volume *v);
 kf_status kf_launch_upload_volume(kf_volume *v, const float *dphi_f32, const int32_t *dw);
 kf_status kf_launch_download_volume(const kf_volume *v, float *dphi_f32);
+// fresh: reset the volume (phi = Max + eps, w = 0) in the same pass, before integrating.
 kf_status kf_launch_integrate(kf_volume *v, const float *ddepth, int w, int h, const double K[9],
-                              const double W2C[16], double min_depth, double max_depth);
+                              const double W2C[16], double min_depth, double max_depth, bool fresh = false);
+// kf_integrate_depth, optionally into a volume reset in the same pass.
+kf_status kf_integrate_depth_impl(kf_volume *v, const float *depth, int32_t width, int32_t height, const double K[9],
+                                  const double C2W[16], double min_depth, double max_depth, bool fresh);
 kf_status kf_launch_warp(kf_volume *v, const kf_field *u, void *tmp_phi, int32_t *tmp_w);
 kf_status kf_launch_fuse(kf_volume *can, const kf_volume *live);
 kf_status kf_launch_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *u);

@@ -80,18 +80,16 @@ kf_status kf_pipeline_process_frame(kf_pipeline *pl, const float *depth, kf_fram
     kf_status s = KF_OK;
     KF_HIP(hipEventRecord(pl->ev[0], st));
     if (pl->frame == 0) {
-        s = kf_volume_reset(pl->can);
-        if (s == KF_OK)
-            s = kf_integrate_depth(pl->can, depth, d.width, d.height, d.K, d.C2W, d.min_depth, d.max_depth);
+        // reset + integrate in one pass
+        s = kf_integrate_depth_impl(pl->can, depth, d.width, d.height, d.K, d.C2W, d.min_depth, d.max_depth, true);
         if (s == KF_OK)
             s = kf_field_fill(pl->field, zero);
         KF_HIP(hipEventRecord(pl->ev[1], st));
         KF_HIP(hipEventRecord(pl->ev[2], st));
         KF_HIP(hipEventRecord(pl->ev[3], st));
     } else {
-        s = kf_volume_reset(pl->live);
-        if (s == KF_OK)
-            s = kf_integrate_depth(pl->live, depth, d.width, d.height, d.K, d.C2W, d.min_depth, d.max_depth);
+        s = kf_integrate_depth_impl(pl->live, depth, d.width, d.height, d.K, d.C2W, d.min_depth, d.max_depth,
+                                    true);
         KF_HIP(hipEventRecord(pl->ev[1], st));
         if (s == KF_OK)
             s = kf_field_fill(pl->field, zero);

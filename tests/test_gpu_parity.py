@@ -48,6 +48,20 @@ def test_integrate_bit_exact(c1):
     assert (w > 0).sum() > 10000
 
 
+def test_integrate_from_page_locked_frame(ctx, c1):
+    """A depth frame in page-locked host memory (kf_host_alloc) integrates as a pageable one."""
+    from killingfusion_amd import host_array
+    print("case: integrate 64^3 sphere frame 2 from a page-locked buffer", flush=True)
+    h = host_array(c1["d2"].shape, np.float32)
+    h[...] = c1["d2"]
+    v = SDF(ctx, C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"], C1["trunc"])
+    v.integrateDepthFrame(h, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+    phi, w = v.download()
+    rphi, rw = c1["live"].download()
+    assert np.array_equal(phi, rphi) and (w == rw).all()
+    v.close()
+
+
 def test_fuse_and_warp_bit_exact(ctx, c1):
     print("case: fuse / fuse_warped / warp 64^3", flush=True)
     g = c1["g"]
