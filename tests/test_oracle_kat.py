@@ -511,6 +511,62 @@ def test_adjoint_deform_steps_equal_dense_gradient_steps():
         assert np.array_equal(ud, u)
 
 
+def test_voxels_outside_the_start_list_never_move():
+    # The active-set sweep lists C = {j : u_j has a nonzero bit pattern, or live[j] in the band}
+    # once per deform call.  Dense Jacobi steps (every voxel's gradient, applied where the step's
+    # band mask is set) must leave every voxel outside C at exactly +0, never mark it active,
+    # and the energy of every iterate must be carried by C and the voxels just below C
+    # (i with some i + e_b in C) -- the set the active-set energy pass visits.
+    n = 20
+    rng = np.random.default_rng(9)
+    g = O.OGrid(dims=(n, n + 1, n - 1), voxel=0.01, oob=0.03 + 1e-5)
+    p = O.OParams(voxel_size=0.01, max_surface=0.03, clip=0.03, alpha=0.1, omega_k=0.5, gamma=0.1, omega_s=0.2)
+    z, y, x = np.meshgrid(np.arange(n - 1), np.arange(n + 1), np.arange(n), indexing="ij")
+    live = (0.01 * (np.sqrt((x - 9.6) ** 2 + (y - 10.2) ** 2 + (z - 9.1) ** 2) - 4.5)).ravel()
+    live = np.where(np.abs(live) < 0.03, live, 0.03 + 1e-5)
+    can = live + rng.uniform(-0.002, 0.002, live.size)
+    u0 = np.zeros((3, g.n))
+    box = (z < 3) & (y > 15) & (x > 14)  # away from the surface: moved only by the starting field
+    u0[:, box.ravel()] = rng.uniform(-0.3, 0.3, (3, int(box.sum())))
+    hi, lo = p.max_surface - p.epsilon, -p.clip
+    listed = (u0 != 0).any(axis=0) | ~((live > hi) | (live < lo))
+    L3 = listed.reshape(n - 1, n + 1, n)
+    below = np.zeros_like(L3)
+    below[:, :, :-1] |= L3[:, :, 1:]
+    below[:, :-1, :] |= L3[:, 1:, :]
+    below[:-1, :, :] |= L3[1:, :, :]
+    carried = (L3 | below).ravel()
+    assert 0 < listed.sum() < g.n // 3
+    u = u0.copy()
+    for it in range(12):
+        grad, mask = O.adj_gradient(live, can, u, g, p)
+        assert not (mask.astype(bool) & ~listed).any(), it
+        u = u - p.alpha * grad * mask
+        assert not u[:, ~listed].any() and not np.signbit(u[:, ~listed]).any(), it
+        _, ev, _ = O.adj_energy(live, can, u, g, p, per_voxel=True)
+        assert not ev[~carried].any(), it
+        assert ev[carried & ~listed].any()  # the terms just below C are not all zero
+
+
+def test_warp_leaves_unmoved_voxels_alone():
+    # SDF::update (SDF.cpp:300-328) at a voxel with u = 0 reads its own cell with zero weights:
+    # phi and weight come back unchanged for finite phi (a -0 phi may come back as +0), which
+    # is what lets the warp run over the voxels with u != 0 only.
+    n = 10
+    rng = np.random.default_rng(3)
+    g = O.OGrid(dims=(n, n, n), voxel=0.01, oob=0.03 + 1e-5)
+    phi = rng.uniform(-0.04, 0.04, g.n)
+    phi[[5, 77, 400]] = -0.0
+    w = rng.integers(0, 9, g.n).astype(np.int64)
+    u = np.where(rng.random((3, g.n)) < 0.1, rng.uniform(-1.5, 1.5, (3, g.n)), 0.0)
+    moved = (u != 0).any(axis=0)
+    wp, ww = phi.copy(), w.copy()
+    O.warp(wp, ww, u, g)
+    keep = ~moved & ~((phi == 0) & np.signbit(phi))
+    assert np.array_equal(wp[keep], phi[keep]) and np.array_equal(ww[~moved], w[~moved])
+    assert (wp[moved] != phi[moved]).any()
+
+
 def test_adjoint_local_energy_carries_every_term_of_a_voxel():
     # kfo_adj_energy_local(x, y, z) must change exactly as the total energy does when u at
     # (x, y, z) moves (including the zero-padding ghost terms on low faces).
