@@ -61,7 +61,33 @@ private:
     kf_context *h_ = nullptr;
 };
 
-/* Optimiser parameters: the config.cpp:4-57 globals. */
+/* Page-locked host memory for depth frames (kf_host_alloc): uploads from it are DMA without a
+ * staging copy.  Move-only. */
+template <class T>
+class HostBuffer {
+public:
+    explicit HostBuffer(size_t count) : n_(count)
+    {
+        void *p = nullptr;
+        check(kf_host_alloc(sizeof(T) * (count ? count : 1), &p), "kf_host_alloc");
+        p_ = static_cast<T *>(p);
+    }
+    ~HostBuffer() { kf_host_free(p_); }
+    HostBuffer(const HostBuffer &) = delete;
+    HostBuffer &operator=(const HostBuffer &) = delete;
+    HostBuffer(HostBuffer &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr, o.n_ = 0; }
+    T *data() { return p_; }
+    const T *data() const { return p_; }
+    size_t size() const { return n_; }
+    T &operator[](size_t i) { return p_[i]; }
+
+private:
+    T *p_ = nullptr;
+    size_t n_ = 0;
+};
+
+/* Optimiser parameters: the config.cpp:4-57 globals; `sweep` (kf_sweep) picks the adjoint
+ * optimiser's dense or active-set sweep (KF_SWEEP_AUTO by default). */
 struct Params : kf_params {
     Params() { kf_params_default(this); }
     /* The values quoted in config.cpp:32-41 (alpha 0.1, omega_k 0.5, gamma 0.1, omega_s 0.2)
@@ -252,6 +278,15 @@ public:
     /* SDF::integrateDepthFrame(depthFrame, C2W, K, minDepth, maxDepth), SDF.cpp:137-197;
      * depth: row-major float32 metres, 0 = invalid; C2W and K row-major. */
     void integrateDepthFrame(const std::vector<float> &depth, int width, int height, const std::array<double, 16> &C2W,
+                             const std::array<double, 9> &K, double minDepth, double maxDepth)
+    {
+        if (depth.size() != (size_t)width * height)
+            throw std::invalid_argument("depth frame size");
+        check(kf_integrate_depth(h_, depth.data(), width, height, K.data(), C2W.data(), minDepth, maxDepth),
+              "kf_integrate_depth");
+    }
+    /* The same from a page-locked frame. */
+    void integrateDepthFrame(const HostBuffer<float> &depth, int width, int height, const std::array<double, 16> &C2W,
                              const std::array<double, 9> &K, double minDepth, double maxDepth)
     {
         if (depth.size() != (size_t)width * height)

@@ -111,6 +111,29 @@ int main(int argc, char **argv)
         }
         expect(uz[0] > 0.0 && uz[1] > uz[0], "displacement moves the live samples toward the canonical plane");
         {
+            // the dense and active-set sweeps give the same field
+            kf::DisplacementField ud(ctx, dims, voxel), ua(ctx, dims, voxel);
+            kf::Params pd = p, pa = p;
+            pd.max_iters = pa.max_iters = 25;
+            pd.sweep = KF_SWEEP_DENSE;
+            pa.sweep = KF_SWEEP_ACTIVE;
+            kf_deform_stats sd = kf::computeDisplacementField(&live, &plane, &ud, pd);
+            kf_deform_stats sa = kf::computeDisplacementField(&live, &plane, &ua, pa);
+            expect(sd.sweep == KF_SWEEP_DENSE && sa.sweep == KF_SWEEP_ACTIVE && sa.listed_voxels > 0 &&
+                       sa.iters == sd.iters && sa.max_update == sd.max_update && ud.download() == ua.download(),
+                   "dense and active-set sweeps give the same field");
+        }
+        {
+            // a page-locked frame integrates as a std::vector one
+            kf::HostBuffer<float> pinned(depth2.size());
+            for (size_t i = 0; i < depth2.size(); i++)
+                pinned[i] = depth2[i];
+            kf::SDF again(ctx, dims, mn, voxel, trunc, trunc);
+            again.integrateDepthFrame(pinned, W, H, I4, K, 0.1, 3.0);
+            const auto x1 = again.download(), x2 = live.download();
+            expect(x1.first == x2.first && x1.second == x2.second, "integration from a page-locked frame");
+        }
+        {
             kf::DisplacementField u(ctx, dims, voxel);
             std::vector<float> s = u.download();
             s[(size_t)(20 * dims[1] + 12) * dims[0] + 16] = NAN;
