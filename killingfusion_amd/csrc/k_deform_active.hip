@@ -11,7 +11,7 @@
 //
 // Per call:
 //  * k_active_mark streams u and phi_live once: one bit per voxel (64-voxel words) and a count
-//    per span of 16384 voxels; k_active_scan turns the counts into span offsets and the total.
+//    per span of 4096 voxels; k_active_scan turns the counts into span offsets and the total.
 //  * k_active_compact lists C in ascending voxel order (one slot per listed voxel);
 //    k_active_neighbours copies each listed u_j into a compact array (one float4 per slot) and
 //    finds the slots of the 12 neighbours the Killing term reads (-1: outside the grid or not
@@ -59,17 +59,18 @@ __global__ __launch_bounds__(BLOCK) void k_active_mark(PhiRead<S> L, const float
     for (int k = 0; k < SPAN / BLOCK; k++) {
         const unsigned int w0 = base + (unsigned int)(k * BLOCK + w * 64);
         const unsigned int i = w0 + (unsigned int)lane;
-        bool cand = false;
+        bool cand = false, bad = false;
         if (i < n32) {
             unsigned int ub = 0u;
             if (READ_U)
                 ub = __float_as_uint(kf_load_stream(u + i)) | __float_as_uint(kf_load_stream(u + n32 + i)) |
                      __float_as_uint(kf_load_stream(u + 2u * n32 + i));
             const float live = S::load(L.p, (int64_t)i, L.scale);
-            if (!isfinite(live))
-                atomicOr(nonfinite, 1u);
+            bad = !isfinite(live);
             cand = ub != 0u || !(live > hi || live < lo);
         }
+        if (__ballot(bad) != 0ull && lane == 0)
+            atomicOr(nonfinite, 1u);
         const unsigned long long bal = __ballot(cand);
         if (lane == 0 && w0 < n32)
             mask[w0 >> 6] = bal;
@@ -146,17 +147,17 @@ __global__ __launch_bounds__(BLOCK) void k_moved_mark(const typename S::T *__res
     for (int k = 0; k < SPAN / BLOCK; k++) {
         const unsigned int w0 = base + (unsigned int)(k * BLOCK + w * 64);
         const unsigned int i = w0 + (unsigned int)lane;
-        bool cand = false;
+        bool cand = false, bad = false;
         if (i < n32) {
             const float u0 = kf_load_stream(u + i), u1 = kf_load_stream(u + n32 + i);
             const float u2 = kf_load_stream(u + 2u * n32 + i);
             const float f = S::load(phi, (int64_t)i, scale);
-            const bool finite = isfinite(f);
-            if (!finite)
-                atomicOr(flag, 1u);
+            bad = !isfinite(f);
             cand = ((__float_as_uint(u0) | __float_as_uint(u1) | __float_as_uint(u2)) & 0x7fffffffu) != 0u ||
-                   __float_as_uint(f) == 0x80000000u || !finite;
+                   __float_as_uint(f) == 0x80000000u || bad;
         }
+        if (__ballot(bad) != 0ull && lane == 0)
+            atomicOr(flag, 1u);
         const unsigned long long bal = __ballot(cand);
         if (lane == 0 && w0 < n32)
             mask[w0 >> 6] = bal;
