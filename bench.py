@@ -251,6 +251,10 @@ def main(argv=None):
     ap.add_argument("--no-sequence", action="store_true", help="skip the 100-frame sequence measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the known-traffic copy kernel used to scale rocprofv3 PMC counters")
+    ap.add_argument("--sequence-only", action="store_true",
+                    help="only run the 100-frame sequence and print its measurement (for profiling)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only run the batched uneven-volume measurement and print it (for profiling)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build (one GPU per box).",
@@ -258,6 +262,12 @@ def main(argv=None):
         return 2
     if a.calibrate_copy:
         return calibrate_copy()
+    if a.sequence_only or a.batched_only:
+        from killingfusion_amd import Context
+        ctx = Context(0)
+        r = bench_sequence(ctx) if a.sequence_only else bench_batched(ctx, HipEvents())
+        print(json.dumps(r), flush=True)
+        return 0
 
     from killingfusion_amd import Context, KillingParams, SDF, scenes
 
