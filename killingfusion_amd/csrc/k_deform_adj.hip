@@ -54,9 +54,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_any(PhiRead<S> L, PhiRead<S> Cn, 
     double mx = 0.0, esum = 0.0;
     unsigned long long act = 0;
     if (i < g.n) {
-        const int x = (int)(i % g.nx);
-        const int y = (int)((i / g.nx) % g.ny);
-        const int z = (int)(i / ((int64_t)g.nx * g.ny));
+        int x, y, z;
+        kf_xyz(g, i, x, y, z);
         Nbhd nb;
         nb.uc[0] = u_in[i];
         nb.uc[1] = u_in[g.n + i];

@@ -223,9 +223,8 @@ __global__ __launch_bounds__(BLOCK) void k_ref_step(PhiView<S> L, PhiView<S> Cn,
     double mx = 0.0;
     unsigned long long act = 0;
     if (i < g.n) {
-        const int x = (int)(i % g.nx);
-        const int y = (int)((i / g.nx) % g.ny);
-        const int z = (int)(i / ((int64_t)g.nx * g.ny));
+        int x, y, z;
+        kf_xyz(g, i, x, y, z);
         float u0 = u_in[i], u1 = u_in[g.n + i], u2 = u_in[2 * g.n + i];
         // SDF::getDistance(idx, disp), SDF.cpp:570-575; skip test KillingFusion.cpp:272-274.
         float psi = L.trilinear(x, y, z, u0, u1, u2);

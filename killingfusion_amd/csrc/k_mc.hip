@@ -47,9 +47,7 @@ struct Cells {
     // Returns the cube index, or -1 when the cell is skipped or has no triangles.
     __device__ inline int classify(int64_t i, double val[8], int &x, int &y, int &z) const
     {
-        x = (int)(i % g.nx);
-        y = (int)((i / g.nx) % g.ny);
-        z = (int)(i / ((int64_t)g.nx * g.ny));
+        kf_xyz(g, i, x, y, z);
         const double d = at(x, y, z);
         if (d > max_surface || d < -clip)
             return -1;

@@ -53,9 +53,8 @@ __global__ void k_integrate(typename S::T *phi, int32_t *w, GridDims g, double m
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= g.n)
         return;
-    const int x = (int)(i % g.nx);
-    const int y = (int)((i / g.nx) % g.ny);
-    const int z = (int)(i / ((int64_t)g.nx * g.ny));
+    int x, y, z;
+    kf_xyz(g, i, x, y, z);
     const double Z = voxel * (z + 0.5) + mnz;
     const double Y = voxel * (y + 0.5) + mny;
     const double X = voxel * (x + 0.5) + mnx;
@@ -130,9 +129,8 @@ __global__ void k_warp(Reader<S> R, const float *__restrict__ u, typename S::T *
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= g.n)
         return;
-    const int x = (int)(i % g.nx);
-    const int y = (int)((i / g.nx) % g.ny);
-    const int z = (int)(i / ((int64_t)g.nx * g.ny));
+    int x, y, z;
+    kf_xyz(g, i, x, y, z);
     const double lx = (double)x + 0.5 + (double)u[i];
     const double ly = (double)y + 0.5 + (double)u[g.n + i];
     const double lz = (double)z + 0.5 + (double)u[2 * g.n + i];
@@ -154,9 +152,8 @@ __global__ void k_warp_list(Reader<S> R, const float *__restrict__ u, const unsi
         return;
     const GridDims g = R.g;
     const int64_t i = list[t];
-    const int x = (int)(i % g.nx);
-    const int y = (int)((i / g.nx) % g.ny);
-    const int z = (int)(i / ((int64_t)g.nx * g.ny));
+    int x, y, z;
+    kf_xyz(g, i, x, y, z);
     const double lx = (double)x + 0.5 + (double)u[i];
     const double ly = (double)y + 0.5 + (double)u[g.n + i];
     const double lz = (double)z + 0.5 + (double)u[2 * g.n + i];
@@ -212,9 +209,8 @@ __global__ void k_fuse_warped(typename S1::T *phi1, int32_t *w1, float scale1, R
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= g.n)
         return;
-    const int x = (int)(i % g.nx);
-    const int y = (int)((i / g.nx) % g.ny);
-    const int z = (int)(i / ((int64_t)g.nx * g.ny));
+    int x, y, z;
+    kf_xyz(g, i, x, y, z);
     const double lx = (double)x + 0.5 + (double)u[i];
     const double ly = (double)y + 0.5 + (double)u[g.n + i];
     const double lz = (double)z + 0.5 + (double)u[2 * g.n + i];

@@ -272,6 +272,23 @@ struct KfDeviceGuard {
     KfDeviceGuard &operator=(const KfDeviceGuard &) = delete;
 };
 
+// (x, y, z) of linear voxel index i, with 32-bit divisions when N < 2^32 (64-bit integer
+// division is a long emulated sequence per voxel).
+__device__ inline void kf_xyz(const GridDims &g, int64_t i, int &x, int &y, int &z)
+{
+    if (g.n < ((int64_t)1 << 32)) {
+        const unsigned int i32 = (unsigned int)i, nxy = (unsigned int)g.nx * (unsigned int)g.ny;
+        z = (int)(i32 / nxy);
+        const unsigned int r = i32 - (unsigned int)z * nxy;
+        y = (int)(r / (unsigned int)g.nx);
+        x = (int)(r - (unsigned int)y * (unsigned int)g.nx);
+    } else {
+        x = (int)(i % g.nx);
+        y = (int)((i / g.nx) % g.ny);
+        z = (int)(i / ((int64_t)g.nx * g.ny));
+    }
+}
+
 // MI355X dispatches workgroup b to XCD (b mod 8), and each XCD has its own L2.  This maps
 // the dispatch index to a tile index so that every XCD sweeps one contiguous range of tiles:
 // tiles that share halo rows and slices then run on the same XCD and hit its L2.
