@@ -34,6 +34,31 @@ __constant__ signed char c_corner[8][3] = {{1, 0, 0}, {0, 0, 0}, {0, 1, 0}, {1, 
 __constant__ unsigned char c_edge_corners[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
                                                      {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
 
+// Inclusive scan of v over the block in thread order, and the block total: a wavefront scan,
+// then the wavefront totals through LDS (`s`, BLOCK / 64 entries; two barriers).
+__device__ inline unsigned int block_scan_incl(unsigned int v, unsigned int *s, unsigned int &total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned int t = __shfl_up(v, o, 64);
+        if (lane >= o)
+            v += t;
+    }
+    if (lane == 63)
+        s[w] = v;
+    __syncthreads();
+    unsigned int wbase = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < BLOCK / 64; q++) {
+        wbase += q < w ? s[q] : 0u;
+        tot += s[q];
+    }
+    __syncthreads();
+    total = tot;
+    return wbase + v;
+}
+
 template <class S>
 struct Cells {
     const typename S::T *phi;
@@ -121,7 +146,7 @@ template <class S>
 __global__ __launch_bounds__(BLOCK) void k_mc_emit(Cells<S> C, const unsigned long long *offs, float *verts,
                                                    unsigned long long cap)
 {
-    __shared__ unsigned int s[BLOCK];
+    __shared__ unsigned int s[BLOCK / 64];
     const int64_t base = (int64_t)blockIdx.x * CHUNK;
     unsigned long long run = offs[blockIdx.x];
     const double hx = (double)C.g.nx / 2, hy = (double)C.g.ny / 2, hz = (double)C.g.nz / 2;
@@ -135,18 +160,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_emit(Cells<S> C, const unsigned lo
             if (ci >= 0)
                 cnt = c_tri_count[ci];
         }
-        // Block-wide exclusive scan of cnt in thread order (= cell order within the round).
-        s[threadIdx.x] = cnt;
-        __syncthreads();
-        for (int o = 1; o < BLOCK; o <<= 1) {
-            unsigned int v = ((int)threadIdx.x >= o) ? s[threadIdx.x - o] : 0u;
-            __syncthreads();
-            s[threadIdx.x] += v;
-            __syncthreads();
-        }
-        const unsigned int incl = s[threadIdx.x];
-        const unsigned int round_total = s[BLOCK - 1];
-        __syncthreads();
+        // Block-wide scan of cnt in thread order (= cell order within the round).
+        unsigned int round_total;
+        const unsigned int incl = block_scan_incl(cnt, s, round_total);
         if (cnt) {
             // VertexInterp (MarchingCubes.h:326-356) on the edges this case cuts.
             double p[8][3];
@@ -244,22 +260,14 @@ __global__ __launch_bounds__(BLOCK) void k_flag_sums(const unsigned char *flags,
 __global__ __launch_bounds__(BLOCK) void k_flag_ids(const unsigned char *flags, int64_t n,
                                                     const unsigned long long *offs, unsigned int *ids)
 {
-    __shared__ unsigned int s[BLOCK];
+    __shared__ unsigned int s[BLOCK / 64];
     const int64_t base = (int64_t)blockIdx.x * CHUNK;
     unsigned long long run = offs[blockIdx.x];
     for (int k = 0; k < ITEMS; k++) {
         const int64_t j = base + (int64_t)k * BLOCK + threadIdx.x;
         const unsigned int f = j < n ? flags[j] : 0u;
-        s[threadIdx.x] = f;
-        __syncthreads();
-        for (int o = 1; o < BLOCK; o <<= 1) {
-            unsigned int v = ((int)threadIdx.x >= o) ? s[threadIdx.x - o] : 0u;
-            __syncthreads();
-            s[threadIdx.x] += v;
-            __syncthreads();
-        }
-        const unsigned int incl = s[threadIdx.x], round_total = s[BLOCK - 1];
-        __syncthreads();
+        unsigned int round_total;
+        const unsigned int incl = block_scan_incl(f, s, round_total);
         if (f)
             ids[j] = (unsigned int)(run + incl - 1);
         run += round_total;
@@ -318,7 +326,7 @@ template <class S>
 __global__ __launch_bounds__(BLOCK) void k_mc_emit_indexed(Cells<S> C, EdgeGrid E, const unsigned long long *offs,
                                                            const unsigned int *ids, unsigned int *tris)
 {
-    __shared__ unsigned int s[BLOCK];
+    __shared__ unsigned int s[BLOCK / 64];
     const int64_t base = (int64_t)blockIdx.x * CHUNK;
     unsigned long long run = offs[blockIdx.x];
     for (int k = 0; k < ITEMS; k++) {
@@ -331,16 +339,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc_emit_indexed(Cells<S> C, EdgeGrid 
             if (ci >= 0)
                 cnt = c_tri_count[ci];
         }
-        s[threadIdx.x] = cnt;
-        __syncthreads();
-        for (int o = 1; o < BLOCK; o <<= 1) {
-            unsigned int v = ((int)threadIdx.x >= o) ? s[threadIdx.x - o] : 0u;
-            __syncthreads();
-            s[threadIdx.x] += v;
-            __syncthreads();
-        }
-        const unsigned int incl = s[threadIdx.x], round_total = s[BLOCK - 1];
-        __syncthreads();
+        unsigned int round_total;
+        const unsigned int incl = block_scan_incl(cnt, s, round_total);
         const unsigned long long t0 = run + (incl - cnt);
         for (unsigned int t = 0; t < cnt; t++)
             for (int v = 0; v < 3; v++) {
