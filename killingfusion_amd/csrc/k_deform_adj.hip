@@ -27,8 +27,8 @@
 //    every voxel whose stencil sees a nonzero u, take that set from a bit mask of the 4-voxel
 //    groups of u that may be nonzero.
 //  * k_adj_any (any shape): one voxel per thread, neighbours read individually.
-// Blocks are numbered so each XCD sweeps a contiguous z range and the rows and slices shared
-// by neighbouring blocks stay in that XCD's L2.
+// Blocks go to the XCDs in chunks of 64 consecutive tiles dealt round-robin: neighbouring
+// tiles of a chunk share an XCD's L2, and the band's stencils spread over every XCD.
 #include "kf_adjoint.hpp"
 
 namespace {
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_any(PhiRead<S> L, PhiRead<S> Cn, 
     if (!step_gate<MODE>(st, prev, prev_nb, parity))
         return;
     const GridDims g = L.g;
-    const int64_t i = (int64_t)kf_xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const int64_t i = (int64_t)kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     double mx = 0.0, esum = 0.0;
     unsigned long long act = 0;
     if (i < g.n) {
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
     if (!step_gate<MODE>(st, prev, prev_nb, parity))
         return;
     const GridDims g = L.g;
-    const unsigned int tile = kf_xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned int tile = kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x);
     const int seg = (int)(tile % (unsigned)nseg);
     const int yt = (int)((tile / (unsigned)nseg) % (unsigned)nyt);
     const int z = (int)(tile / ((unsigned)nseg * (unsigned)nyt));
@@ -281,10 +281,13 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
         return;
     const GridDims g = L.g;
     const unsigned int n32 = nvox32(g);
+    // Tiles go to the XCDs in chunks of 64 (a z-slice at 256^3) dealt round-robin, so the
+    // stencils of the surface band spread over every XCD instead of keeping the two or three
+    // that hold the band busy after the others finish (256^3: 70 -> 57 us per launch).
     // Odd launches sweep the grid backwards: a launch reads the buffer the previous one wrote,
     // and starting where that one ended finds the most recently written lines still in the
     // last-level cache.  (About 5% per launch at 256^3; the result does not depend on order.)
-    const unsigned int tile0 = kf_xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned int tile0 = kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x);
     const unsigned int tile = parity ? gridDim.x - 1u - tile0 : tile0;
     // group q of thread t: voxels tile FLAT + q FB VEC + t VEC .. + 3 (n is a multiple of 4, so
     // a group is either whole or outside)

@@ -219,7 +219,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_step(PhiView<S> L, PhiView<S> Cn,
     if (MODE == KF_MODE_STEP && st->done)
         return;
     const GridDims g = L.g;
-    const int64_t i = (int64_t)kf_xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const int64_t i = (int64_t)kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     double mx = 0.0;
     unsigned long long act = 0;
     if (i < g.n) {

@@ -300,6 +300,23 @@ __device__ __host__ inline unsigned int kf_xcd_tile(unsigned int b, unsigned int
     return xcd * per + (xcd < rem ? xcd : rem) + local;
 }
 
+// Chunks of C consecutive tiles dealt round-robin over the XCDs (XCD x sweeps chunks x, x + 8,
+// ... in order), the tiles left over by whole rounds of 8 C as kf_xcd_tile's contiguous
+// ranges.  Within a chunk neighbouring tiles share an L2; across the grid every XCD gets an
+// equal share of any spatially concentrated work, such as the stencils of a surface band that
+// a contiguous split would hand to two or three XCDs.
+template <unsigned int C>
+__device__ __host__ inline unsigned int kf_xcd_chunk_tile(unsigned int b, unsigned int nb)
+{
+    const unsigned int xcd = b % KF_NUM_XCD, local = b / KF_NUM_XCD;
+    const unsigned int K = nb / (KF_NUM_XCD * C);
+    if (local < K * C)
+        return (local / C) * (KF_NUM_XCD * C) + xcd * C + local % C;
+    const unsigned int rl = local - K * C, r = nb - K * KF_NUM_XCD * C;
+    const unsigned int per = r / KF_NUM_XCD, rem = r % KF_NUM_XCD;
+    return K * KF_NUM_XCD * C + xcd * per + (xcd < rem ? xcd : rem) + rl;
+}
+
 // Base index and weight of the sample at i + r (i an integer grid index, r a small offset),
 // with the reference's truncating cast: base = (int)(i + r), frac = (i + r) - base
 // (SDF.cpp:509, DisplacementField.cpp:43).  i + r is never formed in FP32: at i ~ 200 that
