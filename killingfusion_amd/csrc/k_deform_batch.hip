@@ -190,8 +190,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_batch(const BatchVol<S> *__restri
     // Odd launches visit the tiles backwards, so they start on the lines the previous launch
     // wrote last (still in the last-level cache).
     for (unsigned int tt = blockIdx.x; tt < total; tt += gridDim.x) {
-        // each XCD (dispatch index mod 8) takes one contiguous range of tiles
-        const unsigned int tx = kf_xcd_tile(tt, total);
+        // tiles go to the XCDs (dispatch index mod 8) in chunks of 64, dealt round-robin
+        const unsigned int tx = kf_xcd_chunk_tile<64>(tt, total);
         const unsigned int t = parity ? total - 1u - tx : tx;
         int lo = 0, hi = count - 1;  // volume of tile t (tile0 ascending)
         while (lo < hi) {
