@@ -283,12 +283,11 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
     const unsigned int n32 = nvox32(g);
     // Tiles go to the XCDs in chunks of 64 (a z-slice at 256^3) dealt round-robin, so the
     // stencils of the surface band spread over every XCD instead of keeping the two or three
-    // that hold the band busy after the others finish (256^3: 70 -> 57 us per launch).
-    // Odd launches sweep the grid backwards: a launch reads the buffer the previous one wrote,
-    // and starting where that one ended finds the most recently written lines still in the
-    // last-level cache.  (About 5% per launch at 256^3; the result does not depend on order.)
-    const unsigned int tile0 = kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x);
-    const unsigned int tile = parity ? gridDim.x - 1u - tile0 : tile0;
+    // that hold the band busy after the others finish (256^3: 70 -> 58 us per launch).  With
+    // this dealing every launch walks the whole grid at once, so sweeping odd launches
+    // backwards (to meet the previous launch's last writes in the last-level cache) no longer
+    // pays: it measured 58.9 against 58.1 us at 256^3.
+    const unsigned int tile = kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x);
     // group q of thread t: voxels tile FLAT + q FB VEC + t VEC .. + 3 (n is a multiple of 4, so
     // a group is either whole or outside)
     unsigned int i0[FPT];

@@ -463,13 +463,17 @@ kf_status kf_field_create(kf_context *ctx, const int32_t dims[3], kf_field **out
     for (int a = 0; a < 3; a++)
         f->dims[a] = dims[a];
     f->n = (int64_t)dims[0] * dims[1] * dims[2];
-    for (int b = 0; b < 2; b++) {
-        if (hipMalloc((void **)&f->buf[b], sizeof(float) * 3 * (size_t)f->n) != hipSuccess) {
-            kf_set_error("field allocation failed");
-            kf_field_destroy(f);
-            return KF_E_OOM;
-        }
+    // Both buffers of the Jacobi pair come from one allocation.  Allocated apart, the second
+    // buffer's physical placement decided the dense step's speed: 6 of 32 placements ran the
+    // 256^3 step 25% slower (58 -> 72-74 us per launch); in one allocation 80 of 80 ran at
+    // 58-59 us.
+    if (hipMalloc((void **)&f->buf[0], sizeof(float) * 6 * (size_t)f->n) != hipSuccess) {
+        f->buf[0] = nullptr;
+        kf_set_error("field allocation failed");
+        kf_field_destroy(f);
+        return KF_E_OOM;
     }
+    f->buf[1] = f->buf[0] + 3 * (size_t)f->n;
     const double zero[3] = {0.0, 0.0, 0.0};
     kf_status s = kf_launch_field_fill(f, zero);
     if (s != KF_OK) {
@@ -486,9 +490,9 @@ kf_status kf_field_destroy(kf_field *f)
         return KF_OK;
     DeviceGuard dg(f->ctx->device);
     (void)hipStreamSynchronize(f->ctx->stream);
+    if (f->buf[0])
+        (void)hipFree(f->buf[0]);  // buf[1] lies in the same allocation
     for (int b = 0; b < 2; b++) {
-        if (f->buf[b])
-            (void)hipFree(f->buf[b]);
         if (f->masks[b])
             (void)hipFree(f->masks[b]);
     }

@@ -140,7 +140,7 @@ struct kf_field {
     kf_context *ctx = nullptr;
     int32_t dims[3] = {0, 0, 0};
     int64_t n = 0;
-    float *buf[2] = {nullptr, nullptr};  // SoA [3][n] each; buf[cur] is the field's value
+    float *buf[2] = {nullptr, nullptr};  // SoA [3][n] each, one allocation (buf[1] = buf[0] + 3n); buf[cur] is the value
     int cur = 0;
     unsigned int *masks[2] = {nullptr, nullptr};  // nonzero masks of buf[0] / buf[1] (adjoint kernels)
     float *u() const { return buf[cur]; }
