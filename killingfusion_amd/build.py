@@ -5,10 +5,10 @@
 Each translation unit is compiled with hipcc in parallel into killingfusion_amd/_build/,
 then linked into one shared library in-tree.  Kernels that reproduce the reference's FP64
 decisions (integration, warp, fuse, marching cubes) are built with -ffp-contract=off.  The
-adjoint deform kernels are built with -ffp-contract=on: a multiply-add is fused exactly where
-one source expression has it, never across statements, so the same per-voxel arithmetic
-inlined into different kernels (dense step, active-set step, batched step, energy passes)
-rounds the same way.
+deform kernels are built with -ffp-contract=on: a multiply-add is fused exactly where one
+source expression has it, never across statements, so the same per-voxel arithmetic inlined
+into different kernels (dense step, active-set step, batched step, energy passes) or spread
+over threads differently rounds the same way.
 """
 from __future__ import annotations
 
@@ -31,7 +31,7 @@ SOURCES = {
     "kf_pipeline.hip": [],
     "k_volume.hip": ["-ffp-contract=off"],
     "k_mc.hip": ["-ffp-contract=off"],
-    "k_deform_ref.hip": [],
+    "k_deform_ref.hip": ["-ffp-contract=on"],
     "k_deform_adj.hip": ["-ffp-contract=on"],
     "k_deform_active.hip": ["-ffp-contract=on"],
     "k_deform_batch.hip": ["-ffp-contract=on"],

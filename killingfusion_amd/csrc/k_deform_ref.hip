@@ -7,7 +7,8 @@
 // warped live SDF, each preceded by a trilinear displacement read, plus 19 displacement
 // samples for the Killing term -- with every quirk of the reference reads (truncating
 // casts, cell-centre offsets, linear-index-only displacement bounds test, clamp before
-// the 1/(4h^2) division).
+// the 1/(4h^2) division).  A block lists its voxels in the band and spreads their samples
+// over its threads, one (voxel, sample) pair each.
 #include "kf_reduce.hpp"
 
 namespace {
@@ -76,53 +77,93 @@ __device__ inline void disp_at_f(const float *__restrict__ u, const GridDims &g,
         o[k] = kf_interp3(c[0][k], c[1][k], c[2][k], c[3][k], c[4][k], c[5][k], c[6][k], c[7][k], fx, fy, fz);
 }
 
-// SDF::getDistancef, SDF.cpp:577-582, at loc = i + r: phi at loc + u_f(loc) + 0.5, whose
-// shifted coordinate is i + (r + u_f(loc)).
-template <class S>
-__device__ inline float distance_f(const PhiView<S> &P, const float *__restrict__ u, int ix, int iy, int iz, float rx,
-                                   float ry, float rz)
-{
-    float d[3];
-    disp_at_f(u, P.g, ix, iy, iz, rx, ry, rz, d);
-    return P.trilinear(ix, iy, iz, rx + d[0], ry + d[1], rz + d[2]);
-}
-
-// Offsets (units of h) of the 19-sample stencils, SDF.cpp:847-865 / DisplacementField.cpp:305-323.
+// Offsets (units of h) of the 19-sample Hessian stencil, SDF.cpp:847-865.
 __constant__ signed char c_off[19][3] = {
     {0, 0, 0}, {2, 0, 0}, {-2, 0, 0}, {0, 2, 0}, {0, -2, 0}, {0, 0, 2}, {0, 0, -2},
     {1, 0, 1}, {-1, 0, 1}, {0, 1, 1}, {0, -1, 1}, {1, 1, 0}, {-1, 1, 0},
     {-1, 0, -1}, {1, 0, -1}, {0, -1, -1}, {0, 1, -1}, {-1, -1, 0}, {1, -1, 0}};
 
+// Offsets (units of h) of the 19 displacement samples of the Killing gradient, in the order
+// DisplacementField.cpp:305-323 combines them: the centre, the pure second differences
+// (+2h, -2h per axis), then (+,+), (-,-), (+,-), (-,+) of the xz, yz and xy planes.
+__constant__ signed char c_koff[19][3] = {
+    {0, 0, 0}, {2, 0, 0}, {-2, 0, 0}, {0, 2, 0}, {0, -2, 0}, {0, 0, 2}, {0, 0, -2},
+    {1, 0, 1}, {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1},
+    {0, 1, 1}, {0, -1, -1}, {0, 1, -1}, {0, -1, 1},
+    {1, 1, 0}, {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}};
+
+// Samples an active voxel takes.  Samples 0-5 are the warped live SDF at the centre +h / -h
+// along x, y, z (SDF::computeDistanceGradient, SDF.cpp:726-744), 6-24 the Hessian stencil
+// (SDF.cpp:842-886) and 25-43 the displacement at the Killing stencil points.
+constexpr int NPHI = 25;
+constexpr int NKIL = 19;
+// Active voxels whose samples one round of a block takes.
+constexpr int RK = 32;
+
 struct RefConsts {
-    float inv_voxel_size_den;  // unused placeholder for alignment
     float voxel_size, hi, lo, eps, alpha, omega_k, gamma, omega_s, h, clamp;
     int use_data, use_ls, use_k;
+    int nphi;   // SDF samples in use: 0, 6 (gradient) or 25 (gradient and Hessian)
+    int nsamp;  // nphi + 19 when the Killing term is in use
 };
 
+// Sample s of the active voxel (x, y, z) at the point i + r.  Every sample reads u
+// trilinearly there (DisplacementField::getDisplacementAtf); an SDF sample then reads the live
+// SDF at the point moved by it (SDF::getDistancef, SDF.cpp:577-582: phi at loc + u_f(loc) +
+// 0.5, whose shifted coordinate is i + (r + u_f)).
 template <class S>
-__device__ inline void ref_gradient(const PhiView<S> &L, const PhiView<S> &Cn, const float *__restrict__ u,
-                                    const RefConsts &c, int x, int y, int z, float psi, float out[3])
+__device__ inline void ref_sample(const PhiView<S> &L, const float *__restrict__ u, float h, int s, int x, int y,
+                                  int z, float *phi_out, float *u_out)
+{
+    float r0, r1, r2;
+    if (s < 6) {
+        const int a = s >> 1;
+        const float d = (s & 1) ? -h : h;
+        r0 = a == 0 ? 0.5f + d : 0.5f;
+        r1 = a == 1 ? 0.5f + d : 0.5f;
+        r2 = a == 2 ? 0.5f + d : 0.5f;
+    } else if (s < NPHI) {
+        const int k = s - 6;
+        r0 = 0.5f + h * (float)c_off[k][0];
+        r1 = 0.5f + h * (float)c_off[k][1];
+        r2 = 0.5f + h * (float)c_off[k][2];
+    } else {
+        const int k = s - NPHI;
+        r0 = h * (float)c_koff[k][0];
+        r1 = h * (float)c_koff[k][1];
+        r2 = h * (float)c_koff[k][2];
+    }
+    float d[3];
+    disp_at_f(u, L.g, x, y, z, r0, r1, r2, d);
+    if (s < NPHI) {
+        phi_out[s] = L.trilinear(x, y, z, r0 + d[0], r1 + d[1], r2 + d[2]);
+    } else {
+        float *o = u_out + 3 * (s - NPHI);
+        o[0] = d[0];
+        o[1] = d[1];
+        o[2] = d[2];
+    }
+}
+
+// KillingFusion::computeEnergyGradient (KillingFusion.cpp:399-442) from the samples of one
+// voxel: f[0..24] the SDF samples, k[0..18] the Killing displacement samples.
+__device__ inline void ref_combine(const RefConsts &c, float psi, float dst, const float *f, const float (*k)[3],
+                                   float out[3])
 {
     const float h = c.h;
-    // Samples sit at gridLocation + offset, gridLocation = idx + 0.5 (SDF.cpp:729,845).
     float dg[3] = {0.f, 0.f, 0.f}, lg[3] = {0.f, 0.f, 0.f}, kg[3] = {0.f, 0.f, 0.f};
     float gr[3] = {0.f, 0.f, 0.f};
     if (c.use_data || c.use_ls) {
         // SDF::computeDistanceGradient(idx, disp), SDF.cpp:726-744.
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-            float lp[3] = {0.5f, 0.5f, 0.5f}, lm[3] = {0.5f, 0.5f, 0.5f};
-            lp[a] = 0.5f + h;
-            lm[a] = 0.5f - h;
-            gr[a] = distance_f(L, u, x, y, z, lp[0], lp[1], lp[2]) - distance_f(L, u, x, y, z, lm[0], lm[1], lm[2]);
-        }
+        for (int a = 0; a < 3; a++)
+            gr[a] = f[2 * a] - f[2 * a + 1];
 #pragma unroll
         for (int a = 0; a < 3; a++)
             gr[a] = gr[a] / (2.0f * h);
     }
     if (c.use_data) {
         // computeDataEnergyGradient, KillingFusion.cpp:444-455.
-        float dst = Cn.at(x, y, z);
         float s = (psi - dst) / c.voxel_size;
 #pragma unroll
         for (int a = 0; a < 3; a++)
@@ -130,65 +171,42 @@ __device__ inline void ref_gradient(const PhiView<S> &L, const PhiView<S> &Cn, c
     }
     if (c.use_ls) {
         // SDF::computeDistanceHessian(idx, disp), SDF.cpp:842-886.
-        float f[19];
-        for (int k = 0; k < 19; k++)
-            f[k] = distance_f(L, u, x, y, z, 0.5f + h * (float)c_off[k][0], 0.5f + h * (float)c_off[k][1],
-                              0.5f + h * (float)c_off[k][2]);
+        const float *q = f + 6;
         float den = (4.0f * h * h);
-        float fxx = (f[1] - 2.0f * f[0] + f[2]) / den;
-        float fyy = (f[3] - 2.0f * f[0] + f[4]) / den;
-        float fzz = (f[5] - 2.0f * f[0] + f[6]) / den;
-        float fxz = (f[7] + f[13] - f[14] - f[8]) / den;
-        float fyz = (f[9] + f[15] - f[16] - f[10]) / den;
-        float fxy = (f[11] + f[17] - f[18] - f[12]) / den;
+        float fxx = (q[1] - 2.0f * q[0] + q[2]) / den;
+        float fyy = (q[3] - 2.0f * q[0] + q[4]) / den;
+        float fzz = (q[5] - 2.0f * q[0] + q[6]) / den;
+        float fxz = (q[7] + q[13] - q[14] - q[8]) / den;
+        float fyz = (q[9] + q[15] - q[16] - q[10]) / den;
+        float fxy = (q[11] + q[17] - q[18] - q[12]) / den;
         // computeLevelSetEnergyGradient, KillingFusion.cpp:464-476.
         float n = sqrtf(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
         float hg0 = fxx * gr[0] + fxy * gr[1] + fxz * gr[2];
         float hg1 = fxy * gr[0] + fyy * gr[1] + fyz * gr[2];
         float hg2 = fxz * gr[0] + fyz * gr[1] + fzz * gr[2];
-        float q = (n - 1.0f);
+        float qn = (n - 1.0f);
         float r = (n + c.eps);
-        lg[0] = hg0 * q / r * c.omega_s;
-        lg[1] = hg1 * q / r * c.omega_s;
-        lg[2] = hg2 * q / r * c.omega_s;
+        lg[0] = hg0 * qn / r * c.omega_s;
+        lg[1] = hg1 * qn / r * c.omega_s;
+        lg[2] = hg2 * qn / r * c.omega_s;
     }
     if (c.use_k) {
         // DisplacementField::computeKillingEnergyGradient2, DisplacementField.cpp:301-347.
-        float xx[3] = {0, 0, 0}, yy[3] = {0, 0, 0}, zz[3] = {0, 0, 0};
-        float xy[3] = {0, 0, 0}, xz[3] = {0, 0, 0}, yz[3] = {0, 0, 0};
-        float d0[3];
-        disp_at_f(u, L.g, x, y, z, 0.0f, 0.0f, 0.0f, d0);
-        float d[3];
-        // pure second differences
-        disp_at_f(u, L.g, x, y, z, 2.0f * h, 0.0f, 0.0f, d);
-        for (int k = 0; k < 3; k++) xx[k] = d[k] - 2.0f * d0[k];
-        disp_at_f(u, L.g, x, y, z, -2.0f * h, 0.0f, 0.0f, d);
-        for (int k = 0; k < 3; k++) xx[k] = xx[k] + d[k];
-        disp_at_f(u, L.g, x, y, z, 0.0f, 2.0f * h, 0.0f, d);
-        for (int k = 0; k < 3; k++) yy[k] = d[k] - 2.0f * d0[k];
-        disp_at_f(u, L.g, x, y, z, 0.0f, -2.0f * h, 0.0f, d);
-        for (int k = 0; k < 3; k++) yy[k] = yy[k] + d[k];
-        disp_at_f(u, L.g, x, y, z, 0.0f, 0.0f, 2.0f * h, d);
-        for (int k = 0; k < 3; k++) zz[k] = d[k] - 2.0f * d0[k];
-        disp_at_f(u, L.g, x, y, z, 0.0f, 0.0f, -2.0f * h, d);
-        for (int k = 0; k < 3; k++) zz[k] = zz[k] + d[k];
-        // mixed: (+,+) + (-,-) - (+,-) - (-,+)
-        float pp[3], mm[3], pm[3], mp[3];
-        disp_at_f(u, L.g, x, y, z, h, 0.0f, h, pp);
-        disp_at_f(u, L.g, x, y, z, -h, 0.0f, -h, mm);
-        disp_at_f(u, L.g, x, y, z, h, 0.0f, -h, pm);
-        disp_at_f(u, L.g, x, y, z, -h, 0.0f, h, mp);
-        for (int k = 0; k < 3; k++) xz[k] = pp[k] + mm[k] - pm[k] - mp[k];
-        disp_at_f(u, L.g, x, y, z, 0.0f, h, h, pp);
-        disp_at_f(u, L.g, x, y, z, 0.0f, -h, -h, mm);
-        disp_at_f(u, L.g, x, y, z, 0.0f, h, -h, pm);
-        disp_at_f(u, L.g, x, y, z, 0.0f, -h, h, mp);
-        for (int k = 0; k < 3; k++) yz[k] = pp[k] + mm[k] - pm[k] - mp[k];
-        disp_at_f(u, L.g, x, y, z, h, h, 0.0f, pp);
-        disp_at_f(u, L.g, x, y, z, -h, -h, 0.0f, mm);
-        disp_at_f(u, L.g, x, y, z, h, -h, 0.0f, pm);
-        disp_at_f(u, L.g, x, y, z, -h, h, 0.0f, mp);
-        for (int k = 0; k < 3; k++) xy[k] = pp[k] + mm[k] - pm[k] - mp[k];
+        float xx[3], yy[3], zz[3], xy[3], xz[3], yz[3];
+        const float *d0 = k[0];
+        for (int j = 0; j < 3; j++) {
+            // pure second differences
+            xx[j] = k[1][j] - 2.0f * d0[j];
+            xx[j] = xx[j] + k[2][j];
+            yy[j] = k[3][j] - 2.0f * d0[j];
+            yy[j] = yy[j] + k[4][j];
+            zz[j] = k[5][j] - 2.0f * d0[j];
+            zz[j] = zz[j] + k[6][j];
+            // mixed: (+,+) + (-,-) - (+,-) - (-,+)
+            xz[j] = k[7][j] + k[8][j] - k[9][j] - k[10][j];
+            yz[j] = k[11][j] + k[12][j] - k[13][j] - k[14][j];
+            xy[j] = k[15][j] + k[16][j] - k[17][j] - k[18][j];
+        }
         float div0 = xx[0] + xy[1] + xz[2];
         float div1 = xy[0] + yy[1] + yz[2];
         float div2 = xz[0] + yz[1] + zz[2];
@@ -198,12 +216,12 @@ __device__ inline void ref_gradient(const PhiView<S> &L, const PhiView<S> &Cn, c
         kg[2] = -2.0f * (xx[2] + yy[2] + zz[2]) + g2 * div2;
         float nrm = sqrtf(kg[0] * kg[0] + kg[1] * kg[1] + kg[2] * kg[2]);
         if (nrm > c.clamp) {
-            for (int k = 0; k < 3; k++)
-                kg[k] = kg[k] / nrm * c.clamp;
+            for (int j = 0; j < 3; j++)
+                kg[j] = kg[j] / nrm * c.clamp;
         }
         float den = (4.0f * h * h);
-        for (int k = 0; k < 3; k++)
-            kg[k] = kg[k] / den * c.omega_k;
+        for (int j = 0; j < 3; j++)
+            kg[j] = kg[j] / den * c.omega_k;
     }
     // computeEnergyGradient, KillingFusion.cpp:441: data + killing + levelset.
 #pragma unroll
@@ -211,6 +229,14 @@ __device__ inline void ref_gradient(const PhiView<S> &L, const PhiView<S> &Cn, c
         out[a] = (dg[a] + kg[a]) + lg[a];
 }
 
+// One Jacobi iteration (MODE STEP) or the gradient alone (MODE GRAD) over a block of
+// BLOCK voxels.  First each thread tests its own voxel (SDF::getDistance(idx, disp),
+// SDF.cpp:570-575; skip test KillingFusion.cpp:272-274) and lists it in LDS when it is in
+// the band; a voxel outside the band keeps u.  Then the block takes the samples of the listed
+// voxels RK at a time, one (voxel, sample) pair per thread, so a voxel's 44 dependent reads
+// run side by side instead of one after another in one lane, and a thread per voxel combines
+// them.  Each sample and the combination are the per-voxel arithmetic of the reference's
+// stencils, so the result does not depend on how the pairs are spread.
 template <class S, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_ref_step(PhiView<S> L, PhiView<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, RefConsts c, DeformState *st,
@@ -218,40 +244,85 @@ __global__ __launch_bounds__(BLOCK) void k_ref_step(PhiView<S> L, PhiView<S> Cn,
 {
     if (MODE == KF_MODE_STEP && st->done)
         return;
+    __shared__ int s_count;
+    __shared__ long long s_i[BLOCK];
+    __shared__ int s_xyz[BLOCK][3];
+    __shared__ float s_u[BLOCK][3];
+    __shared__ float s_psi[BLOCK], s_dst[BLOCK];
+    __shared__ float s_f[RK][NPHI];
+    __shared__ float s_k[RK][NKIL][3];
     const GridDims g = L.g;
     const int64_t i = (int64_t)kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (threadIdx.x == 0)
+        s_count = 0;
+    __syncthreads();
     double mx = 0.0;
     unsigned long long act = 0;
     if (i < g.n) {
         int x, y, z;
         kf_xyz(g, i, x, y, z);
         float u0 = u_in[i], u1 = u_in[g.n + i], u2 = u_in[2 * g.n + i];
-        // SDF::getDistance(idx, disp), SDF.cpp:570-575; skip test KillingFusion.cpp:272-274.
         float psi = L.trilinear(x, y, z, u0, u1, u2);
-        bool active = !(psi > c.hi || psi < c.lo);
-        float gr[3] = {0.f, 0.f, 0.f};
-        if (active) {
+        if (!(psi > c.hi || psi < c.lo)) {
             act = 1;
-            ref_gradient<S>(L, Cn, u_in, c, x, y, z, psi, gr);
-        }
-        if (MODE == KF_MODE_GRAD) {
-            u_out[i] = gr[0];
-            u_out[g.n + i] = gr[1];
-            u_out[2 * g.n + i] = gr[2];
+            const int slot = atomicAdd(&s_count, 1);
+            s_i[slot] = i;
+            s_xyz[slot][0] = x;
+            s_xyz[slot][1] = y;
+            s_xyz[slot][2] = z;
+            s_u[slot][0] = u0;
+            s_u[slot][1] = u1;
+            s_u[slot][2] = u2;
+            s_psi[slot] = psi;
+            s_dst[slot] = c.use_data ? Cn.at(x, y, z) : 0.0f;
+        } else if (MODE == KF_MODE_GRAD) {
+            u_out[i] = 0.0f;
+            u_out[g.n + i] = 0.0f;
+            u_out[2 * g.n + i] = 0.0f;
         } else {
-            float d0 = -c.alpha * gr[0], d1 = -c.alpha * gr[1], d2 = -c.alpha * gr[2];
-            float n0 = u0 + d0, n1 = u1 + d1, n2 = u2 + d2;
-            u_out[i] = n0;
-            u_out[g.n + i] = n1;
-            u_out[2 * g.n + i] = n2;
-            if (active) {
+            // zero gradient: u + (-alpha * 0)
+            const float d = -c.alpha * 0.0f;
+            u_out[i] = u0 + d;
+            u_out[g.n + i] = u1 + d;
+            u_out[2 * g.n + i] = u2 + d;
+        }
+    }
+    __syncthreads();
+    const int count = s_count;
+    for (int c0 = 0; c0 < count; c0 += RK) {
+        const int nv = min(RK, count - c0);
+        const int total = nv * c.nsamp;
+        for (int t = threadIdx.x; t < total; t += BLOCK) {
+            const int v = t / c.nsamp;
+            const int si = t - v * c.nsamp;
+            const int smp = si < c.nphi ? si : NPHI + (si - c.nphi);
+            const int slot = c0 + v;
+            ref_sample<S>(L, u_in, c.h, smp, s_xyz[slot][0], s_xyz[slot][1], s_xyz[slot][2], s_f[v], &s_k[v][0][0]);
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < nv) {
+            const int slot = c0 + (int)threadIdx.x;
+            float gr[3];
+            ref_combine(c, s_psi[slot], s_dst[slot], s_f[threadIdx.x], s_k[threadIdx.x], gr);
+            const int64_t j = s_i[slot];
+            if (MODE == KF_MODE_GRAD) {
+                u_out[j] = gr[0];
+                u_out[g.n + j] = gr[1];
+                u_out[2 * g.n + j] = gr[2];
+            } else {
+                float d0 = -c.alpha * gr[0], d1 = -c.alpha * gr[1], d2 = -c.alpha * gr[2];
+                float n0 = s_u[slot][0] + d0, n1 = s_u[slot][1] + d1, n2 = s_u[slot][2] + d2;
+                u_out[j] = n0;
+                u_out[g.n + j] = n1;
+                u_out[2 * g.n + j] = n2;
                 double dn = sqrt((double)d0 * d0 + (double)d1 * d1 + (double)d2 * d2);
                 if (!(isfinite(n0) && isfinite(n1) && isfinite(n2)))
-                    atomicMin((long long *)&st->div_slot[parity], (long long)i);
-                else
+                    atomicMin((long long *)&st->div_slot[parity], (long long)j);
+                else if (dn > mx)
                     mx = dn;
             }
         }
+        __syncthreads();
     }
     if (MODE == KF_MODE_STEP) {
         double sum = 0.0;
@@ -291,6 +362,8 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
     c.use_data = p->use_data;
     c.use_ls = p->use_levelset;
     c.use_k = p->use_killing;
+    c.nphi = c.use_ls ? NPHI : (c.use_data ? 6 : 0);
+    c.nsamp = c.nphi + (c.use_k ? NKIL : 0);
     const unsigned int nb = (unsigned int)((live->n + BLOCK - 1) / BLOCK);
     kf_status s = kf_ensure_partials(ctx, nb);
     if (s != KF_OK)
