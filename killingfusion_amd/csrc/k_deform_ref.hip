@@ -229,89 +229,206 @@ __device__ inline void ref_combine(const RefConsts &c, float psi, float dst, con
         out[a] = (dg[a] + kg[a]) + lg[a];
 }
 
-// One Jacobi iteration (MODE STEP) or the gradient alone (MODE GRAD) over a block of
-// BLOCK voxels.  First each thread tests its own voxel (SDF::getDistance(idx, disp),
-// SDF.cpp:570-575; skip test KillingFusion.cpp:272-274) and lists it in LDS when it is in
-// the band; a voxel outside the band keeps u.  Then the block takes the samples of the listed
-// voxels RK at a time, one (voxel, sample) pair per thread, so a voxel's 44 dependent reads
-// run side by side instead of one after another in one lane, and a thread per voxel combines
-// them.  Each sample and the combination are the per-voxel arithmetic of the reference's
-// stencils, so the result does not depend on how the pairs are spread.
-template <class S, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_ref_step(PhiView<S> L, PhiView<S> Cn, const float *__restrict__ u_in,
-                                                    float *__restrict__ u_out, RefConsts c, DeformState *st,
-                                                    BlockPartial *partials, int parity)
+// Blocks of the list kernel (at most; fewer for small grids).
+constexpr int LIST_BLOCKS = 2048;
+
+__device__ inline float4 ref_load_stream4(const float *p)
+{
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// Clears the two list lengths and the non-finite flag of the workspace.
+__global__ void k_ref_clear(unsigned long long *aux)
+{
+    if (threadIdx.x < 3)
+        aux[threadIdx.x] = 0ull;
+}
+
+// Sets *flag when some live SDF value, decoded, is not finite.
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_ref_finite(PhiView<S> L, unsigned long long *flag)
+{
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < L.g.n; i += (int64_t)gridDim.x * BLOCK)
+        bad |= !isfinite(S::load(L.p, i, L.scale));
+    if (__ballot(bad) != 0ull && __lane_id() == 0)
+        atomicOr(flag, 1ull);
+}
+
+// Pass 1 of a reference-formula step: every voxel's warped live SDF value
+// psi = SDF::getDistance(idx, disp) (SDF.cpp:570-575) and the band test
+// (KillingFusion.cpp:272-274), V voxels per thread (4: one float4 of each u plane).  A voxel
+// outside the band keeps u: its update is u + (-alpha * 0).  A voxel in the band is appended
+// to the list for pass 2 (one atomic per wavefront), which overwrites its entry of u_out.
+// Where u_j is +0 or -0 the interpolation weights are +0 and the interpolant is
+// phi_j + (finite) * 0 + ...; when every live value is finite (aux[2] == 0) that is phi_j
+// itself unless phi_j is -0, so those voxels read phi_j alone.
+template <class S, int MODE, int V>
+__global__ __launch_bounds__(BLOCK) void k_ref_scan(PhiView<S> L, const float *__restrict__ u_in,
+                                                    float *__restrict__ u_out, float hi, float lo, float alpha,
+                                                    DeformState *st, BlockPartial *partials,
+                                                    unsigned long long *aux, int parity,
+                                                    unsigned long long *__restrict__ list_idx,
+                                                    float *__restrict__ list_psi)
 {
     if (MODE == KF_MODE_STEP && st->done)
         return;
-    __shared__ int s_count;
-    __shared__ long long s_i[BLOCK];
-    __shared__ int s_xyz[BLOCK][3];
-    __shared__ float s_u[BLOCK][3];
-    __shared__ float s_psi[BLOCK], s_dst[BLOCK];
+    const GridDims g = L.g;
+    const int64_t i0 = ((int64_t)kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x) * V;
+    const bool phi_finite = aux[2] == 0ull;
+    bool act[V];
+    float psi[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        act[v] = false;
+        psi[v] = 0.0f;
+    }
+    if (i0 < g.n) {
+        float u[3][V], ph[V];
+        if constexpr (V == 4) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const float4 q = ref_load_stream4(u_in + a * g.n + i0);
+                u[a][0] = q.x;
+                u[a][1] = q.y;
+                u[a][2] = q.z;
+                u[a][3] = q.w;
+            }
+            S::load4(L.p, i0, L.scale, ph);
+        } else {
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                u[a][0] = u_in[a * g.n + i0];
+            ph[0] = S::load(L.p, i0, L.scale);
+        }
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const unsigned int zb =
+                (__float_as_uint(u[0][v]) | __float_as_uint(u[1][v]) | __float_as_uint(u[2][v])) & 0x7fffffffu;
+            if (phi_finite && zb == 0u && __float_as_uint(ph[v]) != 0x80000000u) {
+                psi[v] = ph[v];
+            } else {
+                int x, y, z;
+                kf_xyz(g, i0 + v, x, y, z);
+                psi[v] = L.trilinear(x, y, z, u[0][v], u[1][v], u[2][v]);
+            }
+            act[v] = !(psi[v] > hi || psi[v] < lo);
+        }
+        // Voxels in the band get their value in pass 2.
+        const float d = -alpha * 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float o[V];
+#pragma unroll
+            for (int v = 0; v < V; v++)
+                o[v] = MODE == KF_MODE_GRAD ? 0.0f : u[a][v] + d;
+            if constexpr (V == 4)
+                *reinterpret_cast<float4 *>(u_out + a * g.n + i0) = make_float4(o[0], o[1], o[2], o[3]);
+            else
+                u_out[a * g.n + i0] = o[0];
+        }
+    }
+    unsigned long long m[V];
+    unsigned int tot = 0;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        m[v] = __ballot(act[v]);
+        tot += (unsigned int)__popcll(m[v]);
+    }
+    if (tot != 0u) {
+        unsigned long long base = 0ull;
+        if (__lane_id() == 0)
+            base = atomicAdd(&aux[parity], (unsigned long long)tot);
+        const unsigned int blo = __shfl((unsigned int)base, 0, 64);
+        const unsigned int bhi = __shfl((unsigned int)(base >> 32), 0, 64);
+        unsigned long long pos = ((unsigned long long)bhi << 32) | blo;
+        const unsigned long long below = (1ull << __lane_id()) - 1ull;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            if (act[v]) {
+                const unsigned long long q = pos + (unsigned long long)__popcll(m[v] & below);
+                list_idx[q] = (unsigned long long)(i0 + v);
+                list_psi[q] = psi[v];
+            }
+            pos += (unsigned long long)__popcll(m[v]);
+        }
+    }
+    if (MODE == KF_MODE_STEP) {
+        unsigned long long cnt = 0;
+#pragma unroll
+        for (int v = 0; v < V; v++)
+            cnt += act[v] ? 1ull : 0ull;
+        kf_block_publish<BLOCK>(partials, blockIdx.x, 0.0, 0.0, cnt);
+    }
+}
+
+// Pass 2: the voxels of the list, RK at a time per block.  The block takes their samples, one
+// (voxel, sample) pair per thread, so a voxel's 44 dependent reads run side by side instead of
+// one after another in one lane, and a thread per voxel combines them.  Each sample and the
+// combination are the per-voxel arithmetic of the reference's stencils, so the result does not
+// depend on the list's order or on how the pairs are spread.  A step launch also clears the
+// other list length, which the next launch's pass 1 counts into.
+template <class S, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn, const float *__restrict__ u_in,
+                                                    float *__restrict__ u_out, RefConsts c, DeformState *st,
+                                                    BlockPartial *partials, unsigned long long *aux, int parity,
+                                                    const unsigned long long *__restrict__ list_idx,
+                                                    const float *__restrict__ list_psi)
+{
+    if (MODE == KF_MODE_STEP && st->done)
+        return;
+    __shared__ long long s_i[RK];
+    __shared__ int s_xyz[RK][3];
+    __shared__ float s_u[RK][3];
+    __shared__ float s_psi[RK], s_dst[RK];
     __shared__ float s_f[RK][NPHI];
     __shared__ float s_k[RK][NKIL][3];
     const GridDims g = L.g;
-    const int64_t i = (int64_t)kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    if (threadIdx.x == 0)
-        s_count = 0;
-    __syncthreads();
+    const unsigned long long count = aux[parity];
+    if (MODE == KF_MODE_STEP && blockIdx.x == 0 && threadIdx.x == 0)
+        aux[parity ^ 1] = 0ull;
     double mx = 0.0;
-    unsigned long long act = 0;
-    if (i < g.n) {
-        int x, y, z;
-        kf_xyz(g, i, x, y, z);
-        float u0 = u_in[i], u1 = u_in[g.n + i], u2 = u_in[2 * g.n + i];
-        float psi = L.trilinear(x, y, z, u0, u1, u2);
-        if (!(psi > c.hi || psi < c.lo)) {
-            act = 1;
-            const int slot = atomicAdd(&s_count, 1);
-            s_i[slot] = i;
-            s_xyz[slot][0] = x;
-            s_xyz[slot][1] = y;
-            s_xyz[slot][2] = z;
-            s_u[slot][0] = u0;
-            s_u[slot][1] = u1;
-            s_u[slot][2] = u2;
-            s_psi[slot] = psi;
-            s_dst[slot] = c.use_data ? Cn.at(x, y, z) : 0.0f;
-        } else if (MODE == KF_MODE_GRAD) {
-            u_out[i] = 0.0f;
-            u_out[g.n + i] = 0.0f;
-            u_out[2 * g.n + i] = 0.0f;
-        } else {
-            // zero gradient: u + (-alpha * 0)
-            const float d = -c.alpha * 0.0f;
-            u_out[i] = u0 + d;
-            u_out[g.n + i] = u1 + d;
-            u_out[2 * g.n + i] = u2 + d;
+    for (unsigned long long c0 = (unsigned long long)blockIdx.x * RK; c0 < count;
+         c0 += (unsigned long long)gridDim.x * RK) {
+        const int nv = (int)(count - c0 < (unsigned long long)RK ? count - c0 : (unsigned long long)RK);
+        if ((int)threadIdx.x < nv) {
+            const int t = threadIdx.x;
+            const long long j = (long long)list_idx[c0 + t];
+            int x, y, z;
+            kf_xyz(g, j, x, y, z);
+            s_i[t] = j;
+            s_xyz[t][0] = x;
+            s_xyz[t][1] = y;
+            s_xyz[t][2] = z;
+            s_u[t][0] = u_in[j];
+            s_u[t][1] = u_in[g.n + j];
+            s_u[t][2] = u_in[2 * g.n + j];
+            s_psi[t] = list_psi[c0 + t];
+            s_dst[t] = c.use_data ? Cn.at(x, y, z) : 0.0f;
         }
-    }
-    __syncthreads();
-    const int count = s_count;
-    for (int c0 = 0; c0 < count; c0 += RK) {
-        const int nv = min(RK, count - c0);
+        __syncthreads();
         const int total = nv * c.nsamp;
         for (int t = threadIdx.x; t < total; t += BLOCK) {
             const int v = t / c.nsamp;
             const int si = t - v * c.nsamp;
             const int smp = si < c.nphi ? si : NPHI + (si - c.nphi);
-            const int slot = c0 + v;
-            ref_sample<S>(L, u_in, c.h, smp, s_xyz[slot][0], s_xyz[slot][1], s_xyz[slot][2], s_f[v], &s_k[v][0][0]);
+            ref_sample<S>(L, u_in, c.h, smp, s_xyz[v][0], s_xyz[v][1], s_xyz[v][2], s_f[v], &s_k[v][0][0]);
         }
         __syncthreads();
         if ((int)threadIdx.x < nv) {
-            const int slot = c0 + (int)threadIdx.x;
+            const int t = threadIdx.x;
             float gr[3];
-            ref_combine(c, s_psi[slot], s_dst[slot], s_f[threadIdx.x], s_k[threadIdx.x], gr);
-            const int64_t j = s_i[slot];
+            ref_combine(c, s_psi[t], s_dst[t], s_f[t], s_k[t], gr);
+            const int64_t j = s_i[t];
             if (MODE == KF_MODE_GRAD) {
                 u_out[j] = gr[0];
                 u_out[g.n + j] = gr[1];
                 u_out[2 * g.n + j] = gr[2];
             } else {
                 float d0 = -c.alpha * gr[0], d1 = -c.alpha * gr[1], d2 = -c.alpha * gr[2];
-                float n0 = s_u[slot][0] + d0, n1 = s_u[slot][1] + d1, n2 = s_u[slot][2] + d2;
+                float n0 = s_u[t][0] + d0, n1 = s_u[t][1] + d1, n2 = s_u[t][2] + d2;
                 u_out[j] = n0;
                 u_out[g.n + j] = n1;
                 u_out[2 * g.n + j] = n2;
@@ -324,10 +441,8 @@ __global__ __launch_bounds__(BLOCK) void k_ref_step(PhiView<S> L, PhiView<S> Cn,
         }
         __syncthreads();
     }
-    if (MODE == KF_MODE_STEP) {
-        double sum = 0.0;
-        kf_block_publish<BLOCK>(partials, blockIdx.x, sum, mx, act);
-    }
+    if (MODE == KF_MODE_STEP)
+        kf_block_publish<BLOCK>(partials, blockIdx.x, 0.0, mx, 0ull);
 }
 
 template <class S>
@@ -343,11 +458,24 @@ PhiView<S> make_view(const kf_volume *v)
 
 }  // namespace
 
+void kf_ref_release(kf_context *ctx)
+{
+    kf_ref_ws &w = ctx->ref;
+    if (w.idx)
+        (void)hipFree(w.idx);
+    if (w.psi)
+        (void)hipFree(w.psi);
+    if (w.aux)
+        (void)hipFree(w.aux);
+    w = kf_ref_ws();
+}
+
 kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const float *u_in, float *u_out,
                              const kf_params *p, int mode, int iter)
 {
     kf_context *ctx = live->ctx;
-    const int parity = iter & 1;
+    KF_CHECK_ARG(live->dtype == can->dtype, "live and canonical volumes must share a dtype");
+    const int parity = mode == KF_MODE_GRAD ? 0 : (iter & 1);
     RefConsts c{};
     c.voxel_size = (float)p->voxel_size;
     c.hi = (float)(p->max_surface - p->epsilon);
@@ -364,20 +492,59 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
     c.use_k = p->use_killing;
     c.nphi = c.use_ls ? NPHI : (c.use_data ? 6 : 0);
     c.nsamp = c.nphi + (c.use_k ? NKIL : 0);
-    const unsigned int nb = (unsigned int)((live->n + BLOCK - 1) / BLOCK);
-    kf_status s = kf_ensure_partials(ctx, nb);
-    if (s != KF_OK)
+    const int64_t n = live->n;
+    // The list holds at most every voxel once.
+    kf_ref_ws &w = ctx->ref;
+    if (w.cap < (size_t)n) {
+        kf_ref_release(ctx);
+        if (hipMalloc((void **)&w.idx, sizeof(unsigned long long) * (size_t)n) != hipSuccess ||
+            hipMalloc((void **)&w.psi, sizeof(float) * (size_t)n) != hipSuccess ||
+            hipMalloc((void **)&w.aux, 3 * sizeof(unsigned long long)) != hipSuccess) {
+            (void)hipGetLastError();
+            kf_ref_release(ctx);
+            kf_set_error("reference-step list allocation failed");
+            return KF_E_OOM;
+        }
+        w.cap = (size_t)n;
+    }
+    kf_status s = KF_OK;
+    const bool vec = n % 4 == 0;
+    const unsigned int nb_a = (unsigned int)((n + (vec ? 4 : 1) * BLOCK - 1) / ((vec ? 4 : 1) * BLOCK));
+    const int64_t chunks = (n + RK - 1) / RK;
+    const unsigned int nb_b = (unsigned int)(chunks < LIST_BLOCKS ? chunks : LIST_BLOCKS);
+    if (mode != KF_MODE_GRAD && (s = kf_ensure_partials(ctx, (size_t)nb_a + nb_b)) != KF_OK)
         return s;
-    KF_CHECK_ARG(live->dtype == can->dtype, "live and canonical volumes must share a dtype");
+    BlockPartial *parts = kf_partials(ctx, parity);
     KF_DISPATCH_DTYPE(live->dtype, S, {
         PhiView<S> L = make_view<S>(live), Cn = make_view<S>(can);
-        if (mode == KF_MODE_GRAD)
-            hipLaunchKernelGGL((k_ref_step<S, KF_MODE_GRAD>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in,
-                               u_out, c, ctx->state, kf_partials(ctx, parity), parity);
-        else
-            hipLaunchKernelGGL((k_ref_step<S, KF_MODE_STEP>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in,
-                               u_out, c, ctx->state, kf_partials(ctx, parity), parity);
+        if (mode == KF_MODE_GRAD || iter == 0) {
+            // A deform call (or a gradient evaluation) starts with empty lists and checks the live
+            // SDF once: it does not change while u does.
+            hipLaunchKernelGGL(k_ref_clear, dim3(1), dim3(64), 0, ctx->stream, w.aux);
+            const int64_t fb = (n + BLOCK - 1) / BLOCK;
+            hipLaunchKernelGGL(k_ref_finite<S>, dim3((unsigned int)(fb < 4096 ? fb : 4096)), dim3(BLOCK), 0,
+                               ctx->stream, L, w.aux + 2);
+        }
+        if (mode == KF_MODE_GRAD) {
+            if (vec)
+                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, 4>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
+                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, parts, w.aux, parity, w.idx, w.psi);
+            else
+                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, 1>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
+                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, parts, w.aux, parity, w.idx, w.psi);
+            hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in,
+                               u_out, c, ctx->state, parts, w.aux, parity, w.idx, w.psi);
+        } else {
+            if (vec)
+                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, 4>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
+                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, parts, w.aux, parity, w.idx, w.psi);
+            else
+                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, 1>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
+                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, parts, w.aux, parity, w.idx, w.psi);
+            hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in,
+                               u_out, c, ctx->state, parts + nb_a, w.aux, parity, w.idx, w.psi);
+        }
     });
     KF_LAUNCH_CHECK();
-    return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb, parity);
+    return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb_a + nb_b, parity);
 }

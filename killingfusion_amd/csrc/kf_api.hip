@@ -279,6 +279,7 @@ kf_status kf_context_destroy(kf_context *ctx)
     if (ctx->partials)
         (void)hipFree(ctx->partials);
     kf_active_release(ctx);
+    kf_ref_release(ctx);
     kf_batch_release(ctx);
     if (ctx->own_stream && ctx->stream)
         (void)hipStreamDestroy(ctx->stream);

@@ -108,6 +108,15 @@ struct kf_active_ws {
     unsigned int warp_count = 0;
 };
 
+// Device workspace of the reference-formula step (k_deform_ref.hip), grown on demand and kept
+// by the context: the voxels in the band of a step launch, listed by its first pass.
+struct kf_ref_ws {
+    unsigned long long *idx = nullptr;  // voxel indices, in no particular order
+    float *psi = nullptr;               // their warped live SDF value
+    size_t cap = 0;                     // entries of idx and psi
+    unsigned long long *aux = nullptr;  // list lengths of launch parity 0 and 1; [2]: a live SDF value is not finite
+};
+
 struct kf_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -119,6 +128,7 @@ struct kf_context {
     int num_cus = 256;
     struct kf_batch_cache *batch = nullptr;  // batched-deform workspace, tuned shapes, captured graph
     kf_active_ws active;                     // active-set sweep workspace (k_deform_active.hip)
+    kf_ref_ws ref;                           // reference-formula step workspace (k_deform_ref.hip)
 };
 
 struct kf_volume {
@@ -397,6 +407,7 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
 // as kf_launch_adj_step; kf_active_scatter writes compact set iters & 1 back into u.
 bool kf_active_supported(const kf_volume *live);
 void kf_active_release(kf_context *ctx);
+void kf_ref_release(kf_context *ctx);
 kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params *p, bool u_zero);
 kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int count);
 kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const kf_params *p, int iter,
