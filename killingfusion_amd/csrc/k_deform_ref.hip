@@ -35,9 +35,29 @@ struct PhiView {
         kf_split(ix, rx, bx, fx);
         kf_split(iy, ry, by, fy);
         kf_split(iz, rz, bz, fz);
-        return kf_interp3(at(bx, by, bz), at(bx + 1, by, bz), at(bx, by + 1, bz), at(bx + 1, by + 1, bz),
-                          at(bx, by, bz + 1), at(bx + 1, by, bz + 1), at(bx, by + 1, bz + 1),
-                          at(bx + 1, by + 1, bz + 1), fx, fy, fz);
+        float v[8];
+        if (bx >= 0 && by >= 0 && bz >= 0 && bx + 1 < g.nx && by + 1 < g.ny && bz + 1 < g.nz) {
+            // the whole cell in the grid: one bounds test, taps at fixed strides
+            const int64_t i0 = g.idx(bx, by, bz), sy = g.nx, sz = (int64_t)g.nx * g.ny;
+            v[0] = S::load(p, i0, scale);
+            v[1] = S::load(p, i0 + 1, scale);
+            v[2] = S::load(p, i0 + sy, scale);
+            v[3] = S::load(p, i0 + sy + 1, scale);
+            v[4] = S::load(p, i0 + sz, scale);
+            v[5] = S::load(p, i0 + sz + 1, scale);
+            v[6] = S::load(p, i0 + sz + sy, scale);
+            v[7] = S::load(p, i0 + sz + sy + 1, scale);
+        } else {
+            v[0] = at(bx, by, bz);
+            v[1] = at(bx + 1, by, bz);
+            v[2] = at(bx, by + 1, bz);
+            v[3] = at(bx + 1, by + 1, bz);
+            v[4] = at(bx, by, bz + 1);
+            v[5] = at(bx + 1, by, bz + 1);
+            v[6] = at(bx, by + 1, bz + 1);
+            v[7] = at(bx + 1, by + 1, bz + 1);
+        }
+        return kf_interp3(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], fx, fy, fz);
     }
 };
 
@@ -55,8 +75,12 @@ __device__ inline void disp_at(const float *__restrict__ u, const GridDims &g, i
 }
 
 // DisplacementField::getDisplacementAtf, DisplacementField.cpp:40-58, at the point i + r.
+// NB: the taps come from nb, the 3x3x3 neighbourhood of (ix, iy, iz) read by disp_at (entry
+// ((dz + 1) * 3 + dy + 1) * 3 + dx + 1 holds u at (ix + dx, iy + dy, iz + dz)); the caller
+// guarantees |r| < 1 on each axis, so the cell lies in it.
+template <bool NB = false>
 __device__ inline void disp_at_f(const float *__restrict__ u, const GridDims &g, int ix, int iy, int iz, float rx,
-                                 float ry, float rz, float o[3])
+                                 float ry, float rz, float o[3], const float (*nb)[3] = nullptr)
 {
     int bx, by, bz;
     float fx, fy, fz;
@@ -64,14 +88,25 @@ __device__ inline void disp_at_f(const float *__restrict__ u, const GridDims &g,
     kf_split(iy, ry, by, fy);
     kf_split(iz, rz, bz, fz);
     float c[8][3];
-    disp_at(u, g, bx, by, bz, c[0]);
-    disp_at(u, g, bx + 1, by, bz, c[1]);
-    disp_at(u, g, bx, by + 1, bz, c[2]);
-    disp_at(u, g, bx + 1, by + 1, bz, c[3]);
-    disp_at(u, g, bx, by, bz + 1, c[4]);
-    disp_at(u, g, bx + 1, by, bz + 1, c[5]);
-    disp_at(u, g, bx, by + 1, bz + 1, c[6]);
-    disp_at(u, g, bx + 1, by + 1, bz + 1, c[7]);
+    if constexpr (NB) {
+        const int e = ((bz - iz + 1) * 3 + (by - iy + 1)) * 3 + (bx - ix + 1);
+        constexpr int off[8] = {0, 1, 3, 4, 9, 10, 12, 13};
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            c[k][0] = nb[e + off[k]][0];
+            c[k][1] = nb[e + off[k]][1];
+            c[k][2] = nb[e + off[k]][2];
+        }
+    } else {
+        disp_at(u, g, bx, by, bz, c[0]);
+        disp_at(u, g, bx + 1, by, bz, c[1]);
+        disp_at(u, g, bx, by + 1, bz, c[2]);
+        disp_at(u, g, bx + 1, by + 1, bz, c[3]);
+        disp_at(u, g, bx, by, bz + 1, c[4]);
+        disp_at(u, g, bx + 1, by, bz + 1, c[5]);
+        disp_at(u, g, bx, by + 1, bz + 1, c[6]);
+        disp_at(u, g, bx + 1, by + 1, bz + 1, c[7]);
+    }
 #pragma unroll
     for (int k = 0; k < 3; k++)
         o[k] = kf_interp3(c[0][k], c[1][k], c[2][k], c[3][k], c[4][k], c[5][k], c[6][k], c[7][k], fx, fy, fz);
@@ -111,9 +146,9 @@ struct RefConsts {
 // trilinearly there (DisplacementField::getDisplacementAtf); an SDF sample then reads the live
 // SDF at the point moved by it (SDF::getDistancef, SDF.cpp:577-582: phi at loc + u_f(loc) +
 // 0.5, whose shifted coordinate is i + (r + u_f)).
-template <class S>
+template <class S, bool NB>
 __device__ inline void ref_sample(const PhiView<S> &L, const float *__restrict__ u, float h, int s, int x, int y,
-                                  int z, float *phi_out, float *u_out)
+                                  int z, float *phi_out, float *u_out, const float (*nb)[3])
 {
     float r0, r1, r2;
     if (s < 6) {
@@ -134,7 +169,7 @@ __device__ inline void ref_sample(const PhiView<S> &L, const float *__restrict__
         r2 = h * (float)c_koff[k][2];
     }
     float d[3];
-    disp_at_f(u, L.g, x, y, z, r0, r1, r2, d);
+    disp_at_f<NB>(u, L.g, x, y, z, r0, r1, r2, d, nb);
     if (s < NPHI) {
         phi_out[s] = L.trilinear(x, y, z, r0 + d[0], r1 + d[1], r2 + d[2]);
     } else {
@@ -370,7 +405,11 @@ __global__ __launch_bounds__(BLOCK) void k_ref_scan(PhiView<S> L, const float *_
 // combination are the per-voxel arithmetic of the reference's stencils, so the result does not
 // depend on the list's order or on how the pairs are spread.  A step launch also clears the
 // other list length, which the next launch's pass 1 counts into.
-template <class S, int MODE>
+// NB (0 < h < 1/4): every sample's u cell lies in the voxel's 3x3x3 neighbourhood (the SDF
+// samples sit at i + 0.5 +- up to 2h, the Killing samples at i +- up to 2h), so the block reads
+// that neighbourhood once per voxel, under the same linear-index test, and the samples take
+// their u taps from LDS: 81 loads per voxel instead of 24 per sample.
+template <class S, int MODE, bool NB>
 __global__ __launch_bounds__(BLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, RefConsts c, DeformState *st,
                                                     BlockPartial *partials, unsigned long long *aux, int parity,
@@ -385,6 +424,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn,
     __shared__ float s_psi[RK], s_dst[RK];
     __shared__ float s_f[RK][NPHI];
     __shared__ float s_k[RK][NKIL][3];
+    __shared__ float s_nb[NB ? RK : 1][27][3];
     const GridDims g = L.g;
     const unsigned long long count = aux[parity];
     if (MODE == KF_MODE_STEP && blockIdx.x == 0 && threadIdx.x == 0)
@@ -408,13 +448,26 @@ __global__ __launch_bounds__(BLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn,
             s_psi[t] = list_psi[c0 + t];
             s_dst[t] = c.use_data ? Cn.at(x, y, z) : 0.0f;
         }
+        if constexpr (NB) {
+#pragma unroll
+            for (int q = 0; q < (RK * 27 + BLOCK - 1) / BLOCK; q++) {
+                const int t = (int)threadIdx.x + q * BLOCK;
+                if (t < nv * 27) {
+                    const int v = t / 27, e = t - v * 27;
+                    int x, y, z;
+                    kf_xyz(g, (long long)list_idx[c0 + v], x, y, z);
+                    disp_at(u_in, g, x + e % 3 - 1, y + (e / 3) % 3 - 1, z + e / 9 - 1, s_nb[v][e]);
+                }
+            }
+        }
         __syncthreads();
         const int total = nv * c.nsamp;
         for (int t = threadIdx.x; t < total; t += BLOCK) {
             const int v = t / c.nsamp;
             const int si = t - v * c.nsamp;
             const int smp = si < c.nphi ? si : NPHI + (si - c.nphi);
-            ref_sample<S>(L, u_in, c.h, smp, s_xyz[v][0], s_xyz[v][1], s_xyz[v][2], s_f[v], &s_k[v][0][0]);
+            ref_sample<S, NB>(L, u_in, c.h, smp, s_xyz[v][0], s_xyz[v][1], s_xyz[v][2], s_f[v], &s_k[v][0][0],
+                              s_nb[NB ? v : 0]);
         }
         __syncthreads();
         if ((int)threadIdx.x < nv) {
@@ -509,6 +562,7 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
     }
     kf_status s = KF_OK;
     const bool vec = n % 4 == 0;
+    const bool nbhd = c.h > 0.0f && c.h < 0.25f;
     const unsigned int nb_a = (unsigned int)((n + (vec ? 4 : 1) * BLOCK - 1) / ((vec ? 4 : 1) * BLOCK));
     const int64_t chunks = (n + RK - 1) / RK;
     const unsigned int nb_b = (unsigned int)(chunks < LIST_BLOCKS ? chunks : LIST_BLOCKS);
@@ -532,8 +586,12 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
             else
                 hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, 1>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, parts, w.aux, parity, w.idx, w.psi);
-            hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in,
-                               u_out, c, ctx->state, parts, w.aux, parity, w.idx, w.psi);
+            if (nbhd)
+                hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD, true>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L,
+                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, w.idx, w.psi);
+            else
+                hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD, false>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L,
+                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, w.idx, w.psi);
         } else {
             if (vec)
                 hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, 4>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
@@ -541,8 +599,12 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
             else
                 hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, 1>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, parts, w.aux, parity, w.idx, w.psi);
-            hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in,
-                               u_out, c, ctx->state, parts + nb_a, w.aux, parity, w.idx, w.psi);
+            if (nbhd)
+                hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, true>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L,
+                                   Cn, u_in, u_out, c, ctx->state, parts + nb_a, w.aux, parity, w.idx, w.psi);
+            else
+                hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, false>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L,
+                                   Cn, u_in, u_out, c, ctx->state, parts + nb_a, w.aux, parity, w.idx, w.psi);
         }
     });
     KF_LAUNCH_CHECK();

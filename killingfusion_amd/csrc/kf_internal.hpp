@@ -335,7 +335,10 @@ __device__ __host__ inline unsigned int kf_xcd_chunk_tile(unsigned int b, unsign
 __device__ __host__ inline void kf_split(int i, float r, int &b, float &f)
 {
     const float fl = floorf(r);
-    int bi = i + (int)fl;
+    // The cell index is clamped to +-2^30 before the conversion (the weight keeps the exact
+    // floor): for |r| >= 2^30 every tap lies outside the grid (at most 2^15 per axis) and no
+    // index arithmetic can overflow.  The reference's own (int) cast is undefined there.
+    int bi = i + (int)fminf(fmaxf(fl, -1073741824.0f), 1073741824.0f);
     float fr = r - fl;
     if (bi < 0 && fr > 0.0f) {  // i + r < 0, not an integer: truncation rounds toward zero
         bi += 1;
