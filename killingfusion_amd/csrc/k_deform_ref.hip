@@ -133,7 +133,7 @@ __constant__ signed char c_koff[19][3] = {
 constexpr int NPHI = 25;
 constexpr int NKIL = 19;
 // Active voxels whose samples one round of a block takes.
-constexpr int RK = 32;
+constexpr int RK = 64;
 
 struct RefConsts {
     float voxel_size, hi, lo, eps, alpha, omega_k, gamma, omega_s, h, clamp;
@@ -266,6 +266,12 @@ __device__ inline void ref_combine(const RefConsts &c, float psi, float dst, con
 
 // Blocks of the list kernel (at most; fewer for small grids).
 constexpr int LIST_BLOCKS = 2048;
+// The list has one segment per XCD (block b of pass 1 runs on XCD b % 8), each with its own
+// length counter, so the appends of the eight XCDs do not meet on one address.  Workspace
+// words: the segment lengths of launch parity 0, those of parity 1, the non-finite flag.
+constexpr int NSEG = 8;
+constexpr int AUX_WORDS = 2 * NSEG + 1;
+constexpr int AUX_NONFINITE = 2 * NSEG;
 
 __device__ inline float4 ref_load_stream4(const float *p)
 {
@@ -274,11 +280,22 @@ __device__ inline float4 ref_load_stream4(const float *p)
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
-// Clears the two list lengths and the non-finite flag of the workspace.
+// Clears the list lengths and the non-finite flag of the workspace.
 __global__ void k_ref_clear(unsigned long long *aux)
 {
-    if (threadIdx.x < 3)
+    if (threadIdx.x < AUX_WORDS)
         aux[threadIdx.x] = 0ull;
+}
+
+// Slot of list entry p: segments hold pre[x+1] - pre[x] entries from x * segcap on.
+__device__ inline unsigned long long ref_slot(const unsigned long long pre[NSEG + 1], unsigned long long segcap,
+                                              unsigned long long p)
+{
+    int x = 0;
+#pragma unroll
+    for (int k = 1; k < NSEG; k++)
+        x += p >= pre[k] ? 1 : 0;
+    return (unsigned long long)x * segcap + (p - pre[x]);
 }
 
 // Sets *flag when some live SDF value, decoded, is not finite.
@@ -296,23 +313,26 @@ __global__ __launch_bounds__(BLOCK) void k_ref_finite(PhiView<S> L, unsigned lon
 // psi = SDF::getDistance(idx, disp) (SDF.cpp:570-575) and the band test
 // (KillingFusion.cpp:272-274), V voxels per thread (4: one float4 of each u plane).  A voxel
 // outside the band keeps u: its update is u + (-alpha * 0).  A voxel in the band is appended
-// to the list for pass 2 (one atomic per wavefront), which overwrites its entry of u_out.
+// to its XCD's list segment for pass 2 (one atomic per block), which overwrites its entry of
+// u_out; the list's length is the launch's active count.
 // Where u_j is +0 or -0 the interpolation weights are +0 and the interpolant is
-// phi_j + (finite) * 0 + ...; when every live value is finite (aux[2] == 0) that is phi_j
+// phi_j + (finite) * 0 + ...; when every live value is finite (no non-finite flag) that is phi_j
 // itself unless phi_j is -0, so those voxels read phi_j alone.
 template <class S, int MODE, int V>
 __global__ __launch_bounds__(BLOCK) void k_ref_scan(PhiView<S> L, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, float hi, float lo, float alpha,
-                                                    DeformState *st, BlockPartial *partials,
-                                                    unsigned long long *aux, int parity,
+                                                    const DeformState *st, unsigned long long *aux, int parity,
+                                                    unsigned long long segcap,
                                                     unsigned long long *__restrict__ list_idx,
                                                     float *__restrict__ list_psi)
 {
     if (MODE == KF_MODE_STEP && st->done)
         return;
+    __shared__ unsigned int s_wtot[BLOCK / 64];
+    __shared__ unsigned long long s_base;
     const GridDims g = L.g;
     const int64_t i0 = ((int64_t)kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x) * V;
-    const bool phi_finite = aux[2] == 0ull;
+    const bool phi_finite = aux[AUX_NONFINITE] == 0ull;
     bool act[V];
     float psi[V];
 #pragma unroll
@@ -372,13 +392,23 @@ __global__ __launch_bounds__(BLOCK) void k_ref_scan(PhiView<S> L, const float *_
         m[v] = __ballot(act[v]);
         tot += (unsigned int)__popcll(m[v]);
     }
+    const int wv = (int)(threadIdx.x >> 6);
+    if (__lane_id() == 0)
+        s_wtot[wv] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int bt = 0;
+#pragma unroll
+        for (int k = 0; k < BLOCK / 64; k++)
+            bt += s_wtot[k];
+        const unsigned int seg = blockIdx.x % NSEG;
+        s_base = bt ? seg * segcap + atomicAdd(&aux[parity * NSEG + seg], (unsigned long long)bt) : 0ull;
+    }
+    __syncthreads();
     if (tot != 0u) {
-        unsigned long long base = 0ull;
-        if (__lane_id() == 0)
-            base = atomicAdd(&aux[parity], (unsigned long long)tot);
-        const unsigned int blo = __shfl((unsigned int)base, 0, 64);
-        const unsigned int bhi = __shfl((unsigned int)(base >> 32), 0, 64);
-        unsigned long long pos = ((unsigned long long)bhi << 32) | blo;
+        unsigned long long pos = s_base;
+        for (int k = 0; k < wv; k++)
+            pos += s_wtot[k];
         const unsigned long long below = (1ull << __lane_id()) - 1ull;
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -390,21 +420,15 @@ __global__ __launch_bounds__(BLOCK) void k_ref_scan(PhiView<S> L, const float *_
             pos += (unsigned long long)__popcll(m[v]);
         }
     }
-    if (MODE == KF_MODE_STEP) {
-        unsigned long long cnt = 0;
-#pragma unroll
-        for (int v = 0; v < V; v++)
-            cnt += act[v] ? 1ull : 0ull;
-        kf_block_publish<BLOCK>(partials, blockIdx.x, 0.0, 0.0, cnt);
-    }
 }
 
 // Pass 2: the voxels of the list, RK at a time per block.  The block takes their samples, one
 // (voxel, sample) pair per thread, so a voxel's 44 dependent reads run side by side instead of
 // one after another in one lane, and a thread per voxel combines them.  Each sample and the
 // combination are the per-voxel arithmetic of the reference's stencils, so the result does not
-// depend on the list's order or on how the pairs are spread.  A step launch also clears the
-// other list length, which the next launch's pass 1 counts into.
+// depend on the list's order or on how the pairs are spread.  A step launch publishes the
+// list's length as the active count (block 0, thread 0) and clears the other list length, which the next
+// launch's pass 1 counts into.
 // NB (0 < h < 1/4): every sample's u cell lies in the voxel's 3x3x3 neighbourhood (the SDF
 // samples sit at i + 0.5 +- up to 2h, the Killing samples at i +- up to 2h), so the block reads
 // that neighbourhood once per voxel, under the same linear-index test, and the samples take
@@ -413,6 +437,7 @@ template <class S, int MODE, bool NB>
 __global__ __launch_bounds__(BLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, RefConsts c, DeformState *st,
                                                     BlockPartial *partials, unsigned long long *aux, int parity,
+                                                    unsigned long long segcap,
                                                     const unsigned long long *__restrict__ list_idx,
                                                     const float *__restrict__ list_psi)
 {
@@ -426,16 +451,22 @@ __global__ __launch_bounds__(BLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn,
     __shared__ float s_k[RK][NKIL][3];
     __shared__ float s_nb[NB ? RK : 1][27][3];
     const GridDims g = L.g;
-    const unsigned long long count = aux[parity];
-    if (MODE == KF_MODE_STEP && blockIdx.x == 0 && threadIdx.x == 0)
-        aux[parity ^ 1] = 0ull;
+    unsigned long long pre[NSEG + 1];
+    pre[0] = 0ull;
+#pragma unroll
+    for (int k = 0; k < NSEG; k++)
+        pre[k + 1] = pre[k] + aux[parity * NSEG + k];
+    const unsigned long long count = pre[NSEG];
+    if (MODE == KF_MODE_STEP && blockIdx.x == 0 && threadIdx.x < NSEG)
+        aux[(parity ^ 1) * NSEG + threadIdx.x] = 0ull;
     double mx = 0.0;
     for (unsigned long long c0 = (unsigned long long)blockIdx.x * RK; c0 < count;
          c0 += (unsigned long long)gridDim.x * RK) {
         const int nv = (int)(count - c0 < (unsigned long long)RK ? count - c0 : (unsigned long long)RK);
         if ((int)threadIdx.x < nv) {
             const int t = threadIdx.x;
-            const long long j = (long long)list_idx[c0 + t];
+            const unsigned long long q = ref_slot(pre, segcap, c0 + t);
+            const long long j = (long long)list_idx[q];
             int x, y, z;
             kf_xyz(g, j, x, y, z);
             s_i[t] = j;
@@ -445,7 +476,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn,
             s_u[t][0] = u_in[j];
             s_u[t][1] = u_in[g.n + j];
             s_u[t][2] = u_in[2 * g.n + j];
-            s_psi[t] = list_psi[c0 + t];
+            s_psi[t] = list_psi[q];
             s_dst[t] = c.use_data ? Cn.at(x, y, z) : 0.0f;
         }
         if constexpr (NB) {
@@ -455,7 +486,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn,
                 if (t < nv * 27) {
                     const int v = t / 27, e = t - v * 27;
                     int x, y, z;
-                    kf_xyz(g, (long long)list_idx[c0 + v], x, y, z);
+                    kf_xyz(g, (long long)list_idx[ref_slot(pre, segcap, c0 + v)], x, y, z);
                     disp_at(u_in, g, x + e % 3 - 1, y + (e / 3) % 3 - 1, z + e / 9 - 1, s_nb[v][e]);
                 }
             }
@@ -495,7 +526,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn,
         __syncthreads();
     }
     if (MODE == KF_MODE_STEP)
-        kf_block_publish<BLOCK>(partials, blockIdx.x, 0.0, mx, 0ull);
+        kf_block_publish<BLOCK>(partials, blockIdx.x, 0.0, mx, blockIdx.x == 0 && threadIdx.x == 0 ? count : 0ull);
 }
 
 template <class S>
@@ -546,27 +577,31 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
     c.nphi = c.use_ls ? NPHI : (c.use_data ? 6 : 0);
     c.nsamp = c.nphi + (c.use_k ? NKIL : 0);
     const int64_t n = live->n;
-    // The list holds at most every voxel once.
+    const bool vec = n % 4 == 0;
+    const bool nbhd = c.h > 0.0f && c.h < 0.25f;
+    const int64_t per_block = (vec ? 4 : 1) * BLOCK;
+    const unsigned int nb_a = (unsigned int)((n + per_block - 1) / per_block);
+    // Segment x takes the band voxels of the pass-1 blocks b with b % 8 == x: at most their
+    // voxels, so every segment has room for ceil(nb_a / 8) blocks.
+    const unsigned long long segcap = (unsigned long long)((nb_a + NSEG - 1) / NSEG) * (unsigned long long)per_block;
+    const size_t need = (size_t)segcap * NSEG;
     kf_ref_ws &w = ctx->ref;
-    if (w.cap < (size_t)n) {
+    if (w.cap < need) {
         kf_ref_release(ctx);
-        if (hipMalloc((void **)&w.idx, sizeof(unsigned long long) * (size_t)n) != hipSuccess ||
-            hipMalloc((void **)&w.psi, sizeof(float) * (size_t)n) != hipSuccess ||
-            hipMalloc((void **)&w.aux, 3 * sizeof(unsigned long long)) != hipSuccess) {
+        if (hipMalloc((void **)&w.idx, sizeof(unsigned long long) * need) != hipSuccess ||
+            hipMalloc((void **)&w.psi, sizeof(float) * need) != hipSuccess ||
+            hipMalloc((void **)&w.aux, AUX_WORDS * sizeof(unsigned long long)) != hipSuccess) {
             (void)hipGetLastError();
             kf_ref_release(ctx);
             kf_set_error("reference-step list allocation failed");
             return KF_E_OOM;
         }
-        w.cap = (size_t)n;
+        w.cap = need;
     }
     kf_status s = KF_OK;
-    const bool vec = n % 4 == 0;
-    const bool nbhd = c.h > 0.0f && c.h < 0.25f;
-    const unsigned int nb_a = (unsigned int)((n + (vec ? 4 : 1) * BLOCK - 1) / ((vec ? 4 : 1) * BLOCK));
     const int64_t chunks = (n + RK - 1) / RK;
     const unsigned int nb_b = (unsigned int)(chunks < LIST_BLOCKS ? chunks : LIST_BLOCKS);
-    if (mode != KF_MODE_GRAD && (s = kf_ensure_partials(ctx, (size_t)nb_a + nb_b)) != KF_OK)
+    if (mode != KF_MODE_GRAD && (s = kf_ensure_partials(ctx, nb_b)) != KF_OK)
         return s;
     BlockPartial *parts = kf_partials(ctx, parity);
     KF_DISPATCH_DTYPE(live->dtype, S, {
@@ -577,36 +612,36 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
             hipLaunchKernelGGL(k_ref_clear, dim3(1), dim3(64), 0, ctx->stream, w.aux);
             const int64_t fb = (n + BLOCK - 1) / BLOCK;
             hipLaunchKernelGGL(k_ref_finite<S>, dim3((unsigned int)(fb < 4096 ? fb : 4096)), dim3(BLOCK), 0,
-                               ctx->stream, L, w.aux + 2);
+                               ctx->stream, L, w.aux + AUX_NONFINITE);
         }
         if (mode == KF_MODE_GRAD) {
             if (vec)
                 hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, 4>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
-                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, parts, w.aux, parity, w.idx, w.psi);
+                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             else
                 hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, 1>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
-                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, parts, w.aux, parity, w.idx, w.psi);
+                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             if (nbhd)
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD, true>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L,
-                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, w.idx, w.psi);
+                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
             else
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD, false>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L,
-                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, w.idx, w.psi);
+                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
         } else {
             if (vec)
                 hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, 4>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
-                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, parts, w.aux, parity, w.idx, w.psi);
+                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             else
                 hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, 1>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
-                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, parts, w.aux, parity, w.idx, w.psi);
+                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             if (nbhd)
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, true>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L,
-                                   Cn, u_in, u_out, c, ctx->state, parts + nb_a, w.aux, parity, w.idx, w.psi);
+                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
             else
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, false>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L,
-                                   Cn, u_in, u_out, c, ctx->state, parts + nb_a, w.aux, parity, w.idx, w.psi);
+                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
         }
     });
     KF_LAUNCH_CHECK();
-    return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb_a + nb_b, parity);
+    return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb_b, parity);
 }

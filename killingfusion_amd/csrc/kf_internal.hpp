@@ -111,10 +111,10 @@ struct kf_active_ws {
 // Device workspace of the reference-formula step (k_deform_ref.hip), grown on demand and kept
 // by the context: the voxels in the band of a step launch, listed by its first pass.
 struct kf_ref_ws {
-    unsigned long long *idx = nullptr;  // voxel indices, in no particular order
+    unsigned long long *idx = nullptr;  // voxel indices, one segment per XCD, in no particular order
     float *psi = nullptr;               // their warped live SDF value
     size_t cap = 0;                     // entries of idx and psi
-    unsigned long long *aux = nullptr;  // list lengths of launch parity 0 and 1; [2]: a live SDF value is not finite
+    unsigned long long *aux = nullptr;  // segment lengths of launch parity 0 and 1, then the non-finite flag
 };
 
 struct kf_context {
