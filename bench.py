@@ -244,7 +244,7 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--plain-steps", type=int, default=0, help="iterations timed for the plain kernel (default steps/4)")
+    ap.add_argument("--plain-steps", type=int, default=0, help="iterations timed for the plain kernel (default: steps)")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-volume measurement")
@@ -304,9 +304,9 @@ def main(argv=None):
     kp_act = KillingParams.paper(VOXEL, 3.0, method="adjoint", sweep="active")
     wall_a, gpu_ms_a, st_a = time_deform(ctx, live_bf, can_bf, kp_act, a.steps, a.warmup, events)
 
-    plain_steps = a.plain_steps or max(5, a.steps // 4)
+    plain_steps = a.plain_steps or a.steps
     kp_ref = KillingParams.paper(VOXEL, 3.0, method="reference")
-    wall_p, gpu_ms_p, st_p = time_deform(ctx, live_32, can_32, kp_ref, plain_steps, max(1, a.warmup // 4), events)
+    wall_p, gpu_ms_p, st_p = time_deform(ctx, live_32, can_32, kp_ref, plain_steps, a.warmup, events)
     plain_value = nvox * plain_steps / wall_p
     plain_kernel_ms = gpu_ms_p / plain_steps
 
