@@ -1,0 +1,142 @@
+"""The reference-formula step (kf_deform with method "reference", k_deform_ref.hip) against the
+FP64 oracle on inputs that take its less common code paths:
+
+* a grid whose voxel count is not a multiple of 4: pass 1 runs one voxel per thread;
+* a finite-difference step h >= 1/4: pass 2 reads every sample's u cell from memory instead
+  of the voxel's 3x3x3 neighbourhood;
+* -0 live values: those voxels interpolate in pass 1 instead of reading phi_j;
+* a non-finite live value: every voxel interpolates, and the step diverges where the oracle
+  does (the cells holding +inf give psi = NaN, KillingFusion.cpp:272-274 treats NaN as in the
+  band).
+
+Parameters are the reference's (config.cpp:45-54), under which the FP32 kernels stay within
+1e-4 voxel of the FP64 oracle (test_gpu_parity.py).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from killingfusion_amd import DisplacementField, KillingParams, compute_displacement_field, energy_gradient_ref, scenes
+from killingfusion_amd._lib import KF_E_DIVERGED, KFError
+
+from helpers import C1, make_pair, ogrid, oparams, sphere_frames
+
+pytestmark = pytest.mark.gpu
+
+VOX = scenes.BATCH_VOXEL
+
+
+def ref_params(voxel, trunc, **kw):
+    return KillingParams(voxel_size=voxel, max_surface=trunc, clip_distance=trunc, **kw)
+
+
+def run_both(ctx, live, can, g, kp, u0=None):
+    """GPU deform and the oracle's Jacobi deform from the same start field."""
+    n = g.n
+    u0 = np.zeros((3, n), dtype=np.float32) if u0 is None else u0
+    f = DisplacementField(ctx, live.dims)
+    f.upload(u0)
+    st = compute_displacement_field(live, can, f, kp)
+    ug = f.download().astype(np.float64)
+    f.close()
+    lphi, _ = live.download()
+    cphi, _ = can.download()
+    uo, so = O.deform_jacobi(lphi, cphi, u0.astype(np.float64), g, oparams(kp))
+    err = np.abs(ug - uo).max()
+    print(f"iters {st['iters']}/{so['iters']} active {st['active']}/{so['active']} max err {err:.3e}", flush=True)
+    return st, so, err
+
+
+@pytest.fixture(scope="module")
+def sphere(ctx):
+    d1, d2 = sphere_frames()
+    live, can = make_pair(ctx, C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"], d2, d1)
+    return live, can, ogrid(C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"])
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+@pytest.mark.parametrize("dims", [(61, 45, 58), (63, 47, 29)])
+def test_uneven_grid(ctx, dims, dtype):
+    print(f"case: ellipsoid pair {dims} ({np.prod(dims) % 4} voxels past a multiple of 4), {dtype}, "
+          "16 iterations", flush=True)
+    ext = [d * VOX for d in dims]
+    d1, d2, _ = scenes.ellipsoid_pair(0x5EED, 3, max_axis=0.4 * min(ext))
+    min3d = (-ext[0] / 2, -ext[1] / 2, 0.8 - ext[2] / 2)
+    live, can = make_pair(ctx, dims, min3d, VOX, 3 * VOX, d2, d1, dtype)
+    g = ogrid(dims, min3d, VOX, 3 * VOX)
+    st, so, err = run_both(ctx, live, can, g, ref_params(VOX, 3 * VOX, max_iters=16))
+    assert int(np.prod(dims)) % 4 != 0
+    assert st["iters"] == so["iters"] == 16 and st["active"] == so["active"] > 0
+    assert err <= 1e-4
+
+
+@pytest.mark.parametrize("h", [0.25, 0.3])
+def test_wide_stencil(ctx, sphere, h):
+    live, can, g = sphere
+    print(f"case: 64^3 sphere pair, h = {h} voxel, 24 iterations and the gradient at a seeded u", flush=True)
+    kp = ref_params(C1["voxel"], C1["trunc"], max_iters=24, delta_size=h)
+    st, so, err = run_both(ctx, live, can, g, kp)
+    assert st["iters"] == so["iters"] and st["active"] == so["active"]
+    assert err <= 1e-4
+    u = scenes.seeded_field(C1["dims"], amplitude=0.5, seed=9)
+    f, out = DisplacementField(ctx, C1["dims"]), DisplacementField(ctx, C1["dims"])
+    f.upload(u)
+    energy_gradient_ref(live, can, f, kp, out)
+    gg = out.download().astype(np.float64)
+    lphi, _ = live.download()
+    cphi, _ = can.download()
+    og = O.energy_gradient_grid(lphi.astype(np.float64), cphi.astype(np.float64), u.astype(np.float64), g,
+                                oparams(kp), masked=True)
+    gm, om = np.any(gg != 0, axis=0), np.any(og != 0, axis=0)
+    both = gm & om
+    print(f"mask mismatches {(gm != om).sum()}, compared {both.sum()}", flush=True)
+    assert (gm != om).sum() <= 2 and both.sum() > 3000
+    assert np.abs(gg[:, both] - og[:, both]).max() <= 1e-4 * (1 + np.abs(og[:, both]).max())
+
+
+def test_negative_zero_live_values(ctx, sphere):
+    live, can, g = sphere
+    phi, w = live.download()
+    band = np.nonzero(np.abs(phi) < 0.3 * C1["trunc"])[0]
+    neg = band[::40]
+    print(f"case: 64^3 sphere pair, -0 at {len(neg)} band voxels, 20 iterations", flush=True)
+    p2 = phi.copy()
+    p2[neg] = -0.0
+    live.upload(p2, w)
+    try:
+        st, so, err = run_both(ctx, live, can, g, ref_params(C1["voxel"], C1["trunc"], max_iters=20))
+    finally:
+        live.upload(phi, w)
+    assert st["iters"] == so["iters"] and st["active"] == so["active"]
+    assert err <= 1e-4
+
+
+def test_non_finite_live_value(ctx, sphere):
+    live, can, g = sphere
+    phi, w = live.download()
+    k = 64 * 64 * 40 + 64 * 20 + 33
+    print(f"case: 64^3 sphere pair, +inf live value at voxel {k}", flush=True)
+    p3 = phi.copy()
+    p3[k] = np.inf
+    live.upload(p3, w)
+    try:
+        kp = ref_params(C1["voxel"], C1["trunc"], max_iters=5)
+        f = DisplacementField(ctx, C1["dims"])
+        with pytest.raises(KFError) as ei:
+            compute_displacement_field(live, can, f, kp)
+        f.close()
+        uo, so = O.deform_jacobi(p3.astype(np.float64), can.download()[0], np.zeros((3, g.n)), g, oparams(kp))
+        print(str(ei.value), so, flush=True)
+        assert ei.value.status == KF_E_DIVERGED
+        assert so["diverged_index"] >= 0 and f"diverged at voxel {so['diverged_index']}" in str(ei.value)
+        # the gradient is NaN exactly where the oracle's is
+        u = np.zeros((3, g.n), dtype=np.float32)
+        f, out = DisplacementField(ctx, C1["dims"]), DisplacementField(ctx, C1["dims"])
+        energy_gradient_ref(live, can, f, kp, out)
+        gg = out.download()
+        og = O.energy_gradient_grid(p3.astype(np.float64), can.download()[0].astype(np.float64), u.astype(np.float64),
+                                    g, oparams(kp), masked=True)
+        assert np.isnan(og).any()
+        assert np.array_equal(np.isnan(gg), np.isnan(og))
+    finally:
+        live.upload(phi, w)
