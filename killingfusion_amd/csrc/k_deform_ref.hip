@@ -132,8 +132,9 @@ __constant__ signed char c_koff[19][3] = {
 // (SDF.cpp:842-886) and 25-43 the displacement at the Killing stencil points.
 constexpr int NPHI = 25;
 constexpr int NKIL = 19;
-// Active voxels whose samples one round of a block takes.
+// Active voxels whose samples one round of a list block takes, and the list block's threads.
 constexpr int RK = 64;
+constexpr int LBLOCK = 256;
 
 struct RefConsts {
     float voxel_size, hi, lo, eps, alpha, omega_k, gamma, omega_s, h, clamp;
@@ -298,13 +299,22 @@ __device__ inline unsigned long long ref_slot(const unsigned long long pre[NSEG 
     return (unsigned long long)x * segcap + (p - pre[x]);
 }
 
-// Sets *flag when some live SDF value, decoded, is not finite.
-template <class S>
+// Sets *flag when some live SDF value, decoded, is not finite.  V values per thread and step
+// (4: one vector load; the grid's voxel count is then a multiple of 4).
+template <class S, int V>
 __global__ __launch_bounds__(BLOCK) void k_ref_finite(PhiView<S> L, unsigned long long *flag)
 {
     bool bad = false;
-    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < L.g.n; i += (int64_t)gridDim.x * BLOCK)
-        bad |= !isfinite(S::load(L.p, i, L.scale));
+    for (int64_t i = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * V; i < L.g.n; i += (int64_t)gridDim.x * BLOCK * V) {
+        float v[V];
+        if constexpr (V == 4)
+            S::load4(L.p, i, L.scale, v);
+        else
+            v[0] = S::load(L.p, i, L.scale);
+#pragma unroll
+        for (int k = 0; k < V; k++)
+            bad |= !isfinite(v[k]);
+    }
     if (__ballot(bad) != 0ull && __lane_id() == 0)
         atomicOr(flag, 1ull);
 }
@@ -434,7 +444,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_scan(PhiView<S> L, const float *_
 // that neighbourhood once per voxel, under the same linear-index test, and the samples take
 // their u taps from LDS: 81 loads per voxel instead of 24 per sample.
 template <class S, int MODE, bool NB>
-__global__ __launch_bounds__(BLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn, const float *__restrict__ u_in,
+__global__ __launch_bounds__(LBLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, RefConsts c, DeformState *st,
                                                     BlockPartial *partials, unsigned long long *aux, int parity,
                                                     unsigned long long segcap,
@@ -481,8 +491,8 @@ __global__ __launch_bounds__(BLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn,
         }
         if constexpr (NB) {
 #pragma unroll
-            for (int q = 0; q < (RK * 27 + BLOCK - 1) / BLOCK; q++) {
-                const int t = (int)threadIdx.x + q * BLOCK;
+            for (int q = 0; q < (RK * 27 + LBLOCK - 1) / LBLOCK; q++) {
+                const int t = (int)threadIdx.x + q * LBLOCK;
                 if (t < nv * 27) {
                     const int v = t / 27, e = t - v * 27;
                     int x, y, z;
@@ -493,7 +503,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn,
         }
         __syncthreads();
         const int total = nv * c.nsamp;
-        for (int t = threadIdx.x; t < total; t += BLOCK) {
+        for (int t = threadIdx.x; t < total; t += LBLOCK) {
             const int v = t / c.nsamp;
             const int si = t - v * c.nsamp;
             const int smp = si < c.nphi ? si : NPHI + (si - c.nphi);
@@ -526,7 +536,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn,
         __syncthreads();
     }
     if (MODE == KF_MODE_STEP)
-        kf_block_publish<BLOCK>(partials, blockIdx.x, 0.0, mx, blockIdx.x == 0 && threadIdx.x == 0 ? count : 0ull);
+        kf_block_publish<LBLOCK>(partials, blockIdx.x, 0.0, mx, blockIdx.x == 0 && threadIdx.x == 0 ? count : 0ull);
 }
 
 template <class S>
@@ -610,9 +620,11 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
             // A deform call (or a gradient evaluation) starts with empty lists and checks the live
             // SDF once: it does not change while u does.
             hipLaunchKernelGGL(k_ref_clear, dim3(1), dim3(64), 0, ctx->stream, w.aux);
-            const int64_t fb = (n + BLOCK - 1) / BLOCK;
-            hipLaunchKernelGGL(k_ref_finite<S>, dim3((unsigned int)(fb < 4096 ? fb : 4096)), dim3(BLOCK), 0,
-                               ctx->stream, L, w.aux + AUX_NONFINITE);
+            const unsigned int fb = nb_a < 4096u ? nb_a : 4096u;
+            if (vec)
+                hipLaunchKernelGGL((k_ref_finite<S, 4>), dim3(fb), dim3(BLOCK), 0, ctx->stream, L, w.aux + AUX_NONFINITE);
+            else
+                hipLaunchKernelGGL((k_ref_finite<S, 1>), dim3(fb), dim3(BLOCK), 0, ctx->stream, L, w.aux + AUX_NONFINITE);
         }
         if (mode == KF_MODE_GRAD) {
             if (vec)
@@ -622,10 +634,10 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
                 hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, 1>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             if (nbhd)
-                hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD, true>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L,
+                hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD, true>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
                                    Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
             else
-                hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD, false>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L,
+                hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD, false>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
                                    Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
         } else {
             if (vec)
@@ -635,10 +647,10 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
                 hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, 1>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             if (nbhd)
-                hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, true>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L,
+                hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, true>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
                                    Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
             else
-                hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, false>), dim3(nb_b), dim3(BLOCK), 0, ctx->stream, L,
+                hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, false>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
                                    Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
         }
     });
