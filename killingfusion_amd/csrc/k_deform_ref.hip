@@ -1,14 +1,16 @@
 // k_deform_ref.hip -- the reference energy gradient (KillingFusion::computeEnergyGradient,
 // KillingFusion/src/KillingFusion.cpp:399-476) and one all-voxels Jacobi iteration
-// (KillingFusion.cpp:248-351) as a gfx950 kernel, FP32 arithmetic.
+// (KillingFusion.cpp:248-351) on gfx950, FP32 arithmetic.
 //
-// This is the "plain" kernel: each voxel evaluates the reference's finite-difference
+// This is the "plain" path: each voxel evaluates the reference's finite-difference
 // stencils literally -- 1 data sample, 6 gradient samples and 19 Hessian samples of the
 // warped live SDF, each preceded by a trilinear displacement read, plus 19 displacement
 // samples for the Killing term -- with every quirk of the reference reads (truncating
 // casts, cell-centre offsets, linear-index-only displacement bounds test, clamp before
-// the 1/(4h^2) division).  A block lists its voxels in the band and spreads their samples
-// over its threads, one (voxel, sample) pair each.
+// the 1/(4h^2) division).  A step runs in two passes: k_ref_scan streams every voxel, tests
+// the band and lists the voxels in it; k_ref_list takes the listed voxels' samples, one
+// (voxel, sample) pair per thread, and combines them.  The results do not depend on this
+// layout: each sample and each combination is the same per-voxel arithmetic.
 #include "kf_reduce.hpp"
 
 namespace {
@@ -305,7 +307,8 @@ template <class S, int V>
 __global__ __launch_bounds__(BLOCK) void k_ref_finite(PhiView<S> L, unsigned long long *flag)
 {
     bool bad = false;
-    for (int64_t i = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * V; i < L.g.n; i += (int64_t)gridDim.x * BLOCK * V) {
+    const int64_t stride = (int64_t)gridDim.x * BLOCK * V;
+    for (int64_t i = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * V; i < L.g.n; i += stride) {
         float v[V];
         if constexpr (V == 4)
             S::load4(L.p, i, L.scale, v);
