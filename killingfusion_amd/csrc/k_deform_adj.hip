@@ -115,21 +115,47 @@ __global__ __launch_bounds__(BLOCK) void k_adj_any(PhiRead<S> L, PhiRead<S> Cn, 
 // writes them with a ballot -- no atomics.
 constexpr int MW = ROW / VEC / 32;
 
-// Bits of groups gx - 1 .. gx + 1 of row (y, z) (gx = 64 seg + lane); 0 outside the grid.
-__device__ inline unsigned int nbr_bits(const unsigned int *__restrict__ m, int wpr, const GridDims &g, int y, int z,
-                                        int seg, int lane)
+// Word wi of row (y, z) dilated by one group along x: bit b is set iff group 32 wi + b - 1,
+// + b or + b + 1 of the row is; 0 outside the grid.
+__device__ inline unsigned int row_word_dilated(const unsigned int *__restrict__ m, int wpr, const GridDims &g, int y,
+                                                int z, int wi)
 {
     if (y < 0 || y >= g.ny || z < 0 || z >= g.nz)
         return 0u;
     const unsigned int *row = m + ((unsigned int)z * (unsigned int)g.ny + (unsigned int)y) * (unsigned int)wpr;
+    const unsigned int w = row[wi];
+    const unsigned int lo = wi > 0 ? row[wi - 1] >> 31 : 0u;
+    const unsigned int hi = wi + 1 < wpr ? row[wi + 1] << 31 : 0u;
+    return w | (w << 1) | (w >> 1) | lo | hi;
+}
+
+// Stencil mask: bit (row, group) of d is set iff the nonzero mask m has a bit set among
+// groups gx - 1 .. gx + 1 of row (y, z) and of the rows (y +- 1, z), (y, z +- 1),
+// (y + 1, z - 1), (y - 1, z + 1) -- every u value the adjoint stencil of the group's four
+// voxels reads.  One thread per word; the energy and gradient passes then test one bit.
+__global__ __launch_bounds__(BLOCK) void k_adj_dilate(const unsigned int *__restrict__ m, GridDims g, int wpr,
+                                                      unsigned int *__restrict__ d)
+{
+    const unsigned int words = (unsigned int)wpr * (unsigned int)g.ny * (unsigned int)g.nz;
+    const unsigned int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= words)
+        return;
+    const int wi = (int)(k % (unsigned int)wpr);
+    const unsigned int row = k / (unsigned int)wpr;
+    const int y = (int)(row % (unsigned int)g.ny), z = (int)(row / (unsigned int)g.ny);
+    d[k] = row_word_dilated(m, wpr, g, y, z, wi) | row_word_dilated(m, wpr, g, y + 1, z, wi) |
+           row_word_dilated(m, wpr, g, y - 1, z, wi) | row_word_dilated(m, wpr, g, y, z + 1, wi) |
+           row_word_dilated(m, wpr, g, y, z - 1, wi) | row_word_dilated(m, wpr, g, y + 1, z - 1, wi) |
+           row_word_dilated(m, wpr, g, y - 1, z + 1, wi);
+}
+
+// Bit of group gx = 64 seg + lane of row (y, z) in stencil mask d (y, z in the grid).
+__device__ inline unsigned int stencil_bit(const unsigned int *__restrict__ d, int wpr, const GridDims &g, int y,
+                                           int z, int seg, int lane)
+{
+    const unsigned int *row = d + ((unsigned int)z * (unsigned int)g.ny + (unsigned int)y) * (unsigned int)wpr;
     const int gx = seg * (ROW / VEC) + lane;
-    const int wi = gx >> 5, b = gx & 31;
-    unsigned int r = row[wi] & (b == 0 ? 0x3u : (0x7u << (b - 1)));
-    if (b == 0 && wi > 0)
-        r |= row[wi - 1] >> 31;
-    if (b == 31 && wi + 1 < wpr)
-        r |= row[wi + 1] & 1u;
-    return r;
+    return (row[gx >> 5] >> (gx & 31)) & 1u;
 }
 
 // Writes the wavefront's mask words of row (y, z): bit lane = nz of that lane's group.
@@ -173,8 +199,9 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
     // Phase 1 -- quiet test.  A voxel is quiet when every u value its stencil touches is
     // zero (everywhere away from the surface band); then psi_j = phi_live[j] exactly (zero
     // interpolation weights) and an inactive quiet voxel has zero gradient and energy.  The
-    // stencil of a thread's 4 voxels reaches groups gx - 1 .. gx + 1 of seven rows; their mask
-    // bits decide quietness without touching u.  Both rows' loads are issued up front.
+    // stencil of a thread's 4 voxels reaches groups gx - 1 .. gx + 1 of seven rows; one bit of
+    // the stencil mask (k_adj_dilate) holds their union, so quietness costs one load and no
+    // u values beyond the thread's own.  All loads are issued up front.
     float4 C[RPT][3];
     float live4[RPT][VEC];
     unsigned int nb_bits[RPT];
@@ -190,13 +217,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
         if (valid[r])
             S::load4(L.p, (int64_t)g.idx32(x0, y, z), L.scale, live4[r]);
         nb_bits[r] = 0u;
-        if (MODE != KF_MODE_STEP)
-            nb_bits[r] = nbr_bits(mask_in, wpr, g, y, z, seg, lane) | nbr_bits(mask_in, wpr, g, y + 1, z, seg, lane) |
-                         nbr_bits(mask_in, wpr, g, y - 1, z, seg, lane) |
-                         nbr_bits(mask_in, wpr, g, y, z + 1, seg, lane) |
-                         nbr_bits(mask_in, wpr, g, y, z - 1, seg, lane) |
-                         nbr_bits(mask_in, wpr, g, y + 1, z - 1, seg, lane) |
-                         nbr_bits(mask_in, wpr, g, y - 1, z + 1, seg, lane);
+        if (MODE != KF_MODE_STEP && valid[r])
+            nb_bits[r] = stencil_bit(mask_in, wpr, g, y, z, seg, lane);
     }
     // need bit r * VEC + k: voxel (x0 + k, yr[r]) goes to phase 2.  A Jacobi step moves only
     // active voxels, so it needs the voxels that are active or whose sample moved (u_j != 0,
@@ -349,7 +371,7 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
     kf_block_publish<FB>(partials, blockIdx.x, esum, mx, act);
 }
 
-// Nonzero mask of field values u (see nbr_bits).
+// Nonzero mask of field values u (layout above).
 __global__ __launch_bounds__(BLOCK) void k_adj_mask(const float *__restrict__ u, GridDims g, int nseg, int nyt,
                                                     unsigned int *__restrict__ mask)
 {
@@ -399,9 +421,12 @@ size_t kf_adj_mask_words(const int32_t dims[3])
 kf_status kf_launch_adj_mask(const kf_field *f, const float *u, unsigned int *mask)
 {
     const int nseg = (f->dims[0] + ROW - 1) / ROW, nyt = (f->dims[1] + ROWS - 1) / ROWS;
-    KF_CHECK_ARG(kf_adj_mask_words(f->dims) > 0, "nonzero masks need rows of a multiple of 4 voxels");
+    const size_t words = kf_adj_mask_words(f->dims);
+    KF_CHECK_ARG(words > 0, "nonzero masks need rows of a multiple of 4 voxels");
     const unsigned int nb = (unsigned int)(nseg * nyt * f->dims[2]);
-    hipLaunchKernelGGL(k_adj_mask, dim3(nb), dim3(BLOCK), 0, f->ctx->stream, u, grid_of(f), nseg, nyt, mask);
+    hipLaunchKernelGGL(k_adj_mask, dim3(nb), dim3(BLOCK), 0, f->ctx->stream, u, grid_of(f), nseg, nyt, mask + words);
+    hipLaunchKernelGGL(k_adj_dilate, dim3((unsigned int)((words + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
+                       f->ctx->stream, mask + words, grid_of(f), nseg * MW, mask);
     KF_LAUNCH_CHECK();
     return KF_OK;
 }

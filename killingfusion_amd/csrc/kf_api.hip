@@ -87,13 +87,14 @@ bool invert4(const double m[16], double out[16])
     return true;
 }
 
-// Allocates the field's two nonzero-mask buffers on first use.
+// Allocates the field's two mask buffers on first use: each holds the stencil mask and the
+// nonzero mask it is dilated from (kf_launch_adj_mask).
 kf_status ensure_masks(kf_field *f)
 {
     const size_t words = kf_adj_mask_words(f->dims);
     for (int b = 0; b < 2; b++)
         if (!f->masks[b])
-            KF_HIP(hipMalloc((void **)&f->masks[b], sizeof(unsigned int) * words));
+            KF_HIP(hipMalloc((void **)&f->masks[b], 2 * sizeof(unsigned int) * words));
     return KF_OK;
 }
 

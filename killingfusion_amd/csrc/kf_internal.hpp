@@ -152,7 +152,7 @@ struct kf_field {
     int64_t n = 0;
     float *buf[2] = {nullptr, nullptr};  // SoA [3][n] each, one allocation (buf[1] = buf[0] + 3n); buf[cur] is the value
     int cur = 0;
-    unsigned int *masks[2] = {nullptr, nullptr};  // nonzero masks of buf[0] / buf[1] (adjoint kernels)
+    unsigned int *masks[2] = {nullptr, nullptr};  // stencil + nonzero masks of buf[0] / buf[1] (adjoint kernels)
     float *u() const { return buf[cur]; }
     float *scratch() const { return buf[cur ^ 1]; }
 };
@@ -395,8 +395,8 @@ struct DeformArgs;
 // The reference step folds its own partials right after the launch.
 kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const float *u_in,
                              float *u_out, const kf_params *p, int mode, int iter = 0);
-// mask_in: nonzero mask of u_in, needed by energy and gradient passes of the row kernel (see
-// k_deform_adj.hip); without it those passes run one voxel per thread.
+// mask_in: stencil mask of u_in (kf_launch_adj_mask), needed by energy and gradient passes of
+// the row kernel (see k_deform_adj.hip); without it those passes run one voxel per thread.
 // An adjoint step launch k > 0 folds launch k - 1 in its first block, so a deform loop ends
 // with kf_launch_finalize(ctx, *nb_out, last & 1).  Energy passes fold themselves.
 kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const float *u_in,
@@ -440,7 +440,9 @@ kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *
                          kf_deform_stats *stats, bool u_zero);
 // Words of a nonzero mask for these dims (0: the row kernel does not apply).
 size_t kf_adj_mask_words(const int32_t dims[3]);
-// Computes the nonzero mask of field values u.
+// Computes the stencil mask of field values u into mask[0, words): every bit whose group's
+// adjoint stencil reads a nonzero u.  mask[words, 2 words) holds the nonzero mask it is
+// dilated from.
 kf_status kf_launch_adj_mask(const kf_field *f, const float *u, unsigned int *mask);
 
 kf_status kf_mc_run(const kf_volume *v, double iso, uint64_t *n_tris, float *verts_host, uint64_t cap);
