@@ -47,12 +47,12 @@ __constant__ int c_nb_off[NNB][3] = {{1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1,
 // READ_U false: every u is +0 (the caller has just cleared the field).  *nonfinite is set when
 // some live SDF value is not finite.
 template <class S, bool READ_U>
-__global__ __launch_bounds__(BLOCK) void k_active_mark(PhiRead<S> L, const float *__restrict__ u, float lo, float hi,
-                                                       unsigned long long *__restrict__ mask,
-                                                       unsigned int *__restrict__ span_count, unsigned int *nonfinite)
+__device__ inline void mark_span(const PhiRead<S> &L, const float *__restrict__ u, float lo, float hi,
+                                 unsigned long long *__restrict__ mask, unsigned int *__restrict__ span_count,
+                                 unsigned int *nonfinite, unsigned int sb)
 {
     const unsigned int n32 = nvox32(L.g);
-    const unsigned int base = blockIdx.x * (unsigned int)SPAN;
+    const unsigned int base = sb * (unsigned int)SPAN;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     unsigned int cnt = 0;
 #pragma unroll 4
@@ -84,14 +84,21 @@ __global__ __launch_bounds__(BLOCK) void k_active_mark(PhiRead<S> L, const float
         unsigned int t = 0;
         for (int k = 0; k < BLOCK / 64; k++)
             t += s_cnt[k];
-        span_count[blockIdx.x] = t;
+        span_count[sb] = t;
     }
 }
 
+template <class S, bool READ_U>
+__global__ __launch_bounds__(BLOCK) void k_active_mark(PhiRead<S> L, const float *__restrict__ u, float lo, float hi,
+                                                       unsigned long long *__restrict__ mask,
+                                                       unsigned int *__restrict__ span_count, unsigned int *nonfinite)
+{
+    mark_span<S, READ_U>(L, u, lo, hi, mask, span_count, nonfinite, blockIdx.x);
+}
+
 // Exclusive scan of the span counts (one block); the total goes to *total.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_active_scan(const unsigned int *__restrict__ span_count,
-                                                            unsigned int nspan, unsigned int *__restrict__ span_off,
-                                                            unsigned int *total)
+__device__ inline void scan_spans(const unsigned int *__restrict__ span_count, unsigned int nspan,
+                                  unsigned int *__restrict__ span_off, unsigned int *total)
 {
     const unsigned int per = (nspan + SCAN_BLOCK - 1) / SCAN_BLOCK;
     const unsigned int b0 = threadIdx.x * per;
@@ -128,6 +135,13 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_active_scan(const unsigned int *
         }
     if (threadIdx.x == SCAN_BLOCK - 1)
         *total = run;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_active_scan(const unsigned int *__restrict__ span_count,
+                                                            unsigned int nspan, unsigned int *__restrict__ span_off,
+                                                            unsigned int *total)
+{
+    scan_spans(span_count, nspan, span_off, total);
 }
 
 // The voxels a warp (SDF::update) can change: u_j != 0 (any component; -0 counts as 0, it
@@ -181,15 +195,15 @@ __global__ void k_clear_words(unsigned int *p, int n)
         p[threadIdx.x] = 0u;
 }
 
-// Thread k of block b lists the voxels of mask word b WPB + k, in voxel order.
-__global__ __launch_bounds__(WPB) void k_active_compact(const unsigned long long *__restrict__ mask, unsigned int nwords,
-                                                        const unsigned int *__restrict__ span_count,
-                                                        const unsigned int *__restrict__ span_off, unsigned int base,
-                                                        unsigned int *__restrict__ list, unsigned int *__restrict__ pos_of)
+// Thread k of span block sb lists the voxels of mask word sb WPB + k, in voxel order.
+__device__ inline void compact_span(const unsigned long long *__restrict__ mask, unsigned int nwords,
+                                    const unsigned int *__restrict__ span_count,
+                                    const unsigned int *__restrict__ span_off, unsigned int base,
+                                    unsigned int *__restrict__ list, unsigned int *__restrict__ pos_of, unsigned int sb)
 {
-    if (span_count[blockIdx.x] == 0u)
+    if (span_count[sb] == 0u)
         return;
-    const unsigned int wd = blockIdx.x * (unsigned int)WPB + threadIdx.x;
+    const unsigned int wd = sb * (unsigned int)WPB + threadIdx.x;
     unsigned long long m = wd < nwords ? mask[wd] : 0ull;
     const unsigned int c = (unsigned int)__popcll(m);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -207,7 +221,7 @@ __global__ __launch_bounds__(WPB) void k_active_compact(const unsigned long long
     unsigned int wbase = 0;
     for (int k = 0; k < w; k++)
         wbase += s_w[k];
-    unsigned int pos = base + span_off[blockIdx.x] + wbase + v - c;
+    unsigned int pos = base + span_off[sb] + wbase + v - c;
     while (m) {
         const unsigned int i = wd * 64u + (unsigned int)__builtin_ctzll(m);
         m &= m - 1ull;
@@ -217,17 +231,25 @@ __global__ __launch_bounds__(WPB) void k_active_compact(const unsigned long long
     }
 }
 
+__global__ __launch_bounds__(WPB) void k_active_compact(const unsigned long long *__restrict__ mask, unsigned int nwords,
+                                                        const unsigned int *__restrict__ span_count,
+                                                        const unsigned int *__restrict__ span_off, unsigned int base,
+                                                        unsigned int *__restrict__ list, unsigned int *__restrict__ pos_of)
+{
+    compact_span(mask, nwords, span_count, span_off, base, list, pos_of, blockIdx.x);
+}
+
 // Slot t (base .. base + count - 1): u of the listed voxel (compact set 0) and the list slots
 // of its 12 neighbours.
-__global__ __launch_bounds__(BLOCK) void k_active_neighbours(GridDims g, const unsigned long long *__restrict__ mask,
-                                                             const unsigned int *__restrict__ pos_of,
-                                                             const unsigned int *__restrict__ list, unsigned int base,
-                                                             unsigned int count, const float *__restrict__ u,
-                                                             float4 *__restrict__ uc, int4 *__restrict__ nbr)
+// (Thread of local block lb.)
+__device__ inline void list_neighbours(const GridDims &g, const unsigned long long *__restrict__ mask,
+                                       const unsigned int *__restrict__ pos_of, const unsigned int *__restrict__ list,
+                                       unsigned int base, unsigned int count, const float *__restrict__ u,
+                                       float4 *__restrict__ uc, int4 *__restrict__ nbr, unsigned int lb)
 {
-    if (blockIdx.x * (unsigned int)BLOCK + threadIdx.x >= count)
+    if (lb * (unsigned int)BLOCK + threadIdx.x >= count)
         return;
-    const unsigned int t = base + blockIdx.x * (unsigned int)BLOCK + threadIdx.x;
+    const unsigned int t = base + lb * (unsigned int)BLOCK + threadIdx.x;
     const unsigned int i = list[t];
     const unsigned int n32 = nvox32(g);
     uc[t] = make_float4(u[i], u[n32 + i], u[2u * n32 + i], 0.0f);
@@ -248,6 +270,66 @@ __global__ __launch_bounds__(BLOCK) void k_active_neighbours(GridDims g, const u
 #pragma unroll
     for (int q = 0; q < NNB / 4; q++)
         nbr[3u * t + (unsigned int)q] = make_int4(slot[4 * q], slot[4 * q + 1], slot[4 * q + 2], slot[4 * q + 3]);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_active_neighbours(GridDims g, const unsigned long long *__restrict__ mask,
+                                                             const unsigned int *__restrict__ pos_of,
+                                                             const unsigned int *__restrict__ list, unsigned int base,
+                                                             unsigned int count, const float *__restrict__ u,
+                                                             float4 *__restrict__ uc, int4 *__restrict__ nbr)
+{
+    list_neighbours(g, mask, pos_of, list, base, count, u, uc, nbr, blockIdx.x);
+}
+
+// Listing of a batch of volumes, four launches for all of them.  Volume v's spans are blocks
+// sb0 .. sb0 + nspan - 1 of the marking and compaction launches and its listed voxels blocks
+// nb0 .. of the neighbours launch (span_vol / nb_vol map a block to its volume).  Each volume
+// has its own mask words, span counts and offsets, and pos_of entries.
+template <class S>
+struct ListVol {
+    PhiRead<S> L;
+    const float *u;
+    unsigned long long *mask;
+    unsigned int *span_count;  // nspan counts, then nspan offsets
+    unsigned int *pos_of;
+    unsigned int *nonfinite, *listed;
+    unsigned int nspan, sb0, base, cnt, nb0;
+};
+
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_active_mark_batch(const ListVol<S> *__restrict__ vols,
+                                                             const unsigned int *__restrict__ span_vol, float lo,
+                                                             float hi)
+{
+    const ListVol<S> &V = vols[span_vol[blockIdx.x]];
+    mark_span<S, true>(V.L, V.u, lo, hi, V.mask, V.span_count, V.nonfinite, blockIdx.x - V.sb0);
+}
+
+template <class S>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_active_scan_batch(const ListVol<S> *__restrict__ vols)
+{
+    const ListVol<S> &V = vols[blockIdx.x];
+    scan_spans(V.span_count, V.nspan, V.span_count + V.nspan, V.listed);
+}
+
+template <class S>
+__global__ __launch_bounds__(WPB) void k_active_compact_batch(const ListVol<S> *__restrict__ vols,
+                                                              const unsigned int *__restrict__ span_vol,
+                                                              unsigned int *__restrict__ list)
+{
+    const ListVol<S> &V = vols[span_vol[blockIdx.x]];
+    compact_span(V.mask, (unsigned int)((V.L.g.n + 63) / 64), V.span_count, V.span_count + V.nspan, V.base, list,
+                 V.pos_of, blockIdx.x - V.sb0);
+}
+
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_active_neighbours_batch(const ListVol<S> *__restrict__ vols,
+                                                                   const unsigned int *__restrict__ nb_vol,
+                                                                   const unsigned int *__restrict__ list,
+                                                                   float4 *__restrict__ uc, int4 *__restrict__ nbr)
+{
+    const ListVol<S> &V = vols[nb_vol[blockIdx.x]];
+    list_neighbours(V.L.g, V.mask, V.pos_of, list, V.base, V.cnt, V.u, uc, nbr, blockIdx.x - V.nb0);
 }
 
 // Fold of step launch k - 1 by block 0 of launch k: kf_fold_launch's result, with every load
@@ -707,6 +789,9 @@ void kf_active_release(kf_context *ctx)
     (void)hipFree(a.benergy);
     (void)hipFree(a.blk_vol);
     (void)hipFree(a.vtab);
+    (void)hipFree(a.ltab);
+    (void)hipFree(a.span_vol);
+    (void)hipFree(a.nb_vol);
     if (a.bstates_host)
         (void)hipHostFree(a.bstates_host);
     (void)hipFree(a.aux);
@@ -853,22 +938,30 @@ kf_status batch_active(kf_context *ctx, const kf_volume *const *live, const kf_v
     hipStream_t st = ctx->stream;
     *used = false;
     a.warp_vol = nullptr;
-    std::vector<size_t> moff(count), soff(count);
+    std::vector<size_t> moff(count), soff(count), poff(count);
     std::vector<unsigned int> nspan(count);
     size_t mtot = 0, stot = 0;
-    int64_t nmax = 0, nvox = 0;
+    int64_t nvox = 0;
+    unsigned int sbt = 0;  // span blocks of all volumes
     for (int v = 0; v < count; v++) {
         nspan[v] = (unsigned int)((live[v]->n + SPAN - 1) / SPAN);
         moff[v] = mtot;
         mtot += (size_t)nspan[v] * WPB;
         soff[v] = stot;
         stot += 2 * (size_t)nspan[v];
-        nmax = live[v]->n > nmax ? live[v]->n : nmax;
+        poff[v] = (size_t)nvox;
         nvox += live[v]->n;
+        sbt += nspan[v];
     }
     kf_status s = grow(a.mask, a.mask_cap, mtot, st);
     if (s == KF_OK)
         s = grow(a.span_count, a.span_cap, stot, st);
+    if (s == KF_OK)
+        s = grow(a.pos_of, a.pos_cap, (size_t)nvox, st);
+    if (s == KF_OK)
+        s = grow(a.ltab, a.ltab_cap, sizeof(ListVol<S>) * count, st);
+    if (s == KF_OK)
+        s = grow(a.span_vol, a.span_vol_cap, (size_t)sbt, st);
     if (s == KF_OK && (size_t)count > a.bcap) {
         if (a.bstates_host) {
             KF_HIP(hipStreamSynchronize(st));
@@ -893,14 +986,34 @@ kf_status batch_active(kf_context *ctx, const kf_volume *const *live, const kf_v
     const AdjConsts c = kf_adj_consts(p);
     hipLaunchKernelGGL(k_states_reset, dim3((count + 63) / 64), dim3(64), 0, st, a.bstates, count, p->stop_max_update);
     KF_LAUNCH_CHECK();
+    // Mark and count every volume's candidates: one launch over all spans, one scan block per
+    // volume.
+    std::vector<ListVol<S>> lt(count);
+    std::vector<unsigned int> sv(sbt);
+    unsigned int sb0 = 0;
     for (int v = 0; v < count; v++) {
-        hipLaunchKernelGGL((k_active_mark<S, true>), dim3(nspan[v]), dim3(BLOCK), 0, st, make_read<S>(live[v]),
-                           u[v]->u(), c.lo, c.hi, a.mask + moff[v], a.span_count + soff[v], &a.bstates[v].nonfinite);
-        KF_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_active_scan, dim3(1), dim3(SCAN_BLOCK), 0, st, a.span_count + soff[v], nspan[v],
-                           a.span_count + soff[v] + nspan[v], &a.bstates[v].listed);
-        KF_LAUNCH_CHECK();
+        ListVol<S> &L = lt[v];
+        L.L = make_read<S>(live[v]);
+        L.u = u[v]->u();
+        L.mask = a.mask + moff[v];
+        L.span_count = a.span_count + soff[v];
+        L.pos_of = a.pos_of + poff[v];
+        L.nonfinite = &a.bstates[v].nonfinite;
+        L.listed = &a.bstates[v].listed;
+        L.nspan = nspan[v];
+        L.sb0 = sb0;
+        L.base = L.cnt = L.nb0 = 0u;
+        for (unsigned int b = 0; b < nspan[v]; b++)
+            sv[sb0 + b] = (unsigned int)v;
+        sb0 += nspan[v];
     }
+    const ListVol<S> *dlt = reinterpret_cast<const ListVol<S> *>(a.ltab);
+    KF_HIP(hipMemcpyAsync(a.ltab, lt.data(), sizeof(ListVol<S>) * count, hipMemcpyHostToDevice, st));
+    KF_HIP(hipMemcpyAsync(a.span_vol, sv.data(), sizeof(unsigned int) * sbt, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL((k_active_mark_batch<S>), dim3(sbt), dim3(BLOCK), 0, st, dlt, a.span_vol, c.lo, c.hi);
+    KF_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_active_scan_batch<S>), dim3((unsigned int)count), dim3(SCAN_BLOCK), 0, st, dlt);
+    KF_LAUNCH_CHECK();
     KF_HIP(hipMemcpyAsync(sh, a.bstates, sizeof(DeformState) * count, hipMemcpyDeviceToHost, st));
     KF_HIP(hipStreamSynchronize(st));
     std::vector<unsigned int> cnt(count), blk0(count), nblk(count);
@@ -927,8 +1040,6 @@ kf_status batch_active(kf_context *ctx, const kf_volume *const *live, const kf_v
         a.cap = s == KF_OK ? slots : 0;
     }
     if (s == KF_OK)
-        s = grow(a.pos_of, a.pos_cap, (size_t)nmax, st);
-    if (s == KF_OK)
         s = grow(a.blk_vol, a.blk_cap, (size_t)nwb, st);
     if (s == KF_OK)
         s = grow(a.vtab, a.vtab_cap, sizeof(ActiveVol<S>) * count, st);
@@ -951,18 +1062,28 @@ kf_status batch_active(kf_context *ctx, const kf_volume *const *live, const kf_v
     KF_HIP(hipMemcpyAsync(a.vtab, tab.data(), sizeof(ActiveVol<S>) * count, hipMemcpyHostToDevice, st));
     KF_HIP(hipMemcpyAsync(a.blk_vol, bv.data(), sizeof(unsigned int) * nwb, hipMemcpyHostToDevice, st));
     KF_HIP(hipMemsetAsync(a.list, 0xff, sizeof(unsigned int) * slots, st));
+    // List them: one launch over all spans, one over all listed voxels.
+    unsigned int nbt = 0;
     for (int v = 0; v < count; v++) {
-        const unsigned int nwords = (unsigned int)((live[v]->n + 63) / 64);
-        hipLaunchKernelGGL(k_active_compact, dim3(nspan[v]), dim3(WPB), 0, st, a.mask + moff[v], nwords,
-                           a.span_count + soff[v], a.span_count + soff[v] + nspan[v], blk0[v] * (unsigned int)LB,
-                           a.list, a.pos_of);
+        lt[v].base = blk0[v] * (unsigned int)LB;
+        lt[v].cnt = cnt[v];
+        lt[v].nb0 = nbt;
+        nbt += (cnt[v] + BLOCK - 1) / BLOCK;
+    }
+    std::vector<unsigned int> nv(nbt);
+    for (int v = 0; v < count; v++)
+        for (unsigned int b = 0; b < (cnt[v] + BLOCK - 1) / BLOCK; b++)
+            nv[lt[v].nb0 + b] = (unsigned int)v;
+    if ((s = grow(a.nb_vol, a.nb_vol_cap, (size_t)(nbt ? nbt : 1u), st)) != KF_OK)
+        return s;
+    KF_HIP(hipMemcpyAsync(a.ltab, lt.data(), sizeof(ListVol<S>) * count, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL((k_active_compact_batch<S>), dim3(sbt), dim3(WPB), 0, st, dlt, a.span_vol, a.list);
+    KF_LAUNCH_CHECK();
+    if (nbt > 0) {
+        KF_HIP(hipMemcpyAsync(a.nb_vol, nv.data(), sizeof(unsigned int) * nbt, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL((k_active_neighbours_batch<S>), dim3(nbt), dim3(BLOCK), 0, st, dlt, a.nb_vol, a.list, a.u,
+                           a.nbr);
         KF_LAUNCH_CHECK();
-        if (cnt[v] > 0) {
-            hipLaunchKernelGGL(k_active_neighbours, dim3((cnt[v] + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st,
-                               grid_of(live[v]), a.mask + moff[v], a.pos_of, a.list, blk0[v] * (unsigned int)LB, cnt[v],
-                               u[v]->u(), a.u, a.nbr);
-            KF_LAUNCH_CHECK();
-        }
     }
     // The iterations: launch k reads compact set k & 1 and writes the other one; per-volume
     // gates stop each volume on its own (stop rule or divergence).

@@ -94,6 +94,13 @@ struct kf_active_ws {
     size_t blk_cap = 0;
     unsigned char *vtab = nullptr;
     size_t vtab_cap = 0;                  // bytes
+    // batched listing: volume table, and the volume of each block of its launches
+    unsigned char *ltab = nullptr;
+    size_t ltab_cap = 0;                  // bytes
+    unsigned int *span_vol = nullptr;
+    size_t span_vol_cap = 0;
+    unsigned int *nb_vol = nullptr;
+    size_t nb_vol_cap = 0;
     // listing of the voxels a warp changes: counters (device / pinned host) and the new
     // phi, weight of each listed voxel
     unsigned int *aux = nullptr;
