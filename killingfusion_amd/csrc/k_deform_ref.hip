@@ -600,6 +600,7 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
     const size_t need = (size_t)segcap * NSEG;
     kf_ref_ws &w = ctx->ref;
     if (w.cap < need) {
+        KF_HIP(hipStreamSynchronize(ctx->stream));  // earlier launches may still read the old list
         kf_ref_release(ctx);
         if (hipMalloc((void **)&w.idx, sizeof(unsigned long long) * need) != hipSuccess ||
             hipMalloc((void **)&w.psi, sizeof(float) * need) != hipSuccess ||
