@@ -267,6 +267,8 @@ __device__ inline void ref_combine(const RefConsts &c, float psi, float dst, con
         out[a] = (dg[a] + kg[a]) + lg[a];
 }
 
+// Threads of a pass-1 block.
+constexpr int SBLOCK = 256;
 // Blocks of the list kernel (at most; fewer for small grids).
 constexpr int LIST_BLOCKS = 2048;
 // The list has one segment per XCD (block b of pass 1 runs on XCD b % 8), each with its own
@@ -332,7 +334,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_finite(PhiView<S> L, unsigned lon
 // phi_j + (finite) * 0 + ...; when every live value is finite (no non-finite flag) that is phi_j
 // itself unless phi_j is -0, so those voxels read phi_j alone.
 template <class S, int MODE, int V>
-__global__ __launch_bounds__(BLOCK) void k_ref_scan(PhiView<S> L, const float *__restrict__ u_in,
+__global__ __launch_bounds__(SBLOCK) void k_ref_scan(PhiView<S> L, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, float hi, float lo, float alpha,
                                                     const DeformState *st, unsigned long long *aux, int parity,
                                                     unsigned long long segcap,
@@ -341,10 +343,10 @@ __global__ __launch_bounds__(BLOCK) void k_ref_scan(PhiView<S> L, const float *_
 {
     if (MODE == KF_MODE_STEP && st->done)
         return;
-    __shared__ unsigned int s_wtot[BLOCK / 64];
+    __shared__ unsigned int s_wtot[SBLOCK / 64];
     __shared__ unsigned long long s_base;
     const GridDims g = L.g;
-    const int64_t i0 = ((int64_t)kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x) * V;
+    const int64_t i0 = ((int64_t)kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x) * SBLOCK + threadIdx.x) * V;
     const bool phi_finite = aux[AUX_NONFINITE] == 0ull;
     bool act[V];
     float psi[V];
@@ -412,7 +414,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_scan(PhiView<S> L, const float *_
     if (threadIdx.x == 0) {
         unsigned int bt = 0;
 #pragma unroll
-        for (int k = 0; k < BLOCK / 64; k++)
+        for (int k = 0; k < SBLOCK / 64; k++)
             bt += s_wtot[k];
         const unsigned int seg = blockIdx.x % NSEG;
         s_base = bt ? seg * segcap + atomicAdd(&aux[parity * NSEG + seg], (unsigned long long)bt) : 0ull;
@@ -592,7 +594,7 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
     const int64_t n = live->n;
     const bool vec = n % 4 == 0;
     const bool nbhd = c.h > 0.0f && c.h < 0.25f;
-    const int64_t per_block = (vec ? 4 : 1) * BLOCK;
+    const int64_t per_block = (vec ? 4 : 1) * SBLOCK;
     const unsigned int nb_a = (unsigned int)((n + per_block - 1) / per_block);
     // Segment x takes the band voxels of the pass-1 blocks b with b % 8 == x: at most their
     // voxels, so every segment has room for ceil(nb_a / 8) blocks.
@@ -632,10 +634,10 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
         }
         if (mode == KF_MODE_GRAD) {
             if (vec)
-                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, 4>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
+                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, 4>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             else
-                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, 1>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
+                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, 1>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             if (nbhd)
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD, true>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
@@ -645,10 +647,10 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
                                    Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
         } else {
             if (vec)
-                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, 4>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
+                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, 4>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             else
-                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, 1>), dim3(nb_a), dim3(BLOCK), 0, ctx->stream, L,
+                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, 1>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             if (nbhd)
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, true>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
