@@ -206,3 +206,41 @@ def test_active_batch_with_an_empty_volume(ctx, batch):
     for b, f in enumerate(fields):
         assert np.array_equal(f.download(), ref[b]), b
         assert st[b]["iters"] == ref_st[b]["iters"] and st[b]["listed"] == ref_st[b]["listed"], b
+
+
+def test_active_batch_of_many_small_volumes(ctx):
+    """Thirty volumes of 6^3 to 35^3 in one active-set batch (most one marking span each, grids
+    whose voxel count is not a multiple of 64, empty ones in between, some starting from a
+    nonzero field, which lists voxels away from the surface too): the listing of all volumes
+    in four launches must give every volume the field and stats of its own kf_deform call."""
+    edges = [6 + (7 * b) % 30 for b in range(30)]
+    print(f"case: batch of 30 ellipsoid volumes, edges {min(edges)}..{max(edges)}, active sweep, 10 iterations",
+          flush=True)
+    vols = []
+    kw = dict(voxel_size=VOX, clip_distance=TRUNC, max_surface=TRUNC, dtype="bf16")
+    for b, (n, mn, d1, d2) in enumerate(scenes.batch_scene(30, seed=0x5EED, edges=edges)):
+        if b % 7 == 3:
+            vols.append((SDF(ctx, (n, n, n), mn, **kw), SDF(ctx, (n, n, n), mn, **kw)))  # nothing integrated
+        else:
+            vols.append(make_pair(ctx, (n, n, n), mn, VOX, TRUNC, d2, d1, "bf16"))
+    inits = []
+    for b, (live, _) in enumerate(vols):
+        u = np.zeros((3, live.n), dtype=np.float32)
+        if b % 5 == 1:
+            n = live.dims[0]
+            u3 = u.reshape(3, n, n, n)
+            u3[:, n // 4:n // 2, 1:n // 3, n // 3:] = scenes.seeded_field((n, n, n), amplitude=0.3, seed=b).reshape(
+                3, n, n, n)[:, n // 4:n // 2, 1:n // 3, n // 3:]
+        inits.append(u)
+    kp = params("active", max_iters=10)
+    ref, ref_st = per_volume(ctx, vols, kp, inits)
+    lives, cans = lists(vols)
+    fields = fresh_fields(ctx, vols, inits)
+    st = batch_deform(lives, cans, fields, kp)
+    listed = [s["listed"] for s in st]
+    print(f"listed per volume: {listed}", flush=True)
+    assert sum(1 for x in listed if x == 0) >= 4 and sum(1 for x in listed if x > 0) >= 20
+    for b, f in enumerate(fields):
+        assert np.array_equal(f.download(), ref[b]), b
+        for k in ("iters", "listed", "active", "max_update", "diverged_index"):
+            assert st[b][k] == ref_st[b][k], (b, k, st[b][k], ref_st[b][k])
