@@ -46,6 +46,17 @@ __constant__ int c_nb_off[NNB][3] = {{1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1,
 
 // READ_U false: every u is +0 (the caller has just cleared the field).  *nonfinite is set when
 // some live SDF value is not finite.
+// Spreads the 16 bits of x to bits 0, 4, .., 60.
+__device__ inline unsigned long long spread4(unsigned long long x)
+{
+    x &= 0xFFFFull;
+    x = (x | (x << 24)) & 0x000000FF000000FFull;
+    x = (x | (x << 12)) & 0x000F000F000F000Full;
+    x = (x | (x << 6)) & 0x0303030303030303ull;
+    x = (x | (x << 3)) & 0x1111111111111111ull;
+    return x;
+}
+
 template <class S, bool READ_U>
 __device__ inline void mark_span(const PhiRead<S> &L, const float *__restrict__ u, float lo, float hi,
                                  unsigned long long *__restrict__ mask, unsigned int *__restrict__ span_count,
@@ -55,26 +66,73 @@ __device__ inline void mark_span(const PhiRead<S> &L, const float *__restrict__ 
     const unsigned int base = sb * (unsigned int)SPAN;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     unsigned int cnt = 0;
-#pragma unroll 4
-    for (int k = 0; k < SPAN / BLOCK; k++) {
-        const unsigned int w0 = base + (unsigned int)(k * BLOCK + w * 64);
-        const unsigned int i = w0 + (unsigned int)lane;
-        bool cand = false, bad = false;
-        if (i < n32) {
-            unsigned int ub = 0u;
-            if (READ_U)
-                ub = __float_as_uint(kf_load_stream(u + i)) | __float_as_uint(kf_load_stream(u + n32 + i)) |
-                     __float_as_uint(kf_load_stream(u + 2u * n32 + i));
-            const float live = S::load(L.p, (int64_t)i, L.scale);
-            bad = !isfinite(live);
-            cand = ub != 0u || !(live > hi || live < lo);
+    if (n32 % 4u == 0u) {
+        // Four voxels per lane (one float4 of each u plane): a wavefront covers 256 voxels, four
+        // mask words.  Ballot v holds voxel 4 lane + v of each lane; word j takes lanes
+        // 16 j .. 16 j + 15, bit 4 m + v from bit 16 j + m of ballot v.
+#pragma unroll 2
+        for (int k = 0; k < SPAN / (BLOCK * 4); k++) {
+            const unsigned int w0 = base + (unsigned int)(k * BLOCK * 4 + w * 256);
+            const unsigned int i = w0 + 4u * (unsigned int)lane;
+            bool cand[4] = {false, false, false, false}, bad = false;
+            if (i < n32) {
+                float live[4];
+                S::load4(L.p, (int64_t)i, L.scale, live);
+                unsigned int ub[4] = {0u, 0u, 0u, 0u};
+                if (READ_U) {
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        const float4 q = kf_load_stream4(u + (unsigned int)a * n32 + i);
+                        ub[0] |= __float_as_uint(q.x);
+                        ub[1] |= __float_as_uint(q.y);
+                        ub[2] |= __float_as_uint(q.z);
+                        ub[3] |= __float_as_uint(q.w);
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    bad = bad || !isfinite(live[v]);
+                    cand[v] = ub[v] != 0u || !(live[v] > hi || live[v] < lo);
+                }
+            }
+            if (__ballot(bad) != 0ull && lane == 0)
+                atomicOr(nonfinite, 1u);
+            unsigned long long bal[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                bal[v] = __ballot(cand[v]);
+                cnt += (unsigned int)__popcll(bal[v]);
+            }
+            if (lane < 4 && w0 + 64u * (unsigned int)lane < n32) {
+                unsigned long long word = 0ull;
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    word |= spread4(bal[v] >> (16 * lane)) << v;
+                mask[(w0 >> 6) + (unsigned int)lane] = word;
+            }
         }
-        if (__ballot(bad) != 0ull && lane == 0)
-            atomicOr(nonfinite, 1u);
-        const unsigned long long bal = __ballot(cand);
-        if (lane == 0 && w0 < n32)
-            mask[w0 >> 6] = bal;
-        cnt += (unsigned int)__popcll(bal);
+    } else {
+#pragma unroll 4
+        for (int k = 0; k < SPAN / BLOCK; k++) {
+            const unsigned int w0 = base + (unsigned int)(k * BLOCK + w * 64);
+            const unsigned int i = w0 + (unsigned int)lane;
+            bool cand = false, bad = false;
+            if (i < n32) {
+                unsigned int ub = 0u;
+                if (READ_U)
+                    ub = __float_as_uint(kf_load_stream(u + i)) | __float_as_uint(kf_load_stream(u + n32 + i)) |
+                         __float_as_uint(kf_load_stream(u + 2u * n32 + i));
+                const float live = S::load(L.p, (int64_t)i, L.scale);
+                bad = !isfinite(live);
+                cand = ub != 0u || !(live > hi || live < lo);
+            }
+            if (__ballot(bad) != 0ull && lane == 0)
+                atomicOr(nonfinite, 1u);
+            const unsigned long long bal = __ballot(cand);
+            if (lane == 0 && w0 < n32)
+                mask[w0 >> 6] = bal;
+            cnt += (unsigned int)__popcll(bal);
+        }
     }
     __shared__ unsigned int s_cnt[BLOCK / 64];
     if (lane == 0)
