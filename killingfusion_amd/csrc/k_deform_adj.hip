@@ -371,6 +371,55 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
     kf_block_publish<FB>(partials, blockIdx.x, esum, mx, act);
 }
 
+// The same step when N is not a multiple of 4 (plane starts are not 16-byte aligned): one
+// voxel per thread, scalar streamed loads, otherwise as k_adj_flat -- every voxel copies u, the
+// voxels that are active or moved run the stencil, compacted over the wavefront.  (A 255^3 grid:
+// 108 us per launch, against 256 us for k_adj_any's full stencil per voxel and 59 us for
+// k_adj_flat at 256^3; the scalar streams are what remains between the two.)
+template <class S>
+__global__ __launch_bounds__(FB) void k_adj_flat1(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
+                                                     float *__restrict__ u_out, AdjConsts c, DeformState *st,
+                                                     BlockPartial *partials, int parity, const BlockPartial *prev,
+                                                     unsigned int prev_nb)
+{
+    if (!step_gate<KF_MODE_STEP, FB>(st, prev, prev_nb, parity))
+        return;
+    const GridDims g = L.g;
+    const unsigned int n32 = nvox32(g);
+    const unsigned int tile = kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x);
+    const unsigned int i = tile * (unsigned int)FB + threadIdx.x;
+    bool need = false;
+    if (i < n32) {
+        float uc[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            uc[a] = kf_load_stream(u_in + ((unsigned int)a * n32 + i));
+            u_out[(unsigned int)a * n32 + i] = uc[a];
+        }
+        const float live = S::load(L.p, (int64_t)i, L.scale);
+        const bool moved =
+            ((__float_as_uint(uc[0]) | __float_as_uint(uc[1]) | __float_as_uint(uc[2])) & 0x7fffffffu) != 0u;
+        need = moved || !(live > c.hi || live < c.lo);
+    }
+    unsigned long long bal[1];
+    bal[0] = __ballot(need);
+    const int total = __popcll(bal[0]);
+    double mx = 0.0, esum = 0.0;
+    unsigned long long act = 0;
+    const unsigned int wave0 = tile * (unsigned int)FB + (threadIdx.x & ~63u);
+    const unsigned int nxy = (unsigned int)g.nx * (unsigned int)g.ny;
+    for (int j = threadIdx.x & 63; j < total; j += 64) {
+        int bit, src;
+        kf_job_decode<1>(bal, j, bit, src);
+        const unsigned int v = wave0 + (unsigned int)src;
+        const unsigned int z = v / nxy, r = v - z * nxy;
+        const unsigned int y = r / (unsigned int)g.nx, x = r - y * (unsigned int)g.nx;
+        adj_job<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, &st->div_slot[parity], nullptr, (int)x, (int)y, (int)z, mx,
+                                 esum, act);
+    }
+    kf_block_publish<FB>(partials, blockIdx.x, esum, mx, act);
+}
+
 // Nonzero mask of field values u (layout above).
 __global__ __launch_bounds__(BLOCK) void k_adj_mask(const float *__restrict__ u, GridDims g, int nseg, int nyt,
                                                     unsigned int *__restrict__ mask)
@@ -394,12 +443,15 @@ __global__ __launch_bounds__(BLOCK) void k_adj_mask(const float *__restrict__ u,
 
 template <class S, int MODE>
 void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float *u_out, const AdjConsts &c,
-            kf_context *ctx, float *e_voxel, bool rows, unsigned int nb, int nseg, int nyt,
+            kf_context *ctx, float *e_voxel, bool rows, bool flat1, unsigned int nb, int nseg, int nyt,
             const unsigned int *mask_in, int parity, const BlockPartial *prev, unsigned int prev_nb)
 {
     BlockPartial *parts = kf_partials(ctx, parity);
     if (MODE == KF_MODE_STEP && rows)
         hipLaunchKernelGGL((k_adj_flat<S>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out, c, ctx->state,
+                           parts, parity, prev, prev_nb);
+    else if (MODE == KF_MODE_STEP && flat1)
+        hipLaunchKernelGGL((k_adj_flat1<S>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out, c, ctx->state,
                            parts, parity, prev, prev_nb);
     else if (rows)
         hipLaunchKernelGGL((k_adj_rows<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
@@ -442,16 +494,19 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     // starts), 3 N < 2^30 (32-bit offsets) and, for energy and gradient passes, the nonzero mask
     // of u_in.
     // A step runs the flat kernel when N is a multiple of 4 and 3 N < 2^30 (float4 plane
-    // accesses, 32-bit offsets); energy and gradient passes run the row kernel when rows are a
-    // multiple of 4 and the nonzero mask of u_in is given.  Everything else runs one voxel per
-    // thread.
-    const bool flat = mode == KF_MODE_STEP && live->n % VEC == 0 && 3 * live->n < ((int64_t)1 << 30);
+    // accesses, 32-bit offsets), its one-voxel-per-thread form for other N < 2^30 / 3; energy and
+    // gradient passes run the row kernel when rows are a multiple of 4 and the nonzero mask of
+    // u_in is given.  Everything else runs k_adj_any (a full stencil per voxel).
+    const bool small = 3 * live->n < ((int64_t)1 << 30);
+    const bool flat = mode == KF_MODE_STEP && live->n % VEC == 0 && small;
+    const bool flat1 = mode == KF_MODE_STEP && !flat && small;
     const bool rows = flat || (mode != KF_MODE_STEP && kf_adj_mask_words(live->dims) > 0 && mask_in != nullptr);
     const int nseg = (live->dims[0] + ROW - 1) / ROW;
     const int nyt = (live->dims[1] + ROWS - 1) / ROWS;
-    const unsigned int nb = flat   ? (unsigned int)((live->n + FLAT - 1) / FLAT)
-                            : rows ? (unsigned int)(nseg * nyt * live->dims[2])
-                                   : (unsigned int)((live->n + BLOCK - 1) / BLOCK);
+    const unsigned int nb = flat    ? (unsigned int)((live->n + FLAT - 1) / FLAT)
+                            : flat1 ? (unsigned int)((live->n + FB - 1) / FB)
+                            : rows  ? (unsigned int)(nseg * nyt * live->dims[2])
+                                    : (unsigned int)((live->n + BLOCK - 1) / BLOCK);
     kf_status s = kf_ensure_partials(ctx, nb);
     if (s != KF_OK)
         return s;
@@ -465,13 +520,14 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     KF_DISPATCH_DTYPE(live->dtype, S, {
         PhiRead<S> L = make_read<S>(live), Cn = make_read<S>(can);
         if (mode == KF_MODE_GRAD)
-            launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in, 0, prev, 0u);
+            launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, false, nb, nseg, nyt, mask_in, 0, prev,
+                                    0u);
         else if (mode == KF_MODE_ENERGY)
-            launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, mask_in, 0, prev,
+            launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, false, nb, nseg, nyt, mask_in, 0, prev,
                                       0u);
         else
-            launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, nb, nseg, nyt, nullptr, parity, prev,
-                                    prev_nb);
+            launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, flat1, nb, nseg, nyt, nullptr, parity,
+                                    prev, prev_nb);
     });
     KF_LAUNCH_CHECK();
     return mode == KF_MODE_ENERGY ? kf_launch_finalize(ctx, nb, 0) : KF_OK;
