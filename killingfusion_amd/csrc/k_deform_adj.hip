@@ -293,7 +293,12 @@ constexpr int FPT = 1;                 // float4 groups per thread
 constexpr int FB = BLOCK;              // threads per block
 constexpr int FLAT = FB * VEC * FPT;   // voxels per block
 
-template <class S>
+// UNAL: N is not a multiple of 4, so the u planes a n (a = 1, 2) start off a 16-byte boundary.
+// Their float4 groups are then read and written as 4-byte-aligned 16-byte accesses (one load
+// or store each on this target), and the one group that runs past N goes element by element.
+// (A 255^3 grid: 73 us per launch, against 59 us at 256^3, 108 us with one voxel per thread and
+// 256 us for k_adj_any's full stencil per voxel.)
+template <class S, bool UNAL>
 __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, AdjConsts c, DeformState *st,
                                                     BlockPartial *partials, int parity, const BlockPartial *prev,
@@ -310,24 +315,43 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
     // backwards (to meet the previous launch's last writes in the last-level cache) no longer
     // pays: it measured 58.9 against 58.1 us at 256^3.
     const unsigned int tile = kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x);
-    // group q of thread t: voxels tile FLAT + q FB VEC + t VEC .. + 3 (n is a multiple of 4, so
-    // a group is either whole or outside)
+    // group q of thread t: voxels tile FLAT + q FB VEC + t VEC .. + 3 (whole or outside unless
+    // UNAL, where the last one may be partial)
     unsigned int i0[FPT];
     bool valid[FPT];
     float4 C[FPT][3];
     float live4[FPT][VEC];
+    unsigned int nin[FPT];  // voxels of the group inside the grid
 #pragma unroll
     for (int q = 0; q < FPT; q++) {
         i0[q] = tile * (unsigned int)FLAT + (unsigned int)q * (FB * VEC) + threadIdx.x * (unsigned int)VEC;
         valid[q] = i0[q] < n32;
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-            C[q][a] = valid[q] ? kf_load_stream4(u_in + ((unsigned int)a * n32 + i0[q])) : make_float4(0.f, 0.f, 0.f, 0.f);
+        nin[q] = !valid[q] ? 0u : (UNAL && n32 - i0[q] < (unsigned int)VEC ? n32 - i0[q] : (unsigned int)VEC);
 #pragma unroll
         for (int k = 0; k < VEC; k++)
             live4[q][k] = 0.f;
-        if (valid[q])
-            S::load4(L.p, (int64_t)i0[q], L.scale, live4[q]);
+        if (!UNAL || nin[q] == (unsigned int)VEC) {
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                C[q][a] = !valid[q] ? make_float4(0.f, 0.f, 0.f, 0.f)
+                          : UNAL    ? kf_load_stream4u(u_in + ((unsigned int)a * n32 + i0[q]))
+                                    : kf_load_stream4(u_in + ((unsigned int)a * n32 + i0[q]));
+            if (valid[q])
+                S::load4(L.p, (int64_t)i0[q], L.scale, live4[q]);
+        } else {
+            float t[3][VEC];
+#pragma unroll
+            for (int k = 0; k < VEC; k++) {
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+                    t[a][k] = (unsigned int)k < nin[q] ? u_in[(unsigned int)a * n32 + i0[q] + (unsigned int)k] : 0.f;
+                if ((unsigned int)k < nin[q])
+                    live4[q][k] = S::load(L.p, (int64_t)(i0[q] + (unsigned int)k), L.scale);
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                C[q][a] = make_float4(t[a][0], t[a][1], t[a][2], t[a][3]);
+        }
     }
     unsigned int need = 0u;
 #pragma unroll
@@ -335,10 +359,23 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
         if (!valid[q])
             continue;
 #pragma unroll
-        for (int a = 0; a < 3; a++)
-            *reinterpret_cast<float4 *>(u_out + ((unsigned int)a * n32 + i0[q])) = C[q][a];
+        for (int a = 0; a < 3; a++) {
+            float *dst = u_out + ((unsigned int)a * n32 + i0[q]);
+            if (!UNAL) {
+                *reinterpret_cast<float4 *>(dst) = C[q][a];
+            } else if (nin[q] == (unsigned int)VEC) {
+                *reinterpret_cast<kf_f4u *>(dst) = kf_f4u{C[q][a].x, C[q][a].y, C[q][a].z, C[q][a].w};
+            } else {
+#pragma unroll
+                for (int k = 0; k < VEC; k++)
+                    if ((unsigned int)k < nin[q])
+                        dst[k] = f4(C[q][a], k);
+            }
+        }
 #pragma unroll
         for (int k = 0; k < VEC; k++) {
+            if (UNAL && (unsigned int)k >= nin[q])
+                continue;
             const bool moved = ((__float_as_uint(f4(C[q][0], k)) | __float_as_uint(f4(C[q][1], k)) |
                                  __float_as_uint(f4(C[q][2], k))) & 0x7fffffffu) != 0u;
             const bool in_band = !(live4[q][k] > c.hi || live4[q][k] < c.lo);
@@ -364,55 +401,6 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
         const unsigned int i =
             wave0 + (unsigned int)(bit / VEC) * (FB * VEC) + (unsigned int)src * VEC + (unsigned int)(bit % VEC);
         const unsigned int z = i / nxy, r = i - z * nxy;
-        const unsigned int y = r / (unsigned int)g.nx, x = r - y * (unsigned int)g.nx;
-        adj_job<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, &st->div_slot[parity], nullptr, (int)x, (int)y, (int)z, mx,
-                                 esum, act);
-    }
-    kf_block_publish<FB>(partials, blockIdx.x, esum, mx, act);
-}
-
-// The same step when N is not a multiple of 4 (plane starts are not 16-byte aligned): one
-// voxel per thread, scalar streamed loads, otherwise as k_adj_flat -- every voxel copies u, the
-// voxels that are active or moved run the stencil, compacted over the wavefront.  (A 255^3 grid:
-// 108 us per launch, against 256 us for k_adj_any's full stencil per voxel and 59 us for
-// k_adj_flat at 256^3; the scalar streams are what remains between the two.)
-template <class S>
-__global__ __launch_bounds__(FB) void k_adj_flat1(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
-                                                     float *__restrict__ u_out, AdjConsts c, DeformState *st,
-                                                     BlockPartial *partials, int parity, const BlockPartial *prev,
-                                                     unsigned int prev_nb)
-{
-    if (!step_gate<KF_MODE_STEP, FB>(st, prev, prev_nb, parity))
-        return;
-    const GridDims g = L.g;
-    const unsigned int n32 = nvox32(g);
-    const unsigned int tile = kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x);
-    const unsigned int i = tile * (unsigned int)FB + threadIdx.x;
-    bool need = false;
-    if (i < n32) {
-        float uc[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            uc[a] = kf_load_stream(u_in + ((unsigned int)a * n32 + i));
-            u_out[(unsigned int)a * n32 + i] = uc[a];
-        }
-        const float live = S::load(L.p, (int64_t)i, L.scale);
-        const bool moved =
-            ((__float_as_uint(uc[0]) | __float_as_uint(uc[1]) | __float_as_uint(uc[2])) & 0x7fffffffu) != 0u;
-        need = moved || !(live > c.hi || live < c.lo);
-    }
-    unsigned long long bal[1];
-    bal[0] = __ballot(need);
-    const int total = __popcll(bal[0]);
-    double mx = 0.0, esum = 0.0;
-    unsigned long long act = 0;
-    const unsigned int wave0 = tile * (unsigned int)FB + (threadIdx.x & ~63u);
-    const unsigned int nxy = (unsigned int)g.nx * (unsigned int)g.ny;
-    for (int j = threadIdx.x & 63; j < total; j += 64) {
-        int bit, src;
-        kf_job_decode<1>(bal, j, bit, src);
-        const unsigned int v = wave0 + (unsigned int)src;
-        const unsigned int z = v / nxy, r = v - z * nxy;
         const unsigned int y = r / (unsigned int)g.nx, x = r - y * (unsigned int)g.nx;
         adj_job<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, &st->div_slot[parity], nullptr, (int)x, (int)y, (int)z, mx,
                                  esum, act);
@@ -448,11 +436,11 @@ void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float 
 {
     BlockPartial *parts = kf_partials(ctx, parity);
     if (MODE == KF_MODE_STEP && rows)
-        hipLaunchKernelGGL((k_adj_flat<S>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out, c, ctx->state,
-                           parts, parity, prev, prev_nb);
+        hipLaunchKernelGGL((k_adj_flat<S, false>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out, c,
+                           ctx->state, parts, parity, prev, prev_nb);
     else if (MODE == KF_MODE_STEP && flat1)
-        hipLaunchKernelGGL((k_adj_flat1<S>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out, c, ctx->state,
-                           parts, parity, prev, prev_nb);
+        hipLaunchKernelGGL((k_adj_flat<S, true>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out, c,
+                           ctx->state, parts, parity, prev, prev_nb);
     else if (rows)
         hipLaunchKernelGGL((k_adj_rows<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
                            ctx->state, parts, e_voxel, nseg, nyt, mask_in, parity, prev, prev_nb);
@@ -494,7 +482,7 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     // starts), 3 N < 2^30 (32-bit offsets) and, for energy and gradient passes, the nonzero mask
     // of u_in.
     // A step runs the flat kernel when N is a multiple of 4 and 3 N < 2^30 (float4 plane
-    // accesses, 32-bit offsets), its one-voxel-per-thread form for other N < 2^30 / 3; energy and
+    // accesses, 32-bit offsets), its unaligned form for other N < 2^30 / 3; energy and
     // gradient passes run the row kernel when rows are a multiple of 4 and the nonzero mask of
     // u_in is given.  Everything else runs k_adj_any (a full stencil per voxel).
     const bool small = 3 * live->n < ((int64_t)1 << 30);
@@ -504,7 +492,7 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     const int nseg = (live->dims[0] + ROW - 1) / ROW;
     const int nyt = (live->dims[1] + ROWS - 1) / ROWS;
     const unsigned int nb = flat    ? (unsigned int)((live->n + FLAT - 1) / FLAT)
-                            : flat1 ? (unsigned int)((live->n + FB - 1) / FB)
+                            : flat1 ? (unsigned int)((live->n + FLAT - 1) / FLAT)
                             : rows  ? (unsigned int)(nseg * nyt * live->dims[2])
                                     : (unsigned int)((live->n + BLOCK - 1) / BLOCK);
     kf_status s = kf_ensure_partials(ctx, nb);

@@ -21,6 +21,13 @@ __device__ inline float4 kf_load_stream4(const float *p)
     return make_float4(v.x, v.y, v.z, v.w);
 }
 __device__ inline float kf_load_stream(const float *p) { return __builtin_nontemporal_load(p); }
+// The same for a group that starts on a 4-byte boundary only (one 16-byte load on gfx950).
+typedef float kf_f4u __attribute__((ext_vector_type(4), aligned(4)));
+__device__ inline float4 kf_load_stream4u(const float *p)
+{
+    const kf_f4u v = __builtin_nontemporal_load(reinterpret_cast<const kf_f4u *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 
 struct AdjConsts {
     float inv_2v, inv_v, hi, lo, eps2, alpha, omega_k, gamma, omega_s;
