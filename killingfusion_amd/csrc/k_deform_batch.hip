@@ -30,7 +30,7 @@ struct BatchVol {
     float *u[2];          // launch k reads u[k & 1] and writes u[(k + 1) & 1]
     unsigned int tile0;   // first tile of the volume in the batch's tile space
     unsigned int ntiles;
-    int vec4;             // N is a multiple of 4: tiles stream as float4 groups
+    int vec4;             // N is a multiple of 4: tiles stream as aligned float4 groups
     int pad;
 };
 
@@ -96,10 +96,12 @@ __device__ inline void batch_tile(const BatchVol<S> &V, unsigned int t, const Ad
     }
 }
 
-// The same tile for a volume whose N is a multiple of 4, streamed as float4 groups like
-// k_adj_flat: thread t owns groups base + 1024 q + 4 t (q < G), so a tile still holds BLOCK 4 G
-// consecutive voxels.
-template <class S, int G>
+// The same tile streamed as float4 groups like k_adj_flat: thread t owns groups
+// base + 1024 q + 4 t (q < G), so a tile still holds BLOCK 4 G consecutive voxels.  UNAL: N is
+// not a multiple of 4, so the u planes a n (a = 1, 2) start off a 16-byte boundary; their groups
+// are read and written as 4-byte-aligned 16-byte accesses, and the one group that runs past N
+// goes element by element.
+template <class S, int G, bool UNAL>
 __device__ inline void batch_tile_v4(const BatchVol<S> &V, unsigned int t, const AdjConsts &c, long long *div,
                                      int parity, double &mx, unsigned long long &act)
 {
@@ -113,18 +115,37 @@ __device__ inline void batch_tile_v4(const BatchVol<S> &V, unsigned int t, const
     bool ok[G];
     float4 C[G][3];
     float live4[G][VEC];
+    unsigned int nin[G];  // voxels of the group inside the grid
 #pragma unroll
     for (int q = 0; q < G; q++) {
         i0[q] = base + (unsigned int)q * (BLOCK * VEC) + threadIdx.x * VEC;
         ok[q] = i0[q] < n32;
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-            C[q][a] = ok[q] ? kf_load_stream4(u_in + ((unsigned int)a * n32 + i0[q])) : make_float4(0.f, 0.f, 0.f, 0.f);
+        nin[q] = !ok[q] ? 0u : (UNAL && n32 - i0[q] < (unsigned int)VEC ? n32 - i0[q] : (unsigned int)VEC);
 #pragma unroll
         for (int k = 0; k < VEC; k++)
             live4[q][k] = 0.f;
-        if (ok[q])
-            S::load4(V.L.p, (int64_t)i0[q], V.L.scale, live4[q]);
+        if (!UNAL || nin[q] == (unsigned int)VEC) {
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                C[q][a] = !ok[q] ? make_float4(0.f, 0.f, 0.f, 0.f)
+                          : UNAL ? kf_load_stream4u(u_in + ((unsigned int)a * n32 + i0[q]))
+                                 : kf_load_stream4(u_in + ((unsigned int)a * n32 + i0[q]));
+            if (ok[q])
+                S::load4(V.L.p, (int64_t)i0[q], V.L.scale, live4[q]);
+        } else {
+            float tv[3][VEC];
+#pragma unroll
+            for (int k = 0; k < VEC; k++) {
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+                    tv[a][k] = (unsigned int)k < nin[q] ? u_in[(unsigned int)a * n32 + i0[q] + (unsigned int)k] : 0.f;
+                if ((unsigned int)k < nin[q])
+                    live4[q][k] = S::load(V.L.p, (int64_t)(i0[q] + (unsigned int)k), V.L.scale);
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                C[q][a] = make_float4(tv[a][0], tv[a][1], tv[a][2], tv[a][3]);
+        }
     }
     unsigned int need = 0u;
 #pragma unroll
@@ -132,10 +153,24 @@ __device__ inline void batch_tile_v4(const BatchVol<S> &V, unsigned int t, const
         if (!ok[q])
             continue;
 #pragma unroll
-        for (int a = 0; a < 3; a++)
-            *reinterpret_cast<float4 *>(u_out + ((unsigned int)a * n32 + i0[q])) = C[q][a];
+        for (int a = 0; a < 3; a++) {
+            float *dst = u_out + ((unsigned int)a * n32 + i0[q]);
+            if (!UNAL) {
+                *reinterpret_cast<float4 *>(dst) = C[q][a];
+            } else if (nin[q] == (unsigned int)VEC) {
+                *reinterpret_cast<kf_f4u *>(dst) = kf_f4u{C[q][a].x, C[q][a].y, C[q][a].z, C[q][a].w};
+            } else {
+                const float e[VEC] = {C[q][a].x, C[q][a].y, C[q][a].z, C[q][a].w};
+#pragma unroll
+                for (int k = 0; k < VEC; k++)
+                    if ((unsigned int)k < nin[q])
+                        dst[k] = e[k];
+            }
+        }
 #pragma unroll
         for (int k = 0; k < VEC; k++) {
+            if (UNAL && (unsigned int)k >= nin[q])
+                continue;
             const float c0 = k == 0 ? C[q][0].x : k == 1 ? C[q][0].y : k == 2 ? C[q][0].z : C[q][0].w;
             const float c1 = k == 0 ? C[q][1].x : k == 1 ? C[q][1].y : k == 2 ? C[q][1].z : C[q][1].w;
             const float c2 = k == 0 ? C[q][2].x : k == 1 ? C[q][2].y : k == 2 ? C[q][2].z : C[q][2].w;
@@ -206,8 +241,11 @@ __global__ __launch_bounds__(BLOCK) void k_adj_batch(const BatchVol<S> *__restri
         double mx = 0.0;
         unsigned long long act = 0;
         if (VPT >= 4 && vols[lo].vec4)
-            batch_tile_v4<S, (VPT >= 4 ? VPT / 4 : 1)>(vols[lo], t - vols[lo].tile0, c, &states[lo].div_slot[parity],
-                                                       parity, mx, act);
+            batch_tile_v4<S, (VPT >= 4 ? VPT / 4 : 1), false>(vols[lo], t - vols[lo].tile0, c,
+                                                              &states[lo].div_slot[parity], parity, mx, act);
+        else if (VPT >= 4)
+            batch_tile_v4<S, (VPT >= 4 ? VPT / 4 : 1), true>(vols[lo], t - vols[lo].tile0, c,
+                                                             &states[lo].div_slot[parity], parity, mx, act);
         else
             batch_tile<S, VPT>(vols[lo], t - vols[lo].tile0, c, &states[lo].div_slot[parity], parity, mx, act);
         kf_block_publish<BLOCK>(mine, t, 0.0, mx, act);
