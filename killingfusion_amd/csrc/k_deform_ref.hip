@@ -285,6 +285,15 @@ __device__ inline float4 ref_load_stream4(const float *p)
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// The same for a group that starts on a 4-byte boundary only (one 16-byte access on gfx950): the
+// u planes a N (a = 1, 2) when N is not a multiple of 4.
+typedef float ref_f4u __attribute__((ext_vector_type(4), aligned(4)));
+__device__ inline float4 ref_load_stream4u(const float *p)
+{
+    const ref_f4u v = __builtin_nontemporal_load(reinterpret_cast<const ref_f4u *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // Clears the list lengths and the non-finite flag of the workspace.
 __global__ void k_ref_clear(unsigned long long *aux)
 {
@@ -326,14 +335,16 @@ __global__ __launch_bounds__(BLOCK) void k_ref_finite(PhiView<S> L, unsigned lon
 
 // Pass 1 of a reference-formula step: every voxel's warped live SDF value
 // psi = SDF::getDistance(idx, disp) (SDF.cpp:570-575) and the band test
-// (KillingFusion.cpp:272-274), V voxels per thread (4: one float4 of each u plane).  A voxel
+// (KillingFusion.cpp:272-274), 4 voxels per thread (one float4 of each u plane; UNAL: N is not a
+// multiple of 4, so planes 1 and 2 are read as 4-byte-aligned 16-byte accesses and the group
+// that runs past N goes element by element).  A voxel
 // outside the band keeps u: its update is u + (-alpha * 0).  A voxel in the band is appended
 // to its XCD's list segment for pass 2 (one atomic per block), which overwrites its entry of
 // u_out; the list's length is the launch's active count.
 // Where u_j is +0 or -0 the interpolation weights are +0 and the interpolant is
 // phi_j + (finite) * 0 + ...; when every live value is finite (no non-finite flag) that is phi_j
 // itself unless phi_j is -0, so those voxels read phi_j alone.
-template <class S, int MODE, int V>
+template <class S, int MODE, bool UNAL>
 __global__ __launch_bounds__(SBLOCK) void k_ref_scan(PhiView<S> L, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, float hi, float lo, float alpha,
                                                     const DeformState *st, unsigned long long *aux, int parity,
@@ -346,6 +357,7 @@ __global__ __launch_bounds__(SBLOCK) void k_ref_scan(PhiView<S> L, const float *
     __shared__ unsigned int s_wtot[SBLOCK / 64];
     __shared__ unsigned long long s_base;
     const GridDims g = L.g;
+    constexpr int V = 4;
     const int64_t i0 = ((int64_t)kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x) * SBLOCK + threadIdx.x) * V;
     const bool phi_finite = aux[AUX_NONFINITE] == 0ull;
     bool act[V];
@@ -355,12 +367,14 @@ __global__ __launch_bounds__(SBLOCK) void k_ref_scan(PhiView<S> L, const float *
         act[v] = false;
         psi[v] = 0.0f;
     }
-    if (i0 < g.n) {
+    // voxels of the group inside the grid (4 unless UNAL and the group runs past N)
+    const int nin = i0 >= g.n ? 0 : (UNAL && g.n - i0 < V ? (int)(g.n - i0) : V);
+    if (nin > 0) {
         float u[3][V], ph[V];
-        if constexpr (V == 4) {
+        if (nin == V) {
 #pragma unroll
             for (int a = 0; a < 3; a++) {
-                const float4 q = ref_load_stream4(u_in + a * g.n + i0);
+                const float4 q = UNAL ? ref_load_stream4u(u_in + a * g.n + i0) : ref_load_stream4(u_in + a * g.n + i0);
                 u[a][0] = q.x;
                 u[a][1] = q.y;
                 u[a][2] = q.z;
@@ -369,12 +383,17 @@ __global__ __launch_bounds__(SBLOCK) void k_ref_scan(PhiView<S> L, const float *
             S::load4(L.p, i0, L.scale, ph);
         } else {
 #pragma unroll
-            for (int a = 0; a < 3; a++)
-                u[a][0] = u_in[a * g.n + i0];
-            ph[0] = S::load(L.p, i0, L.scale);
+            for (int v = 0; v < V; v++) {
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+                    u[a][v] = v < nin ? u_in[a * g.n + i0 + v] : 0.0f;
+                ph[v] = v < nin ? S::load(L.p, i0 + v, L.scale) : 0.0f;
+            }
         }
 #pragma unroll
         for (int v = 0; v < V; v++) {
+            if (v >= nin)
+                break;
             const unsigned int zb =
                 (__float_as_uint(u[0][v]) | __float_as_uint(u[1][v]) | __float_as_uint(u[2][v])) & 0x7fffffffu;
             if (phi_finite && zb == 0u && __float_as_uint(ph[v]) != 0x80000000u) {
@@ -394,10 +413,17 @@ __global__ __launch_bounds__(SBLOCK) void k_ref_scan(PhiView<S> L, const float *
 #pragma unroll
             for (int v = 0; v < V; v++)
                 o[v] = MODE == KF_MODE_GRAD ? 0.0f : u[a][v] + d;
-            if constexpr (V == 4)
-                *reinterpret_cast<float4 *>(u_out + a * g.n + i0) = make_float4(o[0], o[1], o[2], o[3]);
-            else
-                u_out[a * g.n + i0] = o[0];
+            float *dst = u_out + a * g.n + i0;
+            if (!UNAL) {
+                *reinterpret_cast<float4 *>(dst) = make_float4(o[0], o[1], o[2], o[3]);
+            } else if (nin == V) {
+                *reinterpret_cast<ref_f4u *>(dst) = ref_f4u{o[0], o[1], o[2], o[3]};
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; v++)
+                    if (v < nin)
+                        dst[v] = o[v];
+            }
         }
     }
     unsigned long long m[V];
@@ -594,7 +620,7 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
     const int64_t n = live->n;
     const bool vec = n % 4 == 0;
     const bool nbhd = c.h > 0.0f && c.h < 0.25f;
-    const int64_t per_block = (vec ? 4 : 1) * SBLOCK;
+    const int64_t per_block = 4 * SBLOCK;
     const unsigned int nb_a = (unsigned int)((n + per_block - 1) / per_block);
     // Segment x takes the band voxels of the pass-1 blocks b with b % 8 == x: at most their
     // voxels, so every segment has room for ceil(nb_a / 8) blocks.
@@ -634,10 +660,10 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
         }
         if (mode == KF_MODE_GRAD) {
             if (vec)
-                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, 4>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
+                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, false>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             else
-                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, 1>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
+                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_GRAD, true>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             if (nbhd)
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD, true>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
@@ -647,10 +673,10 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
                                    Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
         } else {
             if (vec)
-                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, 4>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
+                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, false>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             else
-                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, 1>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
+                hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, true>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             if (nbhd)
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, true>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
