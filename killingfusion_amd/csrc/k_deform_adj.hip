@@ -18,14 +18,15 @@
 // 1-neighbourhood of u (12 neighbours), and writes u_j - alpha dE/du_j for active voxels.
 // Compulsory HBM traffic per voxel-iteration: 2 s_phi + 24 bytes.
 //
-// Two launch shapes share the per-voxel arithmetic (adj_voxel):
-//  * k_adj_rows (rows whose length is a multiple of 4): a thread owns 4 consecutive x voxels
-//    of a row, a wavefront 256 voxels of a row, a block 4 rows of one slice.  A Jacobi step
-//    moves only active voxels: a voxel whose own u is zero samples phi_live[j] exactly, so the
-//    rows of u, phi_live and u_out stream as float4 / 8-byte accesses and only active voxels
-//    (and moved ones, to find out) run the stencil.  Energy and gradient passes, which need
-//    every voxel whose stencil sees a nonzero u, take that set from a bit mask of the 4-voxel
-//    groups of u that may be nonzero.
+// Three launch shapes share the per-voxel arithmetic (adj_voxel):
+//  * k_adj_flat (Jacobi steps): a thread owns 4 consecutive voxels of the flat index.  A step
+//    moves only active voxels: a voxel whose own u is zero samples phi_live[j] exactly, so u,
+//    phi_live and u_out stream as 16-byte / 8-byte accesses and only active voxels (and moved
+//    ones, to find out) run the stencil.
+//  * k_adj_rows (energy and gradient passes): a thread owns 4 consecutive x voxels of a row, a
+//    wavefront 256 voxels of a row, a block 4 rows of one slice.  These passes need every voxel
+//    whose stencil sees a nonzero u, and take that set from a bit mask over the 4-voxel groups
+//    of each row.
 //  * k_adj_any (any shape): one voxel per thread, neighbours read individually.
 // Blocks go to the XCDs in chunks of 64 consecutive tiles dealt round-robin: neighbouring
 // tiles of a chunk share an XCD's L2, and the band's stencils spread over every XCD.
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_any(PhiRead<S> L, PhiRead<S> Cn, 
 }
 
 // ---------------------------------------------------------------------------
-// Rows of a multiple of 4 voxels: 4 voxels per thread, float4 row loads.
+// Energy and gradient passes by rows: 4 voxels per thread, 16-byte row loads.
 // ---------------------------------------------------------------------------
 
 // Nonzero masks.  mask bit (row, group) = 1 iff some u component of the 4-voxel group
@@ -158,6 +159,44 @@ __device__ inline unsigned int stencil_bit(const unsigned int *__restrict__ d, i
     return (row[gx >> 5] >> (gx & 31)) & 1u;
 }
 
+// Row segments for rows that are not a multiple of 4 voxels.  A group then starts off a 16-byte
+// boundary (read as one 4-byte-aligned 16-byte access), and the row's last group may end
+// early: its elements past the row end read as zero, as row4's groups outside the grid do.
+__device__ inline float4 row4u(const float *__restrict__ u, const GridDims &g, int c, int x0, int y, int z)
+{
+    if (!(x0 < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz))
+        return make_float4(0.f, 0.f, 0.f, 0.f);
+    const float *p = u + ((unsigned int)c * nvox32(g) + g.idx32(x0, y, z));
+    if (x0 + VEC <= g.nx)
+        return kf_load_stream4u(p);
+    const int m = g.nx - x0;  // 1 .. 3 voxels left in the row
+    return make_float4(p[0], m > 1 ? p[1] : 0.f, m > 2 ? p[2] : 0.f, 0.f);
+}
+
+template <bool UNAL>
+__device__ inline float4 row_group(const float *__restrict__ u, const GridDims &g, int c, int x0, int y, int z)
+{
+    return UNAL ? row4u(u, g, c, x0, y, z) : row4(u, g, c, x0, y, z);
+}
+
+// Stores the group v at dst, whose row has left >= 1 voxels from dst on (the layout of
+// row_group's reads).
+template <bool UNAL>
+__device__ inline void put_group(float *__restrict__ dst, const float4 &v, int left)
+{
+    if (!UNAL) {
+        *reinterpret_cast<float4 *>(dst) = v;
+    } else if (left >= VEC) {
+        *reinterpret_cast<kf_f4u *>(dst) = kf_f4u{v.x, v.y, v.z, v.w};
+    } else {
+        dst[0] = v.x;
+        if (left > 1)
+            dst[1] = v.y;
+        if (left > 2)
+            dst[2] = v.z;
+    }
+}
+
 // Writes the wavefront's mask words of row (y, z): bit lane = nz of that lane's group.
 __device__ inline void put_bits(unsigned int *__restrict__ m, int wpr, const GridDims &g, int y, int z, int seg,
                                 int lane, bool nz)
@@ -170,7 +209,10 @@ __device__ inline void put_bits(unsigned int *__restrict__ m, int wpr, const Gri
     }
 }
 
-template <class S, int MODE>
+// UNAL: the row length is not a multiple of 4, so a group starts off a 16-byte boundary (u
+// groups move as 4-byte-aligned 16-byte accesses, phi_live element by element) and the last
+// group of a row holds 1 .. 3 voxels.
+template <class S, int MODE, bool UNAL>
 __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, AdjConsts c, DeformState *st,
                                                     BlockPartial *partials, float *__restrict__ e_voxel, int nseg,
@@ -210,12 +252,19 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
         const int y = yr[r];
 #pragma unroll
         for (int a = 0; a < 3; a++)
-            C[r][a] = row4(u_in, g, a, x0, y, z);
+            C[r][a] = row_group<UNAL>(u_in, g, a, x0, y, z);
 #pragma unroll
         for (int k = 0; k < VEC; k++)
             live4[r][k] = 0.f;
-        if (valid[r])
+        if (valid[r] && !UNAL) {
             S::load4(L.p, (int64_t)g.idx32(x0, y, z), L.scale, live4[r]);
+        } else if (valid[r]) {
+            const unsigned int i0 = g.idx32(x0, y, z);
+#pragma unroll
+            for (int k = 0; k < VEC; k++)
+                if (x0 + k < g.nx)
+                    live4[r][k] = S::load(L.p, (int64_t)(i0 + (unsigned int)k), L.scale);
+        }
         nb_bits[r] = 0u;
         if (MODE != KF_MODE_STEP && valid[r])
             nb_bits[r] = stencil_bit(mask_in, wpr, g, y, z, seg, lane);
@@ -233,7 +282,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
             const bool moved = ((__float_as_uint(f4(C[r][0], k)) | __float_as_uint(f4(C[r][1], k)) |
                                  __float_as_uint(f4(C[r][2], k))) & 0x7fffffffu) != 0u;
             const bool in_band = !(live4[r][k] > c.hi || live4[r][k] < c.lo);
-            if (valid[r] && (MODE == KF_MODE_STEP ? (moved || in_band) : (!quiet || in_band)))
+            if (valid[r] && (!UNAL || x0 + k < g.nx) &&
+                (MODE == KF_MODE_STEP ? (moved || in_band) : (!quiet || in_band)))
                 need |= 1u << (r * VEC + k);
         }
     }
@@ -247,12 +297,12 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
         const unsigned int i0 = g.idx32(x0, yr[r], z);
         if (MODE == KF_MODE_ENERGY) {
             if (e_voxel)
-                *reinterpret_cast<float4 *>(e_voxel + i0) = make_float4(0.f, 0.f, 0.f, 0.f);
+                put_group<UNAL>(e_voxel + i0, make_float4(0.f, 0.f, 0.f, 0.f), g.nx - x0);
         } else {
 #pragma unroll
             for (int a = 0; a < 3; a++) {
                 const float4 d = MODE == KF_MODE_GRAD ? make_float4(0.f, 0.f, 0.f, 0.f) : C[r][a];
-                *reinterpret_cast<float4 *>(u_out + ((unsigned int)a * n32 + i0)) = d;
+                put_group<UNAL>(u_out + ((unsigned int)a * n32 + i0), d, g.nx - x0);
             }
         }
     }
@@ -408,7 +458,8 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
     kf_block_publish<FB>(partials, blockIdx.x, esum, mx, act);
 }
 
-// Nonzero mask of field values u (layout above).
+// Nonzero mask of field values u (layout above; UNAL as in k_adj_rows).
+template <bool UNAL>
 __global__ __launch_bounds__(BLOCK) void k_adj_mask(const float *__restrict__ u, GridDims g, int nseg, int nyt,
                                                     unsigned int *__restrict__ mask)
 {
@@ -424,7 +475,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_mask(const float *__restrict__ u,
         unsigned int b = 0u;
 #pragma unroll
         for (int a = 0; a < 3; a++)
-            b |= bits4(row4(u, g, a, x0, y, z));
+            b |= bits4(row_group<UNAL>(u, g, a, x0, y, z));
         put_bits(mask, nseg * MW, g, y, z, seg, lane, (b & 0x7fffffffu) != 0u);
     }
 }
@@ -441,9 +492,12 @@ void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float 
     else if (MODE == KF_MODE_STEP && flat1)
         hipLaunchKernelGGL((k_adj_flat<S, true>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out, c,
                            ctx->state, parts, parity, prev, prev_nb);
+    else if (rows && L.g.nx % VEC == 0)
+        hipLaunchKernelGGL((k_adj_rows<S, MODE, false>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out,
+                           c, ctx->state, parts, e_voxel, nseg, nyt, mask_in, parity, prev, prev_nb);
     else if (rows)
-        hipLaunchKernelGGL((k_adj_rows<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
-                           ctx->state, parts, e_voxel, nseg, nyt, mask_in, parity, prev, prev_nb);
+        hipLaunchKernelGGL((k_adj_rows<S, MODE, true>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out,
+                           c, ctx->state, parts, e_voxel, nseg, nyt, mask_in, parity, prev, prev_nb);
     else
         hipLaunchKernelGGL((k_adj_any<S, MODE>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out, c,
                            ctx->state, parts, e_voxel, parity, prev, prev_nb);
@@ -453,7 +507,7 @@ void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float 
 
 size_t kf_adj_mask_words(const int32_t dims[3])
 {
-    if (dims[0] % VEC != 0 || 3 * (int64_t)dims[0] * dims[1] * dims[2] >= ((int64_t)1 << 30))
+    if (3 * (int64_t)dims[0] * dims[1] * dims[2] >= ((int64_t)1 << 30))
         return 0;
     return (size_t)((dims[0] + ROW - 1) / ROW) * MW * (size_t)dims[1] * (size_t)dims[2];
 }
@@ -462,9 +516,14 @@ kf_status kf_launch_adj_mask(const kf_field *f, const float *u, unsigned int *ma
 {
     const int nseg = (f->dims[0] + ROW - 1) / ROW, nyt = (f->dims[1] + ROWS - 1) / ROWS;
     const size_t words = kf_adj_mask_words(f->dims);
-    KF_CHECK_ARG(words > 0, "nonzero masks need rows of a multiple of 4 voxels");
+    KF_CHECK_ARG(words > 0, "nonzero masks need 3 N < 2^30 (32-bit offsets)");
     const unsigned int nb = (unsigned int)(nseg * nyt * f->dims[2]);
-    hipLaunchKernelGGL(k_adj_mask, dim3(nb), dim3(BLOCK), 0, f->ctx->stream, u, grid_of(f), nseg, nyt, mask + words);
+    if (f->dims[0] % VEC == 0)
+        hipLaunchKernelGGL(k_adj_mask<false>, dim3(nb), dim3(BLOCK), 0, f->ctx->stream, u, grid_of(f), nseg, nyt,
+                           mask + words);
+    else
+        hipLaunchKernelGGL(k_adj_mask<true>, dim3(nb), dim3(BLOCK), 0, f->ctx->stream, u, grid_of(f), nseg, nyt,
+                           mask + words);
     hipLaunchKernelGGL(k_adj_dilate, dim3((unsigned int)((words + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
                        f->ctx->stream, mask + words, grid_of(f), nseg * MW, mask);
     KF_LAUNCH_CHECK();
@@ -478,13 +537,11 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     kf_context *ctx = live->ctx;
     const AdjConsts c = kf_adj_consts(p);
     KF_CHECK_ARG(live->dtype == can->dtype, "live and canonical volumes must share a dtype");
-    // The row kernel needs rows of a multiple of 4 voxels (float4 loads, 16-byte aligned plane
-    // starts), 3 N < 2^30 (32-bit offsets) and, for energy and gradient passes, the nonzero mask
-    // of u_in.
     // A step runs the flat kernel when N is a multiple of 4 and 3 N < 2^30 (float4 plane
     // accesses, 32-bit offsets), its unaligned form for other N < 2^30 / 3; energy and
-    // gradient passes run the row kernel when rows are a multiple of 4 and the nonzero mask of
-    // u_in is given.  Everything else runs k_adj_any (a full stencil per voxel).
+    // gradient passes run the row kernel (its unaligned form for rows that are not a multiple
+    // of 4 voxels) when 3 N < 2^30 and the nonzero mask of u_in is given.  Everything else runs
+    // k_adj_any (a full stencil per voxel).
     const bool small = 3 * live->n < ((int64_t)1 << 30);
     const bool flat = mode == KF_MODE_STEP && live->n % VEC == 0 && small;
     const bool flat1 = mode == KF_MODE_STEP && !flat && small;

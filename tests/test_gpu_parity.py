@@ -202,6 +202,46 @@ def test_adjoint_gradient_and_energy_match_oracle(ctx, c1):
     assert np.abs(ev - oev).max() <= 1e-5 * (1 + np.abs(oev).max())
 
 
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+@pytest.mark.parametrize("dims", [(61, 45, 58), (62, 33, 40), (63, 47, 29)])
+def test_adjoint_gradient_and_energy_on_uneven_rows(ctx, dims, dtype):
+    """Rows of 4 k + 1, 2 and 3 voxels: the row passes' unaligned groups and short last group.
+    u is nonzero on a sparse random set and on the last voxel of every fifth row, so the
+    nonzero mask is mostly clear and its partial end groups are set."""
+    print(f"case: adjoint forward/backward, ellipsoid pair {dims} (rows of {dims[0] % 4} past a multiple "
+          f"of 4), {dtype}", flush=True)
+    vox = scenes.BATCH_VOXEL
+    ext = [d * vox for d in dims]
+    d1, d2, _ = scenes.ellipsoid_pair(0x5EED, 3, max_axis=0.4 * min(ext))
+    min3d = (-ext[0] / 2, -ext[1] / 2, 0.8 - ext[2] / 2)
+    live, can = make_pair(ctx, dims, min3d, vox, 3 * vox, d2, d1, dtype)
+    g = ogrid(dims, min3d, vox, 3 * vox)
+    n = g.n
+    u = scenes.seeded_field(dims, amplitude=0.5, seed=11)
+    keep = scenes.uniform(12, n) < 0.02
+    rows = np.arange(n) // dims[0]
+    keep |= (np.arange(n) % dims[0] == dims[0] - 1) & (rows % 5 == 0)
+    u[:, ~keep] = 0.0
+    kp = KillingParams.paper(vox, 3.0, method="adjoint")
+    field, out = DisplacementField(ctx, dims), DisplacementField(ctx, dims)
+    field.upload(u)
+    out.upload(np.full((3, n), np.nan, dtype=np.float32))  # every element must be written
+    energy_backward(live, can, field, kp, out)
+    gg = out.download().astype(np.float64)
+    e, ev = energy_forward(live, can, field, kp, per_voxel=True)
+    lphi, _ = live.download()
+    cphi, _ = can.download()
+    args = (lphi.astype(np.float64), cphi.astype(np.float64), u.astype(np.float64), g, oparams(kp))
+    og, _ = O.adj_gradient(*args)
+    oe, oev, _ = O.adj_energy(*args, per_voxel=True)
+    print(f"nonzero gradients {np.any(og != 0, axis=0).sum()} of {n}, energy {e:.6e} / {oe:.6e}", flush=True)
+    assert np.any(og != 0, axis=0).sum() > 1000
+    assert np.isfinite(gg).all() and np.isfinite(ev).all()
+    assert np.abs(gg - og).max() <= 1e-4 * (1 + np.abs(og).max())
+    assert e == pytest.approx(oe, rel=1e-5)
+    assert np.abs(ev - oev).max() <= 1e-5 * (1 + np.abs(oev).max())
+
+
 def test_adjoint_deform_matches_fp64_oracle(ctx, c1):
     print("case: adjoint deform 64^3, 100 iterations", flush=True)
     kp = c1_params(method="adjoint", max_iters=100)
