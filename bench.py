@@ -343,7 +343,7 @@ def main(argv=None):
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_adj_flat<StoreBF16>", "bytes_per_voxel_iteration": bytes_per_vi,
+                     "kernel": "k_adj_flat<StoreBF16, false>", "bytes_per_voxel_iteration": bytes_per_vi,
                      "kernel_ms": kernel_ms,
                      "traffic_gbs": traffic / (kernel_ms * 1e-3) / 1e9 if traffic else None,
                      "note": "achieved charges phi_can to every voxel; the kernel reads it only where the "
