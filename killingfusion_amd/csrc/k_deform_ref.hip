@@ -369,8 +369,9 @@ __global__ __launch_bounds__(SBLOCK) void k_ref_scan(PhiView<S> L, const float *
     }
     // voxels of the group inside the grid (4 unless UNAL and the group runs past N)
     const int nin = i0 >= g.n ? 0 : (UNAL && g.n - i0 < V ? (int)(g.n - i0) : V);
+    float u[3][V];
     if (nin > 0) {
-        float u[3][V], ph[V];
+        float ph[V];
         if (nin == V) {
 #pragma unroll
             for (int a = 0; a < 3; a++) {
@@ -404,26 +405,6 @@ __global__ __launch_bounds__(SBLOCK) void k_ref_scan(PhiView<S> L, const float *
                 psi[v] = L.trilinear(x, y, z, u[0][v], u[1][v], u[2][v]);
             }
             act[v] = !(psi[v] > hi || psi[v] < lo);
-        }
-        // Voxels in the band get their value in pass 2.
-        const float d = -alpha * 0.0f;
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            float o[V];
-#pragma unroll
-            for (int v = 0; v < V; v++)
-                o[v] = MODE == KF_MODE_GRAD ? 0.0f : u[a][v] + d;
-            float *dst = u_out + a * g.n + i0;
-            if (!UNAL) {
-                *reinterpret_cast<float4 *>(dst) = make_float4(o[0], o[1], o[2], o[3]);
-            } else if (nin == V) {
-                *reinterpret_cast<ref_f4u *>(dst) = ref_f4u{o[0], o[1], o[2], o[3]};
-            } else {
-#pragma unroll
-                for (int v = 0; v < V; v++)
-                    if (v < nin)
-                        dst[v] = o[v];
-            }
         }
     }
     unsigned long long m[V];
@@ -459,6 +440,30 @@ __global__ __launch_bounds__(SBLOCK) void k_ref_scan(PhiView<S> L, const float *
                 list_psi[q] = psi[v];
             }
             pos += (unsigned long long)__popcll(m[v]);
+        }
+    }
+    // The field values go out last: the wait on the list counter above would otherwise also
+    // wait for these stores (one counter tracks both on this target) while the block's other
+    // waves sit at the barrier.  Voxels in the band get their value in pass 2.
+    if (nin > 0) {
+        const float d = -alpha * 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float o[V];
+#pragma unroll
+            for (int v = 0; v < V; v++)
+                o[v] = MODE == KF_MODE_GRAD ? 0.0f : u[a][v] + d;
+            float *dst = u_out + a * g.n + i0;
+            if (!UNAL) {
+                *reinterpret_cast<float4 *>(dst) = make_float4(o[0], o[1], o[2], o[3]);
+            } else if (nin == V) {
+                *reinterpret_cast<ref_f4u *>(dst) = ref_f4u{o[0], o[1], o[2], o[3]};
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; v++)
+                    if (v < nin)
+                        dst[v] = o[v];
+            }
         }
     }
 }

@@ -1,7 +1,8 @@
 """The reference-formula step (kf_deform with method "reference", k_deform_ref.hip) against the
 FP64 oracle on inputs that take its less common code paths:
 
-* a grid whose voxel count is not a multiple of 4: pass 1 runs one voxel per thread;
+* a grid whose voxel count is not a multiple of 4: pass 1 reads the misaligned u planes as
+  unaligned 4-voxel groups and the group that runs past N element by element;
 * a finite-difference step h >= 1/4: pass 2 reads every sample's u cell from memory instead
   of the voxel's 3x3x3 neighbourhood;
 * -0 live values: those voxels interpolate in pass 1 instead of reading phi_j;
