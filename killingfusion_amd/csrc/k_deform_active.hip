@@ -69,48 +69,67 @@ __device__ inline void mark_span(const PhiRead<S> &L, const float *__restrict__ 
     if (n32 % 4u == 0u) {
         // Four voxels per lane (one float4 of each u plane): a wavefront covers 256 voxels, four
         // mask words.  Ballot v holds voxel 4 lane + v of each lane; word j takes lanes
-        // 16 j .. 16 j + 15, bit 4 m + v from bit 16 j + m of ballot v.
-#pragma unroll 2
-        for (int k = 0; k < SPAN / (BLOCK * 4); k++) {
+        // 16 j .. 16 j + 15, bit 4 m + v from bit 16 j + m of ballot v.  Lanes 0 .. 3 keep their
+        // words in registers and store them (and the non-finite flag) after the loop: one memory
+        // counter tracks loads and stores here, so a store inside the loop would make every later
+        // round's loads wait for it.
+        constexpr int ROUNDS = SPAN / (BLOCK * 4);
+        unsigned long long words[ROUNDS];
+        bool any_bad = false;
+        // Every round's loads go out before any is used.  A round past the grid's end reads the
+        // group at 0 instead (in bounds: N is a nonzero multiple of 4 here) and is ignored.
+        float live[ROUNDS][4];
+        float4 q[ROUNDS][3];
+#pragma unroll
+        for (int k = 0; k < ROUNDS; k++) {
+            const unsigned int i = base + (unsigned int)(k * BLOCK * 4 + w * 256) + 4u * (unsigned int)lane;
+            const unsigned int ic = i < n32 ? i : 0u;
+            S::load4(L.p, (int64_t)ic, L.scale, live[k]);
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                q[k][a] = READ_U ? kf_load_stream4(u + (unsigned int)a * n32 + ic) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int k = 0; k < ROUNDS; k++) {
             const unsigned int w0 = base + (unsigned int)(k * BLOCK * 4 + w * 256);
             const unsigned int i = w0 + 4u * (unsigned int)lane;
             bool cand[4] = {false, false, false, false}, bad = false;
             if (i < n32) {
-                float live[4];
-                S::load4(L.p, (int64_t)i, L.scale, live);
                 unsigned int ub[4] = {0u, 0u, 0u, 0u};
-                if (READ_U) {
 #pragma unroll
-                    for (int a = 0; a < 3; a++) {
-                        const float4 q = kf_load_stream4(u + (unsigned int)a * n32 + i);
-                        ub[0] |= __float_as_uint(q.x);
-                        ub[1] |= __float_as_uint(q.y);
-                        ub[2] |= __float_as_uint(q.z);
-                        ub[3] |= __float_as_uint(q.w);
-                    }
+                for (int a = 0; a < 3; a++) {
+                    ub[0] |= __float_as_uint(q[k][a].x);
+                    ub[1] |= __float_as_uint(q[k][a].y);
+                    ub[2] |= __float_as_uint(q[k][a].z);
+                    ub[3] |= __float_as_uint(q[k][a].w);
                 }
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    bad = bad || !isfinite(live[v]);
-                    cand[v] = ub[v] != 0u || !(live[v] > hi || live[v] < lo);
+                    bad = bad || !isfinite(live[k][v]);
+                    cand[v] = ub[v] != 0u || !(live[k][v] > hi || live[k][v] < lo);
                 }
             }
-            if (__ballot(bad) != 0ull && lane == 0)
-                atomicOr(nonfinite, 1u);
+            const bool wave_bad = __ballot(bad) != 0ull;  // every lane runs the ballot
+            any_bad = any_bad || wave_bad;
             unsigned long long bal[4];
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 bal[v] = __ballot(cand[v]);
                 cnt += (unsigned int)__popcll(bal[v]);
             }
-            if (lane < 4 && w0 + 64u * (unsigned int)lane < n32) {
-                unsigned long long word = 0ull;
+            words[k] = 0ull;
 #pragma unroll
-                for (int v = 0; v < 4; v++)
-                    word |= spread4(bal[v] >> (16 * lane)) << v;
-                mask[(w0 >> 6) + (unsigned int)lane] = word;
-            }
+            for (int v = 0; v < 4; v++)
+                words[k] |= spread4(bal[v] >> (16 * (lane & 3))) << v;
         }
+#pragma unroll
+        for (int k = 0; k < ROUNDS; k++) {
+            const unsigned int w0 = base + (unsigned int)(k * BLOCK * 4 + w * 256);
+            if (lane < 4 && w0 + 64u * (unsigned int)lane < n32)
+                mask[(w0 >> 6) + (unsigned int)lane] = words[k];
+        }
+        if (any_bad && lane == 0)
+            atomicOr(nonfinite, 1u);
     } else {
 #pragma unroll 4
         for (int k = 0; k < SPAN / BLOCK; k++) {
