@@ -131,27 +131,47 @@ __device__ inline void mark_span(const PhiRead<S> &L, const float *__restrict__ 
         if (any_bad && lane == 0)
             atomicOr(nonfinite, 1u);
     } else {
-#pragma unroll 4
-        for (int k = 0; k < SPAN / BLOCK; k++) {
-            const unsigned int w0 = base + (unsigned int)(k * BLOCK + w * 64);
-            const unsigned int i = w0 + (unsigned int)lane;
-            bool cand = false, bad = false;
-            if (i < n32) {
-                unsigned int ub = 0u;
-                if (READ_U)
-                    ub = __float_as_uint(kf_load_stream(u + i)) | __float_as_uint(kf_load_stream(u + n32 + i)) |
-                         __float_as_uint(kf_load_stream(u + 2u * n32 + i));
-                const float live = S::load(L.p, (int64_t)i, L.scale);
-                bad = !isfinite(live);
-                cand = ub != 0u || !(live > hi || live < lo);
+        // One voxel per lane, 16 rounds taken 4 at a time: a group's loads go out before any is
+        // used (a voxel past the grid's end reads voxel 0 instead and is ignored), and the mask
+        // words and the non-finite flag are stored after the loop, as above.
+        constexpr int ROUNDS = SPAN / BLOCK, GROUP = 4;
+        unsigned long long words[ROUNDS];
+        bool any_bad = false;
+#pragma unroll
+        for (int k0 = 0; k0 < ROUNDS; k0 += GROUP) {
+            float live[GROUP];
+            unsigned int ub[GROUP];
+#pragma unroll
+            for (int k = 0; k < GROUP; k++) {
+                const unsigned int i = base + (unsigned int)((k0 + k) * BLOCK + w * 64) + (unsigned int)lane;
+                const unsigned int ic = i < n32 ? i : 0u;
+                ub[k] = READ_U ? __float_as_uint(kf_load_stream(u + ic)) | __float_as_uint(kf_load_stream(u + n32 + ic)) |
+                                     __float_as_uint(kf_load_stream(u + 2u * n32 + ic))
+                               : 0u;
+                live[k] = S::load(L.p, (int64_t)ic, L.scale);
             }
-            if (__ballot(bad) != 0ull && lane == 0)
-                atomicOr(nonfinite, 1u);
-            const unsigned long long bal = __ballot(cand);
-            if (lane == 0 && w0 < n32)
-                mask[w0 >> 6] = bal;
-            cnt += (unsigned int)__popcll(bal);
+#pragma unroll
+            for (int k = 0; k < GROUP; k++) {
+                const unsigned int i = base + (unsigned int)((k0 + k) * BLOCK + w * 64) + (unsigned int)lane;
+                bool cand = false, bad = false;
+                if (i < n32) {
+                    bad = !isfinite(live[k]);
+                    cand = ub[k] != 0u || !(live[k] > hi || live[k] < lo);
+                }
+                const bool wave_bad = __ballot(bad) != 0ull;  // every lane runs the ballot
+                any_bad = any_bad || wave_bad;
+                words[k0 + k] = __ballot(cand);
+                cnt += (unsigned int)__popcll(words[k0 + k]);
+            }
         }
+#pragma unroll
+        for (int k = 0; k < ROUNDS; k++) {
+            const unsigned int w0 = base + (unsigned int)(k * BLOCK + w * 64);
+            if (lane == 0 && w0 < n32)
+                mask[w0 >> 6] = words[k];
+        }
+        if (any_bad && lane == 0)
+            atomicOr(nonfinite, 1u);
     }
     __shared__ unsigned int s_cnt[BLOCK / 64];
     if (lane == 0)
