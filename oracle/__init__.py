@@ -91,7 +91,7 @@ def lib():
             "kfo_fuse_warped": (None, [_dp, _i64p, _dp, _i64p, _dp, g]),
             "kfo_marching_cubes": (C.c_int64, [_dp, g, C.c_double, C.c_double, C.c_double, _dp, C.c_int64]),
             "kfo_mc_edge_mask": (C.c_int, [C.c_int]),
-            "kfo_mc_case": (C.c_char_p, [C.c_int]),
+            "kfo_mc_case": (None, [C.c_int, C.POINTER(C.c_int32)]),
             "kfo_adj_energy": (C.c_double, [_dp, _dp, _dp, g, p, _dp, _i64p]),
             "kfo_adj_gradient": (None, [_dp, _dp, _dp, g, p, _dp, _u8p]),
             "kfo_adj_energy_local": (C.c_double, [_dp, _dp, _dp, g, p, C.c_int, C.c_int, C.c_int]),
@@ -327,7 +327,10 @@ def mc_edge_mask(ci):
 
 
 def mc_case(ci):
-    return lib().kfo_mc_case(ci).decode()
+    """triTable[ci] of the oracle's own table (MarchingCubes.h:61-318): 16 edge ids, -1 padded."""
+    out = (C.c_int32 * 16)()
+    lib().kfo_mc_case(ci, out)
+    return list(out)
 
 
 def adj_energy(live, can, u, g: OGrid, p: OParams, per_voxel=False):

@@ -8,7 +8,7 @@
  * x86-64 build (no -march, hence no FMA) does.
  */
 #include "kf_oracle.h"
-#include "kf_mc_cases.h"
+#include "kf_mc_table.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -749,21 +749,20 @@ void kfo_fuse_warped(double *phi1, int64_t *w1, const double *phi2, const int64_
 /* a21-a23: marching cubes                                                   */
 /* ------------------------------------------------------------------------ */
 
-int kfo_mc_edge_mask(int ci)
+/* Edge e of Polygonise joins corners KFO_EDGE_ENDS[e] (the VertexInterp calls of
+ * MarchingCubes.h:386-409: p1 is the first corner, p2 the second). */
+static const int KFO_EDGE_ENDS[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                         {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+/* edgeTable[cube_index], MarchingCubes.h:26-59. */
+int kfo_mc_edge_mask(int ci) { return kfo_edge_table[ci & 255]; }
+
+/* triTable[cube_index] (MarchingCubes.h:61-318): 16 edge ids, -1 padded. */
+void kfo_mc_case(int ci, int32_t out[16])
 {
-    int m = 0;
-    for (int e = 0; e < 12; e++) {
-        int a = (ci >> KF_MC_EDGE_CORNERS[e][0]) & 1;
-        int b = (ci >> KF_MC_EDGE_CORNERS[e][1]) & 1;
-        if (a != b)
-            m |= 1 << e;
-    }
-    return m;
+    for (int k = 0; k < 16; k++)
+        out[k] = kfo_tri_table[ci & 255][k];
 }
-
-const char *kfo_mc_case(int ci) { return KF_MC_CASES[ci & 255]; }
-
-static inline int hexval(char c) { return c <= '9' ? c - '0' : c - 'a' + 10; }
 
 /* Polygonise + VertexInterp, MarchingCubes.h:326-422. */
 int kfo_polygonise(const double p[8][3], const double val[8], double iso, double tris[5][3][3])
@@ -772,24 +771,23 @@ int kfo_polygonise(const double p[8][3], const double val[8], double iso, double
     for (int k = 0; k < 8; k++)
         if (val[k] < iso)
             ci |= 1 << k;
-    int em = kfo_mc_edge_mask(ci);
+    const int em = kfo_edge_table[ci];
     if (em == 0)
         return 0;
     double vl[12][3];
     for (int e = 0; e < 12; e++) {
         if (!(em & (1 << e)))
             continue;
-        int a = KF_MC_EDGE_CORNERS[e][0], b = KF_MC_EDGE_CORNERS[e][1];
-        double s = (iso - val[a]) / (val[a] - val[b]);
+        const int a = KFO_EDGE_ENDS[e][0], b = KFO_EDGE_ENDS[e][1];
+        const double s = (iso - val[a]) / (val[a] - val[b]);
         for (int c = 0; c < 3; c++)
             vl[e][c] = p[a][c] + s * (p[a][c] - p[b][c]);
     }
-    const char *cs = KF_MC_CASES[ci];
     int nt = 0;
-    for (int i = 0; cs[i]; i += 3) {
+    for (int i = 0; kfo_tri_table[ci][i] != -1; i += 3) {
         for (int v = 0; v < 3; v++)
             for (int c = 0; c < 3; c++)
-                tris[nt][v][c] = vl[hexval(cs[i + v])][c];
+                tris[nt][v][c] = vl[kfo_tri_table[ci][i + v]][c];
         nt++;
     }
     return nt;

@@ -150,7 +150,7 @@ int64_t kfo_marching_cubes(const double *phi, const kfo_grid *g, double max_surf
                            double iso, double *verts, int64_t cap);
 int kfo_polygonise(const double p[8][3], const double val[8], double iso, double tris[5][3][3]);
 int kfo_mc_edge_mask(int cube_index);
-const char *kfo_mc_case(int cube_index);
+void kfo_mc_case(int cube_index, int32_t out[16]);
 
 /* ---- adjoint formulation (no reference counterpart) ---- */
 double kfo_adj_energy(const double *live, const double *can, const double *u, const kfo_grid *g,
