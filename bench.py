@@ -35,6 +35,8 @@ VOXEL = 0.004
 TRUNC = 3 * VOXEL
 MIN3D = (-0.512, -0.512, 0.288)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "adj_step_bf16_256_traffic.md")
+TRACE_FILE = os.path.join(ROOT, "profiles", "adj_step_bf16_256_kernel_trace.md")
+HEADLINE_KERNEL = "k_adj_flat<StoreBF16, false>"
 
 
 class HipEvents:
@@ -63,18 +65,27 @@ class HipEvents:
         return float(ms.value)
 
 
-def read_traffic():
-    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 PMC summary."""
+def read_profile_value(path, key):
+    """A `key: value` line of a committed rocprofv3 summary (None when absent)."""
     try:
-        for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch:"):
+        for line in open(path):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
 
 
+def step_bytes(s_phi, listed_fraction):
+    """Compulsory HBM bytes of one dense Jacobi step per voxel: u read and written (2 x 12 B),
+    phi_live read once (s_phi), phi_can read only where the gradient is evaluated -- the
+    voxels that can move (listed_fraction of the grid, KillingFusion.cpp:272-274)."""
+    return 24 + s_phi + s_phi * listed_fraction
+
+
 def time_deform(ctx, live, can, kp, steps, warmup, events):
+    """One deform call of `steps` iterations from a zero field after `warmup` untimed ones:
+    wall seconds, HIP-event milliseconds on the library's stream, stats."""
     from killingfusion_amd import DisplacementField, compute_displacement_field
     from killingfusion_amd import _lib
     field = DisplacementField(ctx, live.dims)
@@ -220,6 +231,35 @@ def bench_sequence(ctx, frames=100, mesh_every=10):
             "mesh_every": mesh_every, "mesh_triangles": tris}
 
 
+def step_only(grid, steps, warmup):
+    """The dense BF16 adjoint step alone on an N^3 grid of the torus pair (4 mm voxels, the
+    grid centred on the torus), for profiling at sizes other than 256^3: at 320^3 one u buffer
+    (393 MB) is larger than the 256 MB last-level cache, so the step's rate there is DRAM's."""
+    from killingfusion_amd import Context, KillingParams, SDF, scenes
+    d_can, d_live = scenes.torus_pair()
+    ctx = Context(0)
+    half = grid * VOXEL / 2
+    mn = (-half, -half, 0.8 - half)
+    kw = dict(voxel_size=VOXEL, clip_distance=TRUNC, max_surface=TRUNC, dtype="bf16")
+    live, can = SDF(ctx, (grid,) * 3, mn, **kw), SDF(ctx, (grid,) * 3, mn, **kw)
+    live.integrateDepthFrame(d_live, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+    can.integrateDepthFrame(d_can, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+    events = HipEvents()
+    kp = KillingParams.paper(VOXEL, 3.0, method="adjoint", sweep="dense")
+    _, gpu_ms, st = time_deform(ctx, live, can, kp, steps, warmup, events)
+    _, gpu_ms0, _ = time_deform(ctx, live, can, kp, 0, 0, events)
+    kp_a = KillingParams.paper(VOXEL, 3.0, method="adjoint", sweep="active")
+    _, _, st_a = time_deform(ctx, live, can, kp_a, 1, 0, events)
+    n = grid ** 3
+    us = (gpu_ms - gpu_ms0) / steps * 1e3
+    b = step_bytes(2, st_a["listed"] / n)
+    print(json.dumps({"step_only": f"dense adjoint step, bf16, {grid}^3 torus pair", "steps": steps,
+                      "kernel_us": us, "ps_per_voxel": us * 1e6 / n, "listed_voxels": int(st_a["listed"]),
+                      "bytes_per_voxel_iteration": b, "achieved_gbs": b * n / (us * 1e-6) / 1e9,
+                      "u_buffer_mb": 12 * n / 1e6}), flush=True)
+    return 0
+
+
 def calibrate_copy(launches=5):
     """A kernel of known HBM traffic for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters:
     kf_field_add on two 256^3 fields reads 2 x 3N floats and writes 3N floats per launch
@@ -253,6 +293,8 @@ def main(argv=None):
                     help="only run the known-traffic copy kernel used to scale rocprofv3 PMC counters")
     ap.add_argument("--sequence-only", action="store_true",
                     help="only run the 100-frame sequence and print its measurement (for profiling)")
+    ap.add_argument("--step-only", type=int, default=0, metavar="GRID",
+                    help="only time the dense BF16 adjoint step on a GRID^3 torus pair (for profiling)")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched uneven-volume measurement and print it (for profiling)")
     a = ap.parse_args(argv)
@@ -262,6 +304,8 @@ def main(argv=None):
         return 2
     if a.calibrate_copy:
         return calibrate_copy()
+    if a.step_only:
+        return step_only(a.step_only, a.steps, a.warmup)
     if a.sequence_only or a.batched_only:
         from killingfusion_amd import Context
         ctx = Context(0)
@@ -292,10 +336,13 @@ def main(argv=None):
     kp_adj = KillingParams.paper(VOXEL, 3.0, method="adjoint", sweep="dense")
     wall, gpu_ms, st = time_deform(ctx, live_bf, can_bf, kp_adj, a.steps, a.warmup, events)
     value = nvox * a.steps / wall
-    kernel_ms = gpu_ms / a.steps
-    bytes_per_vi = 2 * 2 + 24
-    achieved = bytes_per_vi * nvox / (kernel_ms * 1e-3) / 1e9
-    traffic = read_traffic()
+    # The step kernel's own launch time: a deform call also resets its state, evaluates the
+    # final energy (one forward pass) and reads its stats back once; a call of zero iterations
+    # does exactly that, so the difference is the K step launches (and one fold of the last).
+    _, gpu_ms0, _ = time_deform(ctx, live_bf, can_bf, kp_adj, 0, 0, events)
+    kernel_ms = (gpu_ms - gpu_ms0) / a.steps
+    traffic = read_profile_value(TRAFFIC_FILE, "traffic_bytes_per_launch")
+    profile_us = read_profile_value(TRACE_FILE, "headline_kernel_avg_us")
 
     kp_adj32 = KillingParams.paper(VOXEL, 3.0, method="adjoint", sweep="dense")
     wall32, gpu_ms32, _ = time_deform(ctx, live_32, can_32, kp_adj32, a.steps, a.warmup // 2, events)
@@ -303,6 +350,10 @@ def main(argv=None):
     # The same iterations over the voxels that can move only (the field is the same bit for bit).
     kp_act = KillingParams.paper(VOXEL, 3.0, method="adjoint", sweep="active")
     wall_a, gpu_ms_a, st_a = time_deform(ctx, live_bf, can_bf, kp_act, a.steps, a.warmup, events)
+    listed_fraction = st_a["listed"] / nvox
+    bytes_per_vi = step_bytes(2, listed_fraction)
+    achieved = bytes_per_vi * nvox / (kernel_ms * 1e-3) / 1e9
+    achieved_profile = bytes_per_vi * nvox / (profile_us * 1e-6) / 1e9 if profile_us else None
 
     plain_steps = a.plain_steps or a.steps
     kp_ref = KillingParams.paper(VOXEL, 3.0, method="reference")
@@ -343,12 +394,17 @@ def main(argv=None):
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_adj_flat<StoreBF16, false>", "bytes_per_voxel_iteration": bytes_per_vi,
+                     "kernel": HEADLINE_KERNEL, "bytes_per_voxel_iteration": bytes_per_vi,
+                     "algorithmic_bytes_per_launch": bytes_per_vi * nvox,
                      "kernel_ms": kernel_ms,
+                     "kernel_ms_method": "HIP events on the library stream: (deform call of `steps` iterations - "
+                                         "deform call of 0 iterations) / steps",
                      "traffic_gbs": traffic / (kernel_ms * 1e-3) / 1e9 if traffic else None,
-                     "note": "achieved charges phi_can to every voxel; the kernel reads it only where the "
-                             "stencil runs, so the measured HBM bytes per launch (traffic) are below the "
-                             "algorithmic bytes and traffic_gbs is the rate HBM actually delivers"},
+                     "profile_kernel_us": profile_us,
+                     "profile_frac": achieved_profile / HBM_PEAK_GBS if achieved_profile else None,
+                     "note": "algorithmic bytes per voxel-iteration = 24 (u read + written) + 2 (phi_live, "
+                             "bf16) + 2 x listed fraction (phi_can, read only where the gradient is "
+                             "evaluated)"},
         "cpu_baseline": cpu,
         "plain_fp32": {"value": plain_value, "ms_per_step": wall_p * 1e3 / plain_steps, "steps": plain_steps,
                        "kernel_ms": plain_kernel_ms,
@@ -356,6 +412,8 @@ def main(argv=None):
         "adjoint_fp32": {"value": nvox * a.steps / wall32, "kernel_ms": gpu_ms32 / a.steps,
                          "hbm_gbs_at_32B": 32 * nvox / (gpu_ms32 / a.steps * 1e-3) / 1e9},
         "speedup_vs_plain_fp32": value / plain_value,
+        "speedup_note": "dense vs dense: both steps stream u (24 B per voxel) every iteration, so the ratio is "
+                        "capped near (plain pass 1 + plain pass 2) / adjoint step",
         "active_set": {"value": nvox * a.steps / wall_a, "unit": "voxel-iterations/s (whole grid counted)",
                        "ms_per_step": wall_a * 1e3 / a.steps, "gpu_ms_per_step": gpu_ms_a / a.steps,
                        "listed_voxels": int(st_a["listed"]), "listed_fraction": st_a["listed"] / nvox,

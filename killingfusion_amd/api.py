@@ -222,11 +222,13 @@ class SDF:
 
     def upload(self, phi, weight=None):
         phi = _f32(phi).reshape(-1)
-        assert phi.size == self.n
+        if phi.size != self.n:
+            raise ValueError(f"phi holds {phi.size} values, the volume {self.n}")
         wp = None
         if weight is not None:
             w = np.ascontiguousarray(weight, dtype=np.int32).reshape(-1)
-            assert w.size == self.n
+            if w.size != self.n:
+                raise ValueError(f"weight holds {w.size} values, the volume {self.n}")
             wp = w.ctypes.data_as(_i32p)
         check(self._L.kf_volume_upload(self.handle, phi.ctypes.data_as(_fp), wp), "kf_volume_upload")
 
@@ -319,12 +321,17 @@ class DisplacementField:
         formats.write_displacement_bin(path, self.dims, voxel_size, self.download())
 
     def initializeAllVoxels(self, value):
-        v = np.array(value, dtype=np.float64)
+        """DisplacementField::initializeAllVoxels (DisplacementField.cpp:82-88): every voxel = value."""
+        v = np.ascontiguousarray(value, dtype=np.float64).reshape(-1)
+        if v.size != 3:
+            raise ValueError(f"a displacement has 3 components, not {v.size}")
         check(self._L.kf_field_fill(self.handle, v.ctypes.data_as(_dp)), "kf_field_fill")
 
     def upload(self, soa):
-        soa = _f32(soa).reshape(3, -1)
-        assert soa.shape[1] == self.n
+        soa = _f32(soa)
+        if soa.size != 3 * self.n:
+            raise ValueError(f"field upload holds {soa.size} values, the field 3 x {self.n}")
+        soa = soa.reshape(3, -1)
         check(self._L.kf_field_upload(self.handle, soa.ctypes.data_as(_fp)), "kf_field_upload")
 
     def download(self):
@@ -491,13 +498,28 @@ class KillingFusion:
         return dict(frame=st.frame_index, deform=_stats(st.deform), ms_integrate=st.ms_integrate,
                     ms_deform=st.ms_deform, ms_fuse=st.ms_fuse, ms_total=st.ms_total)
 
-    def canonical_download(self):
-        h = self._L.kf_pipeline_canonical(self.handle)
+    def _volume_download(self, h):
         n = self.dims[0] * self.dims[1] * self.dims[2]
         phi = np.empty(n, dtype=np.float32)
         w = np.empty(n, dtype=np.int32)
         check(self._L.kf_volume_download(h, phi.ctypes.data_as(_fp), w.ctypes.data_as(_i32p)), "kf_volume_download")
         return phi, w
+
+    def canonical_download(self):
+        """The canonical SDF (m_canonicalSdf, KillingFusion.h:99): phi (metres), weights."""
+        return self._volume_download(self._L.kf_pipeline_canonical(self.handle))
+
+    def live_download(self):
+        """The last frame's live SDF, warped into the canonical frame (SDF::update, KillingFusion.cpp:158)."""
+        return self._volume_download(self._L.kf_pipeline_live(self.handle))
+
+    def field_download(self):
+        """The last frame's displacement field (live -> canonical), SoA float32 [3, N], voxels."""
+        n = self.dims[0] * self.dims[1] * self.dims[2]
+        out = np.empty((3, n), dtype=np.float32)
+        check(self._L.kf_field_download(self._L.kf_pipeline_field(self.handle), out.ctypes.data_as(_fp)),
+              "kf_field_download")
+        return out
 
     def canonical_mesh(self, iso=0.0) -> SimpleMesh:
         h = C.c_void_p(self._L.kf_pipeline_canonical(self.handle))

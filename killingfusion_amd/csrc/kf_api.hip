@@ -443,6 +443,7 @@ kf_status kf_volume_copy(kf_volume *dst, const kf_volume *src)
 {
     KF_CHECK_ARG(dst && src, "null argument");
     KF_CHECK_ARG(same_dims(dst->dims, src->dims) && dst->dtype == src->dtype, "volumes differ in dims or dtype");
+    KF_CHECK_ARG(dst->ctx == src->ctx, "handles belong to different contexts");
     DeviceGuard dg(dst->ctx->device);
     hipStream_t s = dst->ctx->stream;
     KF_HIP(hipMemcpyAsync(dst->phi, src->phi, dst->elem_size() * (size_t)dst->n, hipMemcpyDeviceToDevice, s));

@@ -66,14 +66,26 @@ def _rays():
     return dx.ravel(), dy.ravel()
 
 
-def _march(sdf, z0: float, z1: float, lipschitz: float = 1.0, iters: int = 400, tol: float = 1e-7):
+def _march(sdf, z0: float, z1: float, lipschitz: float = 1.0, iters: int = 400, tol: float = 1e-7,
+           bound=None):
     """Sphere-trace f (negative inside) along every pixel ray parametrised by camera z.
     `lipschitz` bounds |grad f| along the ray per unit z (>= 1 for a true SDF; the ray
-    direction has length >= 1 per unit z).  Returns float32 depth (0 where no hit)."""
+    direction has length >= 1 per unit z).  `bound` = (centre, radius), a sphere holding the
+    whole surface: rays that miss it are misses, the others start where they enter it.
+    Returns float32 depth (0 where no hit)."""
     dx, dy = _rays()
     scale = np.sqrt(dx * dx + dy * dy + 1.0)  # ray length per unit z
     z = np.full(dx.shape, z0)
     alive = np.ones(dx.shape, dtype=bool)
+    if bound is not None:
+        # |(dx, dy, 1) z - c|^2 = R^2 as a quadratic in z: a z^2 - 2 b z + cc = 0
+        c = np.asarray(bound[0], dtype=np.float64)
+        a = scale * scale
+        b = dx * c[0] + dy * c[1] + c[2]
+        cc = float(c @ c) - float(bound[1]) ** 2
+        disc = b * b - a * cc
+        alive = disc >= 0.0
+        z = np.where(alive, np.maximum(z0, (b - np.sqrt(np.maximum(disc, 0.0))) / a), z0)
     hit = np.zeros(dx.shape, dtype=bool)
     for _ in range(iters):
         idx = np.nonzero(alive)[0]
@@ -226,12 +238,16 @@ def ellipsoid_pair(seed: int, index: int, max_axis: float = 0.2):
         dirv = dirv / max(np.linalg.norm(dirv), 1e-9)
         terms.append((float(amps[t]), k, float(2 * math.pi * u[24 + t]), dirv))
     c = (0.0, 0.0, 0.8)
-    d1 = _march(ellipsoid_fn(c, axes), 0.3, 1.5, lipschitz=2.5)
-    d2 = _march(ellipsoid_fn(c, axes, terms), 0.3, 1.5, lipschitz=3.0)
+    bound = (c, float(axes.max()) + 0.03)  # the deformation moves the surface by well under 3 cm
+    d1 = _march(ellipsoid_fn(c, axes), 0.3, 1.5, lipschitz=2.5, bound=bound)
+    d2 = _march(ellipsoid_fn(c, axes, terms), 0.3, 1.5, lipschitz=3.0, bound=bound)
     return d1, d2, axes
 
 
-BATCH_VOXEL = 0.006  # one voxel size for the whole batch (one set of optimiser params)
+# One voxel size for the whole batch (one set of optimiser params).  At 10 mm the smallest
+# grid, 48^3, spans 0.48 m, so the largest ellipsoid (semi-axes up to 0.2 m) fits whole.
+BATCH_VOXEL = 0.010
+BATCH_AXIS_FRAC = 0.45
 
 
 def batch_edges(count: int = 16, seed: int = 0x5EED, lo: int = 48, hi: int = 160):
@@ -240,15 +256,17 @@ def batch_edges(count: int = 16, seed: int = 0x5EED, lo: int = 48, hi: int = 160
     return [int(lo + min(int(x * (hi - lo + 1)), hi - lo)) for x in u]
 
 
-def batch_scene(count: int = 16, seed: int = 0x5EED, edges=None):
-    """Independent volumes of uneven size: volume b is an n_b^3 grid at BATCH_VOXEL centred on
-    (0, 0, 0.8), holding ellipsoid_pair(seed, b) with its semi-axes clipped to 0.4 of the grid
-    extent so the object fits.  Returns [(n, min_corner, depth_canonical, depth_live)]."""
+def batch_scene(count: int = 16, seed: int = 0x5EED, edges=None, voxel: float = BATCH_VOXEL):
+    """Independent volumes of uneven size: volume b is an n_b^3 grid of `voxel` centred on
+    (0, 0, 0.8), holding ellipsoid_pair(seed, b).  Semi-axes are clipped to 0.45 of the grid
+    extent so the object fits; at BATCH_VOXEL that never binds for edges of 45 or more, so the
+    batched workload's ellipsoids keep their full U(0.08, 0.2) m range.
+    Returns [(n, min_corner, depth_canonical, depth_live)]."""
     edges = batch_edges(count, seed) if edges is None else list(edges)
     out = []
     for b, n in enumerate(edges):
-        ext = n * BATCH_VOXEL
-        d1, d2, _ = ellipsoid_pair(seed, b, max_axis=0.4 * ext)
+        ext = n * voxel
+        d1, d2, _ = ellipsoid_pair(seed, b, max_axis=BATCH_AXIS_FRAC * ext)
         out.append((n, (-ext / 2, -ext / 2, 0.8 - ext / 2), d1, d2))
     return out
 

@@ -61,3 +61,17 @@ def make_pair(ctx, dims, min3d, voxel, trunc, d_live, d_can, dtype="f32"):
     live.integrateDepthFrame(d_live, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
     can.integrateDepthFrame(d_can, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
     return live, can
+
+
+def listed_on_cpu(live, u0, kp):
+    """The voxels an active-set sweep lists: some component of u_j with a nonzero bit pattern
+    (-0 included) or phi_live[j] in the band -clip <= phi <= Max - eps (KillingFusion.cpp:273),
+    compared in FP32 as the kernels compare them."""
+    phi, _ = live.download()
+    hi = np.float32(kp.max_surface - kp.epsilon)
+    lo = np.float32(-kp.clip_distance)
+    band = ~((phi > hi) | (phi < lo))
+    if u0 is None:
+        return int(band.sum())
+    moved = (np.asarray(u0, np.float32).view(np.uint32) != 0).any(axis=0)
+    return int((band | moved).sum())

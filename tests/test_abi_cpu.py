@@ -84,3 +84,23 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "libkf.so"))
     with pytest.raises(RuntimeError, match="missing"):
         _lib.load()
+
+
+def test_python_mirror_rejects_short_host_buffers():
+    # The size checks run before any C call (they must survive python -O, so they are not
+    # asserts): a short buffer would otherwise be read past its end by the upload copy.
+    from killingfusion_amd import SDF, DisplacementField
+    v = SDF.__new__(SDF)
+    v.dims, v._L = (3, 4, 5), None
+    with pytest.raises(ValueError):
+        v.upload(np.zeros(59, dtype=np.float32))
+    with pytest.raises(ValueError):
+        v.upload(np.zeros(60, dtype=np.float32), np.zeros(59, dtype=np.int32))
+    f = DisplacementField.__new__(DisplacementField)
+    f.dims, f._L = (3, 4, 5), None
+    with pytest.raises(ValueError):
+        f.upload(np.zeros((3, 59), dtype=np.float32))
+    with pytest.raises(ValueError):
+        f.initializeAllVoxels((0.1, 0.2))
+    with pytest.raises(ValueError):
+        f.initializeAllVoxels(np.zeros(4))

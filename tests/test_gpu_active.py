@@ -19,11 +19,11 @@ import pytest
 from killingfusion_amd import DisplacementField, KillingParams, compute_displacement_field, energy_forward, scenes
 from killingfusion_amd._lib import KF_E_DIVERGED, KFError
 
-from helpers import C1, make_pair, sphere_frames
+from helpers import C1, listed_on_cpu, make_pair, sphere_frames
 
 pytestmark = pytest.mark.gpu
 
-VOX = scenes.BATCH_VOXEL
+VOX = 0.006
 TRUNC = 3 * VOX
 
 
@@ -49,19 +49,6 @@ def run(ctx, live, can, kp, sweep, u0=None):
 
 def same_bits(a, b):
     return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
-
-
-def listed_on_cpu(live, u0, kp):
-    """The voxels the sweep lists: some component of u_j with a nonzero bit pattern (-0 included)
-    or phi_live[j] in the band."""
-    phi, _ = live.download()
-    hi = np.float32(kp.max_surface - kp.epsilon)
-    lo = np.float32(-kp.clip_distance)
-    band = ~((phi > hi) | (phi < lo))
-    if u0 is None:
-        return int(band.sum())
-    moved = (np.asarray(u0, np.float32).view(np.uint32) != 0).any(axis=0)
-    return int((band | moved).sum())
 
 
 def assert_same(a, b, tag):
@@ -115,6 +102,26 @@ def test_active_equals_dense_uneven(ctx, dims):
     live, can = make_pair(ctx, dims, (-ext[0] / 2, -ext[1] / 2, 0.8 - ext[2] / 2), VOX, TRUNC, d2, d1, "bf16")
     kp = KillingParams.paper(VOX, 3.0, method="adjoint", max_iters=50)
     assert_same(run(ctx, live, can, kp, "dense"), run(ctx, live, can, kp, "active"), f"ellipsoid {dims}")
+
+
+@pytest.mark.parametrize("dims", [(61, 45, 58), (63, 47, 29)])
+def test_active_with_initial_field_uneven(ctx, dims):
+    """A nonzero starting field in a sub-box on grids whose voxel count is not a multiple of 4
+    (the marking pass's one-voxel-per-lane path): the sweeps agree bit for bit and the list
+    holds exactly the voxels with a nonzero u bit pattern or an in-band live value."""
+    print(f"case: ellipsoid pair {dims}, 6 mm, bf16, start field in a sub-box, 30 iterations", flush=True)
+    ext = [d * VOX for d in dims]
+    d1, d2, _ = scenes.ellipsoid_pair(0x5EED, 5, max_axis=0.4 * min(ext))
+    live, can = make_pair(ctx, dims, (-ext[0] / 2, -ext[1] / 2, 0.8 - ext[2] / 2), VOX, TRUNC, d2, d1, "bf16")
+    kp = KillingParams.paper(VOX, 3.0, method="adjoint", max_iters=30)
+    u0 = scenes.seeded_field(dims, amplitude=0.4, seed=11).reshape(3, dims[2], dims[1], dims[0])
+    part = np.zeros_like(u0)
+    part[:, 3:17, 20:41, 7:33] = u0[:, 3:17, 20:41, 7:33]
+    part[1, dims[2] - 1, dims[1] - 1, dims[0] - 1] = -0.0  # the last voxel: listed, never moves
+    part = part.reshape(3, -1)
+    b = run(ctx, live, can, kp, "active", part)
+    assert_same(run(ctx, live, can, kp, "dense", part), b, f"sub-box field {dims}")
+    assert b[1]["listed"] == listed_on_cpu(live, part, kp)
 
 
 def test_active_with_initial_field(ctx, sphere):

@@ -12,7 +12,7 @@ from killingfusion_amd import (SDF, DisplacementField, KillingParams, batch_auto
                                compute_displacement_field, scenes)
 from killingfusion_amd._lib import KF_E_DIVERGED, KF_E_INVALID_ARG, KFError
 
-from helpers import make_pair
+from helpers import listed_on_cpu, make_pair
 
 pytestmark = pytest.mark.gpu
 
@@ -244,3 +244,4 @@ def test_active_batch_of_many_small_volumes(ctx):
         assert np.array_equal(f.download(), ref[b]), b
         for k in ("iters", "listed", "active", "max_update", "diverged_index"):
             assert st[b][k] == ref_st[b][k], (b, k, st[b][k], ref_st[b][k])
+        assert st[b]["listed"] == listed_on_cpu(lives[b], inits[b], kp), b

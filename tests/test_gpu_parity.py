@@ -210,7 +210,7 @@ def test_adjoint_gradient_and_energy_on_uneven_rows(ctx, dims, dtype):
     nonzero mask is mostly clear and its partial end groups are set."""
     print(f"case: adjoint forward/backward, ellipsoid pair {dims} (rows of {dims[0] % 4} past a multiple "
           f"of 4), {dtype}", flush=True)
-    vox = scenes.BATCH_VOXEL
+    vox = 0.006
     ext = [d * vox for d in dims]
     d1, d2, _ = scenes.ellipsoid_pair(0x5EED, 3, max_axis=0.4 * min(ext))
     min3d = (-ext[0] / 2, -ext[1] / 2, 0.8 - ext[2] / 2)

@@ -24,7 +24,7 @@ from helpers import C1, make_pair, ogrid, oparams, sphere_frames
 
 pytestmark = pytest.mark.gpu
 
-VOX = scenes.BATCH_VOXEL
+VOX = 0.006
 
 
 def ref_params(voxel, trunc, **kw):

@@ -1,0 +1,141 @@
+"""The batched and full-sequence workloads at their own sizes, against the FP64 oracle.
+
+Batched: 16 independent ellipsoid volumes, edges drawn from 48..160, BF16 storage, 50 adjoint
+iterations per call (kf_batch_deform).  Each volume's field must match the oracle's adjoint
+deform run on the volume's stored (decoded) SDFs within 1e-6 voxel, with the same iteration
+count and energy, on both sweeps: the active-set sweep and the dense sweep, whose loop is a
+captured hipGraph replayed on every call (checked on three calls).
+
+Sequence: the first frames of the 100-frame capsule-hinge sequence through the per-frame
+driver (kf_pipeline_process_frame, KillingFusion::processNextFrame, KillingFusion.cpp:109-175)
+at 192^3 with BF16 storage and 100 adjoint iterations per frame, stage by stage against the
+oracle run on the grids the GPU stored: integrate, warp and fuse bit for bit after the BF16
+rounding, the deform within 1e-6 voxel, and the canonical marching-cubes soup bit for bit.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle as O
+from killingfusion_amd import (DisplacementField, KillingFusion, KillingParams, batch_autotune, batch_deform,
+                               scenes)
+
+from helpers import bf16_roundtrip, make_pair, ogrid, oparams
+
+pytestmark = pytest.mark.gpu
+
+
+def u_err(ug, uo):
+    moved = np.any(uo != 0, axis=0) | np.any(ug != 0, axis=0)
+    e = np.abs(np.asarray(ug, np.float64) - uo).max(axis=0)[moved]
+    return (float(e.max()) if e.size else 0.0), int(moved.sum())
+
+
+# ---------------------------------------------------------------------------
+# Batched: 16 volumes of 48^3..160^3
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def batch16(ctx):
+    vox = scenes.BATCH_VOXEL
+    print("case: batched workload, 16 ellipsoid volumes (edges 48..160, 10 mm), bf16: rendering", flush=True)
+    scene = scenes.batch_scene(16)
+    kp = KillingParams.paper(vox, 3.0, method="adjoint", max_iters=50)
+    vols, oracle_out = [], []
+    for b, (n, mn, d1, d2) in enumerate(scene):
+        live, can = make_pair(ctx, (n, n, n), mn, vox, 3 * vox, d2, d1, "bf16")
+        lphi, _ = live.download()
+        cphi, _ = can.download()
+        g = ogrid((n, n, n), mn, vox, 3 * vox)
+        uo, so = O.adj_deform(lphi.astype(np.float64), cphi.astype(np.float64), np.zeros((3, g.n)), g, oparams(kp))
+        vols.append((live, can))
+        oracle_out.append((uo, so))
+    print("edges", [v[0].dims[0] for v in vols], "oracle iterations", [o[1]["iters"] for o in oracle_out], flush=True)
+    return dict(vols=vols, oracle=oracle_out, kp=kp)
+
+
+@pytest.mark.parametrize("sweep", ["active", "dense"])
+def test_batched_16_volumes_against_oracle(ctx, batch16, sweep):
+    print(f"case: batched workload, 16 volumes, 50 iterations, {sweep} sweep, 3 calls", flush=True)
+    vols, kp = batch16["vols"], dataclasses.replace(batch16["kp"], sweep=sweep)
+    lives, cans = [v[0] for v in vols], [v[1] for v in vols]
+    fields = [DisplacementField(ctx, v.dims) for v in lives]
+    if sweep == "dense":
+        cfg = batch_autotune(lives, cans, fields, kp)
+        print("tile shape", cfg, flush=True)
+    for call in range(3):
+        for f in fields:
+            f.initializeAllVoxels((0.0, 0.0, 0.0))
+        st = batch_deform(lives, cans, fields, kp)
+        worst = 0.0
+        for b, f in enumerate(fields):
+            uo, so = batch16["oracle"][b]
+            err, moved = u_err(f.download(), uo)
+            worst = max(worst, err)
+            assert st[b]["iters"] == so["iters"] == 50, (call, b)
+            assert st[b]["sweep"] == sweep
+            assert err <= 1e-6, (call, b, err)
+            assert st[b]["energy"] == pytest.approx(so["energy"], rel=1e-6), (call, b)
+            assert moved > 100
+        print(f"call {call}: max |u - u_oracle| over all volumes {worst:.3e} voxel", flush=True)
+    for f in fields:
+        f.close()
+
+
+# ---------------------------------------------------------------------------
+# Sequence: capsule hinge, 192^3, BF16
+# ---------------------------------------------------------------------------
+
+SEQ_FRAMES = 5
+
+
+def test_sequence_192_bf16_stages_against_oracle(ctx):
+    n, vox = scenes.SEQ_GRID, scenes.SEQ_VOXEL
+    trunc = 3 * vox
+    scale = np.float32(trunc + 1e-5)
+    dims = (n, n, n)
+    g = ogrid(dims, scenes.SEQ_MIN, vox, trunc)
+    kp = KillingParams.paper(vox, 3.0, method="adjoint", max_iters=100)
+    pl = KillingFusion(ctx, dims, scenes.SEQ_MIN, vox, kp, scenes.K, dtype="bf16")
+
+    def stored(phi64):
+        return bf16_roundtrip(np.asarray(phi64, np.float64).astype(np.float32), scale)
+
+    for k in range(SEQ_FRAMES):
+        depth = scenes.hinge_frame_exact(k / 99.0, 0x5EED, k)
+        print(f"case: hinge sequence 192^3 bf16, frame {k}", flush=True)
+        cphi0, cw0 = pl.canonical_download()
+        st = pl.processNextFrame(depth)
+        lphi_o, lw_o = O.sdf_init(g)
+        O.integrate(lphi_o, lw_o, g, trunc, depth, scenes.K, scenes.C2W, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+        cphi, cw = pl.canonical_download()
+        if k == 0:
+            assert (cw == lw_o).all() and np.array_equal(cphi, stored(lphi_o)), "frame 0 integration"
+            continue
+        lstored = stored(lphi_o)
+        # deform on the stored grids: live (as integrated) onto the previous canonical
+        uo, so = O.adj_deform(lstored.astype(np.float64), cphi0.astype(np.float64), np.zeros((3, g.n)), g,
+                              oparams(kp))
+        ug = pl.field_download()
+        err, moved = u_err(ug, uo)
+        print(f"frame {k}: deform iters {st['deform']['iters']} listed {st['deform']['listed']} "
+              f"moved {moved} max |u - u_oracle| {err:.3e} voxel", flush=True)
+        assert st["deform"]["iters"] == so["iters"] == 100
+        assert err <= 1e-6 and moved > 1000
+        # warp of the integrated live SDF by the GPU's field, then fuse into the previous canonical
+        wphi, ww = lstored.astype(np.float64), lw_o.copy()
+        O.warp(wphi, ww, ug.astype(np.float64), g)
+        gl_phi, gl_w = pl.live_download()
+        assert (gl_w == ww).all(), f"frame {k}: warp weights"
+        assert np.array_equal(gl_phi, stored(wphi)), f"frame {k}: warp"
+        fphi, fw = cphi0.astype(np.float64), cw0.astype(np.int64)
+        O.fuse(fphi, fw, gl_phi.astype(np.float64), gl_w.astype(np.int64))
+        assert (cw == fw).all(), f"frame {k}: fuse weights"
+        assert np.array_equal(cphi, stored(fphi)), f"frame {k}: fuse"
+    mesh = pl.canonical_mesh()
+    ref = O.marching_cubes(cphi.astype(np.float64), g, trunc, trunc)
+    print(f"canonical mesh after {SEQ_FRAMES} frames: {mesh.tris.shape[0]} triangles", flush=True)
+    assert mesh.tris.shape[0] == ref.shape[0] > 10000
+    assert np.array_equal(mesh.tris, ref.astype(np.float32))
+    pl.close()
