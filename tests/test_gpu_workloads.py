@@ -10,7 +10,17 @@ Sequence: the first frames of the 100-frame capsule-hinge sequence through the p
 driver (kf_pipeline_process_frame, KillingFusion::processNextFrame, KillingFusion.cpp:109-175)
 at 192^3 with BF16 storage and 100 adjoint iterations per frame, stage by stage against the
 oracle run on the grids the GPU stored: integrate, warp and fuse bit for bit after the BF16
-rounding, the deform within 1e-6 voxel, and the canonical marching-cubes soup bit for bit.
+rounding, and the canonical marching-cubes soup bit for bit.  The deform is held to two bars:
+  * one step from the GPU's own field equals the oracle's step from the same field within
+    1e-6 voxel at every voxel (the kernel's FP32 arithmetic alone; measured ~4e-9);
+  * the 100-iteration trajectory from zero stays within 1e-3 voxel of the oracle's, and
+    within 1e-6 at 99% of the voxels that move.  The trajectories can part at a voxel whose
+    displacement component is within rounding of zero (or of another integer) after an
+    iteration: FP32 and FP64 then put j + u_j in different trilinear cells, whose
+    derivatives differ, and the update at that voxel jumps by ~1e-5 voxel.  Measured on
+    frame 2: after iteration 1 voxel 3,780,895 holds u_y = -6.1e-13 on the GPU and
+    +7.9e-13 in the oracle; after iteration 2 it differs by 2.9e-5 voxel, and the Killing
+    term spreads that to its neighbours (8.7e-4 voxel at most after 100 iterations).
 """
 import dataclasses
 
@@ -18,8 +28,8 @@ import numpy as np
 import pytest
 
 import oracle as O
-from killingfusion_amd import (DisplacementField, KillingFusion, KillingParams, batch_autotune, batch_deform,
-                               scenes)
+from killingfusion_amd import (SDF, DisplacementField, KillingFusion, KillingParams, batch_autotune, batch_deform,
+                               compute_displacement_field, scenes)
 
 from helpers import bf16_roundtrip, make_pair, ogrid, oparams
 
@@ -118,11 +128,30 @@ def test_sequence_192_bf16_stages_against_oracle(ctx):
         uo, so = O.adj_deform(lstored.astype(np.float64), cphi0.astype(np.float64), np.zeros((3, g.n)), g,
                               oparams(kp))
         ug = pl.field_download()
-        err, moved = u_err(ug, uo)
+        moved_mask = np.any(uo != 0, axis=0) | np.any(ug != 0, axis=0)
+        e = np.abs(ug.astype(np.float64) - uo).max(axis=0)[moved_mask]
+        # one more step from the GPU's final field, on the GPU and in the oracle
+        kw = dict(voxel_size=vox, clip_distance=trunc, max_surface=trunc, dtype="bf16")
+        sl, sc = SDF(ctx, dims, scenes.SEQ_MIN, **kw), SDF(ctx, dims, scenes.SEQ_MIN, **kw)
+        sl.upload(lstored, lw_o.astype(np.int32))
+        sc.upload(cphi0, cw0)
+        f = DisplacementField(ctx, dims)
+        f.upload(ug)
+        one = dataclasses.replace(kp, max_iters=1)
+        compute_displacement_field(sl, sc, f, one)
+        step_g = f.download().astype(np.float64)
+        step_o, _ = O.adj_deform(lstored.astype(np.float64), cphi0.astype(np.float64), ug.astype(np.float64), g,
+                                 oparams(one))
+        step_e = float(np.abs(step_g - step_o).max())
+        for h in (f, sl, sc):
+            h.close()
         print(f"frame {k}: deform iters {st['deform']['iters']} listed {st['deform']['listed']} "
-              f"moved {moved} max |u - u_oracle| {err:.3e} voxel", flush=True)
+              f"moved {moved_mask.sum()} trajectory max {e.max():.3e} p99 {np.quantile(e, 0.99):.3e} "
+              f"n>1e-6 {(e > 1e-6).sum()} | one step from the GPU's field: max {step_e:.3e} voxel", flush=True)
         assert st["deform"]["iters"] == so["iters"] == 100
-        assert err <= 1e-6 and moved > 1000
+        assert moved_mask.sum() > 1000
+        assert step_e <= 1e-6
+        assert e.max() <= 1e-3 and (e > 1e-6).sum() <= 0.01 * e.size
         # warp of the integrated live SDF by the GPU's field, then fuse into the previous canonical
         wphi, ww = lstored.astype(np.float64), lw_o.copy()
         O.warp(wphi, ww, ug.astype(np.float64), g)
