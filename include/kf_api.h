@@ -279,11 +279,27 @@ typedef struct kf_frame_stats {
     double ms_integrate, ms_deform, ms_fuse, ms_total;  /* the reference's printf columns */
 } kf_frame_stats;
 
+/* The meshes KillingFusion::processNextFrame returns (KillingFusion.cpp:171-173), as bits. */
+typedef enum kf_frame_mesh {
+    KF_MESH_LIVE = 1,         /* the frame's live SDF before the warp: currSdf->getMesh() (:157) */
+    KF_MESH_LIVE_WARPED = 2,  /* the live SDF warped into the canonical frame (:160) */
+    KF_MESH_CANONICAL = 4     /* the canonical SDF after the fuse: m_canonicalSdf->getMesh() (:170) */
+} kf_frame_mesh;
+
 kf_status kf_pipeline_create(kf_context *ctx, const kf_pipeline_desc *desc, kf_pipeline **out);
 kf_status kf_pipeline_destroy(kf_pipeline *pl);
 /* One call = one processNextFrame: frame 1 becomes the canonical SDF; later frames are
  * integrated into a fresh live SDF, deformed onto the canonical one, warped and fused. */
 kf_status kf_pipeline_process_frame(kf_pipeline *pl, const float *depth, kf_frame_stats *stats);
+/* Selects which of the reference's per-frame meshes later kf_pipeline_process_frame calls
+ * extract (a kf_frame_mesh mask; 0, the default, extracts none), at isovalue iso (the
+ * reference's 0, SDF.cpp:410).  On the first frame all three are the canonical SDF's mesh, as
+ * in the reference (KillingFusion.cpp:117-118). */
+kf_status kf_pipeline_set_meshes(kf_pipeline *pl, int32_t which, double iso);
+/* One mesh (a single kf_frame_mesh bit) of the last processed frame, as kf_marching_cubes returns
+ * a soup: *n_tris always receives the count; verts may be NULL (size query); KF_E_CAPACITY if
+ * cap_tris < count; KF_E_INVALID_ARG if that mesh was not extracted for the frame. */
+kf_status kf_pipeline_mesh(kf_pipeline *pl, int32_t which, uint64_t *n_tris, float *verts, uint64_t cap_tris);
 kf_volume *kf_pipeline_canonical(kf_pipeline *pl);
 kf_volume *kf_pipeline_live(kf_pipeline *pl);
 kf_field *kf_pipeline_field(kf_pipeline *pl);

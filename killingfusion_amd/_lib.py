@@ -15,6 +15,7 @@ KF_OK, KF_E_INVALID_ARG, KF_E_DIVERGED, KF_E_OOM, KF_E_HIP, KF_E_NO_DEVICE, KF_E
 KF_F32, KF_BF16, KF_F16 = 0, 1, 2
 KF_METHOD_REFERENCE, KF_METHOD_ADJOINT = 0, 1
 KF_SWEEP_AUTO, KF_SWEEP_DENSE, KF_SWEEP_ACTIVE = 0, 1, 2
+KF_MESH_LIVE, KF_MESH_LIVE_WARPED, KF_MESH_CANONICAL = 1, 2, 4
 
 DTYPES = {"f32": KF_F32, "bf16": KF_BF16, "f16": KF_F16}
 
@@ -113,6 +114,8 @@ SIGNATURES = {
     "kf_pipeline_create": (C.c_int, [_vp, C.POINTER(PipelineDesc), C.POINTER(_vp)]),
     "kf_pipeline_destroy": (C.c_int, [_vp]),
     "kf_pipeline_process_frame": (C.c_int, [_vp, _fp, C.POINTER(FrameStats)]),
+    "kf_pipeline_set_meshes": (C.c_int, [_vp, C.c_int32, C.c_double]),
+    "kf_pipeline_mesh": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_uint64), _fp, C.c_uint64]),
     "kf_pipeline_canonical": (_vp, [_vp]),
     "kf_pipeline_live": (_vp, [_vp]),
     "kf_pipeline_field": (_vp, [_vp]),

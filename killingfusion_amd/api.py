@@ -490,6 +490,34 @@ class KillingFusion:
         ctx._adopt(self)
         self.dims = tuple(dims)
 
+    MESHES = {"live": _lib.KF_MESH_LIVE, "live_warped": _lib.KF_MESH_LIVE_WARPED,
+              "canonical": _lib.KF_MESH_CANONICAL}
+
+    def setMeshes(self, which=("live", "live_warped", "canonical"), iso: float = 0.0):
+        """Selects the per-frame meshes processNextFrame extracts (kf_pipeline_set_meshes): any of
+        "live" (the live SDF before the warp, KillingFusion.cpp:157), "live_warped" (after it,
+        :160) and "canonical" (after the fuse, :170)."""
+        mask = 0
+        for w in which:
+            mask |= self.MESHES[w]
+        check(self._L.kf_pipeline_set_meshes(self.handle, mask, iso), "kf_pipeline_set_meshes")
+
+    def frameMesh(self, which: str) -> SimpleMesh:
+        """One mesh of the last processed frame (kf_pipeline_mesh)."""
+        bit = self.MESHES[which]
+        n = C.c_uint64(0)
+        check(self._L.kf_pipeline_mesh(self.handle, bit, C.byref(n), None, 0), "kf_pipeline_mesh")
+        out = np.empty((max(n.value, 1), 3, 3), dtype=np.float32)
+        if n.value:
+            check(self._L.kf_pipeline_mesh(self.handle, bit, C.byref(n), out.ctypes.data_as(_fp), n.value),
+                  "kf_pipeline_mesh")
+        return SimpleMesh(out[: n.value])
+
+    def frameMeshes(self):
+        """The reference's return value of processNextFrame (KillingFusion.cpp:171-173): the live
+        mesh, the warped live mesh and the canonical mesh of the last frame (all three selected)."""
+        return [self.frameMesh(w) for w in ("live", "live_warped", "canonical")]
+
     def processNextFrame(self, depth) -> dict:
         depth = _f32(depth)
         st = _lib.FrameStats()

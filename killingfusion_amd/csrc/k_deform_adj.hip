@@ -179,19 +179,6 @@ __device__ inline float4 row_group(const float *__restrict__ u, const GridDims &
     return UNAL ? row4u(u, g, c, x0, y, z) : row4(u, g, c, x0, y, z);
 }
 
-// Four stored SDF values from p, aligned to their element size only (one 8- or 16-byte load).
-template <class S>
-__device__ inline void load4_unaligned(const typename S::T *p, float scale, float o[4])
-{
-    typedef unsigned int raw_t
-        __attribute__((ext_vector_type(sizeof(typename S::T)), aligned(sizeof(typename S::T))));
-    static_assert(sizeof(raw_t) == 4 * sizeof(typename S::T), "four elements per raw load");
-    typename S::T t[4] __attribute__((aligned(16)));
-    const raw_t v = *reinterpret_cast<const raw_t *>(p);
-    __builtin_memcpy(t, &v, sizeof(t));
-    S::load4(t, 0, scale, o);
-}
-
 // Stores the group v at dst, whose row has left >= 1 voxels from dst on (the layout of
 // row_group's reads).
 template <bool UNAL>
@@ -272,7 +259,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
         if (valid[r] && !UNAL) {
             S::load4(L.p, (int64_t)g.idx32(x0, y, z), L.scale, live4[r]);
         } else if (valid[r] && x0 + VEC <= g.nx) {
-            load4_unaligned<S>(L.p + g.idx32(x0, y, z), L.scale, live4[r]);
+            kf_load4_unaligned<S>(L.p + g.idx32(x0, y, z), L.scale, live4[r]);
         } else if (valid[r]) {
             const unsigned int i0 = g.idx32(x0, y, z);
 #pragma unroll

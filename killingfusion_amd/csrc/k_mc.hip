@@ -5,11 +5,15 @@
 // appends each cell's triangles (Polygonise, MarchingCubes.h:366-422) as three fresh
 // vertices with y negated (SimpleMesh.h:66-80).  The GPU keeps that order with an ordered
 // compaction instead of atomics:
-//   1. count: each block owns CHUNK consecutive cells and sums their triangle counts;
-//   2. scan: one block turns the per-block totals into exclusive offsets;
-//   3. emit: each block recounts its cells round by round (BLOCK cells per round, in cell
-//      order), prefix-sums the counts across the block and writes every triangle at its
-//      final position.
+//   1. count: a thread classifies four consecutive cells (the 4 x 5 node values it needs
+//      come in as one vector load and one scalar per node row), a block of 256 threads owns
+//      CELLS consecutive cells and writes their triangle count;
+//   2. scan: one block turns the block counts into exclusive offsets and the total;
+//   3. emit: each block classifies its cells again, prefix-sums the thread counts over the
+//      block and writes every triangle at its final position; only the cells that hold
+//      triangles gather their eight corners and interpolate.
+// The counts and offsets of the last volume counted are kept (kf_mc_cache) until the volume
+// changes, so the reference-style size query followed by the extraction counts once.
 // Corner values are compared and interpolated in FP64 like the reference (built with
 // -ffp-contract=off), so counts, order and vertex values match an FP64 evaluation of the
 // same stored grid.
@@ -21,8 +25,11 @@
 namespace {
 
 constexpr int BLOCK = 256;
+constexpr int CPT = 4;               // cells per thread
+constexpr int CELLS = BLOCK * CPT;   // cells per block of the count and emit passes
+static_assert(CELLS == KF_MC_CELLS, "block size of the count and emit passes");
 constexpr int ITEMS = 16;
-constexpr int CHUNK = BLOCK * ITEMS;
+constexpr int FLAG_CHUNK = BLOCK * ITEMS;  // edge flags per block of the indexed variant's scan
 
 __constant__ unsigned char c_tri_count[256];
 __constant__ unsigned char c_tri_edges[256][15];
@@ -59,6 +66,22 @@ __device__ inline unsigned int block_scan_incl(unsigned int v, unsigned int *s, 
     return wbase + v;
 }
 
+// Sum of v over the block, valid in thread 0.
+__device__ inline unsigned int block_sum(unsigned int v, unsigned int *s)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0)
+        s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    unsigned int t = 0;
+#pragma unroll
+    for (int q = 0; q < BLOCK / 64; q++)
+        t += s[q];
+    return t;
+}
+
 template <class S>
 struct Cells {
     const typename S::T *phi;
@@ -69,10 +92,10 @@ struct Cells {
     {
         return g.in(x, y, z) ? (double)S::load(phi, g.idx(x, y, z), scale) : oob;
     }
-    // Returns the cube index, or -1 when the cell is skipped or has no triangles.
-    __device__ inline int classify(int64_t i, double val[8], int &x, int &y, int &z) const
+    // Cube index of the cell at (x, y, z), or -1 when the cell is skipped or has no triangles;
+    // val receives its corner values.
+    __device__ inline int classify_at(int x, int y, int z, double val[8]) const
     {
-        kf_xyz(g, i, x, y, z);
         const double d = at(x, y, z);
         if (d > max_surface || d < -clip)
             return -1;
@@ -85,42 +108,92 @@ struct Cells {
         }
         return c_edge_mask[ci] ? ci : -1;
     }
-};
-
-template <class S>
-__global__ __launch_bounds__(BLOCK) void k_mc_count(Cells<S> C, unsigned long long *block_tot)
-{
-    const int64_t base = (int64_t)blockIdx.x * CHUNK;
-    unsigned int cnt = 0;
-    for (int k = 0; k < ITEMS; k++) {
-        const int64_t i = base + (int64_t)k * BLOCK + threadIdx.x;
-        if (i < C.g.n) {
-            double val[8];
-            int x, y, z;
-            const int ci = C.classify(i, val, x, y, z);
-            if (ci >= 0)
-                cnt += c_tri_count[ci];
+    __device__ inline int classify(int64_t i, double val[8], int &x, int &y, int &z) const
+    {
+        kf_xyz(g, i, x, y, z);
+        return classify_at(x, y, z, val);
+    }
+    // Cube indices (-1: skipped, no triangles or past the grid) and coordinates of the four
+    // cells i0 .. i0 + 3.  When they lie in one row with their x + 1 corners inside it, the
+    // 4 x 5 node values come from the four node rows (y, z), (y + 1, z), (y, z + 1),
+    // (y + 1, z + 1) as one vector load and one scalar each; otherwise cell by cell.
+    __device__ inline void quad(int64_t i0, int ci[CPT], int cx[CPT], int cy[CPT], int cz[CPT]) const
+    {
+        int x0, y0, z0;
+        kf_xyz(g, i0, x0, y0, z0);
+        if (i0 + CPT <= g.n && x0 + CPT < g.nx) {
+            float v[4][CPT + 1];
+            bool in[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int y = y0 + (r & 1), z = z0 + (r >> 1);
+                in[r] = y < g.ny && z < g.nz;
+                if (in[r]) {
+                    const int64_t b = g.idx(x0, y, z);
+                    kf_load4_unaligned<S>(phi + b, scale, v[r]);
+                    v[r][CPT] = S::load(phi, b + CPT, scale);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                cx[c] = x0 + c;
+                cy[c] = y0;
+                cz[c] = z0;
+                // corner k = (x + dx, y + dy, z + dz) is node row dy + 2 dz, column c + dx
+                const double d = (double)v[0][c];
+                if (d > max_surface || d < -clip) {
+                    ci[c] = -1;
+                    continue;
+                }
+                int m = 0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int r = c_corner[k][1] + 2 * c_corner[k][2];
+                    const double val = in[r] ? (double)v[r][c + c_corner[k][0]] : oob;
+                    if (val < iso)
+                        m |= 1 << k;
+                }
+                ci[c] = c_edge_mask[m] ? m : -1;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                double val[8];
+                ci[c] = -1;
+                cx[c] = cy[c] = cz[c] = 0;
+                if (i0 + c < g.n)
+                    ci[c] = classify(i0 + c, val, cx[c], cy[c], cz[c]);
+            }
         }
     }
-    __shared__ unsigned int s[BLOCK];
-    s[threadIdx.x] = cnt;
-    __syncthreads();
-    for (int o = BLOCK / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            s[threadIdx.x] += s[threadIdx.x + o];
-        __syncthreads();
-    }
+};
+
+// Triangles per block of CELLS cells.
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_mc_count(Cells<S> C, unsigned int *__restrict__ block_tot)
+{
+    __shared__ unsigned int s[BLOCK / 64];
+    const int64_t i0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * CPT;
+    int ci[CPT], x[CPT], y[CPT], z[CPT];
+    C.quad(i0, ci, x, y, z);
+    unsigned int cnt = 0;
+#pragma unroll
+    for (int c = 0; c < CPT; c++)
+        cnt += ci[c] >= 0 ? c_tri_count[ci[c]] : 0u;
+    const unsigned int t = block_sum(cnt, s);
     if (threadIdx.x == 0)
-        block_tot[blockIdx.x] = s[0];
+        block_tot[blockIdx.x] = t;
 }
 
-// Exclusive scan of nb block totals in one block of 1024 threads; offs[nb] = total.
-__global__ __launch_bounds__(1024) void k_mc_scan(const unsigned long long *tot, unsigned long long *offs, int nb)
+// Exclusive scan of nb block totals in one block of 1024 threads; offs[nb] = total, also
+// written to *total_out when given.
+__global__ __launch_bounds__(1024) void k_mc_scan(const unsigned int *__restrict__ tot, unsigned long long *offs,
+                                                  int nb, unsigned long long *total_out)
 {
     __shared__ unsigned long long s[1024];
     const int t = threadIdx.x;
     const int per = (nb + 1023) / 1024;
-    const int b0 = t * per, b1 = min(nb, b0 + per);
+    const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
     unsigned long long sum = 0;
     for (int b = b0; b < b1; b++)
         sum += tot[b];
@@ -138,67 +211,81 @@ __global__ __launch_bounds__(1024) void k_mc_scan(const unsigned long long *tot,
         offs[b] = run;
         run += tot[b];
     }
-    if (t == 1023)
+    if (t == 1023) {
         offs[nb] = s[1023];
+        if (total_out)
+            *total_out = s[1023];
+    }
+}
+
+// VertexInterp (MarchingCubes.h:326-356) on the edges cell (x, y, z) of case ci cuts, then its
+// triangles (Polygonise, MarchingCubes.h:411-419) from position t0 on, y negated; triangles
+// at or past cap are dropped.
+template <class S>
+__device__ inline void emit_cell(const Cells<S> &C, int x, int y, int z, int ci, unsigned long long t0,
+                                 float *__restrict__ verts, unsigned long long cap)
+{
+    double val[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        val[k] = C.at(x + c_corner[k][0], y + c_corner[k][1], z + c_corner[k][2]);
+    const double hx = (double)C.g.nx / 2, hy = (double)C.g.ny / 2, hz = (double)C.g.nz / 2;
+    double p[8][3];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        p[c][0] = (double)(x + c_corner[c][0]) / hx - 1;
+        p[c][1] = (double)(y + c_corner[c][1]) / hy - 1;
+        p[c][2] = (double)(z + c_corner[c][2]) / hz - 1;
+    }
+    double vl[12][3];
+    const unsigned int em = c_edge_mask[ci];
+#pragma unroll
+    for (int e = 0; e < 12; e++) {
+        if (em & (1u << e)) {
+            const int a = c_edge_corners[e][0], b = c_edge_corners[e][1];
+            const double sc = (C.iso - val[a]) / (val[a] - val[b]);
+            vl[e][0] = p[a][0] + sc * (p[a][0] - p[b][0]);
+            vl[e][1] = p[a][1] + sc * (p[a][1] - p[b][1]);
+            vl[e][2] = p[a][2] + sc * (p[a][2] - p[b][2]);
+        }
+    }
+    const unsigned int cnt = c_tri_count[ci];
+    for (unsigned int t = 0; t < cnt; t++) {
+        const unsigned long long tri = t0 + t;
+        if (tri >= cap)
+            break;
+        float *o = verts + tri * 9;
+        for (int v = 0; v < 3; v++) {
+            const int e = c_tri_edges[ci][t * 3 + v];
+            o[v * 3 + 0] = (float)vl[e][0];
+            o[v * 3 + 1] = (float)(-vl[e][1]);
+            o[v * 3 + 2] = (float)vl[e][2];
+        }
+    }
 }
 
 template <class S>
-__global__ __launch_bounds__(BLOCK) void k_mc_emit(Cells<S> C, const unsigned long long *offs, float *verts,
-                                                   unsigned long long cap)
+__global__ __launch_bounds__(BLOCK) void k_mc_emit(Cells<S> C, const unsigned long long *__restrict__ offs,
+                                                   float *__restrict__ verts, unsigned long long cap)
 {
     __shared__ unsigned int s[BLOCK / 64];
-    const int64_t base = (int64_t)blockIdx.x * CHUNK;
-    unsigned long long run = offs[blockIdx.x];
-    const double hx = (double)C.g.nx / 2, hy = (double)C.g.ny / 2, hz = (double)C.g.nz / 2;
-    for (int k = 0; k < ITEMS; k++) {
-        const int64_t i = base + (int64_t)k * BLOCK + threadIdx.x;
-        double val[8];
-        int x = 0, y = 0, z = 0, ci = -1;
-        unsigned int cnt = 0;
-        if (i < C.g.n) {
-            ci = C.classify(i, val, x, y, z);
-            if (ci >= 0)
-                cnt = c_tri_count[ci];
-        }
-        // Block-wide scan of cnt in thread order (= cell order within the round).
-        unsigned int round_total;
-        const unsigned int incl = block_scan_incl(cnt, s, round_total);
-        if (cnt) {
-            // VertexInterp (MarchingCubes.h:326-356) on the edges this case cuts.
-            double p[8][3];
+    const int64_t i0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * CPT;
+    int ci[CPT], x[CPT], y[CPT], z[CPT];
+    C.quad(i0, ci, x, y, z);
+    unsigned int cnt = 0;
 #pragma unroll
-            for (int c = 0; c < 8; c++) {
-                p[c][0] = (double)(x + c_corner[c][0]) / hx - 1;
-                p[c][1] = (double)(y + c_corner[c][1]) / hy - 1;
-                p[c][2] = (double)(z + c_corner[c][2]) / hz - 1;
-            }
-            double vl[12][3];
-            const unsigned int em = c_edge_mask[ci];
-#pragma unroll
-            for (int e = 0; e < 12; e++) {
-                if (em & (1u << e)) {
-                    const int a = c_edge_corners[e][0], b = c_edge_corners[e][1];
-                    const double sc = (C.iso - val[a]) / (val[a] - val[b]);
-                    vl[e][0] = p[a][0] + sc * (p[a][0] - p[b][0]);
-                    vl[e][1] = p[a][1] + sc * (p[a][1] - p[b][1]);
-                    vl[e][2] = p[a][2] + sc * (p[a][2] - p[b][2]);
-                }
-            }
-            unsigned long long t0 = run + (incl - cnt);
-            for (unsigned int t = 0; t < cnt; t++) {
-                const unsigned long long tri = t0 + t;
-                if (tri >= cap)
-                    break;
-                float *o = verts + tri * 9;
-                for (int v = 0; v < 3; v++) {
-                    const int e = c_tri_edges[ci][t * 3 + v];
-                    o[v * 3 + 0] = (float)vl[e][0];
-                    o[v * 3 + 1] = (float)(-vl[e][1]);
-                    o[v * 3 + 2] = (float)vl[e][2];
-                }
-            }
-        }
-        run += round_total;
+    for (int c = 0; c < CPT; c++)
+        cnt += ci[c] >= 0 ? c_tri_count[ci[c]] : 0u;
+    unsigned int total;
+    const unsigned int incl = block_scan_incl(cnt, s, total);
+    if (cnt == 0)
+        return;
+    unsigned long long t0 = offs[blockIdx.x] + (incl - cnt);
+    for (int c = 0; c < CPT; c++) {
+        if (ci[c] < 0)
+            continue;
+        emit_cell(C, x[c], y[c], z[c], ci[c], t0, verts, cap);
+        t0 += c_tri_count[ci[c]];
     }
 }
 
@@ -234,10 +321,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc_edge_flags(Cells<S> C, EdgeGrid E,
             flags[E.id(x + c_edge_node[e][0], y + c_edge_node[e][1], z + c_edge_node[e][2], c_edge_node[e][3])] = 1;
 }
 
-// Per-block totals of CHUNK consecutive flags.
-__global__ __launch_bounds__(BLOCK) void k_flag_sums(const unsigned char *flags, int64_t n, unsigned long long *tot)
+// Per-block totals of FLAG_CHUNK consecutive flags.
+__global__ __launch_bounds__(BLOCK) void k_flag_sums(const unsigned char *flags, int64_t n, unsigned int *tot)
 {
-    const int64_t base = (int64_t)blockIdx.x * CHUNK;
+    const int64_t base = (int64_t)blockIdx.x * FLAG_CHUNK;
     unsigned int cnt = 0;
     for (int k = 0; k < ITEMS; k++) {
         const int64_t j = base + (int64_t)k * BLOCK + threadIdx.x;
@@ -261,7 +348,7 @@ __global__ __launch_bounds__(BLOCK) void k_flag_ids(const unsigned char *flags, 
                                                     const unsigned long long *offs, unsigned int *ids)
 {
     __shared__ unsigned int s[BLOCK / 64];
-    const int64_t base = (int64_t)blockIdx.x * CHUNK;
+    const int64_t base = (int64_t)blockIdx.x * FLAG_CHUNK;
     unsigned long long run = offs[blockIdx.x];
     for (int k = 0; k < ITEMS; k++) {
         const int64_t j = base + (int64_t)k * BLOCK + threadIdx.x;
@@ -327,28 +414,27 @@ __global__ __launch_bounds__(BLOCK) void k_mc_emit_indexed(Cells<S> C, EdgeGrid 
                                                            const unsigned int *ids, unsigned int *tris)
 {
     __shared__ unsigned int s[BLOCK / 64];
-    const int64_t base = (int64_t)blockIdx.x * CHUNK;
-    unsigned long long run = offs[blockIdx.x];
-    for (int k = 0; k < ITEMS; k++) {
-        const int64_t i = base + (int64_t)k * BLOCK + threadIdx.x;
-        double val[8];
-        int x = 0, y = 0, z = 0, ci = -1;
-        unsigned int cnt = 0;
-        if (i < C.g.n) {
-            ci = C.classify(i, val, x, y, z);
-            if (ci >= 0)
-                cnt = c_tri_count[ci];
-        }
-        unsigned int round_total;
-        const unsigned int incl = block_scan_incl(cnt, s, round_total);
-        const unsigned long long t0 = run + (incl - cnt);
-        for (unsigned int t = 0; t < cnt; t++)
+    const int64_t i0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * CPT;
+    int ci[CPT], x[CPT], y[CPT], z[CPT];
+    C.quad(i0, ci, x, y, z);
+    unsigned int cnt = 0;
+#pragma unroll
+    for (int c = 0; c < CPT; c++)
+        cnt += ci[c] >= 0 ? c_tri_count[ci[c]] : 0u;
+    unsigned int total;
+    const unsigned int incl = block_scan_incl(cnt, s, total);
+    unsigned long long t0 = offs[blockIdx.x] + (incl - cnt);
+    for (int c = 0; c < CPT; c++) {
+        if (ci[c] < 0)
+            continue;
+        const unsigned int n = c_tri_count[ci[c]];
+        for (unsigned int t = 0; t < n; t++)
             for (int v = 0; v < 3; v++) {
-                const int e = c_tri_edges[ci][t * 3 + v];
-                tris[3 * (t0 + t) + v] =
-                    ids[E.id(x + c_edge_node[e][0], y + c_edge_node[e][1], z + c_edge_node[e][2], c_edge_node[e][3])];
+                const int e = c_tri_edges[ci[c]][t * 3 + v];
+                tris[3 * (t0 + t) + v] = ids[E.id(x[c] + c_edge_node[e][0], y[c] + c_edge_node[e][1],
+                                                  z[c] + c_edge_node[e][2], c_edge_node[e][3])];
             }
-        run += round_total;
+        t0 += n;
     }
 }
 
@@ -413,56 +499,145 @@ kf_status kf_mc_init_tables()
     return KF_OK;
 }
 
+int64_t kf_mc_blocks(int64_t n) { return (n + CELLS - 1) / CELLS; }
+
+kf_status kf_mc_launch_into(const kf_volume *v, double iso, unsigned int *tot, unsigned long long *offs,
+                            unsigned long long *total_out, float *verts, uint64_t cap)
+{
+    const int nb = (int)kf_mc_blocks(v->n);
+    hipStream_t st = v->ctx->stream;
+    KF_DISPATCH_DTYPE(v->dtype, S, {
+        const Cells<S> C = make_cells<S>(v, iso);
+        hipLaunchKernelGGL((k_mc_count<S>), dim3(nb), dim3(BLOCK), 0, st, C, tot);
+        hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, st, (const unsigned int *)tot, offs, nb, total_out);
+        if (cap > 0)
+            hipLaunchKernelGGL((k_mc_emit<S>), dim3(nb), dim3(BLOCK), 0, st, C, (const unsigned long long *)offs,
+                               verts, (unsigned long long)cap);
+    });
+    KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
+namespace {
+
+int mc_blocks(const kf_volume *v) { return (int)kf_mc_blocks(v->n); }
+
+// Counts v's triangles into the context's cache (block offsets on the device, the total on
+// the host), unless the cache already holds this version of v at this iso value.
+kf_status mc_count_cached(const kf_volume *v, double iso, uint64_t *total)
+{
+    kf_context *ctx = v->ctx;
+    kf_mc_cache &c = ctx->mc;
+    if (c.valid && c.vol == v && c.version == v->version && c.iso == iso) {
+        *total = *c.total_host;
+        return KF_OK;
+    }
+    c.valid = false;
+    const int nb = mc_blocks(v);
+    if ((size_t)nb > c.cap) {
+        KF_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipFree(c.tot);
+        (void)hipFree(c.offs);
+        c.tot = nullptr;
+        c.offs = nullptr;
+        c.cap = 0;
+        KF_HIP(hipMalloc((void **)&c.tot, sizeof(unsigned int) * (size_t)nb));
+        KF_HIP(hipMalloc((void **)&c.offs, sizeof(unsigned long long) * ((size_t)nb + 1)));
+        c.cap = (size_t)nb;
+    }
+    if (!c.total_host)
+        KF_HIP(hipHostMalloc((void **)&c.total_host, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+    kf_status s = kf_mc_launch_into(v, iso, c.tot, c.offs, nullptr, nullptr, 0);
+    if (s != KF_OK)
+        return s;
+    KF_HIP(hipMemcpyAsync(c.total_host, c.offs + nb, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+    c.vol = v;
+    c.version = v->version;
+    c.iso = iso;
+    c.valid = true;
+    *total = *c.total_host;
+    return KF_OK;
+}
+
+// Emits the cached count's triangles of v (mc_count_cached just ran for it) into dverts.
+kf_status mc_emit_cached(const kf_volume *v, double iso, float *dverts, uint64_t cap)
+{
+    kf_context *ctx = v->ctx;
+    const int nb = mc_blocks(v);
+    KF_DISPATCH_DTYPE(v->dtype, S, {
+        hipLaunchKernelGGL((k_mc_emit<S>), dim3(nb), dim3(BLOCK), 0, ctx->stream, make_cells<S>(v, iso),
+                           (const unsigned long long *)ctx->mc.offs, dverts, (unsigned long long)cap);
+    });
+    KF_LAUNCH_CHECK();
+    return KF_OK;
+}
+
+kf_status grow_verts(float **buf, size_t *cap, uint64_t need, hipStream_t st)
+{
+    if (need <= *cap)
+        return KF_OK;
+    if (*buf) {
+        KF_HIP(hipStreamSynchronize(st));
+        KF_HIP(hipFree(*buf));
+        *buf = nullptr;
+        *cap = 0;
+    }
+    KF_HIP(hipMalloc((void **)buf, sizeof(float) * 9 * (size_t)need));
+    *cap = (size_t)need;
+    return KF_OK;
+}
+
+}  // namespace
+
+void kf_mc_release(kf_context *ctx)
+{
+    kf_mc_cache &c = ctx->mc;
+    (void)hipFree(c.tot);
+    (void)hipFree(c.offs);
+    (void)hipFree(c.verts);
+    if (c.total_host)
+        (void)hipHostFree(c.total_host);
+    c = kf_mc_cache{};
+}
+
 kf_status kf_mc_run(const kf_volume *v, double iso, uint64_t *n_tris, float *verts_host, uint64_t cap)
 {
     kf_context *ctx = v->ctx;
-    const int nb = (int)((v->n + CHUNK - 1) / CHUNK);
-    unsigned long long *tot = nullptr, *offs = nullptr;
-    KF_HIP(hipMallocAsync((void **)&tot, sizeof(unsigned long long) * (size_t)(2 * nb + 1), ctx->stream));
-    offs = tot + nb;
-    kf_status st = KF_OK;
-    unsigned long long total = 0;
-    KF_DISPATCH_DTYPE(v->dtype, S, {
-        Cells<S> C = make_cells<S>(v, iso);
-        hipLaunchKernelGGL((k_mc_count<S>), dim3(nb), dim3(BLOCK), 0, ctx->stream, C, tot);
-        hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, ctx->stream, (const unsigned long long *)tot, offs, nb);
-        hipError_t e = hipMemcpyAsync(&total, offs + nb, sizeof(total), hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess)
-            e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) {
-            kf_set_error(std::string("marching cubes count: ") + hipGetErrorString(e));
-            st = KF_E_HIP;
-        }
-        if (st == KF_OK) {
-            *n_tris = total;
-            if (verts_host && total > 0) {
-                if (cap < total) {
-                    kf_set_error("marching cubes: capacity below triangle count");
-                    st = KF_E_CAPACITY;
-                } else {
-                    float *dverts = nullptr;
-                    e = hipMallocAsync((void **)&dverts, sizeof(float) * 9 * (size_t)total, ctx->stream);
-                    if (e == hipSuccess) {
-                        hipLaunchKernelGGL((k_mc_emit<S>), dim3(nb), dim3(BLOCK), 0, ctx->stream, C,
-                                           (const unsigned long long *)offs, dverts, (unsigned long long)total);
-                        e = hipGetLastError();
-                        if (e == hipSuccess)
-                            e = hipMemcpyAsync(verts_host, dverts, sizeof(float) * 9 * (size_t)total,
-                                               hipMemcpyDeviceToHost, ctx->stream);
-                        (void)hipFreeAsync(dverts, ctx->stream);
-                        if (e == hipSuccess)
-                            e = hipStreamSynchronize(ctx->stream);
-                    }
-                    if (e != hipSuccess) {
-                        kf_set_error(std::string("marching cubes emit: ") + hipGetErrorString(e));
-                        st = e == hipErrorOutOfMemory ? KF_E_OOM : KF_E_HIP;
-                    }
-                }
-            }
-        }
-    });
-    (void)hipFreeAsync(tot, ctx->stream);
-    return st;
+    uint64_t total = 0;
+    kf_status s = mc_count_cached(v, iso, &total);
+    if (s != KF_OK)
+        return s;
+    *n_tris = total;
+    if (!verts_host || total == 0)
+        return KF_OK;
+    if (cap < total) {
+        kf_set_error("marching cubes: capacity below triangle count");
+        return KF_E_CAPACITY;
+    }
+    kf_mc_cache &c = ctx->mc;
+    if ((s = grow_verts(&c.verts, &c.verts_cap, total, ctx->stream)) != KF_OK)
+        return s;
+    if ((s = mc_emit_cached(v, iso, c.verts, total)) != KF_OK)
+        return s;
+    KF_HIP(hipMemcpyAsync(verts_host, c.verts, sizeof(float) * 9 * (size_t)total, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+    return KF_OK;
+}
+
+kf_status kf_mc_to_device(const kf_volume *v, double iso, float **dverts, size_t *dverts_cap, uint64_t *n_tris)
+{
+    uint64_t total = 0;
+    kf_status s = mc_count_cached(v, iso, &total);
+    if (s != KF_OK)
+        return s;
+    *n_tris = total;
+    if (total == 0)
+        return KF_OK;
+    if ((s = grow_verts(dverts, dverts_cap, total, v->ctx->stream)) != KF_OK)
+        return s;
+    return mc_emit_cached(v, iso, *dverts, total);
 }
 
 kf_status kf_mc_run_indexed(const kf_volume *v, double iso, uint64_t *n_verts, uint64_t *n_tris, float *verts_host,
@@ -470,45 +645,46 @@ kf_status kf_mc_run_indexed(const kf_volume *v, double iso, uint64_t *n_verts, u
 {
     kf_context *ctx = v->ctx;
     hipStream_t st = ctx->stream;
-    const int nb = (int)((v->n + CHUNK - 1) / CHUNK);
+    uint64_t ntri = 0;
+    kf_status status = mc_count_cached(v, iso, &ntri);
+    if (status != KF_OK)
+        return status;
+    const unsigned long long *toffs = ctx->mc.offs;
     EdgeGrid E{(int64_t)v->dims[0] + 1, (int64_t)v->dims[1] + 1, (int64_t)v->dims[2] + 1};
     const int64_t ne = E.count();
-    const int nbe = (int)((ne + CHUNK - 1) / CHUNK);
-    // One scratch block: triangle block totals/offsets, edge block totals/offsets, edge flags.
-    const size_t tri_words = (size_t)(2 * nb + 1), edge_words = (size_t)(2 * nbe + 1);
-    char *scratch = nullptr;
-    KF_HIP(hipMallocAsync((void **)&scratch, 8 * (tri_words + edge_words) + (size_t)ne, st));
-    unsigned long long *ttot = (unsigned long long *)scratch, *toffs = ttot + nb;
-    unsigned long long *etot = ttot + tri_words, *eoffs = etot + nbe;
-    unsigned char *flags = (unsigned char *)(etot + edge_words);
-    kf_status status = KF_OK;
-    unsigned long long counts[2] = {0, 0};
+    const int nbe = (int)((ne + FLAG_CHUNK - 1) / FLAG_CHUNK);
+    // One scratch block: edge block totals (u32) and offsets (u64), edge flags.
+    const size_t tot_bytes = ((size_t)nbe * 4 + 7) / 8 * 8;
+    char *scr = nullptr;
+    KF_HIP(hipMallocAsync((void **)&scr, tot_bytes + 8 * ((size_t)nbe + 1) + (size_t)ne, st));
+    unsigned int *etot = (unsigned int *)scr;
+    unsigned long long *eoffs = (unsigned long long *)(scr + tot_bytes);
+    unsigned char *flags = (unsigned char *)(eoffs + nbe + 1);
+    unsigned long long nvert = 0;
+    unsigned long long *nvert_host = ctx->mc.total_host + 1;  // pinned
     hipError_t e = hipSuccess;
     unsigned int *ids = nullptr, *dtris = nullptr;
     float *dverts = nullptr, *dnorm = nullptr;
     KF_DISPATCH_DTYPE(v->dtype, S, {
         Cells<S> C = make_cells<S>(v, iso);
-        hipLaunchKernelGGL((k_mc_count<S>), dim3(nb), dim3(BLOCK), 0, st, C, ttot);
-        hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, st, (const unsigned long long *)ttot, toffs, nb);
         e = hipMemsetAsync(flags, 0, (size_t)ne, st);
         if (e == hipSuccess) {
             hipLaunchKernelGGL((k_mc_edge_flags<S>), dim3((unsigned int)((v->n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
                                st, C, E, flags);
             hipLaunchKernelGGL(k_flag_sums, dim3(nbe), dim3(BLOCK), 0, st, (const unsigned char *)flags, ne, etot);
-            hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, st, (const unsigned long long *)etot, eoffs, nbe);
+            hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, st, (const unsigned int *)etot, eoffs, nbe,
+                               (unsigned long long *)nullptr);
             e = hipGetLastError();
         }
         if (e == hipSuccess)
-            e = hipMemcpyAsync(&counts[0], toffs + nb, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(&counts[1], eoffs + nbe, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+            e = hipMemcpyAsync(nvert_host, eoffs + nbe, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess)
             e = hipStreamSynchronize(st);
         if (e != hipSuccess) {
             kf_set_error(std::string("marching cubes (indexed) count: ") + hipGetErrorString(e));
             status = KF_E_HIP;
         }
-        const unsigned long long ntri = counts[0], nvert = counts[1];
+        nvert = *nvert_host;
         if (status == KF_OK) {
             *n_tris = ntri;
             *n_verts = nvert;
@@ -533,8 +709,8 @@ kf_status kf_mc_run_indexed(const kf_volume *v, double iso, uint64_t *n_verts, u
                     hipLaunchKernelGGL((k_mc_vertices<S>), dim3((unsigned int)((ne + BLOCK - 1) / BLOCK)), dim3(BLOCK),
                                        0, st, C, E, (const unsigned char *)flags, (const unsigned int *)ids, dverts,
                                        dnorm);
-                    hipLaunchKernelGGL((k_mc_emit_indexed<S>), dim3(nb), dim3(BLOCK), 0, st, C, E,
-                                       (const unsigned long long *)toffs, (const unsigned int *)ids, dtris);
+                    hipLaunchKernelGGL((k_mc_emit_indexed<S>), dim3(mc_blocks(v)), dim3(BLOCK), 0, st, C, E, toffs,
+                                       (const unsigned int *)ids, dtris);
                     e = hipGetLastError();
                 }
                 if (e == hipSuccess && verts_host)
@@ -557,6 +733,6 @@ kf_status kf_mc_run_indexed(const kf_volume *v, double iso, uint64_t *n_verts, u
     for (void *ptr : {(void *)ids, (void *)dverts, (void *)dnorm, (void *)dtris})
         if (ptr)
             (void)hipFreeAsync(ptr, st);
-    (void)hipFreeAsync(scratch, st);
+    (void)hipFreeAsync(scr, st);
     return status;
 }

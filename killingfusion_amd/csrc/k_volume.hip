@@ -263,6 +263,7 @@ Reader<S> make_reader(const kf_volume *v)
 
 kf_status kf_launch_fill_volume(kf_volume *v)
 {
+    kf_touch(v);
     KF_DISPATCH_DTYPE(v->dtype, S, {
         hipLaunchKernelGGL((k_fill<S>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream,
                            (typename S::T *)v->phi, v->w, v->n, (float)v->oob(), v->scale());
@@ -273,6 +274,7 @@ kf_status kf_launch_fill_volume(kf_volume *v)
 
 kf_status kf_launch_upload_volume(kf_volume *v, const float *dphi, const int32_t *dw)
 {
+    kf_touch(v);
     (void)dw;
     KF_DISPATCH_DTYPE(v->dtype, S, {
         hipLaunchKernelGGL((k_encode<S>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream,
@@ -295,6 +297,7 @@ kf_status kf_launch_download_volume(const kf_volume *v, float *dphi)
 kf_status kf_launch_integrate(kf_volume *v, const float *ddepth, int w, int h, const double K[9],
                               const double W2C[16], double min_depth, double max_depth, bool fresh)
 {
+    kf_touch(v);
     Mat9 k;
     Mat16 m;
     for (int i = 0; i < 9; i++)
@@ -319,6 +322,7 @@ kf_status kf_launch_integrate(kf_volume *v, const float *ddepth, int w, int h, c
 
 kf_status kf_launch_warp(kf_volume *v, const kf_field *u, void *tmp_phi, int32_t *tmp_w)
 {
+    kf_touch(v);
     KF_DISPATCH_DTYPE(v->dtype, S, {
         hipLaunchKernelGGL((k_warp<S>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream, make_reader<S>(v),
                            (const float *)u->u(), (typename S::T *)tmp_phi, tmp_w);
@@ -329,6 +333,7 @@ kf_status kf_launch_warp(kf_volume *v, const kf_field *u, void *tmp_phi, int32_t
 
 kf_status kf_launch_fuse(kf_volume *can, const kf_volume *live)
 {
+    kf_touch(can);
     KF_DISPATCH_DTYPE(can->dtype, S1, {
         KF_DISPATCH_DTYPE(live->dtype, S2, {
             hipLaunchKernelGGL((k_fuse<S1, S2>), dim3(blocks_for(can->n)), dim3(BLOCK), 0, can->ctx->stream,
@@ -342,6 +347,7 @@ kf_status kf_launch_fuse(kf_volume *can, const kf_volume *live)
 
 kf_status kf_launch_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *u)
 {
+    kf_touch(can);
     KF_DISPATCH_DTYPE(can->dtype, S1, {
         KF_DISPATCH_DTYPE(live->dtype, S2, {
             hipLaunchKernelGGL((k_fuse_warped<S1, S2>), dim3(blocks_for(can->n)), dim3(BLOCK), 0,
@@ -374,6 +380,7 @@ namespace {
 // Warps the count voxels of ctx->active.list in place (k_warp_list, then k_warp_scatter).
 kf_status warp_listed(kf_volume *v, const kf_field *u, unsigned int count)
 {
+    kf_touch(v);
     kf_context *ctx = v->ctx;
     kf_active_ws &a = ctx->active;
     if (count > 0) {

@@ -2,6 +2,7 @@
 // optimiser driver.  Kernels live in k_*.hip.
 #include "kf_reduce.hpp"
 
+#include <atomic>
 #include <climits>
 #include <cstring>
 #include <string>
@@ -108,6 +109,12 @@ kf_status read_state(kf_context *ctx)
 }  // namespace
 
 void kf_set_error(const std::string &msg) { g_last_error = msg; }
+
+uint64_t kf_next_version()
+{
+    static std::atomic<uint64_t> counter{0};
+    return counter.fetch_add(1, std::memory_order_relaxed) + 1;
+}
 
 bool kf_valid_params(const kf_params *p) { return valid_params_impl(p); }
 
@@ -282,6 +289,7 @@ kf_status kf_context_destroy(kf_context *ctx)
     kf_active_release(ctx);
     kf_ref_release(ctx);
     kf_batch_release(ctx);
+    kf_mc_release(ctx);
     if (ctx->own_stream && ctx->stream)
         (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -446,6 +454,7 @@ kf_status kf_volume_copy(kf_volume *dst, const kf_volume *src)
     KF_CHECK_ARG(dst->ctx == src->ctx, "handles belong to different contexts");
     DeviceGuard dg(dst->ctx->device);
     hipStream_t s = dst->ctx->stream;
+    kf_touch(dst);
     KF_HIP(hipMemcpyAsync(dst->phi, src->phi, dst->elem_size() * (size_t)dst->n, hipMemcpyDeviceToDevice, s));
     KF_HIP(hipMemcpyAsync(dst->w, src->w, sizeof(int32_t) * (size_t)dst->n, hipMemcpyDeviceToDevice, s));
     return KF_OK;
@@ -808,6 +817,7 @@ kf_status kf_warp(kf_volume *v, const kf_field *u)
     (void)hipFreeAsync(v->w, s);
     v->phi = np;
     v->w = nw;
+    kf_touch(v);
     return KF_OK;
 }
 

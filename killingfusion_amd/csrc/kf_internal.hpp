@@ -124,6 +124,21 @@ struct kf_ref_ws {
     unsigned long long *aux = nullptr;  // segment lengths of launch parity 0 and 1, then the non-finite flag
 };
 
+// Marching-cubes block offsets of the last volume counted (k_mc.hip): a size query followed by
+// the extraction of the same, unchanged volume counts once.
+struct kf_mc_cache {
+    const kf_volume *vol = nullptr;
+    uint64_t version = 0;
+    double iso = 0.0;
+    bool valid = false;
+    unsigned int *tot = nullptr;           // triangles per block
+    unsigned long long *offs = nullptr;    // exclusive block offsets, then the total
+    size_t cap = 0;                        // blocks
+    unsigned long long *total_host = nullptr;  // pinned: the cached triangle total, then a spare slot
+    float *verts = nullptr;                // device staging of a soup copied to the host
+    size_t verts_cap = 0;                  // triangles
+};
+
 struct kf_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -136,6 +151,7 @@ struct kf_context {
     struct kf_batch_cache *batch = nullptr;  // batched-deform workspace, tuned shapes, captured graph
     kf_active_ws active;                     // active-set sweep workspace (k_deform_active.hip)
     kf_ref_ws ref;                           // reference-formula step workspace (k_deform_ref.hip)
+    kf_mc_cache mc;                          // marching-cubes counts (k_mc.hip)
 };
 
 struct kf_volume {
@@ -147,6 +163,9 @@ struct kf_volume {
     kf_dtype dtype = KF_F32;
     void *phi = nullptr;     // device: n elements of dtype
     int32_t *w = nullptr;    // device: n
+    // Changes whenever work that writes phi is enqueued (kf_touch); unique across volumes, so
+    // results cached for one version (marching-cubes counts) are never taken for another.
+    uint64_t version = 0;
     double oob() const { return max_surface + epsilon; }
     // Decode scale: stored value * scale = metres (1 for F32).
     float scale() const { return dtype == KF_F32 ? 1.0f : (float)(max_surface + epsilon); }
@@ -233,6 +252,20 @@ struct StoreF16 {
     __device__ __host__ static inline T enc(float v, float scale) { return (T)(v / scale); }
 };
 
+// Four consecutive stored SDF values from p, decoded, where p is aligned to its element size
+// only: one 8-byte (BF16/F16) or 16-byte (F32) load.
+template <class S>
+__device__ inline void kf_load4_unaligned(const typename S::T *p, float scale, float o[4])
+{
+    typedef unsigned int raw_t
+        __attribute__((ext_vector_type(sizeof(typename S::T)), aligned(sizeof(typename S::T))));
+    static_assert(sizeof(raw_t) == 4 * sizeof(typename S::T), "four elements per raw load");
+    typename S::T t[4] __attribute__((aligned(16)));
+    const raw_t v = *reinterpret_cast<const raw_t *>(p);
+    __builtin_memcpy(t, &v, sizeof(t));
+    S::load4(t, 0, scale, o);
+}
+
 // Dispatch a templated functor on a volume's dtype.
 #define KF_DISPATCH_DTYPE(dt, S, ...)                                                         \
     switch (dt) {                                                                             \
@@ -263,6 +296,10 @@ struct GridDims {
         return ((unsigned int)z * (unsigned int)ny + (unsigned int)y) * (unsigned int)nx + (unsigned int)x;
     }
 };
+
+// A fresh version for a volume whose phi is about to change (see kf_volume::version).
+uint64_t kf_next_version();
+inline void kf_touch(kf_volume *v) { v->version = kf_next_version(); }
 
 inline GridDims grid_of(const kf_volume *v) { return GridDims{v->dims[0], v->dims[1], v->dims[2], v->n}; }
 inline GridDims grid_of(const kf_field *f) { return GridDims{f->dims[0], f->dims[1], f->dims[2], f->n}; }
@@ -453,6 +490,19 @@ size_t kf_adj_mask_words(const int32_t dims[3]);
 kf_status kf_launch_adj_mask(const kf_field *f, const float *u, unsigned int *mask);
 
 kf_status kf_mc_run(const kf_volume *v, double iso, uint64_t *n_tris, float *verts_host, uint64_t cap);
+// The triangle soup of v into device memory: *dverts (capacity *dverts_cap triangles, grown with
+// hipMalloc when short), *n_tris the count.  Stream-ordered; one host wait for the count.
+kf_status kf_mc_to_device(const kf_volume *v, double iso, float **dverts, size_t *dverts_cap, uint64_t *n_tris);
+void kf_mc_release(kf_context *ctx);
+// Cells per block of the marching-cubes count and emit passes (one block total and one offset
+// each): a grid of n voxels has kf_mc_blocks(n) = ceil(n / KF_MC_CELLS) blocks.
+constexpr int KF_MC_CELLS = 1024;
+int64_t kf_mc_blocks(int64_t n);
+// Count, scan and (cap > 0) emit of v's triangle soup with caller-provided device buffers:
+// tot[kf_mc_blocks(n)], offs[kf_mc_blocks(n) + 1], verts[9 cap]; *total_out (may be null, device)
+// receives the triangle count.  Stream-ordered kernels only: no allocation, no host wait.
+kf_status kf_mc_launch_into(const kf_volume *v, double iso, unsigned int *tot, unsigned long long *offs,
+                            unsigned long long *total_out, float *verts, uint64_t cap);
 kf_status kf_mc_run_indexed(const kf_volume *v, double iso, uint64_t *n_verts, uint64_t *n_tris, float *verts_host,
                             float *normals_host, uint32_t *tris_host, uint64_t cap_verts, uint64_t cap_tris);
 

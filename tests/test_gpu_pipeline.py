@@ -57,3 +57,43 @@ def test_pipeline_matches_oracle_stages(ctx, method):
     tol = 1e-5 if method == "reference" else 5e-5  # metres
     assert e[same].max() <= tol
     assert all(s["ms_total"] > 0 for s in stats)
+
+
+def test_pipeline_frame_meshes_match_oracle(ctx):
+    """The reference's three per-frame meshes (KillingFusion.cpp:157, :160, :170): the live SDF
+    before the warp, after it, and the canonical SDF after the fuse, each bit-exact against the
+    oracle's marching cubes on that stage's grid; on the first frame all three are the canonical
+    mesh (:117-118)."""
+    print("case: pipeline meshes, 3 frames of the 64^3 sphere pair, adjoint method", flush=True)
+    d1, d2 = sphere_frames()
+    kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint", max_iters=20)
+    pl = KillingFusion(ctx, C1["dims"], C1["min3d"], C1["voxel"], kp, scenes.K)
+    g = ogrid(C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"])
+    trunc = C1["trunc"]
+
+    def ref_mesh(phi):
+        return O.marching_cubes(np.asarray(phi, np.float64), g, trunc, trunc).astype(np.float32)
+
+    with pytest.raises(Exception):
+        pl.frameMesh("live")  # nothing extracted yet
+    pl.setMeshes()
+    for k, d in enumerate([d1, d2, d1]):
+        pl.processNextFrame(d)
+        live_m, warped_m, can_m = pl.frameMeshes()
+        cphi, _ = pl.canonical_download()
+        assert np.array_equal(can_m.tris, ref_mesh(cphi)) and can_m.tris.shape[0] > 1000, k
+        if k == 0:
+            assert np.array_equal(live_m.tris, can_m.tris) and np.array_equal(warped_m.tris, can_m.tris)
+            continue
+        lphi, lw = O.sdf_init(g)
+        O.integrate(lphi, lw, g, trunc, d, scenes.K, scenes.C2W, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+        assert np.array_equal(live_m.tris, ref_mesh(lphi.astype(np.float32))) and live_m.tris.shape[0] > 1000, k
+        wphi, _ = pl.live_download()
+        assert np.array_equal(warped_m.tris, ref_mesh(wphi)), k
+        assert not np.array_equal(warped_m.tris, live_m.tris)  # the warp moved the surface
+    pl.setMeshes(("canonical",))
+    pl.processNextFrame(d2)
+    assert pl.frameMesh("canonical").tris.shape[0] > 1000
+    with pytest.raises(Exception):
+        pl.frameMesh("live")
+    pl.close()

@@ -10,7 +10,8 @@ Sequence: the first frames of the 100-frame capsule-hinge sequence through the p
 driver (kf_pipeline_process_frame, KillingFusion::processNextFrame, KillingFusion.cpp:109-175)
 at 192^3 with BF16 storage and 100 adjoint iterations per frame, stage by stage against the
 oracle run on the grids the GPU stored: integrate, warp and fuse bit for bit after the BF16
-rounding, and the canonical marching-cubes soup bit for bit.  The deform is held to two bars:
+rounding, and the reference's three per-frame meshes (the live SDF before and after the warp,
+the canonical SDF) bit for bit.  The deform is held to two bars:
   * one step from the GPU's own field equals the oracle's step from the same field within
     1e-6 voxel at every voxel (the kernel's FP32 arithmetic alone; measured ~4e-9);
   * the 100-iteration trajectory from zero stays within 1e-3 voxel of the oracle's, and
@@ -108,9 +109,13 @@ def test_sequence_192_bf16_stages_against_oracle(ctx):
     g = ogrid(dims, scenes.SEQ_MIN, vox, trunc)
     kp = KillingParams.paper(vox, 3.0, method="adjoint", max_iters=100)
     pl = KillingFusion(ctx, dims, scenes.SEQ_MIN, vox, kp, scenes.K, dtype="bf16")
+    pl.setMeshes()
 
     def stored(phi64):
         return bf16_roundtrip(np.asarray(phi64, np.float64).astype(np.float32), scale)
+
+    def mc(phi):
+        return O.marching_cubes(np.asarray(phi, np.float64), g, trunc, trunc).astype(np.float32)
 
     for k in range(SEQ_FRAMES):
         depth = scenes.hinge_frame_exact(k / 99.0, 0x5EED, k)
@@ -120,10 +125,13 @@ def test_sequence_192_bf16_stages_against_oracle(ctx):
         lphi_o, lw_o = O.sdf_init(g)
         O.integrate(lphi_o, lw_o, g, trunc, depth, scenes.K, scenes.C2W, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
         cphi, cw = pl.canonical_download()
+        live_m, warped_m, can_m = pl.frameMeshes()
+        assert np.array_equal(can_m.tris, mc(cphi)) and can_m.tris.shape[0] > 10000, f"frame {k}: canonical mesh"
         if k == 0:
             assert (cw == lw_o).all() and np.array_equal(cphi, stored(lphi_o)), "frame 0 integration"
             continue
         lstored = stored(lphi_o)
+        assert np.array_equal(live_m.tris, mc(lstored)), f"frame {k}: live mesh before the warp"
         # deform on the stored grids: live (as integrated) onto the previous canonical
         uo, so = O.adj_deform(lstored.astype(np.float64), cphi0.astype(np.float64), np.zeros((3, g.n)), g,
                               oparams(kp))
@@ -158,13 +166,12 @@ def test_sequence_192_bf16_stages_against_oracle(ctx):
         gl_phi, gl_w = pl.live_download()
         assert (gl_w == ww).all(), f"frame {k}: warp weights"
         assert np.array_equal(gl_phi, stored(wphi)), f"frame {k}: warp"
+        assert np.array_equal(warped_m.tris, mc(gl_phi)), f"frame {k}: warped live mesh"
         fphi, fw = cphi0.astype(np.float64), cw0.astype(np.int64)
         O.fuse(fphi, fw, gl_phi.astype(np.float64), gl_w.astype(np.int64))
         assert (cw == fw).all(), f"frame {k}: fuse weights"
         assert np.array_equal(cphi, stored(fphi)), f"frame {k}: fuse"
     mesh = pl.canonical_mesh()
-    ref = O.marching_cubes(cphi.astype(np.float64), g, trunc, trunc)
     print(f"canonical mesh after {SEQ_FRAMES} frames: {mesh.tris.shape[0]} triangles", flush=True)
-    assert mesh.tris.shape[0] == ref.shape[0] > 10000
-    assert np.array_equal(mesh.tris, ref.astype(np.float32))
+    assert np.array_equal(mesh.tris, mc(cphi)) and mesh.tris.shape[0] > 10000
     pl.close()
