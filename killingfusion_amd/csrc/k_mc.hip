@@ -5,9 +5,9 @@
 // appends each cell's triangles (Polygonise, MarchingCubes.h:366-422) as three fresh
 // vertices with y negated (SimpleMesh.h:66-80).  The GPU keeps that order with an ordered
 // compaction instead of atomics:
-//   1. count: a thread classifies four consecutive cells (the 4 x 5 node values it needs
-//      come in as one vector load and one scalar per node row), a block of 256 threads owns
-//      CELLS consecutive cells and writes their triangle count;
+//   1. count: a thread classifies CPT consecutive cells (the node values it needs come in as
+//      vector loads, all issued before any is used), a block of 256 threads owns CELLS
+//      consecutive cells and writes their triangle count;
 //   2. scan: one block turns the block counts into exclusive offsets and the total;
 //   3. emit: each block classifies its cells again, prefix-sums the thread counts over the
 //      block and writes every triangle at its final position; only the cells that hold
@@ -25,17 +25,21 @@
 namespace {
 
 constexpr int BLOCK = 256;
-constexpr int CPT = 4;               // cells per thread
+constexpr int CPT = 8;               // cells per thread (a multiple of 4)
 constexpr int CELLS = BLOCK * CPT;   // cells per block of the count and emit passes
 static_assert(CELLS == KF_MC_CELLS, "block size of the count and emit passes");
 constexpr int ITEMS = 16;
 constexpr int FLAG_CHUNK = BLOCK * ITEMS;  // edge flags per block of the indexed variant's scan
 
 __constant__ unsigned char c_tri_count[256];
-__constant__ unsigned char c_tri_edges[256][15];
+__constant__ __align__(16) unsigned char c_tri_edges[256][15];
 __constant__ unsigned short c_edge_mask[256];
 
-// Corner k of cell (x, y, z): ProcessVolumeCell, SDF.cpp:338-355.
+// Corner k of cell (x, y, z): ProcessVolumeCell, SDF.cpp:338-355 (compile-time copies for
+// the unrolled loops of the count and emit passes).
+constexpr int kCornerX[8] = {1, 0, 0, 1, 1, 0, 0, 1};
+constexpr int kCornerY[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+constexpr int kCornerZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 __constant__ signed char c_corner[8][3] = {{1, 0, 0}, {0, 0, 0}, {0, 1, 0}, {1, 1, 0},
                                            {1, 0, 1}, {0, 0, 1}, {0, 1, 1}, {1, 1, 1}};
 __constant__ unsigned char c_edge_corners[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
@@ -106,33 +110,46 @@ struct Cells {
             if (val[k] < iso)
                 ci |= 1 << k;
         }
-        return c_edge_mask[ci] ? ci : -1;
+        return ci == 0 || ci == 255 ? -1 : ci;  // every other case cuts some edge
     }
     __device__ inline int classify(int64_t i, double val[8], int &x, int &y, int &z) const
     {
         kf_xyz(g, i, x, y, z);
         return classify_at(x, y, z, val);
     }
-    // Cube indices (-1: skipped, no triangles or past the grid) and coordinates of the four
-    // cells i0 .. i0 + 3.  When they lie in one row with their x + 1 corners inside it, the
-    // 4 x 5 node values come from the four node rows (y, z), (y + 1, z), (y, z + 1),
-    // (y + 1, z + 1) as one vector load and one scalar each; otherwise cell by cell.
+    // The value at node (x, y, z), or oob outside the grid, with the load issued whatever the
+    // node (from node 0 when outside): a caller's loads are then all in flight together
+    // instead of one round trip per bounds test.
+    __device__ inline double at_flat(int x, int y, int z) const
+    {
+        const bool inside = g.in(x, y, z);
+        const double v = (double)S::load(phi, inside ? g.idx(x, y, z) : 0, scale);
+        return inside ? v : oob;
+    }
+    // Cube indices (-1: skipped, no triangles or past the grid) and coordinates of the CPT
+    // cells i0 .. i0 + CPT - 1.  When they lie in one row, the 4 x (CPT + 1) node values come
+    // from the four node rows (y, z), (y + 1, z), (y, z + 1), (y + 1, z + 1) as vector loads of
+    // four and one scalar each; otherwise (a group that wraps into the next row or runs past
+    // the grid) cell by cell.
+    // Either way every load is issued before any is used.
     __device__ inline void quad(int64_t i0, int ci[CPT], int cx[CPT], int cy[CPT], int cz[CPT]) const
     {
         int x0, y0, z0;
         kf_xyz(g, i0, x0, y0, z0);
-        if (i0 + CPT <= g.n && x0 + CPT < g.nx) {
+        if (i0 + CPT <= g.n && x0 + CPT <= g.nx) {
             float v[4][CPT + 1];
             bool in[4];
+            const int64_t b0 = g.idx(x0, y0, z0);
+            const bool x4 = x0 + CPT < g.nx;  // the x + 1 corner of the last cell is inside
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int y = y0 + (r & 1), z = z0 + (r >> 1);
                 in[r] = y < g.ny && z < g.nz;
-                if (in[r]) {
-                    const int64_t b = g.idx(x0, y, z);
-                    kf_load4_unaligned<S>(phi + b, scale, v[r]);
-                    v[r][CPT] = S::load(phi, b + CPT, scale);
-                }
+                const int64_t b = in[r] ? g.idx(x0, y, z) : b0;
+#pragma unroll
+                for (int q = 0; q < CPT; q += 4)
+                    kf_load4_unaligned<S>(phi + b + q, scale, &v[r][q]);
+                v[r][CPT] = S::load(phi, x4 ? b + CPT : b, scale);
             }
 #pragma unroll
             for (int c = 0; c < CPT; c++) {
@@ -141,151 +158,247 @@ struct Cells {
                 cz[c] = z0;
                 // corner k = (x + dx, y + dy, z + dz) is node row dy + 2 dz, column c + dx
                 const double d = (double)v[0][c];
-                if (d > max_surface || d < -clip) {
-                    ci[c] = -1;
-                    continue;
-                }
                 int m = 0;
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    const int r = c_corner[k][1] + 2 * c_corner[k][2];
-                    const double val = in[r] ? (double)v[r][c + c_corner[k][0]] : oob;
+                    const int r = kCornerY[k] + 2 * kCornerZ[k];
+                    const bool inside = in[r] && (c + kCornerX[k] < CPT || x4);
+                    const double val = inside ? (double)v[r][c + kCornerX[k]] : oob;
                     if (val < iso)
                         m |= 1 << k;
                 }
-                ci[c] = c_edge_mask[m] ? m : -1;
+                ci[c] = (d > max_surface || d < -clip || m == 0 || m == 255) ? -1 : m;
             }
         } else {
+            int x = x0, y = y0, z = z0;
 #pragma unroll
             for (int c = 0; c < CPT; c++) {
-                double val[8];
-                ci[c] = -1;
-                cx[c] = cy[c] = cz[c] = 0;
-                if (i0 + c < g.n)
-                    ci[c] = classify(i0 + c, val, cx[c], cy[c], cz[c]);
+                if (x == g.nx) {
+                    x = 0;
+                    if (++y == g.ny) {
+                        y = 0;
+                        ++z;
+                    }
+                }
+                cx[c] = x;
+                cy[c] = y;
+                cz[c] = z;
+                int m = 0;
+                double d = 0.0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const double val = at_flat(x + kCornerX[k], y + kCornerY[k], z + kCornerZ[k]);
+                    if (k == 1)
+                        d = val;  // corner 1 is (x, y, z)
+                    if (val < iso)
+                        m |= 1 << k;
+                }
+                ci[c] = (i0 + c >= g.n || d > max_surface || d < -clip || m == 0 || m == 255) ? -1 : m;
+                ++x;
             }
         }
     }
 };
 
 // Triangles per block of CELLS cells.
+static_assert(BLOCK == 256, "one thread per case loads the triangle-count table");
+
 template <class S>
 __global__ __launch_bounds__(BLOCK) void k_mc_count(Cells<S> C, unsigned int *__restrict__ block_tot)
 {
     __shared__ unsigned int s[BLOCK / 64];
+    __shared__ unsigned char s_cnt[256];
+    s_cnt[threadIdx.x] = c_tri_count[threadIdx.x];
     const int64_t i0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * CPT;
     int ci[CPT], x[CPT], y[CPT], z[CPT];
     C.quad(i0, ci, x, y, z);
+    __syncthreads();
     unsigned int cnt = 0;
 #pragma unroll
     for (int c = 0; c < CPT; c++)
-        cnt += ci[c] >= 0 ? c_tri_count[ci[c]] : 0u;
+        cnt += ci[c] >= 0 ? s_cnt[ci[c]] : 0u;
     const unsigned int t = block_sum(cnt, s);
     if (threadIdx.x == 0)
         block_tot[blockIdx.x] = t;
 }
 
-// Exclusive scan of nb block totals in one block of 1024 threads; offs[nb] = total, also
-// written to *total_out when given.
+// Exclusive scan of nb block totals in one block of 1024 threads (each sums a run of
+// consecutive totals; wavefront scans, then the 16 wavefront totals through LDS); offs[nb] =
+// total, also written to *total_out when given.
 __global__ __launch_bounds__(1024) void k_mc_scan(const unsigned int *__restrict__ tot, unsigned long long *offs,
                                                   int nb, unsigned long long *total_out)
 {
-    __shared__ unsigned long long s[1024];
-    const int t = threadIdx.x;
+    __shared__ unsigned long long s[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int per = (nb + 1023) / 1024;
     const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+    // runs of 8 totals, loaded together (indices clamped inside the array, the excess ignored);
+    // the first run stays in registers for the offsets
+    unsigned int q0[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        q0[j] = tot[min(b0 + j, nb - 1)];
     unsigned long long sum = 0;
-    for (int b = b0; b < b1; b++)
-        sum += tot[b];
-    s[t] = sum;
-    __syncthreads();
-    // Hillis-Steele inclusive scan.
-    for (int o = 1; o < 1024; o <<= 1) {
-        unsigned long long v = (t >= o) ? s[t - o] : 0ull;
-        __syncthreads();
-        s[t] += v;
-        __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        sum += b0 + j < b1 ? q0[j] : 0u;
+    for (int b = b0 + 8; b < b1; b += 8) {
+        unsigned int q[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            q[j] = tot[min(b + j, nb - 1)];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            sum += b + j < b1 ? q[j] : 0u;
     }
-    unsigned long long run = s[t] - sum;
-    for (int b = b0; b < b1; b++) {
-        offs[b] = run;
-        run += tot[b];
+    unsigned long long v = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long y = __shfl_up(v, o, 64);
+        if (lane >= o)
+            v += y;
+    }
+    if (lane == 63)
+        s[w] = v;
+    __syncthreads();
+    unsigned long long run = v - sum, all = 0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        run += q < w ? s[q] : 0ull;
+        all += s[q];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        if (b0 + j < b1) {
+            offs[b0 + j] = run;
+            run += q0[j];
+        }
+    for (int b = b0 + 8; b < b1; b += 8) {
+        unsigned int q[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            q[j] = tot[min(b + j, nb - 1)];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (b + j < b1) {
+                offs[b + j] = run;
+                run += q[j];
+            }
     }
     if (t == 1023) {
-        offs[nb] = s[1023];
+        offs[nb] = all;
         if (total_out)
-            *total_out = s[1023];
+            *total_out = all;
     }
 }
 
-// VertexInterp (MarchingCubes.h:326-356) on the edges cell (x, y, z) of case ci cuts, then its
-// triangles (Polygonise, MarchingCubes.h:411-419) from position t0 on, y negated; triangles
-// at or past cap are dropped.
+// The triangles of cell (x, y, z), case ci (Polygonise, MarchingCubes.h:411-419), from
+// position t0 on, y negated (SimpleMesh.h:66-72); triangles at or past cap are dropped.  Each
+// vertex is VertexInterp (MarchingCubes.h:326-356) on its edge, p1 + (iso - v1) / (v1 - v2)
+// (p1 - p2) with p1 the edge's first corner, computed where the triangle uses it (the same
+// operations as once per edge, so the same values).  The corner values sit in the thread's
+// column of the block's LDS array sv (an edge picks its corners by runtime index, which
+// registers cannot serve without spilling).
 template <class S>
-__device__ inline void emit_cell(const Cells<S> &C, int x, int y, int z, int ci, unsigned long long t0,
-                                 float *__restrict__ verts, unsigned long long cap)
+__device__ __forceinline__ void emit_cell(const Cells<S> &C, int x, int y, int z, int ci, unsigned long long t0,
+                                          float *__restrict__ verts, unsigned long long cap, double (*sv)[BLOCK],
+                                          const unsigned char *s_edges, const unsigned char *s_cnt)
 {
+    const int tid = threadIdx.x;
     double val[8];
 #pragma unroll
     for (int k = 0; k < 8; k++)
-        val[k] = C.at(x + c_corner[k][0], y + c_corner[k][1], z + c_corner[k][2]);
+        val[k] = C.at_flat(x + kCornerX[k], y + kCornerY[k], z + kCornerZ[k]);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        sv[k][tid] = val[k];
+    // corner positions in the soup's normalised coordinates (SDF.cpp:338-355: v / (dims / 2) -
+    // 1), two per axis: corners 0, 3, 4, 7 are at x + 1 (bits of 0x99), 2, 3, 6, 7 at y + 1
+    // (0xcc), 4..7 at z + 1
     const double hx = (double)C.g.nx / 2, hy = (double)C.g.ny / 2, hz = (double)C.g.nz / 2;
-    double p[8][3];
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-        p[c][0] = (double)(x + c_corner[c][0]) / hx - 1;
-        p[c][1] = (double)(y + c_corner[c][1]) / hy - 1;
-        p[c][2] = (double)(z + c_corner[c][2]) / hz - 1;
-    }
-    double vl[12][3];
-    const unsigned int em = c_edge_mask[ci];
-#pragma unroll
-    for (int e = 0; e < 12; e++) {
-        if (em & (1u << e)) {
-            const int a = c_edge_corners[e][0], b = c_edge_corners[e][1];
-            const double sc = (C.iso - val[a]) / (val[a] - val[b]);
-            vl[e][0] = p[a][0] + sc * (p[a][0] - p[b][0]);
-            vl[e][1] = p[a][1] + sc * (p[a][1] - p[b][1]);
-            vl[e][2] = p[a][2] + sc * (p[a][2] - p[b][2]);
-        }
-    }
-    const unsigned int cnt = c_tri_count[ci];
+    const double px[2] = {(double)x / hx - 1, (double)(x + 1) / hx - 1};
+    const double py[2] = {(double)y / hy - 1, (double)(y + 1) / hy - 1};
+    const double pz[2] = {(double)z / hz - 1, (double)(z + 1) / hz - 1};
+    const unsigned int cnt = s_cnt[ci];
     for (unsigned int t = 0; t < cnt; t++) {
         const unsigned long long tri = t0 + t;
         if (tri >= cap)
             break;
         float *o = verts + tri * 9;
+#pragma unroll
         for (int v = 0; v < 3; v++) {
-            const int e = c_tri_edges[ci][t * 3 + v];
-            o[v * 3 + 0] = (float)vl[e][0];
-            o[v * 3 + 1] = (float)(-vl[e][1]);
-            o[v * 3 + 2] = (float)vl[e][2];
+            const int e = s_edges[ci * 15 + t * 3 + v];
+            // ends of edge e as Polygonise's VertexInterp calls take them (MarchingCubes.h:386-409):
+            // 0..3 join corners e, e + 1 (mod 4); 4..7 join e, e + 1 (4..7 mod 4); 8..11 join e - 8, e - 4
+            const int a = e < 8 ? e : e - 8;
+            const int b = e < 4 ? (e + 1) & 3 : e < 8 ? 4 + ((e + 1) & 3) : e - 4;
+            const double va = sv[a][tid], vb = sv[b][tid];
+            const double pa[3] = {(0x99 >> a) & 1 ? px[1] : px[0], (0xcc >> a) & 1 ? py[1] : py[0],
+                                  a >> 2 ? pz[1] : pz[0]};
+            const double pb[3] = {(0x99 >> b) & 1 ? px[1] : px[0], (0xcc >> b) & 1 ? py[1] : py[0],
+                                  b >> 2 ? pz[1] : pz[0]};
+            const double sc = (C.iso - va) / (va - vb);
+            o[v * 3 + 0] = (float)(pa[0] + sc * (pa[0] - pb[0]));
+            o[v * 3 + 1] = (float)(-(pa[1] + sc * (pa[1] - pb[1])));
+            o[v * 3 + 2] = (float)(pa[2] + sc * (pa[2] - pb[2]));
         }
     }
 }
 
+// Emit: a block whose cells hold no triangles (equal offsets) returns at once.  Otherwise it
+// classifies its cells, lists the ones with triangles in LDS in cell order
+// with their triangle offsets (one block scan of the packed counts), and then spreads the
+// listed cells over its threads, one cell per thread, so the gathers and FP64 interpolation of
+// a surface-dense stretch of cells run in parallel rather than one after another in one lane.
 template <class S>
 __global__ __launch_bounds__(BLOCK) void k_mc_emit(Cells<S> C, const unsigned long long *__restrict__ offs,
                                                    float *__restrict__ verts, unsigned long long cap)
 {
     __shared__ unsigned int s[BLOCK / 64];
-    const int64_t i0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * CPT;
+    __shared__ unsigned char s_cnt[256];
+    __shared__ unsigned int s_cell[CELLS];   // (cell within the block << 8) | case
+    __shared__ unsigned short s_tri[CELLS];  // its first triangle within the block
+    __shared__ double s_val[8][BLOCK];       // corner values of the cell each thread emits
+    __shared__ unsigned int s_edges[256 * 15 / 4];  // the case table's edge ids
+    const unsigned long long base = offs[blockIdx.x];
+    if (offs[blockIdx.x + 1] == base)
+        return;  // no triangles in this block's cells (the count pass's total)
+    s_cnt[threadIdx.x] = c_tri_count[threadIdx.x];
+    for (int k = threadIdx.x; k < 256 * 15 / 4; k += BLOCK)
+        s_edges[k] = reinterpret_cast<const unsigned int *>(&c_tri_edges[0][0])[k];
+    const int64_t blk0 = (int64_t)blockIdx.x * CELLS;
+    const int64_t i0 = blk0 + (int64_t)threadIdx.x * CPT;
     int ci[CPT], x[CPT], y[CPT], z[CPT];
     C.quad(i0, ci, x, y, z);
-    unsigned int cnt = 0;
+    __syncthreads();
+    unsigned int ns = 0, nt = 0;
 #pragma unroll
     for (int c = 0; c < CPT; c++)
-        cnt += ci[c] >= 0 ? c_tri_count[ci[c]] : 0u;
+        if (ci[c] >= 0) {
+            ns += 1;
+            nt += s_cnt[ci[c]];
+        }
+    // cells in the high half, triangles (at most 5 CELLS < 2^16) in the low half
+    static_assert(5 * CELLS < (1 << 16), "packed counts");
     unsigned int total;
-    const unsigned int incl = block_scan_incl(cnt, s, total);
-    if (cnt == 0)
-        return;
-    unsigned long long t0 = offs[blockIdx.x] + (incl - cnt);
-    for (int c = 0; c < CPT; c++) {
-        if (ci[c] < 0)
-            continue;
-        emit_cell(C, x[c], y[c], z[c], ci[c], t0, verts, cap);
-        t0 += c_tri_count[ci[c]];
+    const unsigned int incl = block_scan_incl((ns << 16) | nt, s, total);
+    unsigned int slot = (incl >> 16) - ns, tri = (incl & 0xffffu) - nt;
+#pragma unroll
+    for (int c = 0; c < CPT; c++)
+        if (ci[c] >= 0) {
+            s_cell[slot] = ((unsigned int)(threadIdx.x * CPT + c) << 8) | (unsigned int)ci[c];
+            s_tri[slot] = (unsigned short)tri;
+            slot += 1;
+            tri += s_cnt[ci[c]];
+        }
+    __syncthreads();
+    const unsigned int ncells = total >> 16;
+    for (unsigned int e = threadIdx.x; e < ncells; e += BLOCK) {
+        const unsigned int cell = s_cell[e];
+        int cx, cy, cz;
+        kf_xyz(C.g, blk0 + (cell >> 8), cx, cy, cz);
+        emit_cell(C, cx, cy, cz, (int)(cell & 255u), base + s_tri[e], verts, cap, s_val,
+                  reinterpret_cast<const unsigned char *>(s_edges), s_cnt);
     }
 }
 

@@ -45,50 +45,153 @@ struct Mat16 { double m[16]; };
 // SDF::integrateDepthFrame, SDF.cpp:149-196.  FRESH: the same pass also resets the volume
 // (SDF.cpp:133-134: phi = Max + eps, w = 0) before integrating, as a reset followed by an
 // integration would -- same values, one write per voxel and no read.
-template <class S, bool FRESH>
-__global__ void k_integrate(typename S::T *phi, int32_t *w, GridDims g, double mnx, double mny, double mnz,
-                            double voxel, double clip, float scale, const float *__restrict__ depth, int W,
-                            int H, Mat9 K, Mat16 M, double min_depth, double max_depth, float oob_m)
+//
+// A thread takes IPT consecutive voxels.  The camera coordinates keep the reference's
+// left-to-right sums, c_r = ((W2C[r][0] X + W2C[r][1] Y) + W2C[r][2] Z) + W2C[r][3] 1, but the
+// products of Y, Z and 1 are formed once per row, not per voxel (the same roundings).  The
+// running average (old w + dist) / (w + 1) is dist itself when w = 0 and old is finite
+// (old 0 = +-0, +-0 + dist = dist for dist != 0 and +0 for dist = +0, dist / 1 = dist), so that
+// case skips the multiply, add and FP64 division.  The stores of a FRESH pass are one vector store per
+// thread for phi and for w.
+constexpr int IPT = 4;
+
+template <class S>
+struct IntegrateArgs {
+    typename S::T *phi;
+    int32_t *w;
+    GridDims g;
+    double mn[3];
+    double voxel, clip;
+    float scale, oob_m;
+    const float *__restrict__ depth;
+    int W, H;
+    Mat9 K;
+    Mat16 M;
+    double min_depth, max_depth;
+};
+
+// The row constants of row (y, z): P[r][0] = W2C[r][1] Y, P[r][1] = W2C[r][2] Z, P[r][2] = W2C[r][3] 1.
+template <class S>
+__device__ inline void row_products(const IntegrateArgs<S> &A, int y, int z, double P[3][3])
 {
-    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= g.n)
-        return;
-    int x, y, z;
-    kf_xyz(g, i, x, y, z);
-    const double Z = voxel * (z + 0.5) + mnz;
-    const double Y = voxel * (y + 0.5) + mny;
-    const double X = voxel * (x + 0.5) + mnx;
-    double c[4];
+    const double Z = A.voxel * (z + 0.5) + A.mn[2];
+    const double Y = A.voxel * (y + 0.5) + A.mn[1];
 #pragma unroll
-    for (int r = 0; r < 4; r++)
-        c[r] = M.m[r * 4 + 0] * X + M.m[r * 4 + 1] * Y + M.m[r * 4 + 2] * Z + M.m[r * 4 + 3] * 1.0;
+    for (int r = 0; r < 3; r++) {
+        P[r][0] = A.M.m[r * 4 + 1] * Y;
+        P[r][1] = A.M.m[r * 4 + 2] * Z;
+        P[r][2] = A.M.m[r * 4 + 3] * 1.0;
+    }
+}
+
+// Projection of voxel x of the row with products P (SDF.cpp:158-175): its camera depth and
+// the index of the pixel it reads, or -1 outside the image.
+struct Projected {
+    double zc;
+    int pix;
+};
+
+template <class S>
+__device__ inline Projected project_voxel(const IntegrateArgs<S> &A, const double P[3][3], int x)
+{
+    const double X = A.voxel * (x + 0.5) + A.mn[0];
+    double c[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        c[r] = ((A.M.m[r * 4 + 0] * X + P[r][0]) + P[r][1]) + P[r][2];
     const double q0 = c[0] / c[2], q1 = c[1] / c[2];
-    const double px = K.m[0] * q0 + K.m[1] * q1 + K.m[2] * 1.0;
-    const double py = K.m[3] * q0 + K.m[4] * q1 + K.m[5] * 1.0;
+    const double px = A.K.m[0] * q0 + A.K.m[1] * q1 + A.K.m[2] * 1.0;
+    const double py = A.K.m[3] * q0 + A.K.m[4] * q1 + A.K.m[5] * 1.0;
     // roundf on the float-narrowed coordinate, SDF.cpp:167,171.
     const int col = (int)roundf((float)(px + 0.5));
     const int row = (int)roundf((float)(py + 0.5));
-    bool take = col >= 0 && col < W && row >= 0 && row < H;
-    double dist = 0.0;
-    if (take) {
-        const double d = (double)depth[(int64_t)row * W + col];
-        take = !(d < min_depth || d > max_depth);
-        dist = d - c[2];
-        take = take && !(dist < -clip);
-    }
-    const typename S::T init = S::enc(oob_m, scale);
-    if (!take) {
-        if (FRESH) {
-            phi[i] = init;
-            w[i] = 0;
+    const bool in = col >= 0 && col < A.W && row >= 0 && row < A.H;
+    return Projected{c[2], in ? row * A.W + col : -1};
+}
+
+// The running average of SDF.cpp:189-193 for a voxel that read depth d (pix >= 0); false when
+// the frame leaves the voxel alone (depth out of range, or too far behind the surface).
+template <class S>
+__device__ inline bool average_voxel(const IntegrateArgs<S> &A, const Projected &pr, float depth, int32_t wi,
+                                     double old, double &nv)
+{
+    if (pr.pix < 0)
+        return false;
+    const double d = (double)depth;
+    if (d < A.min_depth || d > A.max_depth)
+        return false;
+    const double dist = d - pr.zc;
+    if (dist < -A.clip)
+        return false;
+    nv = wi == 0 && isfinite(old) ? dist : (old * (double)wi + dist) / (double)(wi + 1);
+    return true;
+}
+
+template <class S, bool FRESH>
+__global__ __launch_bounds__(BLOCK) void k_integrate(IntegrateArgs<S> A)
+{
+    const GridDims &g = A.g;
+    const int64_t i0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * IPT;
+    if (i0 >= g.n)
+        return;
+    int x, y, z;
+    kf_xyz(g, i0, x, y, z);
+    double P[3][3];
+    row_products(A, y, z, P);
+    // project the group's voxels, then read their pixels together
+    Projected pr[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        if (x == g.nx) {  // the group runs into the next row
+            x = 0;
+            if (++y == g.ny) {
+                y = 0;
+                ++z;
+            }
+            row_products(A, y, z, P);
         }
+        pr[k] = project_voxel(A, P, x);
+        if (i0 + k >= g.n)
+            pr[k].pix = -1;
+        ++x;
+    }
+    float dep[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++)
+        dep[k] = A.depth[pr[k].pix >= 0 ? pr[k].pix : 0];
+    const typename S::T init = S::enc(A.oob_m, A.scale);
+    if (FRESH && i0 + IPT <= g.n) {
+        // all IPT voxels written: phi and w as one vector store each
+        typedef typename S::T vec_t __attribute__((ext_vector_type(IPT)));
+        vec_t pv;
+        int32_t wv[IPT];
+#pragma unroll
+        for (int k = 0; k < IPT; k++) {
+            double nv;
+            const bool take = average_voxel(A, pr[k], dep[k], 0, 0.0, nv);
+            pv[k] = take ? S::enc((float)nv, A.scale) : init;
+            wv[k] = take ? 1 : 0;
+        }
+        *reinterpret_cast<vec_t *>(A.phi + i0) = pv;
+        *reinterpret_cast<int4 *>(A.w + i0) = make_int4(wv[0], wv[1], wv[2], wv[3]);
         return;
     }
-    const int32_t wi = FRESH ? 0 : w[i];
-    const double old = FRESH ? (double)S::dec(init, scale) : (double)S::load(phi, i, scale);
-    const double nv = (old * (double)wi + dist) / (double)(wi + 1);
-    phi[i] = S::enc((float)nv, scale);
-    w[i] = wi + 1;
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const int64_t i = i0 + k;
+        if (i >= g.n)
+            break;
+        const int32_t wi = FRESH ? 0 : A.w[i];
+        const double old = FRESH ? (double)S::dec(init, A.scale) : (double)S::load(A.phi, i, A.scale);
+        double nv;
+        if (average_voxel(A, pr[k], dep[k], wi, old, nv)) {
+            A.phi[i] = S::enc((float)nv, A.scale);
+            A.w[i] = wi + 1;
+        } else if (FRESH) {
+            A.phi[i] = init;
+            A.w[i] = 0;
+        }
+    }
 }
 
 template <class S>
@@ -304,17 +407,29 @@ kf_status kf_launch_integrate(kf_volume *v, const float *ddepth, int w, int h, c
         k.m[i] = K[i];
     for (int i = 0; i < 16; i++)
         m.m[i] = W2C[i];
+    const unsigned int nb = (unsigned int)((v->n + (int64_t)BLOCK * IPT - 1) / ((int64_t)BLOCK * IPT));
     KF_DISPATCH_DTYPE(v->dtype, S, {
+        IntegrateArgs<S> A;
+        A.phi = (typename S::T *)v->phi;
+        A.w = v->w;
+        A.g = grid_of(v);
+        for (int a = 0; a < 3; a++)
+            A.mn[a] = v->min_corner[a];
+        A.voxel = v->voxel;
+        A.clip = v->clip;
+        A.scale = v->scale();
+        A.oob_m = (float)v->oob();
+        A.depth = ddepth;
+        A.W = w;
+        A.H = h;
+        A.K = k;
+        A.M = m;
+        A.min_depth = min_depth;
+        A.max_depth = max_depth;
         if (fresh)
-            hipLaunchKernelGGL((k_integrate<S, true>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream,
-                               (typename S::T *)v->phi, v->w, grid_of(v), v->min_corner[0], v->min_corner[1],
-                               v->min_corner[2], v->voxel, v->clip, v->scale(), ddepth, w, h, k, m, min_depth,
-                               max_depth, (float)v->oob());
+            hipLaunchKernelGGL((k_integrate<S, true>), dim3(nb), dim3(BLOCK), 0, v->ctx->stream, A);
         else
-            hipLaunchKernelGGL((k_integrate<S, false>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream,
-                               (typename S::T *)v->phi, v->w, grid_of(v), v->min_corner[0], v->min_corner[1],
-                               v->min_corner[2], v->voxel, v->clip, v->scale(), ddepth, w, h, k, m, min_depth,
-                               max_depth, (float)v->oob());
+            hipLaunchKernelGGL((k_integrate<S, false>), dim3(nb), dim3(BLOCK), 0, v->ctx->stream, A);
     });
     KF_LAUNCH_CHECK();
     return KF_OK;

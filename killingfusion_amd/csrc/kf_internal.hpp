@@ -496,7 +496,7 @@ kf_status kf_mc_to_device(const kf_volume *v, double iso, float **dverts, size_t
 void kf_mc_release(kf_context *ctx);
 // Cells per block of the marching-cubes count and emit passes (one block total and one offset
 // each): a grid of n voxels has kf_mc_blocks(n) = ceil(n / KF_MC_CELLS) blocks.
-constexpr int KF_MC_CELLS = 1024;
+constexpr int KF_MC_CELLS = 2048;
 int64_t kf_mc_blocks(int64_t n);
 // Count, scan and (cap > 0) emit of v's triangle soup with caller-provided device buffers:
 // tot[kf_mc_blocks(n)], offs[kf_mc_blocks(n) + 1], verts[9 cap]; *total_out (may be null, device)

@@ -148,3 +148,39 @@ def test_volume_copy_across_contexts_is_rejected(ctx, pairs):
         v.copy_from(pairs["vols"]["bf16"][0])
     assert e.value.status == KF_E_INVALID_ARG
     other.close()
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+@pytest.mark.parametrize("dims", [(61, 45, 58), (63, 2, 3), (5, 9, 7)])
+def test_integrate_uneven_rows_and_held_values(ctx, dims, dtype):
+    """Rows that are not a multiple of the four voxels a thread integrates (its group wraps into
+    the next row), fresh and into held values: weights 0..2, and untouched voxels holding +-inf
+    and NaN with weight 0, which the running average turns into NaN (SDF.cpp:189-193)."""
+    print(f"case: integrate sphere frame into {dims}, {dtype}, fresh and into held values", flush=True)
+    d1, _ = sphere_frames()
+    vox = 0.3 / max(dims)
+    mn = (-0.15, -0.15, 0.65)
+    g = ogrid(dims, mn, vox, 3 * vox)
+    rng = np.random.default_rng(sum(dims))
+    n = g.n
+    scale = np.float32(3 * vox + 1e-5)
+    rt = bf16_roundtrip if dtype == "bf16" else (lambda a, s: np.asarray(a, np.float32))
+    v = SDF(ctx, dims, mn, vox, 3 * vox, 3 * vox, dtype=dtype)
+    v.integrateDepthFrame(d1, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+    ophi, ow = O.sdf_init(g)
+    O.integrate(ophi, ow, g, 3 * vox, d1, scenes.K, scenes.C2W, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+    phi, w = v.download()
+    assert (w == ow).all() and np.array_equal(phi, rt(ophi.astype(np.float32), scale))
+    held = rng.uniform(-0.02, 0.02, n).astype(np.float32)
+    hw = rng.integers(0, 3, n).astype(np.int32)
+    held[:7] = [np.inf, -np.inf, np.nan, np.inf, -np.inf, np.nan, 0.0]
+    hw[:7] = 0
+    v.upload(held, hw)
+    hphi, hw2 = v.download()
+    v.integrateDepthFrame(d1, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+    ophi, ow = hphi.astype(np.float64), hw2.astype(np.int64)
+    O.integrate(ophi, ow, g, 3 * vox, d1, scenes.K, scenes.C2W, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+    phi, w = v.download()
+    assert (w == ow).all()
+    assert np.array_equal(phi, rt(ophi.astype(np.float32), scale), equal_nan=True)
+    v.close()
