@@ -114,6 +114,7 @@ SIGNATURES = {
     "kf_pipeline_create": (C.c_int, [_vp, C.POINTER(PipelineDesc), C.POINTER(_vp)]),
     "kf_pipeline_destroy": (C.c_int, [_vp]),
     "kf_pipeline_process_frame": (C.c_int, [_vp, _fp, C.POINTER(FrameStats)]),
+    "kf_marching_cubes_noalloc": (C.c_int, [_vp, C.c_double, _vp, C.c_uint64, _vp]),
     "kf_pipeline_set_meshes": (C.c_int, [_vp, C.c_int32, C.c_double]),
     "kf_pipeline_mesh": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_uint64), _fp, C.c_uint64]),
     "kf_pipeline_canonical": (_vp, [_vp]),

@@ -31,6 +31,7 @@ SOURCES = {
     "kf_pipeline.hip": [],
     "k_volume.hip": ["-ffp-contract=off"],
     "k_mc.hip": ["-ffp-contract=off"],
+    "k_mc_noalloc.hip": [],
     "k_deform_ref.hip": ["-ffp-contract=on"],
     "k_deform_adj.hip": ["-ffp-contract=on"],
     "k_deform_active.hip": ["-ffp-contract=on"],

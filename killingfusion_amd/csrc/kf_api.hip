@@ -8,6 +8,7 @@
 #include <string>
 
 kf_status kf_mc_init_tables();
+kf_status kf_noalloc_init();
 
 namespace {
 
@@ -263,6 +264,8 @@ kf_status kf_context_create(int device, void *stream, kf_context **out)
         return KF_E_OOM;
     }
     s = kf_mc_init_tables();
+    if (s == KF_OK)
+        s = kf_noalloc_init();
     if (s == KF_OK)
         s = kf_state_reset(ctx, 0.0);
     if (s != KF_OK) {
