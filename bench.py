@@ -356,10 +356,14 @@ def main(argv=None):
     achieved_profile = bytes_per_vi * nvox / (profile_us * 1e-6) / 1e9 if profile_us else None
 
     plain_steps = a.plain_steps or a.steps
-    kp_ref = KillingParams.paper(VOXEL, 3.0, method="reference")
+    kp_ref = KillingParams.paper(VOXEL, 3.0, method="reference", sweep="dense")
     wall_p, gpu_ms_p, st_p = time_deform(ctx, live_32, can_32, kp_ref, plain_steps, a.warmup, events)
     plain_value = nvox * plain_steps / wall_p
     plain_kernel_ms = gpu_ms_p / plain_steps
+    # the plain step over the voxels that can move only (same field bit for bit)
+    kp_ref_a = KillingParams.paper(VOXEL, 3.0, method="reference", sweep="active")
+    wall_pa, gpu_ms_pa, st_pa = time_deform(ctx, live_32, can_32, kp_ref_a, plain_steps, a.warmup, events)
+    plain_active_value = nvox * plain_steps / wall_pa
 
     batched = None if a.no_batched else bench_batched(ctx, events)
     sequence = None if a.no_sequence else bench_sequence(ctx)
@@ -411,7 +415,12 @@ def main(argv=None):
                        "hbm_gbs_at_32B": 32 * nvox / (plain_kernel_ms * 1e-3) / 1e9},
         "adjoint_fp32": {"value": nvox * a.steps / wall32, "kernel_ms": gpu_ms32 / a.steps,
                          "hbm_gbs_at_32B": 32 * nvox / (gpu_ms32 / a.steps * 1e-3) / 1e9},
+        "plain_fp32_active": {"value": plain_active_value, "ms_per_step": wall_pa * 1e3 / plain_steps,
+                              "gpu_ms_per_step": gpu_ms_pa / plain_steps, "listed_voxels": int(st_pa["listed"]),
+                              "note": "the plain reference-formula step over the voxels that can move (listed "
+                                      "once per call); same field as the dense plain step bit for bit"},
         "speedup_vs_plain_fp32": value / plain_value,
+        "speedup_active_vs_plain_fp32_active": (nvox * a.steps / wall_a) / plain_active_value,
         "speedup_note": "dense vs dense: both steps stream u (24 B per voxel) every iteration, so the ratio is "
                         "capped near (plain pass 1 + plain pass 2) / adjoint step",
         "active_set": {"value": nvox * a.steps / wall_a, "unit": "voxel-iterations/s (whole grid counted)",

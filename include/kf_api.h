@@ -56,7 +56,8 @@ typedef enum kf_method {
     KF_METHOD_ADJOINT = 1    /* exact gradient of the discrete energy of kf_energy_forward */
 } kf_method;
 
-/* Which voxels an adjoint Jacobi step visits (the field it produces is the same bit for bit).
+/* Which voxels a Jacobi step visits, with either method (the field it produces is the same bit
+ * for bit).
  * A step moves voxel j only when -clip <= phi_live(j + u_j) <= Max - eps
  * (KillingFusion.cpp:272-274), and with u_j = 0 that reads phi_live[j]; so the voxels that
  * can move during one kf_deform call are those with u_j != 0 or phi_live[j] in that band,
@@ -102,7 +103,7 @@ typedef struct kf_params {
     int32_t max_iters;      /* KILLING_MAX_ITERATIONS (config.cpp:42) */
     double stop_max_update; /* stop once max |du| < this (voxels); <= 0 runs max_iters */
     int32_t method;         /* kf_method */
-    int32_t sweep;          /* kf_sweep (adjoint method only; the reference method is dense) */
+    int32_t sweep;          /* kf_sweep (either method) */
 } kf_params;
 
 typedef struct kf_deform_stats {
@@ -183,8 +184,8 @@ kf_status kf_energy_backward(const kf_volume *live, const kf_volume *can, const 
                              const kf_params *p, kf_field *grad_out);
 /* KillingFusion::computeDisplacementField, all-voxels (Jacobi) mode
  * (KillingFusion.cpp:248-351): live = src, can = dest, u = srcToDest, updated in place.
- * p->method selects the reference gradient or the fused adjoint kernel; with the adjoint,
- * p->sweep selects the dense or the active-set sweep (same field; see kf_sweep). */
+ * p->method selects the reference gradient or the fused adjoint kernel; p->sweep selects the
+ * dense or the active-set sweep (same field; see kf_sweep). */
 kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, const kf_params *p,
                     kf_deform_stats *stats);
 /* SDF::update(displacementField), SDF.cpp:300-328. */

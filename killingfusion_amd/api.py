@@ -62,7 +62,7 @@ class KillingParams:
     max_iters: int = 1024
     stop_max_update: float = 0.0
     method: str = "reference"  # or "adjoint"
-    sweep: str = "auto"  # adjoint steps: "auto", "dense" (every voxel) or "active" (voxels that can move)
+    sweep: str = "auto"  # "auto", "dense" (every voxel) or "active" (the voxels that can move): the same field
 
     @classmethod
     def paper(cls, voxel_size: float, trunc_voxels: float = 3.0, **kw):

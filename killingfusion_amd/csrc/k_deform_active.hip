@@ -926,7 +926,7 @@ kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params
     return KF_OK;
 }
 
-kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int count)
+kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int count, bool neighbours)
 {
     kf_context *ctx = live->ctx;
     kf_active_ws &a = ctx->active;
@@ -948,7 +948,7 @@ kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int 
     hipLaunchKernelGGL(k_active_compact, dim3(nspan), dim3(WPB), 0, ctx->stream, a.mask, nwords, a.span_count,
                        a.span_count + nspan, 0u, a.list, a.pos_of);
     KF_LAUNCH_CHECK();
-    if (count > 0) {
+    if (count > 0 && neighbours) {
         hipLaunchKernelGGL(k_active_neighbours, dim3((count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream,
                            grid_of(live), a.mask, a.pos_of, a.list, 0u, count, u, a.u, a.nbr);
         KF_LAUNCH_CHECK();

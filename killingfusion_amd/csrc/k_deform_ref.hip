@@ -468,6 +468,79 @@ __global__ __launch_bounds__(SBLOCK) void k_ref_scan(PhiView<S> L, const float *
     }
 }
 
+// Pass 1 over the voxels that can move only (the active-set sweep; vlist holds them, count of
+// them): the same per-voxel work as k_ref_scan, one voxel per thread.  Every other voxel holds
+// u = +0 in both Jacobi buffers and keeps it (its band test reads phi_live[j] alone and fails,
+// or its warped value is NaN), so neither buffer needs a store there.
+template <class S, int MODE>
+__global__ __launch_bounds__(SBLOCK) void k_ref_active_scan(PhiView<S> L, const float *__restrict__ u_in,
+                                                           float *__restrict__ u_out, float hi, float lo,
+                                                           float alpha, const DeformState *st,
+                                                           unsigned long long *aux, int parity,
+                                                           unsigned long long segcap,
+                                                           unsigned long long *__restrict__ list_idx,
+                                                           float *__restrict__ list_psi,
+                                                           const unsigned int *__restrict__ vlist, unsigned int count)
+{
+    if (MODE == KF_MODE_STEP && st->done)
+        return;
+    __shared__ unsigned int s_wtot[SBLOCK / 64];
+    __shared__ unsigned long long s_base;
+    const GridDims g = L.g;
+    const unsigned int t = blockIdx.x * SBLOCK + threadIdx.x;
+    const bool have = t < count;
+    const int64_t j = have ? (int64_t)vlist[t] : 0;
+    const bool phi_finite = aux[AUX_NONFINITE] == 0ull;
+    float u[3] = {0.0f, 0.0f, 0.0f};
+    float psi = 0.0f;
+    bool act = false;
+    if (have) {
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            u[a] = u_in[a * g.n + j];
+        const float ph = S::load(L.p, j, L.scale);
+        const unsigned int zb = (__float_as_uint(u[0]) | __float_as_uint(u[1]) | __float_as_uint(u[2])) & 0x7fffffffu;
+        if (phi_finite && zb == 0u && __float_as_uint(ph) != 0x80000000u) {
+            psi = ph;
+        } else {
+            int x, y, z;
+            kf_xyz(g, j, x, y, z);
+            psi = L.trilinear(x, y, z, u[0], u[1], u[2]);
+        }
+        act = !(psi > hi || psi < lo);
+    }
+    const unsigned long long m = __ballot(act);
+    const unsigned int tot = (unsigned int)__popcll(m);
+    const int wv = (int)(threadIdx.x >> 6);
+    if (__lane_id() == 0)
+        s_wtot[wv] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int bt = 0;
+#pragma unroll
+        for (int k = 0; k < SBLOCK / 64; k++)
+            bt += s_wtot[k];
+        const unsigned int seg = blockIdx.x % NSEG;
+        s_base = bt ? seg * segcap + atomicAdd(&aux[parity * NSEG + seg], (unsigned long long)bt) : 0ull;
+    }
+    __syncthreads();
+    if (act) {
+        unsigned long long pos = s_base;
+        for (int k = 0; k < wv; k++)
+            pos += s_wtot[k];
+        const unsigned long long q = pos + (unsigned long long)__popcll(m & ((1ull << __lane_id()) - 1ull));
+        list_idx[q] = (unsigned long long)j;
+        list_psi[q] = psi;
+    }
+    // field values last (see k_ref_scan); voxels in the band get theirs in pass 2
+    if (have) {
+        const float d = -alpha * 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            u_out[a * g.n + j] = MODE == KF_MODE_GRAD ? 0.0f : u[a] + d;
+    }
+}
+
 // Pass 2: the voxels of the list, RK at a time per block.  The block takes their samples, one
 // (voxel, sample) pair per thread, so a voxel's 44 dependent reads run side by side instead of
 // one after another in one lane, and a thread per voxel combines them.  Each sample and the
@@ -601,7 +674,7 @@ void kf_ref_release(kf_context *ctx)
 }
 
 kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const float *u_in, float *u_out,
-                             const kf_params *p, int mode, int iter)
+                             const kf_params *p, int mode, int iter, const unsigned int *vlist, unsigned int vcount)
 {
     kf_context *ctx = live->ctx;
     KF_CHECK_ARG(live->dtype == can->dtype, "live and canonical volumes must share a dtype");
@@ -645,6 +718,11 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
         }
         w.cap = need;
     }
+    // The active-set pass 1 runs blocks of SBLOCK listed voxels; its segments then hold at most
+    // ceil(blocks / 8) blocks' voxels each, within the capacity sized for the whole grid.
+    const unsigned int nb_v = vlist ? (vcount + SBLOCK - 1) / SBLOCK : 0u;
+    const unsigned long long segcap_v = (unsigned long long)((nb_v + NSEG - 1) / NSEG) * SBLOCK;
+    KF_CHECK_ARG(!vlist || (mode != KF_MODE_GRAD && segcap_v * NSEG <= w.cap), "active list larger than the workspace");
     kf_status s = KF_OK;
     const int64_t chunks = (n + RK - 1) / RK;
     const unsigned int nb_b = (unsigned int)(chunks < LIST_BLOCKS ? chunks : LIST_BLOCKS);
@@ -677,18 +755,25 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD, false>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
                                    Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
         } else {
-            if (vec)
+            const unsigned long long sc = vlist ? segcap_v : segcap;
+            if (vlist) {
+                if (nb_v > 0)
+                    hipLaunchKernelGGL((k_ref_active_scan<S, KF_MODE_STEP>), dim3(nb_v), dim3(SBLOCK), 0, ctx->stream,
+                                       L, u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, sc, w.idx,
+                                       w.psi, vlist, vcount);
+            } else if (vec) {
                 hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, false>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
-                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
-            else
+                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, sc, w.idx, w.psi);
+            } else {
                 hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, true>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
-                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
+                                   u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, sc, w.idx, w.psi);
+            }
             if (nbhd)
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, true>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
-                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
+                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, sc, w.idx, w.psi);
             else
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, false>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
-                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
+                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, sc, w.idx, w.psi);
         }
     });
     KF_LAUNCH_CHECK();

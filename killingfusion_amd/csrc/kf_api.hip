@@ -601,23 +601,35 @@ kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *
     const int cur0 = u->cur;
     const bool check = p->stop_max_update > 0.0;
     const bool adj = p->method == KF_METHOD_ADJOINT;
-    // Active-set sweep (k_deform_active.hip): count the voxels that can move, list them, step
-    // only them (in compact arrays) and write the result back into the field's buffer.
-    bool listed = false;
+    // Active-set sweep (k_deform_active.hip): count the voxels that can move and list them.  The
+    // adjoint method steps only them, in compact arrays, and writes the result back into the
+    // field's buffer; the reference method steps only them in the field's own two buffers,
+    // which both hold +0 everywhere else (the other buffer starts as a copy of the field).
+    bool listed = false, ref_listed = false;
     unsigned int count = 0;
-    if (adj && p->max_iters > 0 && p->sweep != KF_SWEEP_DENSE && kf_active_supported(live)) {
+    if (p->max_iters > 0 && p->sweep != KF_SWEEP_DENSE && kf_active_supported(live)) {
         s = kf_active_count(live, u->buf[cur0], p, u_zero);
         if (s == KF_OK)
             s = read_state(ctx);
         if (s != KF_OK)
             return s;
         count = ctx->state_host->listed;
-        listed = p->sweep == KF_SWEEP_ACTIVE || (int64_t)count * 8 <= live->n;
-        if (listed && (s = kf_active_prepare(live, u->buf[cur0], count)) != KF_OK)
+        // The reference method's band test passes a NaN warped value, and with u_j = +0 next to
+        // a non-finite live value the warped value is NaN (inf * 0): such a voxel moves though
+        // phi_live[j] is outside the band.  The list covers every voxel that can move only when
+        // every live value is finite, so the reference method steps densely otherwise.
+        const bool use = (p->sweep == KF_SWEEP_ACTIVE || (int64_t)count * 8 <= live->n) &&
+                         (adj || !ctx->state_host->nonfinite);
+        if (use && (s = kf_active_prepare(live, u->buf[cur0], count, adj)) != KF_OK)
             return s;
+        listed = use && adj;
+        ref_listed = use && !adj;
+        if (ref_listed)
+            KF_HIP(hipMemcpyAsync(u->buf[cur0 ^ 1], u->buf[cur0], sizeof(float) * 3 * (size_t)live->n,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
     }
     // A list step is a few microseconds; check the stop rule less often so the host stays ahead.
-    const int CHUNK = listed ? 64 : 16;
+    const int CHUNK = listed ? 64 : ref_listed ? 32 : 16;
     // Launch k reads buffer cur0 ^ (k & 1) and writes the other one.  An adjoint launch k folds
     // launch k - 1 in its first block, so launch k itself may run before the stop rule (or a
     // divergence) of k - 1 is known: it only writes the buffer that is not the result, and
@@ -629,7 +641,8 @@ kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *
         float *out = u->buf[cur0 ^ ((k + 1) & 1)];
         s = listed ? kf_launch_adj_list(live, can, p, k, count, &nb)
             : adj  ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr, nullptr, k, &nb)
-                   : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP, k);
+                   : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP, k,
+                                        ref_listed ? ctx->active.list : nullptr, count);
         if (s != KF_OK)
             return s;
         launched = k + 1;
@@ -673,7 +686,7 @@ kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *
                 return s;
         }
     }
-    if (listed && !h.nonfinite) {
+    if ((listed || ref_listed) && !h.nonfinite) {
         // The list covers every voxel a warp of live by u can change (see k_deform_active.hip).
         ctx->active.warp_vol = live;
         ctx->active.warp_field = u;
@@ -686,7 +699,7 @@ kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *
         stats->max_update = h.last_max;
         stats->energy = energy;
         stats->active_voxels = h.last_active;
-        stats->sweep = listed ? KF_SWEEP_ACTIVE : KF_SWEEP_DENSE;
+        stats->sweep = listed || ref_listed ? KF_SWEEP_ACTIVE : KF_SWEEP_DENSE;
         stats->listed_voxels = count;
     }
     if (h.diverged != LLONG_MAX) {

@@ -436,9 +436,12 @@ kf_status kf_launch_field_add(kf_field *dst, const kf_field *src);
 
 struct DeformArgs;
 // Step launches take their index `iter` within a deform call: launch k writes partial set k & 1.
-// The reference step folds its own partials right after the launch.
+// The reference step folds its own partials right after the launch.  vlist (vcount entries):
+// the active-set sweep's list of the voxels that can move (step mode only), whose pass 1 then
+// visits only them; every other voxel must hold +0 in both u_in and u_out.
 kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const float *u_in,
-                             float *u_out, const kf_params *p, int mode, int iter = 0);
+                             float *u_out, const kf_params *p, int mode, int iter = 0,
+                             const unsigned int *vlist = nullptr, unsigned int vcount = 0);
 // mask_in: stencil mask of u_in (kf_launch_adj_mask), needed by energy and gradient passes of
 // the row kernel (see k_deform_adj.hip); without it those passes run one voxel per thread.
 // An adjoint step launch k > 0 folds launch k - 1 in its first block, so a deform loop ends
@@ -456,7 +459,9 @@ bool kf_active_supported(const kf_volume *live);
 void kf_active_release(kf_context *ctx);
 void kf_ref_release(kf_context *ctx);
 kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params *p, bool u_zero);
-kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int count);
+// neighbours = false lists the voxels only (ctx->active.list), without the adjoint step's
+// compact u and neighbour slots.
+kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int count, bool neighbours = true);
 kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const kf_params *p, int iter,
                              unsigned int count, unsigned int *nb_out);
 // Writes the final compact set (state->iters & 1, read on the device) back into u.
