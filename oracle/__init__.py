@@ -371,3 +371,62 @@ def num_threads() -> int:
 
 def set_num_threads(n: int) -> None:
     lib().kfo_set_num_threads(int(n))
+
+
+# ---------------------------------------------------------------------------
+# The same oracle evaluated in FP32 (build/libkf_oracle_f32.so from kf_oracle_f32.c): every
+# double of kf_oracle.c, structs and arrays included, read as float.  Used only to measure
+# what FP32 rounding alone does to the reference optimiser.
+# ---------------------------------------------------------------------------
+
+_LIB32_PATH = os.path.join(_HERE, "build", "libkf_oracle_f32.so")
+
+
+def _float_struct(name, base):
+    fields = [(f, C.c_float if t is C.c_double else (C.c_float * t._length_ if getattr(t, "_type_", None) is C.c_double
+                                                     else t)) for f, t in base._fields_]
+    return type(name, (C.Structure,), {"_fields_": fields})
+
+
+Grid32 = _float_struct("Grid32", Grid)
+Params32 = _float_struct("Params32", Params)
+Stats32 = _float_struct("Stats32", Stats)
+_lib32 = None
+
+
+def lib32():
+    global _lib32
+    if _lib32 is None:
+        if not os.path.exists(_LIB32_PATH):
+            subprocess.run(["make", "-s", "-C", _HERE], check=True)
+        L = C.CDLL(_LIB32_PATH)
+        fp = C.POINTER(C.c_float)
+        L.kfo_deform_jacobi.restype = C.c_int
+        L.kfo_deform_jacobi.argtypes = [fp, fp, fp, C.POINTER(Grid32), C.POINTER(Params32), C.POINTER(Stats32)]
+        _lib32 = L
+    return _lib32
+
+
+def _to32(struct32, src):
+    out = struct32()
+    for f, _ in src._fields_:
+        v = getattr(src, f)
+        if hasattr(v, "__len__"):
+            for i in range(len(v)):
+                getattr(out, f)[i] = v[i]
+        else:
+            setattr(out, f, v)
+    return out
+
+
+def deform_jacobi_f32(live, can, u, g: OGrid, p: OParams):
+    """kfo_deform_jacobi evaluated in FP32 throughout; returns (u_new as float64, stats)."""
+    fp = C.POINTER(C.c_float)
+    live = np.ascontiguousarray(live, dtype=np.float32)
+    can = np.ascontiguousarray(can, dtype=np.float32)
+    u32 = np.array(u, dtype=np.float32, copy=True, order="C")
+    st = Stats32()
+    lib32().kfo_deform_jacobi(live.ctypes.data_as(fp), can.ctypes.data_as(fp), u32.ctypes.data_as(fp),
+                              C.byref(_to32(Grid32, g.c())), C.byref(_to32(Params32, p.c())), C.byref(st))
+    return u32.astype(np.float64), dict(iters=st.iters, diverged_index=st.diverged_index,
+                                        max_update=float(st.max_update), active=st.active)

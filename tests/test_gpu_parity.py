@@ -161,9 +161,10 @@ def test_reference_deform_fixed_iterations(ctx, c1):
 def test_reference_deform_paper_params_bounded(ctx, c1):
     # With the paper parameters (alpha 0.1, omega_k 0.5) the clamped Killing step of
     # DisplacementField.cpp:342-345 (0.5 voxel) makes the Jacobi map expanding: per-evaluation
-    # FP32 rounding (~1e-5 on a Killing gradient of ~5) grows to ~1e-3 voxel at a few hundred
-    # voxels within 12 iterations.  The same C code compiled in float shows the same spread, so
-    # this bounds the FP32 divergence rather than asserting 1e-4 parity.
+    # FP32 rounding grows to ~1e-3 voxel at some voxels within 12 iterations.  The oracle's own
+    # code evaluated in FP32 (oracle/kf_oracle_f32.c) spreads as far from its FP64 result
+    # (tests/test_oracle_fp32_spread.py), so the GPU's FP32 path is held to that spread,
+    # measured here on the same grids, rather than to 1e-4 parity.
     print("case: reference deform 64^3, 12 iterations, paper parameters", flush=True)
     kp = c1_params(max_iters=12)
     field = DisplacementField(ctx, C1["dims"])
@@ -172,12 +173,15 @@ def test_reference_deform_paper_params_bounded(ctx, c1):
     lphi, _ = c1["live"].download()
     cphi, _ = c1["can"].download()
     uo, so = O.deform_jacobi(lphi, cphi, np.zeros((3, c1["g"].n)), c1["g"], oparams(kp))
-    err = err_report("ref 12 it paper", ug, uo)
+    u32, _ = O.deform_jacobi_f32(lphi, cphi, np.zeros((3, c1["g"].n)), c1["g"], oparams(kp))
+    err = err_report("ref 12 it paper, GPU FP32 vs FP64", ug, uo)
+    e32 = err_report("ref 12 it paper, CPU FP32 vs FP64", u32, uo)
     active = int(so["active"])
     assert st["iters"] == so["iters"] == 12
     assert abs(int(st["active"]) - active) <= 5
-    assert np.quantile(err, 0.999) <= 5e-4 and err.max() <= 1e-2
-    assert (err > 1e-4).sum() <= 0.15 * active
+    assert err.max() <= 4 * e32.max()
+    assert np.quantile(err, 0.999) <= 4 * np.quantile(e32, 0.999)
+    assert (err > 1e-4).sum() <= 1.5 * (e32 > 1e-4).sum()
 
 
 def test_adjoint_gradient_and_energy_match_oracle(ctx, c1):
