@@ -27,6 +27,8 @@
 // has the same per-voxel terms, summed in FP64 in another order.
 #include "kf_adjoint.hpp"
 
+#include <cstdio>
+
 #include <cstring>
 #include <string>
 #include <vector>
@@ -876,6 +878,8 @@ bool kf_active_supported(const kf_volume *live) { return 3 * live->n < ((int64_t
 void kf_active_release(kf_context *ctx)
 {
     kf_active_ws &a = ctx->active;
+    a.loop.drop();
+    a.batch_loop.drop();
     (void)hipFree(a.mask);
     (void)hipFree(a.span_count);
     (void)hipFree(a.pos_of);
@@ -988,6 +992,36 @@ kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const 
     });
     KF_LAUNCH_CHECK();
     return KF_OK;
+}
+
+kf_status kf_active_loop(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count)
+{
+    kf_context *ctx = live->ctx;
+    kf_active_ws &a = ctx->active;
+    const int K = p->max_iters;
+    const unsigned int nb = count ? (count + LB - 1) / LB : 1u;
+    kf_status s = kf_ensure_partials(ctx, nb);  // before any capture: the launches allocate nothing
+    if (s != KF_OK || K <= 0)
+        return s;
+    const AdjConsts c = kf_adj_consts(p);
+    char buf[320];
+    std::snprintf(buf, sizeof(buf), "%d/%u/%d/%p/%p/%p/%p/%p/%zu/%p/%zu/%p/%d/%d/%d/%d/%d", live->dtype, count, K,
+                  live->phi, can->phi, (void *)a.u, (void *)a.list, (void *)a.nbr, a.cap, (void *)ctx->partials,
+                  ctx->partials_cap, (void *)ctx->state, live->dims[0], live->dims[1], live->dims[2],
+                  (int)(live->scale() != can->scale()), (int)(p->stop_max_update > 0.0));
+    std::string key(buf);
+    key.append(reinterpret_cast<const char *>(&c), sizeof(c));
+    const float sc[4] = {live->scale(), can->scale(), (float)live->oob(), (float)can->oob()};
+    key.append(reinterpret_cast<const char *>(sc), sizeof(sc));
+    return kf_run_captured(ctx, a.loop, key, [&]() -> kf_status {
+        unsigned int nbo = 0;
+        for (int k = 0; k < K; k++) {
+            const kf_status r = kf_launch_adj_list(live, can, p, k, count, &nbo);
+            if (r != KF_OK)
+                return r;
+        }
+        return kf_launch_finalize(ctx, nbo, (K - 1) & 1);
+    });
 }
 
 kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count)
@@ -1187,7 +1221,37 @@ kf_status batch_active(kf_context *ctx, const kf_volume *const *live, const kf_v
     const bool check = p->stop_max_update > 0.0;
     const unsigned int set = (unsigned int)a.cap;
     int launched = 0;
-    for (int k = 0; k < p->max_iters; k++) {
+    if (!check && p->max_iters > 0) {
+        // Without a stop rule the loop needs no host wait: all launches and the last fold run
+        // as one captured graph when the same batch shape comes back.
+        const int K = p->max_iters;
+        char buf[320];
+        std::snprintf(buf, sizeof(buf), "%d/%d/%u/%d/%p/%p/%p/%u/%p/%p/%zu/%p/%p", (int)S::dtype, count, nwb, K,
+                      (void *)dtab, (void *)a.blk_vol, (void *)a.u, set, (void *)a.bstates, (void *)ctx->partials,
+                      ctx->partials_cap, (void *)a.list, (void *)a.nbr);
+        std::string key(buf);
+        key.append(reinterpret_cast<const char *>(&c), sizeof(c));
+        s = kf_run_captured(ctx, a.batch_loop, key, [&]() -> kf_status {
+            for (int k = 0; k < K; k++) {
+                const int par = k & 1;
+                hipLaunchKernelGGL((k_adj_list_batch<S>), dim3((unsigned int)count + nwb), dim3(LB), 0, st, dtab,
+                                   count, a.blk_vol, a.u + (size_t)par * set, a.u + (size_t)(par ^ 1) * set, c,
+                                   a.bstates, kf_partials(ctx, par), par, kf_partials(ctx, par ^ 1), k > 0 ? 1 : 0, 0,
+                                   k == K - 1 ? 1 : 0, a.list, a.nbr);
+                KF_LAUNCH_CHECK();
+            }
+            const int par = K & 1;
+            hipLaunchKernelGGL((k_adj_list_batch<S>), dim3((unsigned int)count), dim3(LB), 0, st, dtab, count,
+                               a.blk_vol, a.u, a.u, c, a.bstates, kf_partials(ctx, par), par,
+                               kf_partials(ctx, par ^ 1), 1, 1, 0, a.list, a.nbr);
+            KF_LAUNCH_CHECK();
+            return KF_OK;
+        });
+        if (s != KF_OK)
+            return s;
+        launched = -1;  // folded already
+    }
+    for (int k = 0; launched >= 0 && k < p->max_iters; k++) {
         const int par = k & 1;
         const int publish = check || k == p->max_iters - 1;
         hipLaunchKernelGGL((k_adj_list_batch<S>), dim3((unsigned int)count + nwb), dim3(LB), 0, st, dtab, count,

@@ -636,7 +636,14 @@ kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *
     // launch k + 1 sees `done` and does nothing.
     int launched = 0;
     unsigned int nb = 0;
-    for (int k = 0; k < p->max_iters; k++) {
+    if (listed && !check && p->max_iters > 0) {
+        // the whole loop and its last fold, as one captured graph when this list shape recurs
+        s = kf_active_loop(live, can, p, count);
+        if (s != KF_OK)
+            return s;
+        launched = -1;  // folded already
+    }
+    for (int k = 0; launched >= 0 && k < p->max_iters; k++) {
         float *in = u->buf[cur0 ^ (k & 1)];
         float *out = u->buf[cur0 ^ ((k + 1) & 1)];
         s = listed ? kf_launch_adj_list(live, can, p, k, count, &nb)

@@ -73,6 +73,26 @@ struct BlockPartial {
 // Host handles
 // ---------------------------------------------------------------------------
 
+// A launch sequence captured as a hipGraph and replayed while its key (everything the captured
+// launches read by value: pointers, counts, constants) stays the same.  A sequence runs
+// directly the first time a key is seen and is captured when the same key comes back.
+struct kf_loop_graph {
+    std::string key;   // key of the captured graph
+    std::string seen;  // key of the last sequence run directly
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void drop()
+    {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        key.clear();
+    }
+};
+
 // Device workspace of the active-set sweep, grown on demand and kept by the context.
 struct kf_active_ws {
     unsigned long long *mask = nullptr;  // one bit per voxel: listed
@@ -113,6 +133,8 @@ struct kf_active_ws {
     const kf_volume *warp_vol = nullptr;
     const kf_field *warp_field = nullptr;
     unsigned int warp_count = 0;
+    // the iteration loops of kf_deform and kf_batch_deform over the lists (no stop rule)
+    kf_loop_graph loop, batch_loop;
 };
 
 // Device workspace of the reference-formula step (k_deform_ref.hip), grown on demand and kept
@@ -464,6 +486,48 @@ kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params
 kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int count, bool neighbours = true);
 kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const kf_params *p, int iter,
                              unsigned int count, unsigned int *nb_out);
+// All p->max_iters step launches of a call without a stop rule and the fold of the last one,
+// replayed as a captured hipGraph when the same list shape comes back (kf_loop_graph).
+kf_status kf_active_loop(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count);
+// Runs launches() directly, or as the hipGraph g holds for key (capturing it on the key's
+// second run); launches() must only launch kernels on the context's stream (no allocation, no
+// memset, no host wait).
+template <class F>
+kf_status kf_run_captured(kf_context *ctx, kf_loop_graph &g, const std::string &key, F launches)
+{
+    if (g.exec && key == g.key) {
+        KF_HIP(hipGraphLaunch(g.exec, ctx->stream));
+        return KF_OK;
+    }
+    if (key != g.seen) {
+        g.seen = key;
+        return launches();
+    }
+    g.drop();
+    KF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    kf_status s = launches();
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+    if (s != KF_OK || ec != hipSuccess || !graph) {
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        if (s == KF_OK) {
+            kf_set_error(std::string("loop capture: ") + hipGetErrorString(ec));
+            s = KF_E_HIP;
+        }
+        return s;
+    }
+    g.graph = graph;
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        g.drop();
+        kf_set_error(std::string("loop graph instantiate: ") + hipGetErrorString(ei));
+        return KF_E_HIP;
+    }
+    g.key = key;
+    KF_HIP(hipGraphLaunch(g.exec, ctx->stream));
+    return KF_OK;
+}
 // Writes the final compact set (state->iters & 1, read on the device) back into u.
 kf_status kf_active_scatter(const kf_volume *live, float *u, unsigned int count);
 // kf_batch_deform over the voxels that can move in each volume (one launch per iteration for

@@ -196,3 +196,19 @@ def test_active_equals_dense_256(ctx):
     assert b[1]["listed"] == listed_on_cpu(live, None, kp)
     live.close()
     can.close()
+
+
+def test_active_loop_replayed_as_a_graph(ctx, sphere):
+    """Without a stop rule the iteration loop of an active-set call runs directly the first time
+    its list shape is seen, is captured as a hipGraph the second time and replayed after that:
+    every call gives the dense sweep's field and statistics bit for bit, with other work on the
+    stream in between (a call of another shape, which the cache must not mistake for this one)."""
+    live, can = sphere["bf16"]
+    kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint", max_iters=30)
+    ref = run(ctx, live, can, kp, "dense")
+    l32, c32 = sphere["f32"]
+    for call in range(5):
+        print(f"case: 64^3 sphere, bf16, 30 iterations, active call {call} (direct, captured, replayed)", flush=True)
+        assert_same(ref, run(ctx, live, can, kp, "active"), f"active call {call}")
+        if call == 2:  # a different shape in between, then this one again
+            run(ctx, l32, c32, dataclasses.replace(kp, max_iters=7), "active")
