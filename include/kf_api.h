@@ -190,10 +190,18 @@ kf_status kf_deform(const kf_volume *live, const kf_volume *can, kf_field *u, co
                     kf_deform_stats *stats);
 /* SDF::update(displacementField), SDF.cpp:300-328. */
 kf_status kf_warp(kf_volume *vol, const kf_field *u);
+/* The FUSE_BY_MERGE switch of the reference's fuse (config.cpp:7). */
+typedef enum kf_fuse_mode_t {
+    KF_FUSE_MERGE = 0,   /* weighted average, weights summed (FUSE_BY_MERGE true, the reference's value) */
+    KF_FUSE_NEAREST = 1  /* the value nearer the surface wins, with its own weight (FUSE_BY_MERGE false) */
+} kf_fuse_mode_t;
 /* SDF::fuse(otherSdf), SDF.cpp:199-241 (FUSE_BY_MERGE). */
 kf_status kf_fuse(kf_volume *can, const kf_volume *live);
+/* The same with either branch of FUSE_BY_MERGE (a kf_fuse_mode_t). */
+kf_status kf_fuse_mode(kf_volume *can, const kf_volume *live, int32_t mode);
 /* SDF::fuse(otherSdf, displacementField), SDF.cpp:244-298. */
 kf_status kf_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *u);
+kf_status kf_fuse_warped_mode(kf_volume *can, const kf_volume *live, const kf_field *u, int32_t mode);
 /* SDF::getMesh (SDF.cpp:396-414) + ProcessVolumeCell (SDF.cpp:330-378): triangle soup in the
  * reference's order, 9 floats per triangle (3 vertices, y negated as SimpleMesh::AddVertex,
  * SimpleMesh.h:66-72); faces are implicit (3t, 3t+1, 3t+2).  *n_tris always receives the
@@ -286,6 +294,8 @@ typedef struct kf_pipeline_desc {
     double C2W[16];
     double min_depth, max_depth;
     int32_t width, height;
+    int32_t fuse_mode;       /* kf_fuse_mode_t of the per-frame fuse (0: KF_FUSE_MERGE, the reference's) */
+    int32_t reserved;
 } kf_pipeline_desc;
 
 typedef struct kf_frame_stats {

@@ -16,6 +16,7 @@ KF_F32, KF_BF16, KF_F16 = 0, 1, 2
 KF_METHOD_REFERENCE, KF_METHOD_ADJOINT = 0, 1
 KF_SWEEP_AUTO, KF_SWEEP_DENSE, KF_SWEEP_ACTIVE = 0, 1, 2
 KF_MESH_LIVE, KF_MESH_LIVE_WARPED, KF_MESH_CANONICAL = 1, 2, 4
+KF_FUSE_MERGE, KF_FUSE_NEAREST = 0, 1
 
 DTYPES = {"f32": KF_F32, "bf16": KF_BF16, "f16": KF_F16}
 
@@ -54,7 +55,7 @@ class RigidStats(C.Structure):
 class PipelineDesc(C.Structure):
     _fields_ = [("volume", VolumeDesc), ("params", Params), ("K", C.c_double * 9), ("C2W", C.c_double * 16),
                 ("min_depth", C.c_double), ("max_depth", C.c_double), ("width", C.c_int32),
-                ("height", C.c_int32)]
+                ("height", C.c_int32), ("fuse_mode", C.c_int32), ("reserved", C.c_int32)]
 
 
 class FrameStats(C.Structure):
@@ -102,6 +103,8 @@ SIGNATURES = {
     "kf_warp": (C.c_int, [_vp, _vp]),
     "kf_fuse": (C.c_int, [_vp, _vp]),
     "kf_fuse_warped": (C.c_int, [_vp, _vp, _vp]),
+    "kf_fuse_mode": (C.c_int, [_vp, _vp, C.c_int32]),
+    "kf_fuse_warped_mode": (C.c_int, [_vp, _vp, _vp, C.c_int32]),
     "kf_marching_cubes": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_uint64), _fp, C.c_uint64]),
     "kf_marching_cubes_indexed": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _fp, _fp,
                                             C.POINTER(C.c_uint32), C.c_uint64, C.c_uint64]),

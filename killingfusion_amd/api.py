@@ -251,12 +251,15 @@ class SDF:
         check(self._L.kf_integrate_depth(self.handle, depth.ctypes.data_as(_fp), w, h, Km.ctypes.data_as(_dp),
                                          Cm.ctypes.data_as(_dp), min_depth, max_depth), "kf_integrate_depth")
 
-    def fuse(self, other: "SDF", displacement_field: "DisplacementField" = None):
-        """SDF::fuse(other) (SDF.cpp:199-241) or SDF::fuse(other, disp) (SDF.cpp:244-298)."""
+    def fuse(self, other: "SDF", displacement_field: "DisplacementField" = None, merge: bool = True):
+        """SDF::fuse(other) (SDF.cpp:199-241) or SDF::fuse(other, disp) (SDF.cpp:244-298); merge is
+        FUSE_BY_MERGE (config.cpp:7): False keeps, per voxel, the value nearer the surface."""
+        mode = _lib.KF_FUSE_MERGE if merge else _lib.KF_FUSE_NEAREST
         if displacement_field is None:
-            check(self._L.kf_fuse(self.handle, other.handle), "kf_fuse")
+            check(self._L.kf_fuse_mode(self.handle, other.handle, mode), "kf_fuse_mode")
         else:
-            check(self._L.kf_fuse_warped(self.handle, other.handle, displacement_field.handle), "kf_fuse_warped")
+            check(self._L.kf_fuse_warped_mode(self.handle, other.handle, displacement_field.handle, mode),
+                  "kf_fuse_warped_mode")
 
     def update(self, displacement_field: "DisplacementField"):
         """SDF::update (SDF.cpp:300-328): warp by the field."""
@@ -464,7 +467,8 @@ class KillingFusion:
     """Per-frame driver (KillingFusion::processNextFrame, KillingFusion.cpp:109-175) on the GPU."""
 
     def __init__(self, ctx: Context, dims, min3d, voxel_size, params: KillingParams, K, C2W=None,
-                 min_depth=0.1, max_depth=3.0, width=640, height=480, dtype="f32", epsilon=1e-5):
+                 min_depth=0.1, max_depth=3.0, width=640, height=480, dtype="f32", epsilon=1e-5,
+                 fuse_by_merge: bool = True):
         self._L = _lib.load()
         self.ctx = ctx
         d = _lib.PipelineDesc()
@@ -484,6 +488,7 @@ class KillingFusion:
         for i in range(16):
             d.C2W[i] = Cm[i]
         d.min_depth, d.max_depth, d.width, d.height = min_depth, max_depth, width, height
+        d.fuse_mode = _lib.KF_FUSE_MERGE if fuse_by_merge else _lib.KF_FUSE_NEAREST
         h = C.c_void_p()
         check(self._L.kf_pipeline_create(ctx.handle, C.byref(d), C.byref(h)), "kf_pipeline_create")
         self.handle = h

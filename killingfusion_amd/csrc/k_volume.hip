@@ -279,8 +279,9 @@ __global__ void k_warp_scatter(const unsigned int *__restrict__ list, unsigned i
     w[i] = new_w[t];
 }
 
-// SDF::fuse(otherSdf), SDF.cpp:199-241, FUSE_BY_MERGE.
-template <class S1, class S2>
+// SDF::fuse(otherSdf), SDF.cpp:199-241: MERGE = FUSE_BY_MERGE (config.cpp:7), the weighted
+// average; otherwise the value nearer the surface wins with its own weight.
+template <class S1, class S2, bool MERGE>
 __global__ void k_fuse(typename S1::T *phi1, int32_t *w1, float scale1, const typename S2::T *__restrict__ phi2,
                        const int32_t *__restrict__ w2, float scale2, int64_t n)
 {
@@ -295,16 +296,19 @@ __global__ void k_fuse(typename S1::T *phi1, int32_t *w1, float scale1, const ty
     if (a == 0) {
         phi1[i] = S1::enc((float)d2, scale1);
         w1[i] = b;
-    } else {
+    } else if (MERGE) {
         const double d1 = (double)S1::load(phi1, i, scale1);
         const double nv = ((double)a * d1 + (double)b * d2) / (double)(a + b);
         phi1[i] = S1::enc((float)nv, scale1);
         w1[i] = a + b;
+    } else if (!(fabs((double)S1::load(phi1, i, scale1)) < fabs(d2))) {
+        phi1[i] = S1::enc((float)d2, scale1);
+        w1[i] = b;
     }
 }
 
 // SDF::fuse(otherSdf, displacementField), SDF.cpp:244-298.
-template <class S1, class S2>
+template <class S1, class S2, bool MERGE>
 __global__ void k_fuse_warped(typename S1::T *phi1, int32_t *w1, float scale1, Reader<S2> R,
                               const float *__restrict__ u)
 {
@@ -325,11 +329,14 @@ __global__ void k_fuse_warped(typename S1::T *phi1, int32_t *w1, float scale1, R
     if (a == 0) {
         phi1[i] = S1::enc((float)d2, scale1);
         w1[i] = (int32_t)(long long)b;
-    } else {
+    } else if (MERGE) {
         const double d1 = (double)S1::load(phi1, i, scale1);
         const double nv = ((double)a * d1 + b * d2) / ((double)a + b);
         phi1[i] = S1::enc((float)nv, scale1);
         w1[i] = (int32_t)(long long)((double)a + b);
+    } else if (!(fabs((double)S1::load(phi1, i, scale1)) < fabs(d2))) {
+        phi1[i] = S1::enc((float)d2, scale1);
+        w1[i] = (int32_t)(long long)b;
     }
 }
 
@@ -446,28 +453,38 @@ kf_status kf_launch_warp(kf_volume *v, const kf_field *u, void *tmp_phi, int32_t
     return KF_OK;
 }
 
-kf_status kf_launch_fuse(kf_volume *can, const kf_volume *live)
+kf_status kf_launch_fuse(kf_volume *can, const kf_volume *live, bool merge)
 {
     kf_touch(can);
     KF_DISPATCH_DTYPE(can->dtype, S1, {
         KF_DISPATCH_DTYPE(live->dtype, S2, {
-            hipLaunchKernelGGL((k_fuse<S1, S2>), dim3(blocks_for(can->n)), dim3(BLOCK), 0, can->ctx->stream,
-                               (typename S1::T *)can->phi, can->w, can->scale(),
-                               (const typename S2::T *)live->phi, (const int32_t *)live->w, live->scale(), can->n);
+            if (merge)
+                hipLaunchKernelGGL((k_fuse<S1, S2, true>), dim3(blocks_for(can->n)), dim3(BLOCK), 0, can->ctx->stream,
+                                   (typename S1::T *)can->phi, can->w, can->scale(),
+                                   (const typename S2::T *)live->phi, (const int32_t *)live->w, live->scale(), can->n);
+            else
+                hipLaunchKernelGGL((k_fuse<S1, S2, false>), dim3(blocks_for(can->n)), dim3(BLOCK), 0,
+                                   can->ctx->stream, (typename S1::T *)can->phi, can->w, can->scale(),
+                                   (const typename S2::T *)live->phi, (const int32_t *)live->w, live->scale(), can->n);
         });
     });
     KF_LAUNCH_CHECK();
     return KF_OK;
 }
 
-kf_status kf_launch_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *u)
+kf_status kf_launch_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *u, bool merge)
 {
     kf_touch(can);
     KF_DISPATCH_DTYPE(can->dtype, S1, {
         KF_DISPATCH_DTYPE(live->dtype, S2, {
-            hipLaunchKernelGGL((k_fuse_warped<S1, S2>), dim3(blocks_for(can->n)), dim3(BLOCK), 0,
-                               can->ctx->stream, (typename S1::T *)can->phi, can->w, can->scale(),
-                               make_reader<S2>(live), (const float *)u->u());
+            if (merge)
+                hipLaunchKernelGGL((k_fuse_warped<S1, S2, true>), dim3(blocks_for(can->n)), dim3(BLOCK), 0,
+                                   can->ctx->stream, (typename S1::T *)can->phi, can->w, can->scale(),
+                                   make_reader<S2>(live), (const float *)u->u());
+            else
+                hipLaunchKernelGGL((k_fuse_warped<S1, S2, false>), dim3(blocks_for(can->n)), dim3(BLOCK), 0,
+                                   can->ctx->stream, (typename S1::T *)can->phi, can->w, can->scale(),
+                                   make_reader<S2>(live), (const float *)u->u());
         });
     });
     KF_LAUNCH_CHECK();

@@ -844,22 +844,33 @@ kf_status kf_warp(kf_volume *v, const kf_field *u)
     return KF_OK;
 }
 
-kf_status kf_fuse(kf_volume *can, const kf_volume *live)
+kf_status kf_fuse_mode(kf_volume *can, const kf_volume *live, int32_t mode)
 {
     KF_CHECK_ARG(can && live, "null argument");
     KF_CHECK_ARG(same_dims(can->dims, live->dims), "volumes differ in dims");
     KF_CHECK_ARG(can != live, "fusing a volume into itself needs a copy first");
+    KF_CHECK_ARG(can->ctx == live->ctx, "handles belong to different contexts");
+    KF_CHECK_ARG(mode == KF_FUSE_MERGE || mode == KF_FUSE_NEAREST, "unknown fuse mode");
     DeviceGuard dg(can->ctx->device);
-    return kf_launch_fuse(can, live);
+    return kf_launch_fuse(can, live, mode == KF_FUSE_MERGE);
 }
 
-kf_status kf_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *u)
+kf_status kf_fuse(kf_volume *can, const kf_volume *live) { return kf_fuse_mode(can, live, KF_FUSE_MERGE); }
+
+kf_status kf_fuse_warped_mode(kf_volume *can, const kf_volume *live, const kf_field *u, int32_t mode)
 {
     KF_CHECK_ARG(can && live && u, "null argument");
     KF_CHECK_ARG(same_dims(can->dims, live->dims) && same_dims(can->dims, u->dims), "dims differ");
     KF_CHECK_ARG(can != live, "fusing a volume into itself needs a copy first");
+    KF_CHECK_ARG(can->ctx == live->ctx && can->ctx == u->ctx, "handles belong to different contexts");
+    KF_CHECK_ARG(mode == KF_FUSE_MERGE || mode == KF_FUSE_NEAREST, "unknown fuse mode");
     DeviceGuard dg(can->ctx->device);
-    return kf_launch_fuse_warped(can, live, u);
+    return kf_launch_fuse_warped(can, live, u, mode == KF_FUSE_MERGE);
+}
+
+kf_status kf_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *u)
+{
+    return kf_fuse_warped_mode(can, live, u, KF_FUSE_MERGE);
 }
 
 kf_status kf_marching_cubes(const kf_volume *v, double iso, uint64_t *n_tris, float *verts, uint64_t cap_tris)

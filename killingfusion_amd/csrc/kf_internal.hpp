@@ -451,8 +451,10 @@ kf_status kf_launch_integrate(kf_volume *v, const float *ddepth, int w, int h, c
 kf_status kf_integrate_depth_impl(kf_volume *v, const float *depth, int32_t width, int32_t height, const double K[9],
                                   const double C2W[16], double min_depth, double max_depth, bool fresh);
 kf_status kf_launch_warp(kf_volume *v, const kf_field *u, void *tmp_phi, int32_t *tmp_w);
-kf_status kf_launch_fuse(kf_volume *can, const kf_volume *live);
-kf_status kf_launch_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *u);
+// merge: FUSE_BY_MERGE (config.cpp:7), the weighted average; otherwise the value nearer the
+// surface wins (SDF.cpp:226-237, 285-295).
+kf_status kf_launch_fuse(kf_volume *can, const kf_volume *live, bool merge = true);
+kf_status kf_launch_fuse_warped(kf_volume *can, const kf_volume *live, const kf_field *u, bool merge = true);
 kf_status kf_launch_field_fill(kf_field *f, const double v[3]);
 kf_status kf_launch_field_add(kf_field *dst, const kf_field *src);
 

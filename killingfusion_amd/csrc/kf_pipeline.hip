@@ -89,6 +89,7 @@ kf_status kf_pipeline_create(kf_context *ctx, const kf_pipeline_desc *desc, kf_p
     KF_CHECK_ARG(ctx && desc && out, "null argument");
     *out = nullptr;
     KF_CHECK_ARG(desc->width > 0 && desc->height > 0, "bad frame size");
+    KF_CHECK_ARG(desc->fuse_mode == KF_FUSE_MERGE || desc->fuse_mode == KF_FUSE_NEAREST, "unknown fuse mode");
     kf_pipeline *pl = new kf_pipeline();
     pl->ctx = ctx;
     pl->desc = *desc;
@@ -161,7 +162,7 @@ kf_status kf_pipeline_process_frame(kf_pipeline *pl, const float *depth, kf_fram
                 s = kf_warp(pl->live, pl->field);
         }
         if (s == KF_OK)
-            s = kf_fuse(pl->can, pl->live);
+            s = kf_fuse_mode(pl->can, pl->live, d.fuse_mode);
         if (s == KF_OK && (want & KF_MESH_LIVE_WARPED))
             s = extract(pl, 1, pl->live);
         KF_HIP(hipEventRecord(pl->ev[3], st));

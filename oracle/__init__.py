@@ -89,6 +89,8 @@ def lib():
             "kfo_warp": (None, [_dp, _i64p, _dp, g]),
             "kfo_fuse": (None, [_dp, _i64p, _dp, _i64p, C.c_int64]),
             "kfo_fuse_warped": (None, [_dp, _i64p, _dp, _i64p, _dp, g]),
+            "kfo_fuse_mode": (None, [_dp, _i64p, _dp, _i64p, C.c_int64, C.c_int]),
+            "kfo_fuse_warped_mode": (None, [_dp, _i64p, _dp, _i64p, _dp, g, C.c_int]),
             "kfo_marching_cubes": (C.c_int64, [_dp, g, C.c_double, C.c_double, C.c_double, _dp, C.c_int64]),
             "kfo_mc_edge_mask": (C.c_int, [C.c_int]),
             "kfo_mc_case": (None, [C.c_int, C.POINTER(C.c_int32)]),
@@ -300,17 +302,20 @@ def warp(phi, w, u, g: OGrid):
     lib().kfo_warp(_d(phi), w.ctypes.data_as(_i64p), _d(u), C.byref(g.c()))
 
 
-def fuse(phi1, w1, phi2, w2):
+def fuse(phi1, w1, phi2, w2, merge=True):
+    """SDF::fuse (SDF.cpp:199-241) in place; merge = FUSE_BY_MERGE (config.cpp:7)."""
     phi2 = _c64(phi2)
     w2 = np.ascontiguousarray(w2, dtype=np.int64)
-    lib().kfo_fuse(_d(phi1), w1.ctypes.data_as(_i64p), _d(phi2), w2.ctypes.data_as(_i64p), phi1.size)
+    lib().kfo_fuse_mode(_d(phi1), w1.ctypes.data_as(_i64p), _d(phi2), w2.ctypes.data_as(_i64p), phi1.size,
+                        int(bool(merge)))
 
 
-def fuse_warped(phi1, w1, phi2, w2, u, g: OGrid):
+def fuse_warped(phi1, w1, phi2, w2, u, g: OGrid, merge=True):
+    """SDF::fuse(other, disp) (SDF.cpp:244-298) in place; merge = FUSE_BY_MERGE."""
     phi2, u = _c64(phi2), _c64(u)
     w2 = np.ascontiguousarray(w2, dtype=np.int64)
-    lib().kfo_fuse_warped(_d(phi1), w1.ctypes.data_as(_i64p), _d(phi2), w2.ctypes.data_as(_i64p),
-                          _d(u), C.byref(g.c()))
+    lib().kfo_fuse_warped_mode(_d(phi1), w1.ctypes.data_as(_i64p), _d(phi2), w2.ctypes.data_as(_i64p),
+                               _d(u), C.byref(g.c()), int(bool(merge)))
 
 
 def marching_cubes(phi, g: OGrid, max_surface, clip, iso=0.0):

@@ -696,7 +696,7 @@ void kfo_warp(double *phi, int64_t *w, const double *u, const kfo_grid *g)
 }
 
 /* SDF::fuse(other), SDF.cpp:199-241 with FUSE_BY_MERGE = true (config.cpp:7). */
-void kfo_fuse(double *phi1, int64_t *w1, const double *phi2, const int64_t *w2, int64_t n)
+void kfo_fuse_mode(double *phi1, int64_t *w1, const double *phi2, const int64_t *w2, int64_t n, int merge)
 {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
@@ -708,18 +708,25 @@ void kfo_fuse(double *phi1, int64_t *w1, const double *phi2, const int64_t *w2, 
         if (a == 0) {
             phi1[i] = d2;
             w1[i] = b;
-        } else {
+        } else if (merge) {
             double d1 = phi1[i];
             phi1[i] = ((double)a * d1 + (double)b * d2) / (double)(a + b);
             w1[i] = a + b;
+        } else if (!(fabs(phi1[i]) < fabs(d2))) {
+            /* FUSE_BY_MERGE false (SDF.cpp:229-236): the value nearer the surface wins */
+            phi1[i] = d2;
+            w1[i] = b;
         }
     }
 }
 
-/* SDF::fuse(other, disp), SDF.cpp:244-298: weight interpolated in double, skip
- * when |w2| < 1e-5f, weights truncated to long on store. */
-void kfo_fuse_warped(double *phi1, int64_t *w1, const double *phi2, const int64_t *w2,
-                     const double *u, const kfo_grid *g)
+void kfo_fuse(double *phi1, int64_t *w1, const double *phi2, const int64_t *w2, int64_t n)
+{
+    kfo_fuse_mode(phi1, w1, phi2, w2, n, 1);
+}
+
+void kfo_fuse_warped_mode(double *phi1, int64_t *w1, const double *phi2, const int64_t *w2,
+                          const double *u, const kfo_grid *g, int merge)
 {
 #pragma omp parallel for schedule(static)
     for (int z = 0; z < g->n[2]; z++)
@@ -737,12 +744,22 @@ void kfo_fuse_warped(double *phi1, int64_t *w1, const double *phi2, const int64_
                 if (a == 0) {
                     phi1[i] = d2;
                     w1[i] = (int64_t)b;
-                } else {
+                } else if (merge) {
                     double d1 = phi1[i];
                     phi1[i] = ((double)a * d1 + b * d2) / ((double)a + b);
                     w1[i] = (int64_t)((double)a + b);
+                } else if (!(fabs(phi1[i]) < fabs(d2))) {
+                    /* FUSE_BY_MERGE false (SDF.cpp:285-295) */
+                    phi1[i] = d2;
+                    w1[i] = (int64_t)b;
                 }
             }
+}
+
+void kfo_fuse_warped(double *phi1, int64_t *w1, const double *phi2, const int64_t *w2,
+                     const double *u, const kfo_grid *g)
+{
+    kfo_fuse_warped_mode(phi1, w1, phi2, w2, u, g, 1);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -140,8 +140,14 @@ int kfo_invert4(const double m[16], double out[16]);
 /* ---- a18-a20: warp and fuse ---- */
 void kfo_warp(double *phi, int64_t *w, const double *u, const kfo_grid *g);
 void kfo_fuse(double *phi1, int64_t *w1, const double *phi2, const int64_t *w2, int64_t n);
+/* SDF::fuse with FUSE_BY_MERGE = merge (config.cpp:7; SDF.cpp:199-241): 0 keeps, per voxel, the
+ * value nearer the surface with its own weight. */
+void kfo_fuse_mode(double *phi1, int64_t *w1, const double *phi2, const int64_t *w2, int64_t n, int merge);
 void kfo_fuse_warped(double *phi1, int64_t *w1, const double *phi2, const int64_t *w2,
                      const double *u, const kfo_grid *g);
+/* SDF::fuse(other, disp) with FUSE_BY_MERGE = merge (SDF.cpp:244-298). */
+void kfo_fuse_warped_mode(double *phi1, int64_t *w1, const double *phi2, const int64_t *w2,
+                          const double *u, const kfo_grid *g, int merge);
 
 /* ---- a21-a23: marching cubes ---- */
 /* Returns the triangle count; writes min(count, cap) triangles (9 doubles each,

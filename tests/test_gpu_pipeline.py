@@ -12,7 +12,7 @@ from helpers import C1, ogrid, oparams, sphere_frames
 pytestmark = pytest.mark.gpu
 
 
-def oracle_sequence(frames, g, kp, adjoint):
+def oracle_sequence(frames, g, kp, adjoint, merge=True):
     trunc = C1["trunc"]
     can_phi, can_w = O.sdf_init(g)
     iters = []
@@ -27,24 +27,26 @@ def oracle_sequence(frames, g, kp, adjoint):
         u, so = fn(lphi, can_phi, np.zeros((3, g.n)), g, oparams(kp))
         iters.append(so["iters"])
         O.warp(lphi, lw, u, g)
-        O.fuse(can_phi, can_w, lphi, lw)
+        O.fuse(can_phi, can_w, lphi, lw, merge=merge)
     return can_phi, can_w, iters
 
 
-@pytest.mark.parametrize("method", ["reference", "adjoint"])
+@pytest.mark.parametrize("method", ["reference", "adjoint", "reference-nearest"])
 def test_pipeline_matches_oracle_stages(ctx, method):
     print(f"case: pipeline, 3 frames of the 64^3 sphere pair, {method} method", flush=True)
     d1, d2 = sphere_frames()
     frames = [d1, d2, d1]
+    merge = not method.endswith("nearest")  # FUSE_BY_MERGE (config.cpp:7)
+    method = method.split("-")[0]
     if method == "reference":  # config.cpp parameters: FP32 deform parity 1e-4 voxel
         kp = KillingParams(voxel_size=C1["voxel"], max_surface=C1["trunc"], clip_distance=C1["trunc"], max_iters=16)
     else:
         kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint", max_iters=32)
-    pl = KillingFusion(ctx, C1["dims"], C1["min3d"], C1["voxel"], kp, scenes.K)
+    pl = KillingFusion(ctx, C1["dims"], C1["min3d"], C1["voxel"], kp, scenes.K, fuse_by_merge=merge)
     g = ogrid(C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"])
     stats = [pl.processNextFrame(d) for d in frames]
     gphi, gw = pl.canonical_download()
-    ophi, ow, oiters = oracle_sequence(frames, g, kp, method == "adjoint")
+    ophi, ow, oiters = oracle_sequence(frames, g, kp, method == "adjoint", merge)
     assert [s["deform"]["iters"] for s in stats[1:]] == oiters[1:]
     touched = ow > 0
     wdiff = gw != ow

@@ -184,3 +184,38 @@ def test_integrate_uneven_rows_and_held_values(ctx, dims, dtype):
     assert (w == ow).all()
     assert np.array_equal(phi, rt(ophi.astype(np.float32), scale), equal_nan=True)
     v.close()
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f16"])
+def test_fuse_nearest_mode_bit_exact(ctx, dtype):
+    """FUSE_BY_MERGE false (config.cpp:7; SDF.cpp:226-237, 285-295): per voxel the value nearer
+    the surface wins with its own weight; plain and displacement-aware fuse against the oracle."""
+    print(f"case: fuse / fuse_warped, nearest-surface mode, 64^3, {dtype}", flush=True)
+    d1, d2 = sphere_frames()
+    live, can = make_pair(ctx, C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"], d2, d1, dtype=dtype)
+    g = ogrid(C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"])
+    rt = {"f32": lambda a: np.asarray(a, np.float32), "bf16": lambda a: stored("bf16", a),
+          "f16": lambda a: stored("f16", a)}[dtype]
+    # a canonical SDF holding two frames (weights 2) and the live one
+    can.fuse(live)
+    lphi, lw = live.download()
+    cphi, cw = can.download()
+    u = scenes.seeded_field(C1["dims"], amplitude=1.2, seed=33)
+    field = DisplacementField(ctx, C1["dims"])
+    field.upload(u)
+    a = SDF(ctx, C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"], C1["trunc"], dtype=dtype)
+    a.copy_from(can)
+    a.fuse(live, merge=False)
+    ophi, ow = cphi.astype(np.float64), cw.astype(np.int64)
+    O.fuse(ophi, ow, lphi.astype(np.float64), lw.astype(np.int64), merge=False)
+    assert (ow != cw).any() and (ow == cw).any()  # both branches taken
+    gphi, gw = a.download()
+    assert (gw == ow).all() and np.array_equal(gphi, rt(ophi.astype(np.float32)))
+    a.copy_from(can)
+    a.fuse(live, field, merge=False)
+    ophi, ow = cphi.astype(np.float64), cw.astype(np.int64)
+    O.fuse_warped(ophi, ow, lphi.astype(np.float64), lw.astype(np.int64), u.astype(np.float64), g, merge=False)
+    gphi, gw = a.download()
+    assert (gw == ow).all() and np.array_equal(gphi, rt(ophi.astype(np.float32)))
+    for h in (a, field, live, can):
+        h.close()

@@ -633,3 +633,28 @@ def test_gauss_seidel_serial_equals_a_transcription_of_the_per_voxel_loop():
                         break
     assert visited > 20
     assert np.array_equal(uo, u)
+
+
+def test_fuse_nearest_surface_mode():
+    # FUSE_BY_MERGE = false (config.cpp:7, SDF.cpp:226-237 and :285-295): a voxel the other SDF
+    # never saw (w2 = 0) is left alone, an empty one takes the other's value and weight, and
+    # otherwise the value with the smaller magnitude wins with its own weight (ties take the
+    # other's: the test is |d1| < |d2|).
+    phi1 = np.array([0.5, 0.1, 0.02, -0.03, 0.04, -0.04])
+    w1 = np.array([3, 0, 2, 2, 5, 5], dtype=np.int64)
+    phi2 = np.array([0.9, -0.2, 0.05, 0.01, -0.04, 0.04])
+    w2 = np.array([0, 4, 1, 1, 7, 7], dtype=np.int64)
+    a, aw = phi1.copy(), w1.copy()
+    O.fuse(a, aw, phi2, w2, merge=False)
+    assert a.tolist() == [0.5, -0.2, 0.02, 0.01, -0.04, 0.04]
+    assert aw.tolist() == [3, 4, 2, 1, 7, 7]
+    # the warped variant with a zero field reads the other SDF at the nodes: the same result
+    n = 6
+    g = O.OGrid(dims=(n, 1, 1), voxel=1.0, oob=1.0)
+    b, bw = phi1.copy(), w1.copy()
+    O.fuse_warped(b, bw, phi2, w2, np.zeros((3, n)), g, merge=False)
+    assert b.tolist() == a.tolist() and bw.tolist() == aw.tolist()
+    # and the merge mode is the weighted average of the same inputs
+    c, cw = phi1.copy(), w1.copy()
+    O.fuse(c, cw, phi2, w2, merge=True)
+    assert c[2] == (2 * 0.02 + 1 * 0.05) / 3 and cw.tolist() == [3, 4, 3, 3, 12, 12]
