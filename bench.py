@@ -138,10 +138,11 @@ def cpu_baseline(live_phi, can_phi, kp, iters):
 def bench_batched(ctx, events, iters=50, reps=8, count=16):
     """Independent volumes of uneven size in one batch: cube edges drawn from 48..160, random
     ellipsoid pairs, BF16 storage, `iters` adjoint iterations per call.  The batched active-set
-    sweep (one launch per iteration for all volumes' lists) is `value`; the dense batched sweep
-    (one persistent launch per iteration over every voxel, the loop a replayed hipGraph, tile
-    shape from the autotuner) and the same volumes deformed one kf_deform call at a time are
-    reported beside it."""
+    sweep (one launch per iteration for all volumes' lists, launched directly) is `value`; the
+    same with its loop replayed as a captured hipGraph, the dense batched sweep (one persistent
+    launch per iteration over every voxel, the loop a replayed hipGraph, tile shape from the
+    autotuner) and the same volumes deformed one kf_deform call at a time are reported beside
+    it."""
     from killingfusion_amd import (DisplacementField, KillingParams, SDF, batch_autotune, batch_deform,
                                    compute_displacement_field, scenes, _lib)
     vox = scenes.BATCH_VOXEL
@@ -158,8 +159,8 @@ def bench_batched(ctx, events, iters=50, reps=8, count=16):
     stream = _lib.load().kf_context_stream(ctx.handle)
 
     def timed(kp):
-        # warm: the first call of a shape launches its loop directly, the second captures it as
-        # a hipGraph, which the timed calls replay
+        # warm: with loop capture on, the first call of a shape launches its loop directly, the
+        # second captures it as a hipGraph, which the timed calls replay
         for _ in range(2):
             batch_deform(lives, cans, fields, kp, stats=False)
         e0, e1 = events.event(), events.event()
@@ -177,6 +178,9 @@ def bench_batched(ctx, events, iters=50, reps=8, count=16):
     wall_d, gpu_d = timed(kp_d)
     kp = KillingParams.paper(vox, 3.0, method="adjoint", max_iters=iters, sweep="active")
     wall, gpu = timed(kp)
+    ctx.set_loop_capture(True)
+    wall_g, gpu_g = timed(kp)
+    ctx.set_loop_capture(False)
     # one volume at a time (kf_deform per volume, each with its stats readback)
     t1 = time.perf_counter()
     for _ in range(2):
@@ -193,6 +197,9 @@ def bench_batched(ctx, events, iters=50, reps=8, count=16):
             "sweep": "active", "listed_voxels": listed, "listed_fraction": listed / nvox,
             "ms_per_call": wall * 1e3, "gpu_ms_per_call": gpu, "ms_per_call_with_stats": wall_stats * 1e3,
             "looped_ms_per_batch": wall_loop * 1e3, "speedup_vs_looped": wall_loop / wall,
+            "captured_replay": {"ms_per_call": wall_g * 1e3, "gpu_ms_per_call": gpu_g,
+                                "note": "the same calls with the iteration loop replayed as a captured hipGraph "
+                                        "(kf_context_set_loop_capture)"},
             "dense": {"value": nvox * iters / wall_d, "ms_per_call": wall_d * 1e3, "gpu_ms_per_call": gpu_d,
                       "tile_shape": cfg},
             "speedup_vs_dense": wall_d / wall, "phi_storage": "bf16"}
@@ -355,9 +362,12 @@ def main(argv=None):
 
     # The same iterations over the voxels that can move only (the field is the same bit for bit).
     kp_act = KillingParams.paper(VOXEL, 3.0, method="adjoint", sweep="active")
-    wall_a, gpu_ms_a, st_a = time_deform(ctx, live_bf, can_bf, kp_act, a.steps, a.warmup, events)
-    # the same call after two of the same shape: its loop replays as the captured hipGraph
+    wall_a, gpu_ms_a, st_a = time_deform(ctx, live_bf, can_bf, kp_act, a.steps, a.warmup, events, warm_calls=2)
+    # the same call with loop capture on, after two of the same shape: its loop replays as the
+    # captured hipGraph
+    ctx.set_loop_capture(True)
     wall_ag, gpu_ms_ag, _ = time_deform(ctx, live_bf, can_bf, kp_act, a.steps, a.steps, events, warm_calls=2)
+    ctx.set_loop_capture(False)
     listed_fraction = st_a["listed"] / nvox
     bytes_per_vi = step_bytes(2, listed_fraction)
     achieved = bytes_per_vi * nvox / (kernel_ms * 1e-3) / 1e9
@@ -435,13 +445,14 @@ def main(argv=None):
                        "ms_per_step": wall_a * 1e3 / a.steps, "gpu_ms_per_step": gpu_ms_a / a.steps,
                        "listed_voxels": int(st_a["listed"]), "listed_fraction": st_a["listed"] / nvox,
                        "speedup_vs_dense": (nvox * a.steps / wall_a) / value,
-                       "replayed": {"value": nvox * a.steps / wall_ag, "ms_per_step": wall_ag * 1e3 / a.steps,
-                                    "gpu_ms_per_step": gpu_ms_ag / a.steps,
-                                    "note": "a call of the same list shape as the two before it: its iteration "
-                                            "loop runs as the captured hipGraph"},
+                       "captured_replay": {"value": nvox * a.steps / wall_ag, "ms_per_step": wall_ag * 1e3 / a.steps,
+                                           "gpu_ms_per_step": gpu_ms_ag / a.steps,
+                                           "note": "loop capture on, a call of the same launch shape as the two "
+                                                   "before it: its iteration loop runs as the captured hipGraph"},
                        "note": "same iterations, same field bit for bit; steps only the voxels that can move "
-                               "(listed once per call; the list build, count readback and final energy pass "
-                               "are inside the timed call)"},
+                               "(listed once per call, the list launches sized from the previous call's list "
+                               "length with no readback before them; the list build and final energy pass are "
+                               "inside the timed call)"},
         "batched": batched,
         "sequence": sequence,
         "active_voxels": int(st["active"]),

@@ -129,6 +129,13 @@ kf_status kf_context_destroy(kf_context *ctx);
 kf_status kf_context_synchronize(kf_context *ctx);
 /* The hipStream_t the context's work is ordered on (for callers that time or chain work). */
 void *kf_context_stream(kf_context *ctx);
+/* Iteration loops of the active-set sweep (kf_deform and kf_batch_deform without a stop rule)
+ * as captured hipGraphs (no reference counterpart): with enable != 0, a loop whose launch shape
+ * recurs on consecutive calls is captured the second time and replayed after that; with 0 (the
+ * default) every loop is launched directly.  Results are the same bit for bit either way; on
+ * MI355X with ROCm 7.2 direct launches keep the GPU as busy as a replay and came out 1-3%
+ * faster, so capture is for callers whose host cannot keep ahead of the GPU. */
+kf_status kf_context_set_loop_capture(kf_context *ctx, int32_t enable);
 /* Page-locked host memory (no reference counterpart): depth frames held in it reach the GPU
  * by DMA, without the staging copy a pageable buffer costs (kf_integrate_depth,
  * kf_pipeline_process_frame).  Free with kf_host_free. */

@@ -79,6 +79,7 @@ SIGNATURES = {
     "kf_context_destroy": (C.c_int, [_vp]),
     "kf_context_synchronize": (C.c_int, [_vp]),
     "kf_context_stream": (_vp, [_vp]),
+    "kf_context_set_loop_capture": (C.c_int, [_vp, C.c_int32]),
     "kf_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(_vp)]),
     "kf_host_free": (C.c_int, [_vp]),
     "kf_dims_from_bounds": (C.c_int, [_dp, _dp, C.c_double, _i32p]),

@@ -40,6 +40,9 @@ constexpr int SPAN = BLOCK * 16;       // voxels per block of the marking and li
 constexpr int WPB = SPAN / 64;         // 64-voxel mask words per such block (one per thread)
 constexpr int NNB = 12;                // neighbours of the Killing term: 6 faces, 6 edges
 constexpr int SCAN_BLOCK = 1024;
+// The single-volume list kernels run over a number of slots that is a multiple of this (so of
+// every block size they use); slots past the list's length do nothing.
+constexpr unsigned int SLOT_QUANTUM = 1024;
 
 // Neighbour k of voxel (x, y, z): faces +x -x +y -y +z -z, then the edge neighbours of Nbhd::dgl
 // in the order dgl[0][1], dgl[0][2], dgl[1][0], dgl[1][2], dgl[2][0], dgl[2][1].
@@ -236,11 +239,29 @@ __device__ inline void scan_spans(const unsigned int *__restrict__ span_count, u
         *total = run;
 }
 
+// The single-volume scan also decides, on the device, whether the list kernels run: they were
+// launched for `cap` slots, so a longer list (or, with need_finite, a non-finite live value
+// seen by the marking pass) sets list_bad and list_n = 0, and every later list kernel of the
+// call does nothing.  The host reads list_bad with the call's statistics.
 __global__ __launch_bounds__(SCAN_BLOCK) void k_active_scan(const unsigned int *__restrict__ span_count,
                                                             unsigned int nspan, unsigned int *__restrict__ span_off,
                                                             unsigned int *total)
 {
     scan_spans(span_count, nspan, span_off, total);
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_active_scan_sized(const unsigned int *__restrict__ span_count,
+                                                                  unsigned int nspan,
+                                                                  unsigned int *__restrict__ span_off, DeformState *st,
+                                                                  unsigned int cap, int need_finite)
+{
+    scan_spans(span_count, nspan, span_off, &st->listed);
+    if (threadIdx.x == SCAN_BLOCK - 1) {
+        const unsigned int total = st->listed;  // this thread's own store
+        const bool bad = total > cap || (need_finite && st->nonfinite != 0u);
+        st->list_n = bad ? 0u : total;
+        st->list_bad = bad ? 1u : 0u;
+    }
 }
 
 // The voxels a warp (SDF::update) can change: u_j != 0 (any component; -0 counts as 0, it
@@ -338,6 +359,18 @@ __global__ __launch_bounds__(WPB) void k_active_compact(const unsigned long long
     compact_span(mask, nwords, span_count, span_off, base, list, pos_of, blockIdx.x);
 }
 
+__global__ __launch_bounds__(WPB) void k_active_compact_sized(const unsigned long long *__restrict__ mask,
+                                                              unsigned int nwords,
+                                                              const unsigned int *__restrict__ span_count,
+                                                              const unsigned int *__restrict__ span_off,
+                                                              const DeformState *st, unsigned int *__restrict__ list,
+                                                              unsigned int *__restrict__ pos_of)
+{
+    if (st->list_bad != 0u)
+        return;  // the list does not fit the slots it was given
+    compact_span(mask, nwords, span_count, span_off, 0u, list, pos_of, blockIdx.x);
+}
+
 // Slot t (base .. base + count - 1): u of the listed voxel (compact set 0) and the list slots
 // of its 12 neighbours.
 // (Thread of local block lb.)
@@ -373,11 +406,11 @@ __device__ inline void list_neighbours(const GridDims &g, const unsigned long lo
 
 __global__ __launch_bounds__(BLOCK) void k_active_neighbours(GridDims g, const unsigned long long *__restrict__ mask,
                                                              const unsigned int *__restrict__ pos_of,
-                                                             const unsigned int *__restrict__ list, unsigned int base,
-                                                             unsigned int count, const float *__restrict__ u,
+                                                             const unsigned int *__restrict__ list,
+                                                             const DeformState *st, const float *__restrict__ u,
                                                              float4 *__restrict__ uc, int4 *__restrict__ nbr)
 {
-    list_neighbours(g, mask, pos_of, list, base, count, u, uc, nbr, blockIdx.x);
+    list_neighbours(g, mask, pos_of, list, 0u, st->list_n, u, uc, nbr, blockIdx.x);
 }
 
 // Listing of a batch of volumes, four launches for all of them.  Volume v's spans are blocks
@@ -643,6 +676,14 @@ __device__ inline double list_energy(const PhiRead<S> &L, const PhiRead<S> &Cn, 
     return esum;
 }
 
+// Work block w -> block of list slots.  The first nmap blocks (the list's expected length) are
+// dealt so that consecutive runs of the list (neighbouring voxels) go to one XCD and every
+// XCD gets an equal share; blocks past them (slots the list may not reach) keep their index.
+__device__ inline unsigned int list_tile(unsigned int w, unsigned int nmap)
+{
+    return w < nmap ? kf_xcd_tile(w, nmap) : w;
+}
+
 // One Jacobi iteration over the list.  Block 0 only folds the previous launch and arms the
 // next one (step_gate's first-block duty), beside the stencils rather than in front of one.
 // A work block loads its list entries, own u and neighbour slots together with the gate flag
@@ -653,26 +694,30 @@ __global__ __launch_bounds__(LB) void k_adj_list(PhiRead<S> L, PhiRead<S> Cn, co
                                                  BlockPartial *partials, int parity, const BlockPartial *prev,
                                                  unsigned int prev_nb, int has_prev, int publish,
                                                  const unsigned int *__restrict__ list,
-                                                 const int4 *__restrict__ nbr, unsigned int count)
+                                                 const int4 *__restrict__ nbr, unsigned int nmap)
 {
     if (blockIdx.x == 0) {
         list_fold<LB>(st, prev, prev_nb, has_prev != 0, parity);
         return;
     }
-    const unsigned int nw = gridDim.x - 1u, w = blockIdx.x - 1u;
-    // Consecutive runs of the list (neighbouring voxels) go to one XCD.
-    const unsigned int t = kf_xcd_tile(w, nw) * (unsigned int)LB + threadIdx.x;
-    const bool mine = t < count;
-    unsigned int i = 0u;
-    int ns[NNB];
-    float4 own = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (mine) {
-        i = list[t];
-        own = uin[t];
-        load_slots(nbr, t, ns);
+    const unsigned int w = blockIdx.x - 1u;
+    if (w >= nmap && w * (unsigned int)LB >= st->list_n) {
+        // slots past the expected length and past the list's end: only the (empty) partial
+        if (publish && st->run[parity] != 0u)
+            kf_block_publish<LB>(partials, w, 0.0, 0.0, 0ull);
+        return;
     }
+    const unsigned int t = list_tile(w, nmap) * (unsigned int)LB + threadIdx.x;
+    // Every slot of the grid lies within the list's capacity, so the entries are loaded with
+    // the list length (on the device) in one round trip and used only below it.
+    const unsigned int i = list[t];
+    const float4 own = uin[t];
+    int ns[NNB];
+    load_slots(nbr, t, ns);
+    const unsigned int count = st->list_n;
     if (st->run[parity] == 0u)
         return;
+    const bool mine = t < count;
     double mx = 0.0;
     unsigned long long act = 0;
     if (mine)
@@ -686,12 +731,12 @@ template <class S>
 __global__ __launch_bounds__(LB) void k_active_energy(PhiRead<S> L, PhiRead<S> Cn, const float4 *__restrict__ u0,
                                                       unsigned int set, const DeformState *st, AdjConsts c,
                                                       BlockPartial *partials, const unsigned int *__restrict__ list,
-                                                      const int4 *__restrict__ nbr, unsigned int count)
+                                                      const int4 *__restrict__ nbr, unsigned int nmap)
 {
     const float4 *uc = u0 + (size_t)(st->iters & 1u) * set;
-    const unsigned int t = kf_xcd_tile(blockIdx.x, gridDim.x) * (unsigned int)LB + threadIdx.x;
+    const unsigned int t = list_tile(blockIdx.x, nmap) * (unsigned int)LB + threadIdx.x;
     double esum = 0.0;
-    if (t < count) {
+    if (t < st->list_n) {
         int ns[NNB];
         load_slots(nbr, t, ns);
         esum = list_energy(L, Cn, uc, c, t, list[t], uc[t], ns);
@@ -842,11 +887,11 @@ __global__ void k_states_reset(DeformState *st, int n, double stop)
 
 __global__ __launch_bounds__(BLOCK) void k_active_scatter(const unsigned int *__restrict__ list,
                                                           const float4 *__restrict__ u0, unsigned int set,
-                                                          const DeformState *st, unsigned int count,
-                                                          float *__restrict__ u, unsigned int n32)
+                                                          const DeformState *st, float *__restrict__ u,
+                                                          unsigned int n32)
 {
     const unsigned int t = blockIdx.x * (unsigned int)BLOCK + threadIdx.x;
-    if (t >= count)
+    if (t >= st->list_n)
         return;
     const unsigned int i = list[t];
     const float4 v = u0[(size_t)(st->iters & 1u) * set + t];
@@ -902,7 +947,25 @@ void kf_active_release(kf_context *ctx)
     a = kf_active_ws{};
 }
 
-kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params *p, bool u_zero)
+unsigned int kf_list_slots(unsigned int count, bool headroom)
+{
+    uint64_t c = count;
+    if (headroom) {
+        // an eighth more than the count, rounded up to a sixteenth of its power of two, so that
+        // lists of similar length get the same launch shape (and the same captured loop)
+        c += c / 8 + 1;
+        int b = 63 - __builtin_clzll(c | 1ull);
+        const uint64_t q = b > 14 ? (1ull << (b - 4)) : 1024ull;
+        c = (c + q - 1) / q * q;
+    }
+    c = (c + SLOT_QUANTUM - 1) / SLOT_QUANTUM * SLOT_QUANTUM;
+    if (c < SLOT_QUANTUM)
+        c = SLOT_QUANTUM;
+    return c > 0xffffffffull - SLOT_QUANTUM ? 0u : (unsigned int)c;
+}
+
+kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params *p, bool u_zero, unsigned int slots,
+                          bool need_finite)
 {
     kf_context *ctx = live->ctx;
     kf_active_ws &a = ctx->active;
@@ -924,44 +987,47 @@ kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params
                                u, c.lo, c.hi, a.mask, a.span_count, &ctx->state->nonfinite);
     });
     KF_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_active_scan, dim3(1), dim3(SCAN_BLOCK), 0, ctx->stream, a.span_count, nspan,
-                       a.span_count + nspan, &ctx->state->listed);
+    hipLaunchKernelGGL(k_active_scan_sized, dim3(1), dim3(SCAN_BLOCK), 0, ctx->stream, a.span_count, nspan,
+                       a.span_count + nspan, ctx->state, slots, need_finite ? 1 : 0);
     KF_LAUNCH_CHECK();
     return KF_OK;
 }
 
-kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int count, bool neighbours)
+kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int slots, bool neighbours,
+                            unsigned int reserve)
 {
     kf_context *ctx = live->ctx;
     kf_active_ws &a = ctx->active;
+    KF_CHECK_ARG(slots > 0 && slots % SLOT_QUANTUM == 0, "list slots must be a positive multiple of the quantum");
     const unsigned int nspan = (unsigned int)((live->n + SPAN - 1) / SPAN);
     const unsigned int nwords = (unsigned int)((live->n + 63) / 64);
     kf_status s = grow(a.pos_of, a.pos_cap, (size_t)live->n, ctx->stream);
-    if (s == KF_OK && (size_t)count > a.cap) {
+    if (s == KF_OK && (size_t)slots > a.cap) {
         // the list, two compact sets of u and the neighbour slots share one capacity
+        const size_t want = reserve > slots ? reserve : slots;
         size_t c1 = a.cap, c2 = 2 * a.cap, c3 = 3 * a.cap;
-        s = grow(a.list, c1, count, ctx->stream);
+        s = grow(a.list, c1, want, ctx->stream);
         if (s == KF_OK)
-            s = grow(a.u, c2, 2 * (size_t)count, ctx->stream);
+            s = grow(a.u, c2, 2 * want, ctx->stream);
         if (s == KF_OK)
-            s = grow(a.nbr, c3, 3 * (size_t)count, ctx->stream);
-        a.cap = s == KF_OK ? count : 0;
+            s = grow(a.nbr, c3, 3 * want, ctx->stream);
+        a.cap = s == KF_OK ? want : 0;
     }
     if (s != KF_OK)
         return s;
-    hipLaunchKernelGGL(k_active_compact, dim3(nspan), dim3(WPB), 0, ctx->stream, a.mask, nwords, a.span_count,
-                       a.span_count + nspan, 0u, a.list, a.pos_of);
+    hipLaunchKernelGGL(k_active_compact_sized, dim3(nspan), dim3(WPB), 0, ctx->stream, a.mask, nwords, a.span_count,
+                       a.span_count + nspan, (const DeformState *)ctx->state, a.list, a.pos_of);
     KF_LAUNCH_CHECK();
-    if (count > 0 && neighbours) {
-        hipLaunchKernelGGL(k_active_neighbours, dim3((count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream,
-                           grid_of(live), a.mask, a.pos_of, a.list, 0u, count, u, a.u, a.nbr);
+    if (neighbours) {
+        hipLaunchKernelGGL(k_active_neighbours, dim3(slots / BLOCK), dim3(BLOCK), 0, ctx->stream, grid_of(live),
+                           a.mask, a.pos_of, a.list, (const DeformState *)ctx->state, u, a.u, a.nbr);
         KF_LAUNCH_CHECK();
     }
     return KF_OK;
 }
 
 kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const kf_params *p, int iter,
-                             unsigned int count, unsigned int *nb_out)
+                             unsigned int slots, unsigned int expect, unsigned int *nb_out)
 {
     // Per-iteration partials (max |du|, active count) feed only the stop rule and the last
     // iteration's statistics: without a stop rule only the last launch publishes them.
@@ -971,10 +1037,11 @@ kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const 
     kf_context *ctx = live->ctx;
     kf_active_ws &a = ctx->active;
     KF_CHECK_ARG(live->dtype == can->dtype, "live and canonical volumes must share a dtype");
-    KF_CHECK_ARG(count <= a.cap, "active list not prepared");
+    KF_CHECK_ARG(slots > 0 && slots % SLOT_QUANTUM == 0 && slots <= a.cap, "active list not prepared");
+    KF_CHECK_ARG(expect <= slots && expect % SLOT_QUANTUM == 0, "expected list slots beyond the launch's");
     const AdjConsts c = kf_adj_consts(p);
-    // nb work blocks (at least one, publishing a partial) and the folding block.
-    const unsigned int nb = count ? (count + LB - 1) / LB : 1u;
+    // nb work blocks over the list's slots (each publishes a partial) and the folding block.
+    const unsigned int nb = slots / LB;
     kf_status s = kf_ensure_partials(ctx, nb);
     if (s != KF_OK)
         return s;
@@ -988,24 +1055,27 @@ kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const 
     KF_DISPATCH_DTYPE(live->dtype, S, {
         hipLaunchKernelGGL((k_adj_list<S>), dim3(nb + 1), dim3(LB), 0, ctx->stream, make_read<S>(live),
                            make_read<S>(can), uin, uout, c, ctx->state, kf_partials(ctx, parity), parity, prev, prev_nb,
-                           iter > 0 ? 1 : 0, publish ? 1 : 0, a.list, a.nbr, count);
+                           iter > 0 ? 1 : 0, publish ? 1 : 0, a.list, a.nbr, expect / LB);
     });
     KF_LAUNCH_CHECK();
     return KF_OK;
 }
 
-kf_status kf_active_loop(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count)
+kf_status kf_active_loop(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int slots,
+                         unsigned int expect)
 {
     kf_context *ctx = live->ctx;
     kf_active_ws &a = ctx->active;
     const int K = p->max_iters;
-    const unsigned int nb = count ? (count + LB - 1) / LB : 1u;
+    KF_CHECK_ARG(slots > 0 && slots % SLOT_QUANTUM == 0 && slots <= a.cap, "active list not prepared");
+    const unsigned int nb = slots / LB;
     kf_status s = kf_ensure_partials(ctx, nb);  // before any capture: the launches allocate nothing
     if (s != KF_OK || K <= 0)
         return s;
     const AdjConsts c = kf_adj_consts(p);
     char buf[320];
-    std::snprintf(buf, sizeof(buf), "%d/%u/%d/%p/%p/%p/%p/%p/%zu/%p/%zu/%p/%d/%d/%d/%d/%d", live->dtype, count, K,
+    std::snprintf(buf, sizeof(buf), "%d/%u/%u/%d/%p/%p/%p/%p/%p/%zu/%p/%zu/%p/%d/%d/%d/%d/%d", live->dtype, slots,
+                  expect, K,
                   live->phi, can->phi, (void *)a.u, (void *)a.list, (void *)a.nbr, a.cap, (void *)ctx->partials,
                   ctx->partials_cap, (void *)ctx->state, live->dims[0], live->dims[1], live->dims[2],
                   (int)(live->scale() != can->scale()), (int)(p->stop_max_update > 0.0));
@@ -1016,7 +1086,7 @@ kf_status kf_active_loop(const kf_volume *live, const kf_volume *can, const kf_p
     return kf_run_captured(ctx, a.loop, key, [&]() -> kf_status {
         unsigned int nbo = 0;
         for (int k = 0; k < K; k++) {
-            const kf_status r = kf_launch_adj_list(live, can, p, k, count, &nbo);
+            const kf_status r = kf_launch_adj_list(live, can, p, k, slots, expect, &nbo);
             if (r != KF_OK)
                 return r;
         }
@@ -1024,20 +1094,21 @@ kf_status kf_active_loop(const kf_volume *live, const kf_volume *can, const kf_p
     });
 }
 
-kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count)
+kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int slots,
+                           unsigned int expect)
 {
     kf_context *ctx = live->ctx;
     kf_active_ws &a = ctx->active;
-    KF_CHECK_ARG(count <= a.cap, "active list not prepared");
+    KF_CHECK_ARG(slots > 0 && slots % SLOT_QUANTUM == 0 && slots <= a.cap, "active list not prepared");
     const AdjConsts c = kf_adj_consts(p);
-    const unsigned int nb = count ? (count + LB - 1) / LB : 1u;
+    const unsigned int nb = slots / LB;
     kf_status s = kf_ensure_partials(ctx, nb);
     if (s != KF_OK)
         return s;
     KF_DISPATCH_DTYPE(live->dtype, S, {
         hipLaunchKernelGGL((k_active_energy<S>), dim3(nb), dim3(LB), 0, ctx->stream, make_read<S>(live),
                            make_read<S>(can), a.u, (unsigned int)a.cap, ctx->state, c, kf_partials(ctx, 0), a.list,
-                           a.nbr, count);
+                           a.nbr, expect / LB);
     });
     KF_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_list_energy_fold, dim3(1), dim3(1024), 0, ctx->stream, ctx->state,
@@ -1046,14 +1117,13 @@ kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf
     return KF_OK;
 }
 
-kf_status kf_active_scatter(const kf_volume *live, float *u, unsigned int count)
+kf_status kf_active_scatter(const kf_volume *live, float *u, unsigned int slots)
 {
     kf_context *ctx = live->ctx;
     kf_active_ws &a = ctx->active;
-    if (count == 0)
-        return KF_OK;
-    hipLaunchKernelGGL(k_active_scatter, dim3((count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, a.list, a.u,
-                       (unsigned int)a.cap, ctx->state, count, u, (unsigned int)live->n);
+    KF_CHECK_ARG(slots > 0 && slots % SLOT_QUANTUM == 0 && slots <= a.cap, "active list not prepared");
+    hipLaunchKernelGGL(k_active_scatter, dim3(slots / BLOCK), dim3(BLOCK), 0, ctx->stream, a.list, a.u,
+                       (unsigned int)a.cap, ctx->state, u, (unsigned int)live->n);
     KF_LAUNCH_CHECK();
     return KF_OK;
 }

@@ -468,8 +468,9 @@ __global__ __launch_bounds__(SBLOCK) void k_ref_scan(PhiView<S> L, const float *
     }
 }
 
-// Pass 1 over the voxels that can move only (the active-set sweep; vlist holds them, count of
-// them): the same per-voxel work as k_ref_scan, one voxel per thread.  Every other voxel holds
+// Pass 1 over the voxels that can move only (the active-set sweep; vlist holds them, st->list_n
+// of them, within the launch's slots): the same per-voxel work as k_ref_scan, one voxel per
+// thread.  Every other voxel holds
 // u = +0 in both Jacobi buffers and keeps it (its band test reads phi_live[j] alone and fails,
 // or its warped value is NaN), so neither buffer needs a store there.
 template <class S, int MODE>
@@ -480,8 +481,9 @@ __global__ __launch_bounds__(SBLOCK) void k_ref_active_scan(PhiView<S> L, const 
                                                            unsigned long long segcap,
                                                            unsigned long long *__restrict__ list_idx,
                                                            float *__restrict__ list_psi,
-                                                           const unsigned int *__restrict__ vlist, unsigned int count)
+                                                           const unsigned int *__restrict__ vlist)
 {
+    const unsigned int count = st->list_n;
     if (MODE == KF_MODE_STEP && st->done)
         return;
     __shared__ unsigned int s_wtot[SBLOCK / 64];
@@ -674,7 +676,7 @@ void kf_ref_release(kf_context *ctx)
 }
 
 kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const float *u_in, float *u_out,
-                             const kf_params *p, int mode, int iter, const unsigned int *vlist, unsigned int vcount)
+                             const kf_params *p, int mode, int iter, const unsigned int *vlist, unsigned int vslots)
 {
     kf_context *ctx = live->ctx;
     KF_CHECK_ARG(live->dtype == can->dtype, "live and canonical volumes must share a dtype");
@@ -718,9 +720,10 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
         }
         w.cap = need;
     }
-    // The active-set pass 1 runs blocks of SBLOCK listed voxels; its segments then hold at most
-    // ceil(blocks / 8) blocks' voxels each, within the capacity sized for the whole grid.
-    const unsigned int nb_v = vlist ? (vcount + SBLOCK - 1) / SBLOCK : 0u;
+    // The active-set pass 1 runs blocks of SBLOCK list slots (vslots of them, the list's length
+    // read on the device); its segments then hold at most ceil(blocks / 8) blocks' voxels each,
+    // within the capacity sized for the whole grid.
+    const unsigned int nb_v = vlist ? (vslots + SBLOCK - 1) / SBLOCK : 0u;
     const unsigned long long segcap_v = (unsigned long long)((nb_v + NSEG - 1) / NSEG) * SBLOCK;
     KF_CHECK_ARG(!vlist || (mode != KF_MODE_GRAD && segcap_v * NSEG <= w.cap), "active list larger than the workspace");
     kf_status s = KF_OK;
@@ -760,7 +763,7 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
                 if (nb_v > 0)
                     hipLaunchKernelGGL((k_ref_active_scan<S, KF_MODE_STEP>), dim3(nb_v), dim3(SBLOCK), 0, ctx->stream,
                                        L, u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, sc, w.idx,
-                                       w.psi, vlist, vcount);
+                                       w.psi, vlist);
             } else if (vec) {
                 hipLaunchKernelGGL((k_ref_scan<S, KF_MODE_STEP, false>), dim3(nb_a), dim3(SBLOCK), 0, ctx->stream, L,
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, sc, w.idx, w.psi);

@@ -32,7 +32,9 @@ __global__ void k_state_reset(DeformState *st, double stop)
         st->last_active = 0ull;
         st->listed = 0u;
         st->nonfinite = 0u;
+        st->list_n = 0u;
         st->list_energy = 0.0;
+        st->list_bad = 0u;
     }
 }
 
@@ -308,6 +310,13 @@ kf_status kf_context_synchronize(kf_context *ctx)
 }
 
 void *kf_context_stream(kf_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+kf_status kf_context_set_loop_capture(kf_context *ctx, int32_t enable)
+{
+    KF_CHECK_ARG(ctx, "context is null");
+    ctx->capture_loops = enable != 0;
+    return KF_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Volumes
@@ -585,19 +594,22 @@ kf_status kf_integrate_depth_impl(kf_volume *v, const float *depth, int32_t widt
 }
 
 
-kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *u, const kf_params *p,
-                         kf_deform_stats *stats, bool u_zero)
+namespace {
+
+// One deform call.  predict: size the list launches from the list length of the previous call
+// on a volume of this size instead of reading the length back first.  *rerun: the list was
+// longer than predicted (or, for the reference method, met a non-finite live value), every
+// list kernel did nothing and the field is unchanged, so the call must run again unpredicted.
+kf_status deform_pass(const kf_volume *live, const kf_volume *can, kf_field *u, const kf_params *p,
+                      kf_deform_stats *stats, bool u_zero, bool predict, bool *rerun)
 {
-    kf_status s = kf_check_triplet(live, can, u);
-    if (s != KF_OK)
-        return s;
-    KF_CHECK_ARG(kf_valid_params(p), "invalid params");
+    *rerun = false;
     kf_context *ctx = live->ctx;
-    DeviceGuard dg(ctx->device);
-    s = kf_state_reset(ctx, p->stop_max_update);
+    kf_active_ws &a = ctx->active;
+    kf_status s = kf_state_reset(ctx, p->stop_max_update);
     if (s != KF_OK)
         return s;
-    ctx->active.warp_vol = nullptr;
+    a.warp_vol = nullptr;
     const int cur0 = u->cur;
     const bool check = p->stop_max_update > 0.0;
     const bool adj = p->method == KF_METHOD_ADJOINT;
@@ -606,21 +618,57 @@ kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *
     // field's buffer; the reference method steps only them in the field's own two buffers,
     // which both hold +0 everywhere else (the other buffer starts as a copy of the field).
     bool listed = false, ref_listed = false;
-    unsigned int count = 0;
+    unsigned int slots = 0, expect = 0;  // list launch slots; the length they are dealt for
     if (p->max_iters > 0 && p->sweep != KF_SWEEP_DENSE && kf_active_supported(live)) {
-        s = kf_active_count(live, u->buf[cur0], p, u_zero);
-        if (s == KF_OK)
-            s = read_state(ctx);
-        if (s != KF_OK)
-            return s;
-        count = ctx->state_host->listed;
+        // Both sweeps give the same field bit for bit, so which one runs is a question of speed
+        // only, and a predicted list length decides it as well as the read-back one.
+        bool predicted = predict && a.hint_n == live->n &&
+                         (p->sweep == KF_SWEEP_ACTIVE || (int64_t)a.hint_listed * 8 <= live->n);
+        if (predicted) {
+            // never more slots than a list of every voxel needs
+            const unsigned int most = kf_list_slots((unsigned int)live->n, false);
+            slots = kf_list_slots(a.hint_listed, true);
+            slots = slots < most ? slots : most;
+            expect = kf_list_slots(a.hint_listed, false);
+            expect = expect < slots ? expect : slots;
+            // A list that drifts a little from call to call (a frame sequence) keeps the launch
+            // shape of the last predicted call while it still fits with some room, so the
+            // captured loop of that shape keeps being replayed.
+            const unsigned int room = a.hint_listed + a.hint_listed / 32;
+            if (a.shape_n == live->n && room <= a.shape_slots && a.shape_expect <= a.shape_slots &&
+                expect + 2048 >= a.shape_expect && a.shape_expect + 2048 >= expect) {
+                slots = a.shape_slots;
+                expect = a.shape_expect;
+            }
+            a.shape_n = live->n;
+            a.shape_slots = slots;
+            a.shape_expect = expect;
+        }
+        predicted = predicted && slots > 0;
         // The reference method's band test passes a NaN warped value, and with u_j = +0 next to
         // a non-finite live value the warped value is NaN (inf * 0): such a voxel moves though
         // phi_live[j] is outside the band.  The list covers every voxel that can move only when
         // every live value is finite, so the reference method steps densely otherwise.
-        const bool use = (p->sweep == KF_SWEEP_ACTIVE || (int64_t)count * 8 <= live->n) &&
-                         (adj || !ctx->state_host->nonfinite);
-        if (use && (s = kf_active_prepare(live, u->buf[cur0], count, adj)) != KF_OK)
+        s = kf_active_count(live, u->buf[cur0], p, u_zero, predicted ? slots : 0xffffffffu, !adj);
+        if (s != KF_OK)
+            return s;
+        bool use = predicted;
+        if (!predicted) {
+            s = read_state(ctx);
+            if (s != KF_OK)
+                return s;
+            const unsigned int count = ctx->state_host->listed;
+            const bool finite_ok = adj || !ctx->state_host->nonfinite;
+            slots = expect = kf_list_slots(count, false);
+            use = (p->sweep == KF_SWEEP_ACTIVE || (int64_t)count * 8 <= live->n) && finite_ok && slots > 0;
+            a.hint_n = finite_ok ? live->n : 0;
+            a.hint_listed = count;
+        }
+        // room for the launch shape the next call predicts from this list
+        const unsigned int most = kf_list_slots((unsigned int)live->n, false);
+        const unsigned int ahead = kf_list_slots(ctx->state_host->listed, true);
+        const unsigned int reserve = predicted ? 0u : (ahead < most ? ahead : most);
+        if (use && (s = kf_active_prepare(live, u->buf[cur0], slots, adj, reserve)) != KF_OK)
             return s;
         listed = use && adj;
         ref_listed = use && !adj;
@@ -637,8 +685,8 @@ kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *
     int launched = 0;
     unsigned int nb = 0;
     if (listed && !check && p->max_iters > 0) {
-        // the whole loop and its last fold, as one captured graph when this list shape recurs
-        s = kf_active_loop(live, can, p, count);
+        // the whole loop and its last fold, as one captured graph when this launch shape recurs
+        s = kf_active_loop(live, can, p, slots, expect);
         if (s != KF_OK)
             return s;
         launched = -1;  // folded already
@@ -646,10 +694,10 @@ kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *
     for (int k = 0; launched >= 0 && k < p->max_iters; k++) {
         float *in = u->buf[cur0 ^ (k & 1)];
         float *out = u->buf[cur0 ^ ((k + 1) & 1)];
-        s = listed ? kf_launch_adj_list(live, can, p, k, count, &nb)
+        s = listed ? kf_launch_adj_list(live, can, p, k, slots, expect, &nb)
             : adj  ? kf_launch_adj_step(live, can, in, out, p, KF_MODE_STEP, nullptr, nullptr, k, &nb)
-                   : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP, k,
-                                        ref_listed ? ctx->active.list : nullptr, count);
+                   : kf_launch_ref_step(live, can, in, out, p, KF_MODE_STEP, k, ref_listed ? a.list : nullptr,
+                                        slots);
         if (s != KF_OK)
             return s;
         launched = k + 1;
@@ -671,9 +719,9 @@ kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *
     // set (iters & 1) is chosen on the device, so the field, the energy and the state need one
     // readback; the energy of a diverged field is computed and ignored.
     if (listed) {
-        s = kf_active_scatter(live, u->buf[cur0], count);
+        s = kf_active_scatter(live, u->buf[cur0], slots);
         if (s == KF_OK)
-            s = kf_active_energy(live, can, p, count);
+            s = kf_active_energy(live, can, p, slots, expect);
         if (s != KF_OK)
             return s;
     }
@@ -681,6 +729,15 @@ kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *
     if (s != KF_OK)
         return s;
     const DeformState h = *ctx->state_host;
+    if ((listed || ref_listed) && h.list_bad) {
+        a.hint_n = 0;
+        *rerun = true;
+        return KF_OK;
+    }
+    if (listed || ref_listed) {
+        a.hint_n = live->n;
+        a.hint_listed = h.listed;
+    }
     if (!listed)
         u->cur = cur0 ^ (int)(h.iters & 1u);
     double energy = 0.0;
@@ -695,9 +752,9 @@ kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *
     }
     if ((listed || ref_listed) && !h.nonfinite) {
         // The list covers every voxel a warp of live by u can change (see k_deform_active.hip).
-        ctx->active.warp_vol = live;
-        ctx->active.warp_field = u;
-        ctx->active.warp_count = count;
+        a.warp_vol = live;
+        a.warp_field = u;
+        a.warp_count = h.listed;
     }
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
@@ -707,13 +764,30 @@ kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *
         stats->energy = energy;
         stats->active_voxels = h.last_active;
         stats->sweep = listed || ref_listed ? KF_SWEEP_ACTIVE : KF_SWEEP_DENSE;
-        stats->listed_voxels = count;
+        stats->listed_voxels = h.listed;  // 0 when nothing was counted
     }
     if (h.diverged != LLONG_MAX) {
         kf_set_error("deformation field diverged at voxel " + std::to_string((long long)h.diverged));
         return KF_E_DIVERGED;
     }
     return KF_OK;
+}
+
+}  // namespace
+
+kf_status kf_deform_impl(const kf_volume *live, const kf_volume *can, kf_field *u, const kf_params *p,
+                         kf_deform_stats *stats, bool u_zero)
+{
+    kf_status s = kf_check_triplet(live, can, u);
+    if (s != KF_OK)
+        return s;
+    KF_CHECK_ARG(kf_valid_params(p), "invalid params");
+    DeviceGuard dg(live->ctx->device);
+    bool rerun = false;
+    s = deform_pass(live, can, u, p, stats, u_zero, true, &rerun);
+    if (s == KF_OK && rerun)
+        s = deform_pass(live, can, u, p, stats, u_zero, false, &rerun);
+    return s;
 }
 
 kf_status kf_check_triplet(const kf_volume *live, const kf_volume *can, const kf_field *u)

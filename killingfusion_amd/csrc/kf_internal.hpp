@@ -57,8 +57,11 @@ struct DeformState {
     double energy;                  // energy of the last completed launch
     unsigned long long last_active; // active voxels of the last completed launch
     unsigned int nonfinite;         // the active-set listing met a non-finite live SDF value
-    unsigned int pad2;
+    unsigned int list_n;            // voxels the list kernels step: `listed`, or 0 when list_bad
     double list_energy;             // energy of the final field, active-set sweep
+    unsigned int list_bad;          // the list outgrew the capacity its launches were sized for, or
+                                    // (reference method) met a non-finite live value: nothing ran
+    unsigned int pad2;
 };
 
 // Per-block partial results of one step launch, folded by k_finalize.
@@ -133,6 +136,13 @@ struct kf_active_ws {
     const kf_volume *warp_vol = nullptr;
     const kf_field *warp_field = nullptr;
     unsigned int warp_count = 0;
+    // The list length of the last kf_deform call on a volume of hint_n voxels: the next call
+    // on such a volume sizes its list launches from it without reading the length back first.
+    int64_t hint_n = 0;
+    unsigned int hint_listed = 0;
+    // launch shape (slots, expected length) of the last predicted call, on a volume of shape_n
+    int64_t shape_n = 0;
+    unsigned int shape_slots = 0, shape_expect = 0;
     // the iteration loops of kf_deform and kf_batch_deform over the lists (no stop rule)
     kf_loop_graph loop, batch_loop;
 };
@@ -170,6 +180,7 @@ struct kf_context {
     BlockPartial *partials = nullptr;  // device, two sets (launch parity) of one slot per block
     size_t partials_cap = 0;           // slots per set
     int num_cus = 256;
+    bool capture_loops = false;              // kf_context_set_loop_capture
     struct kf_batch_cache *batch = nullptr;  // batched-deform workspace, tuned shapes, captured graph
     kf_active_ws active;                     // active-set sweep workspace (k_deform_active.hip)
     kf_ref_ws ref;                           // reference-formula step workspace (k_deform_ref.hip)
@@ -460,12 +471,13 @@ kf_status kf_launch_field_add(kf_field *dst, const kf_field *src);
 
 struct DeformArgs;
 // Step launches take their index `iter` within a deform call: launch k writes partial set k & 1.
-// The reference step folds its own partials right after the launch.  vlist (vcount entries):
-// the active-set sweep's list of the voxels that can move (step mode only), whose pass 1 then
-// visits only them; every other voxel must hold +0 in both u_in and u_out.
+// The reference step folds its own partials right after the launch.  vlist: the active-set
+// sweep's list of the voxels that can move (step mode only; state->list_n entries, launched
+// over vslots slots), whose pass 1 then visits only them; every other voxel must hold +0 in
+// both u_in and u_out.
 kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const float *u_in,
                              float *u_out, const kf_params *p, int mode, int iter = 0,
-                             const unsigned int *vlist = nullptr, unsigned int vcount = 0);
+                             const unsigned int *vlist = nullptr, unsigned int vslots = 0);
 // mask_in: stencil mask of u_in (kf_launch_adj_mask), needed by energy and gradient passes of
 // the row kernel (see k_deform_adj.hip); without it those passes run one voxel per thread.
 // An adjoint step launch k > 0 folds launch k - 1 in its first block, so a deform loop ends
@@ -474,29 +486,45 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
                              float *u_out, const kf_params *p, int mode, float *e_voxel,
                              const unsigned int *mask_in = nullptr, int iter = 0,
                              unsigned int *nb_out = nullptr);
-// Active-set sweep (k_deform_active.hip), one deform call: kf_active_count lists nothing yet but
-// leaves the number of voxels that can move in state->listed (reset the state first);
-// kf_active_prepare lists them and copies u into compact set 0; kf_launch_adj_list is step
-// launch `iter` (compact set iter & 1 -> the other), with the same launch-to-launch folding
-// as kf_launch_adj_step; kf_active_scatter writes compact set iters & 1 back into u.
+// Active-set sweep (k_deform_active.hip), one deform call.  The list's kernels are launched
+// over `slots` list slots (a multiple of 1024, kf_list_slots) chosen before its length is known
+// on the host; they read the length from state->list_n.  kf_active_count marks the voxels that
+// can move and leaves their number in state->listed (reset the state first); when it exceeds
+// `slots` (or, with need_finite, a live value is not finite) it sets state->list_bad and the
+// list kernels do nothing.  kf_active_prepare lists them and copies u into compact set 0;
+// kf_launch_adj_list is step launch `iter` (compact set iter & 1 -> the other), with the same
+// launch-to-launch folding as kf_launch_adj_step; kf_active_scatter writes compact set
+// iters & 1 back into u.
 bool kf_active_supported(const kf_volume *live);
 void kf_active_release(kf_context *ctx);
 void kf_ref_release(kf_context *ctx);
-kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params *p, bool u_zero);
+// Slots for a list of `count` voxels: count rounded up to the quantum, or, with headroom (a
+// length predicted from an earlier call), an eighth more rounded to a sixteenth of its power
+// of two.  0 when that does not fit 32 bits.
+unsigned int kf_list_slots(unsigned int count, bool headroom);
+kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params *p, bool u_zero, unsigned int slots,
+                          bool need_finite);
 // neighbours = false lists the voxels only (ctx->active.list), without the adjoint step's
-// compact u and neighbour slots.
-kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int count, bool neighbours = true);
+// compact u and neighbour slots.  When the workspace has to grow, it grows to `reserve` slots
+// if that is more (so later calls with somewhat longer lists need not grow it again).
+kf_status kf_active_prepare(const kf_volume *live, const float *u, unsigned int slots, bool neighbours = true,
+                            unsigned int reserve = 0);
+// expect (a multiple of 1024, at most slots): the list length the launch's blocks are dealt
+// over the XCDs for (kf_list_slots of the predicted or read-back length).
 kf_status kf_launch_adj_list(const kf_volume *live, const kf_volume *can, const kf_params *p, int iter,
-                             unsigned int count, unsigned int *nb_out);
+                             unsigned int slots, unsigned int expect, unsigned int *nb_out);
 // All p->max_iters step launches of a call without a stop rule and the fold of the last one,
-// replayed as a captured hipGraph when the same list shape comes back (kf_loop_graph).
-kf_status kf_active_loop(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count);
-// Runs launches() directly, or as the hipGraph g holds for key (capturing it on the key's
-// second run); launches() must only launch kernels on the context's stream (no allocation, no
-// memset, no host wait).
+// replayed as a captured hipGraph when the same launch shape comes back (kf_loop_graph).
+kf_status kf_active_loop(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int slots,
+                         unsigned int expect);
+// Runs launches() directly, or, when the context captures loops, as the hipGraph g holds for
+// key (capturing it on the key's second run); launches() must only launch kernels on the
+// context's stream (no allocation, no memset, no host wait).
 template <class F>
 kf_status kf_run_captured(kf_context *ctx, kf_loop_graph &g, const std::string &key, F launches)
 {
+    if (!ctx->capture_loops)
+        return launches();
     if (g.exec && key == g.key) {
         KF_HIP(hipGraphLaunch(g.exec, ctx->stream));
         return KF_OK;
@@ -531,7 +559,7 @@ kf_status kf_run_captured(kf_context *ctx, kf_loop_graph &g, const std::string &
     return KF_OK;
 }
 // Writes the final compact set (state->iters & 1, read on the device) back into u.
-kf_status kf_active_scatter(const kf_volume *live, float *u, unsigned int count);
+kf_status kf_active_scatter(const kf_volume *live, float *u, unsigned int slots);
 // kf_batch_deform over the voxels that can move in each volume (one launch per iteration for
 // the whole batch).  Sets *used = false and does nothing more when, without `force`, the
 // lists hold more than 1/8 of the voxels.
@@ -546,7 +574,8 @@ kf_status kf_list_moved(const kf_volume *v, const float *u, unsigned int *count,
 // nothing, when the list holds more than 1/8 of the voxels or some phi is not finite.
 kf_status kf_warp_sparse(kf_volume *v, const kf_field *u, bool *done);
 // Energy of the final compact set into state->list_energy (no other state field changes).
-kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int count);
+kf_status kf_active_energy(const kf_volume *live, const kf_volume *can, const kf_params *p, unsigned int slots,
+                           unsigned int expect);
 // SDF::update of v by u over the list a kf_deform active-set sweep of (v, u) just left behind
 // (k_volume.hip); *done = false, changing nothing, when there is none.
 kf_status kf_warp_after_deform(kf_volume *v, const kf_field *u, bool *done);

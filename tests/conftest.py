@@ -19,3 +19,12 @@ def ctx():
     c = Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture
+def loop_capture(ctx):
+    """The shared context with its active-set iteration loops captured as hipGraphs and
+    replayed (kf_context_set_loop_capture) for one test; direct launches again afterwards."""
+    ctx.set_loop_capture(True)
+    yield ctx
+    ctx.set_loop_capture(False)

@@ -198,11 +198,13 @@ def test_active_equals_dense_256(ctx):
     can.close()
 
 
-def test_active_loop_replayed_as_a_graph(ctx, sphere):
-    """Without a stop rule the iteration loop of an active-set call runs directly the first time
-    its list shape is seen, is captured as a hipGraph the second time and replayed after that:
+def test_active_loop_replayed_as_a_graph(loop_capture, sphere):
+    """With loop capture on and without a stop rule, the iteration loop of an active-set call
+    runs directly the first time its launch shape is seen, is captured as a hipGraph the second
+    time and replayed after that:
     every call gives the dense sweep's field and statistics bit for bit, with other work on the
     stream in between (a call of another shape, which the cache must not mistake for this one)."""
+    ctx = loop_capture
     live, can = sphere["bf16"]
     kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint", max_iters=30)
     ref = run(ctx, live, can, kp, "dense")
@@ -212,3 +214,29 @@ def test_active_loop_replayed_as_a_graph(ctx, sphere):
         assert_same(ref, run(ctx, live, can, kp, "active"), f"active call {call}")
         if call == 2:  # a different shape in between, then this one again
             run(ctx, l32, c32, dataclasses.replace(kp, max_iters=7), "active")
+
+
+def test_list_longer_than_predicted(ctx, sphere):
+    """A call sizes its list launches from the list length of the previous call on a volume of
+    the same size, without reading the length back first.  A start field that is nonzero over
+    a sub-box lists far more voxels than the call before it: the list kernels then do nothing
+    and the call runs again with the length read back.  Every call still gives the dense
+    sweep's field and statistics bit for bit, and so does a short list right after the long
+    one (launched over more slots than it needs)."""
+    live, can = sphere["bf16"]
+    dims = C1["dims"]
+    kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint", max_iters=12)
+    u0 = scenes.seeded_field(dims, amplitude=0.3, seed=4).reshape(3, dims[2], dims[1], dims[0])
+    box = np.zeros_like(u0)
+    box[:, 10:50, 5:60, 5:60] = u0[:, 10:50, 5:60, 5:60]
+    box = box.reshape(3, -1)
+    print("case: 64^3 sphere, bf16, short list, then a long one, then short again", flush=True)
+    short_d = run(ctx, live, can, kp, "dense")
+    short_a = run(ctx, live, can, kp, "active")  # sets the prediction
+    assert_same(short_d, short_a, "short list")
+    long_a = run(ctx, live, can, kp, "active", box)
+    assert long_a[1]["listed"] > 2 * short_a[1]["listed"] + 4096
+    assert long_a[1]["listed"] == listed_on_cpu(live, box, kp)
+    assert_same(run(ctx, live, can, kp, "dense", box), long_a, "long list after a short one")
+    again = run(ctx, live, can, kp, "active")
+    assert_same(short_d, again, "short list after a long one")

@@ -64,9 +64,18 @@ def lists(vols):
     return [v[0] for v in vols], [v[1] for v in vols]
 
 
+@pytest.mark.parametrize("capture", [False, True])
 @pytest.mark.parametrize("sweep", ["dense", "active"])
-def test_batch_equals_per_volume_across_replays(ctx, batch, sweep):
-    print(f"case: batch of 4, 50 iterations, {sweep} sweep", flush=True)
+def test_batch_equals_per_volume_across_replays(ctx, batch, sweep, capture):
+    print(f"case: batch of 4, 50 iterations, {sweep} sweep, loop capture {capture}", flush=True)
+    ctx.set_loop_capture(capture)
+    try:
+        _batch_across_replays(ctx, batch, sweep)
+    finally:
+        ctx.set_loop_capture(False)
+
+
+def _batch_across_replays(ctx, batch, sweep):
     kp = params(sweep, max_iters=50)
     ref, ref_st = per_volume(ctx, batch, kp)
     assert all(s["active"] > 0 for s in ref_st)

@@ -118,6 +118,9 @@ def test_reference_sweep_with_a_non_finite_live_value(ctx, sphere):
     phi_live[j] is outside the band, so the list would miss it.  The active sweep then steps
     densely, and both requests give the dense result (here the divergence at that voxel)."""
     live, can = sphere["f32"]
+    kp = KillingParams(voxel_size=C1["voxel"], max_surface=C1["trunc"], clip_distance=C1["trunc"], max_iters=5)
+    # a finite call first, so that the calls below size their list launches from its list
+    assert run(ctx, live, can, kp, "active")[1]["sweep"] == "active"
     phi, w = live.download()
     k = 64 * 64 * 40 + 64 * 20 + 33
     p3 = phi.copy()
@@ -129,9 +132,7 @@ def test_reference_sweep_with_a_non_finite_live_value(ctx, sphere):
             print(f"case: reference method, +inf live value at voxel {k}, {sweep}", flush=True)
             f = DisplacementField(ctx, C1["dims"])
             with pytest.raises(KFError) as e:
-                compute_displacement_field(live, can, f, KillingParams(voxel_size=C1["voxel"], max_surface=C1["trunc"],
-                                                                       clip_distance=C1["trunc"], max_iters=5,
-                                                                       sweep=sweep))
+                compute_displacement_field(live, can, f, dataclasses.replace(kp, sweep=sweep))
             assert e.value.status == KF_E_DIVERGED
             msgs.append(str(e.value))
             f.close()
@@ -149,3 +150,23 @@ def test_reference_active_equals_dense_256(ctx):
     same(run(ctx, live, can, kp, "dense"), run(ctx, live, can, kp, "active"), "torus 256^3")
     live.close()
     can.close()
+
+
+def test_reference_list_longer_than_predicted(ctx, sphere):
+    """The reference method's active sweep sizes its list launches from the previous call's
+    list as well: a start field nonzero over a sub-box lists far more voxels than the call
+    before, the list kernels do nothing and the call runs again with the length read back;
+    the field is the dense sweep's bit for bit."""
+    live, can = sphere["bf16"]
+    dims = C1["dims"]
+    kp = KillingParams.paper(C1["voxel"], 3.0, max_iters=10)
+    u0 = scenes.seeded_field(dims, amplitude=0.3, seed=5).reshape(3, dims[2], dims[1], dims[0])
+    box = np.zeros_like(u0)
+    box[:, 8:52, 4:60, 6:58] = u0[:, 8:52, 4:60, 6:58]
+    box = box.reshape(3, -1)
+    print("case: reference method, 64^3 sphere, bf16, short list then a long one", flush=True)
+    short = run(ctx, live, can, kp, "active")
+    long_a = run(ctx, live, can, kp, "active", box)
+    assert long_a[1]["listed"] > 2 * short[1]["listed"] + 4096
+    same(run(ctx, live, can, kp, "dense", box), long_a, "reference, long list after a short one")
+    same(run(ctx, live, can, kp, "dense"), run(ctx, live, can, kp, "active"), "reference, short list again")

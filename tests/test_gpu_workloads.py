@@ -67,8 +67,9 @@ def batch16(ctx):
 
 
 @pytest.mark.parametrize("sweep", ["active", "dense"])
-def test_batched_16_volumes_against_oracle(ctx, batch16, sweep):
-    print(f"case: batched workload, 16 volumes, 50 iterations, {sweep} sweep, 3 calls", flush=True)
+def test_batched_16_volumes_against_oracle(loop_capture, batch16, sweep):
+    ctx = loop_capture
+    print(f"case: batched workload, 16 volumes, 50 iterations, {sweep} sweep, 3 calls (loops captured)", flush=True)
     vols, kp = batch16["vols"], dataclasses.replace(batch16["kp"], sweep=sweep)
     lives, cans = [v[0] for v in vols], [v[1] for v in vols]
     fields = [DisplacementField(ctx, v.dims) for v in lives]
