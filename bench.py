@@ -273,6 +273,31 @@ def step_only(grid, steps, warmup):
     return 0
 
 
+def frame_kernels(reps=5):
+    """Integration and marching cubes alone on the 192^3 BF16 volume of the hinge sequence (its
+    first depth frame), `reps` of each, for profiling them (kernel trace, FETCH/WRITE counters).
+    Each repetition resets the volume, integrates the frame (SDF::integrateDepthFrame: reads and
+    writes phi and w) and extracts the mesh of the volume just written, so every extraction
+    counts its cells afresh."""
+    from killingfusion_amd import Context, SDF, host_array, scenes
+    ctx = Context(0)
+    n, vox = scenes.SEQ_GRID, scenes.SEQ_VOXEL
+    d = next(iter(scenes.hinge_sequence(1)))
+    depth = host_array(d.shape, np.float32)
+    depth[...] = d
+    v = SDF(ctx, (n, n, n), scenes.SEQ_MIN, voxel_size=vox, clip_distance=3 * vox, max_surface=3 * vox,
+            dtype="bf16")
+    tris = []
+    for _ in range(reps):
+        v.reset()
+        v.integrateDepthFrame(depth, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+        tris.append(int(v.getMesh().tris.shape[0]))
+    ctx.synchronize()
+    print(json.dumps({"frame_kernels": f"reset + integrate + marching cubes, {n}^3 bf16 hinge frame 0",
+                      "reps": reps, "voxels": n ** 3, "triangles": tris}), flush=True)
+    return 0
+
+
 def calibrate_copy(launches=5):
     """A kernel of known HBM traffic for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters:
     kf_field_add on two 256^3 fields reads 2 x 3N floats and writes 3N floats per launch
@@ -308,6 +333,8 @@ def main(argv=None):
                     help="only run the 100-frame sequence and print its measurement (for profiling)")
     ap.add_argument("--step-only", type=int, default=0, metavar="GRID",
                     help="only time the dense BF16 adjoint step on a GRID^3 torus pair (for profiling)")
+    ap.add_argument("--frame-kernels", action="store_true",
+                    help="only run integration and marching cubes on the 192^3 BF16 hinge volume (for profiling)")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched uneven-volume measurement and print it (for profiling)")
     a = ap.parse_args(argv)
@@ -319,6 +346,8 @@ def main(argv=None):
         return calibrate_copy()
     if a.step_only:
         return step_only(a.step_only, a.steps, a.warmup)
+    if a.frame_kernels:
+        return frame_kernels()
     if a.sequence_only or a.batched_only:
         from killingfusion_amd import Context
         ctx = Context(0)

@@ -20,6 +20,7 @@
 #include "kf_internal.hpp"
 #include "kf_mc_cases.h"
 
+#include <cmath>
 #include <vector>
 
 namespace {
@@ -45,8 +46,9 @@ __constant__ signed char c_corner[8][3] = {{1, 0, 0}, {0, 0, 0}, {0, 1, 0}, {1, 
 __constant__ unsigned char c_edge_corners[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
                                                      {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
 
-// Inclusive scan of v over the block in thread order, and the block total: a wavefront scan,
-// then the wavefront totals through LDS (`s`, BLOCK / 64 entries; two barriers).
+// Inclusive scan of v over the block (NT threads) in thread order, and the block total: a
+// wavefront scan, then the wavefront totals through LDS (`s`, NT / 64 entries; two barriers).
+template <int NT>
 __device__ inline unsigned int block_scan_incl(unsigned int v, unsigned int *s, unsigned int &total)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -61,7 +63,7 @@ __device__ inline unsigned int block_scan_incl(unsigned int v, unsigned int *s, 
     __syncthreads();
     unsigned int wbase = 0, tot = 0;
 #pragma unroll
-    for (int q = 0; q < BLOCK / 64; q++) {
+    for (int q = 0; q < NT / 64; q++) {
         wbase += q < w ? s[q] : 0u;
         tot += s[q];
     }
@@ -126,53 +128,80 @@ struct Cells {
         const double v = (double)S::load(phi, inside ? g.idx(x, y, z) : 0, scale);
         return inside ? v : oob;
     }
-    // Cube indices (-1: skipped, no triangles or past the grid) and coordinates of the CPT
-    // cells i0 .. i0 + CPT - 1.  When they lie in one row, the 4 x (CPT + 1) node values come
-    // from the four node rows (y, z), (y + 1, z), (y, z + 1), (y + 1, z + 1) as vector loads of
-    // four and one scalar each; otherwise (a group that wraps into the next row or runs past
-    // the grid) cell by cell.
-    // Either way every load is issued before any is used.
-    __device__ inline void quad(int64_t i0, int ci[CPT], int cx[CPT], int cy[CPT], int cz[CPT]) const
+    // The classification compares, done in FP32 with the same outcome as the reference's FP64
+    // ones: a stored value decodes to a float f, and for a double X, (double) f < X exactly
+    // when f < the least float >= X (iso_up, clip_up), and (double) f > X exactly when f > the
+    // greatest float <= X (max_dn); a node outside the grid reads oob, and stands in the FP32
+    // test as oob_f (-inf when oob < iso, else +inf), which compares with iso_up as oob does
+    // with iso.  NaN fails every comparison either way.
+    float iso_up, max_dn, clip_up, oob_f;
+    // Cube index from the corner-below bits m and the cell's own node value d (corner 1).
+    __device__ inline int case_of(float d, int m) const
+    {
+        return (d > max_dn || d < clip_up || m == 0 || m == 255) ? -1 : m;
+    }
+    // Cube indices (-1: skipped, no triangles or past the grid) and coordinates of the N
+    // cells i0 .. i0 + N - 1.  When they lie in one row, the 4 x (N + 1) node values come from
+    // the four node rows (y, z), (y + 1, z), (y, z + 1), (y + 1, z + 1) (vector loads of four,
+    // and one scalar each); otherwise (a group that wraps into the next row or runs past the
+    // grid) cell by cell.  Either way every load is issued before any is used.  With VALS, also
+    // each cell's eight corner values (as decoded; meaningless for a corner past the grid) in
+    // cv and, in om, bit k set for a corner k past the grid.
+    template <int N, bool VALS = false>
+    __device__ inline void quad(int64_t i0, int ci[N], int cx[N], int cy[N], int cz[N], float (*cv)[8] = nullptr,
+                                unsigned int *om = nullptr) const
     {
         int x0, y0, z0;
         kf_xyz(g, i0, x0, y0, z0);
-        if (i0 + CPT <= g.n && x0 + CPT <= g.nx) {
-            float v[4][CPT + 1];
+        if (i0 + N <= g.n && x0 + N <= g.nx) {
+            float v[4][N + 1];
             bool in[4];
-            const int64_t b0 = g.idx(x0, y0, z0);
-            const bool x4 = x0 + CPT < g.nx;  // the x + 1 corner of the last cell is inside
+            const bool xl = x0 + N < g.nx;  // the x + 1 corner of the last cell is inside
 #pragma unroll
             for (int r = 0; r < 4; r++) {
+                // a row past the grid is read from the last row instead (and its values not
+                // used): every row's loads are then issued unconditionally, together
                 const int y = y0 + (r & 1), z = z0 + (r >> 1);
                 in[r] = y < g.ny && z < g.nz;
-                const int64_t b = in[r] ? g.idx(x0, y, z) : b0;
+                const int64_t b = g.idx(x0, y < g.ny ? y : g.ny - 1, z < g.nz ? z : g.nz - 1);
+                if constexpr (N % 4 == 0) {
 #pragma unroll
-                for (int q = 0; q < CPT; q += 4)
-                    kf_load4_unaligned<S>(phi + b + q, scale, &v[r][q]);
-                v[r][CPT] = S::load(phi, x4 ? b + CPT : b, scale);
+                    for (int q = 0; q < N; q += 4)
+                        kf_load4_unaligned<S>(phi + b + q, scale, &v[r][q]);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < N; q++)
+                        v[r][q] = S::load(phi, b + q, scale);
+                }
+                v[r][N] = S::load(phi, xl ? b + N : b, scale);
             }
 #pragma unroll
-            for (int c = 0; c < CPT; c++) {
+            for (int c = 0; c < N; c++) {
                 cx[c] = x0 + c;
                 cy[c] = y0;
                 cz[c] = z0;
                 // corner k = (x + dx, y + dy, z + dz) is node row dy + 2 dz, column c + dx
-                const double d = (double)v[0][c];
                 int m = 0;
+                unsigned int out = 0;
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
                     const int r = kCornerY[k] + 2 * kCornerZ[k];
-                    const bool inside = in[r] && (c + kCornerX[k] < CPT || x4);
-                    const double val = inside ? (double)v[r][c + kCornerX[k]] : oob;
-                    if (val < iso)
-                        m |= 1 << k;
+                    const bool inside = in[r] && (c + kCornerX[k] < N || xl);
+                    const float f = inside ? v[r][c + kCornerX[k]] : oob_f;
+                    m |= f < iso_up ? 1 << k : 0;
+                    if constexpr (VALS) {
+                        cv[c][k] = v[r][c + kCornerX[k]];
+                        out |= inside ? 0u : 1u << k;
+                    }
                 }
-                ci[c] = (d > max_surface || d < -clip || m == 0 || m == 255) ? -1 : m;
+                if constexpr (VALS)
+                    om[c] = out;
+                ci[c] = case_of(v[0][c], m);
             }
         } else {
             int x = x0, y = y0, z = z0;
 #pragma unroll
-            for (int c = 0; c < CPT; c++) {
+            for (int c = 0; c < N; c++) {
                 if (x == g.nx) {
                     x = 0;
                     if (++y == g.ny) {
@@ -184,16 +213,24 @@ struct Cells {
                 cy[c] = y;
                 cz[c] = z;
                 int m = 0;
-                double d = 0.0;
+                unsigned int out = 0;
+                float d = 0.0f;
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    const double val = at_flat(x + kCornerX[k], y + kCornerY[k], z + kCornerZ[k]);
+                    const int xk = x + kCornerX[k], yk = y + kCornerY[k], zk = z + kCornerZ[k];
+                    const bool inside = g.in(xk, yk, zk);
+                    const float f = S::load(phi, inside ? g.idx(xk, yk, zk) : 0, scale);
                     if (k == 1)
-                        d = val;  // corner 1 is (x, y, z)
-                    if (val < iso)
-                        m |= 1 << k;
+                        d = f;  // corner 1 is (x, y, z), inside for every cell of the grid
+                    m |= (inside ? f : oob_f) < iso_up ? 1 << k : 0;
+                    if constexpr (VALS) {
+                        cv[c][k] = f;
+                        out |= inside ? 0u : 1u << k;
+                    }
                 }
-                ci[c] = (i0 + c >= g.n || d > max_surface || d < -clip || m == 0 || m == 255) ? -1 : m;
+                if constexpr (VALS)
+                    om[c] = out;
+                ci[c] = i0 + c >= g.n ? -1 : case_of(d, m);
                 ++x;
             }
         }
@@ -211,7 +248,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_count(Cells<S> C, unsigned int *__
     s_cnt[threadIdx.x] = c_tri_count[threadIdx.x];
     const int64_t i0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * CPT;
     int ci[CPT], x[CPT], y[CPT], z[CPT];
-    C.quad(i0, ci, x, y, z);
+    C.template quad<CPT>(i0, ci, x, y, z);
     __syncthreads();
     unsigned int cnt = 0;
 #pragma unroll
@@ -224,11 +261,13 @@ __global__ __launch_bounds__(BLOCK) void k_mc_count(Cells<S> C, unsigned int *__
 
 // Exclusive scan of nb block totals in one block of 1024 threads (each sums a run of
 // consecutive totals; wavefront scans, then the 16 wavefront totals through LDS); offs[nb] =
-// total, also written to *total_out when given.
+// total, also written to *total_out when given.  With blist, also the blocks whose total is
+// not zero, ascending: blist[0] = their number, blist[1 ..] = their indices.
 __global__ __launch_bounds__(1024) void k_mc_scan(const unsigned int *__restrict__ tot, unsigned long long *offs,
-                                                  int nb, unsigned long long *total_out)
+                                                  int nb, unsigned long long *total_out, unsigned int *blist)
 {
     __shared__ unsigned long long s[16];
+    __shared__ unsigned int s_ne[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int per = (nb + 1023) / 1024;
     const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
@@ -239,9 +278,13 @@ __global__ __launch_bounds__(1024) void k_mc_scan(const unsigned int *__restrict
     for (int j = 0; j < 8; j++)
         q0[j] = tot[min(b0 + j, nb - 1)];
     unsigned long long sum = 0;
+    unsigned int ne = 0;  // blocks of the run with triangles
 #pragma unroll
     for (int j = 0; j < 8; j++)
-        sum += b0 + j < b1 ? q0[j] : 0u;
+        if (b0 + j < b1) {
+            sum += q0[j];
+            ne += q0[j] != 0u;
+        }
     for (int b = b0 + 8; b < b1; b += 8) {
         unsigned int q[8];
 #pragma unroll
@@ -249,29 +292,43 @@ __global__ __launch_bounds__(1024) void k_mc_scan(const unsigned int *__restrict
             q[j] = tot[min(b + j, nb - 1)];
 #pragma unroll
         for (int j = 0; j < 8; j++)
-            sum += b + j < b1 ? q[j] : 0u;
+            if (b + j < b1) {
+                sum += q[j];
+                ne += q[j] != 0u;
+            }
     }
     unsigned long long v = sum;
+    unsigned int vn = ne;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const unsigned long long y = __shfl_up(v, o, 64);
-        if (lane >= o)
+        const unsigned int yn = __shfl_up(vn, o, 64);
+        if (lane >= o) {
             v += y;
+            vn += yn;
+        }
     }
-    if (lane == 63)
+    if (lane == 63) {
         s[w] = v;
+        s_ne[w] = vn;
+    }
     __syncthreads();
     unsigned long long run = v - sum, all = 0;
+    unsigned int pos = vn - ne, nall = 0;
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         run += q < w ? s[q] : 0ull;
         all += s[q];
+        pos += q < w ? s_ne[q] : 0u;
+        nall += s_ne[q];
     }
 #pragma unroll
     for (int j = 0; j < 8; j++)
         if (b0 + j < b1) {
             offs[b0 + j] = run;
             run += q0[j];
+            if (blist && q0[j] != 0u)
+                blist[1 + pos++] = (unsigned int)(b0 + j);
         }
     for (int b = b0 + 8; b < b1; b += 8) {
         unsigned int q[8];
@@ -283,35 +340,49 @@ __global__ __launch_bounds__(1024) void k_mc_scan(const unsigned int *__restrict
             if (b + j < b1) {
                 offs[b + j] = run;
                 run += q[j];
+                if (blist && q[j] != 0u)
+                    blist[1 + pos++] = (unsigned int)(b + j);
             }
     }
     if (t == 1023) {
         offs[nb] = all;
         if (total_out)
             *total_out = all;
+        if (blist)
+            blist[0] = nall;
     }
 }
+
+// Threads per block of the emit pass: the CELLS cells of a count block, ECPT per thread, so
+// that a surface-dense block still has a thread for nearly every cell that emits.
+constexpr int EB = 1024;
+constexpr int ECPT = CELLS / EB;
+static_assert(ECPT * EB == CELLS, "emit blocks cover the count blocks");
 
 // The triangles of cell (x, y, z), case ci (Polygonise, MarchingCubes.h:411-419), from
 // position t0 on, y negated (SimpleMesh.h:66-72); triangles at or past cap are dropped.  Each
 // vertex is VertexInterp (MarchingCubes.h:326-356) on its edge, p1 + (iso - v1) / (v1 - v2)
 // (p1 - p2) with p1 the edge's first corner, computed where the triangle uses it (the same
 // operations as once per edge, so the same values).  The corner values sit in the thread's
-// column of the block's LDS array sv (an edge picks its corners by runtime index, which
-// registers cannot serve without spilling).
-template <class S>
-__device__ __forceinline__ void emit_cell(const Cells<S> &C, int x, int y, int z, int ci, unsigned long long t0,
-                                          float *__restrict__ verts, unsigned long long cap, double (*sv)[BLOCK],
-                                          const unsigned char *s_edges, const unsigned char *s_cnt)
+// column of the block's LDS array sv as the floats they decode to (exact), with bit k of
+// `outside` set for a corner past the grid, which reads oob (an edge picks its corners by
+// runtime index, which registers cannot serve without spilling).  The classification pass
+// left them in column col; with GATHER, the thread loads them into that column first.
+template <class S, bool GATHER>
+__device__ __forceinline__ void emit_cell(const Cells<S> &C, int x, int y, int z, int ci, unsigned int outside,
+                                          unsigned long long t0, float *__restrict__ verts, unsigned long long cap,
+                                          float (*sv)[EB], int col, const unsigned char *s_edges,
+                                          const unsigned char *s_cnt)
 {
-    const int tid = threadIdx.x;
-    double val[8];
+    const int tid = col;
+    if constexpr (GATHER) {
 #pragma unroll
-    for (int k = 0; k < 8; k++)
-        val[k] = C.at_flat(x + kCornerX[k], y + kCornerY[k], z + kCornerZ[k]);
-#pragma unroll
-    for (int k = 0; k < 8; k++)
-        sv[k][tid] = val[k];
+        for (int k = 0; k < 8; k++) {
+            const int xk = x + kCornerX[k], yk = y + kCornerY[k], zk = z + kCornerZ[k];
+            const bool inside = C.g.in(xk, yk, zk);
+            sv[k][tid] = S::load(C.phi, inside ? C.g.idx(xk, yk, zk) : 0, C.scale);
+        }
+    }
     // corner positions in the soup's normalised coordinates (SDF.cpp:338-355: v / (dims / 2) -
     // 1), two per axis: corners 0, 3, 4, 7 are at x + 1 (bits of 0x99), 2, 3, 6, 7 at y + 1
     // (0xcc), 4..7 at z + 1
@@ -332,7 +403,12 @@ __device__ __forceinline__ void emit_cell(const Cells<S> &C, int x, int y, int z
             // 0..3 join corners e, e + 1 (mod 4); 4..7 join e, e + 1 (4..7 mod 4); 8..11 join e - 8, e - 4
             const int a = e < 8 ? e : e - 8;
             const int b = e < 4 ? (e + 1) & 3 : e < 8 ? 4 + ((e + 1) & 3) : e - 4;
-            const double va = sv[a][tid], vb = sv[b][tid];
+            // both LDS reads are issued whatever the corners (a corner past the grid reads its
+            // column's placeholder, then takes oob): no branch, one wait
+            float fa = sv[a][tid], fb = sv[b][tid];
+            asm volatile("" : "+v"(fa), "+v"(fb));
+            const double va = (outside >> a) & 1u ? C.oob : (double)fa;
+            const double vb = (outside >> b) & 1u ? C.oob : (double)fb;
             const double pa[3] = {(0x99 >> a) & 1 ? px[1] : px[0], (0xcc >> a) & 1 ? py[1] : py[0],
                                   a >> 2 ? pz[1] : pz[0]};
             const double pb[3] = {(0x99 >> b) & 1 ? px[1] : px[0], (0xcc >> b) & 1 ? py[1] : py[0],
@@ -345,60 +421,90 @@ __device__ __forceinline__ void emit_cell(const Cells<S> &C, int x, int y, int z
     }
 }
 
-// Emit: a block whose cells hold no triangles (equal offsets) returns at once.  Otherwise it
-// classifies its cells, lists the ones with triangles in LDS in cell order
-// with their triangle offsets (one block scan of the packed counts), and then spreads the
-// listed cells over its threads, one cell per thread, so the gathers and FP64 interpolation of
-// a surface-dense stretch of cells run in parallel rather than one after another in one lane.
+// Emit: only the count blocks that hold triangles are visited (blist, from the scan), in a
+// grid of at most one workgroup per such block that strides over them, so the many blocks of
+// cells without triangles cost no dispatch.  A workgroup classifies its count block's CELLS
+// cells (ECPT per thread), lists the ones with triangles in LDS in cell order with their
+// triangle offsets (one block scan of the packed counts) and corner values, and then spreads
+// the listed cells over its threads, one cell per thread, so the FP64 interpolation of a
+// surface-dense stretch of cells runs in parallel rather than one after another in one lane.
+// The list length, the first block index and its triangle offset are loaded together, and
+// the corner values come from the classification's own loads: two round trips to memory
+// before the stores.
 template <class S>
-__global__ __launch_bounds__(BLOCK) void k_mc_emit(Cells<S> C, const unsigned long long *__restrict__ offs,
-                                                   float *__restrict__ verts, unsigned long long cap)
+__global__ __launch_bounds__(EB) void k_mc_emit(Cells<S> C, const unsigned long long *__restrict__ offs,
+                                                const unsigned int *__restrict__ blist, float *__restrict__ verts,
+                                                unsigned long long cap)
 {
-    __shared__ unsigned int s[BLOCK / 64];
+    __shared__ unsigned int s[EB / 64];
     __shared__ unsigned char s_cnt[256];
-    __shared__ unsigned int s_cell[CELLS];   // (cell within the block << 8) | case
+    __shared__ unsigned int s_cell[CELLS];   // (outside corners << 24) | (cell within the block << 8) | case
     __shared__ unsigned short s_tri[CELLS];  // its first triangle within the block
-    __shared__ double s_val[8][BLOCK];       // corner values of the cell each thread emits
+    __shared__ float s_val[8][EB];           // corner values of listed cells 0 .. EB - 1
     __shared__ unsigned int s_edges[256 * 15 / 4];  // the case table's edge ids
-    const unsigned long long base = offs[blockIdx.x];
-    if (offs[blockIdx.x + 1] == base)
-        return;  // no triangles in this block's cells (the count pass's total)
-    s_cnt[threadIdx.x] = c_tri_count[threadIdx.x];
-    for (int k = threadIdx.x; k < 256 * 15 / 4; k += BLOCK)
+    // blist holds nb + 1 entries and the grid has at most nb workgroups
+    const unsigned int nne = blist[0], first = blist[1 + blockIdx.x];
+    if (blockIdx.x >= nne)
+        return;
+    if (threadIdx.x < 256)
+        s_cnt[threadIdx.x] = c_tri_count[threadIdx.x];
+    for (int k = threadIdx.x; k < 256 * 15 / 4; k += EB)
         s_edges[k] = reinterpret_cast<const unsigned int *>(&c_tri_edges[0][0])[k];
-    const int64_t blk0 = (int64_t)blockIdx.x * CELLS;
-    const int64_t i0 = blk0 + (int64_t)threadIdx.x * CPT;
-    int ci[CPT], x[CPT], y[CPT], z[CPT];
-    C.quad(i0, ci, x, y, z);
-    __syncthreads();
-    unsigned int ns = 0, nt = 0;
+    for (unsigned int q = blockIdx.x; q < nne; q += gridDim.x) {
+        const unsigned int blk = q == blockIdx.x ? first : blist[1 + q];
+        const unsigned long long base = offs[blk];
+        const int64_t blk0 = (int64_t)blk * CELLS;
+        const int64_t i0 = blk0 + (int64_t)threadIdx.x * ECPT;
+        int ci[ECPT], x[ECPT], y[ECPT], z[ECPT];
+        float cv[ECPT][8];
+        unsigned int om[ECPT];
+        C.template quad<ECPT, true>(i0, ci, x, y, z, cv, om);
+        __syncthreads();  // the tables are in place, the previous block's cells emitted
+        unsigned int ns = 0, nt = 0;
 #pragma unroll
-    for (int c = 0; c < CPT; c++)
-        if (ci[c] >= 0) {
-            ns += 1;
-            nt += s_cnt[ci[c]];
-        }
-    // cells in the high half, triangles (at most 5 CELLS < 2^16) in the low half
-    static_assert(5 * CELLS < (1 << 16), "packed counts");
-    unsigned int total;
-    const unsigned int incl = block_scan_incl((ns << 16) | nt, s, total);
-    unsigned int slot = (incl >> 16) - ns, tri = (incl & 0xffffu) - nt;
+        for (int c = 0; c < ECPT; c++)
+            if (ci[c] >= 0) {
+                ns += 1;
+                nt += s_cnt[ci[c]];
+            }
+        // cells in the high half, triangles (at most 5 CELLS < 2^16) in the low half
+        static_assert(5 * CELLS < (1 << 16), "packed counts");
+        unsigned int total;
+        const unsigned int incl = block_scan_incl<EB>((ns << 16) | nt, s, total);
+        unsigned int slot = (incl >> 16) - ns, tri = (incl & 0xffffu) - nt;
 #pragma unroll
-    for (int c = 0; c < CPT; c++)
-        if (ci[c] >= 0) {
-            s_cell[slot] = ((unsigned int)(threadIdx.x * CPT + c) << 8) | (unsigned int)ci[c];
-            s_tri[slot] = (unsigned short)tri;
-            slot += 1;
-            tri += s_cnt[ci[c]];
+        for (int c = 0; c < ECPT; c++)
+            if (ci[c] >= 0) {
+                s_cell[slot] = (om[c] << 24) | ((unsigned int)(threadIdx.x * ECPT + c) << 8) | (unsigned int)ci[c];
+                s_tri[slot] = (unsigned short)tri;
+                if (slot < EB) {
+#pragma unroll
+                    for (int k = 0; k < 8; k++)
+                        s_val[k][slot] = cv[c][k];
+                }
+                slot += 1;
+                tri += s_cnt[ci[c]];
+            }
+        __syncthreads();
+        const unsigned int ncells = total >> 16;
+        // listed cell e: its values in column e for e < EB; past that (a block with more
+        // listed cells than threads) the thread gathers them into its own column, which its
+        // first cell no longer needs
+        const unsigned char *edges = reinterpret_cast<const unsigned char *>(s_edges);
+        if (threadIdx.x < ncells) {
+            const unsigned int e = threadIdx.x, cell = s_cell[e];
+            int cx, cy, cz;
+            kf_xyz(C.g, blk0 + ((cell >> 8) & 0xffffu), cx, cy, cz);
+            emit_cell<S, false>(C, cx, cy, cz, (int)(cell & 255u), cell >> 24, base + s_tri[e], verts, cap, s_val,
+                                (int)threadIdx.x, edges, s_cnt);
         }
-    __syncthreads();
-    const unsigned int ncells = total >> 16;
-    for (unsigned int e = threadIdx.x; e < ncells; e += BLOCK) {
-        const unsigned int cell = s_cell[e];
-        int cx, cy, cz;
-        kf_xyz(C.g, blk0 + (cell >> 8), cx, cy, cz);
-        emit_cell(C, cx, cy, cz, (int)(cell & 255u), base + s_tri[e], verts, cap, s_val,
-                  reinterpret_cast<const unsigned char *>(s_edges), s_cnt);
+        for (unsigned int e = threadIdx.x + EB; e < ncells; e += EB) {
+            const unsigned int cell = s_cell[e];
+            int cx, cy, cz;
+            kf_xyz(C.g, blk0 + ((cell >> 8) & 0xffffu), cx, cy, cz);
+            emit_cell<S, true>(C, cx, cy, cz, (int)(cell & 255u), cell >> 24, base + s_tri[e], verts, cap, s_val,
+                               (int)threadIdx.x, edges, s_cnt);
+        }
     }
 }
 
@@ -467,7 +573,7 @@ __global__ __launch_bounds__(BLOCK) void k_flag_ids(const unsigned char *flags, 
         const int64_t j = base + (int64_t)k * BLOCK + threadIdx.x;
         const unsigned int f = j < n ? flags[j] : 0u;
         unsigned int round_total;
-        const unsigned int incl = block_scan_incl(f, s, round_total);
+        const unsigned int incl = block_scan_incl<BLOCK>(f, s, round_total);
         if (f)
             ids[j] = (unsigned int)(run + incl - 1);
         run += round_total;
@@ -529,13 +635,13 @@ __global__ __launch_bounds__(BLOCK) void k_mc_emit_indexed(Cells<S> C, EdgeGrid 
     __shared__ unsigned int s[BLOCK / 64];
     const int64_t i0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * CPT;
     int ci[CPT], x[CPT], y[CPT], z[CPT];
-    C.quad(i0, ci, x, y, z);
+    C.template quad<CPT>(i0, ci, x, y, z);
     unsigned int cnt = 0;
 #pragma unroll
     for (int c = 0; c < CPT; c++)
         cnt += ci[c] >= 0 ? c_tri_count[ci[c]] : 0u;
     unsigned int total;
-    const unsigned int incl = block_scan_incl(cnt, s, total);
+    const unsigned int incl = block_scan_incl<BLOCK>(cnt, s, total);
     unsigned long long t0 = offs[blockIdx.x] + (incl - cnt);
     for (int c = 0; c < CPT; c++) {
         if (ci[c] < 0)
@@ -562,6 +668,14 @@ Cells<S> make_cells(const kf_volume *v, double iso)
     C.max_surface = v->max_surface;
     C.clip = v->clip;
     C.iso = iso;
+    // least float >= iso and >= -clip, greatest float <= max_surface
+    float f = (float)iso;
+    C.iso_up = (double)f < iso ? std::nextafterf(f, INFINITY) : f;
+    f = (float)(-v->clip);
+    C.clip_up = (double)f < -v->clip ? std::nextafterf(f, INFINITY) : f;
+    f = (float)v->max_surface;
+    C.max_dn = (double)f > v->max_surface ? std::nextafterf(f, -INFINITY) : f;
+    C.oob_f = v->oob() < iso ? -INFINITY : INFINITY;
     return C;
 }
 
@@ -614,18 +728,32 @@ kf_status kf_mc_init_tables()
 
 int64_t kf_mc_blocks(int64_t n) { return (n + CELLS - 1) / CELLS; }
 
+namespace {
+
+// Workgroups of the emit pass: it strides over the count blocks that hold triangles, two
+// workgroups of EB threads per CU at most.
+unsigned int emit_grid(const kf_context *ctx, int nb)
+{
+    const int g = 2 * (ctx->num_cus > 0 ? ctx->num_cus : 256);
+    return (unsigned int)(nb < g ? nb : g);
+}
+
+}  // namespace
+
 kf_status kf_mc_launch_into(const kf_volume *v, double iso, unsigned int *tot, unsigned long long *offs,
-                            unsigned long long *total_out, float *verts, uint64_t cap)
+                            unsigned int *blist, unsigned long long *total_out, float *verts, uint64_t cap)
 {
     const int nb = (int)kf_mc_blocks(v->n);
     hipStream_t st = v->ctx->stream;
     KF_DISPATCH_DTYPE(v->dtype, S, {
         const Cells<S> C = make_cells<S>(v, iso);
         hipLaunchKernelGGL((k_mc_count<S>), dim3(nb), dim3(BLOCK), 0, st, C, tot);
-        hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, st, (const unsigned int *)tot, offs, nb, total_out);
+        hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, st, (const unsigned int *)tot, offs, nb, total_out,
+                           blist);
         if (cap > 0)
-            hipLaunchKernelGGL((k_mc_emit<S>), dim3(nb), dim3(BLOCK), 0, st, C, (const unsigned long long *)offs,
-                               verts, (unsigned long long)cap);
+            hipLaunchKernelGGL((k_mc_emit<S>), dim3(emit_grid(v->ctx, nb)), dim3(EB), 0, st, C,
+                               (const unsigned long long *)offs, (const unsigned int *)blist, verts,
+                               (unsigned long long)cap);
     });
     KF_LAUNCH_CHECK();
     return KF_OK;
@@ -651,16 +779,19 @@ kf_status mc_count_cached(const kf_volume *v, double iso, uint64_t *total)
         KF_HIP(hipStreamSynchronize(ctx->stream));
         (void)hipFree(c.tot);
         (void)hipFree(c.offs);
+        (void)hipFree(c.blist);
         c.tot = nullptr;
         c.offs = nullptr;
+        c.blist = nullptr;
         c.cap = 0;
         KF_HIP(hipMalloc((void **)&c.tot, sizeof(unsigned int) * (size_t)nb));
         KF_HIP(hipMalloc((void **)&c.offs, sizeof(unsigned long long) * ((size_t)nb + 1)));
+        KF_HIP(hipMalloc((void **)&c.blist, sizeof(unsigned int) * ((size_t)nb + 1)));
         c.cap = (size_t)nb;
     }
     if (!c.total_host)
         KF_HIP(hipHostMalloc((void **)&c.total_host, 2 * sizeof(unsigned long long), hipHostMallocDefault));
-    kf_status s = kf_mc_launch_into(v, iso, c.tot, c.offs, nullptr, nullptr, 0);
+    kf_status s = kf_mc_launch_into(v, iso, c.tot, c.offs, c.blist, nullptr, nullptr, 0);
     if (s != KF_OK)
         return s;
     KF_HIP(hipMemcpyAsync(c.total_host, c.offs + nb, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
@@ -679,8 +810,9 @@ kf_status mc_emit_cached(const kf_volume *v, double iso, float *dverts, uint64_t
     kf_context *ctx = v->ctx;
     const int nb = mc_blocks(v);
     KF_DISPATCH_DTYPE(v->dtype, S, {
-        hipLaunchKernelGGL((k_mc_emit<S>), dim3(nb), dim3(BLOCK), 0, ctx->stream, make_cells<S>(v, iso),
-                           (const unsigned long long *)ctx->mc.offs, dverts, (unsigned long long)cap);
+        hipLaunchKernelGGL((k_mc_emit<S>), dim3(emit_grid(ctx, nb)), dim3(EB), 0, ctx->stream, make_cells<S>(v, iso),
+                           (const unsigned long long *)ctx->mc.offs, (const unsigned int *)ctx->mc.blist, dverts,
+                           (unsigned long long)cap);
     });
     KF_LAUNCH_CHECK();
     return KF_OK;
@@ -708,6 +840,7 @@ void kf_mc_release(kf_context *ctx)
     kf_mc_cache &c = ctx->mc;
     (void)hipFree(c.tot);
     (void)hipFree(c.offs);
+    (void)hipFree(c.blist);
     (void)hipFree(c.verts);
     if (c.total_host)
         (void)hipHostFree(c.total_host);
@@ -786,7 +919,7 @@ kf_status kf_mc_run_indexed(const kf_volume *v, double iso, uint64_t *n_verts, u
                                st, C, E, flags);
             hipLaunchKernelGGL(k_flag_sums, dim3(nbe), dim3(BLOCK), 0, st, (const unsigned char *)flags, ne, etot);
             hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, st, (const unsigned int *)etot, eoffs, nbe,
-                               (unsigned long long *)nullptr);
+                               (unsigned long long *)nullptr, (unsigned int *)nullptr);
             e = hipGetLastError();
         }
         if (e == hipSuccess)

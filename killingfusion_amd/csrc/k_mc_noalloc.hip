@@ -11,7 +11,8 @@
 // created there (kf_noalloc_init), and is what the kernels receive.
 //
 // The count kernel writes every block total and the scan every offset of the grid it runs on,
-// so no part of the array has to start at zero.  Every call on one GPU shares its array: calls
+// and the list of blocks with triangles with its length, so no part of the array has to start
+// at zero.  Every call on one GPU shares its array: calls
 // that overlap on one GPU (other streams, other threads) race and must be ordered by the caller.
 #include "kf_internal.hpp"
 
@@ -21,8 +22,10 @@
 namespace {
 
 constexpr int64_t MAX_BLOCKS = (KF_MC_NOALLOC_MAX_VOXELS + KF_MC_CELLS - 1) / KF_MC_CELLS;
-constexpr size_t TOT_WORDS = (size_t)(MAX_BLOCKS + 1) / 2;  // u32 block totals, two per word
-constexpr size_t SCRATCH_WORDS = TOT_WORDS + (size_t)MAX_BLOCKS + 1;  // then u64 offsets and the total
+constexpr size_t TOT_WORDS = (size_t)(MAX_BLOCKS + 1) / 2;   // u32 block totals, two per word
+constexpr size_t OFFS_WORDS = (size_t)MAX_BLOCKS + 1;        // then u64 offsets and the total
+constexpr size_t LIST_WORDS = (size_t)(MAX_BLOCKS + 2) / 2;  // then the u32 list of blocks with triangles
+constexpr size_t SCRATCH_WORDS = TOT_WORDS + OFFS_WORDS + LIST_WORDS;
 static_assert(SCRATCH_WORDS * 8 <= (size_t)2 << 20, "scratch stays under 2 MiB");
 
 static __device__ __align__(256) unsigned long long scratch[SCRATCH_WORDS];
@@ -66,5 +69,6 @@ extern "C" kf_status kf_marching_cubes_noalloc(const kf_volume *v, double iso, f
     KF_CHECK_ARG(base, "kf_marching_cubes_noalloc: no scratch address for this device");
     KfDeviceGuard dg(ctx->device);
     return kf_mc_launch_into(v, iso, reinterpret_cast<unsigned int *>(base), base + TOT_WORDS,
+                             reinterpret_cast<unsigned int *>(base + TOT_WORDS + OFFS_WORDS),
                              reinterpret_cast<unsigned long long *>(count), verts, cap_tris);
 }

@@ -165,6 +165,7 @@ struct kf_mc_cache {
     bool valid = false;
     unsigned int *tot = nullptr;           // triangles per block
     unsigned long long *offs = nullptr;    // exclusive block offsets, then the total
+    unsigned int *blist = nullptr;         // blocks with triangles: their number, then their indices
     size_t cap = 0;                        // blocks
     unsigned long long *total_host = nullptr;  // pinned: the cached triangle total, then a spare slot
     float *verts = nullptr;                // device staging of a soup copied to the host
@@ -599,10 +600,11 @@ void kf_mc_release(kf_context *ctx);
 constexpr int KF_MC_CELLS = 2048;
 int64_t kf_mc_blocks(int64_t n);
 // Count, scan and (cap > 0) emit of v's triangle soup with caller-provided device buffers:
-// tot[kf_mc_blocks(n)], offs[kf_mc_blocks(n) + 1], verts[9 cap]; *total_out (may be null, device)
-// receives the triangle count.  Stream-ordered kernels only: no allocation, no host wait.
+// tot[kf_mc_blocks(n)], offs[kf_mc_blocks(n) + 1], blist[kf_mc_blocks(n) + 1], verts[9 cap];
+// *total_out (may be null, device) receives the triangle count.  Stream-ordered kernels only:
+// no allocation, no host wait.
 kf_status kf_mc_launch_into(const kf_volume *v, double iso, unsigned int *tot, unsigned long long *offs,
-                            unsigned long long *total_out, float *verts, uint64_t cap);
+                            unsigned int *blist, unsigned long long *total_out, float *verts, uint64_t cap);
 kf_status kf_mc_run_indexed(const kf_volume *v, double iso, uint64_t *n_verts, uint64_t *n_tris, float *verts_host,
                             float *normals_host, uint32_t *tris_host, uint64_t cap_verts, uint64_t cap_tris);
 
