@@ -91,3 +91,36 @@ def test_marching_cubes_count_cache(ctx):
     assert np.array_equal(d.getMesh().tris, soup_ref(d))
     for h in (a, c, d, field):
         h.close()
+
+
+def threshold_neighbours(t):
+    """The float nearest t and the floats on either side of it."""
+    f = np.float32(t)
+    return [np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))]
+
+
+@pytest.mark.parametrize("iso", [0.0123456789, 0.0, 0.05, -0.0299999])
+def test_marching_cubes_threshold_edges(ctx, iso):
+    """The passes compare corner values in FP32 against thresholds rounded outward from the
+    FP64 ones (iso, MaxSurfaceVoxelDistance, -UnknownClipDistance; none of them a float here
+    but 0): values drawn from the floats at and next to every threshold must classify, skip
+    and interpolate exactly as the oracle's FP64 comparisons do.  iso = 0.05 lies above the
+    out-of-grid value (Max + eps), so boundary corners count as below the surface."""
+    dims = (23, 17, 19)
+    print(f"case: marching cubes at threshold-adjacent values, {dims}, iso {iso}", flush=True)
+    pool = []
+    for t in (iso, TRUNC, -TRUNC, TRUNC + 1e-5):
+        pool += threshold_neighbours(t)
+    rng = np.random.default_rng(17)
+    pool = np.array(pool + list(rng.uniform(-0.06, 0.06, 24)), dtype=np.float32)
+    n = dims[0] * dims[1] * dims[2]
+    phi = rng.choice(pool, n).astype(np.float32)
+    v = SDF(ctx, dims, (0.0, 0.0, 0.0), VOXEL, TRUNC, TRUNC, dtype="f32")
+    v.upload(phi)
+    g = ogrid(dims, (0.0, 0.0, 0.0), VOXEL, TRUNC)
+    ref = O.marching_cubes(phi.astype(np.float64), g, TRUNC, TRUNC, iso).astype(np.float32)
+    mesh = v.getMesh(iso)
+    assert ref.shape[0] > 500
+    assert mesh.tris.shape[0] == ref.shape[0]
+    assert np.array_equal(mesh.tris, ref)
+    v.close()
