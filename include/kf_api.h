@@ -113,7 +113,7 @@ typedef struct kf_deform_stats {
     double max_update;       /* max |du| of the last iteration, voxels (FP64) */
     double energy;           /* adjoint: energy of the final field (FP64); reference: 0 */
     uint64_t active_voxels;  /* voxels updated in the last iteration */
-    uint64_t listed_voxels;  /* voxels that could move (adjoint; 0 when the list was not built) */
+    uint64_t listed_voxels;  /* voxels that could move (0 when they were not counted: a dense sweep) */
 } kf_deform_stats;
 
 /* ---- library / context ---- */
