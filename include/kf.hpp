@@ -230,18 +230,22 @@ public:
     }
     /* DisplacementField::dumpToBinFile, DisplacementField.cpp:349-366: int32 dims[3], double
      * voxel size, then N x 3 doubles (x, y, z per voxel). */
-    void dumpToBinFile(const std::string &path) const
+    void dumpToBinFile(const std::string &path) const { writeFieldBin(h_, dims_, voxel_, path); }
+    /* The same dump of any field handle (SoA float32 on the device, dims voxels). */
+    static void writeFieldBin(kf_field *h, const std::array<int32_t, 3> &dims, double voxel, const std::string &path)
     {
-        const std::vector<float> s = download();
+        const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+        std::vector<float> s(3 * n);
+        check(kf_field_download(h, s.data()), "kf_field_download");
         std::ofstream out(path, std::ios::binary);
         if (!out.is_open())
             throw std::runtime_error("cannot write " + path);
-        out.write((const char *)dims_.data(), 3 * sizeof(int32_t));
-        out.write((const char *)&voxel_, sizeof(double));
-        std::vector<double> aos(3 * n());
-        for (size_t i = 0; i < n(); i++)
+        out.write((const char *)dims.data(), 3 * sizeof(int32_t));
+        out.write((const char *)&voxel, sizeof(double));
+        std::vector<double> aos(3 * n);
+        for (size_t i = 0; i < n; i++)
             for (int a = 0; a < 3; a++)
-                aos[3 * i + a] = s[(size_t)a * n() + i];
+                aos[3 * i + a] = s[(size_t)a * n + i];
         out.write((const char *)aos.data(), (std::streamsize)(aos.size() * sizeof(double)));
     }
     size_t n() const { return (size_t)dims_[0] * dims_[1] * dims_[2]; }
@@ -430,10 +434,13 @@ inline std::vector<kf_deform_stats> batchDeform(const std::vector<const SDF *> &
  * canonical SDF (frame 1 becomes the canonical SDF). */
 class KillingFusion {
 public:
+    /* fuseMode: KF_FUSE_MERGE (FUSE_BY_MERGE = true, config.cpp:7) or KF_FUSE_NEAREST. */
     KillingFusion(Context &ctx, const std::array<int32_t, 3> &dims, const std::array<double, 3> &min3d,
                   const Params &p, const std::array<double, 9> &K, const std::array<double, 16> &C2W,
-                  double minDepth, double maxDepth, int width, int height, kf_dtype dtype = KF_F32)
-        : width_(width), height_(height)
+                  double minDepth, double maxDepth, int width, int height, kf_dtype dtype = KF_F32,
+                  int32_t fuseMode = KF_FUSE_MERGE)
+        : width_(width), height_(height), dims_(dims), voxel_(p.voxel_size),
+          deforms_(p.use_data || p.use_levelset || p.use_killing)
     {
         kf_pipeline_desc d{};
         for (int a = 0; a < 3; a++) {
@@ -454,6 +461,7 @@ public:
         d.max_depth = maxDepth;
         d.width = width;
         d.height = height;
+        d.fuse_mode = fuseMode;
         check(kf_pipeline_create(ctx.handle(), &d, &h_), "kf_pipeline_create");
     }
     ~KillingFusion() { kf_pipeline_destroy(h_); }
@@ -466,7 +474,40 @@ public:
             throw std::invalid_argument("depth frame size");
         kf_frame_stats st{};
         check(kf_pipeline_process_frame(h_, depth.data(), &st), "kf_pipeline_process_frame");
+        // KillingFusion.cpp:146-151: after the deform of a frame (every frame but the first,
+        // when some energy is on) its field goes to <prefix><frame, 3 digits>.bin
+        if (!dumpPrefix_.empty() && st.frame_index > 0 && deforms_) {
+            char num[16];
+            std::snprintf(num, sizeof(num), "%03d", dumpStart_ + dumpStride_ * (int)st.frame_index);
+            DisplacementField::writeFieldBin(kf_pipeline_field(h_), dims_, voxel_, dumpPrefix_ + num + ".bin");
+        }
         return st;
+    }
+    /* The per-frame displacement dump of the reference (OUTPUT_DIR + dataset directory, then the
+     * dataset frame number m_currFrameIndex = startFrame + stride x the frame's index, ".bin");
+     * an empty prefix (the default) writes nothing. */
+    void setFieldDumpPrefix(const std::string &prefix, int startFrame = 0, int stride = 1)
+    {
+        dumpPrefix_ = prefix;
+        dumpStart_ = startFrame;
+        dumpStride_ = stride;
+    }
+    /* Which of the frame's meshes each processNextFrame extracts (KF_MESH_* bits, iso value):
+     * the live SDF before the warp (:157), after it (:160), the canonical SDF (:170). */
+    void setMeshes(int32_t which, double iso = 0.0)
+    {
+        check(kf_pipeline_set_meshes(h_, which, iso), "kf_pipeline_set_meshes");
+    }
+    /* One of the last frame's meshes (a KF_MESH_* value chosen with setMeshes). */
+    SimpleMesh frameMesh(int32_t which) const
+    {
+        uint64_t nt = 0;
+        check(kf_pipeline_mesh(h_, which, &nt, nullptr, 0), "kf_pipeline_mesh");
+        SimpleMesh m;
+        m.vertices().resize((size_t)nt * 9);
+        if (nt)
+            check(kf_pipeline_mesh(h_, which, &nt, m.vertices().data(), nt), "kf_pipeline_mesh");
+        return m;
     }
     /* Canonical mesh (SDF::getMesh on m_canonicalSdf, KillingFusion.cpp:170). */
     SimpleMesh canonicalMesh(double iso = 0.0) const
@@ -481,10 +522,16 @@ public:
         return m;
     }
     kf_volume *canonical() const { return kf_pipeline_canonical(h_); }
+    kf_pipeline *handle() const { return h_; }
 
 private:
     kf_pipeline *h_ = nullptr;
     int width_, height_;
+    std::array<int32_t, 3> dims_;
+    double voxel_;
+    bool deforms_;
+    std::string dumpPrefix_;
+    int dumpStart_ = 0, dumpStride_ = 1;
 };
 
 }  // namespace kf

@@ -499,6 +499,16 @@ class KillingFusion:
         self.handle = h
         ctx._adopt(self)
         self.dims = tuple(dims)
+        self.voxel_size = voxel_size
+        self._deforms = bool(params.use_data or params.use_levelset or params.use_killing)
+        self._dump = None
+
+    def setFieldDumpPrefix(self, prefix, start_frame: int = 0, stride: int = 1):
+        """The reference's per-frame displacement dump (KillingFusion.cpp:146-151): after the
+        deform of each frame but the first (when some energy is on), the frame's field goes to
+        prefix + the dataset frame number (start_frame + stride x the frame's index, 3 digits) +
+        ".bin" in DisplacementField::dumpToBinFile's layout.  None (the default) writes nothing."""
+        self._dump = None if prefix is None else (str(prefix), int(start_frame), int(stride))
 
     MESHES = {"live": _lib.KF_MESH_LIVE, "live_warped": _lib.KF_MESH_LIVE_WARPED,
               "canonical": _lib.KF_MESH_CANONICAL}
@@ -533,6 +543,10 @@ class KillingFusion:
         st = _lib.FrameStats()
         check(self._L.kf_pipeline_process_frame(self.handle, depth.ctypes.data_as(_fp), C.byref(st)),
               "kf_pipeline_process_frame")
+        if self._dump is not None and st.frame_index > 0 and self._deforms:
+            prefix, start, stride = self._dump
+            formats.write_displacement_bin(f"{prefix}{start + stride * st.frame_index:03d}.bin", self.dims,
+                                           self.voxel_size, self.field_download())
         return dict(frame=st.frame_index, deform=_stats(st.deform), ms_integrate=st.ms_integrate,
                     ms_deform=st.ms_deform, ms_fuse=st.ms_fuse, ms_total=st.ms_total)
 

@@ -155,7 +155,13 @@ int main(int argc, char **argv)
         pp.max_iters = 100;
         pp.use_levelset = 0;
         kf::KillingFusion kfu(ctx, dims, mn, pp, K, I4, 0.1, 3.0, W, H);
+        kfu.setFieldDumpPrefix(out + "/frame_", 10, 2);
+        kfu.setMeshes(KF_MESH_LIVE | KF_MESH_LIVE_WARPED | KF_MESH_CANONICAL);
         kfu.processNextFrame(depth);
+        const size_t m0 = kfu.frameMesh(KF_MESH_CANONICAL).triangles();
+        expect(m0 > 100 && kfu.frameMesh(KF_MESH_LIVE).triangles() == m0 &&
+                   kfu.frameMesh(KF_MESH_LIVE_WARPED).triangles() == m0,
+               "frame 0: all three meshes are the canonical mesh");
         kf_frame_stats f2 = kfu.processNextFrame(depth);
         kf_volume_desc d{};
         kf::check(kf_volume_describe(kfu.canonical(), &d), "kf_volume_describe");
@@ -170,6 +176,12 @@ int main(int argc, char **argv)
         expect(f3.frame_index == 2 && f3.deform.diverged_index < 0 && f3.ms_total > 0.0, "pipeline frame 3");
         kf::SimpleMesh mesh = kfu.canonicalMesh();
         expect(mesh.triangles() > 100, "canonical mesh");
+        expect(kfu.frameMesh(KF_MESH_CANONICAL).vertices() == mesh.vertices(), "frame mesh = canonical mesh");
+        // the frame's field dumps: frames 1 and 2 at dataset numbers 12 and 14, none for frame 0
+        kf::DisplacementField::writeFieldBin(kf_pipeline_field(kfu.handle()), dims, voxel, out + "/last_field.bin");
+        expect(!std::ifstream(out + "/frame_010.bin").good() && std::ifstream(out + "/frame_012.bin").good() &&
+                   std::ifstream(out + "/frame_014.bin").good(),
+               "per-frame field dumps");
 
         // Welded mesh of the same plane: the soup's triangles over shared vertices.
         kf::IndexedMesh wm = plane.getMeshIndexed();

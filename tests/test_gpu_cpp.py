@@ -32,6 +32,11 @@ def test_cpp_mirror_self_checks(tmp_path):
     assert u["dims"] == (32, 24, 40) and np.array_equal(u["u"][:, 0], [0.75, 0.0, 1.5])
     v, f = formats.read_off(tmp_path / "plane.off")
     assert f.shape[0] > 100 and v.shape[0] == 3 * f.shape[0]
+    # the driver's per-frame field dump of its last frame equals a dump of its field afterwards
+    assert (tmp_path / "frame_014.bin").read_bytes() == (tmp_path / "last_field.bin").read_bytes()
+    last = formats.read_displacement_bin(tmp_path / "frame_014.bin")
+    assert last["dims"] == (32, 24, 40) and np.isfinite(last["u"]).all()
+    assert not (tmp_path / "frame_010.bin").exists()
 
 
 def test_python_and_cpp_dumps_are_byte_identical(ctx, tmp_path):

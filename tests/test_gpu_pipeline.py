@@ -99,3 +99,25 @@ def test_pipeline_frame_meshes_match_oracle(ctx):
     with pytest.raises(Exception):
         pl.frameMesh("live")
     pl.close()
+
+
+def test_pipeline_field_dumps(ctx, tmp_path):
+    """The reference writes each frame's displacement field after its deform
+    (KillingFusion.cpp:146-151), named by the dataset frame number; frame 0 has no deform."""
+    print("case: pipeline per-frame field dumps, 3 frames of the 64^3 sphere pair", flush=True)
+    from killingfusion_amd import formats
+    d1, d2 = sphere_frames()
+    kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint", max_iters=20)
+    pl = KillingFusion(ctx, C1["dims"], C1["min3d"], C1["voxel"], kp, scenes.K)
+    pl.setFieldDumpPrefix(tmp_path / "frame_", start_frame=5, stride=3)
+    fields = []
+    for d in (d1, d2, d1):
+        pl.processNextFrame(d)
+        fields.append(pl.field_download())
+    assert not (tmp_path / "frame_005.bin").exists()
+    for k, name in ((1, "frame_008.bin"), (2, "frame_011.bin")):
+        u = formats.read_displacement_bin(tmp_path / name)
+        assert u["dims"] == tuple(C1["dims"]) and u["voxel"] == C1["voxel"]
+        assert np.array_equal(u["u"], fields[k].astype(np.float64)), name
+    assert np.abs(fields[1]).max() > 0
+    pl.close()
