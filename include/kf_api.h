@@ -104,6 +104,9 @@ typedef struct kf_params {
     double stop_max_update; /* stop once max |du| < this (voxels); <= 0 runs max_iters */
     int32_t method;         /* kf_method */
     int32_t sweep;          /* kf_sweep (either method) */
+    int32_t use_trust;      /* UseTrustStrategy (config.cpp:27, off): the reference method's trust region,
+                               honoured with the data energy alone (KillingFusion.cpp:288-314) */
+    int32_t reserved;       /* 0 */
 } kf_params;
 
 typedef struct kf_deform_stats {

@@ -33,7 +33,7 @@ class Params(C.Structure):
                 ("gamma_killing", C.c_double), ("omega_levelset", C.c_double), ("delta_size", C.c_double),
                 ("killing_clamp", C.c_double), ("use_data", C.c_int32), ("use_levelset", C.c_int32),
                 ("use_killing", C.c_int32), ("max_iters", C.c_int32), ("stop_max_update", C.c_double),
-                ("method", C.c_int32), ("sweep", C.c_int32)]
+                ("method", C.c_int32), ("sweep", C.c_int32), ("use_trust", C.c_int32), ("reserved", C.c_int32)]
 
 
 class DeformStats(C.Structure):

@@ -63,6 +63,7 @@ class KillingParams:
     stop_max_update: float = 0.0
     method: str = "reference"  # or "adjoint"
     sweep: str = "auto"  # "auto", "dense" (every voxel) or "active" (the voxels that can move): the same field
+    use_trust: bool = False  # UseTrustStrategy (config.cpp:27): reference method, data energy alone
 
     @classmethod
     def paper(cls, voxel_size: float, trunc_voxels: float = 3.0, **kw):

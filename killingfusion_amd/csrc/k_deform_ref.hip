@@ -61,6 +61,31 @@ struct PhiView {
         }
         return kf_interp3(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], fx, fy, fz);
     }
+    // SDF::getDistance(Vector3d), SDF.cpp:503-524, in the reference's own FP64 arithmetic at grid
+    // location loc: loc - 0.5, truncating casts, the lerps in double, taps past the grid reading
+    // oob64 (Max + eps in FP64).  A non-finite location reads NaN; one beyond 2^30 voxels reads
+    // only taps past the grid (the reference's cast is undefined there).
+    __device__ inline double distance64(const double loc[3], double oob64) const
+    {
+        double t[3];
+        int b[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            t[a] = loc[a] - 0.5;
+            if (!isfinite(t[a]))
+                return (double)NAN;
+            const double c = fmin(fmax(t[a], -1073741824.0), 1073741824.0);
+            b[a] = (int)c;
+        }
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int x = b[0] + (k & 1), y = b[1] + ((k >> 1) & 1), z = b[2] + (k >> 2);
+            v[k] = g.in(x, y, z) ? (double)S::load(p, g.idx(x, y, z), scale) : oob64;
+        }
+        return kf_interp3(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], t[0] - (double)b[0], t[1] - (double)b[1],
+                          t[2] - (double)b[2]);
+    }
 };
 
 // DisplacementField::getDisplacementAt, DisplacementField.cpp:30-38 (linear-index test only).
@@ -143,6 +168,9 @@ struct RefConsts {
     int use_data, use_ls, use_k;
     int nphi;   // SDF samples in use: 0, 6 (gradient) or 25 (gradient and Hessian)
     int nsamp;  // nphi + 19 when the Killing term is in use
+    int trust;  // the trust region (UseTrustStrategy with the data energy alone)
+    double alpha64;  // alpha as the trust region shrinks it (FP64, as the reference's _alpha)
+    double oob64;    // the live volume's Max + eps in FP64, for the trust region's reads
 };
 
 // Sample s of the active voxel (x, y, z) at the point i + r.  Every sample reads u
@@ -338,7 +366,8 @@ __global__ __launch_bounds__(BLOCK) void k_ref_finite(PhiView<S> L, unsigned lon
 // (KillingFusion.cpp:272-274), 4 voxels per thread (one float4 of each u plane; UNAL: N is not a
 // multiple of 4, so planes 1 and 2 are read as 4-byte-aligned 16-byte accesses and the group
 // that runs past N goes element by element).  A voxel
-// outside the band keeps u: its update is u + (-alpha * 0).  A voxel in the band is appended
+// outside the band takes no update: u plus the +0 its iteration's fresh delta field holds
+// (KillingFusion.cpp:254-256, 345-348), so a -0 component becomes +0.  A voxel in the band is appended
 // to its XCD's list segment for pass 2 (one atomic per block), which overwrites its entry of
 // u_out; the list's length is the launch's active count.
 // Where u_j is +0 or -0 the interpolation weights are +0 and the interpolant is
@@ -446,7 +475,7 @@ __global__ __launch_bounds__(SBLOCK) void k_ref_scan(PhiView<S> L, const float *
     // wait for these stores (one counter tracks both on this target) while the block's other
     // waves sit at the barrier.  Voxels in the band get their value in pass 2.
     if (nin > 0) {
-        const float d = -alpha * 0.0f;
+        const float d = 0.0f;  // the fresh delta field's +0: a -0 component becomes +0
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             float o[V];
@@ -536,7 +565,7 @@ __global__ __launch_bounds__(SBLOCK) void k_ref_active_scan(PhiView<S> L, const 
     }
     // field values last (see k_ref_scan); voxels in the band get theirs in pass 2
     if (have) {
-        const float d = -alpha * 0.0f;
+        const float d = 0.0f;  // the fresh delta field's +0: a -0 component becomes +0
 #pragma unroll
         for (int a = 0; a < 3; a++)
             u_out[a * g.n + j] = MODE == KF_MODE_GRAD ? 0.0f : u[a] + d;
@@ -560,11 +589,12 @@ __global__ __launch_bounds__(LBLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn
                                                     BlockPartial *partials, unsigned long long *aux, int parity,
                                                     unsigned long long segcap,
                                                     const unsigned long long *__restrict__ list_idx,
-                                                    const float *__restrict__ list_psi)
+                                                    float *__restrict__ list_psi, unsigned int *__restrict__ row_fail)
 {
     if (MODE == KF_MODE_STEP && st->done)
         return;
     __shared__ long long s_i[RK];
+    __shared__ unsigned long long s_q[RK];
     __shared__ int s_xyz[RK][3];
     __shared__ float s_u[RK][3];
     __shared__ float s_psi[RK], s_dst[RK];
@@ -591,6 +621,7 @@ __global__ __launch_bounds__(LBLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn
             int x, y, z;
             kf_xyz(g, j, x, y, z);
             s_i[t] = j;
+            s_q[t] = q;
             s_xyz[t][0] = x;
             s_xyz[t][1] = y;
             s_xyz[t][2] = z;
@@ -633,21 +664,126 @@ __global__ __launch_bounds__(LBLOCK) void k_ref_list(PhiView<S> L, PhiView<S> Cn
                 u_out[2 * g.n + j] = gr[2];
             } else {
                 float d0 = -c.alpha * gr[0], d1 = -c.alpha * gr[1], d2 = -c.alpha * gr[2];
+                bool fail = false;
+                if (c.trust) {
+                    // KillingFusion.cpp:288-314: shrink the step until the data residual at the
+                    // moved sample, |src - dest|, is no larger than at the current one.  The
+                    // comparison decides which voxels take a step and where a row stops, so it
+                    // runs in the reference's FP64 (positions, reads and residuals): in FP32 the
+                    // residual change of the smallest steps falls below one ulp and would read as
+                    // "no larger" where FP64 keeps shrinking.
+                    const int x = s_xyz[t][0], y = s_xyz[t][1], z = s_xyz[t][2];
+                    const double ux = s_u[t][0], uy = s_u[t][1], uz = s_u[t][2];
+                    const double dest = (double)s_dst[t];
+                    const double at_u[3] = {((double)x + ux) + 0.5, ((double)y + uy) + 0.5, ((double)z + uz) + 0.5};
+                    const double prev = L.distance64(at_u, c.oob64);
+                    const double g0 = gr[0], g1 = gr[1], g2 = gr[2];
+                    double a = c.alpha64;
+                    double e0 = -a * g0, e1 = -a * g1, e2 = -a * g2;
+                    for (;;) {
+                        const double loc[3] = {(((double)x + ux) + e0) + 0.5, (((double)y + uy) + e1) + 0.5,
+                                               (((double)z + uz) + e2) + 0.5};
+                        const double src = L.distance64(loc, c.oob64);
+                        if (!(fabs(src - dest) - fabs(prev - dest) > 0.0))
+                            break;
+                        a /= 1.5;
+                        e0 = -a * g0;
+                        e1 = -a * g1;
+                        e2 = -a * g2;
+                        if (!(a > 1e-7))
+                            break;
+                    }
+                    d0 = (float)e0;
+                    d1 = (float)e1;
+                    d2 = (float)e2;
+                    fail = a < 1e-7;
+                    if (fail) {
+                        // the reference's `break`: this voxel and the rest of its x-row take no step
+                        atomicMin(&row_fail[(size_t)parity * (size_t)g.ny * g.nz + (size_t)z * g.ny + y],
+                                  (unsigned int)x);
+                        d0 = d1 = d2 = 0.0f;
+                    }
+                }
                 float n0 = s_u[t][0] + d0, n1 = s_u[t][1] + d1, n2 = s_u[t][2] + d2;
                 u_out[j] = n0;
                 u_out[g.n + j] = n1;
                 u_out[2 * g.n + j] = n2;
                 double dn = sqrt((double)d0 * d0 + (double)d1 * d1 + (double)d2 * d2);
-                if (!(isfinite(n0) && isfinite(n1) && isfinite(n2)))
+                if (c.trust) {
+                    // the statistics and the divergence test wait for the row's first failure
+                    // (k_ref_trust_rows); the entry's warped value is used up
+                    list_psi[s_q[t]] = fail ? -1.0f : (float)dn;
+                } else if (!(isfinite(n0) && isfinite(n1) && isfinite(n2))) {
                     atomicMin((long long *)&st->div_slot[parity], (long long)j);
-                else if (dn > mx)
+                } else if (dn > mx) {
                     mx = dn;
+                }
             }
         }
         __syncthreads();
     }
-    if (MODE == KF_MODE_STEP)
+    if (MODE == KF_MODE_STEP && !c.trust)
         kf_block_publish<LBLOCK>(partials, blockIdx.x, 0.0, mx, blockIdx.x == 0 && threadIdx.x == 0 ? count : 0ull);
+}
+
+// Pass 3 of a trust-region step: the reference's `break` (KillingFusion.cpp:313) leaves the rest
+// of an x-row unvisited once a voxel's step fails, so every listed voxel past its row's first
+// failure (row_fail, set by pass 2) takes no step after all, u_out = u_in + the delta field's
+// +0.  The statistics are the reference loop's: the voxels it visited in the band (up to and
+// including each row's failure) and the largest step it kept; the divergence test covers the
+// kept steps.  The row failures of the other launch parity are cleared for the next launch.
+__global__ __launch_bounds__(LBLOCK) void k_ref_trust_rows(GridDims g, const float *__restrict__ u_in,
+                                                           float *__restrict__ u_out, DeformState *st,
+                                                           BlockPartial *partials, const unsigned long long *aux,
+                                                           int parity, unsigned long long segcap,
+                                                           const unsigned long long *__restrict__ list_idx,
+                                                           const float *__restrict__ list_dn,
+                                                           unsigned int *__restrict__ row_fail)
+{
+    if (st->done)
+        return;
+    const size_t rows = (size_t)g.ny * g.nz;
+    unsigned int *fail_now = row_fail + (size_t)parity * rows;
+    unsigned int *fail_next = row_fail + (size_t)(parity ^ 1) * rows;
+    for (size_t r = (size_t)blockIdx.x * LBLOCK + threadIdx.x; r < rows; r += (size_t)gridDim.x * LBLOCK)
+        fail_next[r] = 0xffffffffu;
+    unsigned long long pre[NSEG + 1];
+    pre[0] = 0ull;
+#pragma unroll
+    for (int k = 0; k < NSEG; k++)
+        pre[k + 1] = pre[k] + aux[parity * NSEG + k];
+    const unsigned long long count = pre[NSEG];
+    double mx = 0.0;
+    unsigned long long act = 0;
+    for (unsigned long long p = (unsigned long long)blockIdx.x * LBLOCK + threadIdx.x; p < count;
+         p += (unsigned long long)gridDim.x * LBLOCK) {
+        const unsigned long long q = ref_slot(pre, segcap, p);
+        const long long j = (long long)list_idx[q];
+        int x, y, z;
+        kf_xyz(g, j, x, y, z);
+        const unsigned int f = fail_now[(size_t)z * g.ny + y];
+        if ((unsigned int)x > f) {
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                u_out[a * g.n + j] = u_in[a * g.n + j] + 0.0f;
+            continue;
+        }
+        act += 1;
+        if ((unsigned int)x == f)
+            continue;  // the failing voxel: no step (pass 2 wrote u + 0)
+        const float n0 = u_out[j], n1 = u_out[g.n + j], n2 = u_out[2 * g.n + j];
+        if (!(isfinite(n0) && isfinite(n1) && isfinite(n2)))
+            atomicMin((long long *)&st->div_slot[parity], (long long)j);
+        else if ((double)list_dn[q] > mx)
+            mx = (double)list_dn[q];
+    }
+    kf_block_publish<LBLOCK>(partials, blockIdx.x, 0.0, mx, act);
+}
+
+__global__ void k_ref_rows_clear(unsigned int *row_fail, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        row_fail[i] = 0xffffffffu;
 }
 
 template <class S>
@@ -672,6 +808,8 @@ void kf_ref_release(kf_context *ctx)
         (void)hipFree(w.psi);
     if (w.aux)
         (void)hipFree(w.aux);
+    if (w.row_fail)
+        (void)hipFree(w.row_fail);
     w = kf_ref_ws();
 }
 
@@ -697,6 +835,10 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
     c.use_k = p->use_killing;
     c.nphi = c.use_ls ? NPHI : (c.use_data ? 6 : 0);
     c.nsamp = c.nphi + (c.use_k ? NKIL : 0);
+    // KillingFusion.cpp:288: the trust region only with the data energy alone
+    c.trust = mode != KF_MODE_GRAD && p->use_trust && c.use_data && !c.use_ls && !c.use_k;
+    c.alpha64 = p->alpha;
+    c.oob64 = live->oob();
     const int64_t n = live->n;
     const bool vec = n % 4 == 0;
     const bool nbhd = c.h > 0.0f && c.h < 0.25f;
@@ -726,6 +868,21 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
     const unsigned int nb_v = vlist ? (vslots + SBLOCK - 1) / SBLOCK : 0u;
     const unsigned long long segcap_v = (unsigned long long)((nb_v + NSEG - 1) / NSEG) * SBLOCK;
     KF_CHECK_ARG(!vlist || (mode != KF_MODE_GRAD && segcap_v * NSEG <= w.cap), "active list larger than the workspace");
+    // the first failing x of every (y, z) row, one set per launch parity
+    const size_t rows = (size_t)live->dims[1] * (size_t)live->dims[2];
+    if (c.trust && w.rows_cap < 2 * rows) {
+        KF_HIP(hipStreamSynchronize(ctx->stream));
+        if (w.row_fail)
+            (void)hipFree(w.row_fail);
+        w.row_fail = nullptr;
+        w.rows_cap = 0;
+        if (hipMalloc((void **)&w.row_fail, sizeof(unsigned int) * 2 * rows) != hipSuccess) {
+            (void)hipGetLastError();
+            kf_set_error("trust-region row table allocation failed");
+            return KF_E_OOM;
+        }
+        w.rows_cap = 2 * rows;
+    }
     kf_status s = KF_OK;
     const int64_t chunks = (n + RK - 1) / RK;
     const unsigned int nb_b = (unsigned int)(chunks < LIST_BLOCKS ? chunks : LIST_BLOCKS);
@@ -738,6 +895,8 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
             // A deform call (or a gradient evaluation) starts with empty lists and checks the live
             // SDF once: it does not change while u does.
             hipLaunchKernelGGL(k_ref_clear, dim3(1), dim3(64), 0, ctx->stream, w.aux);
+            if (c.trust)
+                hipLaunchKernelGGL(k_ref_rows_clear, dim3(64), dim3(256), 0, ctx->stream, w.row_fail, 2 * rows);
             const unsigned int fb = nb_a < 4096u ? nb_a : 4096u;
             if (vec)
                 hipLaunchKernelGGL((k_ref_finite<S, 4>), dim3(fb), dim3(BLOCK), 0, ctx->stream, L, w.aux + AUX_NONFINITE);
@@ -753,10 +912,10 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
                                    u_in, u_out, c.hi, c.lo, c.alpha, ctx->state, w.aux, parity, segcap, w.idx, w.psi);
             if (nbhd)
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD, true>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
-                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
+                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi, w.row_fail);
             else
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_GRAD, false>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
-                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi);
+                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, segcap, w.idx, w.psi, w.row_fail);
         } else {
             const unsigned long long sc = vlist ? segcap_v : segcap;
             if (vlist) {
@@ -773,12 +932,19 @@ kf_status kf_launch_ref_step(const kf_volume *live, const kf_volume *can, const 
             }
             if (nbhd)
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, true>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
-                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, sc, w.idx, w.psi);
+                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, sc, w.idx, w.psi, w.row_fail);
             else
                 hipLaunchKernelGGL((k_ref_list<S, KF_MODE_STEP, false>), dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, L,
-                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, sc, w.idx, w.psi);
+                                   Cn, u_in, u_out, c, ctx->state, parts, w.aux, parity, sc, w.idx, w.psi, w.row_fail);
         }
     });
     KF_LAUNCH_CHECK();
+    if (c.trust) {
+        const unsigned long long sc = vlist ? segcap_v : segcap;
+        hipLaunchKernelGGL(k_ref_trust_rows, dim3(nb_b), dim3(LBLOCK), 0, ctx->stream, grid_of(live), u_in, u_out,
+                           ctx->state, parts, (const unsigned long long *)w.aux, parity, sc,
+                           (const unsigned long long *)w.idx, (const float *)w.psi, w.row_fail);
+        KF_LAUNCH_CHECK();
+    }
     return mode == KF_MODE_GRAD ? KF_OK : kf_launch_finalize(ctx, nb_b, parity);
 }

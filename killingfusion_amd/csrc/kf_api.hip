@@ -52,9 +52,12 @@ bool valid_dims(const int32_t d[3])
 
 bool valid_params_impl(const kf_params *p)
 {
+    // the trust region is the reference formula's (KillingFusion.cpp:288-314); the adjoint
+    // method has no counterpart of it
     return p && p->voxel_size > 0.0 && p->max_iters >= 0 && p->delta_size > 0.0 &&
            (p->method == KF_METHOD_REFERENCE || p->method == KF_METHOD_ADJOINT) &&
-           (p->sweep == KF_SWEEP_AUTO || p->sweep == KF_SWEEP_DENSE || p->sweep == KF_SWEEP_ACTIVE);
+           (p->sweep == KF_SWEEP_AUTO || p->sweep == KF_SWEEP_DENSE || p->sweep == KF_SWEEP_ACTIVE) &&
+           (p->use_trust == 0 || (p->use_trust == 1 && p->method == KF_METHOD_REFERENCE)) && p->reserved == 0;
 }
 
 // Gauss-Jordan inverse with partial pivoting (row-major 4x4).
@@ -225,6 +228,7 @@ kf_status kf_params_default(kf_params *p)
     p->max_iters = 1024;                   // config.cpp:42
     p->stop_max_update = 0.0;
     p->method = KF_METHOD_REFERENCE;
+    p->use_trust = 0;                      // config.cpp:27
     return KF_OK;
 }
 

@@ -154,6 +154,8 @@ struct kf_ref_ws {
     float *psi = nullptr;               // their warped live SDF value
     size_t cap = 0;                     // entries of idx and psi
     unsigned long long *aux = nullptr;  // segment lengths of launch parity 0 and 1, then the non-finite flag
+    unsigned int *row_fail = nullptr;   // trust region: first failing x of each (y, z) row, per parity
+    size_t rows_cap = 0;                // entries of row_fail
 };
 
 // Marching-cubes block offsets of the last volume counted (k_mc.hip): a size query followed by

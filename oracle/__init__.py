@@ -36,7 +36,7 @@ class Params(C.Structure):
         ("gamma", C.c_double), ("omega_s", C.c_double), ("h", C.c_double),
         ("killing_clamp", C.c_double), ("use_data", C.c_int32), ("use_levelset", C.c_int32),
         ("use_killing", C.c_int32), ("max_iters", C.c_int32), ("stop_max_update", C.c_double),
-        ("gs_threshold", C.c_double),
+        ("gs_threshold", C.c_double), ("use_trust", C.c_int32), ("pad", C.c_int32),
     ]
 
 
@@ -159,6 +159,8 @@ class OParams:
     max_iters: int = 1024
     stop_max_update: float = 0.0
     gs_threshold: float = 1e-6
+    use_trust: bool = False
+    pad: int = 0
 
     def c(self) -> Params:
         p = Params()

@@ -473,6 +473,37 @@ void kfo_energy_gradient_grid(const double *live, const double *can, const doubl
  * stop rule (max |du| < stop_max_update) is not in the reference loop, which
  * always runs max_iters; it is off when stop_max_update <= 0.  Returns 0, or 1
  * when a non-finite displacement appears (st->diverged_index = lowest index). */
+/* Trust region of the all-voxels mode, KillingFusion.cpp:288-314 (UseTrustStrategy with the data
+ * energy alone): du = -alpha g is shrunk (alpha / 1.5 each time) until the data residual at the
+ * moved sample, |src - dest|, is no larger than at the current one, |prev - dest|; src is read at
+ * ((j + u_j) + du) + 0.5 through SDF::getDistance(Vector3d) (SDF.cpp:503-524).  Returns 0 when
+ * alpha fell below 1e-7: the voxel takes no step and the reference's `break` leaves its x loop,
+ * so the rest of the row takes none either in this iteration. */
+static int trust_step(const double *live, const double *can, const double *u, const kfo_grid *g,
+                      const kfo_params *p, int x, int y, int z, const double gr[3], double du[3])
+{
+    double a = p->alpha;
+    const double dest = kfo_distance_at_index(can, g, x, y, z);
+    const double prev = kfo_distance_idx_disp(live, u, g, x, y, z);
+    double uj[3];
+    kfo_disp_at(u, g, x, y, z, uj);
+    int decreased = 0;
+    do {
+        const double loc[3] = {((double)x + uj[0] + du[0]) + 0.5, ((double)y + uj[1] + du[1]) + 0.5,
+                               ((double)z + uj[2] + du[2]) + 0.5};
+        const double src = kfo_get_distance(live, g, loc);
+        const double conv = fabs(src - dest) - fabs(prev - dest);
+        if (conv > 0) {
+            a /= 1.5;
+            for (int k = 0; k < 3; k++)
+                du[k] = -a * gr[k];
+        } else {
+            decreased = 1;
+        }
+    } while (!decreased && a > 1e-7);
+    return !(a < 1e-7);
+}
+
 int kfo_deform_jacobi(const double *live, const double *can, double *u, const kfo_grid *g,
                       const kfo_params *p, kfo_stats *st)
 {
@@ -485,6 +516,7 @@ int kfo_deform_jacobi(const double *live, const double *can, double *u, const kf
     st->max_update = 0.0;
     st->energy = 0.0;
     st->active = 0;
+    const int trust = p->use_trust && p->use_data && !p->use_levelset && !p->use_killing;
     for (int it = 0; it < p->max_iters; it++) {
         memset(delta, 0, sizeof(double) * 3 * (size_t)n);
 #pragma omp parallel for schedule(dynamic)
@@ -501,6 +533,8 @@ int kfo_deform_jacobi(const double *live, const double *can, double *u, const kf
                     kfo_energy_gradient(live, can, u, g, p, x, y, z, gr);
                     int64_t i = IDX(g, x, y, z);
                     double du[3] = {-p->alpha * gr[0], -p->alpha * gr[1], -p->alpha * gr[2]};
+                    if (trust && !trust_step(live, can, u, g, p, x, y, z, gr, du))
+                        break;
                     delta[i] = du[0];
                     delta[n + i] = du[1];
                     delta[2 * n + i] = du[2];

@@ -72,6 +72,8 @@ typedef struct kfo_params {
     int32_t max_iters;                             /* KILLING_MAX_ITERATIONS (config.cpp:42) */
     double stop_max_update;  /* stop once max |du| < this (voxel); <= 0 disables */
     double gs_threshold;     /* `threshold` of the per-voxel loop (config.cpp:43) */
+    int32_t use_trust;       /* UseTrustStrategy (config.cpp:27), honoured with the data energy alone */
+    int32_t pad;
 } kfo_params;
 
 typedef struct kfo_stats {

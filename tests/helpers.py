@@ -28,7 +28,8 @@ def oparams(kp) -> O.OParams:
                      epsilon=kp.epsilon, alpha=kp.alpha, omega_k=kp.omega_killing, gamma=kp.gamma_killing,
                      omega_s=kp.omega_levelset, h=kp.delta_size, killing_clamp=kp.killing_clamp,
                      use_data=kp.use_data, use_levelset=kp.use_levelset, use_killing=kp.use_killing,
-                     max_iters=kp.max_iters, stop_max_update=kp.stop_max_update)
+                     max_iters=kp.max_iters, stop_max_update=kp.stop_max_update,
+                     use_trust=getattr(kp, "use_trust", False))
 
 
 def oracle_integrate(g, trunc, depth):

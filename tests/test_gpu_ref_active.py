@@ -3,7 +3,8 @@ method) against its dense sweep, through the C ABI.
 
 The reference's Jacobi step (KillingFusion.cpp:248-351) updates voxel j only when the warped
 live SDF at j + u_j is in the band (:272-274), and with u_j = 0 and every live value finite
-that value is phi_live[j]; a skipped voxel keeps u_j + (-alpha 0).  So the voxels with
+that value is phi_live[j]; a skipped voxel takes u_j plus the +0 of its iteration's fresh delta
+field (:254-256, 345-348), so it keeps u_j except that a -0 component becomes +0.  So the voxels with
 u_j = +0 and phi_live[j] outside the band never move, and the active-set sweep visits only the
 others, in the field's own two buffers.  (With a non-finite live value the warped value next
 to it is NaN, which passes the band test; the sweep then steps densely.)  Both sweeps run the
@@ -170,3 +171,26 @@ def test_reference_list_longer_than_predicted(ctx, sphere):
     assert long_a[1]["listed"] > 2 * short[1]["listed"] + 4096
     same(run(ctx, live, can, kp, "dense", box), long_a, "reference, long list after a short one")
     same(run(ctx, live, can, kp, "dense"), run(ctx, live, can, kp, "active"), "reference, short list again")
+
+
+def test_skipped_voxels_turn_minus_zero_into_plus_zero(ctx, sphere):
+    """The reference adds each iteration's delta field, created as zeros (+0), to u
+    (KillingFusion.cpp:254-256, 345-348); a voxel outside the band gets no update, so its -0
+    components become +0 after one iteration, as in the oracle.  Both sweeps, bit for bit."""
+    import oracle as O
+    from helpers import ogrid, oparams
+    live, can = sphere["f32"]
+    phi, _ = live.download()
+    cphi, _ = can.download()
+    far = np.nonzero(np.abs(phi) > 2 * C1["trunc"])[0][::97][:64]
+    u0 = np.zeros((3, live.n), dtype=np.float32)
+    u0[:, far] = -0.0
+    kp = KillingParams.paper(C1["voxel"], 3.0, max_iters=1)
+    g = ogrid(C1["dims"], C1["min3d"], C1["voxel"], C1["trunc"])
+    uo, _ = O.deform_jacobi(phi.astype(np.float64), cphi.astype(np.float64), u0.astype(np.float64), g, oparams(kp))
+    assert not np.signbit(uo[:, far]).any()
+    for sweep in ("dense", "active"):
+        print(f"case: reference method, -0 start components outside the band, {sweep}", flush=True)
+        ug, st = run(ctx, live, can, kp, sweep, u0)
+        assert st["sweep"] == sweep
+        assert (ug[:, far].view(np.uint32) == 0).all(), sweep

@@ -6,6 +6,8 @@ CPU only.
 """
 import math
 
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -352,6 +354,108 @@ def test_deform_jacobi_matches_naive_restatement():
     un, st = O.deform_jacobi(live, can, u, g, p)
     ref = _naive_jacobi(live, can, u, n, p)
     assert np.allclose(un, ref, atol=1e-12, rtol=1e-10)
+
+
+def test_trust_region_matches_naive_restatement():
+    # The trust-region step of the all-voxels mode (KillingFusion.cpp:288-314, UseTrustStrategy
+    # with the data energy alone): one iteration of the C oracle equals an independent Python
+    # transcription, including the reference's `break`, which leaves the rest of an x-row
+    # without a step once a voxel's step size falls below 1e-7.  The random fields make some
+    # voxels shrink their step and some rows break.
+    n = 6
+    rng = np.random.default_rng(21)
+    g = O.OGrid(dims=(n, n, n), voxel=0.01, oob=0.03 + 1e-5)
+    p = O.OParams(voxel_size=0.01, max_surface=0.03, clip=0.03, alpha=0.1, use_levelset=False, use_killing=False,
+                  use_trust=True, max_iters=1)
+    live = rng.uniform(-0.03, 0.03, n ** 3)
+    can = rng.uniform(-0.03, 0.03, n ** 3)
+    u = rng.uniform(-0.4, 0.4, (3, n ** 3))
+    un, st = O.deform_jacobi(live, can, u, g, p)
+    ref, shrunk, breaks, skipped = _naive_trust(live, can, u, n, p)
+    assert np.array_equal(un, ref)
+    assert shrunk > 0 and breaks > 0 and skipped > 0, (shrunk, breaks, skipped)
+    # without the trust region (or with another energy on) the step is the plain one
+    plain, _ = O.deform_jacobi(live, can, u, g, dataclasses.replace(p, use_trust=False))
+    assert not np.array_equal(plain, ref)
+    other, _ = O.deform_jacobi(live, can, u, g, dataclasses.replace(p, use_levelset=True))
+    plain_ls, _ = O.deform_jacobi(live, can, u, g, dataclasses.replace(p, use_levelset=True, use_trust=False))
+    assert np.array_equal(other, plain_ls)
+
+
+def _naive_trust(live, can, u, n, p):
+    """One all-voxels iteration with the data energy alone and the trust region, transcribed
+    from KillingFusion.cpp:251-319 and SDF.cpp:503-524 (trilinear read with truncating casts);
+    returns the field and the counts of shrunk steps, row breaks and voxels they skipped."""
+    N = n ** 3
+    oob = p.max_surface + p.epsilon
+
+    def phi_at(arr, x, y, z):
+        if 0 <= x < n and 0 <= y < n and 0 <= z < n:
+            return arr[z * n * n + y * n + x]
+        return oob
+
+    def getdist(arr, l):
+        t = [c - 0.5 for c in l]
+        b = [int(c) for c in t]
+        f = [t[k] - b[k] for k in range(3)]
+        v = {(dx, dy, dz): phi_at(arr, b[0] + dx, b[1] + dy, b[2] + dz)
+             for dz in (0, 1) for dy in (0, 1) for dx in (0, 1)}
+        L = lambda a, c, x: a * (1 - x) + c * x
+        s = L(L(v[0, 0, 0], v[1, 0, 0], f[0]), L(v[0, 1, 0], v[1, 1, 0], f[0]), f[1])
+        t2 = L(L(v[0, 0, 1], v[1, 0, 1], f[0]), L(v[0, 1, 1], v[1, 1, 1], f[0]), f[1])
+        return L(s, t2, f[2])
+
+    def dat(x, y, z):
+        i = z * n * n + y * n + x
+        return u[:, i] if 0 <= i < N else np.zeros(3)
+
+    def dispf(l):
+        b = [int(c) for c in l]
+        f = [l[k] - b[k] for k in range(3)]
+        L = lambda a, c, x: a * (1 - x) + c * x
+        c = {(dx, dy, dz): dat(b[0] + dx, b[1] + dy, b[2] + dz) for dx in (0, 1) for dy in (0, 1) for dz in (0, 1)}
+        s = L(L(c[0, 0, 0], c[1, 0, 0], f[0]), L(c[0, 1, 0], c[1, 1, 0], f[0]), f[1])
+        t2 = L(L(c[0, 0, 1], c[1, 0, 1], f[0]), L(c[0, 1, 1], c[1, 1, 1], f[0]), f[1])
+        return L(s, t2, f[2])
+
+    def distf(l):
+        d = dispf(l)
+        return getdist(live, [l[k] + d[k] + 0.5 for k in range(3)])
+
+    h = p.h
+    out = u.copy()
+    shrunk = breaks = skipped = 0
+    E = np.eye(3)
+    for z in range(n):
+        for y in range(n):
+            for x in range(n):
+                i = z * n * n + y * n + x
+                psi = getdist(live, [x + u[0, i] + 0.5, y + u[1, i] + 0.5, z + u[2, i] + 0.5])
+                if psi > p.max_surface - p.epsilon or psi < -p.clip:
+                    continue
+                gl = np.array([x + 0.5, y + 0.5, z + 0.5])
+                gr = np.array([distf(gl + h * E[a]) - distf(gl - h * E[a]) for a in range(3)]) / (2 * h)
+                dg = (psi - can[i]) / p.voxel_size * gr
+                a = p.alpha
+                du = -a * dg
+                dest, prev = can[i], psi
+                while True:
+                    src = getdist(live, [x + u[0, i] + du[0] + 0.5, y + u[1, i] + du[1] + 0.5,
+                                         z + u[2, i] + du[2] + 0.5])
+                    if abs(src - dest) - abs(prev - dest) > 0:
+                        a /= 1.5
+                        du = -a * dg
+                        shrunk += 1
+                        if not a > 1e-7:
+                            break
+                    else:
+                        break
+                if a < 1e-7:
+                    breaks += 1
+                    skipped += n - 1 - x
+                    break
+                out[:, i] = u[:, i] + du
+    return out, shrunk, breaks, skipped
 
 
 def _naive_jacobi(live, can, u, n, p):
