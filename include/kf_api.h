@@ -228,8 +228,8 @@ kf_status kf_marching_cubes_indexed(const kf_volume *vol, double iso, uint64_t *
                                     float *verts, float *normals, uint32_t *tris, uint64_t cap_verts,
                                     uint64_t cap_tris);
 
-/* Largest grid (voxels) kf_marching_cubes_noalloc takes: its scratch holds the block totals,
- * offsets and list of blocks with triangles of 2^27 voxels (1 MiB per GPU). */
+/* Largest grid (voxels) kf_marching_cubes_noalloc takes: its scratch holds the unit and block
+ * totals, offsets and list of blocks with triangles of 2^27 voxels (2 MiB per GPU). */
 #define KF_MC_NOALLOC_MAX_VOXELS (1LL << 27)
 /* kf_marching_cubes into caller device memory, allocating nothing.  No counterpart in the
  * reference: for callers that cannot allocate (code under HIP graph capture, serving loops,
@@ -237,7 +237,7 @@ kf_status kf_marching_cubes_indexed(const kf_volume *vol, double iso, uint64_t *
  * without waiting: *count (device memory, one uint64) receives the triangle count, verts (device
  * memory) the first min(count, cap_tris) triangles in kf_marching_cubes's order and layout;
  * cap_tris = 0 (verts may then be NULL) only counts.  Grids above KF_MC_NOALLOC_MAX_VOXELS are
- * rejected (KF_E_INVALID_ARG).  The block totals, offsets and list live in one device array per GPU,
+ * rejected (KF_E_INVALID_ARG).  The totals, offsets and list live in one device array per GPU,
  * held until the process ends, that every call on that GPU shares: calls that overlap on one GPU
  * (other streams or threads) race and must be ordered by the caller. */
 kf_status kf_marching_cubes_noalloc(const kf_volume *vol, double iso, float *verts, uint64_t cap_tris,

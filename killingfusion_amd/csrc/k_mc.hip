@@ -6,12 +6,13 @@
 // vertices with y negated (SimpleMesh.h:66-80).  The GPU keeps that order with an ordered
 // compaction instead of atomics:
 //   1. count: a thread classifies CPT consecutive cells (the node values it needs come in as
-//      vector loads, all issued before any is used), a block of 256 threads owns CELLS
-//      consecutive cells and writes their triangle count;
-//   2. scan: one block turns the block counts into exclusive offsets and the total;
-//   3. emit: each block classifies its cells again, prefix-sums the thread counts over the
-//      block and writes every triangle at its final position; only the cells that hold
-//      triangles gather their eight corners and interpolate.
+//      vector loads, all issued before any is used); a wavefront owns UNIT consecutive cells,
+//      a workgroup CELLS, and each writes its triangle count;
+//   2. scan: one block turns the workgroup counts into exclusive offsets and the total, and
+//      lists the workgroups whose cells hold triangles;
+//   3. emit: a workgroup per unit of a listed block classifies the unit's cells again,
+//      prefix-sums the thread counts and writes every triangle at its final position; only
+//      the cells that hold triangles interpolate.
 // The counts and offsets of the last volume counted are kept (kf_mc_cache) until the volume
 // changes, so the reference-style size query followed by the extraction counts once.
 // Corner values are compared and interpolated in FP64 like the reference (built with
@@ -21,14 +22,17 @@
 #include "kf_mc_cases.h"
 
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 namespace {
 
 constexpr int BLOCK = 256;
 constexpr int CPT = 8;               // cells per thread (a multiple of 4)
-constexpr int CELLS = BLOCK * CPT;   // cells per block of the count and emit passes
-static_assert(CELLS == KF_MC_CELLS, "block size of the count and emit passes");
+constexpr int CELLS = BLOCK * CPT;   // cells per workgroup of the count pass
+constexpr int UNIT = 64 * CPT;       // cells per wavefront of the count pass: one unit
+constexpr int UNITS_PER_BLOCK = CELLS / UNIT;
+static_assert(CELLS == KF_MC_CELLS && UNIT == KF_MC_UNIT, "cells per count workgroup and per unit");
 constexpr int ITEMS = 16;
 constexpr int FLAG_CHUNK = BLOCK * ITEMS;  // edge flags per block of the indexed variant's scan
 
@@ -72,21 +76,63 @@ __device__ inline unsigned int block_scan_incl(unsigned int v, unsigned int *s, 
     return wbase + v;
 }
 
-// Sum of v over the block, valid in thread 0.
-__device__ inline unsigned int block_sum(unsigned int v, unsigned int *s)
+// Stored SDF values as raw floats: the value before the decode scale (F32 the value itself,
+// BF16 its bits as the high half of a float, F16 converted), so that decode = raw * scale for
+// BF16 and F16 and decode = raw for F32.
+template <class S>
+__device__ inline float raw_lo(unsigned int w)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63) == 0)
-        s[threadIdx.x >> 6] = v;
-    __syncthreads();
-    unsigned int t = 0;
-#pragma unroll
-    for (int q = 0; q < BLOCK / 64; q++)
-        t += s[q];
-    return t;
+    if constexpr (S::dtype == KF_BF16)
+        return __uint_as_float(w << 16);
+    else
+        return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu));
 }
+template <class S>
+__device__ inline float raw_hi(unsigned int w)
+{
+    if constexpr (S::dtype == KF_BF16)
+        return __uint_as_float(w & 0xffff0000u);
+    else
+        return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16));
+}
+template <class S>
+__device__ inline float raw1(const typename S::T *p, int64_t i)
+{
+    if constexpr (S::dtype == KF_F32)
+        return p[i];
+    else if constexpr (S::dtype == KF_BF16)
+        return __uint_as_float((unsigned int)p[i] << 16);
+    else
+        return (float)p[i];
+}
+// Eight consecutive raw values from a 16-byte aligned p: one 16-byte load (two for F32).
+template <class S>
+__device__ inline void raw8(const typename S::T *p, float o[8])
+{
+    if constexpr (S::dtype == KF_F32) {
+        const float4 a = reinterpret_cast<const float4 *>(p)[0], b = reinterpret_cast<const float4 *>(p)[1];
+        o[0] = a.x;
+        o[1] = a.y;
+        o[2] = a.z;
+        o[3] = a.w;
+        o[4] = b.x;
+        o[5] = b.y;
+        o[6] = b.z;
+        o[7] = b.w;
+    } else {
+        const uint4 q = *reinterpret_cast<const uint4 *>(p);
+        const unsigned int w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            o[2 * k] = raw_lo<S>(w[k]);
+            o[2 * k + 1] = raw_hi<S>(w[k]);
+        }
+    }
+}
+
+// Corner k of a cell in the node-major bit order of Cells::count8: bit r + 4 dx for the node
+// at x + dx in node row r = dy + 2 dz (corners as in kCornerX/Y/Z).
+constexpr int kNodeBit[8] = {4, 0, 1, 5, 6, 2, 3, 7};
 
 template <class S>
 struct Cells {
@@ -139,6 +185,62 @@ struct Cells {
     __device__ inline int case_of(float d, int m) const
     {
         return (d > max_dn || d < clip_up || m == 0 || m == 255) ? -1 : m;
+    }
+    // The same tests on raw values (see raw1): decode is monotone in the raw value, so for a
+    // raw r, decode(r) < iso_up exactly when r < r_iso (the least raw value that decodes to
+    // >= iso_up), decode(r) < clip_up exactly when r < r_clip, and decode(r) > max_dn exactly
+    // when r > r_max (the greatest raw value that decodes to <= max_dn); the host finds the
+    // three thresholds by bisection over the floats.  NaN fails every comparison either way.
+    // oob_bit is the below-iso bit of a node outside the grid (oob_f < iso_up).
+    float r_iso, r_clip, r_max;
+    unsigned int oob_bit;
+
+    // Count pass of a grid whose rows are a multiple of 8 nodes long: the 8 cells i0 .. i0 + 7
+    // (i0 a multiple of 8, i0 < n) lie in one row, and each of their four node rows starts
+    // 16-byte aligned.  load8 issues the loads (a row past the grid is read from the last
+    // row, and node 8 past the row's end from node 7, their values then replaced);
+    // count8 packs the below-iso bits node-major, bit r + 4 j for node j of row r, so that
+    // cell c's eight corners are the byte at bit 4 c, and sums the triangle counts that the
+    // table sc holds for that byte (kNodeBit order).
+    __device__ inline void load8(int64_t i0, float v[4][8], float e[4], unsigned int &rows_out, bool &xl) const
+    {
+        int x0, y0, z0;
+        kf_xyz(g, i0, x0, y0, z0);
+        xl = x0 + 8 < g.nx;
+        rows_out = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int y = y0 + (r & 1), z = z0 + (r >> 1);
+            rows_out |= y < g.ny && z < g.nz ? 0u : 1u << r;
+            const int64_t b = g.idx(x0, y < g.ny ? y : g.ny - 1, z < g.nz ? z : g.nz - 1);
+            raw8<S>(phi + b, v[r]);
+            e[r] = raw1<S>(phi, xl ? b + 8 : b + 7);
+        }
+    }
+    __device__ inline unsigned int count8(const float v[4][8], const float e[4], unsigned int rows_out, bool xl,
+                                          const unsigned char *sc) const
+    {
+        unsigned int w = 0, w8 = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                w |= v[r][j] < r_iso ? 1u << (4 * j + r) : 0u;
+            w8 |= e[r] < r_iso ? 1u << r : 0u;
+        }
+        // nodes past the grid read oob: whole rows, and node 8 past the row's end
+        const unsigned int mw = rows_out * 0x11111111u, m8 = xl ? rows_out : 0xfu;
+        w = oob_bit ? w | mw : w & ~mw;
+        w8 = oob_bit ? w8 | m8 : w8 & ~m8;
+        unsigned int cnt = 0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const unsigned int m = c < 7 ? (w >> (4 * c)) & 0xffu : (w >> 28) | (w8 << 4);
+            const float d = v[0][c];  // corner 1, node (x0 + c, y0, z0), inside the grid
+            const unsigned int t = sc[m];
+            cnt += d > r_max || d < r_clip ? 0u : t;
+        }
+        return cnt;
     }
     // Cube indices (-1: skipped, no triangles or past the grid) and coordinates of the N
     // cells i0 .. i0 + N - 1.  When they lie in one row, the 4 x (N + 1) node values come from
@@ -237,26 +339,59 @@ struct Cells {
     }
 };
 
-// Triangles per block of CELLS cells.
+// Triangles per unit of UNIT cells (a wavefront's: a wavefront sum) and per workgroup of
+// CELLS cells (its UNITS_PER_BLOCK unit totals summed through LDS).
 static_assert(BLOCK == 256, "one thread per case loads the triangle-count table");
 
-template <class S>
-__global__ __launch_bounds__(BLOCK) void k_mc_count(Cells<S> C, unsigned int *__restrict__ block_tot)
+// ROWS8: the grid's rows are a multiple of 8 nodes long (Cells::load8 / count8); otherwise
+// the general classification (Cells::quad).
+template <class S, bool ROWS8>
+__global__ __launch_bounds__(BLOCK) void k_mc_count(Cells<S> C, unsigned int *__restrict__ unit_tot,
+                                                    unsigned int *__restrict__ block_tot)
 {
-    __shared__ unsigned int s[BLOCK / 64];
+    static_assert(CPT == 8, "one load8 group of cells per thread");
     __shared__ unsigned char s_cnt[256];
-    s_cnt[threadIdx.x] = c_tri_count[threadIdx.x];
+    __shared__ unsigned int s_w[UNITS_PER_BLOCK];
     const int64_t i0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * CPT;
-    int ci[CPT], x[CPT], y[CPT], z[CPT];
-    C.template quad<CPT>(i0, ci, x, y, z);
-    __syncthreads();
     unsigned int cnt = 0;
+    if constexpr (ROWS8) {
+        // the count table indexed by node-major corner bits
+        unsigned int ci = 0;
 #pragma unroll
-    for (int c = 0; c < CPT; c++)
-        cnt += ci[c] >= 0 ? s_cnt[ci[c]] : 0u;
-    const unsigned int t = block_sum(cnt, s);
-    if (threadIdx.x == 0)
+        for (int k = 0; k < 8; k++)
+            ci |= ((threadIdx.x >> kNodeBit[k]) & 1u) << k;
+        s_cnt[threadIdx.x] = c_tri_count[ci];
+        float v[4][8], e[4];
+        unsigned int rows_out = 0;
+        bool xl = false;
+        const bool live = i0 < C.g.n;
+        C.load8(live ? i0 : 0, v, e, rows_out, xl);
+        __syncthreads();
+        cnt = live ? C.count8(v, e, rows_out, xl, s_cnt) : 0u;
+    } else {
+        s_cnt[threadIdx.x] = c_tri_count[threadIdx.x];
+        int ci[CPT], x[CPT], y[CPT], z[CPT];
+        C.template quad<CPT>(i0, ci, x, y, z);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+            cnt += ci[c] >= 0 ? s_cnt[ci[c]] : 0u;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        cnt += __shfl_xor(cnt, o, 64);
+    if ((threadIdx.x & 63) == 0) {
+        unit_tot[(int64_t)blockIdx.x * UNITS_PER_BLOCK + (threadIdx.x >> 6)] = cnt;
+        s_w[threadIdx.x >> 6] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int t = 0;
+#pragma unroll
+        for (int u = 0; u < UNITS_PER_BLOCK; u++)
+            t += s_w[u];
         block_tot[blockIdx.x] = t;
+    }
 }
 
 // Exclusive scan of nb block totals in one block of 1024 threads (each sums a run of
@@ -353,11 +488,11 @@ __global__ __launch_bounds__(1024) void k_mc_scan(const unsigned int *__restrict
     }
 }
 
-// Threads per block of the emit pass: the CELLS cells of a count block, ECPT per thread, so
-// that a surface-dense block still has a thread for nearly every cell that emits.
-constexpr int EB = 1024;
-constexpr int ECPT = CELLS / EB;
-static_assert(ECPT * EB == CELLS, "emit blocks cover the count blocks");
+// Threads per workgroup of the emit pass: the UNIT cells of a unit, ECPT per thread, so that
+// a surface-dense unit still has a thread for nearly every cell that emits.
+constexpr int EB = 512;
+constexpr int ECPT = UNIT / EB;
+static_assert(ECPT * EB == UNIT, "emit workgroups cover the units");
 
 // The triangles of cell (x, y, z), case ci (Polygonise, MarchingCubes.h:411-419), from
 // position t0 on, y negated (SimpleMesh.h:66-72); triangles at or past cap are dropped.  Each
@@ -421,45 +556,52 @@ __device__ __forceinline__ void emit_cell(const Cells<S> &C, int x, int y, int z
     }
 }
 
-// Emit: only the count blocks that hold triangles are visited (blist, from the scan), in a
-// grid of at most one workgroup per such block that strides over them, so the many blocks of
-// cells without triangles cost no dispatch.  A workgroup classifies its count block's CELLS
-// cells (ECPT per thread), lists the ones with triangles in LDS in cell order with their
-// triangle offsets (one block scan of the packed counts) and corner values, and then spreads
-// the listed cells over its threads, one cell per thread, so the FP64 interpolation of a
-// surface-dense stretch of cells runs in parallel rather than one after another in one lane.
-// The list length, the first block index and its triangle offset are loaded together, and
-// the corner values come from the classification's own loads: two round trips to memory
-// before the stores.
+// Emit: only the units of the count blocks that hold triangles are visited (blist, from the
+// scan), in a grid of at most one workgroup per such unit that strides over them, so the many
+// blocks of cells without triangles cost no dispatch, and a surface spreads over as many
+// workgroups as it has units.  A workgroup takes unit `part` of listed block q / 4,
+// classifies its UNIT cells (ECPT per thread; a unit without triangles lists none), lists
+// the ones with triangles in LDS in cell order with their triangle offsets (one block scan of
+// the packed counts) and corner values, and then spreads the listed cells over its threads,
+// one cell per thread, so the FP64 interpolation of a surface-dense stretch of cells runs in
+// parallel rather than one after another in one lane.  The unit's first triangle is its
+// block's offset plus the totals of the block's units before it (one 16-byte load beside the
+// offset's).  The list length and the first block index are loaded together, then the offset,
+// the unit totals and the unit's nodes, and the corner values come from the classification's
+// own loads: two round trips to memory before the stores.
 template <class S>
 __global__ __launch_bounds__(EB) void k_mc_emit(Cells<S> C, const unsigned long long *__restrict__ offs,
+                                                const uint4 *__restrict__ unit_tot4,
                                                 const unsigned int *__restrict__ blist, float *__restrict__ verts,
                                                 unsigned long long cap)
 {
     __shared__ unsigned int s[EB / 64];
     __shared__ unsigned char s_cnt[256];
-    __shared__ unsigned int s_cell[CELLS];   // (outside corners << 24) | (cell within the block << 8) | case
-    __shared__ unsigned short s_tri[CELLS];  // its first triangle within the block
-    __shared__ float s_val[8][EB];           // corner values of listed cells 0 .. EB - 1
+    __shared__ unsigned int s_cell[UNIT];   // (outside corners << 24) | (cell within the unit << 8) | case
+    __shared__ unsigned short s_tri[UNIT];  // its first triangle within the unit
+    __shared__ float s_val[8][EB];          // corner values of listed cells 0 .. EB - 1
     __shared__ unsigned int s_edges[256 * 15 / 4];  // the case table's edge ids
-    // blist holds nb + 1 entries and the grid has at most nb workgroups
-    const unsigned int nne = blist[0], first = blist[1 + blockIdx.x];
-    if (blockIdx.x >= nne)
+    static_assert(UNITS_PER_BLOCK == 4, "a block's unit totals are one 16-byte load");
+    // blist holds nb + 1 entries and the grid has at most 4 nb workgroups
+    const unsigned int nne = blist[0], first = blist[1 + blockIdx.x / 4];
+    if (blockIdx.x >= 4 * nne)
         return;
     if (threadIdx.x < 256)
         s_cnt[threadIdx.x] = c_tri_count[threadIdx.x];
     for (int k = threadIdx.x; k < 256 * 15 / 4; k += EB)
         s_edges[k] = reinterpret_cast<const unsigned int *>(&c_tri_edges[0][0])[k];
-    for (unsigned int q = blockIdx.x; q < nne; q += gridDim.x) {
-        const unsigned int blk = q == blockIdx.x ? first : blist[1 + q];
-        const unsigned long long base = offs[blk];
-        const int64_t blk0 = (int64_t)blk * CELLS;
+    for (unsigned int q = blockIdx.x; q < 4 * nne; q += gridDim.x) {
+        const unsigned int blk = q == blockIdx.x ? first : blist[1 + q / 4], part = q % 4;
+        const uint4 ut = unit_tot4[blk];
+        const unsigned int before = (part > 0 ? ut.x : 0u) + (part > 1 ? ut.y : 0u) + (part > 2 ? ut.z : 0u);
+        const unsigned long long base = offs[blk] + before;
+        const int64_t blk0 = ((int64_t)blk * UNITS_PER_BLOCK + part) * UNIT;
         const int64_t i0 = blk0 + (int64_t)threadIdx.x * ECPT;
         int ci[ECPT], x[ECPT], y[ECPT], z[ECPT];
         float cv[ECPT][8];
         unsigned int om[ECPT];
         C.template quad<ECPT, true>(i0, ci, x, y, z, cv, om);
-        __syncthreads();  // the tables are in place, the previous block's cells emitted
+        __syncthreads();  // the tables are in place, the previous unit's cells emitted
         unsigned int ns = 0, nt = 0;
 #pragma unroll
         for (int c = 0; c < ECPT; c++)
@@ -467,8 +609,8 @@ __global__ __launch_bounds__(EB) void k_mc_emit(Cells<S> C, const unsigned long 
                 ns += 1;
                 nt += s_cnt[ci[c]];
             }
-        // cells in the high half, triangles (at most 5 CELLS < 2^16) in the low half
-        static_assert(5 * CELLS < (1 << 16), "packed counts");
+        // cells in the high half, triangles (at most 5 UNIT < 2^16) in the low half
+        static_assert(5 * UNIT < (1 << 16), "packed counts");
         unsigned int total;
         const unsigned int incl = block_scan_incl<EB>((ns << 16) | nt, s, total);
         unsigned int slot = (incl >> 16) - ns, tri = (incl & 0xffffu) - nt;
@@ -487,7 +629,7 @@ __global__ __launch_bounds__(EB) void k_mc_emit(Cells<S> C, const unsigned long 
             }
         __syncthreads();
         const unsigned int ncells = total >> 16;
-        // listed cell e: its values in column e for e < EB; past that (a block with more
+        // listed cell e: its values in column e for e < EB; past that (a unit with more
         // listed cells than threads) the thread gathers them into its own column, which its
         // first cell no longer needs
         const unsigned char *edges = reinterpret_cast<const unsigned char *>(s_edges);
@@ -657,6 +799,58 @@ __global__ __launch_bounds__(BLOCK) void k_mc_emit_indexed(Cells<S> C, EdgeGrid 
     }
 }
 
+// Floats in increasing order as unsigned keys (-0 just below +0; NaN outside [key(-inf), key(+inf)]).
+uint32_t float_key(float f)
+{
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return (b & 0x80000000u) ? ~b : b | 0x80000000u;
+}
+float key_float(uint32_t k)
+{
+    const uint32_t b = (k & 0x80000000u) ? k & 0x7fffffffu : ~k;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+// The raw value's decode, in float like the kernels: raw * scale (BF16, F16) or raw (F32).
+float decode_raw(float r, float scale, bool scaled)
+{
+    volatile float p = scaled ? r * scale : r;  // one rounded float product, as on the device
+    return p;
+}
+// The least float r with decode(r) >= t, by bisection over the float keys: decode is
+// nondecreasing in r (scale > 0) and decode(+inf) = +inf >= t for every non-NaN t.
+float raw_least_ge(float t, float scale, bool scaled)
+{
+    uint32_t lo = float_key(-INFINITY), hi = float_key(INFINITY);
+    if (decode_raw(-INFINITY, scale, scaled) >= t)
+        return -INFINITY;
+    while (hi - lo > 1) {  // decode(key lo) < t <= decode(key hi)
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (decode_raw(key_float(mid), scale, scaled) >= t)
+            hi = mid;
+        else
+            lo = mid;
+    }
+    return key_float(hi);
+}
+// The greatest float r with decode(r) <= t (decode(-inf) = -inf <= t for every non-NaN t).
+float raw_greatest_le(float t, float scale, bool scaled)
+{
+    uint32_t lo = float_key(-INFINITY), hi = float_key(INFINITY);
+    if (decode_raw(INFINITY, scale, scaled) <= t)
+        return INFINITY;
+    while (hi - lo > 1) {  // decode(key lo) <= t < decode(key hi)
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (decode_raw(key_float(mid), scale, scaled) <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return key_float(lo);
+}
+
 template <class S>
 Cells<S> make_cells(const kf_volume *v, double iso)
 {
@@ -676,6 +870,11 @@ Cells<S> make_cells(const kf_volume *v, double iso)
     f = (float)v->max_surface;
     C.max_dn = (double)f > v->max_surface ? std::nextafterf(f, -INFINITY) : f;
     C.oob_f = v->oob() < iso ? -INFINITY : INFINITY;
+    C.oob_bit = C.oob_f < C.iso_up ? 1u : 0u;
+    const bool scaled = S::dtype != KF_F32;
+    C.r_iso = raw_least_ge(C.iso_up, C.scale, scaled);
+    C.r_clip = raw_least_ge(C.clip_up, C.scale, scaled);
+    C.r_max = raw_greatest_le(C.max_dn, C.scale, scaled);
     return C;
 }
 
@@ -730,30 +929,35 @@ int64_t kf_mc_blocks(int64_t n) { return (n + CELLS - 1) / CELLS; }
 
 namespace {
 
-// Workgroups of the emit pass: it strides over the count blocks that hold triangles, two
-// workgroups of EB threads per CU at most.
+// Workgroups of the emit pass: it strides over the units of the blocks that hold triangles,
+// with at most as many workgroups of EB threads as fit on the CUs at once (2048 threads each).
 unsigned int emit_grid(const kf_context *ctx, int nb)
 {
-    const int g = 2 * (ctx->num_cus > 0 ? ctx->num_cus : 256);
-    return (unsigned int)(nb < g ? nb : g);
+    const int64_t g = (2048 / EB) * (ctx->num_cus > 0 ? ctx->num_cus : 256);
+    const int64_t nu = (int64_t)nb * UNITS_PER_BLOCK;
+    return (unsigned int)(nu < g ? nu : g);
 }
 
 }  // namespace
 
-kf_status kf_mc_launch_into(const kf_volume *v, double iso, unsigned int *tot, unsigned long long *offs,
-                            unsigned int *blist, unsigned long long *total_out, float *verts, uint64_t cap)
+kf_status kf_mc_launch_into(const kf_volume *v, double iso, unsigned int *unit_tot, unsigned int *tot,
+                            unsigned long long *offs, unsigned int *blist, unsigned long long *total_out,
+                            float *verts, uint64_t cap)
 {
     const int nb = (int)kf_mc_blocks(v->n);
     hipStream_t st = v->ctx->stream;
     KF_DISPATCH_DTYPE(v->dtype, S, {
         const Cells<S> C = make_cells<S>(v, iso);
-        hipLaunchKernelGGL((k_mc_count<S>), dim3(nb), dim3(BLOCK), 0, st, C, tot);
+        if (v->dims[0] % 8 == 0)
+            hipLaunchKernelGGL((k_mc_count<S, true>), dim3(nb), dim3(BLOCK), 0, st, C, unit_tot, tot);
+        else
+            hipLaunchKernelGGL((k_mc_count<S, false>), dim3(nb), dim3(BLOCK), 0, st, C, unit_tot, tot);
         hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, st, (const unsigned int *)tot, offs, nb, total_out,
                            blist);
         if (cap > 0)
             hipLaunchKernelGGL((k_mc_emit<S>), dim3(emit_grid(v->ctx, nb)), dim3(EB), 0, st, C,
-                               (const unsigned long long *)offs, (const unsigned int *)blist, verts,
-                               (unsigned long long)cap);
+                               (const unsigned long long *)offs, (const uint4 *)unit_tot, (const unsigned int *)blist,
+                               verts, (unsigned long long)cap);
     });
     KF_LAUNCH_CHECK();
     return KF_OK;
@@ -763,8 +967,8 @@ namespace {
 
 int mc_blocks(const kf_volume *v) { return (int)kf_mc_blocks(v->n); }
 
-// Counts v's triangles into the context's cache (block offsets on the device, the total on
-// the host), unless the cache already holds this version of v at this iso value.
+// Counts v's triangles into the context's cache (block offsets and unit totals on the device,
+// the total on the host), unless the cache already holds this version of v at this iso value.
 kf_status mc_count_cached(const kf_volume *v, double iso, uint64_t *total)
 {
     kf_context *ctx = v->ctx;
@@ -777,13 +981,16 @@ kf_status mc_count_cached(const kf_volume *v, double iso, uint64_t *total)
     const int nb = mc_blocks(v);
     if ((size_t)nb > c.cap) {
         KF_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipFree(c.unit_tot);
         (void)hipFree(c.tot);
         (void)hipFree(c.offs);
         (void)hipFree(c.blist);
+        c.unit_tot = nullptr;
         c.tot = nullptr;
         c.offs = nullptr;
         c.blist = nullptr;
         c.cap = 0;
+        KF_HIP(hipMalloc((void **)&c.unit_tot, sizeof(unsigned int) * UNITS_PER_BLOCK * (size_t)nb));
         KF_HIP(hipMalloc((void **)&c.tot, sizeof(unsigned int) * (size_t)nb));
         KF_HIP(hipMalloc((void **)&c.offs, sizeof(unsigned long long) * ((size_t)nb + 1)));
         KF_HIP(hipMalloc((void **)&c.blist, sizeof(unsigned int) * ((size_t)nb + 1)));
@@ -791,7 +998,7 @@ kf_status mc_count_cached(const kf_volume *v, double iso, uint64_t *total)
     }
     if (!c.total_host)
         KF_HIP(hipHostMalloc((void **)&c.total_host, 2 * sizeof(unsigned long long), hipHostMallocDefault));
-    kf_status s = kf_mc_launch_into(v, iso, c.tot, c.offs, c.blist, nullptr, nullptr, 0);
+    kf_status s = kf_mc_launch_into(v, iso, c.unit_tot, c.tot, c.offs, c.blist, nullptr, nullptr, 0);
     if (s != KF_OK)
         return s;
     KF_HIP(hipMemcpyAsync(c.total_host, c.offs + nb, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
@@ -811,8 +1018,8 @@ kf_status mc_emit_cached(const kf_volume *v, double iso, float *dverts, uint64_t
     const int nb = mc_blocks(v);
     KF_DISPATCH_DTYPE(v->dtype, S, {
         hipLaunchKernelGGL((k_mc_emit<S>), dim3(emit_grid(ctx, nb)), dim3(EB), 0, ctx->stream, make_cells<S>(v, iso),
-                           (const unsigned long long *)ctx->mc.offs, (const unsigned int *)ctx->mc.blist, dverts,
-                           (unsigned long long)cap);
+                           (const unsigned long long *)ctx->mc.offs, (const uint4 *)ctx->mc.unit_tot,
+                           (const unsigned int *)ctx->mc.blist, dverts, (unsigned long long)cap);
     });
     KF_LAUNCH_CHECK();
     return KF_OK;
@@ -840,6 +1047,7 @@ void kf_mc_release(kf_context *ctx)
     kf_mc_cache &c = ctx->mc;
     (void)hipFree(c.tot);
     (void)hipFree(c.offs);
+    (void)hipFree(c.unit_tot);
     (void)hipFree(c.blist);
     (void)hipFree(c.verts);
     if (c.total_host)

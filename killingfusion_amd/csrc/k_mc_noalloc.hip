@@ -2,18 +2,19 @@
 // for callers that cannot allocate (code under HIP graph capture, serving loops, harnesses that
 // forbid allocation).
 //
-// The count, scan and emit kernels of k_mc.hip (kf_mc_launch_into) run with their block totals
-// and block offsets in `scratch`, a file-scope device array that every GPU holds until the
+// The count, scan and emit kernels of k_mc.hip (kf_mc_launch_into) run with their unit and
+// block totals and block offsets in `scratch`, a file-scope device array that every GPU holds until the
 // process ends, and write the soup and its triangle count to caller device memory.  Nothing is
 // allocated and nothing waits on the host.  Host code never passes `scratch` itself to a
 // kernel (in host code the name stands for a host-side stand-in, not the device copy): each
 // device's address comes from hipGetSymbolAddress, taken once per device when a context is
 // created there (kf_noalloc_init), and is what the kernels receive.
 //
-// The count kernel writes every block total and the scan every offset of the grid it runs on,
-// and the list of blocks with triangles with its length, so no part of the array has to start
-// at zero.  Every call on one GPU shares its array: calls
-// that overlap on one GPU (other streams, other threads) race and must be ordered by the caller.
+// The count kernel writes every unit and block total and the scan every offset of the grid it
+// runs on, and the list of blocks with triangles with its length, so no part of the array has
+// to start at zero (2 MiB: totals, offsets and list for KF_MC_NOALLOC_MAX_VOXELS voxels).
+// Every call on one GPU shares its array: calls that overlap on one GPU (other streams, other
+// threads) race and must be ordered by the caller.
 #include "kf_internal.hpp"
 
 #include <mutex>
@@ -22,11 +23,12 @@
 namespace {
 
 constexpr int64_t MAX_BLOCKS = (KF_MC_NOALLOC_MAX_VOXELS + KF_MC_CELLS - 1) / KF_MC_CELLS;
-constexpr size_t TOT_WORDS = (size_t)(MAX_BLOCKS + 1) / 2;   // u32 block totals, two per word
+constexpr size_t UNIT_WORDS = (size_t)MAX_BLOCKS * (KF_MC_CELLS / KF_MC_UNIT) / 2;  // u32 unit totals
+constexpr size_t TOT_WORDS = (size_t)(MAX_BLOCKS + 1) / 2;   // then u32 block totals, two per word
 constexpr size_t OFFS_WORDS = (size_t)MAX_BLOCKS + 1;        // then u64 offsets and the total
 constexpr size_t LIST_WORDS = (size_t)(MAX_BLOCKS + 2) / 2;  // then the u32 list of blocks with triangles
-constexpr size_t SCRATCH_WORDS = TOT_WORDS + OFFS_WORDS + LIST_WORDS;
-static_assert(SCRATCH_WORDS * 8 <= (size_t)2 << 20, "scratch stays under 2 MiB");
+constexpr size_t SCRATCH_WORDS = UNIT_WORDS + TOT_WORDS + OFFS_WORDS + LIST_WORDS;
+static_assert(SCRATCH_WORDS * 8 <= ((size_t)2 << 20) + 256, "scratch stays at 2 MiB");
 
 static __device__ __align__(256) unsigned long long scratch[SCRATCH_WORDS];
 
@@ -68,7 +70,8 @@ extern "C" kf_status kf_marching_cubes_noalloc(const kf_volume *v, double iso, f
     }
     KF_CHECK_ARG(base, "kf_marching_cubes_noalloc: no scratch address for this device");
     KfDeviceGuard dg(ctx->device);
-    return kf_mc_launch_into(v, iso, reinterpret_cast<unsigned int *>(base), base + TOT_WORDS,
-                             reinterpret_cast<unsigned int *>(base + TOT_WORDS + OFFS_WORDS),
-                             reinterpret_cast<unsigned long long *>(count), verts, cap_tris);
+    unsigned long long *const tot = base + UNIT_WORDS, *const offs = tot + TOT_WORDS, *const list = offs + OFFS_WORDS;
+    return kf_mc_launch_into(v, iso, reinterpret_cast<unsigned int *>(base), reinterpret_cast<unsigned int *>(tot),
+                             offs, reinterpret_cast<unsigned int *>(list), reinterpret_cast<unsigned long long *>(count),
+                             verts, cap_tris);
 }

@@ -165,6 +165,7 @@ struct kf_mc_cache {
     uint64_t version = 0;
     double iso = 0.0;
     bool valid = false;
+    unsigned int *unit_tot = nullptr;      // triangles per unit of cells
     unsigned int *tot = nullptr;           // triangles per block
     unsigned long long *offs = nullptr;    // exclusive block offsets, then the total
     unsigned int *blist = nullptr;         // blocks with triangles: their number, then their indices
@@ -597,16 +598,20 @@ kf_status kf_mc_run(const kf_volume *v, double iso, uint64_t *n_tris, float *ver
 // hipMalloc when short), *n_tris the count.  Stream-ordered; one host wait for the count.
 kf_status kf_mc_to_device(const kf_volume *v, double iso, float **dverts, size_t *dverts_cap, uint64_t *n_tris);
 void kf_mc_release(kf_context *ctx);
-// Cells per block of the marching-cubes count and emit passes (one block total and one offset
-// each): a grid of n voxels has kf_mc_blocks(n) = ceil(n / KF_MC_CELLS) blocks.
+// Cells per block of the marching-cubes count pass (one triangle total, one offset and one
+// entry of the list of blocks with triangles each), and per unit (the cells of one of its
+// wavefronts: one triangle total and one workgroup of the emit pass each).  A grid of n voxels
+// has kf_mc_blocks(n) = ceil(n / KF_MC_CELLS) blocks of KF_MC_CELLS / KF_MC_UNIT units.
 constexpr int KF_MC_CELLS = 2048;
+constexpr int KF_MC_UNIT = 512;
 int64_t kf_mc_blocks(int64_t n);
-// Count, scan and (cap > 0) emit of v's triangle soup with caller-provided device buffers:
-// tot[kf_mc_blocks(n)], offs[kf_mc_blocks(n) + 1], blist[kf_mc_blocks(n) + 1], verts[9 cap];
-// *total_out (may be null, device) receives the triangle count.  Stream-ordered kernels only:
-// no allocation, no host wait.
-kf_status kf_mc_launch_into(const kf_volume *v, double iso, unsigned int *tot, unsigned long long *offs,
-                            unsigned int *blist, unsigned long long *total_out, float *verts, uint64_t cap);
+// Count, scan and (cap > 0) emit of v's triangle soup with caller-provided device buffers
+// (nb = kf_mc_blocks(n)): unit_tot[4 nb] (16-byte aligned), tot[nb], offs[nb + 1],
+// blist[nb + 1], verts[9 cap]; *total_out (may be null, device) receives the triangle count.
+// Stream-ordered kernels only: no allocation, no host wait.
+kf_status kf_mc_launch_into(const kf_volume *v, double iso, unsigned int *unit_tot, unsigned int *tot,
+                            unsigned long long *offs, unsigned int *blist, unsigned long long *total_out,
+                            float *verts, uint64_t cap);
 kf_status kf_mc_run_indexed(const kf_volume *v, double iso, uint64_t *n_verts, uint64_t *n_tris, float *verts_host,
                             float *normals_host, uint32_t *tris_host, uint64_t cap_verts, uint64_t cap_tris);
 
