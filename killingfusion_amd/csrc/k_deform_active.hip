@@ -385,9 +385,8 @@ __device__ inline void list_neighbours(const GridDims &g, const unsigned long lo
     const unsigned int i = list[t];
     const unsigned int n32 = nvox32(g);
     uc[t] = make_float4(u[i], u[n32 + i], u[2u * n32 + i], 0.0f);
-    const unsigned int nxy = (unsigned int)g.nx * (unsigned int)g.ny;
-    const int z = (int)(i / nxy), r = (int)(i - (unsigned int)z * nxy);
-    const int y = r / g.nx, x = r - y * g.nx;
+    int x, y, z;
+    g.xyz32(i, x, y, z);
     int slot[NNB];
 #pragma unroll
     for (int k = 0; k < NNB; k++) {
@@ -566,11 +565,7 @@ __device__ inline void load_slots(const int4 *__restrict__ nbr, unsigned int t, 
 
 __device__ inline void voxel_xyz(const GridDims &g, unsigned int i, int &x, int &y, int &z)
 {
-    const unsigned int nxy = (unsigned int)g.nx * (unsigned int)g.ny;
-    z = (int)(i / nxy);
-    const int r = (int)(i - (unsigned int)z * nxy);
-    y = r / g.nx;
-    x = r - y * g.nx;
+    g.xyz32(i, x, y, z);
 }
 
 // Round 2 and the update of listed slot t of voxel i, given its own u and neighbour slots:

@@ -446,16 +446,14 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
     double mx = 0.0, esum = 0.0;
     unsigned long long act = 0;
     const unsigned int wave0 = tile * (unsigned int)FLAT + (threadIdx.x & ~63u) * (unsigned int)VEC;
-    const unsigned int nxy = (unsigned int)g.nx * (unsigned int)g.ny;
     for (int j = threadIdx.x & 63; j < total; j += 64) {
         int bit, src;
         kf_job_decode<FPT * VEC>(bal, j, bit, src);
         const unsigned int i =
             wave0 + (unsigned int)(bit / VEC) * (FB * VEC) + (unsigned int)src * VEC + (unsigned int)(bit % VEC);
-        const unsigned int z = i / nxy, r = i - z * nxy;
-        const unsigned int y = r / (unsigned int)g.nx, x = r - y * (unsigned int)g.nx;
-        adj_job<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, &st->div_slot[parity], nullptr, (int)x, (int)y, (int)z, mx,
-                                 esum, act);
+        int x, y, z;
+        g.xyz32(i, x, y, z);
+        adj_job<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, &st->div_slot[parity], nullptr, x, y, z, mx, esum, act);
     }
     kf_block_publish<FB>(partials, blockIdx.x, esum, mx, act);
 }

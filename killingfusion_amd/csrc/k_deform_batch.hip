@@ -85,14 +85,13 @@ __device__ inline void batch_tile(const BatchVol<S> &V, unsigned int t, const Ad
     }
     double esum = 0.0;
     const unsigned int wave0 = base + (threadIdx.x & ~63u);
-    const unsigned int nxy = (unsigned int)g.nx * (unsigned int)g.ny;
     for (int j = threadIdx.x & 63; j < total; j += 64) {
         int bit, src;
         kf_job_decode<VPT>(bal, j, bit, src);
         const unsigned int i = wave0 + (unsigned int)bit * BLOCK + (unsigned int)src;
-        const unsigned int z = i / nxy, r = i - z * nxy;
-        const unsigned int y = r / (unsigned int)g.nx, x = r - y * (unsigned int)g.nx;
-        adj_job<S, KF_MODE_STEP>(V.L, V.Cn, u_in, u_out, c, div, nullptr, (int)x, (int)y, (int)z, mx, esum, act);
+        int x, y, z;
+        g.xyz32(i, x, y, z);
+        adj_job<S, KF_MODE_STEP>(V.L, V.Cn, u_in, u_out, c, div, nullptr, x, y, z, mx, esum, act);
     }
 }
 
@@ -189,15 +188,14 @@ __device__ inline void batch_tile_v4(const BatchVol<S> &V, unsigned int t, const
     }
     double esum = 0.0;
     const unsigned int wave0 = base + (threadIdx.x & ~63u) * VEC;
-    const unsigned int nxy = (unsigned int)g.nx * (unsigned int)g.ny;
     for (int j = threadIdx.x & 63; j < total; j += 64) {
         int bit, src;
         kf_job_decode<G * VEC>(bal, j, bit, src);
         const unsigned int i = wave0 + (unsigned int)(bit / VEC) * (BLOCK * VEC) + (unsigned int)src * VEC +
                                (unsigned int)(bit % VEC);
-        const unsigned int z = i / nxy, r = i - z * nxy;
-        const unsigned int y = r / (unsigned int)g.nx, x = r - y * (unsigned int)g.nx;
-        adj_job<S, KF_MODE_STEP>(V.L, V.Cn, u_in, u_out, c, div, nullptr, (int)x, (int)y, (int)z, mx, esum, act);
+        int x, y, z;
+        g.xyz32(i, x, y, z);
+        adj_job<S, KF_MODE_STEP>(V.L, V.Cn, u_in, u_out, c, div, nullptr, x, y, z, mx, esum, act);
     }
 }
 

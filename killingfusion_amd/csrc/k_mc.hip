@@ -197,8 +197,8 @@ struct Cells {
 
     // Count pass of a grid whose rows are a multiple of 8 nodes long: the 8 cells i0 .. i0 + 7
     // (i0 a multiple of 8, i0 < n) lie in one row, and each of their four node rows starts
-    // 16-byte aligned.  load8 issues the loads (a row past the grid is read from the last
-    // row, and node 8 past the row's end from node 7, their values then replaced);
+    // 16-byte aligned.  load8 issues the loads (a row past the grid is read from row 0, and
+    // node 8 past the row's end from node 7, their values then replaced);
     // count8 packs the below-iso bits node-major, bit r + 4 j for node j of row r, so that
     // cell c's eight corners are the byte at bit 4 c, and sums the triangle counts that the
     // table sc holds for that byte (kNodeBit order).
@@ -207,12 +207,13 @@ struct Cells {
         int x0, y0, z0;
         kf_xyz(g, i0, x0, y0, z0);
         xl = x0 + 8 < g.nx;
-        rows_out = 0;
+        // node row r = dy + 2 dz starts at i0 + dy nx + dz nx ny (i0 is node (x0, y0, z0))
+        const bool iny = y0 + 1 < g.ny, inz = z0 + 1 < g.nz;
+        const int64_t sy = iny ? g.nx : 0, sz = inz ? (int64_t)g.nx * g.ny : 0;
+        rows_out = (iny ? 0u : 0xau) | (inz ? 0u : 0xcu);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int y = y0 + (r & 1), z = z0 + (r >> 1);
-            rows_out |= y < g.ny && z < g.nz ? 0u : 1u << r;
-            const int64_t b = g.idx(x0, y < g.ny ? y : g.ny - 1, z < g.nz ? z : g.nz - 1);
+            const int64_t b = i0 + (r & 1 ? sy : 0) + (r & 2 ? sz : 0);
             raw8<S>(phi + b, v[r]);
             e[r] = raw1<S>(phi, xl ? b + 8 : b + 7);
         }

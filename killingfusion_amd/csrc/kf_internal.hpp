@@ -316,9 +316,37 @@ __device__ inline void kf_load4_unaligned(const typename S::T *p, float scale, f
 // Grid geometry passed to kernels by value
 // ---------------------------------------------------------------------------
 
+// Unsigned 32-bit division by a divisor fixed before a launch, as a multiply-high and two
+// shifts (Granlund and Montgomery, "Division by Invariant Integers using Multiplication",
+// PLDI 1994, figure 4.1): with l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+// s1 = min(l, 1), s2 = max(l - 1, 0) and t = the high word of m n,
+// n / d = (t + ((n - t) >> s1)) >> s2 exactly, for every n < 2^32.  Five integer instructions
+// where a division by a value known only at run time takes about twenty.
+struct U32Div {
+    uint32_t m = 1, s1 = 0, s2 = 0;  // d = 1
+    static inline U32Div of(uint32_t d)  // d >= 1
+    {
+        U32Div r;
+        uint32_t l = 0;
+        while (l < 32 && ((uint64_t)1 << l) < d)
+            ++l;
+        // 2^l - d < d, so m < 2^32
+        r.m = (uint32_t)(((((uint64_t)1 << l) - d) << 32) / d + 1);
+        r.s1 = l < 1 ? l : 1;
+        r.s2 = l > 0 ? l - 1 : 0;
+        return r;
+    }
+    __device__ __host__ inline uint32_t div(uint32_t n) const
+    {
+        const uint32_t t = (uint32_t)(((uint64_t)m * n) >> 32);
+        return (t + ((n - t) >> s1)) >> s2;
+    }
+};
+
 struct GridDims {
     int nx, ny, nz;
     int64_t n;
+    U32Div dx, dxy;  // division by nx and by nx ny (the latter only when nx ny < 2^32)
     __device__ __host__ inline bool in(int x, int y, int z) const
     {
         return x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
@@ -332,14 +360,31 @@ struct GridDims {
     {
         return ((unsigned int)z * (unsigned int)ny + (unsigned int)y) * (unsigned int)nx + (unsigned int)x;
     }
+    // (x, y, z) of voxel index i of a grid of fewer than 2^32 voxels.
+    __device__ __host__ inline void xyz32(uint32_t i, int &x, int &y, int &z) const
+    {
+        const uint32_t zz = dxy.div(i), r = i - zz * ((uint32_t)nx * (uint32_t)ny), yy = dx.div(r);
+        z = (int)zz;
+        y = (int)yy;
+        x = (int)(r - yy * (uint32_t)nx);
+    }
 };
+
+inline GridDims make_grid(int nx, int ny, int nz, int64_t n)
+{
+    GridDims g{nx, ny, nz, n};
+    g.dx = U32Div::of((uint32_t)nx);
+    const uint64_t nxy = (uint64_t)nx * (uint64_t)ny;
+    g.dxy = nxy < ((uint64_t)1 << 32) ? U32Div::of((uint32_t)nxy) : U32Div{};
+    return g;
+}
 
 // A fresh version for a volume whose phi is about to change (see kf_volume::version).
 uint64_t kf_next_version();
 inline void kf_touch(kf_volume *v) { v->version = kf_next_version(); }
 
-inline GridDims grid_of(const kf_volume *v) { return GridDims{v->dims[0], v->dims[1], v->dims[2], v->n}; }
-inline GridDims grid_of(const kf_field *f) { return GridDims{f->dims[0], f->dims[1], f->dims[2], f->n}; }
+inline GridDims grid_of(const kf_volume *v) { return make_grid(v->dims[0], v->dims[1], v->dims[2], v->n); }
+inline GridDims grid_of(const kf_field *f) { return make_grid(f->dims[0], f->dims[1], f->dims[2], f->n); }
 
 inline bool same_dims(const int32_t a[3], const int32_t b[3]) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2]; }
 
@@ -363,16 +408,12 @@ struct KfDeviceGuard {
     KfDeviceGuard &operator=(const KfDeviceGuard &) = delete;
 };
 
-// (x, y, z) of linear voxel index i, with 32-bit divisions when N < 2^32 (64-bit integer
-// division is a long emulated sequence per voxel).
+// (x, y, z) of linear voxel index i: multiply-high divisions (U32Div) when N < 2^32, 64-bit
+// integer divisions (a long emulated sequence) otherwise.
 __device__ inline void kf_xyz(const GridDims &g, int64_t i, int &x, int &y, int &z)
 {
     if (g.n < ((int64_t)1 << 32)) {
-        const unsigned int i32 = (unsigned int)i, nxy = (unsigned int)g.nx * (unsigned int)g.ny;
-        z = (int)(i32 / nxy);
-        const unsigned int r = i32 - (unsigned int)z * nxy;
-        y = (int)(r / (unsigned int)g.nx);
-        x = (int)(r - (unsigned int)y * (unsigned int)g.nx);
+        g.xyz32((uint32_t)i, x, y, z);
     } else {
         x = (int)(i % g.nx);
         y = (int)((i / g.nx) % g.ny);
