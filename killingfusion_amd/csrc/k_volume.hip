@@ -6,6 +6,7 @@
 // operation order, and this file is built with -ffp-contract=off: every product and sum
 // rounds on its own, as in the reference's x86-64 build.  They stream each voxel once.
 #include "kf_internal.hpp"
+#include "kf_project.hpp"
 
 namespace {
 
@@ -39,20 +40,18 @@ __global__ void k_decode(float *dst, const typename S::T *phi, int64_t n, float 
         dst[i] = S::load(phi, i, scale);
 }
 
-struct Mat9 { double m[9]; };
-struct Mat16 { double m[16]; };
-
 // SDF::integrateDepthFrame, SDF.cpp:149-196.  FRESH: the same pass also resets the volume
 // (SDF.cpp:133-134: phi = Max + eps, w = 0) before integrating, as a reset followed by an
 // integration would -- same values, one write per voxel and no read.
 //
-// A thread takes IPT consecutive voxels.  The camera coordinates keep the reference's
-// left-to-right sums, c_r = ((W2C[r][0] X + W2C[r][1] Y) + W2C[r][2] Z) + W2C[r][3] 1, but the
-// products of Y, Z and 1 are formed once per row, not per voxel (the same roundings).  The
-// running average (old w + dist) / (w + 1) is dist itself when w = 0 and old is finite
-// (old 0 = +-0, +-0 + dist = dist for dist != 0 and +0 for dist = +0, dist / 1 = dist), so that
-// case skips the multiply, add and FP64 division.  The stores of a FRESH pass are one vector store per
-// thread for phi and for w.
+// A thread takes IPT consecutive voxels.  Each voxel's pixel is decided by kf_pixel_fast (no
+// FP64 division), or by kf_pixel_rowwise for cameras whose depth and image row stay fixed along
+// a grid row (ROWWISE: the reference's identity pose), and, where that route cannot vouch for
+// the reference's rounding, by kf_pixel_exact (kf_project.hpp); the camera depth c2 is the
+// reference's FP64 value either way.  The running average (old w + dist) / (w + 1) is dist itself when w = 0 and old is
+// finite (old 0 = +-0, +-0 + dist = dist for dist != 0 and +0 for dist = +0, dist / 1 = dist),
+// so that case skips the multiply, add and FP64 division.  The stores of a FRESH pass are one
+// vector store per thread for phi and for w.
 constexpr int IPT = 4;
 
 template <class S>
@@ -62,52 +61,47 @@ struct IntegrateArgs {
     GridDims g;
     double mn[3];
     double voxel, clip;
-    float scale, oob_m;
+    float scale;
+    double rscale;          // kf_quot_rscale(scale)
+    typename S::T init;     // the stored Max + eps of a voxel no frame has reached
     const float *__restrict__ depth;
-    int W, H;
-    Mat9 K;
-    Mat16 M;
+    KfCamera C;
     double min_depth, max_depth;
 };
 
-// The row constants of row (y, z): P[r][0] = W2C[r][1] Y, P[r][1] = W2C[r][2] Z, P[r][2] = W2C[r][3] 1.
+// Per-row constants: the exact camera-depth products of row (y, z) and the fast path's sums.
+struct IntegrateRow {
+    double p2[3];
+    double bx, by;
+};
+
 template <class S>
-__device__ inline void row_products(const IntegrateArgs<S> &A, int y, int z, double P[3][3])
+__device__ inline IntegrateRow integrate_row(const IntegrateArgs<S> &A, int y, int z)
 {
     const double Z = A.voxel * (z + 0.5) + A.mn[2];
     const double Y = A.voxel * (y + 0.5) + A.mn[1];
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        P[r][0] = A.M.m[r * 4 + 1] * Y;
-        P[r][1] = A.M.m[r * 4 + 2] * Z;
-        P[r][2] = A.M.m[r * 4 + 3] * 1.0;
-    }
+    IntegrateRow R;
+    R.p2[0] = A.C.M[9] * Y;
+    R.p2[1] = A.C.M[10] * Z;
+    R.p2[2] = A.C.M[11] * 1.0;
+    kf_row_fast(A.C, Y, Z, R.bx, R.by);
+    return R;
 }
 
-// Projection of voxel x of the row with products P (SDF.cpp:158-175): its camera depth and
-// the index of the pixel it reads, or -1 outside the image.
+template <class S>
+__device__ inline KfRowwise integrate_rowwise(const IntegrateArgs<S> &A, int y, int z)
+{
+    const double Z = A.voxel * (z + 0.5) + A.mn[2];
+    const double Y = A.voxel * (y + 0.5) + A.mn[1];
+    const double c2 = kf_rowwise_depth(A.C, Y, Z);
+    return kf_rowwise_row(A.C, Y, Z, c2, __builtin_amdgcn_rcpf((float)c2), A.voxel, A.mn[0]);
+}
+
+// A voxel's camera depth and the index of the pixel it reads, or -1 outside the image.
 struct Projected {
     double zc;
     int pix;
 };
-
-template <class S>
-__device__ inline Projected project_voxel(const IntegrateArgs<S> &A, const double P[3][3], int x)
-{
-    const double X = A.voxel * (x + 0.5) + A.mn[0];
-    double c[3];
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-        c[r] = ((A.M.m[r * 4 + 0] * X + P[r][0]) + P[r][1]) + P[r][2];
-    const double q0 = c[0] / c[2], q1 = c[1] / c[2];
-    const double px = A.K.m[0] * q0 + A.K.m[1] * q1 + A.K.m[2] * 1.0;
-    const double py = A.K.m[3] * q0 + A.K.m[4] * q1 + A.K.m[5] * 1.0;
-    // roundf on the float-narrowed coordinate, SDF.cpp:167,171.
-    const int col = (int)roundf((float)(px + 0.5));
-    const int row = (int)roundf((float)(py + 0.5));
-    const bool in = col >= 0 && col < A.W && row >= 0 && row < A.H;
-    return Projected{c[2], in ? row * A.W + col : -1};
-}
 
 // The running average of SDF.cpp:189-193 for a voxel that read depth d (pix >= 0); false when
 // the frame leaves the voxel alone (depth out of range, or too far behind the surface).
@@ -127,7 +121,7 @@ __device__ inline bool average_voxel(const IntegrateArgs<S> &A, const Projected 
     return true;
 }
 
-template <class S, bool FRESH>
+template <class S, bool FRESH, bool ROWWISE>
 __global__ __launch_bounds__(BLOCK) void k_integrate(IntegrateArgs<S> A)
 {
     const GridDims &g = A.g;
@@ -136,30 +130,74 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(IntegrateArgs<S> A)
         return;
     int x, y, z;
     kf_xyz(g, i0, x, y, z);
-    double P[3][3];
-    row_products(A, y, z, P);
     // project the group's voxels, then read their pixels together
     Projected pr[IPT];
+    int vx[IPT], vy[IPT], vz[IPT];
+    unsigned int exact = 0;  // voxels whose pixel kf_pixel_exact decides
+    if (ROWWISE) {
+        // depth and image row fixed along the grid row (kf_camera_rowwise)
+        KfRowwise R = integrate_rowwise(A, y, z);
 #pragma unroll
-    for (int k = 0; k < IPT; k++) {
-        if (x == g.nx) {  // the group runs into the next row
-            x = 0;
-            if (++y == g.ny) {
-                y = 0;
-                ++z;
+        for (int k = 0; k < IPT; k++) {
+            if (x == g.nx) {  // the group runs into the next row
+                x = 0;
+                if (++y == g.ny) {
+                    y = 0;
+                    ++z;
+                }
+                R = integrate_rowwise(A, y, z);
             }
-            row_products(A, y, z, P);
+            bool ok;
+            const int pix = kf_pixel_rowwise(A.C, R, x, ok);
+            const bool live = i0 + k < g.n;
+            pr[k] = Projected{R.c2, live ? pix : -1};
+            exact |= (live && !ok ? 1u : 0u) << k;
+            vx[k] = x;
+            vy[k] = y;
+            vz[k] = z;
+            ++x;
         }
-        pr[k] = project_voxel(A, P, x);
-        if (i0 + k >= g.n)
-            pr[k].pix = -1;
-        ++x;
+    } else {
+        IntegrateRow R = integrate_row(A, y, z);
+#pragma unroll
+        for (int k = 0; k < IPT; k++) {
+            if (x == g.nx) {
+                x = 0;
+                if (++y == g.ny) {
+                    y = 0;
+                    ++z;
+                }
+                R = integrate_row(A, y, z);
+            }
+            const double X = A.voxel * (x + 0.5) + A.mn[0];
+            const double c2 = ((A.C.M[8] * X + R.p2[0]) + R.p2[1]) + R.p2[2];
+            bool ok;
+            const int pix = kf_pixel_fast(A.C, R.bx, R.by, X, c2, __builtin_amdgcn_rcpf((float)c2), ok);
+            const bool live = i0 + k < g.n;
+            pr[k] = Projected{c2, live ? pix : -1};
+            exact |= (live && !ok ? 1u : 0u) << k;
+            vx[k] = x;
+            vy[k] = y;
+            vz[k] = z;
+            ++x;
+        }
+    }
+    if (exact) {
+#pragma unroll
+        for (int k = 0; k < IPT; k++) {
+            if (exact >> k & 1u) {
+                double P[3][3];
+                kf_row_exact(A.C, A.voxel * (vy[k] + 0.5) + A.mn[1], A.voxel * (vz[k] + 0.5) + A.mn[2], P);
+                const double X = A.voxel * (vx[k] + 0.5) + A.mn[0];
+                pr[k].zc = kf_cam(A.C, P[2], 2, X);
+                pr[k].pix = kf_pixel_exact(A.C, P, X, pr[k].zc);
+            }
+        }
     }
     float dep[IPT];
 #pragma unroll
     for (int k = 0; k < IPT; k++)
         dep[k] = A.depth[pr[k].pix >= 0 ? pr[k].pix : 0];
-    const typename S::T init = S::enc(A.oob_m, A.scale);
     if (FRESH && i0 + IPT <= g.n) {
         // all IPT voxels written: phi and w as one vector store each
         typedef typename S::T vec_t __attribute__((ext_vector_type(IPT)));
@@ -169,7 +207,7 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(IntegrateArgs<S> A)
         for (int k = 0; k < IPT; k++) {
             double nv;
             const bool take = average_voxel(A, pr[k], dep[k], 0, 0.0, nv);
-            pv[k] = take ? S::enc((float)nv, A.scale) : init;
+            pv[k] = take ? S::enc_r((float)nv, A.scale, A.rscale) : A.init;
             wv[k] = take ? 1 : 0;
         }
         *reinterpret_cast<vec_t *>(A.phi + i0) = pv;
@@ -182,13 +220,13 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(IntegrateArgs<S> A)
         if (i >= g.n)
             break;
         const int32_t wi = FRESH ? 0 : A.w[i];
-        const double old = FRESH ? (double)S::dec(init, A.scale) : (double)S::load(A.phi, i, A.scale);
+        const double old = FRESH ? (double)S::dec(A.init, A.scale) : (double)S::load(A.phi, i, A.scale);
         double nv;
         if (average_voxel(A, pr[k], dep[k], wi, old, nv)) {
-            A.phi[i] = S::enc((float)nv, A.scale);
+            A.phi[i] = S::enc_r((float)nv, A.scale, A.rscale);
             A.w[i] = wi + 1;
         } else if (FRESH) {
-            A.phi[i] = init;
+            A.phi[i] = A.init;
             A.w[i] = 0;
         }
     }
@@ -408,12 +446,17 @@ kf_status kf_launch_integrate(kf_volume *v, const float *ddepth, int w, int h, c
                               const double W2C[16], double min_depth, double max_depth, bool fresh)
 {
     kf_touch(v);
-    Mat9 k;
-    Mat16 m;
+    KfCamera cam;
     for (int i = 0; i < 9; i++)
-        k.m[i] = K[i];
+        cam.K[i] = K[i];
     for (int i = 0; i < 16; i++)
-        m.m[i] = W2C[i];
+        cam.M[i] = W2C[i];
+    cam.W = w;
+    cam.H = h;
+    double xb[3];
+    for (int a = 0; a < 3; a++)
+        xb[a] = kf_coord_bound(v->min_corner[a], v->voxel, v->dims[a]);
+    kf_camera_prepare(cam, xb);
     const unsigned int nb = (unsigned int)((v->n + (int64_t)BLOCK * IPT - 1) / ((int64_t)BLOCK * IPT));
     KF_DISPATCH_DTYPE(v->dtype, S, {
         IntegrateArgs<S> A;
@@ -425,18 +468,21 @@ kf_status kf_launch_integrate(kf_volume *v, const float *ddepth, int w, int h, c
         A.voxel = v->voxel;
         A.clip = v->clip;
         A.scale = v->scale();
-        A.oob_m = (float)v->oob();
+        A.rscale = kf_quot_rscale(A.scale);
+        A.init = S::enc((float)v->oob(), A.scale);
         A.depth = ddepth;
-        A.W = w;
-        A.H = h;
-        A.K = k;
-        A.M = m;
+        A.C = cam;
         A.min_depth = min_depth;
         A.max_depth = max_depth;
-        if (fresh)
-            hipLaunchKernelGGL((k_integrate<S, true>), dim3(nb), dim3(BLOCK), 0, v->ctx->stream, A);
+        const bool rowwise = kf_camera_rowwise(cam);
+        if (fresh && rowwise)
+            hipLaunchKernelGGL((k_integrate<S, true, true>), dim3(nb), dim3(BLOCK), 0, v->ctx->stream, A);
+        else if (fresh)
+            hipLaunchKernelGGL((k_integrate<S, true, false>), dim3(nb), dim3(BLOCK), 0, v->ctx->stream, A);
+        else if (rowwise)
+            hipLaunchKernelGGL((k_integrate<S, false, true>), dim3(nb), dim3(BLOCK), 0, v->ctx->stream, A);
         else
-            hipLaunchKernelGGL((k_integrate<S, false>), dim3(nb), dim3(BLOCK), 0, v->ctx->stream, A);
+            hipLaunchKernelGGL((k_integrate<S, false, false>), dim3(nb), dim3(BLOCK), 0, v->ctx->stream, A);
     });
     KF_LAUNCH_CHECK();
     return KF_OK;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <string>
 
@@ -224,6 +225,24 @@ struct kf_field {
 // Storage codecs.  F32 holds metres; BF16/F16 hold phi / (Max + eps).
 // ---------------------------------------------------------------------------
 
+// v / scale rounded to float, as (float)((double)v * rscale) where rscale is 1 / scale rounded
+// to double (kf_quot_rscale), without a float division.  The two agree for every float v when
+// scale is a normal float below 2: the product lies within 2^-52 relative of v / scale, while a
+// quotient of two floats that is not a midpoint between adjacent floats lies at least 2^-49
+// relative from every midpoint (subnormal results included), and with scale < 2 no quotient is
+// a midpoint (its odd significand would need bits below 2^-149).  So both round to the same
+// float; inf, NaN and signed zeros pass through the product unchanged.  rscale = 0 (a scale
+// outside that range) takes the division.  tests/cpp/integrate_check.cpp compares the two.
+__device__ __host__ inline float kf_quot(float v, float scale, double rscale)
+{
+    return rscale != 0.0 ? (float)((double)v * rscale) : v / scale;
+}
+
+inline double kf_quot_rscale(float scale)
+{
+    return (std::isnormal(scale) && scale > 0.0f && scale < 2.0f) ? 1.0 / (double)scale : 0.0;
+}
+
 struct StoreF32 {
     using T = float;
     static constexpr int dtype = KF_F32;
@@ -238,6 +257,7 @@ struct StoreF32 {
         o[3] = v.w;
     }
     __device__ __host__ static inline T enc(float v, float) { return v; }
+    __device__ __host__ static inline T enc_r(float v, float, double) { return v; }
     __device__ __host__ static inline float dec(T v, float) { return v; }
 };
 
@@ -261,14 +281,20 @@ struct StoreBF16 {
         o[3] = __uint_as_float(v.y & 0xffff0000u) * scale;
     }
     // Round to nearest even (NaN kept quiet).
-    __device__ __host__ static inline T enc(float v, float scale)
+    __device__ __host__ static inline T round(float q)
     {
         union { uint32_t u; float f; } c;
-        c.f = v / scale;
+        c.f = q;
         if ((c.u & 0x7fffffffu) > 0x7f800000u)
             return (T)((c.u >> 16) | 0x40u);
         uint32_t r = c.u + 0x7fffu + ((c.u >> 16) & 1u);
         return (T)(r >> 16);
+    }
+    __device__ __host__ static inline T enc(float v, float scale) { return round(v / scale); }
+    // enc with the quotient formed as kf_quot (rscale = 1 / scale rounded to double, or 0).
+    __device__ __host__ static inline T enc_r(float v, float scale, double rscale)
+    {
+        return round(kf_quot(v, scale, rscale));
     }
 };
 
@@ -287,6 +313,10 @@ struct StoreF16 {
         o[3] = (float)__builtin_bit_cast(_Float16, (unsigned short)(v.y >> 16)) * scale;
     }
     __device__ __host__ static inline T enc(float v, float scale) { return (T)(v / scale); }
+    __device__ __host__ static inline T enc_r(float v, float scale, double rscale)
+    {
+        return (T)kf_quot(v, scale, rscale);
+    }
 };
 
 // Four consecutive stored SDF values from p, decoded, where p is aligned to its element size

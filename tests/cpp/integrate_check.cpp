@@ -1,0 +1,257 @@
+// Host build of the integration kernel's two shortcuts (killingfusion_amd/csrc/kf_project.hpp,
+// kf_internal.hpp), checked against the computations they replace:
+//
+//   project <seed> <cameras>  kf_pixel_fast, and kf_pixel_rowwise with its row depth, against
+//       kf_pixel_exact.  Random cameras (intrinsics, image sizes, poses -- every other one the
+//       identity or a rotation about x, which take the rowwise form --, grids); per camera,
+//       random voxels and voxels whose pixel coordinate
+//       is placed next to a pixel boundary (the principal point shifted so that px + 0.5 lies
+//       0, +-1e-15 .. +-1e-4 from a half-integer); the float reciprocal taken correctly rounded
+//       and pushed one float ulp either way.  Prints "checks fast mismatches random random-fast
+//       rowwise": a mismatch is a voxel where a fast form claims its pixel (ok) and the pixel (or,
+//       for the rowwise form, the camera depth) differs from the exact path's.
+//   quot <seed> <count>  StoreBF16 / StoreF16 enc_r (product by the double reciprocal) against
+//       enc (float division): every scale of a list, random float bit patterns (all exponents,
+//       subnormals, infinities, NaNs) and signed zeros.  Prints "values mismatches".
+#include "kf_internal.hpp"
+#include "kf_project.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+
+namespace {
+
+void rotation(std::mt19937_64 &rng, double R[9])
+{
+    // uniform unit quaternion
+    std::normal_distribution<double> nd(0.0, 1.0);
+    double q[4], n = 0.0;
+    for (double &v : q) {
+        v = nd(rng);
+        n += v * v;
+    }
+    n = std::sqrt(n);
+    const double w = q[0] / n, x = q[1] / n, y = q[2] / n, z = q[3] / n;
+    const double m[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                         2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                         2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)};
+    std::memcpy(R, m, sizeof(m));
+}
+
+struct Grid {
+    double mn[3], voxel;
+    int n[3];
+};
+
+struct Counts {
+    long voxels = 0, fast = 0, bad = 0;
+    long rvoxels = 0, rfast = 0;  // the random voxels alone
+    long rowwise = 0;             // checks of the rowwise form
+};
+
+float rcp_perturbed(double c2, int e)
+{
+    const float cf = (float)c2;
+    float r = 1.0f / cf;
+    for (int k = 0; k < (e < 0 ? -e : e); k++)
+        r = std::nextafter(r, e < 0 ? 0.0f : INFINITY);
+    return r;
+}
+
+void check_voxel(const KfCamera &C, const Grid &G, int x, int y, int z, Counts &c, bool random = false)
+{
+    const double X = G.voxel * (x + 0.5) + G.mn[0];
+    const double Y = G.voxel * (y + 0.5) + G.mn[1];
+    const double Z = G.voxel * (z + 0.5) + G.mn[2];
+    double P[3][3];
+    kf_row_exact(C, Y, Z, P);
+    const double c2 = kf_cam(C, P[2], 2, X);
+    const int pe = kf_pixel_exact(C, P, X, c2);
+    double bx, by;
+    kf_row_fast(C, Y, Z, bx, by);
+    const bool rowwise = kf_camera_rowwise(C);
+    const double c2r = rowwise ? kf_rowwise_depth(C, Y, Z) : 0.0;
+    for (int e = -1; e <= 1; e++) {
+        bool ok;
+        int pf;
+        bool depth_ok = true;
+        if (rowwise) {
+            // the rowwise form: its row depth must be the voxel's own wherever it decides
+            const KfRowwise R = kf_rowwise_row(C, Y, Z, c2r, rcp_perturbed(c2r, e), G.voxel, G.mn[0]);
+            pf = kf_pixel_rowwise(C, R, x, ok);
+            depth_ok = !ok || std::memcmp(&R.c2, &c2, sizeof(double)) == 0;
+        } else {
+            pf = kf_pixel_fast(C, bx, by, X, c2, rcp_perturbed(c2, e), ok);
+        }
+        c.voxels++;
+        c.rvoxels += random;
+        c.rowwise += rowwise;
+        if (ok) {
+            c.fast++;
+            c.rfast += random;
+            if (pf != pe || !depth_ok) {
+                if (c.bad < 5)
+                    std::fprintf(stderr, "mismatch voxel (%d %d %d)%s exact %d fast %d c2 %.17g\n", x, y, z,
+                                 rowwise ? " rowwise" : "", pe, pf, c2);
+                c.bad++;
+            }
+        }
+    }
+}
+
+int run_project(unsigned long seed, int cameras)
+{
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    const int sizes[][2] = {{640, 480}, {1280, 720}, {17, 13}, {4095, 4095}, {1, 1}, {320, 240}};
+    Counts c;
+    const double deltas[] = {0.0, 1e-15, 1e-13, 1e-11, 1e-9, 1e-8, 1e-7, 1e-6, 3e-6, 1e-5, 3e-5, 1e-4};
+    for (int cam = 0; cam < cameras; cam++) {
+        KfCamera C;
+        const int *wh = sizes[cam % 6];
+        C.W = wh[0];
+        C.H = wh[1];
+        const double f = 100.0 + 2000.0 * U(rng);
+        C.K[0] = f * (0.8 + 0.4 * U(rng));
+        C.K[1] = cam % 3 == 0 ? 0.0 : 2.0 * (U(rng) - 0.5);
+        C.K[2] = C.W * U(rng);
+        C.K[3] = 0.0;
+        C.K[4] = f * (0.8 + 0.4 * U(rng));
+        C.K[5] = C.H * U(rng);
+        C.K[6] = 0.0;
+        C.K[7] = 0.0;
+        C.K[8] = 1.0;
+        double R[9];
+        rotation(rng, R);
+        if (cam % 2 == 1) {
+            // a pose whose depth and image row stay fixed along grid rows: the identity, or a
+            // rotation about the x axis (every other such camera keeps the identity)
+            const double t = cam % 4 == 1 ? 0.0 : (U(rng) - 0.5) * 1.5;
+            const double m[9] = {1, 0, 0, 0, std::cos(t), -std::sin(t), 0, std::sin(t), std::cos(t)};
+            std::memcpy(R, m, sizeof(m));
+        }
+        Grid G;
+        G.voxel = std::pow(10.0, -3.0 + 1.5 * U(rng));
+        for (int a = 0; a < 3; a++) {
+            G.n[a] = 8 + (int)(250 * U(rng));
+            G.mn[a] = (U(rng) - 0.5) * 4.0 - 0.5 * G.n[a] * G.voxel;
+        }
+        // camera centre 0.2 - 3 m from the grid centre, looking at it (up to a tilt), or anywhere
+        double ctr[3], eye[3];
+        for (int a = 0; a < 3; a++)
+            ctr[a] = G.mn[a] + 0.5 * G.n[a] * G.voxel;
+        const double dist = 0.2 + 3.0 * U(rng);
+        for (int a = 0; a < 3; a++)
+            eye[a] = ctr[a] - dist * R[6 + a];  // optical axis = row 2 of R
+        for (int r = 0; r < 3; r++) {
+            for (int j = 0; j < 3; j++)
+                C.M[r * 4 + j] = R[r * 3 + j];
+            C.M[r * 4 + 3] = -(R[r * 3 + 0] * eye[0] + R[r * 3 + 1] * eye[1] + R[r * 3 + 2] * eye[2]);
+        }
+        C.M[12] = C.M[13] = C.M[14] = 0.0;
+        C.M[15] = 1.0;
+        double xb[3];
+        for (int a = 0; a < 3; a++)
+            xb[a] = kf_coord_bound(G.mn[a], G.voxel, G.n[a]);
+        kf_camera_prepare(C, xb);
+        if (std::getenv("KF_CHECK_VERBOSE"))
+            std::fprintf(stderr, "camera %d: voxel %.4g grid %d %d %d eye %.3g m: c2min %.3g\n", cam, G.voxel, G.n[0],
+                         G.n[1], G.n[2], dist, C.c2min);
+        // random voxels
+        for (int t = 0; t < 3000; t++)
+            check_voxel(C, G, (int)(U(rng) * G.n[0]), (int)(U(rng) * G.n[1]), (int)(U(rng) * G.n[2]), c, true);
+        // voxels placed next to a pixel boundary: shift cx (then cy) so that this voxel's
+        // px + 0.5 sits delta from the half-integer nearest it, and re-prepare the camera
+        for (int t = 0; t < 40; t++) {
+            const int x = (int)(U(rng) * G.n[0]), y = (int)(U(rng) * G.n[1]), z = (int)(U(rng) * G.n[2]);
+            const double X = G.voxel * (x + 0.5) + G.mn[0];
+            const double Y = G.voxel * (y + 0.5) + G.mn[1];
+            const double Z = G.voxel * (z + 0.5) + G.mn[2];
+            double P[3][3];
+            kf_row_exact(C, Y, Z, P);
+            const double c2 = kf_cam(C, P[2], 2, X);
+            const double q0 = kf_cam(C, P[0], 0, X) / c2, q1 = kf_cam(C, P[1], 1, X) / c2;
+            for (int axis = 0; axis < 2; axis++) {
+                for (double d : deltas) {
+                    for (int sgn = -1; sgn <= 1; sgn += 2) {
+                        KfCamera D = C;
+                        const double s = axis == 0 ? C.K[0] * q0 + C.K[1] * q1 + C.K[2] + 0.5
+                                                   : C.K[3] * q0 + C.K[4] * q1 + C.K[5] + 0.5;
+                        const double h = std::floor(s) + 0.5;  // nearest half-integer above floor
+                        D.K[axis == 0 ? 2 : 5] += (h + sgn * d) - s;
+                        kf_camera_prepare(D, xb);
+                        check_voxel(D, G, x, y, z, c);
+                        if (x + 1 < G.n[0])
+                            check_voxel(D, G, x + 1, y, z, c);
+                    }
+                }
+            }
+        }
+    }
+    std::printf("%ld %ld %ld %ld %ld %ld\n", c.voxels, c.fast, c.bad, c.rvoxels, c.rfast, c.rowwise);
+    return 0;
+}
+
+int run_quot(unsigned long seed, long count)
+{
+    std::mt19937_64 rng(seed);
+    std::vector<float> scales = {0.0105f, 0.0125f, 0.0149f, 0.03f, 0.1f, 1.0f, 1.5f, 1.9999999f, 0.5f, 0.75f,
+                                 3e-30f, 1e-10f, 0.007000001f};
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    for (int k = 0; k < 40; k++)
+        scales.push_back((float)std::pow(2.0, -40.0 + 41.0 * U(rng)));
+    long values = 0, bad = 0;
+    auto check = [&](float v, float scale, double rs) {
+        values++;
+        const uint16_t b0 = StoreBF16::enc(v, scale), b1 = StoreBF16::enc_r(v, scale, rs);
+        const _Float16 h0 = StoreF16::enc(v, scale), h1 = StoreF16::enc_r(v, scale, rs);
+        uint16_t hb0, hb1;
+        std::memcpy(&hb0, &h0, 2);
+        std::memcpy(&hb1, &h1, 2);
+        const float q0 = v / scale, q1 = kf_quot(v, scale, rs);
+        uint32_t qb0, qb1;
+        std::memcpy(&qb0, &q0, 4);
+        std::memcpy(&qb1, &q1, 4);
+        const bool nan = q0 != q0;
+        if (b0 != b1 || (!nan && (hb0 != hb1 || qb0 != qb1))) {
+            if (bad < 5)
+                std::fprintf(stderr, "mismatch v %a scale %a: bf16 %x %x f16 %x %x\n", v, scale, b0, b1, hb0, hb1);
+            bad++;
+        }
+    };
+    for (float scale : scales) {
+        const double rs = kf_quot_rscale(scale);
+        if (rs == 0.0) {
+            std::fprintf(stderr, "scale %a has no reciprocal route\n", scale);
+            return 1;
+        }
+        for (float v : {0.0f, -0.0f, INFINITY, -INFINITY, NAN, 1.0f, -1.0f, scale, -scale, 0x1p-149f, 0x1p-126f,
+                        3.4028235e38f})
+            check(v, scale, rs);
+        for (long t = 0; t < count; t++) {
+            uint32_t bits = (uint32_t)rng();
+            float v;
+            std::memcpy(&v, &bits, 4);
+            check(v, scale, rs);
+            // values of the size an SDF holds (0.1 mm .. 1 m) and multiples of the scale
+            check((float)((U(rng) - 0.5) * std::pow(10.0, -4.0 + 4.0 * U(rng))), scale, rs);
+            check(scale * (float)(int)(U(rng) * 4096 - 2048) * 0x1p-10f, scale, rs);
+        }
+    }
+    std::printf("%ld %ld\n", values, bad);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc == 4 && !std::strcmp(argv[1], "project"))
+        return run_project(std::strtoul(argv[2], nullptr, 10), std::atoi(argv[3]));
+    if (argc == 4 && !std::strcmp(argv[1], "quot"))
+        return run_quot(std::strtoul(argv[2], nullptr, 10), std::atol(argv[3]));
+    std::fprintf(stderr, "usage: integrate_check project <seed> <cameras> | quot <seed> <count>\n");
+    return 2;
+}
