@@ -14,13 +14,24 @@ constexpr int BLOCK = 256;
 
 inline unsigned int blocks_for(int64_t n) { return (unsigned int)((n + BLOCK - 1) / BLOCK); }
 
+// SDF.cpp:133-134: phi = Max + eps, w = 0.  Four voxels per thread, phi and w each as one
+// streaming vector store (the volume is far larger than the L2).
 template <class S>
-__global__ void k_fill(typename S::T *phi, int32_t *w, int64_t n, float oob_m, float scale)
+__global__ void k_fill(typename S::T *phi, int32_t *w, int64_t n, typename S::T init)
 {
-    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) {
-        phi[i] = S::enc(oob_m, scale);
-        w[i] = 0;
+    const int64_t i0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * 4;
+    if (i0 + 4 <= n) {
+        typedef typename S::T vec_t __attribute__((ext_vector_type(4)));
+        typedef int32_t ivec_t __attribute__((ext_vector_type(4)));
+        const vec_t pv = {init, init, init, init};
+        const ivec_t wv = {0, 0, 0, 0};
+        __builtin_nontemporal_store(pv, reinterpret_cast<vec_t *>(phi + i0));
+        __builtin_nontemporal_store(wv, reinterpret_cast<ivec_t *>(w + i0));
+    } else {
+        for (int64_t i = i0; i < n; i++) {
+            phi[i] = init;
+            w[i] = 0;
+        }
     }
 }
 
@@ -48,10 +59,10 @@ __global__ void k_decode(float *dst, const typename S::T *phi, int64_t n, float 
 // FP64 division), or by kf_pixel_rowwise for cameras whose depth and image row stay fixed along
 // a grid row (ROWWISE: the reference's identity pose), and, where that route cannot vouch for
 // the reference's rounding, by kf_pixel_exact (kf_project.hpp); the camera depth c2 is the
-// reference's FP64 value either way.  The running average (old w + dist) / (w + 1) is dist itself when w = 0 and old is
-// finite (old 0 = +-0, +-0 + dist = dist for dist != 0 and +0 for dist = +0, dist / 1 = dist),
-// so that case skips the multiply, add and FP64 division.  The stores of a FRESH pass are one
-// vector store per thread for phi and for w.
+// reference's FP64 value either way.  The running average (old w + dist) / (w + 1) is dist
+// itself when w = 0 and old is finite (old 0 = +-0, +-0 + dist = dist for dist != 0 and +0 for
+// dist = +0, dist / 1 = dist), so that case skips the multiply, add and FP64 division.  The
+// stores of a FRESH pass are one streaming vector store per thread for phi and for w.
 constexpr int IPT = 4;
 
 template <class S>
@@ -128,6 +139,7 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(IntegrateArgs<S> A)
     const int64_t i0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * IPT;
     if (i0 >= g.n)
         return;
+    const bool whole = i0 + IPT <= g.n;
     int x, y, z;
     kf_xyz(g, i0, x, y, z);
     // project the group's voxels, then read their pixels together
@@ -198,7 +210,7 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(IntegrateArgs<S> A)
 #pragma unroll
     for (int k = 0; k < IPT; k++)
         dep[k] = A.depth[pr[k].pix >= 0 ? pr[k].pix : 0];
-    if (FRESH && i0 + IPT <= g.n) {
+    if (FRESH && whole) {
         // all IPT voxels written: phi and w as one vector store each
         typedef typename S::T vec_t __attribute__((ext_vector_type(IPT)));
         vec_t pv;
@@ -210,8 +222,9 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(IntegrateArgs<S> A)
             pv[k] = take ? S::enc_r((float)nv, A.scale, A.rscale) : A.init;
             wv[k] = take ? 1 : 0;
         }
-        *reinterpret_cast<vec_t *>(A.phi + i0) = pv;
-        *reinterpret_cast<int4 *>(A.w + i0) = make_int4(wv[0], wv[1], wv[2], wv[3]);
+        typedef int32_t ivec_t __attribute__((ext_vector_type(IPT)));
+        __builtin_nontemporal_store(pv, reinterpret_cast<vec_t *>(A.phi + i0));
+        __builtin_nontemporal_store(ivec_t{wv[0], wv[1], wv[2], wv[3]}, reinterpret_cast<ivec_t *>(A.w + i0));
         return;
     }
 #pragma unroll
@@ -319,29 +332,48 @@ __global__ void k_warp_scatter(const unsigned int *__restrict__ list, unsigned i
 
 // SDF::fuse(otherSdf), SDF.cpp:199-241: MERGE = FUSE_BY_MERGE (config.cpp:7), the weighted
 // average; otherwise the value nearer the surface wins with its own weight.
+// Four voxels per thread: the other volume's weights as one vector load, and a voxel only
+// where the other volume has weight (most voxels of a live frame have none).
 template <class S1, class S2, bool MERGE>
-__global__ void k_fuse(typename S1::T *phi1, int32_t *w1, float scale1, const typename S2::T *__restrict__ phi2,
-                       const int32_t *__restrict__ w2, float scale2, int64_t n)
+__global__ void k_fuse(typename S1::T *phi1, int32_t *w1, float scale1, double rscale1,
+                       const typename S2::T *__restrict__ phi2, const int32_t *__restrict__ w2, float scale2,
+                       int64_t n)
 {
-    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n)
+    const int64_t i0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * 4;
+    if (i0 >= n)
         return;
-    const int32_t b = w2[i];
-    if (b == 0)
-        return;
-    const int32_t a = w1[i];
-    const double d2 = (double)S2::load(phi2, i, scale2);
-    if (a == 0) {
-        phi1[i] = S1::enc((float)d2, scale1);
-        w1[i] = b;
-    } else if (MERGE) {
-        const double d1 = (double)S1::load(phi1, i, scale1);
-        const double nv = ((double)a * d1 + (double)b * d2) / (double)(a + b);
-        phi1[i] = S1::enc((float)nv, scale1);
-        w1[i] = a + b;
-    } else if (!(fabs((double)S1::load(phi1, i, scale1)) < fabs(d2))) {
-        phi1[i] = S1::enc((float)d2, scale1);
-        w1[i] = b;
+    int32_t bw[4];
+    if (i0 + 4 <= n) {
+        const int4 v = *reinterpret_cast<const int4 *>(w2 + i0);
+        bw[0] = v.x;
+        bw[1] = v.y;
+        bw[2] = v.z;
+        bw[3] = v.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            bw[k] = i0 + k < n ? w2[i0 + k] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int32_t b = bw[k];
+        if (b == 0)
+            continue;
+        const int64_t i = i0 + k;
+        const int32_t a = w1[i];
+        const double d2 = (double)S2::load(phi2, i, scale2);
+        if (a == 0) {
+            phi1[i] = S1::enc_r((float)d2, scale1, rscale1);
+            w1[i] = b;
+        } else if (MERGE) {
+            const double d1 = (double)S1::load(phi1, i, scale1);
+            const double nv = ((double)a * d1 + (double)b * d2) / (double)(a + b);
+            phi1[i] = S1::enc_r((float)nv, scale1, rscale1);
+            w1[i] = a + b;
+        } else if (!(fabs((double)S1::load(phi1, i, scale1)) < fabs(d2))) {
+            phi1[i] = S1::enc_r((float)d2, scale1, rscale1);
+            w1[i] = b;
+        }
     }
 }
 
@@ -413,8 +445,8 @@ kf_status kf_launch_fill_volume(kf_volume *v)
 {
     kf_touch(v);
     KF_DISPATCH_DTYPE(v->dtype, S, {
-        hipLaunchKernelGGL((k_fill<S>), dim3(blocks_for(v->n)), dim3(BLOCK), 0, v->ctx->stream,
-                           (typename S::T *)v->phi, v->w, v->n, (float)v->oob(), v->scale());
+        hipLaunchKernelGGL((k_fill<S>), dim3(blocks_for((v->n + 3) / 4)), dim3(BLOCK), 0, v->ctx->stream,
+                           (typename S::T *)v->phi, v->w, v->n, S::enc((float)v->oob(), v->scale()));
     });
     KF_LAUNCH_CHECK();
     return KF_OK;
@@ -504,13 +536,15 @@ kf_status kf_launch_fuse(kf_volume *can, const kf_volume *live, bool merge)
     kf_touch(can);
     KF_DISPATCH_DTYPE(can->dtype, S1, {
         KF_DISPATCH_DTYPE(live->dtype, S2, {
+            const unsigned int nb = blocks_for((can->n + 3) / 4);
+            const double rs = kf_quot_rscale(can->scale());
             if (merge)
-                hipLaunchKernelGGL((k_fuse<S1, S2, true>), dim3(blocks_for(can->n)), dim3(BLOCK), 0, can->ctx->stream,
-                                   (typename S1::T *)can->phi, can->w, can->scale(),
+                hipLaunchKernelGGL((k_fuse<S1, S2, true>), dim3(nb), dim3(BLOCK), 0, can->ctx->stream,
+                                   (typename S1::T *)can->phi, can->w, can->scale(), rs,
                                    (const typename S2::T *)live->phi, (const int32_t *)live->w, live->scale(), can->n);
             else
-                hipLaunchKernelGGL((k_fuse<S1, S2, false>), dim3(blocks_for(can->n)), dim3(BLOCK), 0,
-                                   can->ctx->stream, (typename S1::T *)can->phi, can->w, can->scale(),
+                hipLaunchKernelGGL((k_fuse<S1, S2, false>), dim3(nb), dim3(BLOCK), 0, can->ctx->stream,
+                                   (typename S1::T *)can->phi, can->w, can->scale(), rs,
                                    (const typename S2::T *)live->phi, (const int32_t *)live->w, live->scale(), can->n);
         });
     });
