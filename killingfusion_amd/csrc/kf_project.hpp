@@ -63,8 +63,12 @@ __device__ __host__ inline int kf_pixel_exact(const KfCamera &C, const double P[
     const double q0 = c0 / c2, q1 = c1 / c2;
     const double px = C.K[0] * q0 + C.K[1] * q1 + C.K[2] * 1.0;
     const double py = C.K[3] * q0 + C.K[4] * q1 + C.K[5] * 1.0;
-    const int col = (int)roundf((float)(px + 0.5));
-    const int row = (int)roundf((float)(py + 0.5));
+    const float fc = roundf((float)(px + 0.5)), fr = roundf((float)(py + 0.5));
+    // a coordinate that is NaN or beyond the int range lies outside the image (as the x86
+    // conversion the reference compiles to makes it: INT_MIN)
+    if (!(fabsf(fc) < 0x1p31f) || !(fabsf(fr) < 0x1p31f))
+        return -1;
+    const int col = (int)fc, row = (int)fr;
     return (col >= 0 && col < C.W && row >= 0 && row < C.H) ? row * C.W + col : -1;
 }
 
