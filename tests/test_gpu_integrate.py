@@ -10,7 +10,8 @@ takes the reference's FP64 projection elsewhere; the poses reach each of its rou
     and voxels behind it that still project into the image, which the reference integrates.
 Each pose runs as an update (SDF.integrateDepthFrame into a reset volume) and as the frame
 driver's fresh pass (frame 0 of kf_pipeline: reset and integrate in one write), in FP32, BF16
-and FP16 storage."""
+and FP16 storage.  Grids whose voxel count is not a multiple of four check the per-voxel tails
+of the reset, both integration passes and both fusion rules."""
 import numpy as np
 import pytest
 
@@ -90,3 +91,48 @@ def test_integrate_posed_bit_exact(ctx, name, dtype):
     phi, w = pl.canonical_download()
     check(phi, w, ophi, ow, dtype, f"fresh, {name}")
     pl.close()
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+@pytest.mark.parametrize("dims", [(61, 45, 58), (63, 47, 29), (13, 7, 5)])
+def test_uneven_grids_fill_integrate_fuse(ctx, dims, dtype):
+    """Grids whose voxel count leaves 2, 1 and 3 voxels past the last group of four: the reset
+    (SDF.cpp:133-134), the update and fresh integration, and both fusion rules (SDF.cpp:199-241)
+    take their per-voxel tails there."""
+    n = int(np.prod(dims))
+    print(f"case: uneven grid {dims} (n % 4 = {n % 4}), {dtype}: reset, integrate, fuse", flush=True)
+    mn = (-0.5 * dims[0] * C1["voxel"], -0.5 * dims[1] * C1["voxel"], 0.56)  # centred on the sphere
+    g = ogrid(dims, mn, C1["voxel"], C1["trunc"])
+    d1, d2 = sphere_frames()
+    args = (C1["voxel"], C1["trunc"], C1["trunc"])
+    v = SDF(ctx, dims, mn, *args, dtype=dtype)
+    phi, w = v.download()
+    ophi, ow = O.sdf_init(g)
+    check(phi, w, ophi, ow, dtype, "reset")
+    v.integrateDepthFrame(d1, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+    o1, w1 = oracle_frame(g, d1, scenes.C2W)
+    phi, w = v.download()
+    check(phi, w, o1, w1, dtype, "update")
+    assert (w1 > 0).sum() > min(1000, n // 2)
+    kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint", max_iters=1)
+    pl = KillingFusion(ctx, dims, mn, C1["voxel"], kp, scenes.K, min_depth=scenes.MIN_DEPTH,
+                       max_depth=scenes.MAX_DEPTH, dtype=dtype)
+    pl.processNextFrame(d1)
+    phi, w = pl.canonical_download()
+    check(phi, w, o1, w1, dtype, "fresh")
+    pl.close()
+    live = SDF(ctx, dims, mn, *args, dtype=dtype)
+    live.integrateDepthFrame(d2, scenes.C2W, scenes.K, scenes.MIN_DEPTH, scenes.MAX_DEPTH)
+    lphi, lw = live.download()
+    for merge in (True, False):
+        can = SDF(ctx, dims, mn, *args, dtype=dtype)
+        can.copy_from(v)
+        can.fuse(live, merge=merge)
+        cphi, cw = v.download()
+        ophi, ow = cphi.astype(np.float64), cw.astype(np.int64)
+        O.fuse(ophi, ow, lphi.astype(np.float64), lw.astype(np.int64), merge=merge)
+        phi, w = can.download()
+        check(phi, w, ophi, ow, dtype, f"fuse merge={merge}")
+        can.close()
+    live.close()
+    v.close()
