@@ -10,6 +10,7 @@
 //       and pushed one float ulp either way.  Prints "checks fast mismatches random random-fast
 //       rowwise": a mismatch is a voxel where a fast form claims its pixel (ok) and the pixel (or,
 //       for the rowwise form, the camera depth) differs from the exact path's.
+//   guards  the camera set-ups under which the fast forms stay off.
 //   quot <seed> <count>  StoreBF16 / StoreF16 enc_r (product by the double reciprocal) against
 //       enc (float division): every scale of a list, random float bit patterns (all exponents,
 //       subnormals, infinities, NaNs) and signed zeros.  Prints "values mismatches".
@@ -246,12 +247,51 @@ int run_quot(unsigned long seed, long count)
 
 }  // namespace
 
+// The conditions under which kf_camera_prepare keeps the fast forms off (every voxel then takes
+// the exact projection): images of 4096 or more columns or rows, non-finite intrinsics, pose or
+// coordinate bounds.  Prints "cases off": how many of the cases left them off.
+int run_guards()
+{
+    KfCamera base;
+    const double K[9] = {525, 0, 319.5, 0, 525, 239.5, 0, 0, 1};
+    for (int i = 0; i < 9; i++)
+        base.K[i] = K[i];
+    for (int i = 0; i < 16; i++)
+        base.M[i] = i % 5 == 0 ? 1.0 : 0.0;
+    base.W = 640;
+    base.H = 480;
+    const double xb[3] = {0.4, 0.4, 1.2};
+    int cases = 0, off = 0;
+    auto count = [&](KfCamera C, const double *b) {
+        kf_camera_prepare(C, b);
+        cases++;
+        off += !(C.c2min < INFINITY) && !kf_camera_rowwise(C);
+    };
+    KfCamera C = base;
+    kf_camera_prepare(C, xb);
+    if (!(C.c2min < 0.1) || !kf_camera_rowwise(C)) {
+        std::fprintf(stderr, "the identity camera should take the row form (c2min %g)\n", C.c2min);
+        return 1;
+    }
+    C = base; C.W = 4096; count(C, xb);
+    C = base; C.H = 5000; count(C, xb);
+    C = base; C.K[0] = NAN; count(C, xb);
+    C = base; C.K[2] = INFINITY; count(C, xb);
+    C = base; C.M[3] = NAN; count(C, xb);
+    const double bad[3] = {0.4, INFINITY, 1.2};
+    C = base; count(C, bad);
+    std::printf("%d %d\n", cases, off);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc == 2 && !std::strcmp(argv[1], "guards"))
+        return run_guards();
     if (argc == 4 && !std::strcmp(argv[1], "project"))
         return run_project(std::strtoul(argv[2], nullptr, 10), std::atoi(argv[3]));
     if (argc == 4 && !std::strcmp(argv[1], "quot"))
         return run_quot(std::strtoul(argv[2], nullptr, 10), std::atol(argv[3]));
-    std::fprintf(stderr, "usage: integrate_check project <seed> <cameras> | quot <seed> <count>\n");
+    std::fprintf(stderr, "usage: integrate_check project <seed> <cameras> | guards | quot <seed> <count>\n");
     return 2;
 }

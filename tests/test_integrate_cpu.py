@@ -5,7 +5,8 @@
   never claim a pixel other than kf_pixel_exact's (the reference's FP64 projection and float
   roundf, SDF.cpp:158-175), nor the rowwise form a camera depth other than the voxel's own, on
   random cameras and at voxels placed next to pixel boundaries, with the float reciprocal at the
-  ends of its error range; and they decide nearly every voxel themselves;
+  ends of its error range; and they decide nearly every voxel themselves, and stay off for
+  images of 4096 columns or rows and for non-finite camera or grid data;
 * enc_r, the BF16/FP16 encoding through a double reciprocal of the scale, equals enc (float
   division) bit for bit over random float bit patterns and SDF-sized values at many scales."""
 import os
@@ -41,6 +42,13 @@ def test_fast_pixel_never_disagrees(check_exe, seed):
     # the fast path must decide nearly every voxel of a random camera, or the kernel gains nothing
     # (the boundary-placed voxels are meant to fall back)
     assert rfast >= 0.95 * rvoxels
+
+
+def test_fast_forms_stay_off_where_unproven(check_exe):
+    r = subprocess.run([check_exe, "guards"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    cases, off = map(int, r.stdout.split())
+    assert cases == off == 6
 
 
 def test_encode_by_reciprocal_equals_division(check_exe):
