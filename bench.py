@@ -77,9 +77,15 @@ def read_profile_value(path, key):
 
 
 def step_bytes(s_phi, listed_fraction):
-    """Compulsory HBM bytes of one dense Jacobi step per voxel: u read and written (2 x 12 B),
-    phi_live read once (s_phi), phi_can read only where the gradient is evaluated -- the
-    voxels that can move (listed_fraction of the grid, KillingFusion.cpp:272-274)."""
+    """Compulsory HBM bytes of one dense Jacobi step per voxel: u read once (12 B) and phi_live
+    read once (s_phi) for every voxel; phi_can read and u written (s_phi + 12 B) only at the
+    voxels that can move (listed_fraction of the grid, KillingFusion.cpp:272-274) -- every other
+    voxel keeps its value, which the field's destination buffer already holds."""
+    return 12 + s_phi + (s_phi + 12) * listed_fraction
+
+
+def step_bytes_store_all(s_phi, listed_fraction):
+    """The same step with u written back at every voxel (24 B of u per voxel)."""
     return 24 + s_phi + s_phi * listed_fraction
 
 
@@ -457,15 +463,19 @@ def main(argv=None):
                      "traffic_gbs": traffic / (kernel_ms * 1e-3) / 1e9 if traffic else None,
                      "profile_kernel_us": profile_us,
                      "profile_frac": achieved_profile / HBM_PEAK_GBS if achieved_profile else None,
-                     "note": "algorithmic bytes per voxel-iteration = 24 (u read + written) + 2 (phi_live, "
-                             "bf16) + 2 x listed fraction (phi_can, read only where the gradient is "
-                             "evaluated)"},
+                     "note": "algorithmic bytes per voxel-iteration = 12 (u read) + 2 (phi_live, bf16) + "
+                             "(2 + 12) x listed fraction (phi_can read and u written only where a voxel can "
+                             "move; the step stores a 4-voxel group only where its destination buffer can "
+                             "differ, so elsewhere the field's other buffer already holds the value); the "
+                             "step's traffic is a read stream",
+                     "bytes_per_voxel_iteration_store_all": step_bytes_store_all(2, listed_fraction)},
         "cpu_baseline": cpu,
         "plain_fp32": {"value": plain_value, "ms_per_step": wall_p * 1e3 / plain_steps, "steps": plain_steps,
                        "kernel_ms": plain_kernel_ms,
                        "hbm_gbs_at_32B": 32 * nvox / (plain_kernel_ms * 1e-3) / 1e9},
         "adjoint_fp32": {"value": nvox * a.steps / wall32, "kernel_ms": gpu_ms32 / a.steps,
-                         "hbm_gbs_at_32B": 32 * nvox / (gpu_ms32 / a.steps * 1e-3) / 1e9},
+                         "bytes_per_voxel_iteration": step_bytes(4, listed_fraction),
+                         "achieved_gbs": step_bytes(4, listed_fraction) * nvox / (gpu_ms32 / a.steps * 1e-3) / 1e9},
         "plain_fp32_active": {"value": plain_active_value, "ms_per_step": wall_pa * 1e3 / plain_steps,
                               "gpu_ms_per_step": gpu_ms_pa / plain_steps, "listed_voxels": int(st_pa["listed"]),
                               "note": "the plain reference-formula step over the voxels that can move (listed "

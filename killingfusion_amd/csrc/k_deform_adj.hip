@@ -16,7 +16,8 @@
 //
 // One pass per iteration reads u_j, phi_can_j, the 8 live-SDF corners around q_j and the
 // 1-neighbourhood of u (12 neighbours), and writes u_j - alpha dE/du_j for active voxels.
-// Compulsory HBM traffic per voxel-iteration: 2 s_phi + 24 bytes.
+// Compulsory HBM traffic per voxel-iteration: 12 + s_phi bytes (u and phi_live read), plus
+// 12 + s_phi where a voxel can move (its u written, phi_can read).
 //
 // Three launch shapes share the per-voxel arithmetic (adj_voxel):
 //  * k_adj_flat (Jacobi steps): a thread owns 4 consecutive voxels of the flat index.  A step
@@ -348,13 +349,25 @@ constexpr int FLAT = FB * VEC * FPT;   // voxels per block
 // UNAL: N is not a multiple of 4, so the u planes a n (a = 1, 2) start off a 16-byte boundary.
 // Their float4 groups are then read and written as 4-byte-aligned 16-byte accesses (one load
 // or store each on this target), and the one group that runs past N goes element by element.
-// (A 255^3 grid: 73 us per launch, against 59 us at 256^3, 108 us with one voxel per thread and
-// 256 us for k_adj_any's full stencil per voxel.)
+// (A 255^3 grid, with a store per voxel: 73 us per launch, against 59 us at 256^3, 108 us with
+// one voxel per thread and 256 us for k_adj_any's full stencil per voxel.)
+//
+// Stores.  The destination buffer of launch k already holds u_{k-1}: launch k - 1 read it as
+// its source, and every voxel outside the need set of launch k - 1 (active, or already moved)
+// kept its value there, so u_k = u_{k-1} at those voxels.  A thread therefore stores its
+// group's copy only when one of its voxels was in launch k - 1's need set (need words: one
+// ballot per voxel of the group, four 64-bit words per wavefront, written by the same
+// wavefront of the previous launch), or when `full` is set (the first launch of a deform call,
+// whose destination holds anything); active voxels are stored by the stencil below as before.
+// Every launch leaves its destination equal to u_{k+1} at every voxel, bit for bit as with a
+// store per voxel.  (256^3 BF16: 44.8 us per launch, against 57 us with a store per voxel; the
+// launch then streams u and phi_live once and writes only the band's groups.)
 template <class S, bool UNAL>
 __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, AdjConsts c, DeformState *st,
                                                     BlockPartial *partials, int parity, const BlockPartial *prev,
-                                                    unsigned int prev_nb)
+                                                    unsigned int prev_nb, unsigned long long *__restrict__ needw,
+                                                    int full)
 {
     if (!step_gate<KF_MODE_STEP, FB>(st, prev, prev_nb, parity))
         return;
@@ -367,6 +380,15 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
     // backwards (to meet the previous launch's last writes in the last-level cache) no longer
     // pays: it measured 58.9 against 58.1 us at 256^3.
     const unsigned int tile = kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x);
+    // this wavefront's need words (FPT VEC per wavefront, ballot b = voxel b of each lane's group)
+    static_assert(FPT * VEC == 4, "four need words per wavefront");
+    const size_t nw = ((size_t)tile * (FB / 64) + (threadIdx.x >> 6)) * 4;
+    bool store_group = full != 0;
+    if (!full) {
+        const ulonglong2 w01 = *reinterpret_cast<const ulonglong2 *>(needw + nw);
+        const ulonglong2 w23 = *reinterpret_cast<const ulonglong2 *>(needw + nw + 2);
+        store_group = (((w01.x | w01.y | w23.x | w23.y) >> (threadIdx.x & 63)) & 1ull) != 0ull;
+    }
     // group q of thread t: voxels tile FLAT + q FB VEC + t VEC .. + 3 (whole or outside unless
     // UNAL, where the last one may be partial)
     unsigned int i0[FPT];
@@ -411,20 +433,6 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
         if (!valid[q])
             continue;
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-            float *dst = u_out + ((unsigned int)a * n32 + i0[q]);
-            if (!UNAL) {
-                *reinterpret_cast<float4 *>(dst) = C[q][a];
-            } else if (nin[q] == (unsigned int)VEC) {
-                *reinterpret_cast<kf_f4u *>(dst) = kf_f4u{C[q][a].x, C[q][a].y, C[q][a].z, C[q][a].w};
-            } else {
-#pragma unroll
-                for (int k = 0; k < VEC; k++)
-                    if ((unsigned int)k < nin[q])
-                        dst[k] = f4(C[q][a], k);
-            }
-        }
-#pragma unroll
         for (int k = 0; k < VEC; k++) {
             if (UNAL && (unsigned int)k >= nin[q])
                 continue;
@@ -443,6 +451,25 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
         bal[b] = __ballot((need >> b) & 1u);
         total += __popcll(bal[b]);
     }
+#pragma unroll
+    for (int q = 0; q < FPT; q++) {
+        if (!valid[q] || !store_group)
+            continue;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float *dst = u_out + ((unsigned int)a * n32 + i0[q]);
+            if (!UNAL) {
+                *reinterpret_cast<float4 *>(dst) = C[q][a];
+            } else if (nin[q] == (unsigned int)VEC) {
+                *reinterpret_cast<kf_f4u *>(dst) = kf_f4u{C[q][a].x, C[q][a].y, C[q][a].z, C[q][a].w};
+            } else {
+#pragma unroll
+                for (int k = 0; k < VEC; k++)
+                    if ((unsigned int)k < nin[q])
+                        dst[k] = f4(C[q][a], k);
+            }
+        }
+    }
     double mx = 0.0, esum = 0.0;
     unsigned long long act = 0;
     const unsigned int wave0 = tile * (unsigned int)FLAT + (threadIdx.x & ~63u) * (unsigned int)VEC;
@@ -454,6 +481,10 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
         int x, y, z;
         g.xyz32(i, x, y, z);
         adj_job<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, &st->div_slot[parity], nullptr, x, y, z, mx, esum, act);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        *reinterpret_cast<ulonglong2 *>(needw + nw) = make_ulonglong2(bal[0], bal[1]);
+        *reinterpret_cast<ulonglong2 *>(needw + nw + 2) = make_ulonglong2(bal[2], bal[3]);
     }
     kf_block_publish<FB>(partials, blockIdx.x, esum, mx, act);
 }
@@ -483,15 +514,15 @@ __global__ __launch_bounds__(BLOCK) void k_adj_mask(const float *__restrict__ u,
 template <class S, int MODE>
 void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float *u_out, const AdjConsts &c,
             kf_context *ctx, float *e_voxel, bool rows, bool flat1, unsigned int nb, int nseg, int nyt,
-            const unsigned int *mask_in, int parity, const BlockPartial *prev, unsigned int prev_nb)
+            const unsigned int *mask_in, int parity, const BlockPartial *prev, unsigned int prev_nb, int full)
 {
     BlockPartial *parts = kf_partials(ctx, parity);
     if (MODE == KF_MODE_STEP && rows)
         hipLaunchKernelGGL((k_adj_flat<S, false>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out, c,
-                           ctx->state, parts, parity, prev, prev_nb);
+                           ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
     else if (MODE == KF_MODE_STEP && flat1)
         hipLaunchKernelGGL((k_adj_flat<S, true>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out, c,
-                           ctx->state, parts, parity, prev, prev_nb);
+                           ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
     else if (rows && L.g.nx % VEC == 0)
         hipLaunchKernelGGL((k_adj_rows<S, MODE, false>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out,
                            c, ctx->state, parts, e_voxel, nseg, nyt, mask_in, parity, prev, prev_nb);
@@ -553,6 +584,8 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
                             : rows  ? (unsigned int)(nseg * nyt * live->dims[2])
                                     : (unsigned int)((live->n + BLOCK - 1) / BLOCK);
     kf_status s = kf_ensure_partials(ctx, nb);
+    if (s == KF_OK && (flat || flat1))
+        s = kf_ensure_need_words(ctx, (size_t)nb * (FB / 64) * 4);  // four words per wavefront
     if (s != KF_OK)
         return s;
     if (nb_out)
@@ -566,13 +599,13 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
         PhiRead<S> L = make_read<S>(live), Cn = make_read<S>(can);
         if (mode == KF_MODE_GRAD)
             launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, false, nb, nseg, nyt, mask_in, 0, prev,
-                                    0u);
+                                    0u, 1);
         else if (mode == KF_MODE_ENERGY)
             launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, false, nb, nseg, nyt, mask_in, 0, prev,
-                                      0u);
-        else
+                                      0u, 1);
+        else  // the first step of a deform call stores every voxel
             launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, flat1, nb, nseg, nyt, nullptr, parity,
-                                    prev, prev_nb);
+                                    prev, prev_nb, iter == 0 ? 1 : 0);
     });
     KF_LAUNCH_CHECK();
     return mode == KF_MODE_ENERGY ? kf_launch_finalize(ctx, nb, 0) : KF_OK;

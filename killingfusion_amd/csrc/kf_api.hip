@@ -139,6 +139,21 @@ kf_status kf_ensure_partials(kf_context *ctx, size_t nblocks)
     return KF_OK;
 }
 
+kf_status kf_ensure_need_words(kf_context *ctx, size_t words)
+{
+    if (words <= ctx->need_cap)
+        return KF_OK;
+    if (ctx->need_words) {
+        KF_HIP(hipStreamSynchronize(ctx->stream));
+        KF_HIP(hipFree(ctx->need_words));
+        ctx->need_words = nullptr;
+        ctx->need_cap = 0;
+    }
+    KF_HIP(hipMalloc((void **)&ctx->need_words, sizeof(unsigned long long) * words));
+    ctx->need_cap = words;
+    return KF_OK;
+}
+
 namespace {
 
 constexpr int FIN_BLOCK = 1024;
@@ -295,6 +310,8 @@ kf_status kf_context_destroy(kf_context *ctx)
         (void)hipHostFree(ctx->state_host);
     if (ctx->partials)
         (void)hipFree(ctx->partials);
+    if (ctx->need_words)
+        (void)hipFree(ctx->need_words);
     kf_active_release(ctx);
     kf_ref_release(ctx);
     kf_batch_release(ctx);

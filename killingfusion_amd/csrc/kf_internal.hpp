@@ -184,6 +184,8 @@ struct kf_context {
     DeformState *state_host = nullptr;  // pinned mirror for stats readback
     BlockPartial *partials = nullptr;  // device, two sets (launch parity) of one slot per block
     size_t partials_cap = 0;           // slots per set
+    unsigned long long *need_words = nullptr;  // device: the dense step's need bits of its last launch
+    size_t need_cap = 0;                       // words
     int num_cus = 256;
     bool capture_loops = false;              // kf_context_set_loop_capture
     struct kf_batch_cache *batch = nullptr;  // batched-deform workspace, tuned shapes, captured graph
@@ -518,6 +520,7 @@ __device__ __host__ inline R kf_interp3(R v000, R v001, R v010, R v011, R v100, 
 // ---------------------------------------------------------------------------
 
 kf_status kf_ensure_partials(kf_context *ctx, size_t nblocks);
+kf_status kf_ensure_need_words(kf_context *ctx, size_t words);
 bool kf_valid_params(const kf_params *p);
 kf_status kf_check_triplet(const kf_volume *live, const kf_volume *can, const kf_field *u);
 void kf_batch_release(kf_context *ctx);

@@ -240,3 +240,26 @@ def test_list_longer_than_predicted(ctx, sphere):
     assert_same(run(ctx, live, can, kp, "dense", box), long_a, "long list after a short one")
     again = run(ctx, live, can, kp, "active")
     assert_same(short_d, again, "short list after a long one")
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "f32"])
+def test_dense_calls_split_on_one_field(ctx, sphere, dtype):
+    """The dense step stores a voxel group only where its destination buffer can differ from
+    the new value (the groups the stencil reached in the step before), and stores every group
+    in the first step of a call, whatever the field's other buffer holds.  Twelve iterations
+    as one call, as three calls on the same field, and with an active-set call (which leaves
+    the other buffer stale) between dense ones, give the same field bit for bit."""
+    print(f"case: dense deform split over calls on one field, {dtype}", flush=True)
+    live, can = sphere[dtype]
+    kp = KillingParams.paper(C1["voxel"], 3.0, method="adjoint")
+    whole, _ = run(ctx, live, can, dataclasses.replace(kp, max_iters=12), "dense")
+    f = DisplacementField(ctx, live.dims)
+    try:
+        plans = ((3, "dense"), (4, "dense"), (5, "dense"), (3, "dense"), (4, "active"), (5, "dense"))
+        for start in (0, 3):
+            f.initializeAllVoxels((0.0, 0.0, 0.0))
+            for iters, sweep in plans[start:start + 3]:
+                compute_displacement_field(live, can, f, dataclasses.replace(kp, max_iters=iters, sweep=sweep))
+            assert same_bits(f.download(), whole), f"split {plans[start:start + 3]}: fields differ"
+    finally:
+        f.close()
