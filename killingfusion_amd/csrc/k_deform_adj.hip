@@ -20,10 +20,10 @@
 // 12 + s_phi where a voxel can move (its u written, phi_can read).
 //
 // Three launch shapes share the per-voxel arithmetic (adj_voxel):
-//  * k_adj_flat (Jacobi steps): a thread owns 4 consecutive voxels of the flat index.  A step
-//    moves only active voxels: a voxel whose own u is zero samples phi_live[j] exactly, so u,
-//    phi_live and u_out stream as 16-byte / 8-byte accesses and only active voxels (and moved
-//    ones, to find out) run the stencil.
+//  * k_adj_flat (Jacobi steps): a thread owns two groups of 4 consecutive voxels of the flat
+//    index, 1024 voxels apart.  A step moves only active voxels: a voxel whose own u is zero
+//    samples phi_live[j] exactly, so u, phi_live and u_out stream as 16-byte / 8-byte accesses
+//    and only active voxels (and moved ones, to find out) run the stencil.
 //  * k_adj_rows (energy and gradient passes): a thread owns 4 consecutive x voxels of a row, a
 //    wavefront 256 voxels of a row, a block 4 rows of one slice.  These passes need every voxel
 //    whose stencil sees a nonzero u, and take that set from a bit mask over the 4-voxel groups
@@ -337,12 +337,13 @@ __global__ __launch_bounds__(BLOCK) void k_adj_rows(PhiRead<S> L, PhiRead<S> Cn,
 }
 
 // ---------------------------------------------------------------------------
-// Jacobi step over the flat voxel index: a thread owns 4 consecutive voxels (one float4 of each
-// u plane), a block 1024 consecutive voxels, whatever the row length -- a step's streaming part
-// is the same for every voxel, so rows that are not a multiple of the wavefront's width leave
-// no lanes idle.  (x, y, z) is recovered only for the compacted stencil jobs.
+// Jacobi step over the flat voxel index: a thread owns FPT groups of 4 consecutive voxels (one
+// float4 of each u plane each, FB VEC voxels apart), a block FLAT consecutive voxels, whatever
+// the row length -- a step's streaming part is the same for every voxel, so rows that are not a
+// multiple of the wavefront's width leave no lanes idle.  (x, y, z) is recovered only for the
+// compacted stencil jobs.
 // ---------------------------------------------------------------------------
-constexpr int FPT = 1;                 // float4 groups per thread
+constexpr int FPT = 2;                 // float4 groups per thread
 constexpr int FB = BLOCK;              // threads per block
 constexpr int FLAT = FB * VEC * FPT;   // voxels per block
 
@@ -356,7 +357,7 @@ constexpr int FLAT = FB * VEC * FPT;   // voxels per block
 // its source, and every voxel outside the need set of launch k - 1 (active, or already moved)
 // kept its value there, so u_k = u_{k-1} at those voxels.  A thread therefore stores its
 // group's copy only when one of its voxels was in launch k - 1's need set (need words: one
-// ballot per voxel of the group, four 64-bit words per wavefront, written by the same
+// ballot per voxel of each group, FPT VEC 64-bit words per wavefront, written by the same
 // wavefront of the previous launch), or when `full` is set (the first launch of a deform call,
 // whose destination holds anything); active voxels are stored by the stencil below as before.
 // Every launch leaves its destination equal to u_{k+1} at every voxel, bit for bit as with a
@@ -380,14 +381,18 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
     // backwards (to meet the previous launch's last writes in the last-level cache) no longer
     // pays: it measured 58.9 against 58.1 us at 256^3.
     const unsigned int tile = kf_xcd_chunk_tile<64>(blockIdx.x, gridDim.x);
-    // this wavefront's need words (FPT VEC per wavefront, ballot b = voxel b of each lane's group)
-    static_assert(FPT * VEC == 4, "four need words per wavefront");
-    const size_t nw = ((size_t)tile * (FB / 64) + (threadIdx.x >> 6)) * 4;
-    bool store_group = full != 0;
-    if (!full) {
-        const ulonglong2 w01 = *reinterpret_cast<const ulonglong2 *>(needw + nw);
-        const ulonglong2 w23 = *reinterpret_cast<const ulonglong2 *>(needw + nw + 2);
-        store_group = (((w01.x | w01.y | w23.x | w23.y) >> (threadIdx.x & 63)) & 1ull) != 0ull;
+    // this wavefront's need words (FPT VEC of them: word q VEC + k = voxel k of each lane's group q)
+    constexpr int NWW = FPT * VEC;
+    const size_t nw = ((size_t)tile * (FB / 64) + (threadIdx.x >> 6)) * NWW;
+    bool store_q[FPT];
+#pragma unroll
+    for (int q = 0; q < FPT; q++) {
+        store_q[q] = full != 0;
+        if (!full) {
+            const ulonglong2 w01 = *reinterpret_cast<const ulonglong2 *>(needw + nw + q * 4);
+            const ulonglong2 w23 = *reinterpret_cast<const ulonglong2 *>(needw + nw + q * 4 + 2);
+            store_q[q] = (((w01.x | w01.y | w23.x | w23.y) >> (threadIdx.x & 63)) & 1ull) != 0ull;
+        }
     }
     // group q of thread t: voxels tile FLAT + q FB VEC + t VEC .. + 3 (whole or outside unless
     // UNAL, where the last one may be partial)
@@ -453,7 +458,7 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
     }
 #pragma unroll
     for (int q = 0; q < FPT; q++) {
-        if (!valid[q] || !store_group)
+        if (!valid[q] || !store_q[q])
             continue;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
@@ -483,8 +488,9 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
         adj_job<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, &st->div_slot[parity], nullptr, x, y, z, mx, esum, act);
     }
     if ((threadIdx.x & 63) == 0) {
-        *reinterpret_cast<ulonglong2 *>(needw + nw) = make_ulonglong2(bal[0], bal[1]);
-        *reinterpret_cast<ulonglong2 *>(needw + nw + 2) = make_ulonglong2(bal[2], bal[3]);
+#pragma unroll
+        for (int b = 0; b < NWW; b += 2)
+            *reinterpret_cast<ulonglong2 *>(needw + nw + b) = make_ulonglong2(bal[b], bal[b + 1]);
     }
     kf_block_publish<FB>(partials, blockIdx.x, esum, mx, act);
 }
@@ -585,7 +591,7 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
                                     : (unsigned int)((live->n + BLOCK - 1) / BLOCK);
     kf_status s = kf_ensure_partials(ctx, nb);
     if (s == KF_OK && (flat || flat1))
-        s = kf_ensure_need_words(ctx, (size_t)nb * (FB / 64) * 4);  // four words per wavefront
+        s = kf_ensure_need_words(ctx, (size_t)nb * (FB / 64) * (FPT * VEC));  // a word per ballot
     if (s != KF_OK)
         return s;
     if (nb_out)
