@@ -23,15 +23,19 @@
 //  * k_adj_flat (Jacobi steps): a thread owns two groups of 4 consecutive voxels of the flat
 //    index, 1024 voxels apart.  A step moves only active voxels: a voxel whose own u is zero
 //    samples phi_live[j] exactly, so u, phi_live and u_out stream as 16-byte / 8-byte accesses
-//    and only active voxels (and moved ones, to find out) run the stencil.
+//    and only active voxels (and moved ones, to find out) run the stencil.  Grids of
+//    3 N >= 2^30 take its form with 64-bit offsets.
 //  * k_adj_rows (energy and gradient passes): a thread owns 4 consecutive x voxels of a row, a
 //    wavefront 256 voxels of a row, a block 4 rows of one slice.  These passes need every voxel
 //    whose stencil sees a nonzero u, and take that set from a bit mask over the 4-voxel groups
 //    of each row.
-//  * k_adj_any (any shape): one voxel per thread, neighbours read individually.
+//  * k_adj_any (energy and gradient passes on grids without a mask, 3 N >= 2^30): one voxel
+//    per thread, neighbours read individually, 64-bit offsets.
 // Blocks go to the XCDs in chunks of 64 consecutive tiles dealt round-robin: neighbouring
 // tiles of a chunk share an XCD's L2, and the band's stencils spread over every XCD.
 #include "kf_adjoint.hpp"
+
+#include <type_traits>
 
 namespace {
 
@@ -43,6 +47,63 @@ constexpr int ROWS = BLOCK / 64 * RPT;  // rows per block (a row tile)
 // ---------------------------------------------------------------------------
 // Any shape: one voxel per thread.
 // ---------------------------------------------------------------------------
+
+// The full stencil of voxel i = (x, y, z) with 64-bit offsets (any grid): as adj_job, which
+// indexes with 32-bit offsets.  A step writes the moved u_i of an active voxel or, with
+// WRITE_ALL, the new u_i of every voxel.
+template <class S, int MODE, bool WRITE_ALL>
+__device__ inline void adj_job_wide(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *__restrict__ u_in,
+                                    float *__restrict__ u_out, const AdjConsts &c, long long *div,
+                                    float *__restrict__ e_voxel, int64_t i, int x, int y, int z, double &mx,
+                                    double &esum, unsigned long long &act)
+{
+    const GridDims &g = L.g;
+    Nbhd nb;
+    nb.uc[0] = u_in[i];
+    nb.uc[1] = u_in[g.n + i];
+    nb.uc[2] = u_in[2 * g.n + i];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        nb.up[0][a] = uval(u_in, g, a, x + 1, y, z);
+        nb.um[0][a] = uval(u_in, g, a, x - 1, y, z);
+        nb.up[1][a] = uval(u_in, g, a, x, y + 1, z);
+        nb.um[1][a] = uval(u_in, g, a, x, y - 1, z);
+        nb.up[2][a] = uval(u_in, g, a, x, y, z + 1);
+        nb.um[2][a] = uval(u_in, g, a, x, y, z - 1);
+    }
+    nb.dgl[0][1] = uval(u_in, g, 1, x + 1, y - 1, z);
+    nb.dgl[0][2] = uval(u_in, g, 2, x + 1, y, z - 1);
+    nb.dgl[1][0] = uval(u_in, g, 0, x - 1, y + 1, z);
+    nb.dgl[1][2] = uval(u_in, g, 2, x, y + 1, z - 1);
+    nb.dgl[2][0] = uval(u_in, g, 0, x - 1, y, z + 1);
+    nb.dgl[2][1] = uval(u_in, g, 1, x, y - 1, z + 1);
+    int bx, by, bz;
+    float fx, fy, fz, v[8];
+    kf_split(x, nb.uc[0], bx, fx);
+    kf_split(y, nb.uc[1], by, fy);
+    kf_split(z, nb.uc[2], bz, fz);
+    L.cell(bx, by, bz, v);
+    VoxelOut o;
+    adj_voxel<MODE == KF_MODE_ENERGY>(c, v, fx, fy, fz, c.use_data ? Cn.at(x, y, z) : 0.f, nb, x, y, z, o);
+    if (o.active)
+        act += 1;
+    esum += (double)o.e;
+    if (MODE == KF_MODE_GRAD) {
+        u_out[i] = o.g[0];
+        u_out[g.n + i] = o.g[1];
+        u_out[2 * g.n + i] = o.g[2];
+    } else if (MODE == KF_MODE_ENERGY) {
+        if (e_voxel)
+            e_voxel[i] = o.e;
+    } else if (o.active || WRITE_ALL) {
+        float n[3];
+        adj_update(c, o, nb.uc, i, div, mx, n);
+        u_out[i] = n[0];
+        u_out[g.n + i] = n[1];
+        u_out[2 * g.n + i] = n[2];
+    }
+}
+
 template <class S, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_adj_any(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
                                                    float *__restrict__ u_out, AdjConsts c, DeformState *st,
@@ -58,50 +119,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_any(PhiRead<S> L, PhiRead<S> Cn, 
     if (i < g.n) {
         int x, y, z;
         kf_xyz(g, i, x, y, z);
-        Nbhd nb;
-        nb.uc[0] = u_in[i];
-        nb.uc[1] = u_in[g.n + i];
-        nb.uc[2] = u_in[2 * g.n + i];
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            nb.up[0][a] = uval(u_in, g, a, x + 1, y, z);
-            nb.um[0][a] = uval(u_in, g, a, x - 1, y, z);
-            nb.up[1][a] = uval(u_in, g, a, x, y + 1, z);
-            nb.um[1][a] = uval(u_in, g, a, x, y - 1, z);
-            nb.up[2][a] = uval(u_in, g, a, x, y, z + 1);
-            nb.um[2][a] = uval(u_in, g, a, x, y, z - 1);
-        }
-        nb.dgl[0][1] = uval(u_in, g, 1, x + 1, y - 1, z);
-        nb.dgl[0][2] = uval(u_in, g, 2, x + 1, y, z - 1);
-        nb.dgl[1][0] = uval(u_in, g, 0, x - 1, y + 1, z);
-        nb.dgl[1][2] = uval(u_in, g, 2, x, y + 1, z - 1);
-        nb.dgl[2][0] = uval(u_in, g, 0, x - 1, y, z + 1);
-        nb.dgl[2][1] = uval(u_in, g, 1, x, y - 1, z + 1);
-        int bx, by, bz;
-        float fx, fy, fz, v[8];
-        kf_split(x, nb.uc[0], bx, fx);
-        kf_split(y, nb.uc[1], by, fy);
-        kf_split(z, nb.uc[2], bz, fz);
-        L.cell(bx, by, bz, v);
-        VoxelOut o;
-        adj_voxel<MODE == KF_MODE_ENERGY>(c, v, fx, fy, fz, c.use_data ? Cn.at(x, y, z) : 0.f, nb, x, y, z, o);
-        if (o.active)
-            act = 1;
-        if (MODE == KF_MODE_GRAD) {
-            u_out[i] = o.g[0];
-            u_out[g.n + i] = o.g[1];
-            u_out[2 * g.n + i] = o.g[2];
-        } else if (MODE == KF_MODE_ENERGY) {
-            if (e_voxel)
-                e_voxel[i] = o.e;
-        } else {
-            float n[3];
-            adj_update(c, o, nb.uc, i, &st->div_slot[parity], mx, n);
-            u_out[i] = n[0];
-            u_out[g.n + i] = n[1];
-            u_out[2 * g.n + i] = n[2];
-        }
-        esum = (double)o.e;
+        adj_job_wide<S, MODE, true>(L, Cn, u_in, u_out, c, &st->div_slot[parity], e_voxel, i, x, y, z, mx, esum,
+                                    act);
     }
     if (MODE != KF_MODE_GRAD)
         kf_block_publish<BLOCK>(partials, blockIdx.x, esum, mx, act);
@@ -363,7 +382,10 @@ constexpr int FLAT = FB * VEC * FPT;   // voxels per block
 // Every launch leaves its destination equal to u_{k+1} at every voxel, bit for bit as with a
 // store per voxel.  (256^3 BF16: 44.8 us per launch, against 57 us with a store per voxel; the
 // launch then streams u and phi_live once and writes only the band's groups.)
-template <class S, bool UNAL>
+//
+// WIDE: 3 N >= 2^30, so plane offsets and voxel indices are 64-bit and the stencils run
+// adj_job_wide; everything else is the same.
+template <class S, bool UNAL, bool WIDE>
 __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, AdjConsts c, DeformState *st,
                                                     BlockPartial *partials, int parity, const BlockPartial *prev,
@@ -372,8 +394,9 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
 {
     if (!step_gate<KF_MODE_STEP, FB>(st, prev, prev_nb, parity))
         return;
+    using Ix = typename std::conditional<WIDE, unsigned long long, unsigned int>::type;
     const GridDims g = L.g;
-    const unsigned int n32 = nvox32(g);
+    const Ix n32 = (Ix)g.n;
     // Tiles go to the XCDs in chunks of 64 (a z-slice at 256^3) dealt round-robin, so the
     // stencils of the surface band spread over every XCD instead of keeping the two or three
     // that hold the band busy after the others finish (256^3: 70 -> 58 us per launch).  With
@@ -396,16 +419,17 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
     }
     // group q of thread t: voxels tile FLAT + q FB VEC + t VEC .. + 3 (whole or outside unless
     // UNAL, where the last one may be partial)
-    unsigned int i0[FPT];
+    Ix i0[FPT];
     bool valid[FPT];
     float4 C[FPT][3];
     float live4[FPT][VEC];
     unsigned int nin[FPT];  // voxels of the group inside the grid
 #pragma unroll
     for (int q = 0; q < FPT; q++) {
-        i0[q] = tile * (unsigned int)FLAT + (unsigned int)q * (FB * VEC) + threadIdx.x * (unsigned int)VEC;
+        i0[q] = (Ix)tile * (Ix)FLAT + (Ix)(q * (FB * VEC)) + (Ix)(threadIdx.x * VEC);
         valid[q] = i0[q] < n32;
-        nin[q] = !valid[q] ? 0u : (UNAL && n32 - i0[q] < (unsigned int)VEC ? n32 - i0[q] : (unsigned int)VEC);
+        nin[q] = !valid[q] ? 0u
+                 : (UNAL && n32 - i0[q] < (Ix)VEC ? (unsigned int)(n32 - i0[q]) : (unsigned int)VEC);
 #pragma unroll
         for (int k = 0; k < VEC; k++)
             live4[q][k] = 0.f;
@@ -413,8 +437,8 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
 #pragma unroll
             for (int a = 0; a < 3; a++)
                 C[q][a] = !valid[q] ? make_float4(0.f, 0.f, 0.f, 0.f)
-                          : UNAL    ? kf_load_stream4u(u_in + ((unsigned int)a * n32 + i0[q]))
-                                    : kf_load_stream4(u_in + ((unsigned int)a * n32 + i0[q]));
+                          : UNAL    ? kf_load_stream4u(u_in + ((Ix)a * n32 + i0[q]))
+                                    : kf_load_stream4(u_in + ((Ix)a * n32 + i0[q]));
             if (valid[q])
                 S::load4(L.p, (int64_t)i0[q], L.scale, live4[q]);
         } else {
@@ -423,9 +447,9 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
             for (int k = 0; k < VEC; k++) {
 #pragma unroll
                 for (int a = 0; a < 3; a++)
-                    t[a][k] = (unsigned int)k < nin[q] ? u_in[(unsigned int)a * n32 + i0[q] + (unsigned int)k] : 0.f;
+                    t[a][k] = (unsigned int)k < nin[q] ? u_in[(Ix)a * n32 + i0[q] + (Ix)k] : 0.f;
                 if ((unsigned int)k < nin[q])
-                    live4[q][k] = S::load(L.p, (int64_t)(i0[q] + (unsigned int)k), L.scale);
+                    live4[q][k] = S::load(L.p, (int64_t)(i0[q] + (Ix)k), L.scale);
             }
 #pragma unroll
             for (int a = 0; a < 3; a++)
@@ -462,7 +486,7 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
             continue;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            float *dst = u_out + ((unsigned int)a * n32 + i0[q]);
+            float *dst = u_out + ((Ix)a * n32 + i0[q]);
             if (!UNAL) {
                 *reinterpret_cast<float4 *>(dst) = C[q][a];
             } else if (nin[q] == (unsigned int)VEC) {
@@ -477,15 +501,21 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
     }
     double mx = 0.0, esum = 0.0;
     unsigned long long act = 0;
-    const unsigned int wave0 = tile * (unsigned int)FLAT + (threadIdx.x & ~63u) * (unsigned int)VEC;
+    const Ix wave0 = (Ix)tile * (Ix)FLAT + (Ix)((threadIdx.x & ~63u) * VEC);
     for (int j = threadIdx.x & 63; j < total; j += 64) {
         int bit, src;
         kf_job_decode<FPT * VEC>(bal, j, bit, src);
-        const unsigned int i =
-            wave0 + (unsigned int)(bit / VEC) * (FB * VEC) + (unsigned int)src * VEC + (unsigned int)(bit % VEC);
+        const Ix i = wave0 + (Ix)((bit / VEC) * (FB * VEC) + src * VEC + bit % VEC);
         int x, y, z;
-        g.xyz32(i, x, y, z);
-        adj_job<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, &st->div_slot[parity], nullptr, x, y, z, mx, esum, act);
+        if (WIDE) {
+            kf_xyz(g, (int64_t)i, x, y, z);
+            adj_job_wide<S, KF_MODE_STEP, false>(L, Cn, u_in, u_out, c, &st->div_slot[parity], nullptr,
+                                                 (int64_t)i, x, y, z, mx, esum, act);
+        } else {
+            g.xyz32((unsigned int)i, x, y, z);
+            adj_job<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, &st->div_slot[parity], nullptr, x, y, z, mx, esum,
+                                     act);
+        }
     }
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
@@ -519,16 +549,22 @@ __global__ __launch_bounds__(BLOCK) void k_adj_mask(const float *__restrict__ u,
 
 template <class S, int MODE>
 void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float *u_out, const AdjConsts &c,
-            kf_context *ctx, float *e_voxel, bool rows, bool flat1, unsigned int nb, int nseg, int nyt,
+            kf_context *ctx, float *e_voxel, bool rows, bool flat1, bool wide, unsigned int nb, int nseg, int nyt,
             const unsigned int *mask_in, int parity, const BlockPartial *prev, unsigned int prev_nb, int full)
 {
     BlockPartial *parts = kf_partials(ctx, parity);
     if (MODE == KF_MODE_STEP && rows)
-        hipLaunchKernelGGL((k_adj_flat<S, false>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out, c,
-                           ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
+        hipLaunchKernelGGL((k_adj_flat<S, false, false>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out,
+                           c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
     else if (MODE == KF_MODE_STEP && flat1)
-        hipLaunchKernelGGL((k_adj_flat<S, true>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out, c,
-                           ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
+        hipLaunchKernelGGL((k_adj_flat<S, true, false>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out,
+                           c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
+    else if (MODE == KF_MODE_STEP && wide && L.g.n % VEC == 0)
+        hipLaunchKernelGGL((k_adj_flat<S, false, true>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out,
+                           c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
+    else if (MODE == KF_MODE_STEP && wide)
+        hipLaunchKernelGGL((k_adj_flat<S, true, true>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out,
+                           c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
     else if (rows && L.g.nx % VEC == 0)
         hipLaunchKernelGGL((k_adj_rows<S, MODE, false>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out,
                            c, ctx->state, parts, e_voxel, nseg, nyt, mask_in, parity, prev, prev_nb);
@@ -575,22 +611,22 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     const AdjConsts c = kf_adj_consts(p);
     KF_CHECK_ARG(live->dtype == can->dtype, "live and canonical volumes must share a dtype");
     // A step runs the flat kernel when N is a multiple of 4 and 3 N < 2^30 (float4 plane
-    // accesses, 32-bit offsets), its unaligned form for other N < 2^30 / 3; energy and
-    // gradient passes run the row kernel (its unaligned form for rows that are not a multiple
-    // of 4 voxels) when 3 N < 2^30 and the nonzero mask of u_in is given.  Everything else runs
-    // k_adj_any (a full stencil per voxel).
+    // accesses, 32-bit offsets), its unaligned form for other N < 2^30 / 3, and its 64-bit
+    // forms (aligned or not) on larger grids; energy and gradient passes run the row kernel
+    // (its unaligned form for rows that are not a multiple of 4 voxels) when 3 N < 2^30 and the
+    // nonzero mask of u_in is given.  Everything else runs k_adj_any (a full stencil per voxel).
     const bool small = 3 * live->n < ((int64_t)1 << 30);
     const bool flat = mode == KF_MODE_STEP && live->n % VEC == 0 && small;
     const bool flat1 = mode == KF_MODE_STEP && !flat && small;
+    const bool wide = mode == KF_MODE_STEP && !small;
     const bool rows = flat || (mode != KF_MODE_STEP && kf_adj_mask_words(live->dims) > 0 && mask_in != nullptr);
     const int nseg = (live->dims[0] + ROW - 1) / ROW;
     const int nyt = (live->dims[1] + ROWS - 1) / ROWS;
-    const unsigned int nb = flat    ? (unsigned int)((live->n + FLAT - 1) / FLAT)
-                            : flat1 ? (unsigned int)((live->n + FLAT - 1) / FLAT)
-                            : rows  ? (unsigned int)(nseg * nyt * live->dims[2])
-                                    : (unsigned int)((live->n + BLOCK - 1) / BLOCK);
+    const unsigned int nb = flat || flat1 || wide ? (unsigned int)((live->n + FLAT - 1) / FLAT)
+                            : rows                ? (unsigned int)(nseg * nyt * live->dims[2])
+                                                  : (unsigned int)((live->n + BLOCK - 1) / BLOCK);
     kf_status s = kf_ensure_partials(ctx, nb);
-    if (s == KF_OK && (flat || flat1))
+    if (s == KF_OK && (flat || flat1 || wide))
         s = kf_ensure_need_words(ctx, (size_t)nb * (FB / 64) * (FPT * VEC));  // a word per ballot
     if (s != KF_OK)
         return s;
@@ -604,14 +640,14 @@ kf_status kf_launch_adj_step(const kf_volume *live, const kf_volume *can, const 
     KF_DISPATCH_DTYPE(live->dtype, S, {
         PhiRead<S> L = make_read<S>(live), Cn = make_read<S>(can);
         if (mode == KF_MODE_GRAD)
-            launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, false, nb, nseg, nyt, mask_in, 0, prev,
-                                    0u, 1);
+            launch<S, KF_MODE_GRAD>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, false, false, nb, nseg, nyt, mask_in,
+                                    0, prev, 0u, 1);
         else if (mode == KF_MODE_ENERGY)
-            launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, false, nb, nseg, nyt, mask_in, 0, prev,
-                                      0u, 1);
+            launch<S, KF_MODE_ENERGY>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, false, false, nb, nseg, nyt,
+                                      mask_in, 0, prev, 0u, 1);
         else  // the first step of a deform call stores every voxel
-            launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, flat1, nb, nseg, nyt, nullptr, parity,
-                                    prev, prev_nb, iter == 0 ? 1 : 0);
+            launch<S, KF_MODE_STEP>(L, Cn, u_in, u_out, c, ctx, e_voxel, rows, flat1, wide, nb, nseg, nyt, nullptr,
+                                    parity, prev, prev_nb, iter == 0 ? 1 : 0);
     });
     KF_LAUNCH_CHECK();
     return mode == KF_MODE_ENERGY ? kf_launch_finalize(ctx, nb, 0) : KF_OK;

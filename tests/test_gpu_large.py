@@ -1,8 +1,9 @@
-"""Grids past the 32-bit fast paths: 712^3 voxels (361 M, 3 N >= 2^30), where the deform call
-takes the general kernels (k_adj_any for the adjoint method, the dense scan of the
-reference method; the active-set sweep, the flat step and the nonzero masks are limited to
-3 N < 2^30) and every other pass indexes with 64-bit offsets, and 1296^3 (2.18 G voxels), whose
-voxel indices pass 2^31.
+"""Grids past the 32-bit fast paths: 712^3 and 713^3 voxels (361 M, 3 N >= 2^30), where the
+deform call takes the 64-bit kernels (the flat adjoint step's 64-bit form, aligned on 712^3 and
+unaligned on 713^3, and k_adj_any for the final energy; the dense scan of the reference method;
+the active-set sweep, the 32-bit flat step and the nonzero masks are limited to 3 N < 2^30) and
+every other pass indexes with 64-bit offsets, and 1296^3 (2.18 G voxels), whose voxel indices
+pass 2^31.
 
 The scene is the sphere pair, and the big grid holds the 64^3 grid of the same scene as a
 sub-box (an integer voxel offset; 2^-7 m voxels and corners on the 2^-8 m lattice, so every voxel
@@ -84,6 +85,7 @@ def map_to_big(B, tris):
 @pytest.mark.parametrize("edge, dtype, methods", [
     (712, "bf16", ("adjoint", "reference")),
     (712, "f32", ("adjoint", "reference")),
+    (713, "f32", ("adjoint",)),  # N % 4 = 1: the unaligned 64-bit step
     (1296, "bf16", ("adjoint",)),  # 2.18 G voxels: voxel indices past 2^31
 ])
 def test_big_grid_equals_embedded_small_grid(ctx, edge, dtype, methods):
