@@ -62,7 +62,9 @@ typedef enum kf_method {
  * (KillingFusion.cpp:272-274), and with u_j = 0 that reads phi_live[j]; so the voxels that
  * can move during one kf_deform call are those with u_j != 0 or phi_live[j] in that band,
  * fixed at the start of the call.  The active-set sweep lists them once and steps only
- * them; the dense sweep streams the whole grid every iteration. */
+ * them; the dense sweep streams the whole grid every iteration.  The active-set sweep runs on
+ * grids of fewer than 2^32 voxels and lists at most 2^30 of them; the dense sweep runs otherwise,
+ * whatever the sweep asked for. */
 typedef enum kf_sweep {
     KF_SWEEP_AUTO = 0,   /* active set when it holds at most 1/8 of the voxels, else dense */
     KF_SWEEP_DENSE = 1,  /* every voxel every iteration */

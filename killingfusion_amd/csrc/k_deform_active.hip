@@ -92,7 +92,7 @@ __device__ inline void mark_span(const PhiRead<S> &L, const float *__restrict__ 
             S::load4(L.p, (int64_t)ic, L.scale, live[k]);
 #pragma unroll
             for (int a = 0; a < 3; a++)
-                q[k][a] = READ_U ? kf_load_stream4(u + (unsigned int)a * n32 + ic) : make_float4(0.f, 0.f, 0.f, 0.f);
+                q[k][a] = READ_U ? kf_load_stream4(u + (size_t)a * n32 + ic) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (int k = 0; k < ROUNDS; k++) {
@@ -150,8 +150,9 @@ __device__ inline void mark_span(const PhiRead<S> &L, const float *__restrict__ 
             for (int k = 0; k < GROUP; k++) {
                 const unsigned int i = base + (unsigned int)((k0 + k) * BLOCK + w * 64) + (unsigned int)lane;
                 const unsigned int ic = i < n32 ? i : 0u;
-                ub[k] = READ_U ? __float_as_uint(kf_load_stream(u + ic)) | __float_as_uint(kf_load_stream(u + n32 + ic)) |
-                                     __float_as_uint(kf_load_stream(u + 2u * n32 + ic))
+                ub[k] = READ_U ? __float_as_uint(kf_load_stream(u + ic)) |
+                                     __float_as_uint(kf_load_stream(u + (size_t)n32 + ic)) |
+                                     __float_as_uint(kf_load_stream(u + 2 * (size_t)n32 + ic))
                                : 0u;
                 live[k] = S::load(L.p, (int64_t)ic, L.scale);
             }
@@ -283,8 +284,8 @@ __global__ __launch_bounds__(BLOCK) void k_moved_mark(const typename S::T *__res
         const unsigned int i = w0 + (unsigned int)lane;
         bool cand = false, bad = false;
         if (i < n32) {
-            const float u0 = kf_load_stream(u + i), u1 = kf_load_stream(u + n32 + i);
-            const float u2 = kf_load_stream(u + 2u * n32 + i);
+            const float u0 = kf_load_stream(u + i), u1 = kf_load_stream(u + (size_t)n32 + i);
+            const float u2 = kf_load_stream(u + 2 * (size_t)n32 + i);
             const float f = S::load(phi, (int64_t)i, scale);
             bad = !isfinite(f);
             cand = ((__float_as_uint(u0) | __float_as_uint(u1) | __float_as_uint(u2)) & 0x7fffffffu) != 0u ||
@@ -384,7 +385,7 @@ __device__ inline void list_neighbours(const GridDims &g, const unsigned long lo
     const unsigned int t = base + lb * (unsigned int)BLOCK + threadIdx.x;
     const unsigned int i = list[t];
     const unsigned int n32 = nvox32(g);
-    uc[t] = make_float4(u[i], u[n32 + i], u[2u * n32 + i], 0.0f);
+    uc[t] = make_float4(u[i], u[(size_t)n32 + i], u[2 * (size_t)n32 + i], 0.0f);
     int x, y, z;
     g.xyz32(i, x, y, z);
     int slot[NNB];
@@ -891,8 +892,8 @@ __global__ __launch_bounds__(BLOCK) void k_active_scatter(const unsigned int *__
     const unsigned int i = list[t];
     const float4 v = u0[(size_t)(st->iters & 1u) * set + t];
     u[i] = v.x;
-    u[n32 + i] = v.y;
-    u[2u * n32 + i] = v.z;
+    u[(size_t)n32 + i] = v.y;
+    u[2 * (size_t)n32 + i] = v.z;
 }
 
 template <class T>
@@ -913,7 +914,9 @@ kf_status grow(T *&p, size_t &cap, size_t need, hipStream_t s)
 
 }  // namespace
 
-bool kf_active_supported(const kf_volume *live) { return 3 * live->n < ((int64_t)1 << 30); }
+// Voxel indices, list entries and the slots of the 12 neighbours are 32-bit; plane offsets
+// into the field are 64-bit, so any grid of fewer than 2^32 voxels can be listed.
+bool kf_active_supported(const kf_volume *live) { return live->n < ((int64_t)1 << 32); }
 
 void kf_active_release(kf_context *ctx)
 {
@@ -956,7 +959,7 @@ unsigned int kf_list_slots(unsigned int count, bool headroom)
     c = (c + SLOT_QUANTUM - 1) / SLOT_QUANTUM * SLOT_QUANTUM;
     if (c < SLOT_QUANTUM)
         c = SLOT_QUANTUM;
-    return c > 0xffffffffull - SLOT_QUANTUM ? 0u : (unsigned int)c;
+    return c > KF_LIST_MAX ? 0u : (unsigned int)c;
 }
 
 kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params *p, bool u_zero, unsigned int slots,
@@ -965,7 +968,7 @@ kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params
     kf_context *ctx = live->ctx;
     kf_active_ws &a = ctx->active;
     a.warp_vol = nullptr;
-    KF_CHECK_ARG(kf_active_supported(live), "the active-set sweep needs 3 N < 2^30");
+    KF_CHECK_ARG(kf_active_supported(live), "the active-set sweep needs N < 2^32");
     const AdjConsts c = kf_adj_consts(p);
     const unsigned int nspan = (unsigned int)((live->n + SPAN - 1) / SPAN);
     kf_status s = grow(a.mask, a.mask_cap, (size_t)nspan * WPB, ctx->stream);
@@ -1400,7 +1403,7 @@ kf_status kf_list_moved(const kf_volume *v, const float *u, unsigned int *count,
     kf_active_ws &a = ctx->active;
     a.warp_vol = nullptr;
     hipStream_t st = ctx->stream;
-    KF_CHECK_ARG(kf_active_supported(v), "listing needs 3 N < 2^30");
+    KF_CHECK_ARG(kf_active_supported(v), "listing needs N < 2^32");
     const unsigned int nspan = (unsigned int)((v->n + SPAN - 1) / SPAN);
     const unsigned int nwords = (unsigned int)((v->n + 63) / 64);
     kf_status s = grow(a.mask, a.mask_cap, (size_t)nspan * WPB, st);

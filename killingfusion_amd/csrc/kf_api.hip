@@ -2,6 +2,7 @@
 // optimiser driver.  Kernels live in k_*.hip.
 #include "kf_reduce.hpp"
 
+#include <algorithm>
 #include <atomic>
 #include <climits>
 #include <cstring>
@@ -647,7 +648,7 @@ kf_status deform_pass(const kf_volume *live, const kf_volume *can, kf_field *u, 
                          (p->sweep == KF_SWEEP_ACTIVE || (int64_t)a.hint_listed * 8 <= live->n);
         if (predicted) {
             // never more slots than a list of every voxel needs
-            const unsigned int most = kf_list_slots((unsigned int)live->n, false);
+            const unsigned int most = kf_list_slots((unsigned int)std::min<int64_t>(live->n, KF_LIST_MAX), false);
             slots = kf_list_slots(a.hint_listed, true);
             slots = slots < most ? slots : most;
             expect = kf_list_slots(a.hint_listed, false);
@@ -686,7 +687,7 @@ kf_status deform_pass(const kf_volume *live, const kf_volume *can, kf_field *u, 
             a.hint_listed = count;
         }
         // room for the launch shape the next call predicts from this list
-        const unsigned int most = kf_list_slots((unsigned int)live->n, false);
+        const unsigned int most = kf_list_slots((unsigned int)std::min<int64_t>(live->n, KF_LIST_MAX), false);
         const unsigned int ahead = kf_list_slots(ctx->state_host->listed, true);
         const unsigned int reserve = predicted ? 0u : (ahead < most ? ahead : most);
         if (use && (s = kf_active_prepare(live, u->buf[cur0], slots, adj, reserve)) != KF_OK)

@@ -578,7 +578,9 @@ void kf_active_release(kf_context *ctx);
 void kf_ref_release(kf_context *ctx);
 // Slots for a list of `count` voxels: count rounded up to the quantum, or, with headroom (a
 // length predicted from an earlier call), an eighth more rounded to a sixteenth of its power
-// of two.  0 when that does not fit 32 bits.
+// of two.  0 when that is more than KF_LIST_MAX (the list kernels index the compact sets'
+// floats with 32-bit offsets: 4 per slot).
+constexpr unsigned int KF_LIST_MAX = 1u << 30;
 unsigned int kf_list_slots(unsigned int count, bool headroom);
 kf_status kf_active_count(const kf_volume *live, const float *u, const kf_params *p, bool u_zero, unsigned int slots,
                           bool need_finite);

@@ -1,9 +1,12 @@
 """Grids past the 32-bit fast paths: 712^3 and 713^3 voxels (361 M, 3 N >= 2^30), where the
-deform call takes the 64-bit kernels (the flat adjoint step's 64-bit form, aligned on 712^3 and
-unaligned on 713^3, and k_adj_any for the final energy; the dense scan of the reference method;
-the active-set sweep, the 32-bit flat step and the nonzero masks are limited to 3 N < 2^30) and
-every other pass indexes with 64-bit offsets, and 1296^3 (2.18 G voxels), whose voxel indices
-pass 2^31.
+deform call's dense sweep takes the 64-bit kernels (the flat adjoint step's 64-bit form,
+aligned on 712^3 and unaligned on 713^3, and k_adj_any for the final energy; the dense scan of
+the reference method; the 32-bit flat step and the nonzero masks are limited to 3 N < 2^30),
+its active-set sweep lists 32-bit voxel indices with 64-bit plane offsets, and every other pass
+indexes with 64-bit offsets; and 1296^3 (2.18 G voxels), whose voxel indices pass 2^31.  Each
+deform runs both sweeps on the big grid.  The warp follows an active-set call (it then warps
+the call's list) on every grid, and a call with another field on the grids that run both
+methods (it then lists the voxels the field moves).
 
 The scene is the sphere pair, and the big grid holds the 64^3 grid of the same scene as a
 sub-box (an integer voxel offset; 2^-7 m voxels and corners on the 2^-8 m lattice, so every voxel
@@ -15,6 +18,8 @@ Integration (SDF.cpp:137-197), both optimisers (KillingFusion.cpp:240-351), the 
 give the small grid's results bit for bit inside the sub-box and leave every other voxel as
 the reset left it (phi = Max + eps, w = 0, u = 0).  The small grid's own results are pinned
 against the oracle by the other GPU tests."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -114,25 +119,33 @@ def test_big_grid_equals_embedded_small_grid(ctx, edge, dtype, methods):
     warp_by = None
     for method in methods:
         kp = params[method]
-        f_s, f_b = DisplacementField(ctx, SMALL), DisplacementField(ctx, B.dims)
+        kd = dataclasses.replace(kp, sweep="dense")
+        f_s = DisplacementField(ctx, SMALL)
         f_s.initializeAllVoxels((0.0, 0.0, 0.0))
-        f_b.initializeAllVoxels((0.0, 0.0, 0.0))
         st_s = compute_displacement_field(live_s, can_s, f_s, kp)
-        st_b = compute_displacement_field(live_b, can_b, f_b, kp)
-        assert st_b["sweep"] == "dense", "the active-set sweep is limited to 3 N < 2^30"
-        for k in ("iters", "max_update", "active", "diverged_index"):
-            assert st_b[k] == st_s[k], f"{method}: {k} {st_b[k]} != {st_s[k]}"
-        if method == "adjoint":  # the same per-voxel terms (all >= 0), summed in another order
-            assert abs(st_b["energy"] - st_s["energy"]) <= 1e-11 * abs(st_s["energy"])
-        moved = check_field(B, f_b, f_s, method)
-        print(f"{method}: {st_s['iters']} iterations, {st_s['active']} active, {moved} nonzero components, "
-              f"equal", flush=True)
-        assert moved > 1000
-        if method == "adjoint":
-            warp_by = (f_s, f_b)
-        else:
+        assert st_s["sweep"] == "active"
+        # the dense sweep, then the automatic one (the active set: every grid of fewer than
+        # 2^32 voxels can be listed), so that the warp below follows an active-set call
+        for k_big, sweep in ((kd, "dense"), (kp, "active")):
+            f_b = DisplacementField(ctx, B.dims)
+            f_b.initializeAllVoxels((0.0, 0.0, 0.0))
+            st_b = compute_displacement_field(live_b, can_b, f_b, k_big)
+            tag = f"{method}, {sweep} sweep"
+            assert st_b["sweep"] == sweep, tag
+            for k in ("iters", "max_update", "active", "diverged_index"):
+                assert st_b[k] == st_s[k], f"{tag}: {k} {st_b[k]} != {st_s[k]}"
+            if method == "adjoint":  # the same per-voxel terms (all >= 0), summed in another order
+                assert abs(st_b["energy"] - st_s["energy"]) <= 1e-11 * abs(st_s["energy"]), tag
+            moved = check_field(B, f_b, f_s, tag)
+            print(f"{tag}: {st_s['iters']} iterations, {st_s['active']} active, {st_b.get('listed')} listed, "
+                  f"{moved} nonzero components, equal", flush=True)
+            assert moved > 1000
+            if method == "adjoint" and sweep == "active":
+                warp_by = (f_s, f_b)
+            else:
+                f_b.close()
+        if method != "adjoint":
             f_s.close()
-            f_b.close()
 
     f_s, f_b = warp_by
     live_s.update(f_s)
