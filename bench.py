@@ -253,7 +253,9 @@ def bench_sequence(ctx, frames=100, mesh_every=10):
 def step_only(grid, steps, warmup):
     """The dense BF16 adjoint step alone on an N^3 grid of the torus pair (4 mm voxels, the
     grid centred on the torus), for profiling at sizes other than 256^3: at 320^3 one u buffer
-    (393 MB) is larger than the 256 MB last-level cache, so the step's rate there is DRAM's."""
+    (393 MB) is larger than the 256 MB last-level cache, so the step's rate there is DRAM's; from
+    712^3 (3 N >= 2^30) the step takes its 64-bit form.  Also times one 100-iteration call of the
+    active-set sweep on the same grid (listing included)."""
     from killingfusion_amd import Context, KillingParams, SDF, scenes
     d_can, d_live = scenes.torus_pair()
     ctx = Context(0)
@@ -269,13 +271,17 @@ def step_only(grid, steps, warmup):
     _, gpu_ms0, _ = time_deform(ctx, live, can, kp, 0, 0, events)
     kp_a = KillingParams.paper(VOXEL, 3.0, method="adjoint", sweep="active")
     _, _, st_a = time_deform(ctx, live, can, kp_a, 1, 0, events)
+    # the same grid through the active-set sweep: one call of 100 iterations, listing included
+    _, active_ms, st_a100 = time_deform(ctx, live, can, kp_a, 100, 3, events)
     n = grid ** 3
     us = (gpu_ms - gpu_ms0) / steps * 1e3
     b = step_bytes(2, st_a["listed"] / n)
     print(json.dumps({"step_only": f"dense adjoint step, bf16, {grid}^3 torus pair", "steps": steps,
                       "kernel_us": us, "ps_per_voxel": us * 1e6 / n, "listed_voxels": int(st_a["listed"]),
                       "bytes_per_voxel_iteration": b, "achieved_gbs": b * n / (us * 1e-6) / 1e9,
-                      "u_buffer_mb": 12 * n / 1e6}), flush=True)
+                      "u_buffer_mb": 12 * n / 1e6,
+                      "active_set_call_ms_100_iterations": active_ms, "active_set_sweep": st_a100["sweep"]}),
+          flush=True)
     ctx.close()  # every handle released before the interpreter exits
     return 0
 

@@ -199,45 +199,53 @@ __global__ __launch_bounds__(BLOCK) void k_active_mark(PhiRead<S> L, const float
     mark_span<S, READ_U>(L, u, lo, hi, mask, span_count, nonfinite, blockIdx.x);
 }
 
-// Exclusive scan of the span counts (one block); the total goes to *total.
+// Exclusive scan of the span counts (one block); the total goes to *total.  The block walks
+// the counts in tiles of 8 SCAN_BLOCK, a thread eight consecutive counts of each (loaded
+// together), carrying the running total from tile to tile (712^3: 88 k spans, 162 -> 84 us;
+// 32 counts per thread measured the same).
 __device__ inline void scan_spans(const unsigned int *__restrict__ span_count, unsigned int nspan,
                                   unsigned int *__restrict__ span_off, unsigned int *total)
 {
-    const unsigned int per = (nspan + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    const unsigned int b0 = threadIdx.x * per;
-    unsigned int sum = 0;
-    for (unsigned int k = 0; k < per; k++)
-        if (b0 + k < nspan)
-            sum += span_count[b0 + k];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned int v = sum;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const unsigned int t = __shfl_up(v, o, 64);
-        if (lane >= o)
-            v += t;
-    }
+    constexpr unsigned int PT = 8, TILE = PT * SCAN_BLOCK;
     __shared__ unsigned int s_w[SCAN_BLOCK / 64];
-    if (lane == 63)
-        s_w[w] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int r = 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned int carry = 0;  // the same in every thread
+    for (unsigned int t0 = 0; t0 < nspan; t0 += TILE) {
+        const unsigned int b0 = t0 + threadIdx.x * PT;
+        unsigned int q[PT], sum = 0;
+#pragma unroll
+        for (unsigned int j = 0; j < PT; j++) {
+            q[j] = b0 + j < nspan ? span_count[b0 + j] : 0u;
+            sum += q[j];
+        }
+        unsigned int v = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned int t = __shfl_up(v, o, 64);
+            if (lane >= o)
+                v += t;
+        }
+        if (lane == 63)
+            s_w[w] = v;
+        __syncthreads();
+        unsigned int before = 0, tile_total = 0;
         for (int k = 0; k < SCAN_BLOCK / 64; k++) {
             const unsigned int t = s_w[k];
-            s_w[k] = r;
-            r += t;
+            before += k < w ? t : 0u;
+            tile_total += t;
         }
+        unsigned int run = carry + before + v - sum;
+#pragma unroll
+        for (unsigned int j = 0; j < PT; j++)
+            if (b0 + j < nspan) {
+                span_off[b0 + j] = run;
+                run += q[j];
+            }
+        carry += tile_total;
+        __syncthreads();  // s_w is rewritten by the next tile
     }
-    __syncthreads();
-    unsigned int run = s_w[w] + v - sum;
-    for (unsigned int k = 0; k < per; k++)
-        if (b0 + k < nspan) {
-            span_off[b0 + k] = run;
-            run += span_count[b0 + k];
-        }
     if (threadIdx.x == SCAN_BLOCK - 1)
-        *total = run;
+        *total = carry;
 }
 
 // The single-volume scan also decides, on the device, whether the list kernels run: they were
