@@ -36,7 +36,7 @@ TRUNC = 3 * VOXEL
 MIN3D = (-0.512, -0.512, 0.288)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "adj_step_bf16_256_traffic.md")
 TRACE_FILE = os.path.join(ROOT, "profiles", "adj_step_bf16_256_kernel_trace.md")
-HEADLINE_KERNEL = "k_adj_flat<StoreBF16, false>"
+HEADLINE_KERNEL = "k_adj_flat<StoreBF16, false, false, true>"
 
 
 class HipEvents:

@@ -384,8 +384,10 @@ constexpr int FLAT = FB * VEC * FPT;   // voxels per block
 // launch then streams u and phi_live once and writes only the band's groups.)
 //
 // WIDE: 3 N >= 2^30, so plane offsets and voxel indices are 64-bit and the stencils run
-// adj_job_wide; everything else is the same.
-template <class S, bool UNAL, bool WIDE>
+// adj_job_wide; everything else is the same.  EXACT: N is a multiple of FLAT, so every group of
+// every block lies inside the grid and the loads and stores go without bounds checks (256^3
+// BF16: 42.0-42.3 against 43.0-43.8 us).
+template <class S, bool UNAL, bool WIDE, bool EXACT>
 __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, const float *__restrict__ u_in,
                                                     float *__restrict__ u_out, AdjConsts c, DeformState *st,
                                                     BlockPartial *partials, int parity, const BlockPartial *prev,
@@ -427,7 +429,7 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
 #pragma unroll
     for (int q = 0; q < FPT; q++) {
         i0[q] = (Ix)tile * (Ix)FLAT + (Ix)(q * (FB * VEC)) + (Ix)(threadIdx.x * VEC);
-        valid[q] = i0[q] < n32;
+        valid[q] = EXACT || i0[q] < n32;
         nin[q] = !valid[q] ? 0u
                  : (UNAL && n32 - i0[q] < (Ix)VEC ? (unsigned int)(n32 - i0[q]) : (unsigned int)VEC);
 #pragma unroll
@@ -553,18 +555,25 @@ void launch(const PhiRead<S> &L, const PhiRead<S> &Cn, const float *u_in, float 
             const unsigned int *mask_in, int parity, const BlockPartial *prev, unsigned int prev_nb, int full)
 {
     BlockPartial *parts = kf_partials(ctx, parity);
-    if (MODE == KF_MODE_STEP && rows)
-        hipLaunchKernelGGL((k_adj_flat<S, false, false>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out,
-                           c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
+    const bool exact = L.g.n % FLAT == 0;
+    if (MODE == KF_MODE_STEP && rows && exact)
+        hipLaunchKernelGGL((k_adj_flat<S, false, false, true>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in,
+                           u_out, c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
+    else if (MODE == KF_MODE_STEP && rows)
+        hipLaunchKernelGGL((k_adj_flat<S, false, false, false>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in,
+                           u_out, c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
     else if (MODE == KF_MODE_STEP && flat1)
-        hipLaunchKernelGGL((k_adj_flat<S, true, false>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out,
-                           c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
+        hipLaunchKernelGGL((k_adj_flat<S, true, false, false>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in,
+                           u_out, c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
+    else if (MODE == KF_MODE_STEP && wide && exact)
+        hipLaunchKernelGGL((k_adj_flat<S, false, true, true>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in,
+                           u_out, c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
     else if (MODE == KF_MODE_STEP && wide && L.g.n % VEC == 0)
-        hipLaunchKernelGGL((k_adj_flat<S, false, true>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out,
-                           c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
+        hipLaunchKernelGGL((k_adj_flat<S, false, true, false>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in,
+                           u_out, c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
     else if (MODE == KF_MODE_STEP && wide)
-        hipLaunchKernelGGL((k_adj_flat<S, true, true>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in, u_out,
-                           c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
+        hipLaunchKernelGGL((k_adj_flat<S, true, true, false>), dim3(nb), dim3(FB), 0, ctx->stream, L, Cn, u_in,
+                           u_out, c, ctx->state, parts, parity, prev, prev_nb, ctx->need_words, full);
     else if (rows && L.g.nx % VEC == 0)
         hipLaunchKernelGGL((k_adj_rows<S, MODE, false>), dim3(nb), dim3(BLOCK), 0, ctx->stream, L, Cn, u_in, u_out,
                            c, ctx->state, parts, e_voxel, nseg, nyt, mask_in, parity, prev, prev_nb);
