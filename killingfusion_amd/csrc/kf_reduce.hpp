@@ -64,6 +64,47 @@ __device__ inline void kf_block_reduce3(double &sum, double &mx, unsigned long l
     }
 }
 
+// As kf_block_publish, for blocks whose wavefronts mostly hold zeros: a wavefront with
+// wave_has false (uniform over it) holds (+0, +0, 0) in every lane and skips its reductions,
+// whose result would be the same zeros.  (Dense adjoint step, 256^3 BF16: 41.6-41.8 against
+// 41.5-42.3 us.)
+template <int BLOCK>
+__device__ inline void kf_block_publish_sparse(BlockPartial *parts, unsigned int slot, double sum, double mx,
+                                               unsigned long long cnt, bool wave_has)
+{
+    constexpr int NW = BLOCK / 64;
+    __shared__ double s_sum[NW];
+    __shared__ double s_max[NW];
+    __shared__ unsigned long long s_cnt[NW];
+    if (wave_has) {
+        sum = kf_wave_sum(sum);
+        mx = kf_wave_max(mx);
+        cnt = kf_wave_sum_u64(cnt);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_sum[wid] = sum;
+        s_max[wid] = mx;
+        s_cnt[wid] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, m = 0.0;
+        unsigned long long c = 0;
+        for (int w = 0; w < NW; w++) {
+            a += s_sum[w];
+            m = fmax(m, s_max[w]);
+            c += s_cnt[w];
+        }
+        BlockPartial p;
+        p.sum = a;
+        p.max = m;
+        p.cnt = c;
+        p.pad = 0ull;
+        parts[slot] = p;
+    }
+}
+
 // Thread 0 stores the block's (sum, max, count) in its slot; k_finalize (kf_api.hip) folds
 // the slots in index order after the launch.  No same-address atomics: with tens of
 // thousands of blocks those serialise in L2 and cost more than the stencil itself.
