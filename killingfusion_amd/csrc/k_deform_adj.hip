@@ -380,8 +380,9 @@ constexpr int FLAT = FB * VEC * FPT;   // voxels per block
 // wavefront of the previous launch), or when `full` is set (the first launch of a deform call,
 // whose destination holds anything); active voxels are stored by the stencil below as before.
 // Every launch leaves its destination equal to u_{k+1} at every voxel, bit for bit as with a
-// store per voxel.  (256^3 BF16: 44.8 us per launch, against 57 us with a store per voxel; the
-// launch then streams u and phi_live once and writes only the band's groups.)
+// store per voxel.  (256^3 BF16, one group per thread: 44.8 us per launch, against 57 us with
+// a store per voxel; the launch then streams u and phi_live once and writes only the band's
+// groups.)
 //
 // WIDE: 3 N >= 2^30, so plane offsets and voxel indices are 64-bit and the stencils run
 // adj_job_wide; everything else is the same.  EXACT: N is a multiple of FLAT, so every group of
