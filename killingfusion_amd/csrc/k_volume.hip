@@ -132,14 +132,16 @@ __device__ inline bool average_voxel(const IntegrateArgs<S> &A, const Projected 
     return true;
 }
 
-template <class S, bool FRESH, bool ROWWISE>
+// ALIGNED: rows hold a multiple of IPT voxels, so a thread's group lies in one row and inside
+// the grid (192^3 BF16 update pass: 12.9-13.1 against 14.3-14.4 us at the minimum).
+template <class S, bool FRESH, bool ROWWISE, bool ALIGNED>
 __global__ __launch_bounds__(BLOCK) void k_integrate(IntegrateArgs<S> A)
 {
     const GridDims &g = A.g;
     const int64_t i0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) * IPT;
     if (i0 >= g.n)
         return;
-    const bool whole = i0 + IPT <= g.n;
+    const bool whole = ALIGNED || i0 + IPT <= g.n;
     int x, y, z;
     kf_xyz(g, i0, x, y, z);
     // project the group's voxels, then read their pixels together
@@ -151,7 +153,7 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(IntegrateArgs<S> A)
         KfRowwise R = integrate_rowwise(A, y, z);
 #pragma unroll
         for (int k = 0; k < IPT; k++) {
-            if (x == g.nx) {  // the group runs into the next row
+            if (!ALIGNED && x == g.nx) {  // the group runs into the next row
                 x = 0;
                 if (++y == g.ny) {
                     y = 0;
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(IntegrateArgs<S> A)
             }
             bool ok;
             const int pix = kf_pixel_rowwise(A.C, R, x, ok);
-            const bool live = i0 + k < g.n;
+            const bool live = ALIGNED || i0 + k < g.n;
             pr[k] = Projected{R.c2, live ? pix : -1};
             exact |= (live && !ok ? 1u : 0u) << k;
             vx[k] = x;
@@ -173,7 +175,7 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(IntegrateArgs<S> A)
         IntegrateRow R = integrate_row(A, y, z);
 #pragma unroll
         for (int k = 0; k < IPT; k++) {
-            if (x == g.nx) {
+            if (!ALIGNED && x == g.nx) {
                 x = 0;
                 if (++y == g.ny) {
                     y = 0;
@@ -185,7 +187,7 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(IntegrateArgs<S> A)
             const double c2 = ((A.C.M[8] * X + R.p2[0]) + R.p2[1]) + R.p2[2];
             bool ok;
             const int pix = kf_pixel_fast(A.C, R.bx, R.by, X, c2, __builtin_amdgcn_rcpf((float)c2), ok);
-            const bool live = i0 + k < g.n;
+            const bool live = ALIGNED || i0 + k < g.n;
             pr[k] = Projected{c2, live ? pix : -1};
             exact |= (live && !ok ? 1u : 0u) << k;
             vx[k] = x;
@@ -507,14 +509,27 @@ kf_status kf_launch_integrate(kf_volume *v, const float *ddepth, int w, int h, c
         A.min_depth = min_depth;
         A.max_depth = max_depth;
         const bool rowwise = kf_camera_rowwise(cam);
-        if (fresh && rowwise)
-            hipLaunchKernelGGL((k_integrate<S, true, true>), dim3(nb), dim3(BLOCK), 0, v->ctx->stream, A);
-        else if (fresh)
-            hipLaunchKernelGGL((k_integrate<S, true, false>), dim3(nb), dim3(BLOCK), 0, v->ctx->stream, A);
-        else if (rowwise)
-            hipLaunchKernelGGL((k_integrate<S, false, true>), dim3(nb), dim3(BLOCK), 0, v->ctx->stream, A);
-        else
-            hipLaunchKernelGGL((k_integrate<S, false, false>), dim3(nb), dim3(BLOCK), 0, v->ctx->stream, A);
+        const bool aligned = v->dims[0] % IPT == 0;
+        auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, dim3(nb), dim3(BLOCK), 0, v->ctx->stream, A); };
+        if (aligned) {
+            if (fresh && rowwise)
+                go(k_integrate<S, true, true, true>);
+            else if (fresh)
+                go(k_integrate<S, true, false, true>);
+            else if (rowwise)
+                go(k_integrate<S, false, true, true>);
+            else
+                go(k_integrate<S, false, false, true>);
+        } else {
+            if (fresh && rowwise)
+                go(k_integrate<S, true, true, false>);
+            else if (fresh)
+                go(k_integrate<S, true, false, false>);
+            else if (rowwise)
+                go(k_integrate<S, false, true, false>);
+            else
+                go(k_integrate<S, false, false, false>);
+        }
     });
     KF_LAUNCH_CHECK();
     return KF_OK;
