@@ -525,7 +525,7 @@ __global__ __launch_bounds__(FB) void k_adj_flat(PhiRead<S> L, PhiRead<S> Cn, co
         for (int b = 0; b < NWW; b += 2)
             *reinterpret_cast<ulonglong2 *>(needw + nw + b) = make_ulonglong2(bal[b], bal[b + 1]);
     }
-    kf_block_publish_sparse<FB>(partials, blockIdx.x, esum, mx, act, total > 0);
+    kf_block_publish<FB>(partials, blockIdx.x, esum, mx, act, total > 0);
 }
 
 // Nonzero mask of field values u (layout above; UNAL as in k_adj_rows).

@@ -32,45 +32,12 @@ __device__ inline unsigned long long kf_wave_sum_u64(unsigned long long v)
 }
 
 // Block-wide reduction of (sum, max, count); result valid in thread 0.  Uses a
-// fixed combination order, so the FP64 sum is reproducible for a fixed launch shape.
+// fixed combination order, so the FP64 sum is reproducible for a fixed launch shape.  A
+// wavefront with wave_has false (uniform over it) holds (+0, +0, 0) in every lane and skips its
+// reductions, whose result would be the same zeros (dense adjoint step, 256^3 BF16: 41.6-41.8
+// against 41.5-42.3 us).
 template <int BLOCK>
-__device__ inline void kf_block_reduce3(double &sum, double &mx, unsigned long long &cnt)
-{
-    constexpr int NW = BLOCK / 64;
-    __shared__ double s_sum[NW];
-    __shared__ double s_max[NW];
-    __shared__ unsigned long long s_cnt[NW];
-    sum = kf_wave_sum(sum);
-    mx = kf_wave_max(mx);
-    cnt = kf_wave_sum_u64(cnt);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) {
-        s_sum[wid] = sum;
-        s_max[wid] = mx;
-        s_cnt[wid] = cnt;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0.0, m = 0.0;
-        unsigned long long c = 0;
-        for (int w = 0; w < NW; w++) {
-            a += s_sum[w];
-            m = fmax(m, s_max[w]);
-            c += s_cnt[w];
-        }
-        sum = a;
-        mx = m;
-        cnt = c;
-    }
-}
-
-// As kf_block_publish, for blocks whose wavefronts mostly hold zeros: a wavefront with
-// wave_has false (uniform over it) holds (+0, +0, 0) in every lane and skips its reductions,
-// whose result would be the same zeros.  (Dense adjoint step, 256^3 BF16: 41.6-41.8 against
-// 41.5-42.3 us.)
-template <int BLOCK>
-__device__ inline void kf_block_publish_sparse(BlockPartial *parts, unsigned int slot, double sum, double mx,
-                                               unsigned long long cnt, bool wave_has)
+__device__ inline void kf_block_reduce3(double &sum, double &mx, unsigned long long &cnt, bool wave_has = true)
 {
     constexpr int NW = BLOCK / 64;
     __shared__ double s_sum[NW];
@@ -96,12 +63,9 @@ __device__ inline void kf_block_publish_sparse(BlockPartial *parts, unsigned int
             m = fmax(m, s_max[w]);
             c += s_cnt[w];
         }
-        BlockPartial p;
-        p.sum = a;
-        p.max = m;
-        p.cnt = c;
-        p.pad = 0ull;
-        parts[slot] = p;
+        sum = a;
+        mx = m;
+        cnt = c;
     }
 }
 
@@ -110,9 +74,9 @@ __device__ inline void kf_block_publish_sparse(BlockPartial *parts, unsigned int
 // thousands of blocks those serialise in L2 and cost more than the stencil itself.
 template <int BLOCK>
 __device__ inline void kf_block_publish(BlockPartial *parts, unsigned int slot, double sum, double mx,
-                                        unsigned long long cnt)
+                                        unsigned long long cnt, bool wave_has = true)
 {
-    kf_block_reduce3<BLOCK>(sum, mx, cnt);
+    kf_block_reduce3<BLOCK>(sum, mx, cnt, wave_has);
     if (threadIdx.x == 0) {
         BlockPartial p;
         p.sum = sum;
